@@ -1,0 +1,165 @@
+"""Synthetic blocks: node grids, finite-volume metrics and seeded initial states.
+
+The reference reads its grid from an unformatted file (main.f:1366-1374) and forms face
+areas, unit normals and volumes in AREVC1/VOLINC (gluebm.f:1586, 1889).  That
+preprocessing is outside the hot path; this module builds the same arrays for synthetic
+cases: face-area vectors from the face diagonals, volumes from the divergence theorem,
+two ghost layers of linearly extrapolated nodes.  Face data of direction d is stored at the
+upper cell of the face, as the reference stores it.
+"""
+import numpy as np
+
+from .layout import GHOST, Block, default_params
+
+
+def stretched(n, length, first=None, ratio=None):
+    """n+1 node coordinates on [0, length]: uniform, or geometric with the given ratio
+    (first cell width follows from the ratio)."""
+    if ratio is None or ratio == 1.0:
+        return np.linspace(0.0, length, n + 1)
+    w = ratio ** np.arange(n)
+    x = np.concatenate([[0.0], np.cumsum(w)])
+    return x * (length / x[-1])
+
+
+def geometric_from_wall(n, first, ratio):
+    """n+1 nodes with first cell width `first` growing by `ratio`."""
+    w = first * ratio ** np.arange(n)
+    return np.concatenate([[0.0], np.cumsum(w)])
+
+
+def cosine_clustered(n, length):
+    """n+1 nodes clustered towards both ends (cosine spacing)."""
+    t = np.linspace(0.0, np.pi, n + 1)
+    return 0.5 * length * (1.0 - np.cos(t))
+
+
+def _extend(x):
+    """Add two linearly extrapolated ghost nodes at each end of a 1-D node array."""
+    lo1 = 2 * x[0] - x[1]
+    lo2 = 2 * lo1 - x[0]
+    hi1 = 2 * x[-1] - x[-2]
+    hi2 = 2 * hi1 - x[-1]
+    return np.concatenate([[lo2, lo1], x, [hi1, hi2]])
+
+
+def node_grid(xi, yj, zk, distort=0.0, seed=0):
+    """Nodes (KS+1, JS+1, IS+1, 3) of a box with ghost nodes; `distort` adds a smooth
+    interior deformation (fraction of the local spacing) to exercise curvilinear metrics."""
+    x, y, z = _extend(np.asarray(xi, float)), _extend(np.asarray(yj, float)), _extend(np.asarray(zk, float))
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    if distort:
+        rng = np.random.default_rng(seed)
+        ph = rng.uniform(0, 2 * np.pi, 6)
+        lx, ly, lz = x[-1] - x[0], y[-1] - y[0], z[-1] - z[0]
+        dx = np.gradient(x).min()
+        dy = np.gradient(y).min()
+        dz = np.gradient(z).min()
+        s = distort
+        X = X + s * dx * np.sin(2 * np.pi * Y / ly + ph[0]) * np.sin(2 * np.pi * Z / lz + ph[1])
+        Y = Y + s * dy * np.sin(2 * np.pi * X / lx + ph[2]) * np.sin(2 * np.pi * Z / lz + ph[3])
+        Z = Z + s * dz * np.sin(2 * np.pi * X / lx + ph[4]) * np.sin(2 * np.pi * Y / ly + ph[5])
+    return np.stack([X, Y, Z], axis=-1)
+
+
+def _face_vec(p00, p10, p11, p01):
+    return 0.5 * np.cross(p11 - p00, p01 - p10)
+
+
+def fill_geometry(blk, nodes):
+    """Face areas/normals, volumes, spacings and centres of every cell (ghosts included)."""
+    Xn = nodes
+    ks, js, is_ = blk.kstrid, blk.jstrid, blk.istrid
+    assert Xn.shape == (ks + 1, js + 1, is_ + 1, 3)
+    # i-faces at node planes 0..IS (cells' lower faces plus the last upper face)
+    Si = _face_vec(Xn[:-1, :-1, :], Xn[:-1, 1:, :], Xn[1:, 1:, :], Xn[1:, :-1, :])      # (KS,JS,IS+1)
+    Sj = _face_vec(Xn[:-1, :, :-1], Xn[1:, :, :-1], Xn[1:, :, 1:], Xn[:-1, :, 1:])      # (KS,JS+1,IS)
+    Sk = _face_vec(Xn[:, :-1, :-1], Xn[:, :-1, 1:], Xn[:, 1:, 1:], Xn[:, 1:, :-1])      # (KS+1,JS,IS)
+    Ci = 0.25 * (Xn[:-1, :-1, :] + Xn[:-1, 1:, :] + Xn[1:, 1:, :] + Xn[1:, :-1, :])
+    Cj = 0.25 * (Xn[:-1, :, :-1] + Xn[1:, :, :-1] + Xn[1:, :, 1:] + Xn[:-1, :, 1:])
+    Ck = 0.25 * (Xn[:, :-1, :-1] + Xn[:, :-1, 1:] + Xn[:, 1:, 1:] + Xn[:, 1:, :-1])
+    vol = (np.sum(Si[:, :, 1:] * Ci[:, :, 1:], -1) - np.sum(Si[:, :, :-1] * Ci[:, :, :-1], -1)
+           + np.sum(Sj[:, 1:, :] * Cj[:, 1:, :], -1) - np.sum(Sj[:, :-1, :] * Cj[:, :-1, :], -1)
+           + np.sum(Sk[1:, :, :] * Ck[1:, :, :], -1) - np.sum(Sk[:-1, :, :] * Ck[:-1, :, :], -1)) / 3.0
+    if np.any(vol <= 0):
+        raise ValueError("non-positive cell volume in synthetic grid")
+    for name, S in (("A1", Si[:, :, :-1]), ("A2", Sj[:, :-1, :]), ("A3", Sk[:-1, :, :])):
+        a = np.linalg.norm(S, axis=-1)
+        blk.view3(name)[...] = a
+        blk.view3(name + "X")[...] = S[..., 0] / a
+        blk.view3(name + "Y")[...] = S[..., 1] / a
+        blk.view3(name + "Z")[...] = S[..., 2] / a
+    blk.view3("VOL")[...] = vol
+    cen = 0.125 * (Xn[:-1, :-1, :-1] + Xn[:-1, :-1, 1:] + Xn[:-1, 1:, :-1] + Xn[:-1, 1:, 1:]
+                   + Xn[1:, :-1, :-1] + Xn[1:, :-1, 1:] + Xn[1:, 1:, :-1] + Xn[1:, 1:, 1:])
+    blk.view3("XC")[...] = cen[..., 0]
+    blk.view3("YC")[...] = cen[..., 1]
+    blk.view3("ZC")[...] = cen[..., 2]
+    blk.view3("D1")[...] = np.linalg.norm(Ci[:, :, 1:] - Ci[:, :, :-1], axis=-1)
+    blk.view3("D2")[...] = np.linalg.norm(Cj[:, 1:, :] - Cj[:, :-1, :], axis=-1)
+    blk.view3("D3")[...] = np.linalg.norm(Ck[1:, :, :] - Ck[:-1, :, :], axis=-1)
+    # wall distance to the j = 0 node plane (flat-plate / box cases)
+    blk.view3("DISTW")[...] = np.abs(cen[..., 1] - Xn[:, GHOST:GHOST + 1, :, 1].mean())
+    return blk
+
+
+def fill_state(blk, prm, seed=0, amp=0.01, mach_dir=(1.0, 0.0, 0.0), kind="density",
+               turb=None):
+    """Free stream plus seeded perturbations on every cell (ghosts included).
+
+    kind "density": T = T_inf / (1 + d), d ~ U(-amp, amp) at fixed p (the configs' density
+    perturbation expressed in FINFLO's primitive variables); "velocity": u, v, w perturbed
+    by amp*U_inf; "all": every primitive perturbed (parity tests)."""
+    rng = np.random.default_rng(seed)
+    n = blk.ntot
+    d = np.asarray(mach_dir, float)
+    d = d / np.linalg.norm(d)
+    U = prm.frsvel
+    blk["PDIFF"] = 0.0
+    blk["TEMP"] = prm.frstem
+    blk["U"], blk["V"], blk["W"] = U * d[0], U * d[1], U * d[2]
+    if kind == "density":
+        delta = rng.uniform(-amp, amp, n)
+        blk["TEMP"] = prm.frstem / (1.0 + delta)
+    elif kind == "velocity":
+        for name in ("U", "V", "W"):
+            blk[name] = blk[name] + amp * U * rng.uniform(-1, 1, n)
+    elif kind == "all":
+        blk["PDIFF"] = amp * prm.frspre * rng.uniform(-1, 1, n)
+        blk["TEMP"] = prm.frstem * (1.0 + amp * rng.uniform(-1, 1, n))
+        for name in ("U", "V", "W"):
+            blk[name] = blk[name] + amp * U * rng.uniform(-1, 1, n) * 3
+    else:
+        raise ValueError(kind)
+    if turb is not None:
+        rk, reps = turb
+        blk["RK"] = rk * (1.0 + amp * rng.uniform(-1, 1, n))
+        blk["REPS"] = reps * (1.0 + amp * rng.uniform(-1, 1, n))
+    blk["P"] = blk["PDIFF"] + prm.frspre
+    return blk
+
+
+def make_block(imax, jmax, kmax, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
+               distort=0.0, seed=0, amp=0.01, kind="density", turb=None):
+    """A synthetic block with geometry and state (parity-test cases and workloads)."""
+    prm = prm or default_params()
+    blk = Block(imax, jmax, kmax)
+    xi = stretched(imax, lengths[0])
+    yj = stretched(jmax, lengths[1], ratio=jratio)
+    zk = stretched(kmax, lengths[2])
+    fill_geometry(blk, node_grid(xi, yj, zk, distort=distort, seed=seed + 1000))
+    fill_state(blk, prm, seed=seed, amp=amp, kind=kind, turb=turb)
+    return blk
+
+
+def flat_plate_block(imax, jmax, kmax, prm, first=1e-6, ratio=1.15, length=1.0, width=None,
+                     seed=0, amp=0.005):
+    """Flat-plate block: cosine-clustered i over the plate length, geometric j from the
+    wall with first cell `first` and growth `ratio`, uniform k."""
+    blk = Block(imax, jmax, kmax)
+    xi = cosine_clustered(imax, length)
+    yj = geometric_from_wall(jmax, first * length, ratio)
+    zk = np.linspace(0.0, width if width else 0.5 * length, kmax + 1)
+    fill_geometry(blk, node_grid(xi, yj, zk))
+    fill_state(blk, prm, seed=seed, amp=amp, kind="velocity")
+    return blk

@@ -1,0 +1,150 @@
+/*
+ * ffhip.h — C-ABI of the MI355X (gfx950) implementation of FINFLO's per-iteration hot path.
+ *
+ * The entry points replace FINFLO's per-block, per-level call surface used by CYCLES
+ * (ns3c.f:1354-2349).  Every array is the reference's own layout: one flat FP64 array per
+ * field, i-fastest, IN=JN=KN=2 ghost layers (charac.f:179), the cell (I,J,K) of a block with
+ * IMAX x JMAX x KMAX interior cells at the 0-based offset
+ *     (K+1)*ISTRID*JSTRID + (J+1)*ISTRID + (I+1),   ISTRID=IMAX+4, JSTRID=JMAX+4,
+ * which is the reference's L = (KN+K-1)*ISTRID*JSTRID + (JN+J-1)*ISTRID + IN + I minus one
+ * (res3c.f:127-136, base3c.f:1784-1790).  Face quantities of direction d are stored at the
+ * upper cell of the face (face I-1/2 at cell I), as the reference does (res3c.f:7686-7703).
+ *
+ * No HIP or torch types appear here: streams are passed as void*.
+ * Return codes: 0 on success, a negative FF_E* code on failure; ff_last_error() explains.
+ */
+#ifndef FFHIP_H
+#define FFHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FF_GHOST 2
+
+/* Field slots of a block.  Names follow the reference's MAIN_ARRAYS (charac.f:391-574). */
+enum ff_field {
+    /* solved (primitive) variables: ns3c.f:1517 FIN outputs */
+    FF_PDIFF = 0, FF_TEMP, FF_U, FF_V, FF_W, FF_RK, FF_REPS,
+    /* derived by STATIM (state.f:159 PERFGA, base3c.f:489 CONSVA) and TURBDI */
+    FF_P, FF_RO, FF_E, FF_C, FF_VIS, FF_CH, FF_CP, FF_DRDP, FF_DRDH,
+    FF_RM, FF_RN, FF_RW, FF_EPS2, FF_VIST,
+    /* residuals / increments (RESIDU res3c.f:194-198), local time step (TIME3) */
+    FF_DRO, FF_DM, FF_DN, FF_DW, FF_DE, FF_DRK, FF_DEPS, FF_DTL,
+    /* values at the start of the cycle, read by FIN (res3c.f:1122-1126) */
+    FF_TOLD, FF_UOLD, FF_VOLD, FF_WOLD, FF_POLD, FF_RKOLD, FF_EPSOLD,
+    /* geometry (gluebm.f AREVC1/VOLINC): volume, face areas and unit normals of the
+       i-, j- and k-faces, cell spacings, cell centres, wall distance */
+    FF_VOL, FF_A1, FF_A1X, FF_A1Y, FF_A1Z, FF_A2, FF_A2X, FF_A2Y, FF_A2Z,
+    FF_A3, FF_A3X, FF_A3Y, FF_A3Z, FF_D1, FF_D2, FF_D3, FF_XC, FF_YC, FF_ZC, FF_DISTW,
+    /* k-omega SST work fields (RESIKE res3c.f:3423, TURBEQ res3c.f:5845) */
+    FF_SRK, FF_SEPS, FF_PTUR, FF_FUN1, FF_OHMI,
+    FF_NFIELDS
+};
+
+/* Physical and numerical parameters of a block (BLKS(NGL) and NS3CO scalars). */
+typedef struct ff_params {
+    double gamma;    /* GAMMA, main.f:5306 default 1.4 */
+    double rgas;     /* RGAS 287.05287 */
+    double pr;       /* laminar Prandtl number PR 0.72 */
+    double prt;      /* turbulent Prandtl number PRT 0.9 */
+    double visu0;    /* Sutherland VISU0 1.458e-6 (main.f:5315) */
+    double expsu;    /* Sutherland exponent EXPSU 1.5 */
+    double tsu0;     /* Sutherland TSU0 110.4 */
+    double frspre;   /* free-stream pressure FRSPRE */
+    double frsden;   /* free-stream density FRSDEN */
+    double frstem;   /* free-stream temperature FRSTEM */
+    double frsvel;   /* free-stream speed FRSVEL */
+    double frsvis;   /* free-stream viscosity FRSVIS */
+    double cfl;      /* CFL on the finest level */
+    double cfll;     /* CFLL on the coarse levels */
+    double theta;    /* LU-SGS time-step factor THETA, hard-coded 1.5 (main.f:3499) */
+    double cdiff;    /* TIME3 viscous divisor: 1/CDIFF of the input (main.f:4886), 0.5 */
+    double tuplim;   /* upper temperature limit TUPLIM (6*FRSTEM, main.f:3828) */
+    double tlolim;   /* lower temperature limit TLOLIM (0.05*FRSTEM, main.f:3827) */
+    double rklim;    /* RKLIM floor of rho*k */
+    double epslim;   /* EPSLIM floor of rho*omega */
+    double turlim;   /* TURLIM, upper limit of mu_t/mu */
+    double frsmut;   /* FRSMUT, lower limit of mu_t */
+    double rmult;    /* RMULTV viscosity multiplier, 1 */
+    double reserved[7];
+    int iturb;       /* 0 laminar, 6 k-omega SST */
+    int iflux;       /* 1 Roe (FLUXKE), 0 van Leer (FLUXVL) */
+    int idiff;       /* FLUXDO averaging, 1 = arithmetic (main.f:5754) */
+    int inter[3];    /* MUSCL switch per direction (INTERI/J/K, main.f:5648-5650), -3 */
+    int kover;       /* KOVER of the SST variant */
+    int katol;       /* Kato-Launder production switch */
+    int reserved_i[8];
+} ff_params;
+
+/* One (level M, block N) slab: pointers to the first element of each field, i.e. the
+ * reference's X(IG(M,N)).  Unused slots may be NULL. */
+typedef struct ff_block {
+    int imax, jmax, kmax, reserved;
+    double *f[FF_NFIELDS];
+} ff_block;
+
+typedef struct ff_ctx ff_ctx;
+
+/* Error codes */
+#define FF_OK          0
+#define FF_E_ARG      -1   /* bad argument (NULL pointer, out-of-range index, size) */
+#define FF_E_HIP      -2   /* a HIP runtime call failed */
+#define FF_E_NOMEM    -3   /* device allocation failed */
+#define FF_E_STATE    -4   /* call sequence error (block not registered, ...) */
+#define FF_E_UNSUP    -5   /* configuration outside the implemented path */
+
+/* ---- context ------------------------------------------------------------------------ */
+/* Create a context on the current HIP device.  One stream, one device arena. */
+int  ff_ctx_create(ff_ctx **ctx, const ff_params *prm);
+int  ff_ctx_destroy(ff_ctx *ctx);
+/* Use the caller's HIP stream (a hipStream_t passed as void*); NULL = the context's own. */
+int  ff_ctx_set_stream(ff_ctx *ctx, void *stream);
+const char *ff_last_error(const ff_ctx *ctx);
+/* Per-kernel event timing (0 off, 1 on); results via ff_kernel_stats. */
+int  ff_ctx_profile(ff_ctx *ctx, int enable);
+/* Copy the accumulated timing of kernel `name`: launches and total milliseconds. */
+int  ff_kernel_stats(const ff_ctx *ctx, const char *name, long long *launches, double *ms);
+int  ff_kernel_stats_reset(ff_ctx *ctx);
+
+/* ---- blocks ------------------------------------------------------------------------- */
+/* Register host arrays of block N on level M (the reference's ALLOCA'd slabs).  The
+ * library allocates device mirrors; ff_sync moves data between them. */
+int  ff_block_register(ff_ctx *ctx, int level, int block, const ff_block *host);
+/* Register device-resident arrays (caller-owned device memory, no mirror). */
+int  ff_block_register_device(ff_ctx *ctx, int level, int block, const ff_block *dev);
+/* direction 0: host -> device, 1: device -> host.  field_mask bit f selects field f. */
+int  ff_sync(ff_ctx *ctx, int level, int block, int direction, unsigned long long field_mask);
+/* Synchronise the context's stream. */
+int  ff_finish(ff_ctx *ctx);
+
+/* ---- the hot path (each call works on the device copy of block N, level M) ---------- */
+/* STATIM (ns3c.f:10318): P = PDIFF + FRSPRE (PDTOP state.f:1690), ideal-gas EOS with
+ * Sutherland viscosity (PERFGA state.f:159), sound speed and conservative variables
+ * (CONSVA base3c.f:489), on every cell including ghosts. */
+int  ff_state(ff_ctx *ctx, int level, int block);
+/* TURBDI (ns3c.f:2874) for ITURB=0: EPS2 = 1, VIST = 0 (ADJUIN ns3c.f:2987-2991). */
+int  ff_turb(ff_ctx *ctx, int level, int block);
+/* RESIDI (ns3c.f:2355) -> RESIDU (res3c.f:6): viscous FLUXDO + Roe FLUXKE face fluxes in
+ * the k, i, j directions, SWEEP differences, division by VOL.  With level == 1 the L2
+ * sums of RESL2 (res3c.f:8920) are returned in resi[0..6] (DRO, DM, DN, DW, DE, DRK, DEPS
+ * squared sums; the caller takes sqrt()/N as SUBIN does); resi may be NULL. */
+int  ff_residual(ff_ctx *ctx, int level, int block, int m2, double *resi);
+/* IMPSDI (ns3c.f:2683) -> IMPS (imps3.f:6) with LUSGS=1: TIME3, LOCDT, COPR, the
+ * backward PREDGS, DIAGGS and forward CORRGS sweeps, DRO/CP. */
+int  ff_implicit(ff_ctx *ctx, int level, int block, int m2);
+/* FIN (res3c.f:1077) + the limiters of CYCLES (ns3c.f:1553-1596): UPLIM/LOLIM of TEMP,
+ * U, V, W, PDTOP, LOLIMP. */
+int  ff_update(ff_ctx *ctx, int level, int block);
+/* Copy the solved variables into their *OLD slots (the start-of-cycle values FIN reads). */
+int  ff_save_old(ff_ctx *ctx, int level, int block);
+/* ncycles device-resident iterations of the finest-level sequence of CYCLES for block N:
+ * save_old, STATIM, TURBDI, RESIDI, IMPSDI, FIN + limits.  Ghost cells are left as given
+ * (the caller's boundary routines own them).  resi (may be NULL) receives the RESL2 sums
+ * of the last cycle. */
+int  ff_cycle(ff_ctx *ctx, int block, int ncycles, double *resi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFHIP_H */

@@ -1,0 +1,912 @@
+/*
+ * ff_oracle.c — CPU restatement of FINFLO's per-iteration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product path (finflo_amd, libffhip.so) never
+ * does.  Each function restates one reference routine and keeps that routine's operation
+ * order and literals, so that the HIP path can be compared with it bit for bit.  The
+ * reference is FP64 throughout (-r8 / -fdefault-real-8, Makefile:63,73), so is this file.
+ * Build with -ffp-contract=off (no FMA contraction), as the reference oracle build is.
+ *
+ * Scope restated here (laminar ideal gas, Roe flux, LU-SGS, single-phase, OMEGA=0, no
+ * chimera, ENTROPY_FIX=NO, XXTRAL=NO, FULLNS=NO, IDI* = all cells viscous):
+ *   ffo_state     STATIM ns3c.f:10318 = PDTOP state.f:1690 + PERFGA state.f:159 + CONSVA
+ *                 base3c.f:489
+ *   ffo_turb      TURBDI ITURB=0 -> ADJUIN (ns3c.f:2987-2991)
+ *   ffo_residual  RESIDU res3c.f:6-787 with FLUCYC res3c.f:7572 -> FLUXDO base3c.f:3677 and
+ *                 FLUXKE base3c.f:4753 (XXTRAP base3c.f:750, EFPRO state.f:620, DERIV
+ *                 base3c.f:707, SURFVECTS base3c.f:12245), SWEEP res3c.f:992, RESL2
+ *                 res3c.f:8920
+ *   ffo_implicit  IMPS imps3.f:6 = TIME3 base3c.f:79, LOCDT imps3.f:784, COPR imps3.f:891,
+ *                 PREDGS imps4.f:345, DIAGGS imps4.f:604, CORRGS imps4.f:76 (SYI/SYM1I
+ *                 imps4.f:10-70), DIVV12 by CP (imps3.f:483)
+ *   ffo_update    FIN res3c.f:1077 + UPLIM/LOLIM/PDTOP/LOLIMP of CYCLES ns3c.f:1553-1596
+ *
+ * Parity pins: every function is compared with oracle/_ref (the reference's own Fortran
+ * compiled from its sources, see oracle/Makefile) on seeded blocks, and with the
+ * closed-form checks in tests/test_oracle_math.py.
+ */
+#include <math.h>
+#include <string.h>
+#include <stdlib.h>
+#include "../include/ffhip.h"
+
+#define G FF_GHOST
+
+typedef struct {
+    int imax, jmax, kmax;
+    long is, js, ks, il, ntot;   /* ISTRID, JSTRID, KSTRID, ISTRID*JSTRID, NTOT */
+} dims_t;
+
+static dims_t mkdims(const ff_block *b)
+{
+    dims_t d;
+    d.imax = b->imax; d.jmax = b->jmax; d.kmax = b->kmax;
+    d.is = b->imax + 2 * G; d.js = b->jmax + 2 * G; d.ks = b->kmax + 2 * G;
+    d.il = d.is * d.js; d.ntot = d.il * d.ks;
+    return d;
+}
+
+/* 0-based offset of the reference's cell (I,J,K), 1-based interior indices */
+static inline long cid(const dims_t *d, int i, int j, int k)
+{
+    return (long)(k + 1) * d->il + (long)(j + 1) * d->is + (i + 1);
+}
+
+#define F(b, n) ((b)->f[n])
+
+/* ------------------------------------------------------------------------------------ */
+/* STATIM: PDTOP + PERFGA (EXPSU = 1.5 branch, IUPPTL = 0) + CONSVA                        */
+/* ------------------------------------------------------------------------------------ */
+void ffo_state(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double gm1 = p->gamma - 1.;                /* state.f:180 */
+    const double cp2 = p->gamma / gm1 * p->rgas;     /* state.f:182 */
+    const double cmin = 1.;                          /* base3c.f:509 */
+    const double r23 = 2. / 3.;                      /* base3c.f:511 */
+    double *pd = F(b, FF_PDIFF), *t = F(b, FF_TEMP), *u = F(b, FF_U), *v = F(b, FF_V),
+           *w = F(b, FF_W), *rk = F(b, FF_RK);
+    double *P = F(b, FF_P), *ro = F(b, FF_RO), *e = F(b, FF_E), *c = F(b, FF_C),
+           *vis = F(b, FF_VIS), *ch = F(b, FF_CH), *cp = F(b, FF_CP), *drdp = F(b, FF_DRDP),
+           *drdh = F(b, FF_DRDH), *rm = F(b, FF_RM), *rn = F(b, FF_RN), *rw = F(b, FF_RW);
+    for (long n = 0; n < d.ntot; n++) {
+        /* PDTOP state.f:1707 (ALTITUDE >= GROUND) */
+        P[n] = pd[n] + p->frspre;
+        /* PERFGA state.f:212-229 */
+        double trgs = t[n] * p->rgas;
+        drdp[n] = 1. / trgs;
+        ro[n] = P[n] * drdp[n];
+        vis[n] = p->visu0 * t[n] * p->rmult * sqrt(t[n]) / (t[n] + p->tsu0);
+        e[n] = trgs / gm1;
+        drdh[n] = -ro[n] / e[n] / p->gamma;
+        cp[n] = cp2;
+        ch[n] = vis[n] * cp2 / p->pr;
+        /* CONSVA base3c.f:519-528 (algebraic) and 533-545 (two-equation) */
+        double rkin;
+        if (p->iturb < 3 || p->iturb == 8) {
+            rkin = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]) * .5;
+            double x = 1. / (drdh[n] / ro[n] + drdp[n]);
+            c[n] = sqrt(x > cmin ? x : cmin);
+            e[n] = (e[n] + rkin + 0.) * ro[n];
+        } else {
+            double ypro = 1. / ro[n];
+            double xturb = 1. + r23 * rk[n] * ypro * drdp[n];
+            rkin = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]) * .5;
+            double x = xturb / (drdh[n] * ypro + drdp[n]);
+            c[n] = sqrt(x > cmin ? x : cmin);
+            e[n] = (e[n] + rkin + 0.) * ro[n] + rk[n];
+        }
+        rm[n] = ro[n] * u[n];
+        rn[n] = ro[n] * v[n];
+        rw[n] = ro[n] * w[n];
+    }
+}
+
+/* TURBDI for ITURB = 0: ADJUIN sets EPS2 = 1 and VIST = 0 (ns3c.f:2987-2991) */
+void ffo_turb(const ff_params *p, ff_block *b)
+{
+    (void)p;
+    dims_t d = mkdims(b);
+    for (long n = 0; n < d.ntot; n++) {
+        F(b, FF_EPS2)[n] = 1.;
+        F(b, FF_VIST)[n] = 0.;
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Face fluxes                                                                           */
+/* ------------------------------------------------------------------------------------ */
+
+/* MUSCL kappa-scheme of XXTRAP base3c.f:750-850 for the face between cells L-S and L.
+ * Returns right (ROIP, cell-L side) and left (ROIM) states. */
+static void xxtrap(const double *q, long L, long S, int intem, double *qr, double *ql)
+{
+    static const double rkk1[5] = {0., 1., 1.3333333, 2., 1.5};   /* base3c.f:765 */
+    int inter = intem < 0 ? -intem : intem;
+    int inte2 = inter < 1 ? 1 : (inter > 5 ? 5 : inter);
+    double rk1 = rkk1[inte2 - 1];
+    double rk2 = 2. - rk1;
+    double r1 = .25 * rk1, r2 = .25 * rk2;
+    const double eps = 1.E-10;
+    double q0 = q[L - 2 * S], q1 = q[L - S], q2 = q[L], q3 = q[L + S];
+    if (intem <= 0 && intem >= -5) {                 /* base3c.f:785-803 */
+        double diff0 = q1 - q0, diff1 = q2 - q1, diff2 = q3 - q2;
+        *qr = q2 - (diff1 * r1 + diff2 * r2);
+        *ql = q1 + (diff1 * r1 + diff0 * r2);
+    } else if (inter <= 4) {                          /* base3c.f:807-837 */
+        double diff0 = q1 - q0, diff1 = q2 - q1, diff2 = q3 - q2;
+        double diff12 = diff1 * diff1 + eps;
+        double phibi = (diff12 + diff1 * diff0) / (diff12 + diff0 * diff0);
+        double phifi = (diff12 + diff1 * diff2) / (diff12 + diff2 * diff2);
+        *qr = q2 - phifi * (diff1 * r1 + diff2 * r2);
+        *ql = q1 + phibi * (diff1 * r1 + diff0 * r2);
+    } else {                                          /* base3c.f:841-845 */
+        *qr = q2;
+        *ql = q1;
+    }
+}
+
+/* SURFVECTS base3c.f:12245-12295: tangential unit vectors of a face with unit normal a1 */
+static void surfvects(double a11, double a12, double a13,
+                      double *a21, double *a22, double *a23,
+                      double *a31, double *a32, double *a33)
+{
+    double am = fabs(a11);
+    int idir = 1;
+    if (fabs(a12) < am) { am = fabs(a12); idir = 2; }
+    if (fabs(a13) < am) { idir = 3; }
+    double b1, b2, b3;
+    if (idir == 1) { b1 = 0.0; b2 = -a13; b3 = a12; }
+    else if (idir == 2) { b1 = a13; b2 = 0.0; b3 = -a11; }
+    else { b1 = -a12; b2 = a11; b3 = 0.0; }
+    double palpo = 1. / sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+    b1 = b1 * palpo; b2 = b2 * palpo; b3 = b3 * palpo;
+    *a21 = b1; *a22 = b2; *a23 = b3;
+    *a31 = a12 * b3 - a13 * b2;
+    *a32 = a13 * b1 - a11 * b3;
+    *a33 = a11 * b2 - a12 * b1;
+}
+
+/* EFPRO state.f:636-638 (ISTATE 1), called from FLUXKE with FRSPRE = 0 (base3c.f:4882) */
+static inline double efpro(double pp, double ro, double gamma, double e0ref)
+{
+    double gm1 = gamma - 1.;
+    return (pp + 0.) / (gm1 * ro) - e0ref;
+}
+
+typedef struct { double f[7]; } flux7;
+
+/* FLUXKE base3c.f:4753-5125 at one face (cell L, direction stride S).  `single` selects the
+ * DERIV (base3c.f:719) lower-boundary branch I2 = I that FLUXKE's slab indexing takes on
+ * every k-direction face (local index <= IL for the whole slab). */
+static flux7 roe_face(const ff_params *p, const ff_block *b, long L, long S, int intem,
+                      const double *an, const double *anx, const double *any,
+                      const double *anz, int single)
+{
+    const double *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
+                 *P = F(b, FF_P), *c = F(b, FF_C), *drdp = F(b, FF_DRDP),
+                 *drdh = F(b, FF_DRDH);
+    const int turb = !(p->iturb < 3 || p->iturb == 8);
+    double roip, roim, rmip, rmim, rnip, rnim, rwip, rwim, ppi, pmi;
+    double rkip = 0., rkim = 0., reip = 0., reim = 0.;
+    xxtrap(ro, L, S, intem, &roip, &roim);
+    xxtrap(u, L, S, intem, &rmip, &rmim);
+    xxtrap(v, L, S, intem, &rnip, &rnim);
+    xxtrap(w, L, S, intem, &rwip, &rwim);
+    xxtrap(P, L, S, intem, &ppi, &pmi);
+    if (turb) {
+        xxtrap(F(b, FF_RK), L, S, 9, &rkip, &rkim);   /* INTET = 9: first order */
+        xxtrap(F(b, FF_REPS), L, S, 9, &reip, &reim);
+    }
+    /* floors, base3c.f:4873-4876 */
+    double fd = 0.001 * p->frsden, fp = 0.001 * p->frspre;
+    roip = roip > fd ? roip : fd;   /* MAX(0.001*FRSDEN, ROIP) */
+    ppi = ppi > fp ? ppi : fp;
+    roim = roim > fd ? roim : fd;
+    pmi = pmi > fp ? pmi : fp;
+    double eip = efpro(ppi, roip, p->gamma, 0.);
+    double eim = efpro(pmi, roim, p->gamma, 0.);
+    double elr = efpro(ppi, roim, p->gamma, 0.);
+    double erl = efpro(pmi, roip, p->gamma, 0.);
+    /* DERIV base3c.f:707-742 */
+    long i2 = single ? L : L - S;
+    double ypro1 = 1. / ro[L], ypro2 = 1. / ro[i2];
+    double dedp1 = -drdp[L] / drdh[L] - ypro1;
+    double dedp2 = -drdp[i2] / drdh[i2] - ypro2;
+    double dedph = 0.5 * (dedp1 + dedp2);
+    double dedr1 = 1. / drdh[L] + P[L] * (ypro1 * ypro1);
+    double dedr2 = 1. / drdh[i2] + P[i2] * (ypro2 * ypro2);
+    double dedrh = 0.5 * (dedr1 + dedr2);
+    {
+        double dp = 2. * (ppi - pmi);
+        double dro = 2. * (roip - roim);
+        if (fabs(dp) >= 1.E-2 * (ppi + pmi)) dedph = (eip + elr - eim - erl) / dp;
+        if (fabs(dro) >= 1.E-2 * (roip + roim)) dedrh = (eip + erl - eim - elr) / dro;
+    }
+    /* loop 600, base3c.f:4894-5120 (RKLIM = EPSLIM = 0 in FLUCYC res3c.f:7680-7681) */
+    double rkipi = rkip > 0.001 * 0. ? rkip : 0.001 * 0.;
+    double reipi = reip > 0.001 * 0. ? reip : 0.001 * 0.;
+    double rkimi = rkim > 0.001 * 0. ? rkim : 0.001 * 0.;
+    double reimi = reim > 0.001 * 0. ? reim : 0.001 * 0.;
+    double a11 = anx[L], a12 = any[L], a13 = anz[L];
+    double a21, a22, a23, a31, a32, a33;
+    surfvects(a11, a12, a13, &a21, &a22, &a23, &a31, &a32, &a33);
+    double ur = a11 * rmip + a12 * rnip + a13 * rwip;
+    double vr = a21 * rmip + a22 * rnip + a23 * rwip;
+    double wr = a31 * rmip + a32 * rnip + a33 * rwip;
+    double ul = a11 * rmim + a12 * rnim + a13 * rwim;
+    double vl = a21 * rmim + a22 * rnim + a23 * rwim;
+    double wl = a31 * rmim + a32 * rnim + a33 * rwim;
+
+    double ypror = 1. / roip;
+    double rkr = rkipi * ypror;
+    double yprol = 1. / roim;
+    double rkl = rkimi * yprol;
+    double eir = eip + .5 * (ur * ur + vr * vr + wr * wr) + rkr;
+    double eil = eim + .5 * (ul * ul + vl * vl + wl * wl) + rkl;
+    double pr = ppi;
+    double pkr = .6667 * rkipi;
+    double epsr = reipi * ypror;
+    double hr = eir + pr * ypror;
+    double er = eip;
+    double rohat = sqrt(roip * roim);
+    double yproh = 1. / rohat;
+    double pl = pmi;
+    double pkl = .6667 * rkimi;
+    double epsl = reimi * yprol;
+    double hl = eil + yprol * pl;
+    double el = eim;
+    double sqrl = 1. / (roim + rohat);
+    double cql = roim * sqrl;
+    double cqr = 1. - cql;
+    double uhat = cqr * ur + cql * ul;
+    double vhat = cqr * vr + cql * vl;
+    double what = cqr * wr + cql * wl;
+    double hhat = cqr * hr + cql * hl;
+    double ehat = cqr * er + cql * el;
+    double rkhat = cqr * rkr + cql * rkl;
+    double epshat = cqr * epsr + cql * epsl;
+    double phat = rohat * (hhat - ehat - .5 * (uhat * uhat + vhat * vhat + what * what) - rkhat);
+    double pmin = pr < pl ? pr : pl;                 /* MIN(PR,PL) */
+    double pmax = pr > pl ? pr : pl;
+    phat = pmax < phat ? pmax : phat;                /* MIN(PMAX,PHAT) */
+    phat = pmin > phat ? pmin : phat;                /* MAX(PMIN,PHAT) */
+    double drdhh = 1. / (dedrh - phat * (yproh * yproh));
+    double drdph = -drdhh * (dedph + yproh);
+    double xturb = 1. + .6667 * rkhat * yproh * drdph;
+    double chat2 = xturb / (drdhh * yproh + drdph);
+    double cmin = c[L] < c[L - S] ? c[L] : c[L - S];
+    double cmax = c[L] > c[L - S] ? c[L] : c[L - S];
+    chat2 = (cmax * cmax) < chat2 ? (cmax * cmax) : chat2;
+    chat2 = (cmin * cmin) > chat2 ? (cmin * cmin) : chat2;
+    double chat = sqrt(chat2);
+    double pdpde = dedph;
+    double dro = roip - roim;
+    double du = ur - ul;
+    double dk = rkr - rkl;
+    double dp = pr - pl + .6667 * (rkhat * dro + rohat * dk);
+    double pchat2 = 1. / (chat * chat);
+    double alfa1 = .5 * (dp + rohat * chat * du) * pchat2;
+    double alfa2 = .5 * (dp - rohat * chat * du) * pchat2;
+    double alfa3 = dro - dp * pchat2;
+    double alfa4 = rohat * (vr - vl);
+    double alfa5 = rohat * (wr - wl);
+    double alfa6 = rohat * dk;
+    double alfa7 = rohat * (epsr - epsl);
+    /* UROT = 0 (non-rotating), ENTROPY_FIX = NO */
+    double rl1 = fabs(uhat + chat);
+    double rl2 = fabs(uhat - chat);
+    double rl3 = fabs(uhat);
+    double rl4 = rl3, rl5 = rl3, rl6 = rl3, rl7 = rl3;
+    double roucr = roip * ur;
+    double roucl = roim * ul;
+    double damp = rl1 * alfa1 + rl2 * alfa2 + rl3 * alfa3;
+    double ha = .5 * an[L];
+    flux7 out;
+    double f1 = ha * (roucr + roucl - damp);
+    double f2 = ha * (roucr * ur + pr + pkr + roucl * ul + pl + pkl -
+                      (damp * uhat + (rl1 * alfa1 - rl2 * alfa2) * chat));
+    double f3 = ha * (roucr * vr + roucl * vl - damp * vhat - rl4 * alfa4);
+    double f4 = ha * (roucr * wr + roucl * wl - damp * what - rl5 * alfa5);
+    double f5 = ha * (roucr * (hr + pkr * ypror) + roucl * (hl + pkl * yprol)
+                      - damp * hhat - ((rl1 * alfa1 - rl2 * alfa2) * uhat * chat
+                      - rl3 * alfa3 * rohat * (chat * chat) * pdpde + rl4 * alfa4 * vhat
+                      + rl5 * alfa5 * what) + rl6 * alfa6 * (1. - .6667 * rohat * pdpde));
+    double f6 = ha * (roucr * rkr + roucl * rkl - damp * rkhat - rl6 * alfa6);
+    double f7 = ha * (roucr * epsr + roucl * epsl - damp * epshat - rl7 * alfa7);
+    out.f[0] = f1;
+    out.f[1] = a11 * f2 + a21 * f3 + a31 * f4;
+    out.f[2] = a12 * f2 + a22 * f3 + a32 * f4;
+    out.f[3] = a13 * f2 + a23 * f3 + a33 * f4;
+    out.f[4] = f5;
+    out.f[5] = f6;
+    out.f[6] = f7;
+    return out;
+}
+
+/* FLUXDO base3c.f:3677-3869 at one face, IDIFF <= 3 (arithmetic average).
+ * Returns the momentum fluxes (which FLUXDO stores) and the energy increment added to FE. */
+static void visc_face(const ff_params *p, const ff_block *b, long L, long S,
+                      const double *an, const double *anx, const double *any,
+                      const double *anz, double *fm, double *fn, double *fw, double *fe)
+{
+    const double *vol = F(b, FF_VOL), *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS),
+                 *cp = F(b, FF_CP), *vist = F(b, FF_VIST), *ch = F(b, FF_CH),
+                 *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W), *t = F(b, FF_TEMP);
+    long m = L - S;
+    double ypd2 = an[L] / (vol[L] + vol[m]);
+    double surg = -an[L] * ypd2;
+    double surf = surg * (eps2[L] * vis[L] + eps2[m] * vis[m]);
+    double surfe = surg * (cp[L] * vist[L] / p->prt + ch[L] + cp[m] * vist[m] / p->prt + ch[m]);
+    double duc = .333333 * (anx[L] * (u[L] - u[m]) + any[L] * (v[L] - v[m]) + anz[L] * (w[L] - w[m]));
+    double f2 = surf * (u[L] - u[m] + anx[L] * duc);
+    double f3 = surf * (v[L] - v[m] + any[L] * duc);
+    double f4 = surf * (w[L] - w[m] + anz[L] * duc);
+    double f5 = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4);
+    *fm = f2; *fn = f3; *fw = f4;
+    *fe = f5 + surfe * (t[L] - t[m]);
+}
+
+/* FLUCYC for one direction (dir 0=i, 1=j, 2=k) followed by SWEEP.  Face fluxes are formed
+ * for the faces adjacent to interior cells only; the reference also forms fluxes on ghost
+ * rows and columns (FLUXKE loop 1000), which no interior residual reads. */
+static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last)
+{
+    dims_t d = mkdims(b);
+    long S = dir == 0 ? 1 : (dir == 1 ? d.is : d.il);
+    int nd[3] = {d.imax, d.jmax, d.kmax};
+    const double *an = F(b, FF_A1 + 4 * dir), *anx = F(b, FF_A1X + 4 * dir),
+                 *any = F(b, FF_A1Y + 4 * dir), *anz = F(b, FF_A1Z + 4 * dir);
+    int intem = p->inter[dir];
+    int turb = !(p->iturb < 3 || p->iturb == 8);
+    /* face flux arrays F1R..F1E (F1RK, F1EPS): zeroed (res3c.f:7705-7710) */
+    double *ff[7];
+    for (int q = 0; q < 7; q++) ff[q] = (double *)calloc((size_t)d.ntot, sizeof(double));
+    int lo[3] = {1, 1, 1}, hi[3] = {nd[0], nd[1], nd[2]};
+    hi[dir] = nd[dir] + 1;
+    for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) {
+                long L = cid(&d, i, j, k);
+                double fm, fn, fw, fe;
+                /* FLUXDO first (res3c.f:7856), FRO untouched by FLUXDO */
+                visc_face(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &fe);
+                ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
+                ff[4][L] = ff[4][L] + fe;
+                /* then FLUXKE (res3c.f:7892) adds the inviscid flux */
+                flux7 r = roe_face(p, b, L, S, intem, an, anx, any, anz, dir == 2);
+                ff[0][L] = r.f[0] + ff[0][L];
+                ff[1][L] = r.f[1] + ff[1][L];
+                ff[2][L] = r.f[2] + ff[2][L];
+                ff[3][L] = r.f[3] + ff[3][L];
+                ff[4][L] = r.f[4] + ff[4][L];
+                if (turb) {
+                    ff[5][L] = r.f[5] + ff[5][L];
+                    ff[6][L] = r.f[6] + ff[6][L];
+                }
+            }
+    /* SWEEP res3c.f:992-1036 */
+    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    const double *vol = F(b, FF_VOL);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                if (!last) {
+                    for (int q = 0; q < 5; q++) D[q][n] = D[q][n] - (ff[q][n + S] - ff[q][n]);
+                } else {
+                    double dt1 = 1. / vol[n];
+                    for (int q = 0; q < 5; q++) D[q][n] = (D[q][n] - (ff[q][n + S] - ff[q][n])) * dt1;
+                }
+            }
+    for (int q = 0; q < 7; q++) free(ff[q]);
+}
+
+/* RESIDU res3c.f:6-787 (laminar): zero the residuals, k-, i- and j-direction fluxes with a
+ * SWEEP after each, the last one dividing by VOL; then RESL2 sums on level 1. */
+void ffo_residual(const ff_params *p, ff_block *b, int level, double *resi)
+{
+    dims_t d = mkdims(b);
+    for (int q = FF_DRO; q <= FF_DEPS; q++)
+        if (F(b, q)) memset(F(b, q), 0, sizeof(double) * (size_t)d.ntot);
+    direction_sweep(p, b, 2, 0);   /* res3c.f:270 then SWEEP res3c.f:330 */
+    direction_sweep(p, b, 0, 0);   /* res3c.f:402 then SWEEP res3c.f:448 */
+    direction_sweep(p, b, 1, 1);   /* res3c.f:515 then SWEEP res3c.f:559 */
+    if (level == 1 && resi) {
+        /* RESL2 res3c.f:8979-9006 */
+        for (int q = 0; q < 7; q++) resi[q] = 0.;
+        for (int k = 1; k <= d.kmax; k++)
+            for (int j = 1; j <= d.jmax; j++)
+                for (int i = 1; i <= d.imax; i++) {
+                    long n = cid(&d, i, j, k);
+                    resi[0] += F(b, FF_DRO)[n] * F(b, FF_DRO)[n];
+                    resi[1] += F(b, FF_DM)[n] * F(b, FF_DM)[n];
+                    resi[2] += F(b, FF_DN)[n] * F(b, FF_DN)[n];
+                    resi[3] += F(b, FF_DW)[n] * F(b, FF_DW)[n];
+                    resi[4] += F(b, FF_DE)[n] * F(b, FF_DE)[n];
+                }
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Implicit stage                                                                        */
+/* ------------------------------------------------------------------------------------ */
+
+/* TIME3 base3c.f:79-220 with IVISC = 1, TIMEL = .FALSE., UROT = 0, all cells viscous */
+static void time3(const ff_params *p, ff_block *b, double cfl)
+{
+    dims_t d = mkdims(b);
+    const double eps = 1.E-10, pvisc = 1.8;
+    const double pcfl = 1. / cfl;
+    const double prs = p->pr / p->prt;
+    const double *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z),
+                 *vol = F(b, FF_VOL), *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *c = F(b, FF_C), *P = F(b, FF_P), *vis = F(b, FF_VIS),
+                 *vist = F(b, FF_VIST);
+    double *dtl = F(b, FF_DTL);
+    long IS = d.is, IL = d.il;
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long kk = cid(&d, i, j, k);
+                /* ZZZ: maximum half-sum face area (loops 1000-3000) */
+                double z = .5 * (a1[kk] + a1[kk + 1]);
+                double r2 = .5 * (a2[kk] + a2[kk + IS]);
+                z = r2 > z ? r2 : z;
+                double r3 = .5 * (a3[kk] + a3[kk + IL]);
+                z = r3 > z ? r3 : z;
+                double pvol = 1. / vol[kk];
+                double ypro = 1. / (ro[kk] + eps);
+                double a1k = .5 * (a1[kk] + a1[kk + 1]);
+                double a2k = .5 * (a2[kk] + a2[kk + IS]);
+                double a3k = .5 * (a3[kk] + a3[kk + IL]);
+                double a1xk = .5 * (a1x[kk] * a1[kk] + a1x[kk + 1] * a1[kk + 1]);
+                double a1yk = .5 * (a1y[kk] * a1[kk] + a1y[kk + 1] * a1[kk + 1]);
+                double a1zk = .5 * (a1z[kk] * a1[kk] + a1z[kk + 1] * a1[kk + 1]);
+                double a2xk = .5 * (a2x[kk] * a2[kk] + a2x[kk + IS] * a2[kk + IS]);
+                double a2yk = .5 * (a2y[kk] * a2[kk] + a2y[kk + IS] * a2[kk + IS]);
+                double a2zk = .5 * (a2z[kk] * a2[kk] + a2z[kk + IS] * a2[kk + IS]);
+                double a3xk = .5 * (a3x[kk] * a3[kk] + a3x[kk + IL] * a3[kk + IL]);
+                double a3yk = .5 * (a3y[kk] * a3[kk] + a3y[kk + IL] * a3[kk + IL]);
+                double a3zk = .5 * (a3z[kk] * a3[kk] + a3z[kk + IL] * a3[kk + IL]);
+                double ucii = a1xk * u[kk] + a1yk * v[kk] + a1zk * w[kk] - 0.;
+                double vcii = a2xk * u[kk] + a2yk * v[kk] + a2zk * w[kk] - 0.;
+                double wcii = a3xk * u[kk] + a3yk * v[kk] + a3zk * w[kk] - 0.;
+                double b1 = pvol * (fabs(ucii) + fabs(vcii) + fabs(wcii)
+                                    + c[kk] * sqrt(a1k * a1k + a2k * a2k + a3k * a3k)) * pcfl;
+                double rnum = (z * z) * (pvol * pvol) * ypro;
+                double t = rnum * (vis[kk] + prs * vist[kk]) / p->cdiff;
+                double b2 = 2. * t * pvisc;
+                double b11 = b1 > b2 ? b1 : b2;
+                double x = (p->frspre - 2. * P[kk]) / P[kk];
+                double denlow = 1. + (0. > x ? 0. : x);
+                double y = (ro[kk] - 2. * p->frsden) * ypro;
+                double denmax = 1. + (0. > y ? 0. : y) * 5.;
+                denlow = denlow > denmax ? denlow : denmax;
+                dtl[kk] = 1. / (b11 * denlow);
+            }
+}
+
+/* SYI imps4.f:10-40: primitive increments -> characteristic variables at a face */
+static void syi(double *w1, double *w2, double *w3, double *w4, double *w5,
+                double x1, double x2, double x3, double x4, double x5, double c, double ro,
+                double a2x, double a2y, double a2z,
+                double *a21, double *a22, double *a23, double *a31, double *a32, double *a33)
+{
+    double z1i = x1, z5i = x5;
+    surfvects(a2x, a2y, a2z, a21, a22, a23, a31, a32, a33);
+    double z2i = a2x * x2 + a2y * x3 + a2z * x4;
+    double z3i = *a21 * x2 + *a22 * x3 + *a23 * x4;
+    double z4i = *a31 * x2 + *a32 * x3 + *a33 * x4;
+    double roc = ro * c;
+    *w1 = z1i - z5i / ro;
+    *w2 = roc * z2i + z5i;
+    *w3 = z3i;
+    *w4 = z4i;
+    *w5 = -roc * z2i + z5i;
+}
+
+/* SYM1I imps4.f:46-70: characteristic -> primitive increments */
+static void sym1i(double *x1, double *x2, double *x3, double *x4, double *x5,
+                  double w1, double w2, double w3, double w4, double w5, double c, double ro,
+                  double a2x, double a2y, double a2z,
+                  double a21, double a22, double a23, double a31, double a32, double a33)
+{
+    double ypro = 1. / ro;
+    double pc2 = .5 * ypro;
+    double proc = pc2 / c;
+    *x1 = w1 + pc2 * (w2 + w5);
+    double x2i = proc * (w2 - w5);
+    double x3i = w3, x4i = w4;
+    *x5 = .5 * (w2 + w5);
+    *x2 = a2x * x2i + a21 * x3i + a31 * x4i;
+    *x3 = a2y * x2i + a22 * x3i + a32 * x4i;
+    *x4 = a2z * x2i + a23 * x3i + a33 * x4i;
+}
+
+/* Split-eigenvalue scaling of an increment across one face and back:
+ * SYM1I(XI * SYI(dW)) as used in PREDGS/CORRGS (imps4.f:252-290, 513-552, 316-328). */
+static void face_contrib(const double dw[5], double c, double ro, double nx, double ny,
+                         double nz, double x1, double x2, double x5, double out[5])
+{
+    double a21, a22, a23, a31, a32, a33, w1, w2, w3, w4, w5;
+    syi(&w1, &w2, &w3, &w4, &w5, dw[0], dw[1], dw[2], dw[3], dw[4], c, ro, nx, ny, nz,
+        &a21, &a22, &a23, &a31, &a32, &a33);
+    w1 = x1 * w1;
+    w2 = x2 * w2;
+    w3 = x1 * w3;
+    w4 = x1 * w4;
+    w5 = x5 * w5;
+    sym1i(&out[0], &out[1], &out[2], &out[3], &out[4], w1, w2, w3, w4, w5, c, ro, nx, ny, nz,
+          a21, a22, a23, a31, a32, a33);
+}
+
+/* PREDGS imps4.f:345-598: backward sweep, K = KMAX..1, diagonals M = I+J descending */
+static void predgs(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double cdiff = 2.;
+    long IS = d.is, IL = d.il;
+    const double *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z),
+                 *vol = F(b, FF_VOL), *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *c = F(b, FF_C), *vis = F(b, FF_VIS), *vist = F(b, FF_VIST),
+                 *dtl = F(b, FF_DTL);
+    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    long slab = (long)d.jmax * IS;
+    double *cd = calloc((size_t)slab * 5, sizeof(double));          /* CDRO..CDE(L) */
+    double *xi = malloc(sizeof(double) * (size_t)slab * 9);          /* XI1,2,5 XJ.. XK.. */
+    double *dt = malloc(sizeof(double) * (size_t)slab);
+    double *scinv = malloc(sizeof(double) * (size_t)slab);
+    double *scinp = malloc(sizeof(double) * (size_t)slab);
+    double *ydi = calloc((size_t)(d.imax + 2) * 5, sizeof(double));   /* YD*I(I) */
+    double *bd = calloc((size_t)(d.imax + 2) * 5, sizeof(double));    /* BD*(I) */
+    for (int k = d.kmax; k >= 1; k--) {
+        double ffk = 2.;
+        if (k == 1) ffk = 4.;
+        double cdiffk = ffk * cdiff;
+        for (int j = 1; j <= d.jmax; j++) {
+            double ffj = 2.;
+            if (j == 1) ffj = 4.;
+            double cdiffj = cdiff * ffj;
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                long l = (long)(j - 1) * IS + G + i - 1;   /* local slab index, 0-based */
+                double ffi = 2.;
+                if (i == 1) ffi = 4.;
+                double cdiffi = ffi * cdiff;
+                double rl1, rl2, rl5, rnum, t, rimax, rjmax, rkmax, xia, xja, xka;
+                rl1 = a1x[n] * u[n] + a1y[n] * v[n] + a1z[n] * w[n] - 0.;
+                rl2 = rl1 + c[n];
+                rl5 = rl1 - c[n];
+                rnum = (vol[n] + vol[n - 1]) * (ro[n] + ro[n - 1]);
+                t = cdiffi * a1[n] / rnum * (vist[n] + vis[n] + vist[n - 1] + vis[n - 1]);
+                rimax = 1.00 * (fabs(rl1) + c[n]);
+                xi[9 * l + 0] = (rl1 - rimax) * .5 - t;
+                xi[9 * l + 1] = (rl2 - rimax) * .5 - t;
+                xi[9 * l + 2] = (rl5 - rimax) * .5 - t;
+                xia = rimax + t;
+                rl1 = a2x[n] * u[n] + a2y[n] * v[n] + a2z[n] * w[n] - 0.;
+                rl2 = rl1 + c[n];
+                rl5 = rl1 - c[n];
+                rnum = (vol[n] + vol[n - IS]) * (ro[n] + ro[n - IS]);
+                t = cdiffj * a2[n] / rnum * (vist[n] + vis[n] + vist[n - IS] + vis[n - IS]);
+                rjmax = 1.00 * (fabs(rl1) + c[n]);
+                xi[9 * l + 3] = (rl1 - rjmax) * .5 - t;
+                xi[9 * l + 4] = (rl2 - rjmax) * .5 - t;
+                xi[9 * l + 5] = (rl5 - rjmax) * .5 - t;
+                xja = rjmax + t;
+                rl1 = a3x[n] * u[n] + a3y[n] * v[n] + a3z[n] * w[n] - 0.;
+                rl2 = rl1 + c[n];
+                rl5 = rl1 - c[n];
+                rnum = (vol[n] + vol[n - IL]) * (ro[n] + ro[n - IL]);
+                t = cdiffk * a3[n] / rnum * (vist[n] + vis[n] + vist[n - IL] + vis[n - IL]);
+                rkmax = 1.00 * (fabs(rl1) + c[n]);
+                xi[9 * l + 6] = (rl1 - rkmax) * .5 - t;
+                xi[9 * l + 7] = (rl2 - rkmax) * .5 - t;
+                xi[9 * l + 8] = (rl5 - rkmax) * .5 - t;
+                xka = rkmax + t;
+                double dt1 = p->theta * dtl[n];
+                dt[l] = dt1;
+                for (int q = 0; q < 5; q++)
+                    cd[5 * l + q] = -dt1 * a3[n + IL] * cd[5 * l + q] + vol[n] * D[q][n];
+                scinv[l] = 1. / (vol[n] + dt1 * (xia * a1[n] + xja * a2[n] + xka * a3[n]) + vol[n] * 0.);
+                scinp[l] = 1. / (vol[n] + dt1 * (rimax * a1[n] + rjmax * a2[n] + rkmax * a3[n]) + vol[n] * 0.);
+            }
+        }
+        memset(ydi, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
+        memset(bd, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
+        for (int m = d.imax + d.jmax; m >= 2; m--) {
+            int ialku = m - d.jmax > 1 ? m - d.jmax : 1;
+            int iloppu = d.imax < m - 1 ? d.imax : m - 1;
+            for (int i = ialku; i <= iloppu; i++) {
+                int j = m - i;
+                long n = cid(&d, i, j, k);
+                long l = (long)(j - 1) * IS + G + i - 1;
+                double dt1 = dt[l];
+                for (int q = 0; q < 5; q++)
+                    cd[5 * l + q] = cd[5 * l + q] - dt1 * (a1[n + 1] * ydi[5 * i + q] + a2[n + IS] * bd[5 * i + q]);
+                for (int q = 0; q < 4; q++) cd[5 * l + q] = cd[5 * l + q] * scinv[l];
+                cd[5 * l + 4] = cd[5 * l + 4] * scinp[l];
+                double dw[5];
+                for (int q = 0; q < 5; q++) { D[q][n] = cd[5 * l + q]; dw[q] = D[q][n]; }
+                face_contrib(dw, c[n], ro[n], a1x[n], a1y[n], a1z[n],
+                             xi[9 * l + 0], xi[9 * l + 1], xi[9 * l + 2], &ydi[5 * i]);
+                face_contrib(dw, c[n], ro[n], a2x[n], a2y[n], a2z[n],
+                             xi[9 * l + 3], xi[9 * l + 4], xi[9 * l + 5], &bd[5 * i]);
+            }
+            for (int i = ialku; i <= iloppu - 1; i++)
+                for (int q = 0; q < 5; q++) ydi[5 * i + q] = ydi[5 * (i + 1) + q];
+            for (int q = 0; q < 5; q++) ydi[5 * iloppu + q] = 0.;
+        }
+        /* loop 1600: k-direction contribution for the next slab (interior cells only;
+           ghost-column entries are never read by interior cells) */
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                long l = (long)(j - 1) * IS + G + i - 1;
+                double dw[5];
+                for (int q = 0; q < 5; q++) dw[q] = D[q][n];
+                face_contrib(dw, c[n], ro[n], a3x[n], a3y[n], a3z[n],
+                             xi[9 * l + 6], xi[9 * l + 7], xi[9 * l + 8], &cd[5 * l]);
+            }
+    }
+    free(cd); free(xi); free(dt); free(scinv); free(scinp); free(ydi); free(bd);
+}
+
+/* DIAGGS imps4.f:604-664 (interior cells) */
+static void diaggs(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    long IS = d.is, IL = d.il;
+    const double *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z),
+                 *vol = F(b, FF_VOL), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
+                 *c = F(b, FF_C), *dtl = F(b, FF_DTL);
+    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                double ucon;
+                ucon = a1x[n + 1] * u[n] + a1y[n + 1] * v[n] + a1z[n + 1] * w[n] - 0.;
+                double r1n = 1.00 * (fabs(ucon) + c[n]);
+                ucon = a1x[n] * u[n] + a1y[n] * v[n] + a1z[n] * w[n] - 0.;
+                double r1p = 1.00 * (fabs(ucon) + c[n]);
+                ucon = a2x[n + IS] * u[n] + a2y[n + IS] * v[n] + a2z[n + IS] * w[n] - 0.;
+                double r2n = 1.00 * (fabs(ucon) + c[n]);
+                ucon = a2x[n] * u[n] + a2y[n] * v[n] + a2z[n] * w[n] - 0.;
+                double r2p = 1.00 * (fabs(ucon) + c[n]);
+                ucon = a3x[n + IL] * u[n] + a3y[n + IL] * v[n] + a3z[n + IL] * w[n] - 0.;
+                double r3n = 1.00 * (fabs(ucon) + c[n]);
+                ucon = a3x[n] * u[n] + a3y[n] * v[n] + a3z[n] * w[n] - 0.;
+                double r3p = 1.00 * (fabs(ucon) + c[n]);
+                double dt1 = p->theta * dtl[n] / vol[n];
+                double sc = .5 * (a1[n] * r1p + a1[n + 1] * r1n + a2[n] * r2p + a2[n + IS] * r2n +
+                                  a3[n] * r3p + a3[n + IL] * r3n);
+                double fac = 1. + dt1 * sc + 0.;
+                for (int q = 0; q < 5; q++) D[q][n] = D[q][n] * fac;
+            }
+}
+
+/* CORRGS imps4.f:76-339: forward sweep, K = 1..KMAX, diagonals M = I+J ascending */
+static void corrgs(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double cdiff = 2.;
+    long IS = d.is, IL = d.il;
+    const double *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z),
+                 *vol = F(b, FF_VOL), *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *c = F(b, FF_C), *vis = F(b, FF_VIS), *vist = F(b, FF_VIST),
+                 *dtl = F(b, FF_DTL);
+    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    long slab = (long)d.jmax * IS;
+    double *cd = calloc((size_t)slab * 5, sizeof(double));
+    double *xi = malloc(sizeof(double) * (size_t)slab * 9);
+    double *dt = malloc(sizeof(double) * (size_t)slab);
+    double *scinv = malloc(sizeof(double) * (size_t)slab);
+    double *scinp = malloc(sizeof(double) * (size_t)slab);
+    double *ydi = calloc((size_t)(d.imax + 2) * 5, sizeof(double));
+    double *bd = calloc((size_t)(d.imax + 2) * 5, sizeof(double));
+    for (int k = 1; k <= d.kmax; k++) {
+        double cdiffk = 2. * cdiff;
+        for (int j = 1; j <= d.jmax; j++) {
+            double cdiffj = cdiff * 2.;
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                long l = (long)(j - 1) * IS + G + i - 1;
+                double cdiffi = 2. * cdiff;
+                double rl1, rl2, rl5, rnum, t, rimax, rjmax, rkmax, xia, xja, xka;
+                rl1 = a1x[n + 1] * u[n] + a1y[n + 1] * v[n] + a1z[n + 1] * w[n] - 0.;
+                rl2 = rl1 + c[n];
+                rl5 = rl1 - c[n];
+                rnum = (vol[n] + vol[n + 1]) * (ro[n] + ro[n + 1]);
+                t = cdiffi * a1[n] / rnum * (vist[n] + vis[n] + vist[n + 1] + vis[n + 1]);
+                rimax = 1.00 * (fabs(rl1) + c[n]);
+                xi[9 * l + 0] = (rl1 + rimax) * .5 + t;
+                xi[9 * l + 1] = (rl2 + rimax) * .5 + t;
+                xi[9 * l + 2] = (rl5 + rimax) * .5 + t;
+                xia = rimax + t;
+                rl1 = a2x[n + IS] * u[n] + a2y[n + IS] * v[n] + a2z[n + IS] * w[n] - 0.;
+                rl2 = rl1 + c[n];
+                rl5 = rl1 - c[n];
+                rnum = (vol[n] + vol[n + IS]) * (ro[n] + ro[n + IS]);
+                t = cdiffj * a2[n] / rnum * (vist[n] + vis[n] + vist[n + IS] + vis[n + IS]);
+                rjmax = 1.00 * (fabs(rl1) + c[n]);
+                xi[9 * l + 3] = (rl1 + rjmax) * .5 + t;
+                xi[9 * l + 4] = (rl2 + rjmax) * .5 + t;
+                xi[9 * l + 5] = (rl5 + rjmax) * .5 + t;
+                xja = rjmax + t;
+                rl1 = a3x[n + IL] * u[n] + a3y[n + IL] * v[n] + a3z[n + IL] * w[n] - 0.;
+                rl2 = rl1 + c[n];
+                rl5 = rl1 - c[n];
+                rnum = (vol[n] + vol[n + IL]) * (ro[n] + ro[n + IL]);
+                t = cdiffk * a3[n] / rnum * (vist[n] + vis[n] + vist[n + IL] + vis[n + IL]);
+                rkmax = 1.00 * (fabs(rl1) + c[n]);
+                xi[9 * l + 6] = (rl1 + rkmax) * .5 + t;
+                xi[9 * l + 7] = (rl2 + rkmax) * .5 + t;
+                xi[9 * l + 8] = (rl5 + rkmax) * .5 + t;
+                xka = rkmax + t;
+                double dt1 = p->theta * dtl[n];
+                dt[l] = dt1;
+                for (int q = 0; q < 5; q++)
+                    cd[5 * l + q] = dt1 * a3[n] * cd[5 * l + q] + vol[n] * D[q][n];
+                scinv[l] = 1. / (vol[n] + dt1 * (xia * a1[n + 1] + xja * a2[n + IS] + xka * a3[n + IL]) + vol[n] * 0.);
+                scinp[l] = 1. / (vol[n] + dt1 * (rimax * a1[n] + rjmax * a2[n] + rkmax * a3[n]) + vol[n] * 0.);
+            }
+        }
+        memset(ydi, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
+        memset(bd, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
+        for (int m = 2; m <= d.imax + d.jmax; m++) {
+            int ialku = m - d.jmax > 1 ? m - d.jmax : 1;
+            int iloppu = d.imax < m - 1 ? d.imax : m - 1;
+            for (int i = ialku; i <= iloppu; i++) {
+                int j = m - i;
+                long n = cid(&d, i, j, k);
+                long l = (long)(j - 1) * IS + G + i - 1;
+                double dt1 = dt[l];
+                for (int q = 0; q < 5; q++)
+                    cd[5 * l + q] = cd[5 * l + q] + dt1 * (a1[n] * ydi[5 * i + q] + a2[n] * bd[5 * i + q]);
+                for (int q = 0; q < 4; q++) cd[5 * l + q] = cd[5 * l + q] * scinv[l];
+                cd[5 * l + 4] = cd[5 * l + 4] * scinp[l];
+                double dw[5];
+                for (int q = 0; q < 5; q++) { D[q][n] = cd[5 * l + q]; dw[q] = D[q][n]; }
+                face_contrib(dw, c[n], ro[n], a1x[n + 1], a1y[n + 1], a1z[n + 1],
+                             xi[9 * l + 0], xi[9 * l + 1], xi[9 * l + 2], &ydi[5 * i]);
+                face_contrib(dw, c[n], ro[n], a2x[n + IS], a2y[n + IS], a2z[n + IS],
+                             xi[9 * l + 3], xi[9 * l + 4], xi[9 * l + 5], &bd[5 * i]);
+            }
+            int lo = ialku + 1 > 2 ? ialku + 1 : 2;
+            for (int i = iloppu + 1; i >= lo; i--)
+                for (int q = 0; q < 5; q++) ydi[5 * i + q] = ydi[5 * (i - 1) + q];
+            for (int q = 0; q < 5; q++) ydi[5 * ialku + q] = 0.;
+        }
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                long l = (long)(j - 1) * IS + G + i - 1;
+                double dw[5];
+                for (int q = 0; q < 5; q++) dw[q] = D[q][n];
+                face_contrib(dw, c[n], ro[n], a3x[n + IL], a3y[n + IL], a3z[n + IL],
+                             xi[9 * l + 6], xi[9 * l + 7], xi[9 * l + 8], &cd[5 * l]);
+            }
+    }
+    free(cd); free(xi); free(dt); free(scinv); free(scinp); free(ydi); free(bd);
+}
+
+/* IMPS imps3.f:6-572 with LUSGS = 1, ITURB <= 2, CFL >= .05, OMEGA = 0 */
+void ffo_implicit(const ff_params *p, ff_block *b, int level)
+{
+    dims_t d = mkdims(b);
+    double cfl = level >= 2 ? p->cfll : p->cfl;
+    time3(p, b, cfl);
+    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    const double *dtl = F(b, FF_DTL), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
+                 *ro = F(b, FF_RO), *e = F(b, FF_E), *P = F(b, FF_P), *c = F(b, FF_C),
+                 *drdp = F(b, FF_DRDP), *drdh = F(b, FF_DRDH), *cp = F(b, FF_CP);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                /* LOCDT imps3.f:792-798 */
+                double dt1 = dtl[n];
+                for (int q = 0; q < 5; q++) D[q][n] = dt1 * D[q][n];
+                /* COPR imps3.f:902-927 */
+                double vel2 = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]);
+                double beta = .5 * vel2;
+                double drodh = drdh[n], drodp = drdp[n];
+                double ypro = 1. / ro[n];
+                double dri = D[0][n], dmi = D[1][n], dni = D[2][n], dwi = D[3][n], dre = D[4][n];
+                double dkin = u[n] * dmi + v[n] * dni + w[n] * dwi;
+                D[1][n] = ypro * (-u[n] * dri + dmi);
+                D[2][n] = ypro * (-v[n] * dri + dni);
+                D[3][n] = ypro * (-w[n] * dri + dwi);
+                double hi = ypro * (e[n] + P[n]);
+                double dei = dre - (hi - 2 * beta) * dri - dkin;
+                double dpi = (c[n] * c[n]) * (dri - drodh * ypro * dei);
+                double dhi = (dri - drodp * dpi) / drodh;
+                D[0][n] = dhi;
+                D[4][n] = dpi;
+            }
+    predgs(p, b);
+    diaggs(p, b);
+    corrgs(p, b);
+    /* DIVV12(DRO,DRO,CP) imps3.f:483 */
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                D[0][n] = D[0][n] / cp[n];
+            }
+}
+
+/* Start-of-cycle copies read by FIN (TOLD = TEMP, ...) */
+void ffo_save_old(const ff_params *p, ff_block *b)
+{
+    (void)p;
+    dims_t d = mkdims(b);
+    size_t sz = sizeof(double) * (size_t)d.ntot;
+    memcpy(F(b, FF_TOLD), F(b, FF_TEMP), sz);
+    memcpy(F(b, FF_UOLD), F(b, FF_U), sz);
+    memcpy(F(b, FF_VOLD), F(b, FF_V), sz);
+    memcpy(F(b, FF_WOLD), F(b, FF_W), sz);
+    memcpy(F(b, FF_POLD), F(b, FF_PDIFF), sz);
+    if (F(b, FF_RKOLD) && F(b, FF_RK)) memcpy(F(b, FF_RKOLD), F(b, FF_RK), sz);
+    if (F(b, FF_EPSOLD) && F(b, FF_REPS)) memcpy(F(b, FF_EPSOLD), F(b, FF_REPS), sz);
+}
+
+/* FIN res3c.f:1122-1126 (algebraic) and the limiters of CYCLES ns3c.f:1553-1596 */
+void ffo_update(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    double *t = F(b, FF_TEMP), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
+           *pd = F(b, FF_PDIFF), *P = F(b, FF_P);
+    const double *to = F(b, FF_TOLD), *uo = F(b, FF_UOLD), *vo = F(b, FF_VOLD),
+                 *wo = F(b, FF_WOLD), *po = F(b, FF_POLD);
+    const double *dt = F(b, FF_DRO), *du = F(b, FF_DM), *dv = F(b, FF_DN), *dw = F(b, FF_DW),
+                 *dp = F(b, FF_DE);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                t[n] = to[n] + dt[n];
+                u[n] = uo[n] + du[n];
+                v[n] = vo[n] + dv[n];
+                w[n] = wo[n] + dw[n];
+                pd[n] = po[n] + dp[n];
+            }
+    /* UPLIM (res3c.f:3401: interior k-planes, interior j-rows, all i) and LOLIM (all) */
+    double *lim[4] = {t, u, v, w};
+    double up[4] = {p->tuplim, 5000., 5000., 5000.};
+    double lo[4] = {p->tlolim, -5000., -5000., -5000.};
+    for (int q = 0; q < 4; q++) {
+        double *x = lim[q];
+        for (int k = 1; k <= d.kmax; k++) {
+            long ia = (long)(k + 1) * d.il + (long)G * d.is;
+            for (long i = 0; i < (long)d.jmax * d.is; i++) {
+                long n = ia + i;
+                x[n] = x[n] < up[q] ? x[n] : up[q];        /* MIN(DRK,RLIM) */
+            }
+        }
+        for (long n = 0; n < d.ntot; n++) x[n] = x[n] > lo[q] ? x[n] : lo[q];   /* MAX */
+    }
+    /* PDTOP and LOLIMP res3c.f:3385-3389 over all cells */
+    for (long n = 0; n < d.ntot; n++) P[n] = pd[n] + p->frspre;
+    for (long n = 0; n < d.ntot; n++) {
+        double x = P[n] - 0.01 * p->frspre;
+        double prelim = 0. < x ? 0. : x;                   /* MIN(0., P-0.01*FRSPRE) */
+        P[n] = P[n] - prelim;
+        pd[n] = pd[n] - prelim;
+    }
+}

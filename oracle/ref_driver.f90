@@ -1,0 +1,269 @@
+! ref_driver.f90 — test-infrastructure driver that calls FINFLO's own, unmodified hot-path
+! routines (compiled from the reference's sources by oracle/Makefile into oracle/_ref/).
+!
+! TEST INFRASTRUCTURE ONLY: used to pin the C restatement in ff_oracle.c.  Nothing in the
+! product links it.  Each entry point follows the reference's call sequence for one stage:
+!   ref_state     STATIM ns3c.f:10318-10447: PDTOP, STATE->PERFGA (state.f:159), CONSVA
+!   ref_turb      TURBDI ns3c.f:2987-2991 (ITURB=0): EPS2=1, VIST=0
+!   ref_residual  RESIDU res3c.f:194-559: ZEROZZ, then per direction k,i,j the FLUCYC
+!                 sequence res3c.f:7705-7892 (ZEROZZ of F1*, FLUXDO, FLUXKE) and SWEEP;
+!                 RESL2 res3c.f:8920 on level 1
+!   ref_implicit  IMPS imps3.f:111-483 with LUSGS=1: TIME3, LOCDT, COPR, PREDGS, DIAGGS,
+!                 CORRGS, DIVV12
+!   ref_update    CYCLES ns3c.f:1517-1596: FIN, UPLIM/LOLIM, PDTOP, LOLIMP
+!
+! All fields of a block are packed as F(NTOT, NF) in the order of enum ff_field
+! (include/ffhip.h); the parameter vector PRM and integer vector IPR are filled by the
+! Python test harness (tests/refbridge.py).
+module ffref_ids
+  implicit none
+  integer, parameter :: IPDIFF=1, ITEMP=2, IU=3, IV=4, IW=5, IRK=6, IREPS=7
+  integer, parameter :: IP=8, IRO=9, IE=10, IC=11, IVIS=12, ICH=13, ICP=14, IDRDP=15, IDRDH=16
+  integer, parameter :: IRM=17, IRN=18, IRW=19, IEPS2=20, IVIST=21
+  integer, parameter :: IDRO=22, IDM=23, IDN=24, IDW=25, IDE=26, IDRK=27, IDEPS=28, IDTL=29
+  integer, parameter :: ITOLD=30, IUOLD=31, IVOLD=32, IWOLD=33, IPOLD=34, IRKOLD=35, IEPSOLD=36
+  integer, parameter :: IVOL=37, IA1=38, IA1X=39, IA1Y=40, IA1Z=41, IA2=42, IA2X=43, IA2Y=44
+  integer, parameter :: IA2Z=45, IA3=46, IA3X=47, IA3Y=48, IA3Z=49, ID1=50, ID2=51, ID3=52
+  integer, parameter :: IXC=53, IYC=54, IZC=55, IDISTW=56
+  integer, parameter :: ISRK=57, ISEPS=58, IPTUR=59, IFUN1=60, IOHMI=61
+  ! parameter vector PRM (same order as the doubles of ff_params)
+  integer, parameter :: PGAMMA=1, PRGAS=2, PPR=3, PPRT=4, PVISU0=5, PEXPSU=6, PTSU0=7
+  integer, parameter :: PFRSPRE=8, PFRSDEN=9, PFRSTEM=10, PFRSVEL=11, PFRSVIS=12, PCFL=13
+  integer, parameter :: PCFLL=14, PTHETA=15, PCDIFF=16, PTUPLIM=17, PTLOLIM=18, PRKLIM=19
+  integer, parameter :: PEPSLIM=20, PTURLIM=21, PFRSMUT=22, PRMULT=23
+  ! integer vector IPR
+  integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
+  integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10
+end module ffref_ids
+
+subroutine ref_state(f, ntot, prm, ipr) bind(C, name="ref_state")
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  use NS3CO, only : GX, GY, GZ, GROUND, ALTITUDE
+  implicit none
+  integer(c_int), value :: ntot
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  real(c_double), allocatable :: trgs(:), fi(:), rnut(:)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  integer :: ercode, iturb
+  character(len=10) :: mulphc
+  ALTITUDE = 0.; GROUND = 0.; GX = 0.; GY = 0.; GZ = 0.
+  allocate(trgs(ntot), fi(ntot), rnut(ntot))
+  fi = 0.; rnut = 0.
+  iturb = ipr(QITURB)
+  ercode = 0
+  mulphc = 'FLUID'
+  call PDTOP(f(1,IP), f(1,IPDIFF), f(1,IXC), f(1,IYC), f(1,IZC), prm(PFRSDEN), prm(PFRSPRE), ntot)
+  call PERFGA(f(1,IVIS), f(1,IP), f(1,IRO), f(1,IE), f(1,ITEMP), f(1,ICP), f(1,ICH), &
+       f(1,IDRDP), f(1,IDRDH), prm(PRGAS), prm(PGAMMA), prm(PPR), prm(PVISU0), prm(PEXPSU), &
+       prm(PTSU0), ntot, ercode, 0._c_double, 0._c_double, prm(PRMULT), trgs, prm(PFRSPRE), prm(PFRSTEM), 0)
+  call CONSVA(f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IE), f(1,IC), f(1,IU), f(1,IV), &
+       f(1,IW), f(1,IRK), f(1,IP), f(1,IDRDP), f(1,IDRDH), f(1,IXC), f(1,IYC), f(1,IZC), &
+       pro, var, 0._c_double, prm(PFRSVEL), ntot, iturb, 0, fi, ntot, 0, 0, 0._c_double, mulphc, 1, 1, &
+       rnut, f(1,IREPS))
+  deallocate(trgs, fi, rnut)
+end subroutine ref_state
+
+subroutine ref_turb(f, ntot, prm, ipr) bind(C, name="ref_turb")
+  use iso_c_binding
+  use ffref_ids
+  implicit none
+  integer(c_int), value :: ntot
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  f(1:ntot, IEPS2) = 1.
+  f(1:ntot, IVIST) = 0.
+end subroutine ref_turb
+
+! One direction of RESIDU: the FLUCYC sequence for a block with no boundary patches
+! (NPATCH = 0), FULLNS = .FALSE., IFLUX = 1, followed by SWEEP.
+subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat, zero, trm)
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  implicit none
+  integer :: ntot, idir, maw, maxw
+  real(c_double) :: f(ntot, *), prm(*), fl(ntot, 7), zzz(maxw), rksi(ntot), hat(ntot, 4)
+  real(c_double) :: zero(ntot)
+  integer(c_int) :: ipr(*)
+  type(INTERMITTENCY) :: trm(1)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  real(c_double) :: f1h(1), wave(1)
+  integer :: imax, jmax, kmax, is, js, kstr, kmaxp, ia, iax, iay, iaz, inter, ndi, istr, ido
+  logical :: xxtral, inchiml, entropy_fix, transl, mulphl, fresul
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  is = imax + 4; js = jmax + 4
+  xxtral = .false.; inchiml = .false.; entropy_fix = .false.; transl = .false.
+  mulphl = .false.; fresul = .false.
+  select case (idir)
+  case (1)
+     kstr = 1; kmaxp = kmax; ia = IA1; inter = ipr(QINTI); ndi = imax + 1
+     istr = 1
+  case (2)
+     kstr = is; kmaxp = kmax; ia = IA2; inter = ipr(QINTJ); ndi = jmax + 1
+     istr = is
+  case default
+     kstr = is*js; kmaxp = kmax + 1; ia = IA3; inter = ipr(QINTK); ndi = kmax + 1
+     istr = is*js
+  end select
+  iax = ia + 1; iay = ia + 2; iaz = ia + 3
+  fl = 0.
+  ! FLUXDO (res3c.f:7856): thin-layer viscous flux, arithmetic average
+  call FLUXDO(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IVOL), f(1,ia), f(1,IRO), &
+       f(1,IRM), f(1,IRN), f(1,IRW), f(1,IP), f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), &
+       imax, jmax, kmax, 1, inter, ndi, f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,iax), &
+       f(1,iay), f(1,iaz), f(1,ICP), prm(PPR), prm(PPRT), zero, pro, var, f(1,ITEMP), &
+       f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), prm(PFRSPRE), &
+       hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, mulphl, 1, fresul, 0._c_double, f1h, &
+       wave, ipr(QLEVEL), ipr(QIDIFF))
+  ! FLUXKE (res3c.f:7892): Roe flux; FLUCYC passes RKLIM = EPSLIM = 0 (res3c.f:7680-7681)
+  call FLUXKE(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), &
+       f(1,ia), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), f(1,IP), &
+       f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), imax, jmax, kmax, 1, inter, 9, ndi, &
+       f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,IPDIFF), f(1,iax), f(1,iay), f(1,iaz), &
+       f(1,ID1), prm(PPR), prm(PPRT), zero, f(1,ITEMP), f(1,ICH), f(1,IDRDH), f(1,IDRDP), &
+       1, prm(PRGAS), prm(PGAMMA), 0._c_double, 0._c_double, prm(PFRSDEN), prm(PFRSPRE), 0._c_double, 0._c_double, &
+       hat(1,1), hat(1,2), hat(1,3), hat(1,4), kmaxp, kstr, idir, ipr(QITURB), zzz, maw, &
+       maxw, xxtral, rksi, inchiml, entropy_fix, transl, trm)
+  ! SWEEP (res3c.f:330, 448, 559); the j-direction is last and divides by VOL
+  ido = 0
+  if (idir == 2) ido = 1
+  call SWEEP(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), fl(1,1), fl(1,2), fl(1,3), &
+       fl(1,4), fl(1,5), f(1,IVOL), imax, jmax, kmax, istr, 2, 2, 2, 0._c_double, ido)
+end subroutine ref_direction
+
+subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  use NS3CO, only : TWO_FLUIDL
+  implicit none
+  integer(c_int), value :: ntot
+  real(c_double) :: f(ntot, *), prm(*), resi(*)
+  integer(c_int) :: ipr(*)
+  real(c_double), allocatable :: fl(:,:), zzz(:), rksi(:), hat(:,:), zero(:), dfi(:)
+  type(INTERMITTENCY) :: trm(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(PRE_COR) :: prc(1)
+  real(c_double) :: r(67)
+  integer :: maw, maxw, imax, jmax, kmax
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  maw = ntot
+  maxw = 27*maw
+  allocate(fl(ntot,7), zzz(maxw), rksi(ntot), hat(ntot,4), zero(ntot), dfi(ntot))
+  rksi = 0.; zero = 0.; zzz = 0.; dfi = 0.
+  ! ZEROZZ of the residuals (res3c.f:194-199)
+  call ZEROZZ(f(1,IDRO), ntot); call ZEROZZ(f(1,IDM), ntot); call ZEROZZ(f(1,IDN), ntot)
+  call ZEROZZ(f(1,IDW), ntot); call ZEROZZ(f(1,IDE), ntot); call ZEROZZ(f(1,IDEPS), ntot)
+  call ref_direction(f, ntot, prm, ipr, 3, fl, zzz, maw, maxw, rksi, hat, zero, trm)
+  call ref_direction(f, ntot, prm, ipr, 1, fl, zzz, maw, maxw, rksi, hat, zero, trm)
+  call ref_direction(f, ntot, prm, ipr, 2, fl, zzz, maw, maxw, rksi, hat, zero, trm)
+  if (ipr(QLEVEL) == 1) then
+     TWO_FLUIDL = .false.
+     r = 0.
+     call RESL2(f(1,IVOL), f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), dfi, &
+          f(1,IDRK), f(1,IDEPS), var, r, rksi, imax, jmax, kmax, ipr(QITURB), 0, ntot, 67, &
+          prm(PRKLIM), prm(PEPSLIM), .false., 1, 1, 1, prc)
+     resi(1:7) = r(31:37)
+  end if
+  deallocate(fl, zzz, rksi, hat, zero, dfi)
+end subroutine ref_residual
+
+subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
+  use iso_c_binding
+  use ffref_ids
+  implicit none
+  integer(c_int), value :: ntot
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  real(c_double), allocatable :: zzz(:), rksi(:), zero(:), abv(:)
+  integer :: maw, maxw, imax, jmax, kmax, idm1, idm2, idm3
+  real(c_double) :: cfm, theta
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  maw = ntot
+  maxw = 18*maw
+  allocate(zzz(maxw), rksi(ntot), zero(ntot), abv(ntot))
+  zzz = 0.; rksi = 0.; zero = 0.; abv = 0.
+  cfm = prm(PCFL)
+  if (ipr(QLEVEL) >= 2) cfm = prm(PCFLL)
+  theta = prm(PTHETA)
+  idm1 = imax + 1; idm2 = jmax + 1; idm3 = kmax + 1
+  ! TIME3 (imps3.f:111) with PVISC = 1.8, IVISC = 1
+  call TIME3(f(1,IA1), f(1,IA2), f(1,IA3), f(1,ID1), f(1,ID2), f(1,ID3), f(1,IVOL), &
+       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3X), &
+       f(1,IA3Y), f(1,IA3Z), f(1,IDTL), 0._c_double, f(1,IRO), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+       f(1,IP), imax, jmax, kmax, cfm, f(1,IVIS), f(1,IVIST), zero, zero, zero, &
+       prm(PFRSDEN), prm(PFRSVEL), prm(PPR), prm(PPRT), idm1, idm2, idm3, ipr(QLEVEL), &
+       .false., 1.8_c_double, abv, 1, prm(PFRSPRE), prm(PCDIFF))
+  ! LOCDT (imps3.f:120) and COPR (imps3.f:272)
+  call LOCDT(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDTL), ntot, 0._c_double)
+  call COPR(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IRO), f(1,IE), f(1,IP), &
+       f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), f(1,IDRDP), f(1,IDRDH), &
+       prm(PFRSVEL), ntot)
+  ! LU-SGS (imps3.f:383-391)
+  call PREDGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+       f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+       f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
+       f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+  call DIAGGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+       f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+       f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
+       f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+  call CORRGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+       f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+       f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
+       f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+  ! dh -> dT (imps3.f:483)
+  call DIVV12(f(1,IDRO), f(1,IDRO), f(1,ICP), ntot)
+  deallocate(zzz, rksi, zero, abv)
+end subroutine ref_implicit
+
+subroutine ref_update(f, ntot, prm, ipr) bind(C, name="ref_update")
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  use NS3CO, only : GX, GY, GZ, GROUND, ALTITUDE
+  implicit none
+  integer(c_int), value :: ntot
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(INTERMITTENCY) :: trm(1)
+  type(BLOCKS) :: blks(1)
+  real(c_double), allocatable :: fi(:), dfi(:), fiold(:)
+  real(c_double) :: rlolim(1), upplim(1)
+  integer :: imax, jmax, kmax
+  ALTITUDE = 0.; GROUND = 0.; GX = 0.; GY = 0.; GZ = 0.
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  allocate(fi(ntot), dfi(ntot), fiold(ntot))
+  fi = 0.; dfi = 0.; fiold = 0.
+  blks(1)%SOLUTION_TYPE = 'FLUID'
+  blks(1)%ISTATE = 1
+  call FIN(f(1,ITEMP), f(1,IU), f(1,IV), f(1,IW), f(1,IPDIFF), f(1,IDRO), f(1,IDM), &
+       f(1,IDN), f(1,IDW), f(1,IDE), f(1,ITOLD), f(1,IUOLD), f(1,IVOLD), f(1,IWOLD), &
+       f(1,IPOLD), f(1,IRK), f(1,IREPS), f(1,IDRK), f(1,IDEPS), f(1,IRKOLD), f(1,IEPSOLD), &
+       prm(PRKLIM), prm(PEPSLIM), pro, var, fi, dfi, fiold, rlolim, upplim, imax, jmax, &
+       kmax, ntot, 0, 0, ipr(QITURB), 0, prm(PTURLIM), f(1,IVIS), .false., trm, blks, 1, 1)
+  call UPLIM(f(1,ITEMP), imax, jmax, kmax, 2, 2, 2, prm(PTUPLIM))
+  call LOLIM(f(1,ITEMP), imax, jmax, kmax, 2, 2, 2, prm(PTLOLIM))
+  call UPLIM(f(1,IU), imax, jmax, kmax, 2, 2, 2, 5000._c_double)
+  call LOLIM(f(1,IU), imax, jmax, kmax, 2, 2, 2, -5000._c_double)
+  call UPLIM(f(1,IV), imax, jmax, kmax, 2, 2, 2, 5000._c_double)
+  call LOLIM(f(1,IV), imax, jmax, kmax, 2, 2, 2, -5000._c_double)
+  call UPLIM(f(1,IW), imax, jmax, kmax, 2, 2, 2, 5000._c_double)
+  call LOLIM(f(1,IW), imax, jmax, kmax, 2, 2, 2, -5000._c_double)
+  call PDTOP(f(1,IP), f(1,IPDIFF), f(1,IXC), f(1,IYC), f(1,IZC), prm(PFRSDEN), prm(PFRSPRE), &
+       ntot)
+  if (ipr(QITURB) >= 3) call LOLIM(f(1,IREPS), imax, jmax, kmax, 2, 2, 2, prm(PEPSLIM))
+  call LOLIMP(f(1,IP), f(1,IPDIFF), prm(PFRSDEN), prm(PFRSPRE), 0._c_double, 0._c_double, f(1,IXC), &
+       f(1,IYC), f(1,IZC), imax, jmax, kmax, 2, 2, 2)
+  deallocate(fi, dfi, fiold)
+end subroutine ref_update
