@@ -1,0 +1,912 @@
+// ffhip.hip — MI355X (gfx950) kernels and C-ABI of FINFLO's per-iteration hot path.
+//
+// Entry points are declared in include/ffhip.h.  All arithmetic is FP64 and follows the
+// reference's operation order (see ffhip_device.h); built with -ffp-contract=off.
+//
+// Kernels (one block slab per launch, i-fastest SoA, two ghost layers):
+//   k_state        STATIM: PDTOP + PERFGA + CONSVA on every cell (ns3c.f:10380-10447)
+//   k_turb_lam     TURBDI ITURB=0 (ns3c.f:2987-2991)
+//   k_residual     RESIDU: the k, i, j face fluxes (FLUXDO + FLUXKE) of a cell and their
+//                  SWEEP differences fused, thread per interior cell (res3c.f:194-559)
+//   k_resl2_*      RESL2 sums (res3c.f:8979-9006), deterministic two-stage reduction
+//   k_time3_copr   TIME3 + LOCDT + COPR (base3c.f:79, imps3.f:784, imps3.f:891)
+//   k_pred_plane   PREDGS on one hyperplane I+J+K = m (imps4.f:345-598)
+//   k_corr_plane   DIAGGS + CORRGS on one hyperplane (imps4.f:604-664, 76-339)
+//   k_divcp        DRO/CP (imps3.f:483)
+//   k_fin          FIN (res3c.f:1122-1126)
+//   k_limits       UPLIM/LOLIM of T, U, V, W, PDTOP, LOLIMP (ns3c.f:1553-1596)
+//
+// LU-SGS: the reference sweeps K-planes in order and, inside a plane, the diagonals
+// M = I+J (imps4.f:488-570, 221-308).  Every cell's update reads only the final increments
+// of its three upstream neighbours and its own data, so processing the hyperplanes
+// I+J+K = m in order gives the same results bit for bit.  One quirk of PREDGS is kept:
+// its line-buffer shift (imps4.f:556-562) starts at IALKU, so a cell in the row J = JMAX
+// never receives the contribution of its I+1 neighbour.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ffhip_device.h"
+
+using namespace ff;
+
+// ------------------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_state(Prm p, Blk b)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    const double gm1 = p.gamma - 1.;
+    const double cp2 = p.gamma / gm1 * p.rgas;
+    const double cmin = 1.;
+    double pd = FLD(b, FF_PDIFF)[n], t = FLD(b, FF_TEMP)[n];
+    double u = FLD(b, FF_U)[n], v = FLD(b, FF_V)[n], w = FLD(b, FF_W)[n];
+    double P = pd + p.frspre;                                   // PDTOP state.f:1707
+    double trgs = t * p.rgas;                                   // PERFGA state.f:212-229
+    double drdp = 1. / trgs;
+    double ro = P * drdp;
+    double vis = p.visu0 * t * p.rmult * sqrt(t) / (t + p.tsu0);
+    double e = trgs / gm1;
+    double drdh = -ro / e / p.gamma;
+    double ch = vis * cp2 / p.pr;
+    double rkin = (u * u + v * v + w * w) * .5;                 // CONSVA base3c.f:519-528
+    double x = 1. / (drdh / ro + drdp);
+    double c = sqrt(x > cmin ? x : cmin);
+    e = (e + rkin + 0.) * ro;
+    FLD(b, FF_P)[n] = P;
+    FLD(b, FF_RO)[n] = ro;
+    FLD(b, FF_E)[n] = e;
+    FLD(b, FF_C)[n] = c;
+    FLD(b, FF_VIS)[n] = vis;
+    FLD(b, FF_CH)[n] = ch;
+    FLD(b, FF_CP)[n] = cp2;
+    FLD(b, FF_DRDP)[n] = drdp;
+    FLD(b, FF_DRDH)[n] = drdh;
+    FLD(b, FF_RM)[n] = ro * u;
+    FLD(b, FF_RN)[n] = ro * v;
+    FLD(b, FF_RW)[n] = ro * w;
+}
+
+__global__ void __launch_bounds__(256) k_turb_lam(Blk b)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    FLD(b, FF_EPS2)[n] = 1.;
+    FLD(b, FF_VIST)[n] = 0.;
+}
+
+// Thread per interior cell.  The residual of a cell is
+//   D = ((0 - (Fk(k+1) - Fk(k))) - (Fi(i+1) - Fi(i)) - (Fj(j+1) - Fj(j))) * (1/VOL)
+// in the reference's order (SWEEP after the k, i and j directions).  Both faces of the
+// cell are evaluated by the same code as the neighbour evaluates them, so the shared face
+// values agree bit for bit.
+__global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    int n = cid(b, i, j, k);
+    double d[5] = {0., 0., 0., 0., 0.};
+    {   // k-direction (res3c.f:270, SWEEP res3c.f:330)
+        const int S = b.il;
+        Flux5 lo = face_flux<true>(p, b, n, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
+                                   FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+        Flux5 hi = face_flux<true>(p, b, n + S, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
+                                   FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+#pragma unroll
+        for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
+    }
+    {   // i-direction (res3c.f:402, SWEEP res3c.f:448)
+        const int S = 1;
+        Flux5 lo = face_flux<false>(p, b, n, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
+                                    FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+        Flux5 hi = face_flux<false>(p, b, n + S, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
+                                    FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+#pragma unroll
+        for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
+    }
+    {   // j-direction, last: divide by VOL (res3c.f:515, SWEEP res3c.f:559)
+        const int S = b.is;
+        Flux5 lo = face_flux<false>(p, b, n, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
+                                    FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+        Flux5 hi = face_flux<false>(p, b, n + S, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
+                                    FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+        double dt1 = 1. / FLD(b, FF_VOL)[n];
+#pragma unroll
+        for (int q = 0; q < 5; q++) d[q] = (d[q] - (hi.f[q] - lo.f[q])) * dt1;
+    }
+    FLD(b, FF_DRO)[n] = d[0];
+    FLD(b, FF_DM)[n] = d[1];
+    FLD(b, FF_DN)[n] = d[2];
+    FLD(b, FF_DW)[n] = d[3];
+    FLD(b, FF_DE)[n] = d[4];
+}
+
+// Zero a field on every cell (ZEROZZ res3c.f:9330)
+__global__ void __launch_bounds__(256) k_zero(double *x, int n)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = 0.;
+}
+
+// RESL2 stage 1: per-workgroup partial sums of D^2 over interior cells
+constexpr int RED_THREADS = 256;
+__global__ void __launch_bounds__(RED_THREADS) k_resl2_part(Blk b, double *part)
+{
+    __shared__ double sh[5][RED_THREADS];
+    int ncell = b.imax * b.jmax * b.kmax;
+    double s[5] = {0., 0., 0., 0., 0.};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += gridDim.x * blockDim.x) {
+        int i = t % b.imax + 1;
+        int r = t / b.imax;
+        int j = r % b.jmax + 1;
+        int k = r / b.jmax + 1;
+        int n = cid(b, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double x = FLD(b, FF_DRO + q)[n];
+            s[q] += x * x;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int h = RED_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) part[5 * blockIdx.x + q] = sh[q][0];
+    }
+}
+
+// RESL2 stage 2: fixed-order sum of the partials (run-to-run deterministic)
+__global__ void __launch_bounds__(64) k_resl2_final(const double *part, int nparts, double *out)
+{
+    int q = threadIdx.x;
+    if (q >= 5) return;
+    double s = 0.;
+    for (int i = 0; i < nparts; i++) s += part[5 * i + q];
+    out[q] = s;
+}
+
+// TIME3 (base3c.f:79-220) + LOCDT (imps3.f:792-798) + COPR (imps3.f:902-927), thread per
+// interior cell
+__global__ void __launch_bounds__(256) k_time3_copr(Prm p, Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    const int IS = b.is, IL = b.il;
+    int kk = cid(b, i, j, k);
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3),
+                 *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z),
+                 *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z),
+                 *a3x = FLD(b, FF_A3X), *a3y = FLD(b, FF_A3Y), *a3z = FLD(b, FF_A3Z);
+    const double eps = 1.E-10, pvisc = 1.8;
+    const double pcfl = 1. / p.cfl;
+    const double prs = p.pr / p.prt;
+    double z = .5 * (a1[kk] + a1[kk + 1]);
+    double r2 = .5 * (a2[kk] + a2[kk + IS]);
+    z = r2 > z ? r2 : z;
+    double r3 = .5 * (a3[kk] + a3[kk + IL]);
+    z = r3 > z ? r3 : z;
+    double vol = FLD(b, FF_VOL)[kk], ro = FLD(b, FF_RO)[kk];
+    double u = FLD(b, FF_U)[kk], v = FLD(b, FF_V)[kk], w = FLD(b, FF_W)[kk];
+    double c = FLD(b, FF_C)[kk], P = FLD(b, FF_P)[kk];
+    double pvol = 1. / vol;
+    double ypro = 1. / (ro + eps);
+    double a1k = .5 * (a1[kk] + a1[kk + 1]);
+    double a2k = .5 * (a2[kk] + a2[kk + IS]);
+    double a3k = .5 * (a3[kk] + a3[kk + IL]);
+    double a1xk = .5 * (a1x[kk] * a1[kk] + a1x[kk + 1] * a1[kk + 1]);
+    double a1yk = .5 * (a1y[kk] * a1[kk] + a1y[kk + 1] * a1[kk + 1]);
+    double a1zk = .5 * (a1z[kk] * a1[kk] + a1z[kk + 1] * a1[kk + 1]);
+    double a2xk = .5 * (a2x[kk] * a2[kk] + a2x[kk + IS] * a2[kk + IS]);
+    double a2yk = .5 * (a2y[kk] * a2[kk] + a2y[kk + IS] * a2[kk + IS]);
+    double a2zk = .5 * (a2z[kk] * a2[kk] + a2z[kk + IS] * a2[kk + IS]);
+    double a3xk = .5 * (a3x[kk] * a3[kk] + a3x[kk + IL] * a3[kk + IL]);
+    double a3yk = .5 * (a3y[kk] * a3[kk] + a3y[kk + IL] * a3[kk + IL]);
+    double a3zk = .5 * (a3z[kk] * a3[kk] + a3z[kk + IL] * a3[kk + IL]);
+    double ucii = a1xk * u + a1yk * v + a1zk * w - 0.;
+    double vcii = a2xk * u + a2yk * v + a2zk * w - 0.;
+    double wcii = a3xk * u + a3yk * v + a3zk * w - 0.;
+    double b1 = pvol * (fabs(ucii) + fabs(vcii) + fabs(wcii)
+                        + c * sqrt(a1k * a1k + a2k * a2k + a3k * a3k)) * pcfl;
+    double rnum = (z * z) * (pvol * pvol) * ypro;
+    double t = rnum * (FLD(b, FF_VIS)[kk] + prs * FLD(b, FF_VIST)[kk]) / p.cdiff;
+    double b2 = 2. * t * pvisc;
+    double b11 = b1 > b2 ? b1 : b2;
+    double x = (p.frspre - 2. * P) / P;
+    double denlow = 1. + (0. > x ? 0. : x);
+    double y = (ro - 2. * p.frsden) * ypro;
+    double denmax = 1. + (0. > y ? 0. : y) * 5.;
+    denlow = denlow > denmax ? denlow : denmax;
+    double dtl = 1. / (b11 * denlow);
+    FLD(b, FF_DTL)[kk] = dtl;
+    // LOCDT
+    double dri = dtl * FLD(b, FF_DRO)[kk];
+    double dmi = dtl * FLD(b, FF_DM)[kk];
+    double dni = dtl * FLD(b, FF_DN)[kk];
+    double dwi = dtl * FLD(b, FF_DW)[kk];
+    double dre = dtl * FLD(b, FF_DE)[kk];
+    // COPR
+    double vel2 = (u * u + v * v + w * w);
+    double beta = .5 * vel2;
+    double drodh = FLD(b, FF_DRDH)[kk], drodp = FLD(b, FF_DRDP)[kk];
+    double yp = 1. / ro;
+    double dkin = u * dmi + v * dni + w * dwi;
+    FLD(b, FF_DM)[kk] = yp * (-u * dri + dmi);
+    FLD(b, FF_DN)[kk] = yp * (-v * dri + dni);
+    FLD(b, FF_DW)[kk] = yp * (-w * dri + dwi);
+    double hi = yp * (FLD(b, FF_E)[kk] + P);
+    double dei = dre - (hi - 2 * beta) * dri - dkin;
+    double dpi = (c * c) * (dri - drodh * yp * dei);
+    double dhi = (dri - drodp * dpi) / drodh;
+    FLD(b, FF_DRO)[kk] = dhi;
+    FLD(b, FF_DE)[kk] = dpi;
+}
+
+__device__ __forceinline__ void load_dw(const Blk &b, int n, double dw[5])
+{
+#pragma unroll
+    for (int q = 0; q < 5; q++) dw[q] = FLD(b, FF_DRO + q)[n];
+}
+
+// Hyperplane cell enumeration: thread t -> (j, k) with i = m - j - k inside the block.
+__device__ __forceinline__ bool plane_cell(const Blk &b, int m, int jlo, int klo, int jn,
+                                           int &i, int &j, int &k)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    j = jlo + t % jn;
+    k = klo + t / jn;
+    i = m - j - k;
+    return (k <= b.kmax && j <= b.jmax && i >= 1 && i <= b.imax);
+}
+
+// PREDGS (imps4.f:345-598) for the cells of hyperplane I+J+K = m
+__global__ void __launch_bounds__(128) k_pred_plane(Prm p, Blk b, int m, int jlo, int klo, int jn)
+{
+    int i, j, k;
+    if (!plane_cell(b, m, jlo, klo, jn, i, j, k)) return;
+    const int IS = b.is, IL = b.il;
+    const double cdiff = 2.;
+    int n = cid(b, i, j, k);
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3);
+    const double *c = FLD(b, FF_C), *ro = FLD(b, FF_RO), *vol = FLD(b, FF_VOL);
+    // own coefficients on the lower faces (loop 1100)
+    double ffi = i == 1 ? 4. : 2., ffj = j == 1 ? 4. : 2., ffk = k == 1 ? 4. : 2.;
+    Coef ci = lusgs_coef<true>(b, n, 1, ffi * cdiff, a1, FLD(b, FF_A1X), FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+    Coef cj = lusgs_coef<true>(b, n, IS, cdiff * ffj, a2, FLD(b, FF_A2X), FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+    Coef ck = lusgs_coef<true>(b, n, IL, ffk * cdiff, a3, FLD(b, FF_A3X), FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+    double dt1 = p.theta * FLD(b, FF_DTL)[n];
+    double scinv = 1. / (vol[n] + dt1 * (ci.xa * a1[n] + cj.xa * a2[n] + ck.xa * a3[n]) + vol[n] * 0.);
+    double scinp = 1. / (vol[n] + dt1 * (ci.rmax * a1[n] + cj.rmax * a2[n] + ck.rmax * a3[n]) + vol[n] * 0.);
+    // upstream contributions: k+1 (loop 1600 of the previous slab), i+1 and j+1 (line buffers)
+    double ckc[5] = {0., 0., 0., 0., 0.}, yi[5] = {0., 0., 0., 0., 0.}, bj[5] = {0., 0., 0., 0., 0.};
+    if (k < b.kmax) {
+        int nb = n + IL;
+        Coef kb = lusgs_coef<true>(b, nb, IL, 2. * cdiff, a3, FLD(b, FF_A3X), FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+        double dw[5];
+        load_dw(b, nb, dw);
+        face_contrib(dw, c[nb], ro[nb], FLD(b, FF_A3X)[nb], FLD(b, FF_A3Y)[nb], FLD(b, FF_A3Z)[nb],
+                     kb.x1, kb.x2, kb.x5, ckc);
+    }
+    if (i < b.imax && j < b.jmax) {   // J = JMAX: the shift of imps4.f:1460 leaves 0
+        int nb = n + 1;
+        Coef ib = lusgs_coef<true>(b, nb, 1, 2. * cdiff, a1, FLD(b, FF_A1X), FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+        double dw[5];
+        load_dw(b, nb, dw);
+        face_contrib(dw, c[nb], ro[nb], FLD(b, FF_A1X)[nb], FLD(b, FF_A1Y)[nb], FLD(b, FF_A1Z)[nb],
+                     ib.x1, ib.x2, ib.x5, yi);
+    }
+    if (j < b.jmax) {
+        int nb = n + IS;
+        Coef jb = lusgs_coef<true>(b, nb, IS, cdiff * 2., a2, FLD(b, FF_A2X), FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+        double dw[5];
+        load_dw(b, nb, dw);
+        face_contrib(dw, c[nb], ro[nb], FLD(b, FF_A2X)[nb], FLD(b, FF_A2Y)[nb], FLD(b, FF_A2Z)[nb],
+                     jb.x1, jb.x2, jb.x5, bj);
+    }
+    double a3u = a3[n + IL], a1u = a1[n + 1], a2u = a2[n + IS];
+    double cd[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        cd[q] = -dt1 * a3u * ckc[q] + vol[n] * FLD(b, FF_DRO + q)[n];
+        cd[q] = cd[q] - dt1 * (a1u * yi[q] + a2u * bj[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) FLD(b, FF_DRO + q)[n] = cd[q] * scinv;
+    FLD(b, FF_DE)[n] = cd[4] * scinp;
+}
+
+// DIAGGS factor (imps4.f:621-659) of an interior cell
+__device__ __forceinline__ double diag_factor(const Prm &p, const Blk &b, int n)
+{
+    const int IS = b.is, IL = b.il;
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3),
+                 *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z),
+                 *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z),
+                 *a3x = FLD(b, FF_A3X), *a3y = FLD(b, FF_A3Y), *a3z = FLD(b, FF_A3Z);
+    double u = FLD(b, FF_U)[n], v = FLD(b, FF_V)[n], w = FLD(b, FF_W)[n], c = FLD(b, FF_C)[n];
+    double ucon;
+    ucon = a1x[n + 1] * u + a1y[n + 1] * v + a1z[n + 1] * w - 0.;
+    double r1n = 1.00 * (fabs(ucon) + c);
+    ucon = a1x[n] * u + a1y[n] * v + a1z[n] * w - 0.;
+    double r1p = 1.00 * (fabs(ucon) + c);
+    ucon = a2x[n + IS] * u + a2y[n + IS] * v + a2z[n + IS] * w - 0.;
+    double r2n = 1.00 * (fabs(ucon) + c);
+    ucon = a2x[n] * u + a2y[n] * v + a2z[n] * w - 0.;
+    double r2p = 1.00 * (fabs(ucon) + c);
+    ucon = a3x[n + IL] * u + a3y[n + IL] * v + a3z[n + IL] * w - 0.;
+    double r3n = 1.00 * (fabs(ucon) + c);
+    ucon = a3x[n] * u + a3y[n] * v + a3z[n] * w - 0.;
+    double r3p = 1.00 * (fabs(ucon) + c);
+    double dt1 = p.theta * FLD(b, FF_DTL)[n] / FLD(b, FF_VOL)[n];
+    double sc = .5 * (a1[n] * r1p + a1[n + 1] * r1n + a2[n] * r2p + a2[n + IS] * r2n +
+                      a3[n] * r3p + a3[n + IL] * r3n);
+    return 1. + dt1 * sc + 0.;
+}
+
+// DIAGGS + CORRGS (imps4.f:76-339) for the cells of hyperplane I+J+K = m
+__global__ void __launch_bounds__(128) k_corr_plane(Prm p, Blk b, int m, int jlo, int klo, int jn)
+{
+    int i, j, k;
+    if (!plane_cell(b, m, jlo, klo, jn, i, j, k)) return;
+    const int IS = b.is, IL = b.il;
+    const double cdiff = 2.;
+    int n = cid(b, i, j, k);
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3);
+    const double *c = FLD(b, FF_C), *ro = FLD(b, FF_RO), *vol = FLD(b, FF_VOL);
+    double fac = diag_factor(p, b, n);
+    Coef ci = lusgs_coef<false>(b, n, 1, 2. * cdiff, a1, FLD(b, FF_A1X), FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+    Coef cj = lusgs_coef<false>(b, n, IS, cdiff * 2., a2, FLD(b, FF_A2X), FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+    Coef ck = lusgs_coef<false>(b, n, IL, 2. * cdiff, a3, FLD(b, FF_A3X), FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+    double dt1 = p.theta * FLD(b, FF_DTL)[n];
+    double scinv = 1. / (vol[n] + dt1 * (ci.xa * a1[n + 1] + cj.xa * a2[n + IS] + ck.xa * a3[n + IL]) + vol[n] * 0.);
+    double scinp = 1. / (vol[n] + dt1 * (ci.rmax * a1[n] + cj.rmax * a2[n] + ck.rmax * a3[n]) + vol[n] * 0.);
+    double ckc[5] = {0., 0., 0., 0., 0.}, yi[5] = {0., 0., 0., 0., 0.}, bj[5] = {0., 0., 0., 0., 0.};
+    if (k > 1) {
+        int nb = n - IL;
+        Coef kb = lusgs_coef<false>(b, nb, IL, 2. * cdiff, a3, FLD(b, FF_A3X), FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+        double dw[5];
+        load_dw(b, nb, dw);
+        face_contrib(dw, c[nb], ro[nb], FLD(b, FF_A3X)[n], FLD(b, FF_A3Y)[n], FLD(b, FF_A3Z)[n],
+                     kb.x1, kb.x2, kb.x5, ckc);
+    }
+    if (i > 1) {
+        int nb = n - 1;
+        Coef ib = lusgs_coef<false>(b, nb, 1, 2. * cdiff, a1, FLD(b, FF_A1X), FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+        double dw[5];
+        load_dw(b, nb, dw);
+        face_contrib(dw, c[nb], ro[nb], FLD(b, FF_A1X)[n], FLD(b, FF_A1Y)[n], FLD(b, FF_A1Z)[n],
+                     ib.x1, ib.x2, ib.x5, yi);
+    }
+    if (j > 1) {
+        int nb = n - IS;
+        Coef jb = lusgs_coef<false>(b, nb, IS, cdiff * 2., a2, FLD(b, FF_A2X), FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+        double dw[5];
+        load_dw(b, nb, dw);
+        face_contrib(dw, c[nb], ro[nb], FLD(b, FF_A2X)[n], FLD(b, FF_A2Y)[n], FLD(b, FF_A2Z)[n],
+                     jb.x1, jb.x2, jb.x5, bj);
+    }
+    double a3l = a3[n], a1l = a1[n], a2l = a2[n];
+    double cd[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        double dq = FLD(b, FF_DRO + q)[n] * fac;          // DIAGGS
+        cd[q] = dt1 * a3l * ckc[q] + vol[n] * dq;
+        cd[q] = cd[q] + dt1 * (a1l * yi[q] + a2l * bj[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) FLD(b, FF_DRO + q)[n] = cd[q] * scinv;
+    FLD(b, FF_DE)[n] = cd[4] * scinp;
+}
+
+// DIVV12(DRO, DRO, CP) on interior cells (imps3.f:483)
+__global__ void __launch_bounds__(256) k_divcp(Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    int n = cid(b, i, j, k);
+    FLD(b, FF_DRO)[n] = FLD(b, FF_DRO)[n] / FLD(b, FF_CP)[n];
+}
+
+// Start-of-cycle copies (TOLD ...) on every cell
+__global__ void __launch_bounds__(256) k_save_old(Blk b)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    FLD(b, FF_TOLD)[n] = FLD(b, FF_TEMP)[n];
+    FLD(b, FF_UOLD)[n] = FLD(b, FF_U)[n];
+    FLD(b, FF_VOLD)[n] = FLD(b, FF_V)[n];
+    FLD(b, FF_WOLD)[n] = FLD(b, FF_W)[n];
+    FLD(b, FF_POLD)[n] = FLD(b, FF_PDIFF)[n];
+}
+
+// FIN (res3c.f:1122-1126) on interior cells
+__global__ void __launch_bounds__(256) k_fin(Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    int n = cid(b, i, j, k);
+    FLD(b, FF_TEMP)[n] = FLD(b, FF_TOLD)[n] + FLD(b, FF_DRO)[n];
+    FLD(b, FF_U)[n] = FLD(b, FF_UOLD)[n] + FLD(b, FF_DM)[n];
+    FLD(b, FF_V)[n] = FLD(b, FF_VOLD)[n] + FLD(b, FF_DN)[n];
+    FLD(b, FF_W)[n] = FLD(b, FF_WOLD)[n] + FLD(b, FF_DW)[n];
+    FLD(b, FF_PDIFF)[n] = FLD(b, FF_POLD)[n] + FLD(b, FF_DE)[n];
+}
+
+// Limiters of CYCLES (ns3c.f:1553-1596) on every cell: UPLIM acts on interior K-planes and
+// J-rows (all I, res3c.f:3401-3417), LOLIM / PDTOP / LOLIMP on all cells.
+__global__ void __launch_bounds__(256) k_limits(Prm p, Blk b)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    int kk = n / b.il;
+    int jj = (n / b.is) % b.js;
+    bool up = kk >= 2 && kk < 2 + b.kmax && jj >= 2 && jj < 2 + b.jmax;
+    double *x;
+    x = FLD(b, FF_TEMP);
+    double t = x[n];
+    if (up) t = t < p.tuplim ? t : p.tuplim;
+    t = t > p.tlolim ? t : p.tlolim;
+    x[n] = t;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        x = FLD(b, FF_U + q);
+        double s = x[n];
+        if (up) s = s < 5000. ? s : 5000.;
+        s = s > -5000. ? s : -5000.;
+        x[n] = s;
+    }
+    double pd = FLD(b, FF_PDIFF)[n];
+    double P = pd + p.frspre;
+    double y = P - 0.01 * p.frspre;
+    double prelim = 0. < y ? 0. : y;
+    FLD(b, FF_P)[n] = P - prelim;
+    FLD(b, FF_PDIFF)[n] = pd - prelim;
+}
+
+// ------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------
+
+struct BlockRec {
+    ff_block host{};
+    Blk dev{};
+    bool mirror = false;
+    double *arena = nullptr;   // device arena of a mirrored block
+};
+
+struct PendingEv {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct ff_ctx {
+    ff_params prm{};
+    Prm kp{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::map<long long, BlockRec> blocks;
+    std::string err;
+    double *red_part = nullptr;   // RESL2 partials
+    double *red_out = nullptr;    // device sums
+    double *red_host = nullptr;   // pinned host sums
+    int red_parts = 0;
+    bool profile = false;
+    std::vector<PendingEv> pending;
+    std::map<std::string, std::pair<long long, double>> stats;
+};
+
+static long long key(int level, int block) { return ((long long)level << 32) | (unsigned)block; }
+
+static int fail(ff_ctx *c, int code, const char *fmt, const char *a = "")
+{
+    if (c) {
+        char buf[512];
+        snprintf(buf, sizeof buf, fmt, a);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) return fail(ctx, FF_E_HIP, "%s", hipGetErrorString(e_));   \
+    } while (0)
+
+static Prm make_prm(const ff_params *p, int level)
+{
+    Prm k{};
+    k.gamma = p->gamma; k.rgas = p->rgas; k.pr = p->pr; k.prt = p->prt;
+    k.visu0 = p->visu0; k.tsu0 = p->tsu0; k.rmult = p->rmult;
+    k.frspre = p->frspre; k.frsden = p->frsden; k.frstem = p->frstem;
+    k.cfl = level >= 2 ? p->cfll : p->cfl;
+    k.theta = p->theta; k.cdiff = p->cdiff; k.tuplim = p->tuplim; k.tlolim = p->tlolim;
+    k.epslim = p->epslim;
+    k.iturb = p->iturb;
+    for (int d = 0; d < 3; d++) k.inter[d] = level >= 2 ? 55 : p->inter[d];   // res3c RESIDI ns3c.f:2468-2472
+    return k;
+}
+
+// Launch helper with optional event timing on the context's stream
+template <typename K, typename... A>
+static int launch(ff_ctx *c, const char *name, K kern, dim3 g, dim3 t, A... args)
+{
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (c->profile) {
+        HIPCHK(c, hipEventCreate(&ea));
+        HIPCHK(c, hipEventCreate(&eb));
+        HIPCHK(c, hipEventRecord(ea, c->stream));
+    }
+    hipLaunchKernelGGL(kern, g, t, 0, c->stream, args...);
+    HIPCHK(c, hipGetLastError());
+    if (c->profile) {
+        HIPCHK(c, hipEventRecord(eb, c->stream));
+        c->pending.push_back({name, ea, eb});
+    }
+    return FF_OK;
+}
+
+static int drain_events(ff_ctx *c)
+{
+    for (auto &e : c->pending) {
+        HIPCHK(c, hipEventSynchronize(e.b));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e.a, e.b));
+        auto &s = c->stats[e.name];
+        s.first += 1;
+        s.second += ms;
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    c->pending.clear();
+    return FF_OK;
+}
+
+static BlockRec *find(ff_ctx *c, int level, int block)
+{
+    auto it = c->blocks.find(key(level, block));
+    return it == c->blocks.end() ? nullptr : &it->second;
+}
+
+static Blk make_blk(const ff_block *v)
+{
+    Blk b{};
+    b.imax = v->imax; b.jmax = v->jmax; b.kmax = v->kmax;
+    b.is = b.imax + 2 * FF_GHOST; b.js = b.jmax + 2 * FF_GHOST; b.ks = b.kmax + 2 * FF_GHOST;
+    b.il = b.is * b.js;
+    b.ntot = b.il * b.ks;
+    for (int f = 0; f < FF_NFIELDS; f++) b.f[f] = v->f[f];
+    return b;
+}
+
+static int check_dims(ff_ctx *c, const ff_block *v)
+{
+    if (!v || v->imax < 1 || v->jmax < 1 || v->kmax < 1)
+        return fail(c, FF_E_ARG, "block dimensions must be positive%s");
+    long long n = (long long)(v->imax + 4) * (v->jmax + 4) * (v->kmax + 4);
+    if (n * FF_NFIELDS >= (1LL << 31))
+        return fail(c, FF_E_ARG, "block too large for 32-bit cell offsets%s");
+    return FF_OK;
+}
+
+// fields a kernel sequence needs
+static const int kNeeded[] = {
+    FF_PDIFF, FF_TEMP, FF_U, FF_V, FF_W, FF_P, FF_RO, FF_E, FF_C, FF_VIS, FF_CH, FF_CP,
+    FF_DRDP, FF_DRDH, FF_RM, FF_RN, FF_RW, FF_EPS2, FF_VIST, FF_DRO, FF_DM, FF_DN, FF_DW,
+    FF_DE, FF_DTL, FF_TOLD, FF_UOLD, FF_VOLD, FF_WOLD, FF_POLD, FF_VOL, FF_A1, FF_A1X,
+    FF_A1Y, FF_A1Z, FF_A2, FF_A2X, FF_A2Y, FF_A2Z, FF_A3, FF_A3X, FF_A3Y, FF_A3Z};
+
+extern "C" {
+
+int ff_ctx_create(ff_ctx **out, const ff_params *prm)
+{
+    if (!out || !prm) return FF_E_ARG;
+    *out = nullptr;
+    if (prm->iturb != 0) return FF_E_UNSUP;        // this build covers ITURB = 0
+    if (prm->iflux != 1) return FF_E_UNSUP;        // Roe flux (FLUXKE)
+    if (prm->idiff > 3) return FF_E_UNSUP;         // arithmetic FLUXDO average
+    if (prm->expsu != 1.5) return FF_E_UNSUP;      // Sutherland sqrt branch of PERFGA
+    ff_ctx *c = new ff_ctx();
+    c->prm = *prm;
+    c->kp = make_prm(prm, 1);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return FF_E_HIP;
+    }
+    c->own_stream = true;
+    c->red_parts = 512;
+    if (hipMalloc(&c->red_part, sizeof(double) * 5 * c->red_parts) != hipSuccess ||
+        hipMalloc(&c->red_out, sizeof(double) * 8) != hipSuccess ||
+        hipHostMalloc(&c->red_host, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess) {
+        ff_ctx_destroy(c);
+        return FF_E_NOMEM;
+    }
+    *out = c;
+    return FF_OK;
+}
+
+int ff_ctx_destroy(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drain_events(c);
+    for (auto &kv : c->blocks)
+        if (kv.second.mirror && kv.second.arena) hipFree(kv.second.arena);
+    if (c->red_part) hipFree(c->red_part);
+    if (c->red_out) hipFree(c->red_out);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FF_OK;
+}
+
+int ff_ctx_set_stream(ff_ctx *c, void *stream)
+{
+    if (!c) return FF_E_ARG;
+    if (stream) {
+        if (c->own_stream && c->stream) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipStreamDestroy(c->stream);
+        }
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    }
+    return FF_OK;
+}
+
+const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ff_ctx_profile(ff_ctx *c, int enable)
+{
+    if (!c) return FF_E_ARG;
+    c->profile = enable != 0;
+    return FF_OK;
+}
+
+int ff_kernel_stats(const ff_ctx *cc, const char *name, long long *launches, double *ms)
+{
+    ff_ctx *c = const_cast<ff_ctx *>(cc);
+    if (!c || !name) return FF_E_ARG;
+    int rc = drain_events(c);
+    if (rc) return rc;
+    auto it = c->stats.find(name);
+    if (launches) *launches = it == c->stats.end() ? 0 : it->second.first;
+    if (ms) *ms = it == c->stats.end() ? 0. : it->second.second;
+    return FF_OK;
+}
+
+int ff_kernel_stats_reset(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    int rc = drain_events(c);
+    c->stats.clear();
+    return rc;
+}
+
+int ff_block_register(ff_ctx *c, int level, int block, const ff_block *host)
+{
+    if (!c) return FF_E_ARG;
+    int rc = check_dims(c, host);
+    if (rc) return rc;
+    for (int f : kNeeded)
+        if (!host->f[f]) return fail(c, FF_E_ARG, "host field %s is NULL", std::to_string(f).c_str());
+    BlockRec r;
+    r.host = *host;
+    r.mirror = true;
+    Blk b = make_blk(host);
+    size_t bytes = sizeof(double) * (size_t)b.ntot * FF_NFIELDS;
+    if (hipMalloc(&r.arena, bytes) != hipSuccess) return fail(c, FF_E_NOMEM, "device arena%s");
+    HIPCHK(c, hipMemsetAsync(r.arena, 0, bytes, c->stream));
+    for (int f = 0; f < FF_NFIELDS; f++) b.f[f] = r.arena + (size_t)f * b.ntot;
+    r.dev = b;
+    BlockRec *old = find(c, level, block);
+    if (old && old->mirror && old->arena) (void)hipFree(old->arena);
+    c->blocks[key(level, block)] = r;
+    return FF_OK;
+}
+
+int ff_block_register_device(ff_ctx *c, int level, int block, const ff_block *dev)
+{
+    if (!c) return FF_E_ARG;
+    int rc = check_dims(c, dev);
+    if (rc) return rc;
+    for (int f : kNeeded)
+        if (!dev->f[f]) return fail(c, FF_E_ARG, "device field %s is NULL", std::to_string(f).c_str());
+    BlockRec r;
+    r.mirror = false;
+    r.dev = make_blk(dev);
+    BlockRec *old = find(c, level, block);
+    if (old && old->mirror && old->arena) (void)hipFree(old->arena);
+    c->blocks[key(level, block)] = r;
+    return FF_OK;
+}
+
+int ff_sync(ff_ctx *c, int level, int block, int direction, unsigned long long mask)
+{
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    if (!r->mirror) return FF_OK;
+    size_t bytes = sizeof(double) * (size_t)r->dev.ntot;
+    for (int f = 0; f < FF_NFIELDS; f++) {
+        if (!(mask >> f & 1ULL) || !r->host.f[f]) continue;
+        if (direction == 0)
+            HIPCHK(c, hipMemcpyAsync(r->dev.f[f], r->host.f[f], bytes, hipMemcpyHostToDevice, c->stream));
+        else
+            HIPCHK(c, hipMemcpyAsync(r->host.f[f], r->dev.f[f], bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FF_OK;
+}
+
+int ff_finish(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FF_OK;
+}
+
+static inline dim3 cells_grid(const Blk &b, int tpb) { return dim3((b.imax + tpb - 1) / tpb, b.jmax, b.kmax); }
+static inline dim3 all_grid(const Blk &b) { return dim3((b.ntot + 255) / 256); }
+
+int ff_state(ff_ctx *c, int level, int block)
+{
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return launch(c, "k_state", k_state, all_grid(r->dev), dim3(256), make_prm(&c->prm, level), r->dev);
+}
+
+int ff_turb(ff_ctx *c, int level, int block)
+{
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return launch(c, "k_turb_lam", k_turb_lam, all_grid(r->dev), dim3(256), r->dev);
+}
+
+static int residual_impl(ff_ctx *c, BlockRec *r, int level, double *resi)
+{
+    const Blk &b = r->dev;
+    int rc;
+    // ZEROZZ of DRO..DE on every cell (res3c.f:194-198): ghost residuals stay zero
+    for (int q = 0; q < 5; q++) {
+        rc = launch(c, "k_zero", k_zero, all_grid(b), dim3(256), b.f[FF_DRO + q], b.ntot);
+        if (rc) return rc;
+    }
+    rc = launch(c, "k_residual", k_residual, cells_grid(b, 128), dim3(128), make_prm(&c->prm, level), b);
+    if (rc) return rc;
+    if (level == 1 && resi) {
+        int ncell = b.imax * b.jmax * b.kmax;
+        int parts = (ncell + RED_THREADS - 1) / RED_THREADS;
+        if (parts > c->red_parts) parts = c->red_parts;
+        rc = launch(c, "k_resl2_part", k_resl2_part, dim3(parts), dim3(RED_THREADS), b, c->red_part);
+        if (rc) return rc;
+        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)c->red_part, parts, c->red_out);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int q = 0; q < 5; q++) resi[q] = c->red_host[q];
+        resi[5] = resi[6] = 0.;
+    }
+    return FF_OK;
+}
+
+int ff_residual(ff_ctx *c, int level, int block, int m2, double *resi)
+{
+    (void)m2;
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return residual_impl(c, r, level, resi);
+}
+
+static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
+{
+    const Blk &b = r->dev;
+    Prm kp = make_prm(&c->prm, level);
+    int rc = launch(c, "k_time3_copr", k_time3_copr, cells_grid(b, 128), dim3(128), kp, b);
+    if (rc) return rc;
+    const int mlo = 3, mhi = b.imax + b.jmax + b.kmax;
+    // backward sweep: hyperplanes from the far corner down
+    for (int m = mhi; m >= mlo; m--) {
+        int klo = m - b.imax - b.jmax > 1 ? m - b.imax - b.jmax : 1;
+        int khi = m - 2 < b.kmax ? m - 2 : b.kmax;
+        int jlo = m - b.imax - b.kmax > 1 ? m - b.imax - b.kmax : 1;
+        int jhi = m - 2 < b.jmax ? m - 2 : b.jmax;
+        int jn = jhi - jlo + 1, kn = khi - klo + 1;
+        int nt = jn * kn;
+        rc = launch(c, "k_pred_plane", k_pred_plane, dim3((nt + 127) / 128), dim3(128), kp, b, m, jlo, klo, jn);
+        if (rc) return rc;
+    }
+    for (int m = mlo; m <= mhi; m++) {
+        int klo = m - b.imax - b.jmax > 1 ? m - b.imax - b.jmax : 1;
+        int khi = m - 2 < b.kmax ? m - 2 : b.kmax;
+        int jlo = m - b.imax - b.kmax > 1 ? m - b.imax - b.kmax : 1;
+        int jhi = m - 2 < b.jmax ? m - 2 : b.jmax;
+        int jn = jhi - jlo + 1, kn = khi - klo + 1;
+        int nt = jn * kn;
+        rc = launch(c, "k_corr_plane", k_corr_plane, dim3((nt + 127) / 128), dim3(128), kp, b, m, jlo, klo, jn);
+        if (rc) return rc;
+    }
+    if (divcp) {
+        rc = launch(c, "k_divcp", k_divcp, cells_grid(b, 256), dim3(256), b);
+        if (rc) return rc;
+    }
+    return FF_OK;
+}
+
+int ff_implicit(ff_ctx *c, int level, int block, int m2)
+{
+    (void)m2;
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return implicit_impl(c, r, level, true);
+}
+
+static int update_impl(ff_ctx *c, BlockRec *r, int level)
+{
+    const Blk &b = r->dev;
+    int rc = launch(c, "k_fin", k_fin, cells_grid(b, 256), dim3(256), b);
+    if (rc) return rc;
+    return launch(c, "k_limits", k_limits, all_grid(b), dim3(256), make_prm(&c->prm, level), b);
+}
+
+int ff_update(ff_ctx *c, int level, int block)
+{
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return update_impl(c, r, level);
+}
+
+int ff_save_old(ff_ctx *c, int level, int block)
+{
+    if (!c) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev);
+}
+
+int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
+{
+    if (!c || ncycles < 0) return FF_E_ARG;
+    BlockRec *r = find(c, 1, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    int rc;
+    for (int it = 0; it < ncycles; it++) {
+        bool last = it == ncycles - 1;
+        if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev))) return rc;
+        if ((rc = ff_state(c, 1, block))) return rc;
+        if ((rc = ff_turb(c, 1, block))) return rc;
+        if ((rc = residual_impl(c, r, 1, last ? resi : nullptr))) return rc;
+        if ((rc = implicit_impl(c, r, 1, true))) return rc;
+        if ((rc = update_impl(c, r, 1))) return rc;
+    }
+    return FF_OK;
+}
+
+}  // extern "C"

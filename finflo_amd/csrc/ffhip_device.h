@@ -1,0 +1,373 @@
+// ffhip_device.h — device arithmetic of the FINFLO hot path for gfx950 (FP64).
+//
+// Every function reproduces the operation order and literals of the reference routine it
+// cites, so that with -ffp-contract=off the results equal the reference's FP64 results bit
+// for bit.  Arrays are the reference's flat i-fastest slabs with two ghost layers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/ffhip.h"
+
+namespace ff {
+
+// Field pointers of one block on the device, passed by value to every kernel.
+struct Blk {
+    int imax, jmax, kmax;
+    int is, js, ks;        // ISTRID, JSTRID, KSTRID
+    int il;                // ISTRID*JSTRID
+    int ntot;
+    double *f[FF_NFIELDS];
+};
+
+// Physical constants used by the kernels (subset of ff_params)
+struct Prm {
+    double gamma, rgas, pr, prt, visu0, tsu0, rmult;
+    double frspre, frsden, frstem;
+    double cfl, theta, cdiff, tuplim, tlolim, epslim;
+    int iturb;
+    int inter[3];
+};
+
+#define FLD(b, n) ((b).f[n])
+
+__device__ __forceinline__ int cid(const Blk &b, int i, int j, int k)
+{
+    return (k + 1) * b.il + (j + 1) * b.is + (i + 1);
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+// MUSCL kappa-scheme, XXTRAP base3c.f:750-850, face between cells L-S and L
+__device__ __forceinline__ void xxtrap(const double *__restrict__ q, int L, int S, int intem,
+                                       double &qr, double &ql)
+{
+    int inter = intem < 0 ? -intem : intem;
+    int inte2 = inter < 1 ? 1 : (inter > 5 ? 5 : inter);
+    // RKK1 = {0., 1., 1.3333333, 2., 1.5} (base3c.f:765)
+    double rk1 = inte2 == 1 ? 0. : inte2 == 2 ? 1. : inte2 == 3 ? 1.3333333 : inte2 == 4 ? 2. : 1.5;
+    double rk2 = 2. - rk1;
+    double r1 = .25 * rk1, r2 = .25 * rk2;
+    double q1 = q[L - S], q2 = q[L];
+    if (intem <= 0 && intem >= -5) {
+        double q0 = q[L - 2 * S], q3 = q[L + S];
+        double diff0 = q1 - q0, diff1 = q2 - q1, diff2 = q3 - q2;
+        qr = q2 - (diff1 * r1 + diff2 * r2);
+        ql = q1 + (diff1 * r1 + diff0 * r2);
+    } else if (inter <= 4) {
+        double q0 = q[L - 2 * S], q3 = q[L + S];
+        const double eps = 1.E-10;
+        double diff0 = q1 - q0, diff1 = q2 - q1, diff2 = q3 - q2;
+        double diff12 = diff1 * diff1 + eps;
+        double phibi = (diff12 + diff1 * diff0) / (diff12 + diff0 * diff0);
+        double phifi = (diff12 + diff1 * diff2) / (diff12 + diff2 * diff2);
+        qr = q2 - phifi * (diff1 * r1 + diff2 * r2);
+        ql = q1 + phibi * (diff1 * r1 + diff0 * r2);
+    } else {
+        qr = q2;
+        ql = q1;
+    }
+}
+
+// SURFVECTS base3c.f:12245-12295
+__device__ __forceinline__ void surfvects(double a11, double a12, double a13,
+                                          double &a21, double &a22, double &a23,
+                                          double &a31, double &a32, double &a33)
+{
+    double am = fabs(a11);
+    int idir = 1;
+    if (fabs(a12) < am) { am = fabs(a12); idir = 2; }
+    if (fabs(a13) < am) { idir = 3; }
+    double b1, b2, b3;
+    if (idir == 1) { b1 = 0.0; b2 = -a13; b3 = a12; }
+    else if (idir == 2) { b1 = a13; b2 = 0.0; b3 = -a11; }
+    else { b1 = -a12; b2 = a11; b3 = 0.0; }
+    double palpo = 1. / sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+    b1 = b1 * palpo; b2 = b2 * palpo; b3 = b3 * palpo;
+    a21 = b1; a22 = b2; a23 = b3;
+    a31 = a12 * b3 - a13 * b2;
+    a32 = a13 * b1 - a11 * b3;
+    a33 = a11 * b2 - a12 * b1;
+}
+
+// EFPRO state.f:636-638 with FRSPRE = 0 and E0REF = 0 as FLUXKE calls it (base3c.f:4882)
+__device__ __forceinline__ double efpro(double pp, double ro, double gamma)
+{
+    double gm1 = gamma - 1.;
+    return (pp + 0.) / (gm1 * ro) - 0.;
+}
+
+struct Flux5 { double f[5]; };
+
+// FLUXKE base3c.f:4753-5125 at one face (laminar: k and omega are zero), with DERIV's
+// I2 = I branch (base3c.f:719) on k-direction faces.
+template <bool SINGLE>
+__device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int S, int intem,
+                                          const double *__restrict__ an,
+                                          const double *__restrict__ anx,
+                                          const double *__restrict__ any,
+                                          const double *__restrict__ anz)
+{
+    const double *ro = FLD(b, FF_RO), *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W),
+                 *P = FLD(b, FF_P), *c = FLD(b, FF_C), *drdp = FLD(b, FF_DRDP),
+                 *drdh = FLD(b, FF_DRDH);
+    double roip, roim, rmip, rmim, rnip, rnim, rwip, rwim, ppi, pmi;
+    xxtrap(ro, L, S, intem, roip, roim);
+    xxtrap(u, L, S, intem, rmip, rmim);
+    xxtrap(v, L, S, intem, rnip, rnim);
+    xxtrap(w, L, S, intem, rwip, rwim);
+    xxtrap(P, L, S, intem, ppi, pmi);
+    double fd = 0.001 * p.frsden, fp = 0.001 * p.frspre;
+    roip = roip > fd ? roip : fd;
+    ppi = ppi > fp ? ppi : fp;
+    roim = roim > fd ? roim : fd;
+    pmi = pmi > fp ? pmi : fp;
+    double eip = efpro(ppi, roip, p.gamma);
+    double eim = efpro(pmi, roim, p.gamma);
+    double elr = efpro(ppi, roim, p.gamma);
+    double erl = efpro(pmi, roip, p.gamma);
+    // DERIV base3c.f:707-742
+    int i2 = SINGLE ? L : L - S;
+    double ypro1 = 1. / ro[L], ypro2 = 1. / ro[i2];
+    double dedp1 = -drdp[L] / drdh[L] - ypro1;
+    double dedp2 = -drdp[i2] / drdh[i2] - ypro2;
+    double dedph = 0.5 * (dedp1 + dedp2);
+    double dedr1 = 1. / drdh[L] + P[L] * (ypro1 * ypro1);
+    double dedr2 = 1. / drdh[i2] + P[i2] * (ypro2 * ypro2);
+    double dedrh = 0.5 * (dedr1 + dedr2);
+    {
+        double dp = 2. * (ppi - pmi);
+        double dro = 2. * (roip - roim);
+        if (fabs(dp) >= 1.E-2 * (ppi + pmi)) dedph = (eip + elr - eim - erl) / dp;
+        if (fabs(dro) >= 1.E-2 * (roip + roim)) dedrh = (eip + erl - eim - elr) / dro;
+    }
+    double a11 = anx[L], a12 = any[L], a13 = anz[L];
+    double a21, a22, a23, a31, a32, a33;
+    surfvects(a11, a12, a13, a21, a22, a23, a31, a32, a33);
+    double ur = a11 * rmip + a12 * rnip + a13 * rwip;
+    double vr = a21 * rmip + a22 * rnip + a23 * rwip;
+    double wr = a31 * rmip + a32 * rnip + a33 * rwip;
+    double ul = a11 * rmim + a12 * rnim + a13 * rwim;
+    double vl = a21 * rmim + a22 * rnim + a23 * rwim;
+    double wl = a31 * rmim + a32 * rnim + a33 * rwim;
+    // k and omega of the algebraic path: MAX(0.001*RKLIM, 0) with RKLIM = 0
+    const double rkipi = 0., rkimi = 0.;
+    double ypror = 1. / roip;
+    double rkr = rkipi * ypror;
+    double yprol = 1. / roim;
+    double rkl = rkimi * yprol;
+    double eir = eip + .5 * (ur * ur + vr * vr + wr * wr) + rkr;
+    double eil = eim + .5 * (ul * ul + vl * vl + wl * wl) + rkl;
+    double pr = ppi;
+    double pkr = .6667 * rkipi;
+    double hr = eir + pr * ypror;
+    double er = eip;
+    double rohat = sqrt(roip * roim);
+    double yproh = 1. / rohat;
+    double pl = pmi;
+    double pkl = .6667 * rkimi;
+    double hl = eil + yprol * pl;
+    double el = eim;
+    double sqrl = 1. / (roim + rohat);
+    double cql = roim * sqrl;
+    double cqr = 1. - cql;
+    double uhat = cqr * ur + cql * ul;
+    double vhat = cqr * vr + cql * vl;
+    double what = cqr * wr + cql * wl;
+    double hhat = cqr * hr + cql * hl;
+    double ehat = cqr * er + cql * el;
+    double rkhat = cqr * rkr + cql * rkl;
+    double phat = rohat * (hhat - ehat - .5 * (uhat * uhat + vhat * vhat + what * what) - rkhat);
+    double pmin = pr < pl ? pr : pl;
+    double pmax = pr > pl ? pr : pl;
+    phat = pmax < phat ? pmax : phat;
+    phat = pmin > phat ? pmin : phat;
+    double drdhh = 1. / (dedrh - phat * (yproh * yproh));
+    double drdph = -drdhh * (dedph + yproh);
+    double xturb = 1. + .6667 * rkhat * yproh * drdph;
+    double chat2 = xturb / (drdhh * yproh + drdph);
+    double cL = c[L], cM = c[L - S];
+    double cmin = cL < cM ? cL : cM;
+    double cmax = cL > cM ? cL : cM;
+    chat2 = (cmax * cmax) < chat2 ? (cmax * cmax) : chat2;
+    chat2 = (cmin * cmin) > chat2 ? (cmin * cmin) : chat2;
+    double chat = sqrt(chat2);
+    double pdpde = dedph;
+    double dro = roip - roim;
+    double du = ur - ul;
+    double dk = rkr - rkl;
+    double dp = pr - pl + .6667 * (rkhat * dro + rohat * dk);
+    double pchat2 = 1. / (chat * chat);
+    double alfa1 = .5 * (dp + rohat * chat * du) * pchat2;
+    double alfa2 = .5 * (dp - rohat * chat * du) * pchat2;
+    double alfa3 = dro - dp * pchat2;
+    double alfa4 = rohat * (vr - vl);
+    double alfa5 = rohat * (wr - wl);
+    double alfa6 = rohat * dk;
+    double rl1 = fabs(uhat + chat);
+    double rl2 = fabs(uhat - chat);
+    double rl3 = fabs(uhat);
+    double rl4 = rl3, rl5 = rl3, rl6 = rl3;
+    double roucr = roip * ur;
+    double roucl = roim * ul;
+    double damp = rl1 * alfa1 + rl2 * alfa2 + rl3 * alfa3;
+    double ha = .5 * an[L];
+    double f1 = ha * (roucr + roucl - damp);
+    double f2 = ha * (roucr * ur + pr + pkr + roucl * ul + pl + pkl -
+                      (damp * uhat + (rl1 * alfa1 - rl2 * alfa2) * chat));
+    double f3 = ha * (roucr * vr + roucl * vl - damp * vhat - rl4 * alfa4);
+    double f4 = ha * (roucr * wr + roucl * wl - damp * what - rl5 * alfa5);
+    double f5 = ha * (roucr * (hr + pkr * ypror) + roucl * (hl + pkl * yprol)
+                      - damp * hhat - ((rl1 * alfa1 - rl2 * alfa2) * uhat * chat
+                      - rl3 * alfa3 * rohat * (chat * chat) * pdpde + rl4 * alfa4 * vhat
+                      + rl5 * alfa5 * what) + rl6 * alfa6 * (1. - .6667 * rohat * pdpde));
+    Flux5 out;
+    out.f[0] = f1;
+    out.f[1] = a11 * f2 + a21 * f3 + a31 * f4;
+    out.f[2] = a12 * f2 + a22 * f3 + a32 * f4;
+    out.f[3] = a13 * f2 + a23 * f3 + a33 * f4;
+    out.f[4] = f5;
+    return out;
+}
+
+// FLUXDO base3c.f:3677-3869 (IDIFF <= 3) at one face: momentum fluxes and the energy term
+__device__ __forceinline__ void visc_face(const Prm &p, const Blk &b, int L, int S,
+                                          const double *__restrict__ an,
+                                          const double *__restrict__ anx,
+                                          const double *__restrict__ any,
+                                          const double *__restrict__ anz, double &fm,
+                                          double &fn, double &fw, double &fe)
+{
+    const double *vol = FLD(b, FF_VOL), *eps2 = FLD(b, FF_EPS2), *vis = FLD(b, FF_VIS),
+                 *cp = FLD(b, FF_CP), *vist = FLD(b, FF_VIST), *ch = FLD(b, FF_CH),
+                 *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W),
+                 *t = FLD(b, FF_TEMP);
+    int m = L - S;
+    double ypd2 = an[L] / (vol[L] + vol[m]);
+    double surg = -an[L] * ypd2;
+    double surf = surg * (eps2[L] * vis[L] + eps2[m] * vis[m]);
+    double surfe = surg * (cp[L] * vist[L] / p.prt + ch[L] + cp[m] * vist[m] / p.prt + ch[m]);
+    double duc = .333333 * (anx[L] * (u[L] - u[m]) + any[L] * (v[L] - v[m]) + anz[L] * (w[L] - w[m]));
+    double f2 = surf * (u[L] - u[m] + anx[L] * duc);
+    double f3 = surf * (v[L] - v[m] + any[L] * duc);
+    double f4 = surf * (w[L] - w[m] + anz[L] * duc);
+    double f5 = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4);
+    fm = f2; fn = f3; fw = f4;
+    fe = f5 + surfe * (t[L] - t[m]);
+}
+
+// FLUCYC face flux: FLUXDO then FLUXKE added (res3c.f:7856, 7892)
+template <bool SINGLE>
+__device__ __forceinline__ Flux5 face_flux(const Prm &p, const Blk &b, int L, int S, int intem,
+                                           const double *__restrict__ an,
+                                           const double *__restrict__ anx,
+                                           const double *__restrict__ any,
+                                           const double *__restrict__ anz)
+{
+    double fm, fn, fw, fe;
+    visc_face(p, b, L, S, an, anx, any, anz, fm, fn, fw, fe);
+    Flux5 r = roe_face<SINGLE>(p, b, L, S, intem, an, anx, any, anz);
+    Flux5 o;
+    o.f[0] = r.f[0] + 0.;
+    o.f[1] = r.f[1] + fm;
+    o.f[2] = r.f[2] + fn;
+    o.f[3] = r.f[3] + fw;
+    o.f[4] = r.f[4] + (0. + fe);
+    return o;
+}
+
+// SYI imps4.f:10-40
+__device__ __forceinline__ void syi(double &w1, double &w2, double &w3, double &w4, double &w5,
+                                    double x1, double x2, double x3, double x4, double x5,
+                                    double c, double ro, double a2x, double a2y, double a2z,
+                                    double &a21, double &a22, double &a23, double &a31,
+                                    double &a32, double &a33)
+{
+    double z1i = x1, z5i = x5;
+    surfvects(a2x, a2y, a2z, a21, a22, a23, a31, a32, a33);
+    double z2i = a2x * x2 + a2y * x3 + a2z * x4;
+    double z3i = a21 * x2 + a22 * x3 + a23 * x4;
+    double z4i = a31 * x2 + a32 * x3 + a33 * x4;
+    double roc = ro * c;
+    w1 = z1i - z5i / ro;
+    w2 = roc * z2i + z5i;
+    w3 = z3i;
+    w4 = z4i;
+    w5 = -roc * z2i + z5i;
+}
+
+// SYM1I imps4.f:46-70
+__device__ __forceinline__ void sym1i(double &x1, double &x2, double &x3, double &x4,
+                                      double &x5, double w1, double w2, double w3, double w4,
+                                      double w5, double c, double ro, double a2x, double a2y,
+                                      double a2z, double a21, double a22, double a23,
+                                      double a31, double a32, double a33)
+{
+    double ypro = 1. / ro;
+    double pc2 = .5 * ypro;
+    double proc = pc2 / c;
+    x1 = w1 + pc2 * (w2 + w5);
+    double x2i = proc * (w2 - w5);
+    double x3i = w3, x4i = w4;
+    x5 = .5 * (w2 + w5);
+    x2 = a2x * x2i + a21 * x3i + a31 * x4i;
+    x3 = a2y * x2i + a22 * x3i + a32 * x4i;
+    x4 = a2z * x2i + a23 * x3i + a33 * x4i;
+}
+
+// SYM1I(XI * SYI(dW)) across one face (imps4.f:252-290, 513-552, 316-328)
+__device__ __forceinline__ void face_contrib(const double dw[5], double c, double ro,
+                                             double nx, double ny, double nz, double x1,
+                                             double x2, double x5, double out[5])
+{
+    double a21, a22, a23, a31, a32, a33, w1, w2, w3, w4, w5;
+    syi(w1, w2, w3, w4, w5, dw[0], dw[1], dw[2], dw[3], dw[4], c, ro, nx, ny, nz, a21, a22,
+        a23, a31, a32, a33);
+    w1 = x1 * w1;
+    w2 = x2 * w2;
+    w3 = x1 * w3;
+    w4 = x1 * w4;
+    w5 = x5 * w5;
+    sym1i(out[0], out[1], out[2], out[3], out[4], w1, w2, w3, w4, w5, c, ro, nx, ny, nz, a21,
+          a22, a23, a31, a32, a33);
+}
+
+// Split-eigenvalue coefficients of cell n across one face.
+//   PREDGS (imps4.f:405-454): face = own lower face n, neighbour n-S, sign -1.
+//   CORRGS (imps4.f:150-195): face = own upper face n+S, neighbour n+S, viscous area A(n).
+struct Coef { double x1, x2, x5, rmax, xa; };
+
+template <bool PRED>
+__device__ __forceinline__ Coef lusgs_coef(const Blk &b, int n, int S, double cdiffd,
+                                           const double *__restrict__ an,
+                                           const double *__restrict__ anx,
+                                           const double *__restrict__ any,
+                                           const double *__restrict__ anz)
+{
+    const double *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W), *c = FLD(b, FF_C),
+                 *vol = FLD(b, FF_VOL), *ro = FLD(b, FF_RO), *vis = FLD(b, FF_VIS),
+                 *vist = FLD(b, FF_VIST);
+    int nf = PRED ? n : n + S;          // face whose normal is used
+    int nb = PRED ? n - S : n + S;      // neighbour across it
+    double rl1 = anx[nf] * u[n] + any[nf] * v[n] + anz[nf] * w[n] - 0.;
+    double rl2 = rl1 + c[n];
+    double rl5 = rl1 - c[n];
+    double rnum = (vol[n] + vol[nb]) * (ro[n] + ro[nb]);
+    double t = cdiffd * an[n] / rnum * (vist[n] + vis[n] + vist[nb] + vis[nb]);
+    double rmax = 1.00 * (fabs(rl1) + c[n]);
+    Coef k;
+    if (PRED) {
+        k.x1 = (rl1 - rmax) * .5 - t;
+        k.x2 = (rl2 - rmax) * .5 - t;
+        k.x5 = (rl5 - rmax) * .5 - t;
+    } else {
+        k.x1 = (rl1 + rmax) * .5 + t;
+        k.x2 = (rl2 + rmax) * .5 + t;
+        k.x5 = (rl5 + rmax) * .5 + t;
+    }
+    k.rmax = rmax;
+    k.xa = rmax + t;
+    return k;
+}
+
+}  // namespace ff

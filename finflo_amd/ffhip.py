@@ -1,0 +1,145 @@
+"""ctypes binding of libffhip.so, mirroring FINFLO's per-block call surface.
+
+The method names follow the reference's dispatchers called from CYCLES (ns3c.f:1354-2349):
+``statim`` (STATIM), ``turbdi`` (TURBDI), ``residi`` (RESIDI), ``impsdi`` (IMPSDI) and ``fin``
+(FIN + limiters).  The library must be built (finflo_amd/build.py); a missing or unloadable
+library raises — there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .layout import ALL_MASK, Block, BlockView, Params, mask
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libffhip.so")
+
+EXPORTS = [
+    "ff_ctx_create", "ff_ctx_destroy", "ff_ctx_set_stream", "ff_last_error", "ff_ctx_profile",
+    "ff_kernel_stats", "ff_kernel_stats_reset", "ff_block_register",
+    "ff_block_register_device", "ff_sync", "ff_finish", "ff_state", "ff_turb", "ff_residual",
+    "ff_implicit", "ff_update", "ff_save_old", "ff_cycle",
+]
+
+_lib = None
+
+
+class FFError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libffhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise FFError(f"{LIBPATH} not built: run python -m finflo_amd.build")
+        L = C.CDLL(LIBPATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        L.ff_ctx_create.argtypes = [C.POINTER(vp), C.POINTER(Params)]
+        L.ff_ctx_destroy.argtypes = [vp]
+        L.ff_ctx_set_stream.argtypes = [vp, vp]
+        L.ff_last_error.argtypes = [vp]
+        L.ff_last_error.restype = C.c_char_p
+        L.ff_ctx_profile.argtypes = [vp, i]
+        L.ff_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_longlong), C.POINTER(d)]
+        L.ff_kernel_stats_reset.argtypes = [vp]
+        L.ff_block_register.argtypes = [vp, i, i, C.POINTER(BlockView)]
+        L.ff_block_register_device.argtypes = [vp, i, i, C.POINTER(BlockView)]
+        L.ff_sync.argtypes = [vp, i, i, i, C.c_ulonglong]
+        L.ff_finish.argtypes = [vp]
+        for fn in ("ff_state", "ff_turb", "ff_update", "ff_save_old"):
+            getattr(L, fn).argtypes = [vp, i, i]
+        L.ff_residual.argtypes = [vp, i, i, i, vp]
+        L.ff_implicit.argtypes = [vp, i, i, i]
+        L.ff_cycle.argtypes = [vp, i, i, vp]
+        for fn in EXPORTS:
+            if fn != "ff_last_error":
+                getattr(L, fn).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class Context:
+    """One ff_ctx: parameters, a HIP stream and registered blocks."""
+
+    def __init__(self, prm: Params):
+        self.prm = prm
+        self.h = C.c_void_p()
+        rc = lib().ff_ctx_create(C.byref(self.h), C.byref(prm))
+        if rc != 0:
+            raise FFError(f"ff_ctx_create failed with code {rc}")
+        self._views = {}
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().ff_last_error(self.h)
+            raise FFError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            lib().ff_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- blocks ------------------------------------------------------------------
+    def register(self, blk: Block, level=1, block=1):
+        v = blk.view_struct()
+        self._views[(level, block)] = (blk, v)
+        self._chk(lib().ff_block_register(self.h, level, block, C.byref(v)), "ff_block_register")
+
+    def upload(self, level=1, block=1, fields=None):
+        m = ALL_MASK if fields is None else mask(fields)
+        self._chk(lib().ff_sync(self.h, level, block, 0, m), "ff_sync(h2d)")
+
+    def download(self, level=1, block=1, fields=None):
+        m = ALL_MASK if fields is None else mask(fields)
+        self._chk(lib().ff_sync(self.h, level, block, 1, m), "ff_sync(d2h)")
+
+    def finish(self):
+        self._chk(lib().ff_finish(self.h), "ff_finish")
+
+    # ---- the reference's call surface ------------------------------------------------
+    def statim(self, level=1, block=1):
+        self._chk(lib().ff_state(self.h, level, block), "ff_state")
+
+    def turbdi(self, level=1, block=1):
+        self._chk(lib().ff_turb(self.h, level, block), "ff_turb")
+
+    def residi(self, level=1, block=1, m2=1):
+        resi = np.zeros(7)
+        self._chk(lib().ff_residual(self.h, level, block, m2, resi.ctypes.data), "ff_residual")
+        return resi
+
+    def impsdi(self, level=1, block=1, m2=1):
+        self._chk(lib().ff_implicit(self.h, level, block, m2), "ff_implicit")
+
+    def fin(self, level=1, block=1):
+        self._chk(lib().ff_update(self.h, level, block), "ff_update")
+
+    def save_old(self, level=1, block=1):
+        self._chk(lib().ff_save_old(self.h, level, block), "ff_save_old")
+
+    def cycle(self, block=1, ncycles=1):
+        resi = np.zeros(7)
+        self._chk(lib().ff_cycle(self.h, block, ncycles, resi.ctypes.data), "ff_cycle")
+        return resi
+
+    # ---- timing ------------------------------------------------------------------------
+    def profile(self, on=True):
+        self._chk(lib().ff_ctx_profile(self.h, 1 if on else 0), "ff_ctx_profile")
+
+    def kernel_stats(self, name):
+        n, ms = C.c_longlong(), C.c_double()
+        self._chk(lib().ff_kernel_stats(self.h, name.encode(), C.byref(n), C.byref(ms)),
+                  "ff_kernel_stats")
+        return n.value, ms.value
+
+    def reset_stats(self):
+        self._chk(lib().ff_kernel_stats_reset(self.h), "ff_kernel_stats_reset")

@@ -1,0 +1,34 @@
+"""Shared helpers of the parity tests: seeded cases and field comparisons."""
+import numpy as np
+
+from finflo_amd.cases import make_block
+from finflo_amd.layout import default_params
+
+# (name, dims, params overrides, case kwargs)
+CASES = [
+    ("cart_density", (8, 6, 5), {}, dict(kind="density", amp=0.01)),
+    ("curv_all", (10, 7, 6), {}, dict(kind="all", amp=0.05, distort=0.15, seed=3)),
+    ("curv_viscous", (7, 9, 6), dict(rmult=3000.0), dict(kind="all", amp=0.03, distort=0.1, seed=5)),
+    ("limited_inter3", (9, 6, 7), dict(inter=(3, 2, 1)), dict(kind="all", amp=0.08, distort=0.1, seed=7)),
+    ("first_order", (6, 8, 5), dict(inter=(55, 55, 55), cfl=2.0), dict(kind="all", amp=0.05, seed=9)),
+    ("stretched_j", (12, 10, 4), dict(cfl=10.0), dict(kind="velocity", amp=0.02, jratio=1.3, seed=11)),
+]
+
+
+def make_case(dims, pover, ckw):
+    prm = default_params(**pover)
+    blk = make_block(*dims, prm=prm, **ckw)
+    return prm, blk
+
+
+def assert_fields_equal(a, b, names, interior=True, rtol=0.0, what=""):
+    for n in names:
+        x = a.interior(n) if interior else a[n]
+        y = b.interior(n) if interior else b[n]
+        if rtol == 0.0:
+            bad = np.count_nonzero(x != y)
+            assert bad == 0, f"{what} {n}: {bad} values differ, max |diff| = {np.abs(x - y).max():.3e}"
+        else:
+            scale = max(np.abs(y).max(), 1e-300)
+            err = np.abs(x - y).max() / scale
+            assert err <= rtol, f"{what} {n}: rel err {err:.3e} > {rtol}"
