@@ -910,3 +910,76 @@ void ffo_update(const ff_params *p, ff_block *b)
         pd[n] = pd[n] - prelim;
     }
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Entry points of single reference routines, for the closed-form checks of the oracle   */
+/* ------------------------------------------------------------------------------------ */
+
+/* FLUCYC face flux (FLUXDO + FLUXKE) of direction dir at cell offset L */
+void ffo_face_flux(const ff_params *p, const ff_block *b, long L, int dir, double *out)
+{
+    dims_t d = mkdims(b);
+    long S = dir == 0 ? 1 : (dir == 1 ? d.is : d.il);
+    const double *an = F(b, FF_A1 + 4 * dir), *anx = F(b, FF_A1X + 4 * dir),
+                 *any = F(b, FF_A1Y + 4 * dir), *anz = F(b, FF_A1Z + 4 * dir);
+    double fm, fn, fw, fe;
+    visc_face(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &fe);
+    flux7 r = roe_face(p, b, L, S, p->inter[dir], an, anx, any, anz, dir == 2);
+    out[0] = r.f[0] + 0.;
+    out[1] = r.f[1] + fm;
+    out[2] = r.f[2] + fn;
+    out[3] = r.f[3] + fw;
+    out[4] = r.f[4] + (0. + fe);
+}
+
+/* XXTRAP on a 1-D array q at face L with stride 1 */
+void ffo_xxtrap(const double *q, long L, int intem, double *qr, double *ql)
+{
+    xxtrap(q, L, 1, intem, qr, ql);
+}
+
+void ffo_surfvects(const double *a1, double *a2, double *a3)
+{
+    surfvects(a1[0], a1[1], a1[2], &a2[0], &a2[1], &a2[2], &a3[0], &a3[1], &a3[2]);
+}
+
+/* TIME3 alone (writes DTL on interior cells) */
+void ffo_time3(const ff_params *p, ff_block *b, double cfl) { time3(p, b, cfl); }
+
+/* COPR alone on interior cells (imps3.f:891-932) */
+void ffo_copr(const ff_params *p, ff_block *b)
+{
+    (void)p;
+    dims_t d = mkdims(b);
+    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    const double *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W), *ro = F(b, FF_RO),
+                 *e = F(b, FF_E), *P = F(b, FF_P), *c = F(b, FF_C), *drdp = F(b, FF_DRDP),
+                 *drdh = F(b, FF_DRDH);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                double vel2 = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]);
+                double beta = .5 * vel2;
+                double ypro = 1. / ro[n];
+                double dri = D[0][n], dmi = D[1][n], dni = D[2][n], dwi = D[3][n], dre = D[4][n];
+                double dkin = u[n] * dmi + v[n] * dni + w[n] * dwi;
+                D[1][n] = ypro * (-u[n] * dri + dmi);
+                D[2][n] = ypro * (-v[n] * dri + dni);
+                D[3][n] = ypro * (-w[n] * dri + dwi);
+                double hi = ypro * (e[n] + P[n]);
+                double dei = dre - (hi - 2 * beta) * dri - dkin;
+                double dpi = (c[n] * c[n]) * (dri - drdh[n] * ypro * dei);
+                double dhi = (dri - drdp[n] * dpi) / drdh[n];
+                D[0][n] = dhi;
+                D[4][n] = dpi;
+            }
+}
+
+/* PREDGS + DIAGGS + CORRGS on the current increments (DTL given) */
+void ffo_lusgs(const ff_params *p, ff_block *b)
+{
+    predgs(p, b);
+    diaggs(p, b);
+    corrgs(p, b);
+}

@@ -152,7 +152,8 @@ subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   real(c_double) :: r(67)
   integer :: maw, maxw, imax, jmax, kmax
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
-  maw = ntot
+  ! FLUXKE's slab work arrays hold ISTRID*(JMAX+2) entries (base3c.f:4812-4824)
+  maw = (imax + 4)*(jmax + 4)
   maxw = 27*maw
   allocate(fl(ntot,7), zzz(maxw), rksi(ntot), hat(ntot,4), zero(ntot), dfi(ntot))
   rksi = 0.; zero = 0.; zzz = 0.; dfi = 0.
@@ -184,7 +185,8 @@ subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
   integer :: maw, maxw, imax, jmax, kmax, idm1, idm2, idm3
   real(c_double) :: cfm, theta
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
-  maw = ntot
+  ! the LU-SGS slab work arrays hold JMAX*ISTRID entries (imps4.f:96-106)
+  maw = (imax + 4)*(jmax + 4)
   maxw = 18*maw
   allocate(zzz(maxw), rksi(ntot), zero(ntot), abv(ntot))
   zzz = 0.; rksi = 0.; zero = 0.; abv = 0.
