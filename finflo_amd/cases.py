@@ -136,6 +136,10 @@ def fill_state(blk, prm, seed=0, amp=0.01, mach_dir=(1.0, 0.0, 0.0), kind="densi
         blk["RK"] = rk * (1.0 + amp * rng.uniform(-1, 1, n))
         blk["REPS"] = reps * (1.0 + amp * rng.uniform(-1, 1, n))
     blk["P"] = blk["PDIFF"] + prm.frspre
+    # laminar eddy-viscosity ratio and eddy viscosity on every cell (the boundary routines
+    # own the ghost values; TURBDI writes the interior)
+    blk["EPS2"] = 1.0
+    blk["VIST"] = 0.0
     return blk
 
 

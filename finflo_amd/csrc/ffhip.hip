@@ -74,10 +74,14 @@ __global__ void __launch_bounds__(256) k_state(Prm p, Blk b)
     FLD(b, FF_RW)[n] = ro * w;
 }
 
+// ADJUIN (res3c.f:9447-9467): interior cells only; ghost values belong to the caller
 __global__ void __launch_bounds__(256) k_turb_lam(Blk b)
 {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= b.ntot) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    int n = cid(b, i, j, k);
     FLD(b, FF_EPS2)[n] = 1.;
     FLD(b, FF_VIST)[n] = 0.;
 }
@@ -784,7 +788,7 @@ int ff_turb(ff_ctx *c, int level, int block)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
-    return launch(c, "k_turb_lam", k_turb_lam, all_grid(r->dev), dim3(256), r->dev);
+    return launch(c, "k_turb_lam", k_turb_lam, cells_grid(r->dev, 256), dim3(256), r->dev);
 }
 
 static int residual_impl(ff_ctx *c, BlockRec *r, int level, double *resi)

@@ -103,15 +103,19 @@ void ffo_state(const ff_params *p, ff_block *b)
     }
 }
 
-/* TURBDI for ITURB = 0: ADJUIN sets EPS2 = 1 and VIST = 0 (ns3c.f:2987-2991) */
+/* TURBDI for ITURB = 0: ADJUIN (res3c.f:9447-9467) sets EPS2 = 1 and VIST = 0 on the
+ * interior cells (ns3c.f:2987-2991); ghost values are the caller's boundary values. */
 void ffo_turb(const ff_params *p, ff_block *b)
 {
     (void)p;
     dims_t d = mkdims(b);
-    for (long n = 0; n < d.ntot; n++) {
-        F(b, FF_EPS2)[n] = 1.;
-        F(b, FF_VIST)[n] = 0.;
-    }
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                F(b, FF_EPS2)[n] = 1.;
+                F(b, FF_VIST)[n] = 0.;
+            }
 }
 
 /* ------------------------------------------------------------------------------------ */

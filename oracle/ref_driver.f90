@@ -74,8 +74,9 @@ subroutine ref_turb(f, ntot, prm, ipr) bind(C, name="ref_turb")
   integer(c_int), value :: ntot
   real(c_double) :: f(ntot, *), prm(*)
   integer(c_int) :: ipr(*)
-  f(1:ntot, IEPS2) = 1.
-  f(1:ntot, IVIST) = 0.
+  ! TURBDI ITURB = 0 (ns3c.f:2987-2991)
+  call ADJUIN(f(1,IEPS2), 1._c_double, ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), 2, 2, 2)
+  call ADJUIN(f(1,IVIST), 0._c_double, ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), 2, 2, 2)
 end subroutine ref_turb
 
 ! One direction of RESIDU: the FLUCYC sequence for a block with no boundary patches
