@@ -3,8 +3,8 @@
 
 A step is one solver iteration of the finest-level CYCLES sequence over the block (STATIM,
 TURBDI, RESIDI, IMPSDI with LU-SGS, FIN + limiters), with every array resident in HBM.
-Prints one JSON line; see DESIGN.md for the workload, the roofline accounting and the
-CPU baseline.
+Prints one JSON line with the throughput, the roofline of the dominant kernel (algorithmic
+bytes per cell over its event-timed launch duration) and a CPU baseline timed on the host.
 """
 import argparse
 import json
@@ -24,16 +24,27 @@ IMAX, JMAX, KMAX = 192, 96, 96
 MACH, REYNOLDS = 0.2, 5.0e6
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic (compulsory, unique) FP64 bytes per interior cell of each kernel: the fields
-# each kernel must read and write once ("Roofline accounting").
+# Algorithmic (compulsory, unique) FP64 bytes per cell of each kernel: every field the
+# kernel must read or write, once (neighbour values come from the same arrays).
 KERNEL_BYTES_PER_CELL = {
-    "k_residual": 8 * (8 + 7 + 12 + 5),       # flux state, viscous state, 3x4 face geometry, D out
-    "k_pred_plane": 8 * (5 + 9 + 12 + 5),     # D in, state, lower-face geometry, D out
+    # flux state (RO U V W P C DRDP DRDH), viscous state (VIS EPS2 VIST CP CH TEMP), VOL,
+    # 3 owned faces x (A, nx, ny, nz), D out (5)
+    "k_residual_tile": 8 * (8 + 6 + 1 + 12 + 5),
+    "k_residual": 8 * (8 + 6 + 1 + 12 + 5),
+    # D in (5), state (U V W C RO VOL VIS VIST DTL), 3 owned faces x 4, D out (5)
+    "k_pred_tiles": 8 * (5 + 9 + 12 + 5),
+    "k_pred_plane": 8 * (5 + 9 + 12 + 5),
+    # the same plus CP (DRO/CP at the end of IMPS)
+    "k_corr_tiles": 8 * (5 + 10 + 12 + 5),
     "k_corr_plane": 8 * (5 + 9 + 12 + 5),
+    # 3 faces x 4, VOL, 11 state fields, D in, D and DTL out
     "k_time3_copr": 8 * (12 + 1 + 11 + 5 + 6),
-    "k_state": 8 * (5 + 12),                  # every cell incl. ghosts
+    # every cell incl. ghosts: 5 solved variables in, 12 derived fields out
+    "k_state": 8 * (5 + 12),
 }
-# whole iteration, fused, laminar ideal gas: 704 B per cell-update
+ALL_CELL_KERNELS = {"k_state"}          # launched over ghosts too
+# whole iteration, laminar ideal gas, every field touched once per phase (residual; backward
+# sweep; forward sweep + update): 88 doubles = 704 B per cell-update
 ITER_BYTES_PER_CELL = 704
 
 
@@ -92,7 +103,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design note in DESIGN.md)", file=sys.stderr)
+              "(blocks would shard across GPUs; this build measures one GPU)", file=sys.stderr)
         sys.exit(2)
 
     from finflo_amd.ffhip import Context
@@ -127,22 +138,25 @@ def main():
     ctx.profile(False)
 
     kstats = {}
-    for name in ("k_residual", "k_pred_plane", "k_corr_plane", "k_time3_copr", "k_state",
-                 "k_limits", "k_fin", "k_zero", "k_save_old", "k_divcp", "k_turb_lam",
-                 "k_resl2_part", "k_resl2_final"):
+    for name in ("k_residual_tile", "k_residual", "k_pred_tiles", "k_corr_tiles",
+                 "k_pred_plane", "k_corr_plane", "k_time3_copr", "k_state", "k_limits",
+                 "k_fin", "k_zero", "k_save_old", "k_divcp", "k_turb_lam", "k_resl2_part",
+                 "k_resl2_final"):
         n, ms = ctx.kernel_stats(name)
         if n:
-            kstats[name] = {"launches": n, "ms_total": ms}
-    # dominant single-launch kernel of the sweep-free part, and the sweep kernels per step
-    dom = max(("k_residual", "k_time3_copr", "k_state"), key=lambda k: kstats.get(k, {}).get("ms_total", 0))
+            kstats[name] = {"launches": n, "ms_total": ms, "ms_per_step": ms / args.steps}
+    # dominant kernel: the largest share of the step among those with a byte model
+    dom = max((k for k in kstats if k in KERNEL_BYTES_PER_CELL), key=lambda k: kstats[k]["ms_total"])
     ks = kstats[dom]
     avg_ms = ks["ms_total"] / ks["launches"]
-    cells_per_launch = ncell if dom != "k_state" else blk.ntot
+    cells_per_launch = blk.ntot if dom in ALL_CELL_KERNELS else ncell
     achieved = KERNEL_BYTES_PER_CELL[dom] * cells_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "avg_launch_ms": avg_ms}
-    sweep_ms = sum(kstats.get(k, {}).get("ms_total", 0.0) for k in ("k_pred_plane", "k_corr_plane")) / args.steps
+                "avg_launch_ms": avg_ms,
+                "bytes_per_cell": KERNEL_BYTES_PER_CELL[dom], "cells_per_launch": cells_per_launch}
+    sweep_ms = sum(kstats.get(k, {}).get("ms_total", 0.0)
+                   for k in ("k_pred_plane", "k_corr_plane", "k_pred_tiles", "k_corr_tiles")) / args.steps
     iter_bw = ITER_BYTES_PER_CELL * ncell / (ms_per_step * 1e-3) / 1e9
 
     cpu = None

@@ -144,7 +144,8 @@ def fill_state(blk, prm, seed=0, amp=0.01, mach_dir=(1.0, 0.0, 0.0), kind="densi
 
 
 def make_block(imax, jmax, kmax, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
-               distort=0.0, seed=0, amp=0.01, kind="density", turb=None):
+               distort=0.0, seed=0, amp=0.01, kind="density", turb=None,
+               mach_dir=(1.0, 0.0, 0.0)):
     """A synthetic block with geometry and state (parity-test cases and workloads)."""
     prm = prm or default_params()
     blk = Block(imax, jmax, kmax)
@@ -152,7 +153,7 @@ def make_block(imax, jmax, kmax, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
     yj = stretched(jmax, lengths[1], ratio=jratio)
     zk = stretched(kmax, lengths[2])
     fill_geometry(blk, node_grid(xi, yj, zk, distort=distort, seed=seed + 1000))
-    fill_state(blk, prm, seed=seed, amp=amp, kind=kind, turb=turb)
+    fill_state(blk, prm, seed=seed, amp=amp, kind=kind, turb=turb, mach_dir=mach_dir)
     return blk
 
 

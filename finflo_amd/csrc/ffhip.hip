@@ -91,6 +91,7 @@ __global__ void __launch_bounds__(256) k_turb_lam(Blk b)
 // in the reference's order (SWEEP after the k, i and j directions).  Both faces of the
 // cell are evaluated by the same code as the neighbour evaluates them, so the shared face
 // values agree bit for bit.
+template <int IFLUX>
 __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -101,27 +102,27 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
     double d[5] = {0., 0., 0., 0., 0.};
     {   // k-direction (res3c.f:270, SWEEP res3c.f:330)
         const int S = b.il;
-        Flux5 lo = face_flux<true>(p, b, n, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
+        Flux5 lo = face_flux<true, IFLUX>(p, b, n, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
                                    FLD(b, FF_A3Y), FLD(b, FF_A3Z));
-        Flux5 hi = face_flux<true>(p, b, n + S, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
+        Flux5 hi = face_flux<true, IFLUX>(p, b, n + S, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
                                    FLD(b, FF_A3Y), FLD(b, FF_A3Z));
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
     }
     {   // i-direction (res3c.f:402, SWEEP res3c.f:448)
         const int S = 1;
-        Flux5 lo = face_flux<false>(p, b, n, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
+        Flux5 lo = face_flux<false, IFLUX>(p, b, n, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
                                     FLD(b, FF_A1Y), FLD(b, FF_A1Z));
-        Flux5 hi = face_flux<false>(p, b, n + S, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
+        Flux5 hi = face_flux<false, IFLUX>(p, b, n + S, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
                                     FLD(b, FF_A1Y), FLD(b, FF_A1Z));
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
     }
     {   // j-direction, last: divide by VOL (res3c.f:515, SWEEP res3c.f:559)
         const int S = b.is;
-        Flux5 lo = face_flux<false>(p, b, n, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
+        Flux5 lo = face_flux<false, IFLUX>(p, b, n, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
                                     FLD(b, FF_A2Y), FLD(b, FF_A2Z));
-        Flux5 hi = face_flux<false>(p, b, n + S, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
+        Flux5 hi = face_flux<false, IFLUX>(p, b, n + S, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
                                     FLD(b, FF_A2Y), FLD(b, FF_A2Z));
         double dt1 = 1. / FLD(b, FF_VOL)[n];
 #pragma unroll
@@ -132,6 +133,72 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
     FLD(b, FF_DN)[n] = d[2];
     FLD(b, FF_DW)[n] = d[3];
     FLD(b, FF_DE)[n] = d[4];
+}
+
+// Residual with every face flux formed once.  A workgroup is a tile of RI x RJ cells in
+// (I,J) that marches over RK consecutive K-planes.  At each plane a lane forms the K-face
+// above its cell (kept in registers: it is the lower K-face of the next plane), its lower
+// I-face and its lower J-face; the tile's upper I- and J-faces are formed by two waves in an
+// extra phase.  I- and J-faces meet their two cells through LDS.  A face flux is the same
+// function of the same operands whichever cell forms it, so the differences are those of
+// k_residual bit for bit.
+constexpr int RI = 64, RJ = 4, RKC = 8;
+static_assert(RJ >= 2 && RJ <= RI, "the tile's upper I-faces are formed by the lanes of row 1");
+
+template <int IFLUX>
+__global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
+{
+    __shared__ double fi[5][RJ][RI + 1];
+    __shared__ double fj[5][RJ + 1][RI];
+    const int ti = threadIdx.x % RI, tj = threadIdx.x / RI;
+    const int i0 = blockIdx.x * RI + 1, j0 = blockIdx.y * RJ + 1, k0 = blockIdx.z * RKC + 1;
+    const int i = i0 + ti, j = j0 + tj;
+    const bool cell = i <= b.imax && j <= b.jmax;
+    const int il = min(RI, b.imax - i0 + 1), jl = min(RJ, b.jmax - j0 + 1);   // cells in the tile
+    const int k1 = min(k0 + RKC - 1, b.kmax);
+    const double *a3 = FLD(b, FF_A3), *a3x = FLD(b, FF_A3X), *a3y = FLD(b, FF_A3Y), *a3z = FLD(b, FF_A3Z);
+    const double *a1 = FLD(b, FF_A1), *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z);
+    const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
+    Flux5 klo;
+    if (cell) klo = face_flux<true, IFLUX>(p, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z);
+    for (int k = k0; k <= k1; k++) {
+        const int n = cid(b, i, j, k);
+        Flux5 khi;
+        if (cell) {
+            khi = face_flux<true, IFLUX>(p, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z);
+            Flux5 f = face_flux<false, IFLUX>(p, b, n, 1, p.inter[0], a1, a1x, a1y, a1z);
+#pragma unroll
+            for (int q = 0; q < 5; q++) fi[q][tj][ti] = f.f[q];
+            f = face_flux<false, IFLUX>(p, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z);
+#pragma unroll
+            for (int q = 0; q < 5; q++) fj[q][tj][ti] = f.f[q];
+        }
+        // upper faces of the tile: the J-face above row jl-1 (wave 0), the I-face after
+        // column il-1 of every row (wave 1)
+        if (tj == 0 && ti < il) {
+            Flux5 f = face_flux<false, IFLUX>(p, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z);
+#pragma unroll
+            for (int q = 0; q < 5; q++) fj[q][jl][ti] = f.f[q];
+        } else if (tj == 1 && ti < jl) {
+            Flux5 f = face_flux<false, IFLUX>(p, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z);
+#pragma unroll
+            for (int q = 0; q < 5; q++) fi[q][ti][il] = f.f[q];
+        }
+        __syncthreads();
+        if (cell) {
+            double dt1 = 1. / FLD(b, FF_VOL)[n];
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                double d = 0.;
+                d = d - (khi.f[q] - klo.f[q]);                          // SWEEP, K (res3c.f:330)
+                d = d - (fi[q][tj][ti + 1] - fi[q][tj][ti]);             // SWEEP, I (res3c.f:448)
+                d = (d - (fj[q][tj + 1][ti] - fj[q][tj][ti])) * dt1;     // SWEEP, J, /VOL (res3c.f:559)
+                FLD(b, FF_DRO + q)[n] = d;
+            }
+            klo = khi;
+        }
+        __syncthreads();
+    }
 }
 
 // Zero a field on every cell (ZEROZZ res3c.f:9330)
@@ -420,6 +487,427 @@ __global__ void __launch_bounds__(128) k_corr_plane(Prm p, Blk b, int m, int jlo
     FLD(b, FF_DE)[n] = cd[4] * scinp;
 }
 
+// ------------------------------------------------------------------------------------
+// LU-SGS as a tile wavefront (the default path).
+//
+// A workgroup is one wave that owns an 8x8 tile of (J,K) columns and marches along I.
+// Lane (jj,kk) runs one step behind its upstream lanes, so at every step the wave's cells
+// lie on one hyperplane of the tile and the three contributions a cell needs were produced
+// one step earlier: the I-contribution by the lane itself (a register), the J- and
+// K-contributions by the neighbouring lanes (LDS) or, across a tile face, by the
+// neighbouring tile (halo slots in HBM).  Each cell forms its own split-eigenvalue
+// coefficients once and sends SYM1I(XI*SYI(dW)) downstream, as PREDGS/CORRGS do with their
+// line buffers; the arithmetic per cell is the reference's, so results are bit for bit
+// those of the K-plane/diagonal order.
+//
+// Halo slots hold a signalling-NaN pattern when empty.  The producer stores a value with a
+// relaxed device-scope atomic store; the consumer polls it until it is filled, then empties
+// it again for the next sweep.  Arithmetic never produces a signalling NaN, so the slots
+// need no fences.  Workgroups take tiles through an atomic ticket in a topological order,
+// so a tile only ever waits on tiles that have already started.  Waits are bounded: after
+// SPIN_LIMIT polls (seconds) a wait gives up and raises the error word the host checks.
+// ------------------------------------------------------------------------------------
+constexpr int TJ = 8, TK = 8, TW = TJ * TK;
+constexpr unsigned long long HALO_EMPTY = 0x7FF4DEADBEEF0001ULL;
+constexpr int SPIN_LIMIT = 1 << 22;
+
+struct Sweep {
+    int ntj, ntk;                   // tiles along J and K
+    unsigned long long *ticket;     // monotonic; ticket % (ntj*ntk) orders the tiles
+    unsigned int *err;              // non-zero after a halo wait gave up
+    unsigned long long *hj, *hk;    // halo slots: J-faces (ntj+1, KMAX, IMAX, 5), K-faces
+};
+
+__device__ __forceinline__ unsigned long long *hj_slot(const Sweep &s, const Blk &b, int f, int i,
+                                                       int k)
+{
+    return s.hj + 5 * (((size_t)f * b.kmax + (k - 1)) * b.imax + (i - 1));
+}
+
+__device__ __forceinline__ unsigned long long *hk_slot(const Sweep &s, const Blk &b, int f, int i,
+                                                       int j)
+{
+    return s.hk + 5 * (((size_t)f * b.jmax + (j - 1)) * b.imax + (i - 1));
+}
+
+__device__ __forceinline__ void halo_put(unsigned long long *slot, const double v[5])
+{
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        __hip_atomic_store(slot + q, (unsigned long long)__double_as_longlong(v[q]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void halo_issue(unsigned long long *slot, unsigned long long h[5])
+{
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        h[q] = __hip_atomic_load(slot + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Complete a halo read begun by halo_issue: poll empty slots, then empty them.
+__device__ __forceinline__ void halo_finish(unsigned long long *slot, unsigned long long h[5],
+                                            unsigned int *err, double v[5])
+{
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        unsigned long long x = h[q];
+        int spins = 0;
+        while (x == HALO_EMPTY) {
+            if ((++spins & 255) == 0) {
+                if (spins > SPIN_LIMIT) {
+                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    x = 0;
+                    break;
+                }
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    x = 0;
+                    break;
+                }
+            }
+            __builtin_amdgcn_s_sleep(1);
+            x = __hip_atomic_load(slot + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        v[q] = __longlong_as_double((long long)x);
+        __hip_atomic_store(slot + q, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// The wave's LDS exchange point between two steps (one wave per workgroup: no s_barrier
+// is needed, only LDS ordering).
+__device__ __forceinline__ void wave_step_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ int tile_ticket(const Sweep &s)
+{
+    unsigned long long t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(s.ticket, 1ULL);
+    t = __shfl(t, 0);
+    return (int)(t % (unsigned long long)(s.ntj * s.ntk));
+}
+
+// Values cell n of the backward sweep reads (PREDGS loop 1100 and the diagonal loop)
+struct PredRaw {
+    double d[5], u, v, w, c, ro, vol, vis, vist, dtl;
+    double vol1, ro1, vis1, vist1, vol2, ro2, vis2, vist2, vol3, ro3, vis3, vist3;  // n-1, n-IS, n-IL
+    double a1, a2, a3, n1x, n1y, n1z, n2x, n2y, n2z, n3x, n3y, n3z;               // lower faces
+    double a1u, a2u, a3u;                                                         // upper areas
+};
+
+__device__ __forceinline__ void pred_load(const Blk &b, int n, PredRaw &r)
+{
+    const int IS = b.is, IL = b.il;
+#pragma unroll
+    for (int q = 0; q < 5; q++) r.d[q] = FLD(b, FF_DRO + q)[n];
+    r.u = FLD(b, FF_U)[n]; r.v = FLD(b, FF_V)[n]; r.w = FLD(b, FF_W)[n];
+    r.c = FLD(b, FF_C)[n]; r.ro = FLD(b, FF_RO)[n]; r.vol = FLD(b, FF_VOL)[n];
+    r.vis = FLD(b, FF_VIS)[n]; r.vist = FLD(b, FF_VIST)[n]; r.dtl = FLD(b, FF_DTL)[n];
+    r.vol1 = FLD(b, FF_VOL)[n - 1]; r.ro1 = FLD(b, FF_RO)[n - 1];
+    r.vis1 = FLD(b, FF_VIS)[n - 1]; r.vist1 = FLD(b, FF_VIST)[n - 1];
+    r.vol2 = FLD(b, FF_VOL)[n - IS]; r.ro2 = FLD(b, FF_RO)[n - IS];
+    r.vis2 = FLD(b, FF_VIS)[n - IS]; r.vist2 = FLD(b, FF_VIST)[n - IS];
+    r.vol3 = FLD(b, FF_VOL)[n - IL]; r.ro3 = FLD(b, FF_RO)[n - IL];
+    r.vis3 = FLD(b, FF_VIS)[n - IL]; r.vist3 = FLD(b, FF_VIST)[n - IL];
+    r.a1 = FLD(b, FF_A1)[n]; r.a2 = FLD(b, FF_A2)[n]; r.a3 = FLD(b, FF_A3)[n];
+    r.n1x = FLD(b, FF_A1X)[n]; r.n1y = FLD(b, FF_A1Y)[n]; r.n1z = FLD(b, FF_A1Z)[n];
+    r.n2x = FLD(b, FF_A2X)[n]; r.n2y = FLD(b, FF_A2Y)[n]; r.n2z = FLD(b, FF_A2Z)[n];
+    r.n3x = FLD(b, FF_A3X)[n]; r.n3y = FLD(b, FF_A3Y)[n]; r.n3z = FLD(b, FF_A3Z)[n];
+    r.a1u = FLD(b, FF_A1)[n + 1]; r.a2u = FLD(b, FF_A2)[n + IS]; r.a3u = FLD(b, FF_A3)[n + IL];
+}
+
+// Recurrence-independent part of a cell's LU-SGS update
+struct SweepCell {
+    double d[5], dt1, vol, ro, roc, pc2, proc, scinv, scinp;
+    double wi, wj, wk;             // areas weighting the incoming I-, J-, K-contributions
+    FaceBasis fi, fj, fk;          // faces the outgoing contributions cross
+    Coef ci, cj, ck;
+    double cp;                     // CORRGS only: DRO/CP at the end of IMPS
+};
+
+__device__ __forceinline__ void cell_scalars(SweepCell &s, double ro, double c)
+{
+    s.ro = ro;
+    s.roc = ro * c;                // SYI imps4.f:31
+    double ypro = 1. / ro;         // SYM1I imps4.f:57-59
+    s.pc2 = .5 * ypro;
+    s.proc = s.pc2 / c;
+}
+
+__device__ __forceinline__ void pred_prep(const Prm &p, const PredRaw &r, int i, int j, int k,
+                                          SweepCell &s)
+{
+    const double cdiff = 2.;
+    double ffi = i == 1 ? 4. : 2., ffj = j == 1 ? 4. : 2., ffk = k == 1 ? 4. : 2.;
+    s.ci = coef_vals<true>(r.n1x, r.n1y, r.n1z, r.u, r.v, r.w, r.c, r.a1, r.vol, r.vol1, r.ro,
+                           r.ro1, r.vis, r.vist, r.vis1, r.vist1, ffi * cdiff);
+    s.cj = coef_vals<true>(r.n2x, r.n2y, r.n2z, r.u, r.v, r.w, r.c, r.a2, r.vol, r.vol2, r.ro,
+                           r.ro2, r.vis, r.vist, r.vis2, r.vist2, cdiff * ffj);
+    s.ck = coef_vals<true>(r.n3x, r.n3y, r.n3z, r.u, r.v, r.w, r.c, r.a3, r.vol, r.vol3, r.ro,
+                           r.ro3, r.vis, r.vist, r.vis3, r.vist3, ffk * cdiff);
+    s.dt1 = p.theta * r.dtl;
+    s.vol = r.vol;
+    s.scinv = 1. / (r.vol + s.dt1 * (s.ci.xa * r.a1 + s.cj.xa * r.a2 + s.ck.xa * r.a3) + r.vol * 0.);
+    s.scinp = 1. / (r.vol + s.dt1 * (s.ci.rmax * r.a1 + s.cj.rmax * r.a2 + s.ck.rmax * r.a3) + r.vol * 0.);
+#pragma unroll
+    for (int q = 0; q < 5; q++) s.d[q] = r.d[q];
+    cell_scalars(s, r.ro, r.c);
+    s.wi = r.a1u; s.wj = r.a2u; s.wk = r.a3u;
+    s.fi = face_basis(r.n1x, r.n1y, r.n1z);
+    s.fj = face_basis(r.n2x, r.n2y, r.n2z);
+    s.fk = face_basis(r.n3x, r.n3y, r.n3z);
+}
+
+// Values cell n of the forward sweep reads (DIAGGS, CORRGS loop 1100 and the diagonals)
+struct CorrRaw {
+    double d[5], u, v, w, c, ro, vol, vis, vist, dtl, cp;
+    double vol1, ro1, vis1, vist1, vol2, ro2, vis2, vist2, vol3, ro3, vis3, vist3;  // n+1, n+IS, n+IL
+    double a1, a2, a3, n1x, n1y, n1z, n2x, n2y, n2z, n3x, n3y, n3z;               // lower faces
+    double a1u, a2u, a3u, u1x, u1y, u1z, u2x, u2y, u2z, u3x, u3y, u3z;            // upper faces
+};
+
+__device__ __forceinline__ void corr_load(const Blk &b, int n, CorrRaw &r)
+{
+    const int IS = b.is, IL = b.il;
+#pragma unroll
+    for (int q = 0; q < 5; q++) r.d[q] = FLD(b, FF_DRO + q)[n];
+    r.u = FLD(b, FF_U)[n]; r.v = FLD(b, FF_V)[n]; r.w = FLD(b, FF_W)[n];
+    r.c = FLD(b, FF_C)[n]; r.ro = FLD(b, FF_RO)[n]; r.vol = FLD(b, FF_VOL)[n];
+    r.vis = FLD(b, FF_VIS)[n]; r.vist = FLD(b, FF_VIST)[n]; r.dtl = FLD(b, FF_DTL)[n];
+    r.cp = FLD(b, FF_CP)[n];
+    r.vol1 = FLD(b, FF_VOL)[n + 1]; r.ro1 = FLD(b, FF_RO)[n + 1];
+    r.vis1 = FLD(b, FF_VIS)[n + 1]; r.vist1 = FLD(b, FF_VIST)[n + 1];
+    r.vol2 = FLD(b, FF_VOL)[n + IS]; r.ro2 = FLD(b, FF_RO)[n + IS];
+    r.vis2 = FLD(b, FF_VIS)[n + IS]; r.vist2 = FLD(b, FF_VIST)[n + IS];
+    r.vol3 = FLD(b, FF_VOL)[n + IL]; r.ro3 = FLD(b, FF_RO)[n + IL];
+    r.vis3 = FLD(b, FF_VIS)[n + IL]; r.vist3 = FLD(b, FF_VIST)[n + IL];
+    r.a1 = FLD(b, FF_A1)[n]; r.a2 = FLD(b, FF_A2)[n]; r.a3 = FLD(b, FF_A3)[n];
+    r.n1x = FLD(b, FF_A1X)[n]; r.n1y = FLD(b, FF_A1Y)[n]; r.n1z = FLD(b, FF_A1Z)[n];
+    r.n2x = FLD(b, FF_A2X)[n]; r.n2y = FLD(b, FF_A2Y)[n]; r.n2z = FLD(b, FF_A2Z)[n];
+    r.n3x = FLD(b, FF_A3X)[n]; r.n3y = FLD(b, FF_A3Y)[n]; r.n3z = FLD(b, FF_A3Z)[n];
+    r.a1u = FLD(b, FF_A1)[n + 1]; r.a2u = FLD(b, FF_A2)[n + IS]; r.a3u = FLD(b, FF_A3)[n + IL];
+    r.u1x = FLD(b, FF_A1X)[n + 1]; r.u1y = FLD(b, FF_A1Y)[n + 1]; r.u1z = FLD(b, FF_A1Z)[n + 1];
+    r.u2x = FLD(b, FF_A2X)[n + IS]; r.u2y = FLD(b, FF_A2Y)[n + IS]; r.u2z = FLD(b, FF_A2Z)[n + IS];
+    r.u3x = FLD(b, FF_A3X)[n + IL]; r.u3y = FLD(b, FF_A3Y)[n + IL]; r.u3z = FLD(b, FF_A3Z)[n + IL];
+}
+
+__device__ __forceinline__ void corr_prep(const Prm &p, const CorrRaw &r, SweepCell &s)
+{
+    const double cdiff = 2.;
+    // DIAGGS imps4.f:621-659
+    double ucon;
+    ucon = r.u1x * r.u + r.u1y * r.v + r.u1z * r.w - 0.;
+    double r1n = 1.00 * (fabs(ucon) + r.c);
+    ucon = r.n1x * r.u + r.n1y * r.v + r.n1z * r.w - 0.;
+    double r1p = 1.00 * (fabs(ucon) + r.c);
+    ucon = r.u2x * r.u + r.u2y * r.v + r.u2z * r.w - 0.;
+    double r2n = 1.00 * (fabs(ucon) + r.c);
+    ucon = r.n2x * r.u + r.n2y * r.v + r.n2z * r.w - 0.;
+    double r2p = 1.00 * (fabs(ucon) + r.c);
+    ucon = r.u3x * r.u + r.u3y * r.v + r.u3z * r.w - 0.;
+    double r3n = 1.00 * (fabs(ucon) + r.c);
+    ucon = r.n3x * r.u + r.n3y * r.v + r.n3z * r.w - 0.;
+    double r3p = 1.00 * (fabs(ucon) + r.c);
+    double dtv = p.theta * r.dtl / r.vol;
+    double sc = .5 * (r.a1 * r1p + r.a1u * r1n + r.a2 * r2p + r.a2u * r2n + r.a3 * r3p + r.a3u * r3n);
+    double fac = 1. + dtv * sc + 0.;
+#pragma unroll
+    for (int q = 0; q < 5; q++) s.d[q] = r.d[q] * fac;
+    // CORRGS own coefficients (upper faces, viscous area A(n))
+    s.ci = coef_vals<false>(r.u1x, r.u1y, r.u1z, r.u, r.v, r.w, r.c, r.a1, r.vol, r.vol1, r.ro,
+                            r.ro1, r.vis, r.vist, r.vis1, r.vist1, 2. * cdiff);
+    s.cj = coef_vals<false>(r.u2x, r.u2y, r.u2z, r.u, r.v, r.w, r.c, r.a2, r.vol, r.vol2, r.ro,
+                            r.ro2, r.vis, r.vist, r.vis2, r.vist2, cdiff * 2.);
+    s.ck = coef_vals<false>(r.u3x, r.u3y, r.u3z, r.u, r.v, r.w, r.c, r.a3, r.vol, r.vol3, r.ro,
+                            r.ro3, r.vis, r.vist, r.vis3, r.vist3, 2. * cdiff);
+    s.dt1 = p.theta * r.dtl;
+    s.vol = r.vol;
+    s.scinv = 1. / (r.vol + s.dt1 * (s.ci.xa * r.a1u + s.cj.xa * r.a2u + s.ck.xa * r.a3u) + r.vol * 0.);
+    s.scinp = 1. / (r.vol + s.dt1 * (s.ci.rmax * r.a1 + s.cj.rmax * r.a2 + s.ck.rmax * r.a3) + r.vol * 0.);
+    cell_scalars(s, r.ro, r.c);
+    s.wi = r.a1; s.wj = r.a2; s.wk = r.a3;
+    s.fi = face_basis(r.u1x, r.u1y, r.u1z);
+    s.fj = face_basis(r.u2x, r.u2y, r.u2z);
+    s.fk = face_basis(r.u3x, r.u3y, r.u3z);
+    s.cp = r.cp;
+}
+
+// Backward sweep PREDGS (imps4.f:345-598) as a tile wavefront; I, J, K descending.
+__global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
+{
+    __shared__ double xj[2][5][TW], xk[2][5][TW];
+    const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
+    const int t = tile_ticket(sw);
+    const int tj = sw.ntj - 1 - t % sw.ntj, tk = sw.ntk - 1 - t / sw.ntj;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
+    const bool col = j <= b.jmax && k <= b.kmax;
+    const int lag = (TJ - 1 - jj) + (TK - 1 - kk);
+    const int nsteps = b.imax + TJ + TK - 2;
+    double yi[5] = {0., 0., 0., 0., 0.};
+    PredRaw cur;
+    if (col) pred_load(b, cid(b, b.imax, j, k), cur);
+    for (int s = 0; s < nsteps; s++) {
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.imax) {
+            const int i = b.imax - q;
+            const int n = cid(b, i, j, k);
+            const bool hjin = jj == TJ - 1 && j < b.jmax, hkin = kk == TK - 1 && k < b.kmax;
+            unsigned long long hv[5], hw[5];
+            if (hjin) halo_issue(hj_slot(sw, b, tj + 1, i, k), hv);
+            if (hkin) halo_issue(hk_slot(sw, b, tk + 1, i, j), hw);
+            PredRaw nxt;
+            if (q + 1 < b.imax) pred_load(b, n - 1, nxt);
+            SweepCell sc;
+            pred_prep(p, cur, i, j, k, sc);
+            const int rb = (s + 1) & 1;
+            double bj[5], ckc[5];
+            if (j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) bj[r] = 0.;
+            } else if (!hjin) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) bj[r] = xj[rb][r][lane + 1];
+            } else {
+                halo_finish(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
+            }
+            if (k == b.kmax) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) ckc[r] = 0.;
+            } else if (!hkin) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) ckc[r] = xk[rb][r][lane + TJ];
+            } else {
+                halo_finish(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
+            }
+            // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
+            // contribution to a cell of the row J = JMAX
+            if (j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) yi[r] = 0.;
+            }
+            double dn[5];
+#pragma unroll
+            for (int r = 0; r < 5; r++) {
+                double cd = -sc.dt1 * sc.wk * ckc[r] + sc.vol * sc.d[r];
+                cd = cd - sc.dt1 * (sc.wi * yi[r] + sc.wj * bj[r]);
+                dn[r] = r < 4 ? cd * sc.scinv : cd * sc.scinp;
+            }
+#pragma unroll
+            for (int r = 0; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
+            double oj[5], ok[5];
+            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fi, sc.ci.x1, sc.ci.x2, sc.ci.x5, yi);
+            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fj, sc.cj.x1, sc.cj.x2, sc.cj.x5, oj);
+            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fk, sc.ck.x1, sc.ck.x2, sc.ck.x5, ok);
+            const int wb = s & 1;
+            if (jj > 0) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) xj[wb][r][lane] = oj[r];
+            } else if (j > 1) {
+                halo_put(hj_slot(sw, b, tj, i, k), oj);
+            }
+            if (kk > 0) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) xk[wb][r][lane] = ok[r];
+            } else if (k > 1) {
+                halo_put(hk_slot(sw, b, tk, i, j), ok);
+            }
+            if (q + 1 < b.imax) cur = nxt;
+        }
+        wave_step_sync();
+    }
+}
+
+// DIAGGS + forward sweep CORRGS (imps4.f:604-664, 76-339) as a tile wavefront, I, J, K
+// ascending, with DRO/CP of IMPS (imps3.f:483) applied to the final increment.
+__global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
+{
+    __shared__ double xj[2][5][TW], xk[2][5][TW];
+    const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
+    const int t = tile_ticket(sw);
+    const int tj = t % sw.ntj, tk = t / sw.ntj;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
+    const bool col = j <= b.jmax && k <= b.kmax;
+    const int lag = jj + kk;
+    const int nsteps = b.imax + TJ + TK - 2;
+    double yi[5] = {0., 0., 0., 0., 0.};
+    CorrRaw cur;
+    if (col) corr_load(b, cid(b, 1, j, k), cur);
+    for (int s = 0; s < nsteps; s++) {
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.imax) {
+            const int i = q + 1;
+            const int n = cid(b, i, j, k);
+            const bool hjin = jj == 0 && j > 1, hkin = kk == 0 && k > 1;
+            unsigned long long hv[5], hw[5];
+            if (hjin) halo_issue(hj_slot(sw, b, tj, i, k), hv);
+            if (hkin) halo_issue(hk_slot(sw, b, tk, i, j), hw);
+            CorrRaw nxt;
+            if (q + 1 < b.imax) corr_load(b, n + 1, nxt);
+            SweepCell sc;
+            corr_prep(p, cur, sc);
+            const int rb = (s + 1) & 1;
+            double bj[5], ckc[5];
+            if (j == 1) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) bj[r] = 0.;
+            } else if (!hjin) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) bj[r] = xj[rb][r][lane - 1];
+            } else {
+                halo_finish(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
+            }
+            if (k == 1) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) ckc[r] = 0.;
+            } else if (!hkin) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) ckc[r] = xk[rb][r][lane - TJ];
+            } else {
+                halo_finish(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
+            }
+            double dn[5];
+#pragma unroll
+            for (int r = 0; r < 5; r++) {
+                double cd = sc.dt1 * sc.wk * ckc[r] + sc.vol * sc.d[r];
+                cd = cd + sc.dt1 * (sc.wi * yi[r] + sc.wj * bj[r]);
+                dn[r] = r < 4 ? cd * sc.scinv : cd * sc.scinp;
+            }
+            FLD(b, FF_DRO)[n] = dn[0] / sc.cp;
+#pragma unroll
+            for (int r = 1; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
+            double oj[5], ok[5];
+            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fi, sc.ci.x1, sc.ci.x2, sc.ci.x5, yi);
+            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fj, sc.cj.x1, sc.cj.x2, sc.cj.x5, oj);
+            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fk, sc.ck.x1, sc.ck.x2, sc.ck.x5, ok);
+            const int wb = s & 1;
+            if (jj < TJ - 1) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) xj[wb][r][lane] = oj[r];
+            } else if (j < b.jmax) {
+                halo_put(hj_slot(sw, b, tj + 1, i, k), oj);
+            }
+            if (kk < TK - 1) {
+#pragma unroll
+                for (int r = 0; r < 5; r++) xk[wb][r][lane] = ok[r];
+            } else if (k < b.kmax) {
+                halo_put(hk_slot(sw, b, tk + 1, i, j), ok);
+            }
+            if (q + 1 < b.imax) cur = nxt;
+        }
+        wave_step_sync();
+    }
+}
+
+// Fill 64-bit words (halo slots to HALO_EMPTY at set-up and after a failed sweep)
+__global__ void __launch_bounds__(256) k_fill_u64(unsigned long long *x, size_t n, unsigned long long v)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = v;
+}
+
 // DIVV12(DRO, DRO, CP) on interior cells (imps3.f:483)
 __global__ void __launch_bounds__(256) k_divcp(Blk b)
 {
@@ -498,6 +986,9 @@ struct BlockRec {
     Blk dev{};
     bool mirror = false;
     double *arena = nullptr;   // device arena of a mirrored block
+    Sweep sw{};                // LU-SGS tile-wavefront state (halo slots, ticket, error word)
+    void *sw_mem = nullptr;
+    size_t nhj = 0, nhk = 0;   // halo slot counts
 };
 
 struct PendingEv {
@@ -517,6 +1008,8 @@ struct ff_ctx {
     double *red_host = nullptr;   // pinned host sums
     int red_parts = 0;
     bool profile = false;
+    int lusgs_mode = FF_LUSGS_TILES;
+    int residual_mode = FF_RESIDUAL_TILES;
     std::vector<PendingEv> pending;
     std::map<std::string, std::pair<long long, double>> stats;
 };
@@ -581,8 +1074,8 @@ static int drain_events(ff_ctx *c)
         auto &s = c->stats[e.name];
         s.first += 1;
         s.second += ms;
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
     c->pending.clear();
     return FF_OK;
@@ -615,6 +1108,65 @@ static int check_dims(ff_ctx *c, const ff_block *v)
     return FF_OK;
 }
 
+// LU-SGS tile-wavefront state of a block: halo slots of the interior tile faces, the
+// ticket and the error word, in one allocation; slots start empty.
+static int sweep_alloc(ff_ctx *c, BlockRec &r)
+{
+    const Blk &b = r.dev;
+    Sweep &sw = r.sw;
+    sw.ntj = (b.jmax + TJ - 1) / TJ;
+    sw.ntk = (b.kmax + TK - 1) / TK;
+    r.nhj = (size_t)5 * (sw.ntj + 1) * b.kmax * b.imax;
+    r.nhk = (size_t)5 * (sw.ntk + 1) * b.jmax * b.imax;
+    size_t words = r.nhj + r.nhk + 2;
+    if (hipMalloc(&r.sw_mem, words * sizeof(unsigned long long)) != hipSuccess)
+        return fail(c, FF_E_NOMEM, "LU-SGS halo slots%s");
+    unsigned long long *m = (unsigned long long *)r.sw_mem;
+    sw.ticket = m;
+    sw.err = (unsigned int *)(m + 1);
+    sw.hj = m + 2;
+    sw.hk = sw.hj + r.nhj;
+    HIPCHK(c, hipMemsetAsync(r.sw_mem, 0, 2 * sizeof(unsigned long long), c->stream));
+    size_t nh = r.nhj + r.nhk;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream,
+                       sw.hj, nh, HALO_EMPTY);
+    HIPCHK(c, hipGetLastError());
+    return FF_OK;
+}
+
+static void sweep_free(BlockRec &r)
+{
+    if (r.sw_mem) (void)hipFree(r.sw_mem);
+    r.sw_mem = nullptr;
+}
+
+// Read the error word of the block's sweeps; after a failed halo wait, refill the slots
+// (a wait that gave up may leave filled slots behind) and report the failure.
+static int sweep_check(ff_ctx *c, BlockRec &r)
+{
+    if (!r.sw_mem) return FF_OK;
+    unsigned int *h = (unsigned int *)(c->red_host + 7);
+    HIPCHK(c, hipMemcpyAsync(h, r.sw.err, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (*h == 0) return FF_OK;
+    *h = 0;
+    size_t nh = r.nhj + r.nhk;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream,
+                       r.sw.hj, nh, HALO_EMPTY);
+    HIPCHK(c, hipMemsetAsync(r.sw.err, 0, sizeof(unsigned int), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return fail(c, FF_E_HIP, "LU-SGS tile wavefront: a halo wait timed out%s");
+}
+
+static int check_all(ff_ctx *c)
+{
+    for (auto &kv : c->blocks) {
+        int rc = sweep_check(c, kv.second);
+        if (rc) return rc;
+    }
+    return FF_OK;
+}
+
 // fields a kernel sequence needs
 static const int kNeeded[] = {
     FF_PDIFF, FF_TEMP, FF_U, FF_V, FF_W, FF_P, FF_RO, FF_E, FF_C, FF_VIS, FF_CH, FF_CP,
@@ -629,7 +1181,7 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
     if (!out || !prm) return FF_E_ARG;
     *out = nullptr;
     if (prm->iturb != 0) return FF_E_UNSUP;        // this build covers ITURB = 0
-    if (prm->iflux != 1) return FF_E_UNSUP;        // Roe flux (FLUXKE)
+    if (prm->iflux != 1 && prm->iflux != 0) return FF_E_UNSUP;   // Roe FLUXKE, van Leer FLUXVL
     if (prm->idiff > 3) return FF_E_UNSUP;         // arithmetic FLUXDO average
     if (prm->expsu != 1.5) return FF_E_UNSUP;      // Sutherland sqrt branch of PERFGA
     ff_ctx *c = new ff_ctx();
@@ -654,12 +1206,14 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
 int ff_ctx_destroy(ff_ctx *c)
 {
     if (!c) return FF_E_ARG;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     drain_events(c);
-    for (auto &kv : c->blocks)
-        if (kv.second.mirror && kv.second.arena) hipFree(kv.second.arena);
-    if (c->red_part) hipFree(c->red_part);
-    if (c->red_out) hipFree(c->red_out);
+    for (auto &kv : c->blocks) {
+        if (kv.second.mirror && kv.second.arena) (void)hipFree(kv.second.arena);
+        sweep_free(kv.second);
+    }
+    if (c->red_part) (void)hipFree(c->red_part);
+    if (c->red_out) (void)hipFree(c->red_out);
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -678,6 +1232,24 @@ int ff_ctx_set_stream(ff_ctx *c, void *stream)
         c->own_stream = false;
     }
     return FF_OK;
+}
+
+int ff_ctx_set_option(ff_ctx *c, int option, int value)
+{
+    if (!c) return FF_E_ARG;
+    if (option == FF_OPT_LUSGS) {
+        if (value != FF_LUSGS_TILES && value != FF_LUSGS_PLANES)
+            return fail(c, FF_E_ARG, "FF_OPT_LUSGS: unknown value%s");
+        c->lusgs_mode = value;
+        return FF_OK;
+    }
+    if (option == FF_OPT_RESIDUAL) {
+        if (value != FF_RESIDUAL_TILES && value != FF_RESIDUAL_CELLS)
+            return fail(c, FF_E_ARG, "FF_OPT_RESIDUAL: unknown value%s");
+        c->residual_mode = value;
+        return FF_OK;
+    }
+    return fail(c, FF_E_ARG, "unknown option%s");
 }
 
 const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -725,8 +1297,16 @@ int ff_block_register(ff_ctx *c, int level, int block, const ff_block *host)
     HIPCHK(c, hipMemsetAsync(r.arena, 0, bytes, c->stream));
     for (int f = 0; f < FF_NFIELDS; f++) b.f[f] = r.arena + (size_t)f * b.ntot;
     r.dev = b;
+    if ((rc = sweep_alloc(c, r))) {
+        (void)hipFree(r.arena);
+        return rc;
+    }
     BlockRec *old = find(c, level, block);
-    if (old && old->mirror && old->arena) (void)hipFree(old->arena);
+    if (old) {
+        if (old->mirror && old->arena) (void)hipFree(old->arena);
+        (void)hipStreamSynchronize(c->stream);
+        sweep_free(*old);
+    }
     c->blocks[key(level, block)] = r;
     return FF_OK;
 }
@@ -741,8 +1321,13 @@ int ff_block_register_device(ff_ctx *c, int level, int block, const ff_block *de
     BlockRec r;
     r.mirror = false;
     r.dev = make_blk(dev);
+    if ((rc = sweep_alloc(c, r))) return rc;
     BlockRec *old = find(c, level, block);
-    if (old && old->mirror && old->arena) (void)hipFree(old->arena);
+    if (old) {
+        if (old->mirror && old->arena) (void)hipFree(old->arena);
+        (void)hipStreamSynchronize(c->stream);
+        sweep_free(*old);
+    }
     c->blocks[key(level, block)] = r;
     return FF_OK;
 }
@@ -752,6 +1337,10 @@ int ff_sync(ff_ctx *c, int level, int block, int direction, unsigned long long m
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    if (direction == 1) {
+        int rc = sweep_check(c, *r);
+        if (rc) return rc;
+    }
     if (!r->mirror) return FF_OK;
     size_t bytes = sizeof(double) * (size_t)r->dev.ntot;
     for (int f = 0; f < FF_NFIELDS; f++) {
@@ -769,7 +1358,7 @@ int ff_finish(ff_ctx *c)
 {
     if (!c) return FF_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return FF_OK;
+    return check_all(c);
 }
 
 static inline dim3 cells_grid(const Blk &b, int tpb) { return dim3((b.imax + tpb - 1) / tpb, b.jmax, b.kmax); }
@@ -800,7 +1389,15 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, double *resi)
         rc = launch(c, "k_zero", k_zero, all_grid(b), dim3(256), b.f[FF_DRO + q], b.ntot);
         if (rc) return rc;
     }
-    rc = launch(c, "k_residual", k_residual, cells_grid(b, 128), dim3(128), make_prm(&c->prm, level), b);
+    const bool vl = c->prm.iflux == 0;
+    if (c->residual_mode == FF_RESIDUAL_TILES) {
+        dim3 g((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ, (b.kmax + RKC - 1) / RKC);
+        rc = launch(c, "k_residual_tile", vl ? k_residual_tile<0> : k_residual_tile<1>, g,
+                    dim3(RI * RJ), make_prm(&c->prm, level), b);
+    } else {
+        rc = launch(c, "k_residual", vl ? k_residual<0> : k_residual<1>, cells_grid(b, 128),
+                    dim3(128), make_prm(&c->prm, level), b);
+    }
     if (rc) return rc;
     if (level == 1 && resi) {
         int ncell = b.imax * b.jmax * b.kmax;
@@ -833,6 +1430,12 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
     Prm kp = make_prm(&c->prm, level);
     int rc = launch(c, "k_time3_copr", k_time3_copr, cells_grid(b, 128), dim3(128), kp, b);
     if (rc) return rc;
+    if (c->lusgs_mode == FF_LUSGS_TILES) {
+        // DRO/CP is applied inside the forward sweep
+        dim3 g(r->sw.ntj * r->sw.ntk);
+        if ((rc = launch(c, "k_pred_tiles", k_pred_tiles, g, dim3(TW), kp, b, r->sw))) return rc;
+        return launch(c, "k_corr_tiles", k_corr_tiles, g, dim3(TW), kp, b, r->sw);
+    }
     const int mlo = 3, mhi = b.imax + b.jmax + b.kmax;
     // backward sweep: hyperplanes from the far corner down
     for (int m = mhi; m >= mlo; m--) {
@@ -868,7 +1471,9 @@ int ff_implicit(ff_ctx *c, int level, int block, int m2)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
-    return implicit_impl(c, r, level, true);
+    int rc = implicit_impl(c, r, level, true);
+    if (rc) return rc;
+    return c->lusgs_mode == FF_LUSGS_TILES ? sweep_check(c, *r) : FF_OK;
 }
 
 static int update_impl(ff_ctx *c, BlockRec *r, int level)
@@ -910,7 +1515,7 @@ int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
         if ((rc = implicit_impl(c, r, 1, true))) return rc;
         if ((rc = update_impl(c, r, 1))) return rc;
     }
-    return FF_OK;
+    return c->lusgs_mode == FF_LUSGS_TILES ? sweep_check(c, *r) : FF_OK;
 }
 
 }  // extern "C"

@@ -256,8 +256,106 @@ __device__ __forceinline__ void visc_face(const Prm &p, const Blk &b, int L, int
     fe = f5 + surfe * (t[L] - t[m]);
 }
 
-// FLUCYC face flux: FLUXDO then FLUXKE added (res3c.f:7856, 7892)
-template <bool SINGLE>
+// FLUXVL base3c.f:1733-1961 at one face: van Leer flux-vector splitting of the MUSCL states
+// of RO, RM, RN, RW, E (ideal gas), as the sums FLUCYC adds to the viscous flux.
+__device__ __forceinline__ Flux5 vl_face(const Prm &p, const Blk &b, int L, int S, int intem,
+                                         const double *__restrict__ an,
+                                         const double *__restrict__ anx,
+                                         const double *__restrict__ any,
+                                         const double *__restrict__ anz)
+{
+    const double gamma = p.gamma;
+    const double ygm = 1. / gamma;
+    const double gm1 = gamma - 1.;
+    double roip, roim, rmip, rmim, rnip, rnim, rwip, rwim, eip, eim;
+    xxtrap(FLD(b, FF_RO), L, S, intem, roip, roim);
+    xxtrap(FLD(b, FF_RM), L, S, intem, rmip, rmim);
+    xxtrap(FLD(b, FF_RN), L, S, intem, rnip, rnim);
+    xxtrap(FLD(b, FF_RW), L, S, intem, rwip, rwim);
+    xxtrap(FLD(b, FF_E), L, S, intem, eip, eim);
+    // pressures from the unfloored states, then the floors (base3c.f:1810-1818)
+    double ppi = (gamma - 1.) * (eip - .5 / roip * (rmip * rmip + rnip * rnip + rwip * rwip));
+    double pmi = (gamma - 1.) * (eim - .5 / roim * (rmim * rmim + rnim * rnim + rwim * rwim));
+    double fd = 0.001 * p.frsden, fp = 0.001 * p.frspre;
+    roip = roip > fd ? roip : fd;
+    ppi = ppi > fp ? ppi : fp;
+    roim = roim > fd ? roim : fd;
+    pmi = pmi > fp ? pmi : fp;
+    double a11 = anx[L], a12 = any[L], a13 = anz[L];
+    double a21, a22, a23, a31, a32, a33;
+    surfvects(a11, a12, a13, a21, a22, a23, a31, a32, a33);
+    double cmip = a11 * rmip + a12 * rnip + a13 * rwip;     // contravariant momenta
+    double cnip = a21 * rmip + a22 * rnip + a23 * rwip;
+    double cwip = a31 * rmip + a32 * rnip + a33 * rwip;
+    double cmim = a11 * rmim + a12 * rnim + a13 * rwim;
+    double cnim = a21 * rmim + a22 * rnim + a23 * rwim;
+    double cwim = a31 * rmim + a32 * rnim + a33 * rwim;
+    const double A = an[L];
+    double f1m, f2m, f3m, f4m, f5m;                          // right state, base3c.f:1866-1893
+    {
+        double ypro = 1. / roip;
+        double uip = ypro * cmip, vip = ypro * cnip, wip = ypro * cwip;
+        double pip = 0.01 > ppi ? 0.01 : ppi;
+        double cip = sqrt(gamma * pip * ypro);
+        double rmaip = uip / cip;
+        if (rmaip < -1.) {
+            f1m = A * cmip;
+            f2m = A * (cmip * uip + pip);
+            f3m = A * cmip * vip;
+            f4m = A * cmip * wip;
+            f5m = A * uip * (eip + pip);
+        } else if (rmaip < 1.) {
+            double h = .5 * (rmaip - 1.);
+            double f1 = -A * roip * cip * (h * h);
+            double f2mi = gm1 * uip - 2. * cip;
+            f2m = f1 * f2mi * ygm;
+            f3m = f1 * vip;
+            f4m = f1 * wip;
+            f5m = f1 * (eip + pip) * ypro;
+            f1m = f1;
+        } else {
+            f1m = 0.; f2m = 0.; f3m = 0.; f4m = 0.; f5m = 0.;
+        }
+    }
+    double f1p, f2p, f3p, f4p, f5p;                          // left state, base3c.f:1897-1928
+    {
+        double ypro = 1. / roim;
+        double uim = ypro * cmim, vim = ypro * cnim, wim = ypro * cwim;
+        double pim = 0.01 > pmi ? 0.01 : pmi;
+        double cim = sqrt(gamma * pim * ypro);
+        double rmaim = uim / cim;
+        if (rmaim > 1.) {
+            f1p = A * cmim;
+            f2p = A * (cmim * uim + pim);
+            f3p = A * cmim * vim;
+            f4p = A * cmim * wim;
+            f5p = A * uim * (eim + pim);
+        } else if (rmaim > -1.) {
+            double h = .5 * (rmaim + 1.);
+            double f1 = +A * roim * cim * (h * h);
+            double f2pi = gm1 * uim + 2. * cim;
+            f2p = f1 * f2pi * ygm;
+            f3p = f1 * vim;
+            f4p = f1 * wim;
+            f5p = f1 * (eim + pim) * ypro;
+            f1p = f1;
+        } else {
+            f1p = 0.; f2p = 0.; f3p = 0.; f4p = 0.; f5p = 0.;
+        }
+    }
+    double f2 = f2p + f2m, f3 = f3p + f3m, f4 = f4p + f4m;   // base3c.f:1934-1950
+    Flux5 out;
+    out.f[0] = f1m + f1p;
+    out.f[1] = a11 * f2 + a21 * f3 + a31 * f4;
+    out.f[2] = a12 * f2 + a22 * f3 + a32 * f4;
+    out.f[3] = a13 * f2 + a23 * f3 + a33 * f4;
+    out.f[4] = f5m + f5p;
+    return out;
+}
+
+// FLUCYC face flux: FLUXDO, then the inviscid flux of the scheme IFLUX selects added
+// (res3c.f:7856, 7892 Roe FLUXKE, 7907 van Leer FLUXVL)
+template <bool SINGLE, int IFLUX = 1>
 __device__ __forceinline__ Flux5 face_flux(const Prm &p, const Blk &b, int L, int S, int intem,
                                            const double *__restrict__ an,
                                            const double *__restrict__ anx,
@@ -266,7 +364,8 @@ __device__ __forceinline__ Flux5 face_flux(const Prm &p, const Blk &b, int L, in
 {
     double fm, fn, fw, fe;
     visc_face(p, b, L, S, an, anx, any, anz, fm, fn, fw, fe);
-    Flux5 r = roe_face<SINGLE>(p, b, L, S, intem, an, anx, any, anz);
+    Flux5 r = IFLUX == 0 ? vl_face(p, b, L, S, intem, an, anx, any, anz)
+                         : roe_face<SINGLE>(p, b, L, S, intem, an, anx, any, anz);
     Flux5 o;
     o.f[0] = r.f[0] + 0.;
     o.f[1] = r.f[1] + fm;
@@ -332,10 +431,80 @@ __device__ __forceinline__ void face_contrib(const double dw[5], double c, doubl
           a22, a23, a31, a32, a33);
 }
 
+// A face's unit normal with the tangential basis SURFVECTS builds from it.  It depends on
+// the geometry only, so the LU-SGS sweeps form it before the recurrence reaches the cell.
+struct FaceBasis { double nx, ny, nz, a21, a22, a23, a31, a32, a33; };
+
+__device__ __forceinline__ FaceBasis face_basis(double nx, double ny, double nz)
+{
+    FaceBasis f;
+    f.nx = nx; f.ny = ny; f.nz = nz;
+    surfvects(nx, ny, nz, f.a21, f.a22, f.a23, f.a31, f.a32, f.a33);
+    return f;
+}
+
+// face_contrib with the basis and the cell scalars RO*C (SYI), .5/RO and (.5/RO)/C (SYM1I)
+// formed beforehand; every operation that involves the increments is the same as there.
+__device__ __forceinline__ void face_contrib_b(const double dw[5], double ro, double roc,
+                                               double pc2, double proc, const FaceBasis &f,
+                                               double x1, double x2, double x5, double out[5])
+{
+    double z1i = dw[0], z5i = dw[4];
+    double z2i = f.nx * dw[1] + f.ny * dw[2] + f.nz * dw[3];
+    double z3i = f.a21 * dw[1] + f.a22 * dw[2] + f.a23 * dw[3];
+    double z4i = f.a31 * dw[1] + f.a32 * dw[2] + f.a33 * dw[3];
+    double w1 = z1i - z5i / ro;
+    double w2 = roc * z2i + z5i;
+    double w3 = z3i;
+    double w4 = z4i;
+    double w5 = -roc * z2i + z5i;
+    w1 = x1 * w1;
+    w2 = x2 * w2;
+    w3 = x1 * w3;
+    w4 = x1 * w4;
+    w5 = x5 * w5;
+    out[0] = w1 + pc2 * (w2 + w5);
+    double x2i = proc * (w2 - w5);
+    out[4] = .5 * (w2 + w5);
+    out[1] = f.nx * x2i + f.a21 * w3 + f.a31 * w4;
+    out[2] = f.ny * x2i + f.a22 * w3 + f.a32 * w4;
+    out[3] = f.nz * x2i + f.a23 * w3 + f.a33 * w4;
+}
+
 // Split-eigenvalue coefficients of cell n across one face.
 //   PREDGS (imps4.f:405-454): face = own lower face n, neighbour n-S, sign -1.
 //   CORRGS (imps4.f:150-195): face = own upper face n+S, neighbour n+S, viscous area A(n).
 struct Coef { double x1, x2, x5, rmax, xa; };
+
+// The same coefficients from values: (nx,ny,nz) the face normal, an = A(n), the cell's
+// and the neighbour's VOL, RO, VIS, VIST.
+template <bool PRED>
+__device__ __forceinline__ Coef coef_vals(double nx, double ny, double nz, double u, double v,
+                                          double w, double c, double an, double vol,
+                                          double volb, double ro, double rob, double vis,
+                                          double vist, double visb, double vistb,
+                                          double cdiffd)
+{
+    double rl1 = nx * u + ny * v + nz * w - 0.;
+    double rl2 = rl1 + c;
+    double rl5 = rl1 - c;
+    double rnum = (vol + volb) * (ro + rob);
+    double t = cdiffd * an / rnum * (vist + vis + vistb + visb);
+    double rmax = 1.00 * (fabs(rl1) + c);
+    Coef k;
+    if (PRED) {
+        k.x1 = (rl1 - rmax) * .5 - t;
+        k.x2 = (rl2 - rmax) * .5 - t;
+        k.x5 = (rl5 - rmax) * .5 - t;
+    } else {
+        k.x1 = (rl1 + rmax) * .5 + t;
+        k.x2 = (rl2 + rmax) * .5 + t;
+        k.x5 = (rl5 + rmax) * .5 + t;
+    }
+    k.rmax = rmax;
+    k.xa = rmax + t;
+    return k;
+}
 
 template <bool PRED>
 __device__ __forceinline__ Coef lusgs_coef(const Blk &b, int n, int S, double cdiffd,

@@ -16,11 +16,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libffhip.so")
 
 EXPORTS = [
-    "ff_ctx_create", "ff_ctx_destroy", "ff_ctx_set_stream", "ff_last_error", "ff_ctx_profile",
+    "ff_ctx_create", "ff_ctx_destroy", "ff_ctx_set_stream", "ff_ctx_set_option",
+    "ff_last_error", "ff_ctx_profile",
     "ff_kernel_stats", "ff_kernel_stats_reset", "ff_block_register",
     "ff_block_register_device", "ff_sync", "ff_finish", "ff_state", "ff_turb", "ff_residual",
     "ff_implicit", "ff_update", "ff_save_old", "ff_cycle",
 ]
+
+FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES = 1, 0, 1
+FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 
 _lib = None
 
@@ -40,6 +44,7 @@ def lib():
         L.ff_ctx_create.argtypes = [C.POINTER(vp), C.POINTER(Params)]
         L.ff_ctx_destroy.argtypes = [vp]
         L.ff_ctx_set_stream.argtypes = [vp, vp]
+        L.ff_ctx_set_option.argtypes = [vp, i, i]
         L.ff_last_error.argtypes = [vp]
         L.ff_last_error.restype = C.c_char_p
         L.ff_ctx_profile.argtypes = [vp, i]
@@ -87,6 +92,17 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_option(self, option, value):
+        self._chk(lib().ff_ctx_set_option(self.h, option, value), "ff_ctx_set_option")
+
+    def lusgs_mode(self, mode):
+        """LU-SGS sweep form: "tiles" (persistent tile wavefront) or "planes"."""
+        self.set_option(FF_OPT_LUSGS, {"tiles": FF_LUSGS_TILES, "planes": FF_LUSGS_PLANES}[mode])
+
+    def residual_mode(self, mode):
+        """Residual form: "tiles" (each face once) or "cells" (both faces per cell)."""
+        self.set_option(FF_OPT_RESIDUAL, {"tiles": FF_RESIDUAL_TILES, "cells": FF_RESIDUAL_CELLS}[mode])
 
     # ---- blocks ------------------------------------------------------------------
     def register(self, blk: Block, level=1, block=1):

@@ -101,6 +101,20 @@ int  ff_ctx_destroy(ff_ctx *ctx);
 /* Use the caller's HIP stream (a hipStream_t passed as void*); NULL = the context's own. */
 int  ff_ctx_set_stream(ff_ctx *ctx, void *stream);
 const char *ff_last_error(const ff_ctx *ctx);
+/* Context options (no counterpart in the reference, which has one form of each stage):
+ *   FF_OPT_LUSGS: FF_LUSGS_TILES (default) runs each LU-SGS sweep as one persistent
+ *   tile-wavefront launch; FF_LUSGS_PLANES launches one kernel per I+J+K hyperplane.
+ *   Both give the same results bit for bit. */
+#define FF_OPT_LUSGS     1
+#define FF_LUSGS_TILES   0
+#define FF_LUSGS_PLANES  1
+/*   FF_OPT_RESIDUAL: FF_RESIDUAL_TILES (default) forms every face flux once in (I,J) tiles
+ *   marching along K; FF_RESIDUAL_CELLS forms both faces of every direction per cell.
+ *   Both give the same results bit for bit. */
+#define FF_OPT_RESIDUAL    2
+#define FF_RESIDUAL_TILES  0
+#define FF_RESIDUAL_CELLS  1
+int  ff_ctx_set_option(ff_ctx *ctx, int option, int value);
 /* Per-kernel event timing (0 off, 1 on); results via ff_kernel_stats. */
 int  ff_ctx_profile(ff_ctx *ctx, int enable);
 /* Copy the accumulated timing of kernel `name`: launches and total milliseconds. */
@@ -125,8 +139,9 @@ int  ff_finish(ff_ctx *ctx);
 int  ff_state(ff_ctx *ctx, int level, int block);
 /* TURBDI (ns3c.f:2874) for ITURB=0: EPS2 = 1, VIST = 0 (ADJUIN ns3c.f:2987-2991). */
 int  ff_turb(ff_ctx *ctx, int level, int block);
-/* RESIDI (ns3c.f:2355) -> RESIDU (res3c.f:6): viscous FLUXDO + Roe FLUXKE face fluxes in
- * the k, i, j directions, SWEEP differences, division by VOL.  With level == 1 the L2
+/* RESIDI (ns3c.f:2355) -> RESIDU (res3c.f:6): viscous FLUXDO plus the inviscid flux of
+ * params.iflux (1: Roe FLUXKE base3c.f:4753; 0: van Leer FLUXVL base3c.f:1733) in the k, i,
+ * j directions, SWEEP differences, division by VOL.  With level == 1 the L2
  * sums of RESL2 (res3c.f:8920) are returned in resi[0..6] (DRO, DM, DN, DW, DE, DRK, DEPS
  * squared sums; the caller takes sqrt()/N as SUBIN does); resi may be NULL. */
 int  ff_residual(ff_ctx *ctx, int level, int block, int m2, double *resi);

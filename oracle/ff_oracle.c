@@ -15,8 +15,8 @@
  *   ffo_turb      TURBDI ITURB=0 -> ADJUIN (ns3c.f:2987-2991)
  *   ffo_residual  RESIDU res3c.f:6-787 with FLUCYC res3c.f:7572 -> FLUXDO base3c.f:3677 and
  *                 FLUXKE base3c.f:4753 (XXTRAP base3c.f:750, EFPRO state.f:620, DERIV
- *                 base3c.f:707, SURFVECTS base3c.f:12245), SWEEP res3c.f:992, RESL2
- *                 res3c.f:8920
+ *                 base3c.f:707, SURFVECTS base3c.f:12245) or FLUXVL base3c.f:1733 (van
+ *                 Leer, IFLUX = 0), SWEEP res3c.f:992, RESL2 res3c.f:8920
  *   ffo_implicit  IMPS imps3.f:6 = TIME3 base3c.f:79, LOCDT imps3.f:784, COPR imps3.f:891,
  *                 PREDGS imps4.f:345, DIAGGS imps4.f:604, CORRGS imps4.f:76 (SYI/SYM1I
  *                 imps4.f:10-70), DIVV12 by CP (imps3.f:483)
@@ -329,6 +329,118 @@ static flux7 roe_face(const ff_params *p, const ff_block *b, long L, long S, int
     return out;
 }
 
+/* FLUXVL base3c.f:1733-1961 at one face: van Leer flux-vector splitting of the MUSCL
+ * states of the conservative variables (ideal gas).  Returned as the sums FLUCYC adds to
+ * the viscous face flux: F1M+F1P, the rotated momentum fluxes and F5M+F5P. */
+static flux7 vl_face(const ff_params *p, const ff_block *b, long L, long S, int intem,
+                     const double *an, const double *anx, const double *any,
+                     const double *anz)
+{
+    const double gamma = p->gamma;
+    const double ygm = 1. / gamma;                   /* base3c.f:1775 */
+    const double gm1 = gamma - 1.;
+    double roip, roim, rmip, rmim, rnip, rnim, rwip, rwim, eip, eim;
+    xxtrap(F(b, FF_RO), L, S, intem, &roip, &roim);
+    xxtrap(F(b, FF_RM), L, S, intem, &rmip, &rmim);
+    xxtrap(F(b, FF_RN), L, S, intem, &rnip, &rnim);
+    xxtrap(F(b, FF_RW), L, S, intem, &rwip, &rwim);
+    xxtrap(F(b, FF_E), L, S, intem, &eip, &eim);
+    /* loop 500, base3c.f:1810-1818: pressures from the unfloored states, then floors */
+    double ppi = (gamma - 1.) * (eip - .5 / roip * (rmip * rmip + rnip * rnip + rwip * rwip));
+    double pmi = (gamma - 1.) * (eim - .5 / roim * (rmim * rmim + rnim * rnim + rwim * rwim));
+    double fd = 0.001 * p->frsden, fp = 0.001 * p->frspre;
+    roip = roip > fd ? roip : fd;
+    ppi = ppi > fp ? ppi : fp;
+    roim = roim > fd ? roim : fd;
+    pmi = pmi > fp ? pmi : fp;
+    /* loop 900: contravariant momenta (base3c.f:1831-1858) */
+    double a11 = anx[L], a12 = any[L], a13 = anz[L];
+    double a21, a22, a23, a31, a32, a33;
+    surfvects(a11, a12, a13, &a21, &a22, &a23, &a31, &a32, &a33);
+    double rmipi = rmip, rnipi = rnip, rwipi = rwip, rmimi = rmim, rnimi = rnim, rwimi = rwim;
+    rmip = a11 * rmipi + a12 * rnipi + a13 * rwipi;
+    rnip = a21 * rmipi + a22 * rnipi + a23 * rwipi;
+    rwip = a31 * rmipi + a32 * rnipi + a33 * rwipi;
+    rmim = a11 * rmimi + a12 * rnimi + a13 * rwimi;
+    rnim = a21 * rmimi + a22 * rnimi + a23 * rwimi;
+    rwim = a31 * rmimi + a32 * rnimi + a33 * rwimi;
+    const double A = an[L];
+    /* negative part from the right state (base3c.f:1866-1893) */
+    double f1m, f2m, f3m, f4m, f5m;
+    {
+        double ypro = 1. / roip;
+        double uip = ypro * rmip, vip = ypro * rnip, wip = ypro * rwip;
+        double pip = 0.01 > ppi ? 0.01 : ppi;        /* MAX(0.01,PPI) */
+        double cip = sqrt(gamma * pip * ypro);
+        double rmaip = uip / cip;
+        if (rmaip < -1.) {
+            f1m = A * rmip;
+            f2m = A * (rmip * uip + pip);
+            f3m = A * rmip * vip;
+            f4m = A * rmip * wip;
+            f5m = A * uip * (eip + pip);
+        } else if (rmaip < 1.) {
+            double h = .5 * (rmaip - 1.);
+            double f1 = -A * roip * cip * (h * h);
+            double f2mi = gm1 * uip - 2. * cip;
+            f2m = f1 * f2mi * ygm;
+            f3m = f1 * vip;
+            f4m = f1 * wip;
+            f5m = f1 * (eip + pip) * ypro;
+            f1m = f1;
+        } else {
+            f1m = 0.; f2m = 0.; f3m = 0.; f4m = 0.; f5m = 0.;
+        }
+    }
+    /* positive part from the left state (base3c.f:1897-1928) */
+    double f1p, f2p, f3p, f4p, f5p;
+    {
+        double ypro = 1. / roim;
+        double uim = ypro * rmim, vim = ypro * rnim, wim = ypro * rwim;
+        double pim = 0.01 > pmi ? 0.01 : pmi;
+        double cim = sqrt(gamma * pim * ypro);
+        double rmaim = uim / cim;
+        if (rmaim > 1.) {
+            f1p = A * rmim;
+            f2p = A * (rmim * uim + pim);
+            f3p = A * rmim * vim;
+            f4p = A * rmim * wim;
+            f5p = A * uim * (eim + pim);
+        } else if (rmaim > -1.) {
+            double h = .5 * (rmaim + 1.);
+            double f1 = +A * roim * cim * (h * h);
+            double f2pi = gm1 * uim + 2. * cim;
+            f2p = f1 * f2pi * ygm;
+            f3p = f1 * vim;
+            f4p = f1 * wim;
+            f5p = f1 * (eim + pim) * ypro;
+            f1p = f1;
+        } else {
+            f1p = 0.; f2p = 0.; f3p = 0.; f4p = 0.; f5p = 0.;
+        }
+    }
+    /* base3c.f:1934-1950 */
+    double f2 = f2p + f2m, f3 = f3p + f3m, f4 = f4p + f4m;
+    flux7 out;
+    out.f[0] = f1m + f1p;
+    out.f[1] = a11 * f2 + a21 * f3 + a31 * f4;
+    out.f[2] = a12 * f2 + a22 * f3 + a32 * f4;
+    out.f[3] = a13 * f2 + a23 * f3 + a33 * f4;
+    out.f[4] = f5m + f5p;
+    out.f[5] = 0.;
+    out.f[6] = 0.;
+    return out;
+}
+
+/* Inviscid face flux of the scheme FLUCYC selects with IFLUX (res3c.f:7892-7915) */
+static flux7 inv_face(const ff_params *p, const ff_block *b, long L, long S, int intem,
+                      const double *an, const double *anx, const double *any,
+                      const double *anz, int single)
+{
+    if (p->iflux == 0) return vl_face(p, b, L, S, intem, an, anx, any, anz);
+    return roe_face(p, b, L, S, intem, an, anx, any, anz, single);
+}
+
 /* FLUXDO base3c.f:3677-3869 at one face, IDIFF <= 3 (arithmetic average).
  * Returns the momentum fluxes (which FLUXDO stores) and the energy increment added to FE. */
 static void visc_face(const ff_params *p, const ff_block *b, long L, long S,
@@ -378,8 +490,8 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last)
                 visc_face(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &fe);
                 ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
                 ff[4][L] = ff[4][L] + fe;
-                /* then FLUXKE (res3c.f:7892) adds the inviscid flux */
-                flux7 r = roe_face(p, b, L, S, intem, an, anx, any, anz, dir == 2);
+                /* then FLUXKE / FLUXVL (res3c.f:7892, 7907) adds the inviscid flux */
+                flux7 r = inv_face(p, b, L, S, intem, an, anx, any, anz, dir == 2);
                 ff[0][L] = r.f[0] + ff[0][L];
                 ff[1][L] = r.f[1] + ff[1][L];
                 ff[2][L] = r.f[2] + ff[2][L];
@@ -928,7 +1040,7 @@ void ffo_face_flux(const ff_params *p, const ff_block *b, long L, int dir, doubl
                  *any = F(b, FF_A1Y + 4 * dir), *anz = F(b, FF_A1Z + 4 * dir);
     double fm, fn, fw, fe;
     visc_face(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &fe);
-    flux7 r = roe_face(p, b, L, S, p->inter[dir], an, anx, any, anz, dir == 2);
+    flux7 r = inv_face(p, b, L, S, p->inter[dir], an, anx, any, anz, dir == 2);
     out[0] = r.f[0] + 0.;
     out[1] = r.f[1] + fm;
     out[2] = r.f[2] + fn;

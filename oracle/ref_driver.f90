@@ -6,7 +6,8 @@
 !   ref_state     STATIM ns3c.f:10318-10447: PDTOP, STATE->PERFGA (state.f:159), CONSVA
 !   ref_turb      TURBDI ns3c.f:2987-2991 (ITURB=0): EPS2=1, VIST=0
 !   ref_residual  RESIDU res3c.f:194-559: ZEROZZ, then per direction k,i,j the FLUCYC
-!                 sequence res3c.f:7705-7892 (ZEROZZ of F1*, FLUXDO, FLUXKE) and SWEEP;
+!                 sequence res3c.f:7705-7907 (ZEROZZ of F1*, FLUXDO, FLUXKE or FLUXVL)
+!                 and SWEEP;
 !                 RESL2 res3c.f:8920 on level 1
 !   ref_implicit  IMPS imps3.f:111-483 with LUSGS=1: TIME3, LOCDT, COPR, PREDGS, DIAGGS,
 !                 CORRGS, DIVV12
@@ -14,7 +15,7 @@
 !
 ! All fields of a block are packed as F(NTOT, NF) in the order of enum ff_field
 ! (include/ffhip.h); the parameter vector PRM and integer vector IPR are filled by the
-! Python test harness (tests/refbridge.py).
+! Python test harness (oracle/oracle_py.py, RefBlock).
 module ffref_ids
   implicit none
   integer, parameter :: IPDIFF=1, ITEMP=2, IU=3, IV=4, IW=5, IRK=6, IREPS=7
@@ -80,7 +81,7 @@ subroutine ref_turb(f, ntot, prm, ipr) bind(C, name="ref_turb")
 end subroutine ref_turb
 
 ! One direction of RESIDU: the FLUCYC sequence for a block with no boundary patches
-! (NPATCH = 0), FULLNS = .FALSE., IFLUX = 1, followed by SWEEP.
+! (NPATCH = 0), FULLNS = .FALSE., IFLUX = 1 (Roe) or 0 (van Leer), followed by SWEEP.
 subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat, zero, trm)
   use iso_c_binding
   use ffref_ids
@@ -121,6 +122,17 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
        f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), prm(PFRSPRE), &
        hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, mulphl, 1, fresul, 0._c_double, f1h, &
        wave, ipr(QLEVEL), ipr(QIDIFF))
+  if (ipr(QIFLUX) == 0) then
+  ! FLUXVL (res3c.f:7907): van Leer flux-vector splitting
+  call FLUXVL(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), &
+       f(1,ia), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), f(1,IP), &
+       f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), imax, jmax, kmax, 1, inter, ndi, &
+       f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,IPDIFF), f(1,iax), f(1,iay), f(1,iaz), &
+       prm(PPR), prm(PPRT), zero, f(1,ITEMP), f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, &
+       prm(PGAMMA), 0._c_double, 0._c_double, prm(PFRSDEN), prm(PFRSPRE), 0._c_double, &
+       0._c_double, hat(1,1), hat(1,2), hat(1,3), hat(1,4), kmaxp, kstr, idir, ipr(QITURB), &
+       zzz, maw, maxw, rksi, inchiml)
+  else
   ! FLUXKE (res3c.f:7892): Roe flux; FLUCYC passes RKLIM = EPSLIM = 0 (res3c.f:7680-7681)
   call FLUXKE(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), &
        f(1,ia), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), f(1,IP), &
@@ -130,6 +142,7 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
        1, prm(PRGAS), prm(PGAMMA), 0._c_double, 0._c_double, prm(PFRSDEN), prm(PFRSPRE), 0._c_double, 0._c_double, &
        hat(1,1), hat(1,2), hat(1,3), hat(1,4), kmaxp, kstr, idir, ipr(QITURB), zzz, maw, &
        maxw, xxtral, rksi, inchiml, entropy_fix, transl, trm)
+  end if
   ! SWEEP (res3c.f:330, 448, 559); the j-direction is last and divides by VOL
   ido = 0
   if (idir == 2) ido = 1

@@ -12,6 +12,13 @@ CASES = [
     ("limited_inter3", (9, 6, 7), dict(inter=(3, 2, 1)), dict(kind="all", amp=0.08, distort=0.1, seed=7)),
     ("first_order", (6, 8, 5), dict(inter=(55, 55, 55), cfl=2.0), dict(kind="all", amp=0.05, seed=9)),
     ("stretched_j", (12, 10, 4), dict(cfl=10.0), dict(kind="velocity", amp=0.02, jratio=1.3, seed=11)),
+    # van Leer flux-vector splitting (IFLUX = 0): subsonic faces, faces on both sides of
+    # Mach 1 (oblique supersonic stream with large perturbations) and reversed flow
+    ("vanleer", (9, 7, 6), dict(iflux=0), dict(kind="all", amp=0.05, distort=0.12, seed=13)),
+    ("vanleer_transonic", (8, 6, 7), dict(iflux=0, frsvel=380.0, cfl=2.0),
+     dict(kind="all", amp=0.12, distort=0.1, seed=17, mach_dir=(1.0, 0.6, -0.3))),
+    ("vanleer_reverse", (7, 8, 6), dict(iflux=0, frsvel=420.0, inter=(2, -3, 55), cfl=1.5),
+     dict(kind="all", amp=0.1, distort=0.1, seed=19, mach_dir=(-1.0, -0.4, 0.5))),
 ]
 
 

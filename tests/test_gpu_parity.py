@@ -12,9 +12,10 @@ RES = ["DRO", "DM", "DN", "DW", "DE"]
 PRIM = ["TEMP", "U", "V", "W", "PDIFF", "P"]
 
 
-def _ctx(prm, blk):
+def _ctx(prm, blk, lusgs="tiles"):
     from finflo_amd.ffhip import Context
     ctx = Context(prm)
+    ctx.lusgs_mode(lusgs)
     ctx.register(blk)
     ctx.upload()
     return ctx
@@ -41,13 +42,14 @@ def test_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
+@pytest.mark.parametrize("lusgs", ["tiles", "planes"])
 @pytest.mark.parametrize("name,dims,pover,ckw", CASES[:4], ids=[c[0] for c in CASES[:4]])
-def test_cycles(hip_device, name, dims, pover, ckw):
-    print(f"case {name} dims {dims} 5 cycles", flush=True)
+def test_cycles(hip_device, name, dims, pover, ckw, lusgs):
+    print(f"case {name} dims {dims} 5 cycles, LU-SGS {lusgs}", flush=True)
     prm, blk = make_case(dims, pover, ckw)
     a, g = blk.copy(), blk.copy()
     oa = OracleBlock(prm, a)
-    ctx = _ctx(prm, g)
+    ctx = _ctx(prm, g, lusgs)
     for _ in range(5):
         r1 = oa.cycle()
     r2 = ctx.cycle(ncycles=5)
@@ -57,10 +59,16 @@ def test_cycles(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
-def test_larger_block_cycle(hip_device):
-    """A block with more cells than one workgroup per row/plane."""
-    print("case large 40x24x20", flush=True)
-    prm, blk = make_case((40, 24, 20), {}, dict(kind="all", amp=0.03, distort=0.1, seed=21))
+# blocks spanning several 8x8 (J,K) tiles of the LU-SGS wavefront, with ragged last tiles
+# and a J-extent that is a multiple of the tile edge
+MULTI_TILE = [(40, 24, 20), (20, 19, 13), (9, 17, 26), (33, 8, 16)]
+
+
+@pytest.mark.parametrize("dims", MULTI_TILE, ids=["x".join(map(str, d)) for d in MULTI_TILE])
+def test_larger_block_cycle(hip_device, dims):
+    """Blocks with more cells than one workgroup per row/plane/tile."""
+    print(f"case large {dims}", flush=True)
+    prm, blk = make_case(dims, {}, dict(kind="all", amp=0.03, distort=0.1, seed=21))
     a, g = blk.copy(), blk.copy()
     oa = OracleBlock(prm, a)
     ctx = _ctx(prm, g)
@@ -68,5 +76,43 @@ def test_larger_block_cycle(hip_device):
         oa.cycle()
     ctx.cycle(ncycles=2)
     ctx.download()
-    assert_fields_equal(g, a, PRIM, interior=False, what="40x24x20")
+    assert_fields_equal(g, a, PRIM, interior=False, what=f"{dims}")
     ctx.close()
+
+
+def test_implicit_stage_tiles_vs_planes(hip_device):
+    """The two LU-SGS forms give the same increments bit for bit, stage by stage."""
+    print("case tiles-vs-planes 26x21x18", flush=True)
+    prm, blk = make_case((26, 21, 18), {}, dict(kind="all", amp=0.04, distort=0.1, seed=31))
+    out = {}
+    for mode in ("tiles", "planes"):
+        g = blk.copy()
+        ctx = _ctx(prm, g, mode)
+        ctx.save_old(); ctx.statim(); ctx.turbdi(); ctx.residi(); ctx.impsdi()
+        ctx.download()
+        out[mode] = g
+        ctx.close()
+    assert_fields_equal(out["tiles"], out["planes"], ["DTL"] + RES, what="tiles vs planes")
+
+
+@pytest.mark.parametrize("dims", [(70, 9, 17), (5, 3, 2), (130, 6, 9)], ids=["70x9x17", "5x3x2", "130x6x9"])
+def test_residual_tiles_vs_cells_vs_oracle(hip_device, dims):
+    """The face-once tile residual, the per-cell residual and the oracle agree bit for bit
+    on blocks with ragged I-, J- and K-tiles."""
+    print(f"case residual forms {dims}", flush=True)
+    prm, blk = make_case(dims, {}, dict(kind="all", amp=0.05, distort=0.12, seed=41))
+    a = blk.copy()
+    oa = OracleBlock(prm, a)
+    oa.save_old(); oa.state(); oa.turb(); r0 = oa.residual()
+    for mode in ("tiles", "cells"):
+        g = blk.copy()
+        from finflo_amd.ffhip import Context
+        ctx = Context(prm)
+        ctx.residual_mode(mode)
+        ctx.register(g)
+        ctx.upload()
+        ctx.save_old(); ctx.statim(); ctx.turbdi(); r = ctx.residi()
+        ctx.download()
+        assert_fields_equal(g, a, RES, what=f"RESIDU ({mode})")
+        np.testing.assert_allclose(r[:5], r0[:5], rtol=1e-13)
+        ctx.close()

@@ -99,8 +99,21 @@ def uniform_block(dims=(8, 6, 5), distort=0.2, prm=None, vel=(1.0, 0.3, -0.2)):
     return prm, blk, o
 
 
-def test_uniform_face_flux_is_euler_flux():
-    prm, blk, o = uniform_block()
+# (IFLUX, free-stream speed, direction): Roe and van Leer, subsonic faces, supersonic faces
+# in both directions (van Leer's split parts then vanish on one side)
+UNIFORM_FLUX_CASES = [
+    (1, None, (1.0, 0.3, -0.2)), (0, None, (1.0, 0.3, -0.2)), (1, 520.0, (1.0, 0.3, -0.2)),
+    (0, 520.0, (1.0, 0.3, -0.2)), (0, 520.0, (-1.0, -0.2, 0.1)), (0, 250.0, (0.2, -1.0, 0.7)),
+]
+
+
+@pytest.mark.parametrize("iflux,speed,vel", UNIFORM_FLUX_CASES)
+def test_uniform_face_flux_is_euler_flux(iflux, speed, vel):
+    """Consistency: F(U, U) is the physical flux (Roe); F+(U) + F-(U) = F(U) (van Leer)."""
+    prm = default_params(iflux=iflux)
+    if speed:
+        prm.frsvel = speed
+    prm, blk, o = uniform_block(prm=prm, vel=vel)
     L = blk.cell(3, 2, 2)
     ro, u, v, w = blk["RO"][L], blk["U"][L], blk["V"][L], blk["W"][L]
     p, E = blk["P"][L], blk["E"][L]
@@ -115,8 +128,9 @@ def test_uniform_face_flux_is_euler_flux():
         assert np.all(np.abs(f - exact) <= 1e-12 * scale + 1e-300), (d, f, exact)
 
 
-def test_freestream_preservation_curvilinear():
-    prm, blk, o = uniform_block(distort=0.25)
+@pytest.mark.parametrize("iflux", [1, 0])
+def test_freestream_preservation_curvilinear(iflux):
+    prm, blk, o = uniform_block(distort=0.25, prm=default_params(iflux=iflux))
     o.residual()
     scale = blk["RO"][0] * prm.frsvel * np.mean(blk.interior("A1")) / np.mean(blk.interior("VOL"))
     for n, s in zip(D5, (1, prm.frsvel, prm.frsvel, prm.frsvel, blk["E"][0] / blk["RO"][0])):
