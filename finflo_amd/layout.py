@@ -20,6 +20,8 @@ FIELDS = [
     "VOL", "A1", "A1X", "A1Y", "A1Z", "A2", "A2X", "A2Y", "A2Z",
     "A3", "A3X", "A3Y", "A3Z", "D1", "D2", "D3", "XC", "YC", "ZC", "DISTW",
     "SRK", "SEPS", "PTUR", "FUN1", "OHMI",
+    "W12", "W13", "W23", "S11", "S12", "S13", "S22", "S23", "S33", "STRAIN", "TTS",
+    "B11", "B12", "B13", "B22", "B23", "B33",
 ]
 NF = len(FIELDS)
 FID = {name: i for i, name in enumerate(FIELDS)}
@@ -50,8 +52,8 @@ class Params(C.Structure):
         "gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
         "frspre", "frsden", "frstem", "frsvel", "frsvis",
         "cfl", "cfll", "theta", "cdiff", "tuplim", "tlolim", "rklim", "epslim",
-        "turlim", "frsmut", "rmult")] + [
-        ("reserved", C.c_double * 7),
+        "turlim", "frsmut", "rmult", "re", "csimps")] + [
+        ("reserved", C.c_double * 5),
         ("iturb", C.c_int), ("iflux", C.c_int), ("idiff", C.c_int),
         ("inter", C.c_int * 3), ("kover", C.c_int), ("katol", C.c_int),
         ("reserved_i", C.c_int * 8)]
@@ -59,7 +61,7 @@ class Params(C.Structure):
     DOUBLES = ("gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
                "frspre", "frsden", "frstem", "frsvel", "frsvis",
                "cfl", "cfll", "theta", "cdiff", "tuplim", "tlolim", "rklim", "epslim",
-               "turlim", "frsmut", "rmult")
+               "turlim", "frsmut", "rmult", "re", "csimps")
 
     def dvec(self):
         """Parameters as the PRM vector of the reference driver (same order)."""
@@ -87,6 +89,7 @@ def default_params(**kw):
     p.tuplim, p.tlolim = 6.0 * p.frstem, 0.05 * p.frstem   # main.f:3827-3828
     p.rklim, p.epslim = 1e-10, 1e-12
     p.turlim, p.frsmut, p.rmult = 1e5, 1e-10, 1.0
+    p.re, p.csimps = 1e5, 5.0
     p.iturb, p.iflux, p.idiff = 0, 1, 1
     p.inter[0] = p.inter[1] = p.inter[2] = -3
     p.kover, p.katol = 0, 1

@@ -37,8 +37,16 @@ enum ff_field {
        i-, j- and k-faces, cell spacings, cell centres, wall distance */
     FF_VOL, FF_A1, FF_A1X, FF_A1Y, FF_A1Z, FF_A2, FF_A2X, FF_A2Y, FF_A2Z,
     FF_A3, FF_A3X, FF_A3Y, FF_A3Z, FF_D1, FF_D2, FF_D3, FF_XC, FF_YC, FF_ZC, FF_DISTW,
-    /* k-omega SST work fields (RESIKE res3c.f:3423, TURBEQ res3c.f:5845) */
+    /* k-omega SST work fields (RESIKE res3c.f:3423, TURBEQ res3c.f:5845): the sources
+       of rho*k and rho*omega, production, blending function F1, vorticity magnitude */
     FF_SRK, FF_SEPS, FF_PTUR, FF_FUN1, FF_OHMI,
+    /* velocity gradients of DERFUN (turb3.f:2476): vorticity W12 W13 W23, strain-rate
+       tensor S11 S12 S13 S22 S23 S33 (the reference's W12(:,1..3), SIJ(:,1..6)), strain
+       magnitude STRAIN, turbulence time scale TTS (TTIMES earsm.f:629) */
+    FF_W12, FF_W13, FF_W23, FF_S11, FF_S12, FF_S13, FF_S22, FF_S23, FF_S33,
+    FF_STRAIN, FF_TTS,
+    /* EARSM extra anisotropy BIJ(:,1..6) (earsm.f:209-219) */
+    FF_B11, FF_B12, FF_B13, FF_B22, FF_B23, FF_B33,
     FF_NFIELDS
 };
 
@@ -67,7 +75,9 @@ typedef struct ff_params {
     double turlim;   /* TURLIM, upper limit of mu_t/mu */
     double frsmut;   /* FRSMUT, lower limit of mu_t */
     double rmult;    /* RMULTV viscosity multiplier, 1 */
-    double reserved[7];
+    double re;       /* Reynolds number RE of the reference length (FUN1KO's wall limit) */
+    double csimps;   /* CSIMPS of the k-omega point-implicit sources (main.f:5346), 5 */
+    double reserved[5];
     int iturb;       /* 0 laminar, 6 k-omega SST */
     int iflux;       /* 1 Roe (FLUXKE), 0 van Leer (FLUXVL) */
     int idiff;       /* FLUXDO averaging, 1 = arithmetic (main.f:5754) */

@@ -103,21 +103,6 @@ void ffo_state(const ff_params *p, ff_block *b)
     }
 }
 
-/* TURBDI for ITURB = 0: ADJUIN (res3c.f:9447-9467) sets EPS2 = 1 and VIST = 0 on the
- * interior cells (ns3c.f:2987-2991); ghost values are the caller's boundary values. */
-void ffo_turb(const ff_params *p, ff_block *b)
-{
-    (void)p;
-    dims_t d = mkdims(b);
-    for (int k = 1; k <= d.kmax; k++)
-        for (int j = 1; j <= d.jmax; j++)
-            for (int i = 1; i <= d.imax; i++) {
-                long n = cid(&d, i, j, k);
-                F(b, FF_EPS2)[n] = 1.;
-                F(b, FF_VIST)[n] = 0.;
-            }
-}
-
 /* ------------------------------------------------------------------------------------ */
 /* Face fluxes                                                                           */
 /* ------------------------------------------------------------------------------------ */
@@ -183,10 +168,12 @@ typedef struct { double f[7]; } flux7;
 
 /* FLUXKE base3c.f:4753-5125 at one face (cell L, direction stride S).  `single` selects the
  * DERIV (base3c.f:719) lower-boundary branch I2 = I that FLUXKE's slab indexing takes on
- * every k-direction face (local index <= IL for the whole slab). */
+ * every k-direction face (local index <= IL for the whole slab).  `intet` is the MUSCL switch
+ * of rho*k and rho*omega and rklim/epslim their floors: FLUCYC passes 9, 0, 0
+ * (res3c.f:7680-7681, 7892), FLXCYC the INTET of RESIKE and RKLIM/EPSLIM (res3c.f:6946). */
 static flux7 roe_face(const ff_params *p, const ff_block *b, long L, long S, int intem,
                       const double *an, const double *anx, const double *any,
-                      const double *anz, int single)
+                      const double *anz, int single, int intet, double rklim, double epslim)
 {
     const double *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
                  *P = F(b, FF_P), *c = F(b, FF_C), *drdp = F(b, FF_DRDP),
@@ -200,8 +187,8 @@ static flux7 roe_face(const ff_params *p, const ff_block *b, long L, long S, int
     xxtrap(w, L, S, intem, &rwip, &rwim);
     xxtrap(P, L, S, intem, &ppi, &pmi);
     if (turb) {
-        xxtrap(F(b, FF_RK), L, S, 9, &rkip, &rkim);   /* INTET = 9: first order */
-        xxtrap(F(b, FF_REPS), L, S, 9, &reip, &reim);
+        xxtrap(F(b, FF_RK), L, S, intet, &rkip, &rkim);
+        xxtrap(F(b, FF_REPS), L, S, intet, &reip, &reim);
     }
     /* floors, base3c.f:4873-4876 */
     double fd = 0.001 * p->frsden, fp = 0.001 * p->frspre;
@@ -228,11 +215,11 @@ static flux7 roe_face(const ff_params *p, const ff_block *b, long L, long S, int
         if (fabs(dp) >= 1.E-2 * (ppi + pmi)) dedph = (eip + elr - eim - erl) / dp;
         if (fabs(dro) >= 1.E-2 * (roip + roim)) dedrh = (eip + erl - eim - elr) / dro;
     }
-    /* loop 600, base3c.f:4894-5120 (RKLIM = EPSLIM = 0 in FLUCYC res3c.f:7680-7681) */
-    double rkipi = rkip > 0.001 * 0. ? rkip : 0.001 * 0.;
-    double reipi = reip > 0.001 * 0. ? reip : 0.001 * 0.;
-    double rkimi = rkim > 0.001 * 0. ? rkim : 0.001 * 0.;
-    double reimi = reim > 0.001 * 0. ? reim : 0.001 * 0.;
+    /* loop 600, base3c.f:4894-5120; limits of the turbulence quantities base3c.f:4906-4909 */
+    double rkipi = 0.001 * rklim > rkip ? 0.001 * rklim : rkip;   /* MAX(0.001*RKLIM,RKIP) */
+    double reipi = 0.001 * epslim > reip ? 0.001 * epslim : reip;
+    double rkimi = 0.001 * rklim > rkim ? 0.001 * rklim : rkim;
+    double reimi = 0.001 * epslim > reim ? 0.001 * epslim : reim;
     double a11 = anx[L], a12 = any[L], a13 = anz[L];
     double a21, a22, a23, a31, a32, a33;
     surfvects(a11, a12, a13, &a21, &a22, &a23, &a31, &a32, &a33);
@@ -438,14 +425,23 @@ static flux7 inv_face(const ff_params *p, const ff_block *b, long L, long S, int
                       const double *anz, int single)
 {
     if (p->iflux == 0) return vl_face(p, b, L, S, intem, an, anx, any, anz);
-    return roe_face(p, b, L, S, intem, an, anx, any, anz, single);
+    return roe_face(p, b, L, S, intem, an, anx, any, anz, single, 9, 0., 0.);
+}
+
+/* MUSCL switch of the turbulence quantities, INTETI/J/K of RESIKE res3c.f:3634-3641
+ * (INTERTU = 0): limited (+1) unless the mean flow is first order */
+static int intet_of(int inter)
+{
+    return (inter > -5 && inter < 5) ? 1 : inter;
 }
 
 /* FLUXDO base3c.f:3677-3869 at one face, IDIFF <= 3 (arithmetic average).
- * Returns the momentum fluxes (which FLUXDO stores) and the energy increment added to FE. */
-static void visc_face(const ff_params *p, const ff_block *b, long L, long S,
-                      const double *an, const double *anx, const double *any,
-                      const double *anz, double *fm, double *fn, double *fw, double *fe)
+ * Returns the momentum fluxes (which FLUXDO stores) and the two energy terms FLUXDO adds to
+ * FE as FE = FE + F5VIG + SURFE*(TEMP(I) - TEMP(I-IL)): fe = F5VIG, fh = the heat flux. */
+static void visc_face2(const ff_params *p, const ff_block *b, long L, long S,
+                       const double *an, const double *anx, const double *any,
+                       const double *anz, double *fm, double *fn, double *fw, double *fe,
+                       double *fh)
 {
     const double *vol = F(b, FF_VOL), *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS),
                  *cp = F(b, FF_CP), *vist = F(b, FF_VIST), *ch = F(b, FF_CH),
@@ -461,13 +457,48 @@ static void visc_face(const ff_params *p, const ff_block *b, long L, long S,
     double f4 = surf * (w[L] - w[m] + anz[L] * duc);
     double f5 = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4);
     *fm = f2; *fn = f3; *fw = f4;
-    *fe = f5 + surfe * (t[L] - t[m]);
+    *fe = f5;
+    *fh = surfe * (t[L] - t[m]);
 }
 
-/* FLUCYC for one direction (dir 0=i, 1=j, 2=k) followed by SWEEP.  Face fluxes are formed
- * for the faces adjacent to interior cells only; the reference also forms fluxes on ghost
- * rows and columns (FLUXKE loop 1000), which no interior residual reads. */
-static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last)
+static void visc_face(const ff_params *p, const ff_block *b, long L, long S,
+                      const double *an, const double *anx, const double *any,
+                      const double *anz, double *fm, double *fn, double *fw, double *fe)
+{
+    double f5, fh;
+    visc_face2(p, b, L, S, an, anx, any, anz, fm, fn, fw, &f5, &fh);
+    *fe = f5 + fh;
+}
+
+/* FLUDKE base3c.f:5547-5651 at one face (thin-layer k and omega diffusion, SST blending
+ * with FUN1 of the face's upper cell).  Returns F6VIG and F7VIG. */
+static void kw_diff_face(const ff_block *b, long L, long S, const double *an, double psigk1,
+                         double psige1, double psigk2, double psige2, double *f6, double *f7)
+{
+    const double *vol = F(b, FF_VOL), *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS),
+                 *ro = F(b, FF_RO), *rk = F(b, FF_RK), *reps = F(b, FF_REPS),
+                 *fun1 = F(b, FF_FUN1);
+    long m = L - S;
+    double onemf1 = 0.0 > (1.0 - fun1[L]) ? 0.0 : (1.0 - fun1[L]);   /* MAX(0.0,(1.0-FUN1)) */
+    double psigk = fun1[L] * psigk1 + onemf1 * psigk2;
+    double psige = fun1[L] * psige1 + onemf1 * psige2;
+    double ypd2 = an[L] / (vol[L] + vol[m]);
+    double surg = -an[L] * ypd2;
+    double rmut2 = (eps2[L] - 1.) * vis[L] + (eps2[m] - 1.) * vis[m];
+    double surfk = surg * (vis[L] + vis[m] + rmut2 * psigk);
+    double surfep = surg * (vis[L] + vis[m] + rmut2 * psige);
+    double ypro1 = 1. / ro[L];
+    double ypro2 = 1. / ro[m];
+    *f6 = surfk * (rk[L] * ypro1 - rk[m] * ypro2);
+    *f7 = surfep * (reps[L] * ypro1 - reps[m] * ypro2);
+}
+
+/* FLUCYC (laminar, res3c.f:7572) or FLXCYC (two-equation, res3c.f:6424) for one direction
+ * (dir 0=i, 1=j, 2=k) followed by SWEEP/SWEEPK.  Face fluxes are formed for the faces
+ * adjacent to interior cells only; the reference also forms fluxes on ghost rows and
+ * columns (FLUXKE loop 1000), which no interior residual reads.  `kw` (SST) holds the
+ * FLUDKE sigmas PSIGK1, PSIGE1, PSIGK, PSIGE or is NULL. */
+static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last, const double *kw)
 {
     dims_t d = mkdims(b);
     long S = dir == 0 ? 1 : (dir == 1 ? d.is : d.il);
@@ -476,7 +507,8 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last)
                  *any = F(b, FF_A1Y + 4 * dir), *anz = F(b, FF_A1Z + 4 * dir);
     int intem = p->inter[dir];
     int turb = !(p->iturb < 3 || p->iturb == 8);
-    /* face flux arrays F1R..F1E (F1RK, F1EPS): zeroed (res3c.f:7705-7710) */
+    int nv = kw ? 7 : 5;
+    /* face flux arrays F1R..F1E (F1RK, F1EPS): zeroed (res3c.f:7705-7710, 6740-6748) */
     double *ff[7];
     for (int q = 0; q < 7; q++) ff[q] = (double *)calloc((size_t)d.ntot, sizeof(double));
     int lo[3] = {1, 1, 1}, hi[3] = {nd[0], nd[1], nd[2]};
@@ -485,13 +517,23 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last)
         for (int j = lo[1]; j <= hi[1]; j++)
             for (int i = lo[0]; i <= hi[0]; i++) {
                 long L = cid(&d, i, j, k);
-                double fm, fn, fw, fe;
-                /* FLUXDO first (res3c.f:7856), FRO untouched by FLUXDO */
-                visc_face(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &fe);
+                double fm, fn, fw, f5, fh;
+                if (kw) {
+                    /* FLUDKE (res3c.f:6817): FRK = FE = F6VIG, FEPS = F7VIG; then
+                       SETV12(F1E,F1RK) (res3c.f:6837) */
+                    double f6, f7;
+                    kw_diff_face(b, L, S, an, kw[0], kw[1], kw[2], kw[3], &f6, &f7);
+                    ff[5][L] = f6; ff[6][L] = f7; ff[4][L] = f6;
+                    ff[4][L] = ff[5][L];
+                }
+                /* FLUXDO (res3c.f:7856, 6867), FRO untouched by FLUXDO */
+                visc_face2(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &f5, &fh);
                 ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
-                ff[4][L] = ff[4][L] + fe;
-                /* then FLUXKE / FLUXVL (res3c.f:7892, 7907) adds the inviscid flux */
-                flux7 r = inv_face(p, b, L, S, intem, an, anx, any, anz, dir == 2);
+                ff[4][L] = ff[4][L] + f5 + fh;
+                /* then FLUXKE / FLUXVL (res3c.f:7892, 7907; 6946) adds the inviscid flux */
+                flux7 r = kw ? roe_face(p, b, L, S, intem, an, anx, any, anz, dir == 2,
+                                        intet_of(intem), p->rklim, p->epslim)
+                             : inv_face(p, b, L, S, intem, an, anx, any, anz, dir == 2);
                 ff[0][L] = r.f[0] + ff[0][L];
                 ff[1][L] = r.f[1] + ff[1][L];
                 ff[2][L] = r.f[2] + ff[2][L];
@@ -502,33 +544,401 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last)
                     ff[6][L] = r.f[6] + ff[6][L];
                 }
             }
-    /* SWEEP res3c.f:992-1036 */
-    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    /* SWEEP res3c.f:992-1036 / SWEEPK res3c.f:1368-1426 */
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
     const double *vol = F(b, FF_VOL);
     for (int k = 1; k <= d.kmax; k++)
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {
                 long n = cid(&d, i, j, k);
                 if (!last) {
-                    for (int q = 0; q < 5; q++) D[q][n] = D[q][n] - (ff[q][n + S] - ff[q][n]);
+                    for (int q = 0; q < nv; q++) D[q][n] = D[q][n] - (ff[q][n + S] - ff[q][n]);
                 } else {
                     double dt1 = 1. / vol[n];
-                    for (int q = 0; q < 5; q++) D[q][n] = (D[q][n] - (ff[q][n + S] - ff[q][n])) * dt1;
+                    for (int q = 0; q < nv; q++) D[q][n] = (D[q][n] - (ff[q][n + S] - ff[q][n])) * dt1;
                 }
             }
     for (int q = 0; q < 7; q++) free(ff[q]);
 }
 
-/* RESIDU res3c.f:6-787 (laminar): zero the residuals, k-, i- and j-direction fluxes with a
- * SWEEP after each, the last one dividing by VOL; then RESL2 sums on level 1. */
+/* ------------------------------------------------------------------------------------ */
+/* k-omega SST (ITURB = 6, IDIS = 2, KOVER = 0: SST; KATOL = YES; no transition, no DES) */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    double a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2;
+} sst_t;
+
+/* COEFKO turb3.f:8068-8108 (ITURB = 6) */
+static sst_t coefko(int kover)
+{
+    sst_t s;
+    s.a1kleb = 0.31; s.bstar = 0.09; s.ckappa = 0.41; s.beta1 = 0.075;
+    s.sigrk1 = kover < 5 ? 0.85 : 0.5;
+    s.sigom1 = 0.5; s.beta2 = 0.0828; s.sigrk2 = 1.0; s.sigom2 = 0.856;
+    return s;
+}
+
+/* X**4 as the reference build evaluates it: ((X*X)*X)*X */
+static inline double pow4(double x) { return x * x * x * x; }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
+/* DERFUN turb3.f:2476-2665 with IDERI = 1 (distance-weighted Green-Gauss) on level 1,
+ * first ghost layer included: vorticity, strain-rate tensor, |Omega| (OHMI) and the strain
+ * magnitude (VELLAP, used only by the SAS variant, is not formed). */
+static void derfun(ff_block *b)
+{
+    dims_t d = mkdims(b);
+    long IS = d.is, IL = d.il;
+    const double *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W), *vol = F(b, FF_VOL),
+                 *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *d1 = F(b, FF_D1), *d2 = F(b, FF_D2), *d3 = F(b, FF_D3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z);
+    for (int k = 0; k <= d.kmax + 1; k++)
+        for (int j = 0; j <= d.jmax + 1; j++)
+            for (int i = 0; i <= d.imax + 1; i++) {
+                long L = cid(&d, i, j, k);
+                double w1, w2;
+                w1 = 2.0 * (d1[L] / (d1[L] + d1[L - 1])); w2 = 2.0 - w1;
+                double u1m = a1[L] * (w2 * u[L] + w1 * u[L - 1]);
+                double v1m = a1[L] * (w2 * v[L] + w1 * v[L - 1]);
+                double w1m = a1[L] * (w2 * w[L] + w1 * w[L - 1]);
+                w1 = 2.0 * (d1[L] / (d1[L] + d1[L + 1])); w2 = 2.0 - w1;
+                double u1p = a1[L + 1] * (w2 * u[L] + w1 * u[L + 1]);
+                double v1p = a1[L + 1] * (w2 * v[L] + w1 * v[L + 1]);
+                double w1p = a1[L + 1] * (w2 * w[L] + w1 * w[L + 1]);
+                w1 = 2.0 * (d2[L] / (d2[L] + d2[L - IS])); w2 = 2.0 - w1;
+                double u2m = a2[L] * (w2 * u[L] + w1 * u[L - IS]);
+                double v2m = a2[L] * (w2 * v[L] + w1 * v[L - IS]);
+                double w2m = a2[L] * (w2 * w[L] + w1 * w[L - IS]);
+                w1 = 2.0 * (d2[L] / (d2[L] + d2[L + IS])); w2 = 2.0 - w1;
+                double u2p = a2[L + IS] * (w2 * u[L] + w1 * u[L + IS]);
+                double v2p = a2[L + IS] * (w2 * v[L] + w1 * v[L + IS]);
+                double w2p = a2[L + IS] * (w2 * w[L] + w1 * w[L + IS]);
+                w1 = 2.0 * (d3[L] / (d3[L] + d3[L - IL])); w2 = 2.0 - w1;
+                double u3m = a3[L] * (w2 * u[L] + w1 * u[L - IL]);
+                double v3m = a3[L] * (w2 * v[L] + w1 * v[L - IL]);
+                double w3m = a3[L] * (w2 * w[L] + w1 * w[L - IL]);
+                w1 = 2.0 * (d3[L] / (d3[L] + d3[L + IL])); w2 = 2.0 - w1;
+                double u3p = a3[L + IL] * (w2 * u[L] + w1 * u[L + IL]);
+                double v3p = a3[L + IL] * (w2 * v[L] + w1 * v[L + IL]);
+                double w3p = a3[L + IL] * (w2 * w[L] + w1 * w[L + IL]);
+                double pvol = .5 / vol[L];
+#define GG(cx, q3p, q3m, q2p, q2m, q1p, q1m)                                              \
+    (pvol * (cx##3[L + IL] * q3p - cx##3[L] * q3m + cx##2[L + IS] * q2p - cx##2[L] * q2m + \
+             cx##1[L + 1] * q1p - cx##1[L] * q1m))
+                const double *ax1 = a1x, *ax2 = a2x, *ax3 = a3x;
+                const double *ay1 = a1y, *ay2 = a2y, *ay3 = a3y;
+                const double *az1 = a1z, *az2 = a2z, *az3 = a3z;
+                double dudx = GG(ax, u3p, u3m, u2p, u2m, u1p, u1m);
+                double dudy = GG(ay, u3p, u3m, u2p, u2m, u1p, u1m);
+                double dudz = GG(az, u3p, u3m, u2p, u2m, u1p, u1m);
+                double dvdx = GG(ax, v3p, v3m, v2p, v2m, v1p, v1m);
+                double dvdy = GG(ay, v3p, v3m, v2p, v2m, v1p, v1m);
+                double dvdz = GG(az, v3p, v3m, v2p, v2m, v1p, v1m);
+                double dwdx = GG(ax, w3p, w3m, w2p, w2m, w1p, w1m);
+                double dwdy = GG(ay, w3p, w3m, w2p, w2m, w1p, w1m);
+                double dwdz = GG(az, w3p, w3m, w2p, w2m, w1p, w1m);
+#undef GG
+                double w12 = .5 * (dudy - dvdx), w13 = .5 * (dudz - dwdx), w23 = .5 * (dvdz - dwdy);
+                F(b, FF_W12)[L] = w12; F(b, FF_W13)[L] = w13; F(b, FF_W23)[L] = w23;
+                F(b, FF_OHMI)[L] = 2. * sqrt(w12 * w12 + w13 * w13 + w23 * w23);
+                double s11 = .5 * (dudx + dudx), s22 = .5 * (dvdy + dvdy), s33 = .5 * (dwdz + dwdz);
+                double s12 = .5 * (dudy + dvdx), s13 = .5 * (dudz + dwdx), s23 = .5 * (dvdz + dwdy);
+                double st = s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 + s23 * s23
+                          + s13 * s13 + s23 * s23 + s33 * s33;
+                F(b, FF_STRAIN)[L] = sqrt(2. * st);
+                F(b, FF_S11)[L] = s11; F(b, FF_S12)[L] = s12; F(b, FF_S13)[L] = s13;
+                F(b, FF_S22)[L] = s22; F(b, FF_S23)[L] = s23; F(b, FF_S33)[L] = s33;
+            }
+}
+
+/* TTIMES earsm.f:629-692 (ITURB = 6), first ghost layer included */
+static void ttimes(ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double ct = 6.0, bstar = 0.09, eps = 1.E-20;
+    const double *ro = F(b, FF_RO), *vis = F(b, FF_VIS), *rk = F(b, FF_RK), *reps = F(b, FF_REPS);
+    for (int k = 0; k <= d.kmax + 1; k++)
+        for (int j = 0; j <= d.jmax + 1; j++)
+            for (int i = 0; i <= d.imax + 1; i++) {
+                long L = cid(&d, i, j, k);
+                double bnum = 1.0 / fabs(bstar * reps[L] + eps);
+                double tstur = ro[L] * bnum;
+                double tsvis = ct * sqrt(ro[L] * vis[L] * bnum / fabs(rk[L] + eps));
+                F(b, FF_TTS)[L] = dmax(tstur, tsvis);
+            }
+}
+
+/* EPS2KO turb3.f:7454-7537: SST eddy viscosity on the interior, with the strain magnitude
+ * as the limiter argument (TURBEQ res3c.f:6366-6371 for KOVER = 0 or 5) and FRSMUT =
+ * 1E-3*FRSVIS (res3c.f:5922) */
+static void eps2ko(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    sst_t s = coefko(p->kover);
+    const double eps = 1.E-20;
+    const double sst = p->kover < 5 ? 1.0 : 0.0, bsl = p->kover < 5 ? 0.0 : 1.0;
+    const double frsmut = 1.E-3 * p->frsvis;
+    const double *ro = F(b, FF_RO), *rk = F(b, FF_RK), *rom = F(b, FF_REPS),
+                 *vis = F(b, FF_VIS), *distw = F(b, FF_DISTW), *ohmi = F(b, FF_STRAIN);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long L = cid(&d, i, j, k);
+                double prol = 1. / ro[L];
+                double rkl = dmax(rk[L] * prol, 0.0);
+                double oml = rom[L] * prol;
+                double dis2 = distw[L] * distw[L];
+                double arg21 = 2. * sqrt(rkl) / dmax(eps, s.bstar * oml * distw[L]);
+                double auxvi = vis[L] / dmax(eps, rom[L] * dis2);
+                double arg22 = 500. * auxvi;
+                double arg2 = dmax(arg21, arg22);
+                double arg4 = 150. * auxvi;
+                double fun2 = tanh(arg2 * arg2);
+                double fun3 = 1. - tanh(pow4(arg4));
+                double rmubas = rk[L] / oml;
+                double rmumen = s.a1kleb * rk[L] / dmax(eps, ohmi[L] * fun2 * fun3);
+                double rmutrb = dmin(rmubas, rmumen);
+                rmutrb = rmutrb * sst + rmubas * bsl;
+                rmutrb = dmax(rmutrb, frsmut);
+                rmutrb = dmin(rmutrb, p->turlim * vis[L]);
+                F(b, FF_VIST)[L] = rmutrb;
+                double rmutpv = rmutrb / vis[L];
+                F(b, FF_EPS2)[L] = 1.0 + rmutpv;
+            }
+}
+
+/* CROSKO turb3.f:7790-7899: cross-diffusion 2*SIGOM2*RO*grad(k).grad(omega)/omega on the
+ * interior (into SEPS).  Like the reference it converts RK and REPS to k and omega on every
+ * cell and back again, which rounds them: (RK*(1/RO))*RO. */
+static void crosko(const sst_t *s, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    long IS = d.is, IL = d.il;
+    const double sigo22 = 2.0 * s->sigom2;
+    double *rk = F(b, FF_RK), *om = F(b, FF_REPS), *cross = F(b, FF_SEPS);
+    const double *ro = F(b, FF_RO), *vol = F(b, FF_VOL),
+                 *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z);
+    for (long n = 0; n < d.ntot; n++) {
+        double prol = 1. / ro[n];
+        rk[n] = rk[n] * prol;
+        om[n] = om[n] * prol;
+    }
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long L = cid(&d, i, j, k);
+                double rk1m = a1[L] * (rk[L] + rk[L - 1]), rk1p = a1[L + 1] * (rk[L] + rk[L + 1]);
+                double om1m = a1[L] * (om[L] + om[L - 1]), om1p = a1[L + 1] * (om[L] + om[L + 1]);
+                double rk2m = a2[L] * (rk[L] + rk[L - IS]), rk2p = a2[L + IS] * (rk[L] + rk[L + IS]);
+                double om2m = a2[L] * (om[L] + om[L - IS]), om2p = a2[L + IS] * (om[L] + om[L + IS]);
+                double rk3m = a3[L] * (rk[L] + rk[L - IL]), rk3p = a3[L + IL] * (rk[L] + rk[L + IL]);
+                double om3m = a3[L] * (om[L] + om[L - IL]), om3p = a3[L + IL] * (om[L] + om[L + IL]);
+                double pvol = 0.5 / vol[L];
+#define GG(c1, c2, c3, q3p, q3m, q2p, q2m, q1p, q1m)                                    \
+    (pvol * (c3[L + IL] * q3p - c3[L] * q3m + c2[L + IS] * q2p - c2[L] * q2m + c1[L + 1] * q1p - \
+             c1[L] * q1m))
+                double dkdx = GG(a1x, a2x, a3x, rk3p, rk3m, rk2p, rk2m, rk1p, rk1m);
+                double dkdy = GG(a1y, a2y, a3y, rk3p, rk3m, rk2p, rk2m, rk1p, rk1m);
+                double dkdz = GG(a1z, a2z, a3z, rk3p, rk3m, rk2p, rk2m, rk1p, rk1m);
+                double dodx = GG(a1x, a2x, a3x, om3p, om3m, om2p, om2m, om1p, om1m);
+                double dody = GG(a1y, a2y, a3y, om3p, om3m, om2p, om2m, om1p, om1m);
+                double dodz = GG(a1z, a2z, a3z, om3p, om3m, om2p, om2m, om1p, om1m);
+#undef GG
+                double divdot = dkdx * dodx + dkdy * dody + dkdz * dodz;
+                cross[L] = sigo22 * ro[L] * divdot / om[L];
+            }
+    for (long n = 0; n < d.ntot; n++) {
+        rk[n] = rk[n] * ro[n];
+        om[n] = om[n] * ro[n];
+    }
+}
+
+/* FUN1KO turb3.f:7385-7448: SST blending function F1 on the interior.  The cross-diffusion
+ * floor CDMIN = 1E-8 * max(0, CROSSD) over the reference's linear index range from the
+ * first to the last interior cell (ghost cells in between included). */
+static void fun1ko(const ff_params *p, const sst_t *s, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double eps = 1.E-20, sigo24 = 4. * s->sigom2;
+    const double *cross = F(b, FF_SEPS), *vis = F(b, FF_VIS), *ro = F(b, FF_RO),
+                 *rk = F(b, FF_RK), *om = F(b, FF_REPS), *distw = F(b, FF_DISTW);
+    double cdmax = 0.0;
+    for (long L = cid(&d, 1, 1, 1); L <= cid(&d, d.imax, d.jmax, d.kmax); L++)
+        cdmax = dmax(cross[L], cdmax);
+    double cdmin = 1E-8 * cdmax;
+    double sqrtre = sqrt(p->re);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long L = cid(&d, i, j, k);
+                double prol = 1. / ro[L];
+                double rkl = dmax(rk[L] * prol, 0.0);
+                double oml = om[L] * prol;
+                double dis2 = distw[L] * distw[L];
+                double cdkom = dmax(cross[L], cdmin);
+                double arg11 = sqrt(rkl) / dmax(eps, s->bstar * oml * distw[L]);
+                double arg12 = 500. * vis[L] / dmax(eps, om[L] * dis2);
+                double arg13 = dmax(arg11, arg12);
+                double arg14 = sigo24 * rk[L] / dmax(eps, cdkom * dis2);
+                double arg1 = dmin(arg13, arg14);
+                double f1 = tanh(pow4(arg1));
+                double dislim = .40559E-7 * sqrtre;
+                if (distw[L] < dislim) f1 = 1.;
+                F(b, FF_FUN1)[L] = f1;
+            }
+}
+
+/* MAKE_EPS turb3.f:7905-7921 (ITURB = 6): the true dissipation 0.09*RHO*OMEGA*K/RHO */
+static void make_eps(const ff_block *b, double *eps)
+{
+    dims_t d = mkdims(b);
+    const double *ro = F(b, FF_RO), *rk = F(b, FF_RK), *reps = F(b, FF_REPS);
+    for (long n = 0; n < d.ntot; n++) eps[n] = 0.09 * reps[n] * rk[n] / ro[n];
+}
+
+/* PRODKE turb3.f:4687-4795, eddy-viscosity branch with the Kato-Launder form (KATOL) or the
+ * full form, limited by 20 times the true dissipation; VTRAN = 1 (main.f:7452).  The
+ * reference's loop covers interior K-planes and J-rows with every I (ghost columns too). */
+static void prodke(const ff_params *p, ff_block *b, const double *trueps)
+{
+    dims_t d = mkdims(b);
+    const double r2o3 = 2. / 3.;
+    const double *s11 = F(b, FF_S11), *s12 = F(b, FF_S12), *s13 = F(b, FF_S13),
+                 *s22 = F(b, FF_S22), *s23 = F(b, FF_S23), *s33 = F(b, FF_S33),
+                 *w12 = F(b, FF_W12), *w13 = F(b, FF_W13), *w23 = F(b, FF_W23),
+                 *ohmi = F(b, FF_OHMI), *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS),
+                 *rk = F(b, FF_RK);
+    double *ptur = F(b, FF_PTUR);
+    for (int k = 1; k <= d.kmax; k++) {
+        long ia = (long)(k + 1) * d.il + (long)G * d.is;
+        for (long jj = 0; jj < (long)d.jmax * d.is; jj++) {
+            long L = ia + jj;
+            double dudx = s11[L], dvdy = s22[L], dwdz = s33[L];
+            double dudy = s12[L] + w12[L], dudz = s13[L] + w13[L];
+            double dvdx = s12[L] - w12[L], dvdz = s23[L] + w23[L];
+            double dwdx = s13[L] - w13[L], dwdy = s23[L] - w23[L];
+            double strain = s11[L] * s11[L] + s12[L] * s12[L] + s13[L] * s13[L]
+                          + s12[L] * s12[L] + s22[L] * s22[L] + s23[L] * s23[L]
+                          + s13[L] * s13[L] + s23[L] * s23[L] + s33[L] * s33[L];
+            strain = sqrt(2. * strain);
+            double trans = 0.;
+            double div = dudx + dvdy + dwdz;
+            double visco = (eps2[L] - 1. + trans) * vis[L];
+            double pt;
+            if (p->katol)
+                pt = visco * strain * ohmi[L];
+            else
+                pt = visco * (2. * (dudx * dudx + dvdy * dvdy + dwdz * dwdz)
+                              + (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx)
+                              + (dvdz + dwdy) * (dvdz + dwdy) - r2o3 * (div * div))
+                     - r2o3 * rk[L] * div;
+            pt = dmin(pt, 20. * trueps[L]);
+            ptur[L] = pt * 1.;
+        }
+    }
+}
+
+/* SOURKO turb3.f:7624-7784 (RANS branch, KOVER 0/5: no rotation correction, FCO = 1;
+ * TUR_FRS_SOURCE = no, main.f:5267) on the interior: SRK, and SEPS = (1-F1)*SEPS + the
+ * omega source, SEPS holding the cross-diffusion on entry */
+static void sourko(const ff_params *p, const sst_t *s, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double eps = 1.0E-10;
+    const double pbstar = 1.0 / s->bstar, persbs = 1.0 / sqrt(s->bstar);
+    const double rotco = 0.0, rich = 0.0, bero = 3.6;
+    const double *fun1 = F(b, FF_FUN1), *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS),
+                 *ro = F(b, FF_RO), *rk = F(b, FF_RK), *rom = F(b, FF_REPS),
+                 *ptur = F(b, FF_PTUR);
+    double *srk = F(b, FF_SRK), *som = F(b, FF_SEPS);
+    (void)p;
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long L = cid(&d, i, j, k);
+                double onemf1 = dmax(0.0, (1.0 - fun1[L]));
+                double beta = fun1[L] * s->beta1 + onemf1 * s->beta2;
+                double sigom = fun1[L] * s->sigom1 + onemf1 * s->sigom2;
+                double gamkom = beta * pbstar - sigom * (s->ckappa * s->ckappa) * persbs;
+                double rmutrb = dmax(eps, eps2[L] - 1.0) * vis[L];
+                double fco = dmax(0.0, (1.0 - rotco)) + dmax(0.0, rotco / (1.0 + bero * rich));
+                double trueomega = rom[L];
+                double prol = 1.0 / ro[L];
+                som[L] = onemf1 * som[L];
+                srk[L] = ptur[L] - s->bstar * rk[L] * trueomega * prol;
+                double omepro = gamkom * ro[L] * ptur[L] / (rmutrb * rom[L]);
+                double omedis = fco * beta * prol * trueomega;
+                som[L] = som[L] + rom[L] * (omepro - omedis);
+            }
+}
+
+/* TURBDI (ns3c.f:2874): ITURB = 0 -> ADJUIN (res3c.f:9447-9467) sets EPS2 = 1 and VIST = 0 on
+ * the interior cells (ns3c.f:2987-2991), ghost values are the caller's boundary values.
+ * ITURB = 6 -> TURBEQ res3c.f:5845-6418: DERFUN, TTIMES, EPS2KO. */
+void ffo_turb(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    if (p->iturb == 6) {
+        derfun(b);
+        ttimes(b);
+        eps2ko(p, b);
+        return;
+    }
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                F(b, FF_EPS2)[n] = 1.;
+                F(b, FF_VIST)[n] = 0.;
+            }
+}
+
+/* RESIDU res3c.f:6-787 (laminar) or RESIKE res3c.f:3423-5387 (SST): zero the residuals,
+ * k-, i- and j-direction fluxes with a SWEEP after each, the last one dividing by VOL; SST
+ * adds CROSKO/FUN1KO/PRODKE before the fluxes (res3c.f:3817-3994) and SOURKO/ADDSKE after
+ * (res3c.f:4957-5068); then RESL2 sums on level 1 (res3c.f:8964-9006). */
 void ffo_residual(const ff_params *p, ff_block *b, int level, double *resi)
 {
     dims_t d = mkdims(b);
+    int kwm = p->iturb == 6;
     for (int q = FF_DRO; q <= FF_DEPS; q++)
         if (F(b, q)) memset(F(b, q), 0, sizeof(double) * (size_t)d.ntot);
-    direction_sweep(p, b, 2, 0);   /* res3c.f:270 then SWEEP res3c.f:330 */
-    direction_sweep(p, b, 0, 0);   /* res3c.f:402 then SWEEP res3c.f:448 */
-    direction_sweep(p, b, 1, 1);   /* res3c.f:515 then SWEEP res3c.f:559 */
+    double sig[4];
+    sst_t s = coefko(p->kover);
+    if (kwm) {
+        memset(F(b, FF_SRK), 0, sizeof(double) * (size_t)d.ntot);
+        memset(F(b, FF_FUN1), 0, sizeof(double) * (size_t)d.ntot);
+        sig[0] = s.sigrk1; sig[1] = s.sigom1; sig[2] = s.sigrk2; sig[3] = s.sigom2;
+        crosko(&s, b);
+        fun1ko(p, &s, b);
+        double *trueps = malloc(sizeof(double) * (size_t)d.ntot);
+        make_eps(b, trueps);
+        prodke(p, b, trueps);
+        free(trueps);
+    }
+    direction_sweep(p, b, 2, 0, kwm ? sig : NULL);   /* res3c.f:270 / 4018, SWEEP(K) */
+    direction_sweep(p, b, 0, 0, kwm ? sig : NULL);   /* res3c.f:402 / 4274 */
+    direction_sweep(p, b, 1, 1, kwm ? sig : NULL);   /* res3c.f:515 / 4499 */
+    if (kwm) {
+        sourko(p, &s, b);
+        /* ADDSKE res3c.f:1494-1502 */
+        for (int k = 1; k <= d.kmax; k++)
+            for (int j = 1; j <= d.jmax; j++)
+                for (int i = 1; i <= d.imax; i++) {
+                    long n = cid(&d, i, j, k);
+                    F(b, FF_DRK)[n] = F(b, FF_DRK)[n] + F(b, FF_SRK)[n];
+                    F(b, FF_DEPS)[n] = F(b, FF_DEPS)[n] + F(b, FF_SEPS)[n];
+                }
+    }
     if (level == 1 && resi) {
         /* RESL2 res3c.f:8979-9006 */
         for (int q = 0; q < 7; q++) resi[q] = 0.;
@@ -541,6 +951,11 @@ void ffo_residual(const ff_params *p, ff_block *b, int level, double *resi)
                     resi[2] += F(b, FF_DN)[n] * F(b, FF_DN)[n];
                     resi[3] += F(b, FF_DW)[n] * F(b, FF_DW)[n];
                     resi[4] += F(b, FF_DE)[n] * F(b, FF_DE)[n];
+                    if (kwm) {
+                        double depsia = dmin(fabs(F(b, FF_DEPS)[n]), 1.E15);
+                        resi[5] += F(b, FF_DRK)[n] * F(b, FF_DRK)[n];
+                        resi[6] += depsia * depsia;
+                    }
                 }
     }
 }
@@ -661,8 +1076,66 @@ static void face_contrib(const double dw[5], double c, double ro, double nx, dou
           a21, a22, a23, a31, a32, a33);
 }
 
+/* SYIKE imps4.f:982-1035 and SYM1KE imps4.f:1041-1072 around the split-eigenvalue scaling:
+ * the seven-variable (k-omega) form of face_contrib used by PRKEGS/COKEGS */
+static void face_contrib_ke(const double dw[7], double c, double ro, double rk, double drdp,
+                            double drdh, double nx, double ny, double nz, double x1,
+                            double x2, double x5, double out[7])
+{
+    double a21, a22, a23, a31, a32, a33;
+    surfvects(nx, ny, nz, &a21, &a22, &a23, &a31, &a32, &a33);
+    double z1 = dw[0], z5 = dw[4], z6 = dw[5], z7 = dw[6];
+    double z2 = nx * dw[1] + ny * dw[2] + nz * dw[3];
+    double z3 = a21 * dw[1] + a22 * dw[2] + a23 * dw[3];
+    double z4 = a31 * dw[1] + a32 * dw[2] + a33 * dw[3];
+    double roc = ro * c;
+    double pc2 = 1. / ro;
+    double dpres = z5;
+    double w1 = z1 - pc2 * dpres;
+    double w2 = roc * z2 + dpres;
+    double w3 = z3, w4 = z4, w6 = z6, w7 = z7;
+    double w5 = -roc * z2 + dpres;
+    w1 = x1 * w1;
+    w2 = x2 * w2;
+    w3 = x1 * w3;
+    w4 = x1 * w4;
+    w5 = x5 * w5;
+    w6 = x1 * w6;
+    w7 = x1 * w7;
+    double ypro = 1. / ro;
+    double pc2b = .5 * ypro;
+    double proc = pc2b / c;
+    double drh = .667 * rk * (ypro * ypro) * drdh;
+    double ydrp = 1. / (1. + .667 * rk * ypro * drdp);
+    double dh = w1 + pc2b * (w2 + w5);
+    double x2i = proc * (w2 - w5);
+    double x3i = w3, x4i = w4;
+    double x5v = .5 * (w2 + w5);
+    x5v = ydrp * (x5v - drh * ro * dh - .667 * ro * w6);
+    out[0] = dh;
+    out[1] = nx * x2i + a21 * x3i + a31 * x4i;
+    out[2] = ny * x2i + a22 * x3i + a32 * x4i;
+    out[3] = nz * x2i + a23 * x3i + a33 * x4i;
+    out[4] = x5v;
+    out[5] = w6;
+    out[6] = w7;
+}
+
+/* face_contrib (5 variables) or face_contrib_ke (7) for cell n across a face with unit
+ * normal (nx, ny, nz) */
+static void face_xfer(int ke, const ff_block *b, long n, const double *dw, double nx,
+                      double ny, double nz, double x1, double x2, double x5, double *out)
+{
+    const double *c = F(b, FF_C), *ro = F(b, FF_RO);
+    if (ke)
+        face_contrib_ke(dw, c[n], ro[n], F(b, FF_RK)[n], F(b, FF_DRDP)[n], F(b, FF_DRDH)[n],
+                        nx, ny, nz, x1, x2, x5, out);
+    else
+        face_contrib(dw, c[n], ro[n], nx, ny, nz, x1, x2, x5, out);
+}
+
 /* PREDGS imps4.f:345-598: backward sweep, K = KMAX..1, diagonals M = I+J descending */
-static void predgs(const ff_params *p, ff_block *b)
+static void predgs(const ff_params *p, ff_block *b, int ke)
 {
     dims_t d = mkdims(b);
     const double cdiff = 2.;
@@ -674,15 +1147,17 @@ static void predgs(const ff_params *p, ff_block *b)
                  *vol = F(b, FF_VOL), *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V),
                  *w = F(b, FF_W), *c = F(b, FF_C), *vis = F(b, FF_VIS), *vist = F(b, FF_VIST),
                  *dtl = F(b, FF_DTL);
-    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
+    const int NV = ke ? 7 : 5;
     long slab = (long)d.jmax * IS;
-    double *cd = calloc((size_t)slab * 5, sizeof(double));          /* CDRO..CDE(L) */
+    double *cd = calloc((size_t)slab * NV, sizeof(double));          /* CDRO..CDE(L) */
     double *xi = malloc(sizeof(double) * (size_t)slab * 9);          /* XI1,2,5 XJ.. XK.. */
     double *dt = malloc(sizeof(double) * (size_t)slab);
     double *scinv = malloc(sizeof(double) * (size_t)slab);
     double *scinp = malloc(sizeof(double) * (size_t)slab);
-    double *ydi = calloc((size_t)(d.imax + 2) * 5, sizeof(double));   /* YD*I(I) */
-    double *bd = calloc((size_t)(d.imax + 2) * 5, sizeof(double));    /* BD*(I) */
+    double *ydi = calloc((size_t)(d.imax + 2) * NV, sizeof(double));   /* YD*I(I) */
+    double *bd = calloc((size_t)(d.imax + 2) * NV, sizeof(double));    /* BD*(I) */
     for (int k = d.kmax; k >= 1; k--) {
         double ffk = 2.;
         if (k == 1) ffk = 4.;
@@ -730,14 +1205,14 @@ static void predgs(const ff_params *p, ff_block *b)
                 xka = rkmax + t;
                 double dt1 = p->theta * dtl[n];
                 dt[l] = dt1;
-                for (int q = 0; q < 5; q++)
-                    cd[5 * l + q] = -dt1 * a3[n + IL] * cd[5 * l + q] + vol[n] * D[q][n];
+                for (int q = 0; q < NV; q++)
+                    cd[NV * l + q] = -dt1 * a3[n + IL] * cd[NV * l + q] + vol[n] * D[q][n];
                 scinv[l] = 1. / (vol[n] + dt1 * (xia * a1[n] + xja * a2[n] + xka * a3[n]) + vol[n] * 0.);
                 scinp[l] = 1. / (vol[n] + dt1 * (rimax * a1[n] + rjmax * a2[n] + rkmax * a3[n]) + vol[n] * 0.);
             }
         }
-        memset(ydi, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
-        memset(bd, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
+        memset(ydi, 0, sizeof(double) * (size_t)(d.imax + 2) * NV);
+        memset(bd, 0, sizeof(double) * (size_t)(d.imax + 2) * NV);
         for (int m = d.imax + d.jmax; m >= 2; m--) {
             int ialku = m - d.jmax > 1 ? m - d.jmax : 1;
             int iloppu = d.imax < m - 1 ? d.imax : m - 1;
@@ -746,20 +1221,23 @@ static void predgs(const ff_params *p, ff_block *b)
                 long n = cid(&d, i, j, k);
                 long l = (long)(j - 1) * IS + G + i - 1;
                 double dt1 = dt[l];
-                for (int q = 0; q < 5; q++)
-                    cd[5 * l + q] = cd[5 * l + q] - dt1 * (a1[n + 1] * ydi[5 * i + q] + a2[n + IS] * bd[5 * i + q]);
-                for (int q = 0; q < 4; q++) cd[5 * l + q] = cd[5 * l + q] * scinv[l];
-                cd[5 * l + 4] = cd[5 * l + 4] * scinp[l];
-                double dw[5];
-                for (int q = 0; q < 5; q++) { D[q][n] = cd[5 * l + q]; dw[q] = D[q][n]; }
-                face_contrib(dw, c[n], ro[n], a1x[n], a1y[n], a1z[n],
-                             xi[9 * l + 0], xi[9 * l + 1], xi[9 * l + 2], &ydi[5 * i]);
-                face_contrib(dw, c[n], ro[n], a2x[n], a2y[n], a2z[n],
-                             xi[9 * l + 3], xi[9 * l + 4], xi[9 * l + 5], &bd[5 * i]);
+                for (int q = 0; q < NV; q++)
+                    cd[NV * l + q] = cd[NV * l + q] - dt1 * (a1[n + 1] * ydi[NV * i + q] + a2[n + IS] * bd[NV * i + q]);
+                for (int q = 0; q < NV; q++)   /* SCINP for the pressure, SCINV otherwise */
+                    cd[NV * l + q] = cd[NV * l + q] * (q == 4 ? scinp[l] : scinv[l]);
+                double dw[7];
+                for (int q = 0; q < NV; q++) { D[q][n] = cd[NV * l + q]; dw[q] = D[q][n]; }
+                face_xfer(ke, b, n, dw, a1x[n], a1y[n], a1z[n],
+                             xi[9 * l + 0], xi[9 * l + 1], xi[9 * l + 2], &ydi[NV * i]);
+                face_xfer(ke, b, n, dw, a2x[n], a2y[n], a2z[n],
+                             xi[9 * l + 3], xi[9 * l + 4], xi[9 * l + 5], &bd[NV * i]);
             }
-            for (int i = ialku; i <= iloppu - 1; i++)
-                for (int q = 0; q < 5; q++) ydi[5 * i + q] = ydi[5 * (i + 1) + q];
-            for (int q = 0; q < 5; q++) ydi[5 * iloppu + q] = 0.;
+            /* line-buffer shift, loop 1460: PREDGS starts it at IALKU (imps4.f:556), so a cell
+               of the row J = JMAX never receives its I+1 contribution; PRKEGS starts it at
+               MAX(IALKU-1,1) (imps4.f:1762) and does hand it over */
+            for (int i = ke ? (ialku - 1 > 1 ? ialku - 1 : 1) : ialku; i <= iloppu - 1; i++)
+                for (int q = 0; q < NV; q++) ydi[NV * i + q] = ydi[NV * (i + 1) + q];
+            for (int q = 0; q < NV; q++) ydi[NV * iloppu + q] = 0.;
         }
         /* loop 1600: k-direction contribution for the next slab (interior cells only;
            ghost-column entries are never read by interior cells) */
@@ -767,17 +1245,17 @@ static void predgs(const ff_params *p, ff_block *b)
             for (int i = 1; i <= d.imax; i++) {
                 long n = cid(&d, i, j, k);
                 long l = (long)(j - 1) * IS + G + i - 1;
-                double dw[5];
-                for (int q = 0; q < 5; q++) dw[q] = D[q][n];
-                face_contrib(dw, c[n], ro[n], a3x[n], a3y[n], a3z[n],
-                             xi[9 * l + 6], xi[9 * l + 7], xi[9 * l + 8], &cd[5 * l]);
+                double dw[7];
+                for (int q = 0; q < NV; q++) dw[q] = D[q][n];
+                face_xfer(ke, b, n, dw, a3x[n], a3y[n], a3z[n],
+                             xi[9 * l + 6], xi[9 * l + 7], xi[9 * l + 8], &cd[NV * l]);
             }
     }
     free(cd); free(xi); free(dt); free(scinv); free(scinp); free(ydi); free(bd);
 }
 
 /* DIAGGS imps4.f:604-664 (interior cells) */
-static void diaggs(const ff_params *p, ff_block *b)
+static void diaggs(const ff_params *p, ff_block *b, int ke)
 {
     dims_t d = mkdims(b);
     long IS = d.is, IL = d.il;
@@ -787,7 +1265,9 @@ static void diaggs(const ff_params *p, ff_block *b)
                  *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z),
                  *vol = F(b, FF_VOL), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
                  *c = F(b, FF_C), *dtl = F(b, FF_DTL);
-    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
+    const int NV = ke ? 7 : 5;
     for (int k = 1; k <= d.kmax; k++)
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {
@@ -809,12 +1289,12 @@ static void diaggs(const ff_params *p, ff_block *b)
                 double sc = .5 * (a1[n] * r1p + a1[n + 1] * r1n + a2[n] * r2p + a2[n + IS] * r2n +
                                   a3[n] * r3p + a3[n + IL] * r3n);
                 double fac = 1. + dt1 * sc + 0.;
-                for (int q = 0; q < 5; q++) D[q][n] = D[q][n] * fac;
+                for (int q = 0; q < NV; q++) D[q][n] = D[q][n] * fac;
             }
 }
 
 /* CORRGS imps4.f:76-339: forward sweep, K = 1..KMAX, diagonals M = I+J ascending */
-static void corrgs(const ff_params *p, ff_block *b)
+static void corrgs(const ff_params *p, ff_block *b, int ke)
 {
     dims_t d = mkdims(b);
     const double cdiff = 2.;
@@ -826,15 +1306,17 @@ static void corrgs(const ff_params *p, ff_block *b)
                  *vol = F(b, FF_VOL), *ro = F(b, FF_RO), *u = F(b, FF_U), *v = F(b, FF_V),
                  *w = F(b, FF_W), *c = F(b, FF_C), *vis = F(b, FF_VIS), *vist = F(b, FF_VIST),
                  *dtl = F(b, FF_DTL);
-    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
+    const int NV = ke ? 7 : 5;
     long slab = (long)d.jmax * IS;
-    double *cd = calloc((size_t)slab * 5, sizeof(double));
+    double *cd = calloc((size_t)slab * NV, sizeof(double));
     double *xi = malloc(sizeof(double) * (size_t)slab * 9);
     double *dt = malloc(sizeof(double) * (size_t)slab);
     double *scinv = malloc(sizeof(double) * (size_t)slab);
     double *scinp = malloc(sizeof(double) * (size_t)slab);
-    double *ydi = calloc((size_t)(d.imax + 2) * 5, sizeof(double));
-    double *bd = calloc((size_t)(d.imax + 2) * 5, sizeof(double));
+    double *ydi = calloc((size_t)(d.imax + 2) * NV, sizeof(double));
+    double *bd = calloc((size_t)(d.imax + 2) * NV, sizeof(double));
     for (int k = 1; k <= d.kmax; k++) {
         double cdiffk = 2. * cdiff;
         for (int j = 1; j <= d.jmax; j++) {
@@ -876,14 +1358,14 @@ static void corrgs(const ff_params *p, ff_block *b)
                 xka = rkmax + t;
                 double dt1 = p->theta * dtl[n];
                 dt[l] = dt1;
-                for (int q = 0; q < 5; q++)
-                    cd[5 * l + q] = dt1 * a3[n] * cd[5 * l + q] + vol[n] * D[q][n];
+                for (int q = 0; q < NV; q++)
+                    cd[NV * l + q] = dt1 * a3[n] * cd[NV * l + q] + vol[n] * D[q][n];
                 scinv[l] = 1. / (vol[n] + dt1 * (xia * a1[n + 1] + xja * a2[n + IS] + xka * a3[n + IL]) + vol[n] * 0.);
                 scinp[l] = 1. / (vol[n] + dt1 * (rimax * a1[n] + rjmax * a2[n] + rkmax * a3[n]) + vol[n] * 0.);
             }
         }
-        memset(ydi, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
-        memset(bd, 0, sizeof(double) * (size_t)(d.imax + 2) * 5);
+        memset(ydi, 0, sizeof(double) * (size_t)(d.imax + 2) * NV);
+        memset(bd, 0, sizeof(double) * (size_t)(d.imax + 2) * NV);
         for (int m = 2; m <= d.imax + d.jmax; m++) {
             int ialku = m - d.jmax > 1 ? m - d.jmax : 1;
             int iloppu = d.imax < m - 1 ? d.imax : m - 1;
@@ -892,78 +1374,151 @@ static void corrgs(const ff_params *p, ff_block *b)
                 long n = cid(&d, i, j, k);
                 long l = (long)(j - 1) * IS + G + i - 1;
                 double dt1 = dt[l];
-                for (int q = 0; q < 5; q++)
-                    cd[5 * l + q] = cd[5 * l + q] + dt1 * (a1[n] * ydi[5 * i + q] + a2[n] * bd[5 * i + q]);
-                for (int q = 0; q < 4; q++) cd[5 * l + q] = cd[5 * l + q] * scinv[l];
-                cd[5 * l + 4] = cd[5 * l + 4] * scinp[l];
-                double dw[5];
-                for (int q = 0; q < 5; q++) { D[q][n] = cd[5 * l + q]; dw[q] = D[q][n]; }
-                face_contrib(dw, c[n], ro[n], a1x[n + 1], a1y[n + 1], a1z[n + 1],
-                             xi[9 * l + 0], xi[9 * l + 1], xi[9 * l + 2], &ydi[5 * i]);
-                face_contrib(dw, c[n], ro[n], a2x[n + IS], a2y[n + IS], a2z[n + IS],
-                             xi[9 * l + 3], xi[9 * l + 4], xi[9 * l + 5], &bd[5 * i]);
+                for (int q = 0; q < NV; q++)
+                    cd[NV * l + q] = cd[NV * l + q] + dt1 * (a1[n] * ydi[NV * i + q] + a2[n] * bd[NV * i + q]);
+                for (int q = 0; q < NV; q++)   /* SCINP for the pressure, SCINV otherwise */
+                    cd[NV * l + q] = cd[NV * l + q] * (q == 4 ? scinp[l] : scinv[l]);
+                double dw[7];
+                for (int q = 0; q < NV; q++) { D[q][n] = cd[NV * l + q]; dw[q] = D[q][n]; }
+                face_xfer(ke, b, n, dw, a1x[n + 1], a1y[n + 1], a1z[n + 1],
+                             xi[9 * l + 0], xi[9 * l + 1], xi[9 * l + 2], &ydi[NV * i]);
+                face_xfer(ke, b, n, dw, a2x[n + IS], a2y[n + IS], a2z[n + IS],
+                             xi[9 * l + 3], xi[9 * l + 4], xi[9 * l + 5], &bd[NV * i]);
             }
             int lo = ialku + 1 > 2 ? ialku + 1 : 2;
             for (int i = iloppu + 1; i >= lo; i--)
-                for (int q = 0; q < 5; q++) ydi[5 * i + q] = ydi[5 * (i - 1) + q];
-            for (int q = 0; q < 5; q++) ydi[5 * ialku + q] = 0.;
+                for (int q = 0; q < NV; q++) ydi[NV * i + q] = ydi[NV * (i - 1) + q];
+            for (int q = 0; q < NV; q++) ydi[NV * ialku + q] = 0.;
         }
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {
                 long n = cid(&d, i, j, k);
                 long l = (long)(j - 1) * IS + G + i - 1;
-                double dw[5];
-                for (int q = 0; q < 5; q++) dw[q] = D[q][n];
-                face_contrib(dw, c[n], ro[n], a3x[n + IL], a3y[n + IL], a3z[n + IL],
-                             xi[9 * l + 6], xi[9 * l + 7], xi[9 * l + 8], &cd[5 * l]);
+                double dw[7];
+                for (int q = 0; q < NV; q++) dw[q] = D[q][n];
+                face_xfer(ke, b, n, dw, a3x[n + IL], a3y[n + IL], a3z[n + IL],
+                             xi[9 * l + 6], xi[9 * l + 7], xi[9 * l + 8], &cd[NV * l]);
             }
     }
     free(cd); free(xi); free(dt); free(scinv); free(scinp); free(ydi); free(bd);
 }
 
-/* IMPS imps3.f:6-572 with LUSGS = 1, ITURB <= 2, CFL >= .05, OMEGA = 0 */
+/* IMSOKO imps3.f:972-1020: point-implicit damping of the k and omega increments */
+static void imsoko(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    sst_t s = coefko(p->kover);
+    const double eps = 1.0E-10;
+    const double gamkom = s.beta1 / s.bstar - s.sigom1 * (s.ckappa * s.ckappa) / (sqrt(s.bstar));
+    const double ckmax = 1.0 / (2.0 * p->csimps), comax = 1.0 / (2.0 * p->csimps);
+    const double *rk = F(b, FF_RK), *rom = F(b, FF_REPS), *ptur = F(b, FF_PTUR),
+                 *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS), *dtl = F(b, FF_DTL),
+                 *ro = F(b, FF_RO), *e = F(b, FF_E);
+    double *drk = F(b, FF_DRK), *dom = F(b, FF_DEPS);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long L = cid(&d, i, j, k);
+                double emeanf = e[L] - rk[L];
+                double drkmax = dmax(ckmax * dmin(rk[L], fabs(0.1 * emeanf - 0.9 * rk[L])), eps);
+                double dommax = comax * rom[L];
+                double prol = 1. / ro[L];
+                double atprod = fabs(ptur[L]);
+                double rnutrb = dmax((eps2[L] - 1.0), eps) * vis[L] * prol;
+                double drkden = 1.0 + dtl[L] * (atprod / drkmax + prol * s.bstar * rom[L]);
+                double domden = 1.0 + dtl[L] * (gamkom * atprod / (rnutrb * dommax)
+                                                + 2. * s.beta1 * rom[L]);
+                drk[L] = drk[L] / drkden;
+                dom[L] = dom[L] / domden;
+            }
+}
+
+/* IMPS imps3.f:6-572 with LUSGS = 1, CFL >= .05, OMEGA = 0.  Laminar (ITURB <= 2): LOCDT,
+ * COPR, PREDGS/DIAGGS/CORRGS.  SST (ITURB = 6): LOCDTK (DTURB = 1, imps3.f:61), IMSOKO,
+ * COPRKE (imps3.f:1113-1211), PRKEGS/DIKEGS/COKEGS, then D(rho k), D(rho omega) times RO
+ * (imps3.f:497-500).  Both end with dT = dh/CP (imps3.f:483). */
 void ffo_implicit(const ff_params *p, ff_block *b, int level)
 {
     dims_t d = mkdims(b);
+    int ke = p->iturb == 6;
     double cfl = level >= 2 ? p->cfll : p->cfl;
     time3(p, b, cfl);
-    double *D[5] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE)};
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
     const double *dtl = F(b, FF_DTL), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
                  *ro = F(b, FF_RO), *e = F(b, FF_E), *P = F(b, FF_P), *c = F(b, FF_C),
-                 *drdp = F(b, FF_DRDP), *drdh = F(b, FF_DRDH), *cp = F(b, FF_CP);
+                 *drdp = F(b, FF_DRDP), *drdh = F(b, FF_DRDH), *cp = F(b, FF_CP),
+                 *rk = F(b, FF_RK), *reps = F(b, FF_REPS);
+    const double dturb = 1.;
     for (int k = 1; k <= d.kmax; k++)
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {
                 long n = cid(&d, i, j, k);
-                /* LOCDT imps3.f:792-798 */
+                /* LOCDT imps3.f:792-798 / LOCDTK imps3.f:820-826 */
                 double dt1 = dtl[n];
                 for (int q = 0; q < 5; q++) D[q][n] = dt1 * D[q][n];
-                /* COPR imps3.f:902-927 */
-                double vel2 = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]);
-                double beta = .5 * vel2;
-                double drodh = drdh[n], drodp = drdp[n];
-                double ypro = 1. / ro[n];
-                double dri = D[0][n], dmi = D[1][n], dni = D[2][n], dwi = D[3][n], dre = D[4][n];
-                double dkin = u[n] * dmi + v[n] * dni + w[n] * dwi;
-                D[1][n] = ypro * (-u[n] * dri + dmi);
-                D[2][n] = ypro * (-v[n] * dri + dni);
-                D[3][n] = ypro * (-w[n] * dri + dwi);
-                double hi = ypro * (e[n] + P[n]);
-                double dei = dre - (hi - 2 * beta) * dri - dkin;
-                double dpi = (c[n] * c[n]) * (dri - drodh * ypro * dei);
-                double dhi = (dri - drodp * dpi) / drodh;
-                D[0][n] = dhi;
-                D[4][n] = dpi;
+                if (ke) {
+                    D[5][n] = dturb * dt1 * D[5][n];
+                    D[6][n] = dturb * dt1 * D[6][n];
+                }
             }
-    predgs(p, b);
-    diaggs(p, b);
-    corrgs(p, b);
-    /* DIVV12(DRO,DRO,CP) imps3.f:483 */
+    if (ke) imsoko(p, b);
     for (int k = 1; k <= d.kmax; k++)
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {
                 long n = cid(&d, i, j, k);
+                double drodh = drdh[n], drodp = drdp[n];
+                double ypro = 1. / ro[n];
+                if (!ke) {
+                    /* COPR imps3.f:902-927 */
+                    double vel2 = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]);
+                    double beta = .5 * vel2;
+                    double dri = D[0][n], dmi = D[1][n], dni = D[2][n], dwi = D[3][n], dre = D[4][n];
+                    double dkin = u[n] * dmi + v[n] * dni + w[n] * dwi;
+                    D[1][n] = ypro * (-u[n] * dri + dmi);
+                    D[2][n] = ypro * (-v[n] * dri + dni);
+                    D[3][n] = ypro * (-w[n] * dri + dwi);
+                    double hi = ypro * (e[n] + P[n]);
+                    double dei = dre - (hi - 2 * beta) * dri - dkin;
+                    double dpi = (c[n] * c[n]) * (dri - drodh * ypro * dei);
+                    double dhi = (dri - drodp * dpi) / drodh;
+                    D[0][n] = dhi;
+                    D[4][n] = dpi;
+                } else {
+                    /* COPRKE imps3.f:1145-1205 (SOLUTION_TYPE = FLUID) */
+                    double vel2 = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]);
+                    double rki = rk[n] * ypro;
+                    double dri = D[0][n], dmi = D[1][n], dni = D[2][n], dwi = D[3][n],
+                           dre = D[4][n], dki = D[5][n];
+                    double dkin = u[n] * dmi + v[n] * dni + w[n] * dwi + dki;
+                    D[1][n] = ypro * (-u[n] * D[0][n] + dmi);
+                    D[2][n] = ypro * (-v[n] * D[0][n] + dni);
+                    D[3][n] = ypro * (-w[n] * D[0][n] + dwi);
+                    D[5][n] = ypro * (-rki * D[0][n] + dki);
+                    D[6][n] = ypro * (-reps[n] * ypro * D[0][n] + D[6][n]);
+                    double hi = ypro * (e[n] + P[n]);
+                    double dei = dre - (hi - vel2 - rki) * dri - dkin;
+                    double dpi = (c[n] * c[n]) * (dri - drodh * ypro * dei);
+                    double dhi = (dri - drodp * dpi) / drodh;
+                    D[0][n] = dhi;
+                    D[4][n] = dpi;
+                    D[4][n] = dpi + .667 * dki;
+                }
+            }
+    predgs(p, b, ke);
+    diaggs(p, b, ke);
+    corrgs(p, b, ke);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long n = cid(&d, i, j, k);
+                /* DIVV12(DRO,DRO,CP) imps3.f:483 */
                 D[0][n] = D[0][n] / cp[n];
+                if (ke) {
+                    /* MULV12(DRK,RO), MULV12(DEPS,RO) imps3.f:497-500 */
+                    D[5][n] = D[5][n] * ro[n];
+                    D[6][n] = D[6][n] * ro[n];
+                }
             }
 }
 
@@ -992,6 +1547,7 @@ void ffo_update(const ff_params *p, ff_block *b)
                  *wo = F(b, FF_WOLD), *po = F(b, FF_POLD);
     const double *dt = F(b, FF_DRO), *du = F(b, FF_DM), *dv = F(b, FF_DN), *dw = F(b, FF_DW),
                  *dp = F(b, FF_DE);
+    int ke = p->iturb == 6;
     for (int k = 1; k <= d.kmax; k++)
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {
@@ -1001,7 +1557,26 @@ void ffo_update(const ff_params *p, ff_block *b)
                 v[n] = vo[n] + dv[n];
                 w[n] = wo[n] + dw[n];
                 pd[n] = po[n] + dp[n];
+                if (ke) {
+                    /* res3c.f:1189-1198: relaxed update of rho*k and rho*omega */
+                    const double *rko = F(b, FF_RKOLD), *epo = F(b, FF_EPSOLD),
+                                 *drk = F(b, FF_DRK), *dep = F(b, FF_DEPS);
+                    double rkoldn = .45 * rko[n];
+                    double epoldn = .90 * epo[n];
+                    double absdrk = fabs(drk[n]);
+                    double absdep = fabs(dep[n]);
+                    F(b, FF_RK)[n] = rko[n] + drk[n] / (1. + absdrk / rkoldn);
+                    F(b, FF_REPS)[n] = epo[n] + dep[n] / (1. + absdep / epoldn);
+                }
             }
+    if (ke) {
+        /* LOLIM of RK and REPS (res3c.f:1202-1205, again ns3c.f:1575-1576), all cells */
+        double *rk = F(b, FF_RK), *reps = F(b, FF_REPS);
+        for (long n = 0; n < d.ntot; n++) {
+            rk[n] = rk[n] > p->rklim ? rk[n] : p->rklim;
+            reps[n] = reps[n] > p->epslim ? reps[n] : p->epslim;
+        }
+    }
     /* UPLIM (res3c.f:3401: interior k-planes, interior j-rows, all i) and LOLIM (all) */
     double *lim[4] = {t, u, v, w};
     double up[4] = {p->tuplim, 5000., 5000., 5000.};
@@ -1095,7 +1670,8 @@ void ffo_copr(const ff_params *p, ff_block *b)
 /* PREDGS + DIAGGS + CORRGS on the current increments (DTL given) */
 void ffo_lusgs(const ff_params *p, ff_block *b)
 {
-    predgs(p, b);
-    diaggs(p, b);
-    corrgs(p, b);
+    int ke = p->iturb == 6;
+    predgs(p, b, ke);
+    diaggs(p, b, ke);
+    corrgs(p, b, ke);
 }

@@ -107,7 +107,8 @@ class RefBlock:
         self.prm, self.blk, self.level = prm, blk, level
         self.pv = prm.dvec()
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
-                            prm.inter[0], prm.inter[1], prm.inter[2], level],
+                            prm.inter[0], prm.inter[1], prm.inter[2], level,
+                            prm.kover, prm.katol],
                            dtype=np.int32)
 
     def _a(self):

@@ -4,13 +4,17 @@
 ! TEST INFRASTRUCTURE ONLY: used to pin the C restatement in ff_oracle.c.  Nothing in the
 ! product links it.  Each entry point follows the reference's call sequence for one stage:
 !   ref_state     STATIM ns3c.f:10318-10447: PDTOP, STATE->PERFGA (state.f:159), CONSVA
-!   ref_turb      TURBDI ns3c.f:2987-2991 (ITURB=0): EPS2=1, VIST=0
+!   ref_turb      TURBDI ns3c.f:2987-2991 (ITURB=0): EPS2=1, VIST=0; ITURB=6: TURBEQ
+!                 res3c.f:5951-6371: DERFUN, TTIMES, EPS2KO
 !   ref_residual  RESIDU res3c.f:194-559: ZEROZZ, then per direction k,i,j the FLUCYC
 !                 sequence res3c.f:7705-7907 (ZEROZZ of F1*, FLUXDO, FLUXKE or FLUXVL)
 !                 and SWEEP;
-!                 RESL2 res3c.f:8920 on level 1
+!                 RESL2 res3c.f:8920 on level 1; ITURB=6: the RESIKE sequence
+!                 res3c.f:3700-5068 (CROSKO, FUN1KO, MAKE_EPS, PRODKE, FLXCYC with FLUDKE,
+!                 SETV12, FLUXDO, FLUXKE; SWEEPK; SOURKO, ADDSKE)
 !   ref_implicit  IMPS imps3.f:111-483 with LUSGS=1: TIME3, LOCDT, COPR, PREDGS, DIAGGS,
-!                 CORRGS, DIVV12
+!                 CORRGS, DIVV12; ITURB=6: LOCDTK, IMSOKO, COPRKE, PRKEGS, DIKEGS,
+!                 COKEGS, DIVV12, MULV12
 !   ref_update    CYCLES ns3c.f:1517-1596: FIN, UPLIM/LOLIM, PDTOP, LOLIMP
 !
 ! All fields of a block are packed as F(NTOT, NF) in the order of enum ff_field
@@ -27,14 +31,17 @@ module ffref_ids
   integer, parameter :: IA2Z=45, IA3=46, IA3X=47, IA3Y=48, IA3Z=49, ID1=50, ID2=51, ID3=52
   integer, parameter :: IXC=53, IYC=54, IZC=55, IDISTW=56
   integer, parameter :: ISRK=57, ISEPS=58, IPTUR=59, IFUN1=60, IOHMI=61
+  integer, parameter :: IW12=62, IW13=63, IW23=64, IS11=65, IS12=66, IS13=67, IS22=68
+  integer, parameter :: IS23=69, IS33=70, ISTRAIN=71, ITTS=72
+  integer, parameter :: IB11=73, IB12=74, IB13=75, IB22=76, IB23=77, IB33=78
   ! parameter vector PRM (same order as the doubles of ff_params)
   integer, parameter :: PGAMMA=1, PRGAS=2, PPR=3, PPRT=4, PVISU0=5, PEXPSU=6, PTSU0=7
   integer, parameter :: PFRSPRE=8, PFRSDEN=9, PFRSTEM=10, PFRSVEL=11, PFRSVIS=12, PCFL=13
   integer, parameter :: PCFLL=14, PTHETA=15, PCDIFF=16, PTUPLIM=17, PTLOLIM=18, PRKLIM=19
-  integer, parameter :: PEPSLIM=20, PTURLIM=21, PFRSMUT=22, PRMULT=23
+  integer, parameter :: PEPSLIM=20, PTURLIM=21, PFRSMUT=22, PRMULT=23, PRE=24, PCSIMPS=25
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
-  integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10
+  integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12
 end module ffref_ids
 
 subroutine ref_state(f, ntot, prm, ipr) bind(C, name="ref_state")
@@ -75,6 +82,32 @@ subroutine ref_turb(f, ntot, prm, ipr) bind(C, name="ref_turb")
   integer(c_int), value :: ntot
   real(c_double) :: f(ntot, *), prm(*)
   integer(c_int) :: ipr(*)
+  real(c_double), allocatable :: vellap(:), rksi(:)
+  real(c_double) :: a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2
+  real(c_double) :: sigphi, zeta2, frsmut
+  integer :: kover
+  if (ipr(QITURB) == 6) then
+     ! TURBEQ (res3c.f:5922, 5951-5954, 5984, 6362-6371), KOVER = 0: EPS2KO with STRAIN
+     allocate(vellap(ntot), rksi(ntot))
+     vellap = 0.; rksi = 0.
+     kover = ipr(QKOVER)
+     frsmut = 1.E-3_c_double*prm(PFRSVIS)
+     call DERFUN(f(1,IOHMI), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), &
+          f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IU), f(1,IV), f(1,IW), f(1,IA1), &
+          f(1,IA2), f(1,IA3), f(1,IVOL), f(1,ID1), f(1,ID2), f(1,ID3), f(1,IA1X), f(1,IA1Y), &
+          f(1,IA1Z), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3X), f(1,IA3Y), f(1,IA3Z), &
+          f(1,IXC), f(1,IYC), f(1,IZC), ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), 2, 2, 2, 1, 1, &
+          f(1,ISTRAIN), vellap)
+     call TTIMES(f(1,IRO), f(1,IVIS), f(1,IRK), f(1,IREPS), 6, ipr(QIMAX), ipr(QJMAX), &
+          ipr(QKMAX), f(1,ITTS))
+     call COEFKO(6, a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2, &
+          sigphi, zeta2, kover)
+     call EPS2KO(f(1,IEPS2), f(1,IVIST), f(1,IVIS), f(1,ISTRAIN), f(1,IRO), f(1,IRK), &
+          f(1,IREPS), f(1,IDISTW), prm(PTURLIM), frsmut, ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), &
+          2, 2, 2, prm(PPR), prm(PPRT), bstar, a1kleb, kover, rksi)
+     deallocate(vellap, rksi)
+     return
+  end if
   ! TURBDI ITURB = 0 (ns3c.f:2987-2991)
   call ADJUIN(f(1,IEPS2), 1._c_double, ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), 2, 2, 2)
   call ADJUIN(f(1,IVIST), 0._c_double, ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), 2, 2, 2)
@@ -166,6 +199,10 @@ subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   real(c_double) :: r(67)
   integer :: maw, maxw, imax, jmax, kmax
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  if (ipr(QITURB) == 6) then
+     call ref_residual_kw(f, ntot, prm, ipr, resi)
+     return
+  end if
   ! FLUXKE's slab work arrays hold ISTRID*(JMAX+2) entries (base3c.f:4812-4824)
   maw = (imax + 4)*(jmax + 4)
   maxw = 27*maw
@@ -188,20 +225,171 @@ subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   deallocate(fl, zzz, rksi, hat, zero, dfi)
 end subroutine ref_residual
 
+! One direction of RESIKE for ITURB = 6: the FLXCYC sequence res3c.f:6740-6946 for a block
+! with no boundary patches, FULLNS = .FALSE., IFLUX = 1 (zero F1*, FLUDKE, SETV12, FLUXDO,
+! FLUXKE with the turbulence MUSCL switch INTET and RKLIM/EPSLIM), followed by SWEEPK
+! (res3c.f:4098, 4325, 4551).
+subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat, zero, &
+     trm, sig)
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  implicit none
+  integer :: ntot, idir, maw, maxw
+  real(c_double) :: f(ntot, *), prm(*), fl(ntot, 7), zzz(maxw), rksi(ntot), hat(ntot, 4)
+  real(c_double) :: zero(ntot), sig(4)
+  integer(c_int) :: ipr(*)
+  type(INTERMITTENCY) :: trm(1)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(PRE_COR) :: prc(1)
+  real(c_double) :: f1h(1), wave(1)
+  integer :: imax, jmax, kmax, is, js, kstr, kmaxp, ia, iax, iay, iaz, inter, intet, ndi, ido
+  logical :: xxtral, inchiml, entropy_fix, transl, mulphl, fresul
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  is = imax + 4; js = jmax + 4
+  xxtral = .false.; inchiml = .false.; entropy_fix = .false.; transl = .false.
+  mulphl = .false.; fresul = .false.
+  select case (idir)
+  case (1)
+     kstr = 1; kmaxp = kmax; ia = IA1; inter = ipr(QINTI); ndi = imax + 1
+  case (2)
+     kstr = is; kmaxp = kmax; ia = IA2; inter = ipr(QINTJ); ndi = jmax + 1
+  case default
+     kstr = is*js; kmaxp = kmax + 1; ia = IA3; inter = ipr(QINTK); ndi = kmax + 1
+  end select
+  ! INTETI/J/K of RESIKE (res3c.f:3634-3641, INTERTU = 0)
+  intet = inter
+  if (inter > -5 .and. inter < 5) intet = +1
+  iax = ia + 1; iay = ia + 2; iaz = ia + 3
+  fl = 0.
+  call FLUDKE(fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), f(1,ia), f(1,IRO), f(1,IRK), f(1,IREPS), &
+       imax, jmax, kmax, ndi, f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,IFUN1), sig(1), sig(2), &
+       sig(3), sig(4), kstr, idir, ipr(QIDIFF), transl, trm)
+  call SETV12(fl(1,5), fl(1,6), ntot)
+  call FLUXDO(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IVOL), f(1,ia), f(1,IRO), &
+       f(1,IRM), f(1,IRN), f(1,IRW), f(1,IP), f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), &
+       imax, jmax, kmax, 1, inter, ndi, f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,iax), &
+       f(1,iay), f(1,iaz), f(1,ICP), prm(PPR), prm(PPRT), zero, pro, var, f(1,ITEMP), &
+       f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), prm(PFRSPRE), &
+       hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, mulphl, 1, fresul, 0._c_double, f1h, &
+       wave, ipr(QLEVEL), ipr(QIDIFF))
+  hat = 0.
+  call FLUXKE(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), &
+       f(1,ia), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), f(1,IP), &
+       f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), imax, jmax, kmax, 1, inter, intet, ndi, &
+       f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,IPDIFF), f(1,iax), f(1,iay), f(1,iaz), &
+       f(1,ID1), prm(PPR), prm(PPRT), zero, f(1,ITEMP), f(1,ICH), f(1,IDRDH), f(1,IDRDP), &
+       1, prm(PRGAS), prm(PGAMMA), 0._c_double, 0._c_double, prm(PFRSDEN), prm(PFRSPRE), &
+       prm(PRKLIM), prm(PEPSLIM), hat(1,1), hat(1,2), hat(1,3), hat(1,4), kmaxp, kstr, idir, &
+       ipr(QITURB), zzz, maw, maxw, xxtral, rksi, inchiml, entropy_fix, transl, trm)
+  ido = 0
+  if (idir == 2) ido = 1
+  call SWEEPK(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDRK), f(1,IDEPS), &
+       fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), var, trm, &
+       imax, jmax, kmax, kstr, 2, 2, 2, 0._c_double, ido, mulphl, transl, 'FLUID', prc)
+end subroutine ref_direction_kw
+
+! RESIKE res3c.f:3423-5387 for ITURB = 6 (SST, IDIS = 2), M = M2 = 1, no patches
+subroutine ref_residual_kw(f, ntot, prm, ipr, resi)
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  use MAIN_ARRAYS, only : BLKS
+  use NS3CO, only : TWO_FLUIDL
+  implicit none
+  integer(c_int) :: ntot
+  real(c_double) :: f(ntot, *), prm(*), resi(*)
+  integer(c_int) :: ipr(*)
+  real(c_double), allocatable :: fl(:,:), zzz(:), rksi(:), hat(:,:), zero(:), dfi(:)
+  real(c_double), allocatable :: t2(:), trueps(:), ddeps(:), vtran(:), rich(:), vellap(:)
+  real(c_double), allocatable :: qsas(:), rst(:)
+  type(INTERMITTENCY) :: trm(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(PRE_COR) :: prc(1)
+  real(c_double) :: r(67), sig(4)
+  real(c_double) :: a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2
+  real(c_double) :: sigphi, zeta2
+  integer :: maw, maxw, imax, jmax, kmax, kover
+  logical :: katol
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  kover = ipr(QKOVER)
+  katol = ipr(QKATOL) /= 0
+  if (.not. allocated(BLKS)) allocate(BLKS(1))
+  BLKS(1)%RKLIM = prm(PRKLIM); BLKS(1)%EPSLIM = prm(PEPSLIM)
+  maw = (imax + 4)*(jmax + 4)
+  maxw = 27*maw
+  allocate(fl(ntot,7), zzz(maxw), rksi(ntot), hat(ntot,4), zero(ntot), dfi(ntot))
+  allocate(t2(ntot), trueps(ntot), ddeps(ntot), vtran(ntot), rich(ntot), vellap(ntot))
+  allocate(qsas(ntot), rst(ntot))
+  rksi = 0.; zero = 0.; zzz = 0.; dfi = 0.; t2 = 0.; ddeps = 0.; vtran = 1.; vellap = 0.
+  qsas = 0.; rst = 0.
+  ! res3c.f:3668-3675, 3730
+  call ZEROZZ(f(1,IDRO), ntot); call ZEROZZ(f(1,IDM), ntot); call ZEROZZ(f(1,IDN), ntot)
+  call ZEROZZ(f(1,IDW), ntot); call ZEROZZ(f(1,IDE), ntot); call ZEROZZ(f(1,IDRK), ntot)
+  call ZEROZZ(f(1,IDEPS), ntot); call ZEROZZ(f(1,ISRK), ntot); call ZEROZZ(f(1,IFUN1), ntot)
+  ! COEFKO and the diffusion sigmas (res3c.f:3788-3806)
+  call COEFKO(6, a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2, &
+       sigphi, zeta2, kover)
+  sig(1) = sigrk1; sig(2) = sigom1; sig(3) = sigrk2; sig(4) = sigom2
+  call CROSKO(f(1,ISEPS), f(1,IRO), f(1,IRK), f(1,IREPS), f(1,IVOL), f(1,IA1), f(1,IA1X), &
+       f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), f(1,IA3X), &
+       f(1,IA3Y), f(1,IA3Z), imax, jmax, kmax, 2, 2, 2, sigom2, sigphi, t2, kover)
+  call FUN1KO(f(1,IFUN1), f(1,ISEPS), f(1,IVIS), f(1,IRO), f(1,IRK), f(1,IREPS), &
+       f(1,IDISTW), .false., imax, jmax, kmax, 2, 2, 2, bstar, sigom2, prm(PRKLIM), &
+       1._c_double, prm(PRE), 1)
+  ! MAKE_EPS, PRODKE, ZEROZZ(DDEPS) (res3c.f:3949-3994)
+  trueps = 0.
+  call MAKE_EPS(f(1,IRO), f(1,IRK), f(1,IREPS), ddeps, trueps, 6, ntot)
+  call PRODKE(f(1,IOHMI), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), &
+       f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IPTUR), f(1,IRK), f(1,IREPS), ddeps, &
+       trueps, f(1,IEPS2), f(1,IVIST), f(1,IVIS), rst, rst, rst, rst, rst, rst, vtran, &
+       imax, jmax, kmax, 2, 2, 2, 1, 6, katol)
+  call ZEROZZ(ddeps, ntot)
+  call ref_direction_kw(f, ntot, prm, ipr, 3, fl, zzz, maw, maxw, rksi, hat, zero, trm, sig)
+  call ref_direction_kw(f, ntot, prm, ipr, 1, fl, zzz, maw, maxw, rksi, hat, zero, trm, sig)
+  call ref_direction_kw(f, ntot, prm, ipr, 2, fl, zzz, maw, maxw, rksi, hat, zero, trm, sig)
+  ! MAKE_EPS, RICH = 0 (LRICH = 0), SOURKO, ADDSKE (res3c.f:4850-5068)
+  trueps = 0.
+  call MAKE_EPS(f(1,IRO), f(1,IRK), f(1,IREPS), ddeps, trueps, 6, ntot)
+  rich = 0.
+  call SOURKO(f(1,ISRK), f(1,ISEPS), f(1,IPTUR), f(1,IEPS2), f(1,IVIS), f(1,IRO), f(1,IRK), &
+       f(1,IREPS), f(1,IFUN1), rich, f(1,ID1), f(1,ID2), f(1,ID3), 1, imax, jmax, kmax, &
+       2, 2, 2, kover, bstar, ckappa, beta1, sigom1, beta2, sigom2, zeta2, 0.09_c_double, &
+       .false., f(1,ITTS), .false., .false., trm, t2, f(1,ISTRAIN), vellap, qsas, &
+       f(1,IOHMI), f(1,IDISTW))
+  call ADDSKE(f(1,IDRK), f(1,IDEPS), f(1,ISRK), f(1,IRK), f(1,IREPS), f(1,IPTUR), &
+       f(1,ISEPS), f(1,IVOL), imax, jmax, kmax, (imax+4)*(jmax+4), 2, 2, 2, 0._c_double, trm, &
+       .false.)
+  if (ipr(QLEVEL) == 1) then
+     TWO_FLUIDL = .false.
+     r = 0.
+     call RESL2(f(1,IVOL), f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), dfi, &
+          f(1,IDRK), f(1,IDEPS), var, r, rksi, imax, jmax, kmax, ipr(QITURB), 0, ntot, 67, &
+          prm(PRKLIM), prm(PEPSLIM), .false., 1, 1, 1, prc)
+     resi(1:7) = r(31:37)
+  end if
+  deallocate(fl, zzz, rksi, hat, zero, dfi, t2, trueps, ddeps, vtran, rich, vellap, qsas, rst)
+end subroutine ref_residual_kw
+
 subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
   use iso_c_binding
   use ffref_ids
+  use TYPE_ARRAYS
   implicit none
   integer(c_int), value :: ntot
   real(c_double) :: f(ntot, *), prm(*)
   integer(c_int) :: ipr(*)
   real(c_double), allocatable :: zzz(:), rksi(:), zero(:), abv(:)
+  real(c_double), allocatable :: dxi(:), dgi(:), dreti(:), gii(:), reti(:), pdfor(:)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
   integer :: maw, maxw, imax, jmax, kmax, idm1, idm2, idm3
   real(c_double) :: cfm, theta
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
   ! the LU-SGS slab work arrays hold JMAX*ISTRID entries (imps4.f:96-106)
   maw = (imax + 4)*(jmax + 4)
-  maxw = 18*maw
+  maxw = 21*maw
   allocate(zzz(maxw), rksi(ntot), zero(ntot), abv(ntot))
   zzz = 0.; rksi = 0.; zero = 0.; abv = 0.
   cfm = prm(PCFL)
@@ -215,29 +403,71 @@ subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
        f(1,IP), imax, jmax, kmax, cfm, f(1,IVIS), f(1,IVIST), zero, zero, zero, &
        prm(PFRSDEN), prm(PFRSVEL), prm(PPR), prm(PPRT), idm1, idm2, idm3, ipr(QLEVEL), &
        .false., 1.8_c_double, abv, 1, prm(PFRSPRE), prm(PCDIFF))
+  if (ipr(QITURB) == 6) then
+     ! LOCDTK (DTURB = 1, imps3.f:61, 123), IMSOKO (imps3.f:233), COPRKE (imps3.f:276),
+     ! PRKEGS/DIKEGS/COKEGS (imps3.f:398-430), DIVV12, MULV12 (imps3.f:483, 497-500)
+     allocate(dxi(ntot), dgi(1), dreti(1), gii(1), reti(1), pdfor(ntot))
+     dxi = 0.; dgi = 0.; dreti = 0.; gii = 0.; reti = 0.; pdfor = 0.
+     call LOCDTK(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDRK), f(1,IDEPS), &
+          f(1,IDTL), ntot, 0._c_double, 1._c_double, 'FLUID     ', dxi, .false., dgi, dreti)
+     call IMSOKO(f(1,IDRK), f(1,IDEPS), f(1,IRK), f(1,IREPS), f(1,IPTUR), f(1,IEPS2), &
+          f(1,IVIS), f(1,IDTL), f(1,IRO), f(1,IU), f(1,IV), f(1,IW), f(1,IE), prm(PGAMMA), &
+          imax, jmax, kmax, 1, imax + 4, (imax + 4)*(jmax + 4), 2, 2, 2, prm(PCSIMPS), &
+          ipr(QKOVER))
+     call COPRKE(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDRK), f(1,IDEPS), &
+          f(1,IRO), f(1,IE), f(1,IP), f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRK), f(1,IREPS), f(1,IDRDP), f(1,IDRDH), prm(PFRSVEL), 'FLUID     ', pro, var, &
+          dxi, f(1,IDTL), ntot, .false., gii, reti, dgi, dreti, pdfor, rksi, f(1,IVOL), 1, &
+          imax, jmax, kmax)
+     call PRKEGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+          f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+          f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRO), f(1,IP), f(1,IDRDP), f(1,IDRDH), imax, jmax, kmax, theta, 0._c_double, &
+          idm1, idm2, idm3, f(1,IVIS), f(1,IVIST), f(1,IDRK), f(1,IDEPS), f(1,IRK), f(1,IREPS), &
+          f(1,ISRK), f(1,ISEPS), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, &
+          maw, maxw, .false., dgi, dreti)
+     call DIKEGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+          f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+          f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRO), f(1,IP), f(1,IDRDP), f(1,IDRDH), imax, jmax, kmax, theta, f(1,IVIS), &
+          f(1,IVIST), f(1,IDRK), f(1,IDEPS), f(1,IRK), f(1,IREPS), f(1,ISRK), f(1,ISEPS), &
+          2, 2, 2, zero, zero, zero, rksi, .false., dgi, dreti)
+     call COKEGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+          f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+          f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRO), f(1,IP), f(1,IDRDP), f(1,IDRDH), imax, jmax, kmax, theta, 0._c_double, &
+          idm1, idm2, idm3, f(1,IVIS), f(1,IVIST), f(1,IDRK), f(1,IDEPS), f(1,IRK), f(1,IREPS), &
+          f(1,ISRK), f(1,ISEPS), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, &
+          maw, maxw, .false., dgi, dreti)
+     call DIVV12(f(1,IDRO), f(1,IDRO), f(1,ICP), ntot)
+     call MULV12(f(1,IDRK), f(1,IRO), ntot)
+     call MULV12(f(1,IDEPS), f(1,IRO), ntot)
+     deallocate(dxi, dgi, dreti, gii, reti, pdfor)
+  else
   ! LOCDT (imps3.f:120) and COPR (imps3.f:272)
-  call LOCDT(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDTL), ntot, 0._c_double)
-  call COPR(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IRO), f(1,IE), f(1,IP), &
-       f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), f(1,IDRDP), f(1,IDRDH), &
-       prm(PFRSVEL), ntot)
-  ! LU-SGS (imps3.f:383-391)
-  call PREDGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
-       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
-       f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
-       f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
-       f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
-  call DIAGGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
-       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
-       f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
-       f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
-       f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
-  call CORRGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
-       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
-       f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
-       f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
-       f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
-  ! dh -> dT (imps3.f:483)
-  call DIVV12(f(1,IDRO), f(1,IDRO), f(1,ICP), ntot)
+     call LOCDT(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDTL), ntot, 0._c_double)
+     call COPR(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IRO), f(1,IE), f(1,IP), &
+          f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), f(1,IDRDP), f(1,IDRDH), &
+          prm(PFRSVEL), ntot)
+     ! LU-SGS (imps3.f:383-391)
+     call PREDGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+          f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+          f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
+          f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+     call DIAGGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+          f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+          f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
+          f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+     call CORRGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
+          f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
+          f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
+          f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
+          f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+     ! dh -> dT (imps3.f:483)
+     call DIVV12(f(1,IDRO), f(1,IDRO), f(1,ICP), ntot)
+  end if
   deallocate(zzz, rksi, zero, abv)
 end subroutine ref_implicit
 

@@ -22,6 +22,30 @@ CASES = [
 ]
 
 
+# k-omega SST (ITURB = 6): curvilinear with F1 across (0,1), a stretched wall case whose
+# large RE puts cells under FUN1KO's wall-distance limit, limited/first-order MUSCL mixes,
+# and the full production form (KATOL = NO)
+SST_CASES = [
+    ("sst_curv", (10, 7, 6), dict(iturb=6),
+     dict(kind="all", amp=0.05, distort=0.15, seed=23, turb=(1.0, 50.0))),
+    ("sst_wall", (9, 8, 5), dict(iturb=6, re=1e14, cfl=2.0),
+     dict(kind="velocity", amp=0.03, jratio=1.3, seed=29, turb=(0.5, 400.0))),
+    ("sst_limited", (8, 7, 6), dict(iturb=6, inter=(3, 2, 55)),
+     dict(kind="all", amp=0.04, distort=0.1, seed=31, turb=(2.0, 1000.0))),
+    ("sst_fullprod", (7, 6, 8), dict(iturb=6, katol=0, rklim=0.05, epslim=30.0),
+     dict(kind="all", amp=0.08, distort=0.12, seed=37, turb=(0.3, 80.0))),
+]
+
+# fields each stage writes, for the stage-by-stage comparisons
+STATE = ["P", "RO", "E", "C", "VIS", "CH", "CP", "DRDP", "DRDH", "RM", "RN", "RW"]
+RES = ["DRO", "DM", "DN", "DW", "DE"]
+PRIM = ["TEMP", "U", "V", "W", "PDIFF", "P"]
+SST_TURB = ["OHMI", "W12", "W13", "W23", "S11", "S12", "S13", "S22", "S23", "S33", "STRAIN",
+            "EPS2", "VIST"]
+SST_RES = RES + ["DRK", "DEPS", "FUN1", "PTUR", "SRK", "SEPS"]
+SST_PRIM = PRIM + ["RK", "REPS"]
+
+
 def make_case(dims, pover, ckw):
     prm = default_params(**pover)
     blk = make_block(*dims, prm=prm, **ckw)
