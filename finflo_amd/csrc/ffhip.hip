@@ -57,9 +57,20 @@ __global__ void __launch_bounds__(256) k_state(Prm p, Blk b)
     double drdh = -ro / e / p.gamma;
     double ch = vis * cp2 / p.pr;
     double rkin = (u * u + v * v + w * w) * .5;                 // CONSVA base3c.f:519-528
-    double x = 1. / (drdh / ro + drdp);
-    double c = sqrt(x > cmin ? x : cmin);
-    e = (e + rkin + 0.) * ro;
+    double c;
+    if (p.iturb < 3 || p.iturb == 8) {
+        double x = 1. / (drdh / ro + drdp);
+        c = sqrt(x > cmin ? x : cmin);
+        e = (e + rkin + 0.) * ro;
+    } else {                                                    // base3c.f:533-545
+        const double r23 = 2. / 3.;
+        double rk = FLD(b, FF_RK)[n];
+        double ypro = 1. / ro;
+        double xturb = 1. + r23 * rk * ypro * drdp;
+        double x = xturb / (drdh * ypro + drdp);
+        c = sqrt(x > cmin ? x : cmin);
+        e = (e + rkin + 0.) * ro + rk;
+    }
     FLD(b, FF_P)[n] = P;
     FLD(b, FF_RO)[n] = ro;
     FLD(b, FF_E)[n] = e;
@@ -84,6 +95,343 @@ __global__ void __launch_bounds__(256) k_turb_lam(Blk b)
     int n = cid(b, i, j, k);
     FLD(b, FF_EPS2)[n] = 1.;
     FLD(b, FF_VIST)[n] = 0.;
+}
+
+// ------------------------------------------------------------------------------------
+// k-omega SST (ITURB = 6): TURBEQ and the RESIKE terms outside the face fluxes
+// ------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double pow4(double x) { return x * x * x * x; }   // ((x*x)*x)*x
+
+// TURBEQ (res3c.f:5951-6371) on the cells of level 1 plus one ghost layer: DERFUN
+// (turb3.f:2476-2665, IDERI = 1: distance-weighted Green-Gauss) and TTIMES (earsm.f:671-684)
+// everywhere there, EPS2KO (turb3.f:7454-7537, limiter argument STRAIN, FRSMUT =
+// 1E-3*FRSVIS) on the interior.  Thread per cell, I fastest.
+__global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
+{
+    const int ni = b.imax + 2;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= ni) return;
+    const int IS = b.is, IL = b.il;
+    const int L = cid(b, i, j, k);
+    const double *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W);
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3);
+    const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
+    double w1, w2;
+    w1 = 2.0 * (d1[L] / (d1[L] + d1[L - 1])); w2 = 2.0 - w1;
+    double u1m = a1[L] * (w2 * u[L] + w1 * u[L - 1]);
+    double v1m = a1[L] * (w2 * v[L] + w1 * v[L - 1]);
+    double w1m = a1[L] * (w2 * w[L] + w1 * w[L - 1]);
+    w1 = 2.0 * (d1[L] / (d1[L] + d1[L + 1])); w2 = 2.0 - w1;
+    double u1p = a1[L + 1] * (w2 * u[L] + w1 * u[L + 1]);
+    double v1p = a1[L + 1] * (w2 * v[L] + w1 * v[L + 1]);
+    double w1p = a1[L + 1] * (w2 * w[L] + w1 * w[L + 1]);
+    w1 = 2.0 * (d2[L] / (d2[L] + d2[L - IS])); w2 = 2.0 - w1;
+    double u2m = a2[L] * (w2 * u[L] + w1 * u[L - IS]);
+    double v2m = a2[L] * (w2 * v[L] + w1 * v[L - IS]);
+    double w2m = a2[L] * (w2 * w[L] + w1 * w[L - IS]);
+    w1 = 2.0 * (d2[L] / (d2[L] + d2[L + IS])); w2 = 2.0 - w1;
+    double u2p = a2[L + IS] * (w2 * u[L] + w1 * u[L + IS]);
+    double v2p = a2[L + IS] * (w2 * v[L] + w1 * v[L + IS]);
+    double w2p = a2[L + IS] * (w2 * w[L] + w1 * w[L + IS]);
+    w1 = 2.0 * (d3[L] / (d3[L] + d3[L - IL])); w2 = 2.0 - w1;
+    double u3m = a3[L] * (w2 * u[L] + w1 * u[L - IL]);
+    double v3m = a3[L] * (w2 * v[L] + w1 * v[L - IL]);
+    double w3m = a3[L] * (w2 * w[L] + w1 * w[L - IL]);
+    w1 = 2.0 * (d3[L] / (d3[L] + d3[L + IL])); w2 = 2.0 - w1;
+    double u3p = a3[L + IL] * (w2 * u[L] + w1 * u[L + IL]);
+    double v3p = a3[L + IL] * (w2 * v[L] + w1 * v[L + IL]);
+    double w3p = a3[L + IL] * (w2 * w[L] + w1 * w[L + IL]);
+    const double pvol = .5 / FLD(b, FF_VOL)[L];
+    double g[3][3];   // g[c][q]: d(u,v,w)_q / d(x,y,z)_c
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+        const double *c1 = FLD(b, FF_A1X + cc), *c2 = FLD(b, FF_A2X + cc), *c3 = FLD(b, FF_A3X + cc);
+        double e3p = c3[L + IL], e3m = c3[L], e2p = c2[L + IS], e2m = c2[L], e1p = c1[L + 1], e1m = c1[L];
+        g[cc][0] = pvol * (e3p * u3p - e3m * u3m + e2p * u2p - e2m * u2m + e1p * u1p - e1m * u1m);
+        g[cc][1] = pvol * (e3p * v3p - e3m * v3m + e2p * v2p - e2m * v2m + e1p * v1p - e1m * v1m);
+        g[cc][2] = pvol * (e3p * w3p - e3m * w3m + e2p * w2p - e2m * w2m + e1p * w1p - e1m * w1m);
+    }
+    const double dudx = g[0][0], dudy = g[1][0], dudz = g[2][0];
+    const double dvdx = g[0][1], dvdy = g[1][1], dvdz = g[2][1];
+    const double dwdx = g[0][2], dwdy = g[1][2], dwdz = g[2][2];
+    double w12 = .5 * (dudy - dvdx), w13 = .5 * (dudz - dwdx), w23 = .5 * (dvdz - dwdy);
+    FLD(b, FF_W12)[L] = w12; FLD(b, FF_W13)[L] = w13; FLD(b, FF_W23)[L] = w23;
+    FLD(b, FF_OHMI)[L] = 2. * sqrt(w12 * w12 + w13 * w13 + w23 * w23);
+    double s11 = .5 * (dudx + dudx), s22 = .5 * (dvdy + dvdy), s33 = .5 * (dwdz + dwdz);
+    double s12 = .5 * (dudy + dvdx), s13 = .5 * (dudz + dwdx), s23 = .5 * (dvdz + dwdy);
+    double st = s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 + s23 * s23
+              + s13 * s13 + s23 * s23 + s33 * s33;
+    const double strain = sqrt(2. * st);
+    FLD(b, FF_STRAIN)[L] = strain;
+    FLD(b, FF_S11)[L] = s11; FLD(b, FF_S12)[L] = s12; FLD(b, FF_S13)[L] = s13;
+    FLD(b, FF_S22)[L] = s22; FLD(b, FF_S23)[L] = s23; FLD(b, FF_S33)[L] = s33;
+    // TTIMES
+    const double ro = FLD(b, FF_RO)[L], vis = FLD(b, FF_VIS)[L];
+    const double rk = FLD(b, FF_RK)[L], rom = FLD(b, FF_REPS)[L];
+    {
+        const double ct = 6.0, bstar = 0.09, eps = 1.E-20;
+        double bnum = 1.0 / fabs(bstar * rom + eps);
+        double tstur = ro * bnum;
+        double tsvis = ct * sqrt(ro * vis * bnum / fabs(rk + eps));
+        FLD(b, FF_TTS)[L] = dmax(tstur, tsvis);
+    }
+    if (i < 1 || i > b.imax || j < 1 || j > b.jmax || k < 1 || k > b.kmax) return;
+    // EPS2KO
+    const SstC s = coefko(p.kover);
+    const double eps = 1.E-20;
+    const double sst = p.kover < 5 ? 1.0 : 0.0, bsl = p.kover < 5 ? 0.0 : 1.0;
+    const double frsmut = 1.E-3 * p.frsvis;
+    const double distw = FLD(b, FF_DISTW)[L];
+    double prol = 1. / ro;
+    double rkl = dmax(rk * prol, 0.0);
+    double oml = rom * prol;
+    double dis2 = distw * distw;
+    double arg21 = 2. * sqrt(rkl) / dmax(eps, s.bstar * oml * distw);
+    double auxvi = vis / dmax(eps, rom * dis2);
+    double arg22 = 500. * auxvi;
+    double arg2 = dmax(arg21, arg22);
+    double arg4 = 150. * auxvi;
+    double fun2 = tanh(arg2 * arg2);
+    double fun3 = 1. - tanh(pow4(arg4));
+    double rmubas = rk / oml;
+    double rmumen = s.a1kleb * rk / dmax(eps, strain * fun2 * fun3);
+    double rmutrb = dmin(rmubas, rmumen);
+    rmutrb = rmutrb * sst + rmubas * bsl;
+    rmutrb = dmax(rmutrb, frsmut);
+    rmutrb = dmin(rmutrb, p.turlim * vis);
+    FLD(b, FF_VIST)[L] = rmutrb;
+    double rmutpv = rmutrb / vis;
+    FLD(b, FF_EPS2)[L] = 1.0 + rmutpv;
+}
+
+// k = RK/RO and omega = REPS/RO as CROSKO forms them in place (turb3.f:7817-7821)
+__device__ __forceinline__ double per_ro(const double *x, const double *ro, int n)
+{
+    double prol = 1. / ro[n];
+    return x[n] * prol;
+}
+
+// CROSKO turb3.f:7824-7892 on the interior: SEPS = 2*SIGOM2*RO*grad(k).grad(omega)/omega,
+// with k and omega formed from the unrounded RK, REPS of each stencil cell
+__global__ void __launch_bounds__(256) k_crosko(Prm p, Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    const int IS = b.is, IL = b.il;
+    const int L = cid(b, i, j, k);
+    const SstC s = coefko(p.kover);
+    const double sigo22 = 2.0 * s.sigom2;
+    const double *RK = FLD(b, FF_RK), *OM = FLD(b, FF_REPS), *ro = FLD(b, FF_RO);
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3);
+    const double k0 = per_ro(RK, ro, L), o0 = per_ro(OM, ro, L);
+    double rk1m = a1[L] * (k0 + per_ro(RK, ro, L - 1)), rk1p = a1[L + 1] * (k0 + per_ro(RK, ro, L + 1));
+    double om1m = a1[L] * (o0 + per_ro(OM, ro, L - 1)), om1p = a1[L + 1] * (o0 + per_ro(OM, ro, L + 1));
+    double rk2m = a2[L] * (k0 + per_ro(RK, ro, L - IS)), rk2p = a2[L + IS] * (k0 + per_ro(RK, ro, L + IS));
+    double om2m = a2[L] * (o0 + per_ro(OM, ro, L - IS)), om2p = a2[L + IS] * (o0 + per_ro(OM, ro, L + IS));
+    double rk3m = a3[L] * (k0 + per_ro(RK, ro, L - IL)), rk3p = a3[L + IL] * (k0 + per_ro(RK, ro, L + IL));
+    double om3m = a3[L] * (o0 + per_ro(OM, ro, L - IL)), om3p = a3[L + IL] * (o0 + per_ro(OM, ro, L + IL));
+    const double pvol = 0.5 / FLD(b, FF_VOL)[L];
+    double dk[3], dom[3];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+        const double *c1 = FLD(b, FF_A1X + cc), *c2 = FLD(b, FF_A2X + cc), *c3 = FLD(b, FF_A3X + cc);
+        double e3p = c3[L + IL], e3m = c3[L], e2p = c2[L + IS], e2m = c2[L], e1p = c1[L + 1], e1m = c1[L];
+        dk[cc] = pvol * (e3p * rk3p - e3m * rk3m + e2p * rk2p - e2m * rk2m + e1p * rk1p - e1m * rk1m);
+        dom[cc] = pvol * (e3p * om3p - e3m * om3m + e2p * om2p - e2m * om2m + e1p * om1p - e1m * om1m);
+    }
+    double divdot = dk[0] * dom[0] + dk[1] * dom[1] + dk[2] * dom[2];
+    FLD(b, FF_SEPS)[L] = sigo22 * ro[L] * divdot / o0;
+}
+
+// The rounding CROSKO leaves on RK and REPS of every cell: (X*(1/RO))*RO (turb3.f:7895-7898).
+// Also zeroes FUN1 and SRK (RESIKE res3c.f:3673, 3730) and forms per-workgroup maxima of
+// SEPS over the linear index range FUN1KO scans (turb3.f:7423-7428).
+__global__ void __launch_bounds__(256) k_kw_round(Blk b, double *part)
+{
+    __shared__ double sh[256];
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (n < b.ntot) {
+        const double ro = FLD(b, FF_RO)[n];
+        FLD(b, FF_RK)[n] = per_ro(FLD(b, FF_RK), FLD(b, FF_RO), n) * ro;
+        FLD(b, FF_REPS)[n] = per_ro(FLD(b, FF_REPS), FLD(b, FF_RO), n) * ro;
+        FLD(b, FF_FUN1)[n] = 0.;
+        FLD(b, FF_SRK)[n] = 0.;
+        if (n >= cid(b, 1, 1, 1) && n <= cid(b, b.imax, b.jmax, b.kmax)) m = FLD(b, FF_SEPS)[n];
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = dmax(sh[threadIdx.x + h], sh[threadIdx.x]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Maximum of the partial maxima (max is exact, so the order does not matter)
+__global__ void __launch_bounds__(256) k_max_final(const double *part, int nparts, double *out)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < nparts; t += blockDim.x) m = dmax(part[t], m);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] = dmax(sh[threadIdx.x + h], sh[threadIdx.x]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// FUN1KO (turb3.f:7430-7446) and PRODKE (turb3.f:4720-4787, with MAKE_EPS turb3.f:7913)
+// on the interior
+__global__ void __launch_bounds__(256) k_fun1_prod(Prm p, Blk b, const double *cdmax)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    const int L = cid(b, i, j, k);
+    const SstC s = coefko(p.kover);
+    const double eps = 1.E-20, sigo24 = 4. * s.sigom2;
+    const double ro = FLD(b, FF_RO)[L], rk = FLD(b, FF_RK)[L], om = FLD(b, FF_REPS)[L];
+    const double vis = FLD(b, FF_VIS)[L], distw = FLD(b, FF_DISTW)[L];
+    const double cross = FLD(b, FF_SEPS)[L];
+    double cdmin = 1E-8 * cdmax[0];
+    double prol = 1. / ro;
+    double rkl = dmax(rk * prol, 0.0);
+    double oml = om * prol;
+    double dis2 = distw * distw;
+    double cdkom = dmax(cross, cdmin);
+    double arg11 = sqrt(rkl) / dmax(eps, s.bstar * oml * distw);
+    double arg12 = 500. * vis / dmax(eps, om * dis2);
+    double arg13 = dmax(arg11, arg12);
+    double arg14 = sigo24 * rk / dmax(eps, cdkom * dis2);
+    double arg1 = dmin(arg13, arg14);
+    double f1 = tanh(pow4(arg1));
+    double dislim = .40559E-7 * sqrt(p.re);
+    if (distw < dislim) f1 = 1.;
+    FLD(b, FF_FUN1)[L] = f1;
+    // PRODKE
+    const double r2o3 = 2. / 3.;
+    double s11 = FLD(b, FF_S11)[L], s12 = FLD(b, FF_S12)[L], s13 = FLD(b, FF_S13)[L];
+    double s22 = FLD(b, FF_S22)[L], s23 = FLD(b, FF_S23)[L], s33 = FLD(b, FF_S33)[L];
+    double w12 = FLD(b, FF_W12)[L], w13 = FLD(b, FF_W13)[L], w23 = FLD(b, FF_W23)[L];
+    double dudx = s11, dvdy = s22, dwdz = s33;
+    double dudy = s12 + w12, dudz = s13 + w13;
+    double dvdx = s12 - w12, dvdz = s23 + w23;
+    double dwdx = s13 - w13, dwdy = s23 - w23;
+    double strain = s11 * s11 + s12 * s12 + s13 * s13 + s12 * s12 + s22 * s22 + s23 * s23
+                  + s13 * s13 + s23 * s23 + s33 * s33;
+    strain = sqrt(2. * strain);
+    double trans = 0.;
+    double div = dudx + dvdy + dwdz;
+    double visco = (FLD(b, FF_EPS2)[L] - 1. + trans) * vis;
+    double pt;
+    if (p.katol)
+        pt = visco * strain * FLD(b, FF_OHMI)[L];
+    else
+        pt = visco * (2. * (dudx * dudx + dvdy * dvdy + dwdz * dwdz)
+                      + (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx)
+                      + (dvdz + dwdy) * (dvdz + dwdy) - r2o3 * (div * div))
+             - r2o3 * rk * div;
+    double trueps = 0.09 * om * rk / ro;
+    pt = dmin(pt, 20. * trueps);
+    FLD(b, FF_PTUR)[L] = pt * 1.;
+}
+
+// SOURKO (turb3.f:7693-7778, RANS, KOVER 0/5) for one interior cell: returns SRK and the
+// new SEPS (SEPS holding the cross-diffusion on entry)
+__device__ __forceinline__ void sourko_cell(const Prm &p, const Blk &b, int L, double &srk,
+                                            double &som)
+{
+    const SstC s = coefko(p.kover);
+    const double eps = 1.0E-10;
+    const double pbstar = 1.0 / s.bstar, persbs = 1.0 / sqrt(s.bstar);
+    const double rotco = 0.0, rich = 0.0, bero = 3.6;
+    const double f1 = FLD(b, FF_FUN1)[L], ro = FLD(b, FF_RO)[L], rk = FLD(b, FF_RK)[L];
+    const double rom = FLD(b, FF_REPS)[L], ptur = FLD(b, FF_PTUR)[L];
+    double onemf1 = dmax(0.0, (1.0 - f1));
+    double beta = f1 * s.beta1 + onemf1 * s.beta2;
+    double sigom = f1 * s.sigom1 + onemf1 * s.sigom2;
+    double gamkom = beta * pbstar - sigom * (s.ckappa * s.ckappa) * persbs;
+    double rmutrb = dmax(eps, FLD(b, FF_EPS2)[L] - 1.0) * FLD(b, FF_VIS)[L];
+    double fco = dmax(0.0, (1.0 - rotco)) + dmax(0.0, rotco / (1.0 + bero * rich));
+    double trueomega = rom;
+    double prol = 1.0 / ro;
+    som = onemf1 * FLD(b, FF_SEPS)[L];
+    srk = ptur - s.bstar * rk * trueomega * prol;
+    double omepro = gamkom * ro * ptur / (rmutrb * rom);
+    double omedis = fco * beta * prol * trueomega;
+    som = som + rom * (omepro - omedis);
+}
+
+// RESIKE face fluxes (FLXCYC res3c.f:6424-7566 for k, i, j with SWEEPK) and SOURKO/ADDSKE
+// (res3c.f:4957-5068), with every face flux formed once: the layout of k_residual_tile with
+// seven components.
+template <int RIW, int RJW, int RKW>
+__global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
+{
+    __shared__ double fi[7][RJW][RIW + 1];
+    __shared__ double fj[7][RJW + 1][RIW];
+    const SstC sc = coefko(p.kover);
+    const int ti = threadIdx.x % RIW, tj = threadIdx.x / RIW;
+    const int i0 = blockIdx.x * RIW + 1, j0 = blockIdx.y * RJW + 1, k0 = blockIdx.z * RKW + 1;
+    const int i = i0 + ti, j = j0 + tj;
+    const bool cell = i <= b.imax && j <= b.jmax;
+    const int il = min(RIW, b.imax - i0 + 1), jl = min(RJW, b.jmax - j0 + 1);
+    const int k1 = min(k0 + RKW - 1, b.kmax);
+    const double *a3 = FLD(b, FF_A3), *a3x = FLD(b, FF_A3X), *a3y = FLD(b, FF_A3Y), *a3z = FLD(b, FF_A3Z);
+    const double *a1 = FLD(b, FF_A1), *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z);
+    const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
+    Flux7 klo;
+    if (cell) klo = face_flux_kw<true>(p, sc, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z);
+    for (int k = k0; k <= k1; k++) {
+        const int n = cid(b, i, j, k);
+        Flux7 khi;
+        if (cell) {
+            khi = face_flux_kw<true>(p, sc, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z);
+            Flux7 f = face_flux_kw<false>(p, sc, b, n, 1, p.inter[0], a1, a1x, a1y, a1z);
+#pragma unroll
+            for (int q = 0; q < 7; q++) fi[q][tj][ti] = f.f[q];
+            f = face_flux_kw<false>(p, sc, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z);
+#pragma unroll
+            for (int q = 0; q < 7; q++) fj[q][tj][ti] = f.f[q];
+        }
+        if (tj == 0 && ti < il) {
+            Flux7 f = face_flux_kw<false>(p, sc, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z);
+#pragma unroll
+            for (int q = 0; q < 7; q++) fj[q][jl][ti] = f.f[q];
+        } else if (tj == 1 && ti < jl) {
+            Flux7 f = face_flux_kw<false>(p, sc, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z);
+#pragma unroll
+            for (int q = 0; q < 7; q++) fi[q][ti][il] = f.f[q];
+        }
+        __syncthreads();
+        if (cell) {
+            double dt1 = 1. / FLD(b, FF_VOL)[n];
+            double d[7];
+#pragma unroll
+            for (int q = 0; q < 7; q++) {
+                double x = 0.;
+                x = x - (khi.f[q] - klo.f[q]);                          // SWEEPK, K
+                x = x - (fi[q][tj][ti + 1] - fi[q][tj][ti]);             // SWEEPK, I
+                d[q] = (x - (fj[q][tj + 1][ti] - fj[q][tj][ti])) * dt1;  // SWEEPK, J, /VOL
+            }
+            double srk, som;
+            sourko_cell(p, b, n, srk, som);
+            FLD(b, FF_SRK)[n] = srk;
+            FLD(b, FF_SEPS)[n] = som;
+            d[5] = d[5] + srk;                                          // ADDSKE
+            d[6] = d[6] + som;
+#pragma unroll
+            for (int q = 0; q < 7; q++) FLD(b, FF_DRO + q)[n] = d[q];
+            klo = khi;
+        }
+        __syncthreads();
+    }
 }
 
 // Thread per interior cell.  The residual of a cell is
@@ -208,13 +556,17 @@ __global__ void __launch_bounds__(256) k_zero(double *x, int n)
     if (t < n) x[t] = 0.;
 }
 
-// RESL2 stage 1: per-workgroup partial sums of D^2 over interior cells
+// RESL2 stage 1: per-workgroup partial sums of D^2 over interior cells (res3c.f:8979-9006);
+// NQ = 7 adds D(rho k)^2 and MIN(|D(rho omega)|,1E15)^2 of the two-equation models
 constexpr int RED_THREADS = 256;
+template <int NQ>
 __global__ void __launch_bounds__(RED_THREADS) k_resl2_part(Blk b, double *part)
 {
-    __shared__ double sh[5][RED_THREADS];
+    __shared__ double sh[NQ][RED_THREADS];
     int ncell = b.imax * b.jmax * b.kmax;
-    double s[5] = {0., 0., 0., 0., 0.};
+    double s[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) s[q] = 0.;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += gridDim.x * blockDim.x) {
         int i = t % b.imax + 1;
         int r = t / b.imax;
@@ -222,34 +574,35 @@ __global__ void __launch_bounds__(RED_THREADS) k_resl2_part(Blk b, double *part)
         int k = r / b.jmax + 1;
         int n = cid(b, i, j, k);
 #pragma unroll
-        for (int q = 0; q < 5; q++) {
+        for (int q = 0; q < NQ; q++) {
             double x = FLD(b, FF_DRO + q)[n];
+            if (q == 6) x = dmin(fabs(x), 1.E15);
             s[q] += x * x;
         }
     }
 #pragma unroll
-    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = s[q];
+    for (int q = 0; q < NQ; q++) sh[q][threadIdx.x] = s[q];
     __syncthreads();
     for (int h = RED_THREADS / 2; h > 0; h >>= 1) {
         if (threadIdx.x < h) {
 #pragma unroll
-            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
+            for (int q = 0; q < NQ; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < 5; q++) part[5 * blockIdx.x + q] = sh[q][0];
+        for (int q = 0; q < NQ; q++) part[7 * blockIdx.x + q] = sh[q][0];
     }
 }
 
 // RESL2 stage 2: fixed-order sum of the partials (run-to-run deterministic)
-__global__ void __launch_bounds__(64) k_resl2_final(const double *part, int nparts, double *out)
+__global__ void __launch_bounds__(64) k_resl2_final(const double *part, int nparts, int nq, double *out)
 {
     int q = threadIdx.x;
-    if (q >= 5) return;
+    if (q >= nq) return;
     double s = 0.;
-    for (int i = 0; i < nparts; i++) s += part[5 * i + q];
+    for (int i = 0; i < nparts; i++) s += part[7 * i + q];
     out[q] = s;
 }
 
@@ -308,6 +661,53 @@ __global__ void __launch_bounds__(256) k_time3_copr(Prm p, Blk b)
     denlow = denlow > denmax ? denlow : denmax;
     double dtl = 1. / (b11 * denlow);
     FLD(b, FF_DTL)[kk] = dtl;
+    if (p.iturb == 6) {
+        // LOCDTK (imps3.f:820-826, DTURB = 1)
+        const double dturb = 1.;
+        double dri = dtl * FLD(b, FF_DRO)[kk];
+        double dmi = dtl * FLD(b, FF_DM)[kk];
+        double dni = dtl * FLD(b, FF_DN)[kk];
+        double dwi = dtl * FLD(b, FF_DW)[kk];
+        double dre = dtl * FLD(b, FF_DE)[kk];
+        double dki = dturb * dtl * FLD(b, FF_DRK)[kk];
+        double dep = dturb * dtl * FLD(b, FF_DEPS)[kk];
+        // IMSOKO (imps3.f:972-1020)
+        const double rk = FLD(b, FF_RK)[kk], rom = FLD(b, FF_REPS)[kk];
+        {
+            const SstC s = coefko(p.kover);
+            const double epsi = 1.0E-10;
+            const double gamkom = s.beta1 / s.bstar - s.sigom1 * (s.ckappa * s.ckappa) / (sqrt(s.bstar));
+            const double ckmax = 1.0 / (2.0 * p.csimps), comax = 1.0 / (2.0 * p.csimps);
+            double emeanf = FLD(b, FF_E)[kk] - rk;
+            double drkmax = dmax(ckmax * dmin(rk, fabs(0.1 * emeanf - 0.9 * rk)), epsi);
+            double dommax = comax * rom;
+            double prol = 1. / ro;
+            double atprod = fabs(FLD(b, FF_PTUR)[kk]);
+            double rnutrb = dmax((FLD(b, FF_EPS2)[kk] - 1.0), epsi) * FLD(b, FF_VIS)[kk] * prol;
+            double drkden = 1.0 + dtl * (atprod / drkmax + prol * s.bstar * rom);
+            double domden = 1.0 + dtl * (gamkom * atprod / (rnutrb * dommax) + 2. * s.beta1 * rom);
+            dki = dki / drkden;
+            dep = dep / domden;
+        }
+        // COPRKE (imps3.f:1145-1205, SOLUTION_TYPE = FLUID)
+        double vel2 = (u * u + v * v + w * w);
+        double drodh = FLD(b, FF_DRDH)[kk], drodp = FLD(b, FF_DRDP)[kk];
+        double yp = 1. / ro;
+        double rki = rk * yp;
+        double dkin = u * dmi + v * dni + w * dwi + dki;
+        FLD(b, FF_DM)[kk] = yp * (-u * dri + dmi);
+        FLD(b, FF_DN)[kk] = yp * (-v * dri + dni);
+        FLD(b, FF_DW)[kk] = yp * (-w * dri + dwi);
+        FLD(b, FF_DRK)[kk] = yp * (-rki * dri + dki);
+        FLD(b, FF_DEPS)[kk] = yp * (-rom * yp * dri + dep);
+        double hi = yp * (FLD(b, FF_E)[kk] + P);
+        double dei = dre - (hi - vel2 - rki) * dri - dkin;
+        double dpi = (c * c) * (dri - drodh * yp * dei);
+        double dhi = (dri - drodp * dpi) / drodh;
+        FLD(b, FF_DRO)[kk] = dhi;
+        FLD(b, FF_DE)[kk] = dpi + .667 * dki;
+        return;
+    }
     // LOCDT
     double dri = dtl * FLD(b, FF_DRO)[kk];
     double dmi = dtl * FLD(b, FF_DM)[kk];
@@ -510,47 +910,51 @@ __global__ void __launch_bounds__(128) k_corr_plane(Prm p, Blk b, int m, int jlo
 constexpr int TJ = 8, TK = 8, TW = TJ * TK;
 constexpr unsigned long long HALO_EMPTY = 0x7FF4DEADBEEF0001ULL;
 constexpr int SPIN_LIMIT = 1 << 22;
+constexpr int HS = 7;              // halo words per cell face (7 unknowns of the k-omega form)
 
 struct Sweep {
     int ntj, ntk;                   // tiles along J and K
     unsigned long long *ticket;     // monotonic; ticket % (ntj*ntk) orders the tiles
     unsigned int *err;              // non-zero after a halo wait gave up
-    unsigned long long *hj, *hk;    // halo slots: J-faces (ntj+1, KMAX, IMAX, 5), K-faces
+    unsigned long long *hj, *hk;    // halo slots: J-faces (ntj+1, KMAX, IMAX, HS), K-faces
 };
 
 __device__ __forceinline__ unsigned long long *hj_slot(const Sweep &s, const Blk &b, int f, int i,
                                                        int k)
 {
-    return s.hj + 5 * (((size_t)f * b.kmax + (k - 1)) * b.imax + (i - 1));
+    return s.hj + HS * (((size_t)f * b.kmax + (k - 1)) * b.imax + (i - 1));
 }
 
 __device__ __forceinline__ unsigned long long *hk_slot(const Sweep &s, const Blk &b, int f, int i,
                                                        int j)
 {
-    return s.hk + 5 * (((size_t)f * b.jmax + (j - 1)) * b.imax + (i - 1));
+    return s.hk + HS * (((size_t)f * b.jmax + (j - 1)) * b.imax + (i - 1));
 }
 
-__device__ __forceinline__ void halo_put(unsigned long long *slot, const double v[5])
+template <int NV>
+__device__ __forceinline__ void halo_put(unsigned long long *slot, const double v[NV])
 {
 #pragma unroll
-    for (int q = 0; q < 5; q++)
+    for (int q = 0; q < NV; q++)
         __hip_atomic_store(slot + q, (unsigned long long)__double_as_longlong(v[q]),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void halo_issue(unsigned long long *slot, unsigned long long h[5])
+template <int NV>
+__device__ __forceinline__ void halo_issue(unsigned long long *slot, unsigned long long h[NV])
 {
 #pragma unroll
-    for (int q = 0; q < 5; q++)
+    for (int q = 0; q < NV; q++)
         h[q] = __hip_atomic_load(slot + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Complete a halo read begun by halo_issue: poll empty slots, then empty them.
-__device__ __forceinline__ void halo_finish(unsigned long long *slot, unsigned long long h[5],
-                                            unsigned int *err, double v[5])
+template <int NV>
+__device__ __forceinline__ void halo_finish(unsigned long long *slot, unsigned long long h[NV],
+                                            unsigned int *err, double v[NV])
 {
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < NV; q++) {
         unsigned long long x = h[q];
         int spins = 0;
         while (x == HALO_EMPTY) {
@@ -590,54 +994,125 @@ __device__ __forceinline__ int tile_ticket(const Sweep &s)
     return (int)(t % (unsigned long long)(s.ntj * s.ntk));
 }
 
-// Values cell n of the backward sweep reads (PREDGS loop 1100 and the diagonal loop)
-struct PredRaw {
-    double d[5], u, v, w, c, ro, vol, vis, vist, dtl;
-    double vol1, ro1, vis1, vist1, vol2, ro2, vis2, vist2, vol3, ro3, vis3, vist3;  // n-1, n-IS, n-IL
-    double a1, a2, a3, n1x, n1y, n1z, n2x, n2y, n2z, n3x, n3y, n3z;               // lower faces
-    double a1u, a2u, a3u;                                                         // upper areas
+// Values cell n of a sweep reads: its increments and state, the neighbours' VOL, RO, VIS,
+// VIST across the three faces its coefficients use, and the face geometry.  `own` holds the
+// faces the cell's coefficients and outgoing contributions use (lower faces in PREDGS, upper
+// in CORRGS), `oth` the other three.
+struct SweepRaw {
+    double d[7], u, v, w, c, ro, vol, vis, vist, dtl, cp, rk, drdp, drdh;
+    double vol1, ro1, vis1, vist1, vol2, ro2, vis2, vist2, vol3, ro3, vis3, vist3;
+    double a1, a2, a3, n1x, n1y, n1z, n2x, n2y, n2z, n3x, n3y, n3z;   // lower faces
+    double a1u, a2u, a3u, u1x, u1y, u1z, u2x, u2y, u2z, u3x, u3y, u3z; // upper faces
 };
 
-__device__ __forceinline__ void pred_load(const Blk &b, int n, PredRaw &r)
+// FWD = false: PREDGS (neighbours n-1, n-IS, n-IL); true: CORRGS (n+1, n+IS, n+IL)
+template <bool FWD, bool KE>
+__device__ __forceinline__ void sweep_load(const Blk &b, int n, SweepRaw &r)
 {
     const int IS = b.is, IL = b.il;
+    constexpr int NV = KE ? 7 : 5;
+    const int s1 = FWD ? 1 : -1, s2 = FWD ? IS : -IS, s3 = FWD ? IL : -IL;
 #pragma unroll
-    for (int q = 0; q < 5; q++) r.d[q] = FLD(b, FF_DRO + q)[n];
+    for (int q = 0; q < NV; q++) r.d[q] = FLD(b, FF_DRO + q)[n];
     r.u = FLD(b, FF_U)[n]; r.v = FLD(b, FF_V)[n]; r.w = FLD(b, FF_W)[n];
     r.c = FLD(b, FF_C)[n]; r.ro = FLD(b, FF_RO)[n]; r.vol = FLD(b, FF_VOL)[n];
     r.vis = FLD(b, FF_VIS)[n]; r.vist = FLD(b, FF_VIST)[n]; r.dtl = FLD(b, FF_DTL)[n];
-    r.vol1 = FLD(b, FF_VOL)[n - 1]; r.ro1 = FLD(b, FF_RO)[n - 1];
-    r.vis1 = FLD(b, FF_VIS)[n - 1]; r.vist1 = FLD(b, FF_VIST)[n - 1];
-    r.vol2 = FLD(b, FF_VOL)[n - IS]; r.ro2 = FLD(b, FF_RO)[n - IS];
-    r.vis2 = FLD(b, FF_VIS)[n - IS]; r.vist2 = FLD(b, FF_VIST)[n - IS];
-    r.vol3 = FLD(b, FF_VOL)[n - IL]; r.ro3 = FLD(b, FF_RO)[n - IL];
-    r.vis3 = FLD(b, FF_VIS)[n - IL]; r.vist3 = FLD(b, FF_VIST)[n - IL];
+    if (FWD) r.cp = FLD(b, FF_CP)[n];
+    if (KE) {
+        r.rk = FLD(b, FF_RK)[n]; r.drdp = FLD(b, FF_DRDP)[n]; r.drdh = FLD(b, FF_DRDH)[n];
+    }
+    r.vol1 = FLD(b, FF_VOL)[n + s1]; r.ro1 = FLD(b, FF_RO)[n + s1];
+    r.vis1 = FLD(b, FF_VIS)[n + s1]; r.vist1 = FLD(b, FF_VIST)[n + s1];
+    r.vol2 = FLD(b, FF_VOL)[n + s2]; r.ro2 = FLD(b, FF_RO)[n + s2];
+    r.vis2 = FLD(b, FF_VIS)[n + s2]; r.vist2 = FLD(b, FF_VIST)[n + s2];
+    r.vol3 = FLD(b, FF_VOL)[n + s3]; r.ro3 = FLD(b, FF_RO)[n + s3];
+    r.vis3 = FLD(b, FF_VIS)[n + s3]; r.vist3 = FLD(b, FF_VIST)[n + s3];
     r.a1 = FLD(b, FF_A1)[n]; r.a2 = FLD(b, FF_A2)[n]; r.a3 = FLD(b, FF_A3)[n];
     r.n1x = FLD(b, FF_A1X)[n]; r.n1y = FLD(b, FF_A1Y)[n]; r.n1z = FLD(b, FF_A1Z)[n];
     r.n2x = FLD(b, FF_A2X)[n]; r.n2y = FLD(b, FF_A2Y)[n]; r.n2z = FLD(b, FF_A2Z)[n];
     r.n3x = FLD(b, FF_A3X)[n]; r.n3y = FLD(b, FF_A3Y)[n]; r.n3z = FLD(b, FF_A3Z)[n];
     r.a1u = FLD(b, FF_A1)[n + 1]; r.a2u = FLD(b, FF_A2)[n + IS]; r.a3u = FLD(b, FF_A3)[n + IL];
+    if (FWD) {
+        r.u1x = FLD(b, FF_A1X)[n + 1]; r.u1y = FLD(b, FF_A1Y)[n + 1]; r.u1z = FLD(b, FF_A1Z)[n + 1];
+        r.u2x = FLD(b, FF_A2X)[n + IS]; r.u2y = FLD(b, FF_A2Y)[n + IS]; r.u2z = FLD(b, FF_A2Z)[n + IS];
+        r.u3x = FLD(b, FF_A3X)[n + IL]; r.u3y = FLD(b, FF_A3Y)[n + IL]; r.u3z = FLD(b, FF_A3Z)[n + IL];
+    }
 }
 
 // Recurrence-independent part of a cell's LU-SGS update
 struct SweepCell {
-    double d[5], dt1, vol, ro, roc, pc2, proc, scinv, scinp;
-    double wi, wj, wk;             // areas weighting the incoming I-, J-, K-contributions
-    FaceBasis fi, fj, fk;          // faces the outgoing contributions cross
+    double d[7], dt1, vol, ro, roc, pc2, proc, scinv, scinp;
+    double yp, ydrp, drhro, ro667;  // SYIKE/SYM1KE scalars (k-omega form)
+    double wi, wj, wk;              // areas weighting the incoming I-, J-, K-contributions
+    FaceBasis fi, fj, fk;           // faces the outgoing contributions cross
     Coef ci, cj, ck;
-    double cp;                     // CORRGS only: DRO/CP at the end of IMPS
+    double cp, rho;                 // CORRGS only: DRO/CP and D(rho k), D(rho omega) * RO
 };
 
-__device__ __forceinline__ void cell_scalars(SweepCell &s, double ro, double c)
+template <bool KE>
+__device__ __forceinline__ void cell_scalars(SweepCell &s, const SweepRaw &r)
 {
+    const double ro = r.ro, c = r.c;
     s.ro = ro;
-    s.roc = ro * c;                // SYI imps4.f:31
-    double ypro = 1. / ro;         // SYM1I imps4.f:57-59
+    s.roc = ro * c;                // SYI imps4.f:31, SYIKE imps4.f:1015
+    double ypro = 1. / ro;         // SYM1I imps4.f:57-59, SYM1KE imps4.f:1050-1052
     s.pc2 = .5 * ypro;
     s.proc = s.pc2 / c;
+    if (KE) {
+        s.yp = ypro;               // SYIKE's PC2 = 1./RO
+        double drh = .667 * r.rk * (ypro * ypro) * r.drdh;   // SYM1KE imps4.f:1053-1054
+        s.ydrp = 1. / (1. + .667 * r.rk * ypro * r.drdp);
+        s.drhro = drh * ro;
+        s.ro667 = .667 * ro;
+    }
 }
 
-__device__ __forceinline__ void pred_prep(const Prm &p, const PredRaw &r, int i, int j, int k,
+// SYIKE / SYM1KE (imps4.f:982-1072) around the split-eigenvalue scaling, with the basis and
+// the cell scalars formed beforehand
+__device__ __forceinline__ void face_contrib_ke_b(const double dw[7], const SweepCell &s,
+                                                  const FaceBasis &f, double x1, double x2,
+                                                  double x5, double out[7])
+{
+    double z1 = dw[0], z5 = dw[4], z6 = dw[5], z7 = dw[6];
+    double z2 = f.nx * dw[1] + f.ny * dw[2] + f.nz * dw[3];
+    double z3 = f.a21 * dw[1] + f.a22 * dw[2] + f.a23 * dw[3];
+    double z4 = f.a31 * dw[1] + f.a32 * dw[2] + f.a33 * dw[3];
+    double w1 = z1 - s.yp * z5;
+    double w2 = s.roc * z2 + z5;
+    double w3 = z3, w4 = z4, w6 = z6, w7 = z7;
+    double w5 = -s.roc * z2 + z5;
+    w1 = x1 * w1;
+    w2 = x2 * w2;
+    w3 = x1 * w3;
+    w4 = x1 * w4;
+    w5 = x5 * w5;
+    w6 = x1 * w6;
+    w7 = x1 * w7;
+    double dh = w1 + s.pc2 * (w2 + w5);
+    double x2i = s.proc * (w2 - w5);
+    double x5v = .5 * (w2 + w5);
+    x5v = s.ydrp * (x5v - s.drhro * dh - s.ro667 * w6);
+    out[0] = dh;
+    out[1] = f.nx * x2i + f.a21 * w3 + f.a31 * w4;
+    out[2] = f.ny * x2i + f.a22 * w3 + f.a32 * w4;
+    out[3] = f.nz * x2i + f.a23 * w3 + f.a33 * w4;
+    out[4] = x5v;
+    out[5] = w6;
+    out[6] = w7;
+}
+
+template <bool KE>
+__device__ __forceinline__ void face_xfer(const double *dn, const SweepCell &s, const FaceBasis &f,
+                                          const Coef &cf, double *out)
+{
+    if (KE)
+        face_contrib_ke_b(dn, s, f, cf.x1, cf.x2, cf.x5, out);
+    else
+        face_contrib_b(dn, s.ro, s.roc, s.pc2, s.proc, f, cf.x1, cf.x2, cf.x5, out);
+}
+
+template <bool KE>
+__device__ __forceinline__ void pred_prep(const Prm &p, const SweepRaw &r, int i, int j, int k,
                                           SweepCell &s)
 {
     const double cdiff = 2.;
@@ -653,51 +1128,20 @@ __device__ __forceinline__ void pred_prep(const Prm &p, const PredRaw &r, int i,
     s.scinv = 1. / (r.vol + s.dt1 * (s.ci.xa * r.a1 + s.cj.xa * r.a2 + s.ck.xa * r.a3) + r.vol * 0.);
     s.scinp = 1. / (r.vol + s.dt1 * (s.ci.rmax * r.a1 + s.cj.rmax * r.a2 + s.ck.rmax * r.a3) + r.vol * 0.);
 #pragma unroll
-    for (int q = 0; q < 5; q++) s.d[q] = r.d[q];
-    cell_scalars(s, r.ro, r.c);
+    for (int q = 0; q < (KE ? 7 : 5); q++) s.d[q] = r.d[q];
+    cell_scalars<KE>(s, r);
     s.wi = r.a1u; s.wj = r.a2u; s.wk = r.a3u;
     s.fi = face_basis(r.n1x, r.n1y, r.n1z);
     s.fj = face_basis(r.n2x, r.n2y, r.n2z);
     s.fk = face_basis(r.n3x, r.n3y, r.n3z);
 }
 
-// Values cell n of the forward sweep reads (DIAGGS, CORRGS loop 1100 and the diagonals)
-struct CorrRaw {
-    double d[5], u, v, w, c, ro, vol, vis, vist, dtl, cp;
-    double vol1, ro1, vis1, vist1, vol2, ro2, vis2, vist2, vol3, ro3, vis3, vist3;  // n+1, n+IS, n+IL
-    double a1, a2, a3, n1x, n1y, n1z, n2x, n2y, n2z, n3x, n3y, n3z;               // lower faces
-    double a1u, a2u, a3u, u1x, u1y, u1z, u2x, u2y, u2z, u3x, u3y, u3z;            // upper faces
-};
-
-__device__ __forceinline__ void corr_load(const Blk &b, int n, CorrRaw &r)
+template <bool KE>
+__device__ __forceinline__ void corr_prep(const Prm &p, const SweepRaw &r, SweepCell &s)
 {
-    const int IS = b.is, IL = b.il;
-#pragma unroll
-    for (int q = 0; q < 5; q++) r.d[q] = FLD(b, FF_DRO + q)[n];
-    r.u = FLD(b, FF_U)[n]; r.v = FLD(b, FF_V)[n]; r.w = FLD(b, FF_W)[n];
-    r.c = FLD(b, FF_C)[n]; r.ro = FLD(b, FF_RO)[n]; r.vol = FLD(b, FF_VOL)[n];
-    r.vis = FLD(b, FF_VIS)[n]; r.vist = FLD(b, FF_VIST)[n]; r.dtl = FLD(b, FF_DTL)[n];
-    r.cp = FLD(b, FF_CP)[n];
-    r.vol1 = FLD(b, FF_VOL)[n + 1]; r.ro1 = FLD(b, FF_RO)[n + 1];
-    r.vis1 = FLD(b, FF_VIS)[n + 1]; r.vist1 = FLD(b, FF_VIST)[n + 1];
-    r.vol2 = FLD(b, FF_VOL)[n + IS]; r.ro2 = FLD(b, FF_RO)[n + IS];
-    r.vis2 = FLD(b, FF_VIS)[n + IS]; r.vist2 = FLD(b, FF_VIST)[n + IS];
-    r.vol3 = FLD(b, FF_VOL)[n + IL]; r.ro3 = FLD(b, FF_RO)[n + IL];
-    r.vis3 = FLD(b, FF_VIS)[n + IL]; r.vist3 = FLD(b, FF_VIST)[n + IL];
-    r.a1 = FLD(b, FF_A1)[n]; r.a2 = FLD(b, FF_A2)[n]; r.a3 = FLD(b, FF_A3)[n];
-    r.n1x = FLD(b, FF_A1X)[n]; r.n1y = FLD(b, FF_A1Y)[n]; r.n1z = FLD(b, FF_A1Z)[n];
-    r.n2x = FLD(b, FF_A2X)[n]; r.n2y = FLD(b, FF_A2Y)[n]; r.n2z = FLD(b, FF_A2Z)[n];
-    r.n3x = FLD(b, FF_A3X)[n]; r.n3y = FLD(b, FF_A3Y)[n]; r.n3z = FLD(b, FF_A3Z)[n];
-    r.a1u = FLD(b, FF_A1)[n + 1]; r.a2u = FLD(b, FF_A2)[n + IS]; r.a3u = FLD(b, FF_A3)[n + IL];
-    r.u1x = FLD(b, FF_A1X)[n + 1]; r.u1y = FLD(b, FF_A1Y)[n + 1]; r.u1z = FLD(b, FF_A1Z)[n + 1];
-    r.u2x = FLD(b, FF_A2X)[n + IS]; r.u2y = FLD(b, FF_A2Y)[n + IS]; r.u2z = FLD(b, FF_A2Z)[n + IS];
-    r.u3x = FLD(b, FF_A3X)[n + IL]; r.u3y = FLD(b, FF_A3Y)[n + IL]; r.u3z = FLD(b, FF_A3Z)[n + IL];
-}
-
-__device__ __forceinline__ void corr_prep(const Prm &p, const CorrRaw &r, SweepCell &s)
-{
+    constexpr int NV = KE ? 7 : 5;
     const double cdiff = 2.;
-    // DIAGGS imps4.f:621-659
+    // DIAGGS imps4.f:621-659 / DIKEGS imps4.f:1877-1912
     double ucon;
     ucon = r.u1x * r.u + r.u1y * r.v + r.u1z * r.w - 0.;
     double r1n = 1.00 * (fabs(ucon) + r.c);
@@ -715,8 +1159,8 @@ __device__ __forceinline__ void corr_prep(const Prm &p, const CorrRaw &r, SweepC
     double sc = .5 * (r.a1 * r1p + r.a1u * r1n + r.a2 * r2p + r.a2u * r2n + r.a3 * r3p + r.a3u * r3n);
     double fac = 1. + dtv * sc + 0.;
 #pragma unroll
-    for (int q = 0; q < 5; q++) s.d[q] = r.d[q] * fac;
-    // CORRGS own coefficients (upper faces, viscous area A(n))
+    for (int q = 0; q < NV; q++) s.d[q] = r.d[q] * fac;
+    // CORRGS / COKEGS own coefficients (upper faces, viscous area A(n))
     s.ci = coef_vals<false>(r.u1x, r.u1y, r.u1z, r.u, r.v, r.w, r.c, r.a1, r.vol, r.vol1, r.ro,
                             r.ro1, r.vis, r.vist, r.vis1, r.vist1, 2. * cdiff);
     s.cj = coef_vals<false>(r.u2x, r.u2y, r.u2z, r.u, r.v, r.w, r.c, r.a2, r.vol, r.vol2, r.ro,
@@ -727,18 +1171,22 @@ __device__ __forceinline__ void corr_prep(const Prm &p, const CorrRaw &r, SweepC
     s.vol = r.vol;
     s.scinv = 1. / (r.vol + s.dt1 * (s.ci.xa * r.a1u + s.cj.xa * r.a2u + s.ck.xa * r.a3u) + r.vol * 0.);
     s.scinp = 1. / (r.vol + s.dt1 * (s.ci.rmax * r.a1 + s.cj.rmax * r.a2 + s.ck.rmax * r.a3) + r.vol * 0.);
-    cell_scalars(s, r.ro, r.c);
+    cell_scalars<KE>(s, r);
     s.wi = r.a1; s.wj = r.a2; s.wk = r.a3;
     s.fi = face_basis(r.u1x, r.u1y, r.u1z);
     s.fj = face_basis(r.u2x, r.u2y, r.u2z);
     s.fk = face_basis(r.u3x, r.u3y, r.u3z);
     s.cp = r.cp;
+    s.rho = r.ro;
 }
 
-// Backward sweep PREDGS (imps4.f:345-598) as a tile wavefront; I, J, K descending.
+// Backward sweep PREDGS (imps4.f:345-598) or PRKEGS (imps4.f:1462-1841, KE) as a tile
+// wavefront; I, J, K descending.
+template <bool KE>
 __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
 {
-    __shared__ double xj[2][5][TW], xk[2][5][TW];
+    constexpr int NV = KE ? 7 : 5;
+    __shared__ double xj[2][NV][TW], xk[2][NV][TW];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
     const int t = tile_ticket(sw);
     const int tj = sw.ntj - 1 - t % sw.ntj, tk = sw.ntk - 1 - t / sw.ntj;
@@ -746,73 +1194,75 @@ __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
     const bool col = j <= b.jmax && k <= b.kmax;
     const int lag = (TJ - 1 - jj) + (TK - 1 - kk);
     const int nsteps = b.imax + TJ + TK - 2;
-    double yi[5] = {0., 0., 0., 0., 0.};
-    PredRaw cur;
-    if (col) pred_load(b, cid(b, b.imax, j, k), cur);
+    double yi[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yi[r] = 0.;
+    SweepRaw cur;
+    if (col) sweep_load<false, KE>(b, cid(b, b.imax, j, k), cur);
     for (int s = 0; s < nsteps; s++) {
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = b.imax - q;
             const int n = cid(b, i, j, k);
             const bool hjin = jj == TJ - 1 && j < b.jmax, hkin = kk == TK - 1 && k < b.kmax;
-            unsigned long long hv[5], hw[5];
-            if (hjin) halo_issue(hj_slot(sw, b, tj + 1, i, k), hv);
-            if (hkin) halo_issue(hk_slot(sw, b, tk + 1, i, j), hw);
-            PredRaw nxt;
-            if (q + 1 < b.imax) pred_load(b, n - 1, nxt);
+            unsigned long long hv[NV], hw[NV];
+            if (hjin) halo_issue<NV>(hj_slot(sw, b, tj + 1, i, k), hv);
+            if (hkin) halo_issue<NV>(hk_slot(sw, b, tk + 1, i, j), hw);
+            SweepRaw nxt;
+            if (q + 1 < b.imax) sweep_load<false, KE>(b, n - 1, nxt);
             SweepCell sc;
-            pred_prep(p, cur, i, j, k, sc);
+            pred_prep<KE>(p, cur, i, j, k, sc);
             const int rb = (s + 1) & 1;
-            double bj[5], ckc[5];
+            double bj[NV], ckc[NV];
             if (j == b.jmax) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) bj[r] = 0.;
+                for (int r = 0; r < NV; r++) bj[r] = 0.;
             } else if (!hjin) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) bj[r] = xj[rb][r][lane + 1];
+                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + 1];
             } else {
-                halo_finish(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
+                halo_finish<NV>(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
             }
             if (k == b.kmax) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) ckc[r] = 0.;
+                for (int r = 0; r < NV; r++) ckc[r] = 0.;
             } else if (!hkin) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) ckc[r] = xk[rb][r][lane + TJ];
+                for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane + TJ];
             } else {
-                halo_finish(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
+                halo_finish<NV>(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
             }
             // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
-            // contribution to a cell of the row J = JMAX
-            if (j == b.jmax) {
+            // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
+            if (!KE && j == b.jmax) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) yi[r] = 0.;
+                for (int r = 0; r < NV; r++) yi[r] = 0.;
             }
-            double dn[5];
+            double dn[NV];
 #pragma unroll
-            for (int r = 0; r < 5; r++) {
+            for (int r = 0; r < NV; r++) {
                 double cd = -sc.dt1 * sc.wk * ckc[r] + sc.vol * sc.d[r];
                 cd = cd - sc.dt1 * (sc.wi * yi[r] + sc.wj * bj[r]);
-                dn[r] = r < 4 ? cd * sc.scinv : cd * sc.scinp;
+                dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
             }
 #pragma unroll
-            for (int r = 0; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
-            double oj[5], ok[5];
-            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fi, sc.ci.x1, sc.ci.x2, sc.ci.x5, yi);
-            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fj, sc.cj.x1, sc.cj.x2, sc.cj.x5, oj);
-            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fk, sc.ck.x1, sc.ck.x2, sc.ck.x5, ok);
+            for (int r = 0; r < NV; r++) FLD(b, FF_DRO + r)[n] = dn[r];
+            double oj[NV], ok[NV];
+            face_xfer<KE>(dn, sc, sc.fi, sc.ci, yi);
+            face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
+            face_xfer<KE>(dn, sc, sc.fk, sc.ck, ok);
             const int wb = s & 1;
             if (jj > 0) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) xj[wb][r][lane] = oj[r];
+                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
             } else if (j > 1) {
-                halo_put(hj_slot(sw, b, tj, i, k), oj);
+                halo_put<NV>(hj_slot(sw, b, tj, i, k), oj);
             }
             if (kk > 0) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) xk[wb][r][lane] = ok[r];
+                for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
             } else if (k > 1) {
-                halo_put(hk_slot(sw, b, tk, i, j), ok);
+                halo_put<NV>(hk_slot(sw, b, tk, i, j), ok);
             }
             if (q + 1 < b.imax) cur = nxt;
         }
@@ -820,11 +1270,15 @@ __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
     }
 }
 
-// DIAGGS + forward sweep CORRGS (imps4.f:604-664, 76-339) as a tile wavefront, I, J, K
-// ascending, with DRO/CP of IMPS (imps3.f:483) applied to the final increment.
+// DIAGGS + forward sweep CORRGS (imps4.f:604-664, 76-339), or DIKEGS + COKEGS
+// (imps4.f:1841-1933, 1078-1456, KE), as a tile wavefront, I, J, K ascending, with the end
+// of IMPS applied to the final increment: DRO/CP (imps3.f:483) and, KE, D(rho k) and
+// D(rho omega) times RO (imps3.f:497-500).
+template <bool KE>
 __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
 {
-    __shared__ double xj[2][5][TW], xk[2][5][TW];
+    constexpr int NV = KE ? 7 : 5;
+    __shared__ double xj[2][NV][TW], xk[2][NV][TW];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
     const int t = tile_ticket(sw);
     const int tj = t % sw.ntj, tk = t / sw.ntj;
@@ -832,68 +1286,74 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
     const bool col = j <= b.jmax && k <= b.kmax;
     const int lag = jj + kk;
     const int nsteps = b.imax + TJ + TK - 2;
-    double yi[5] = {0., 0., 0., 0., 0.};
-    CorrRaw cur;
-    if (col) corr_load(b, cid(b, 1, j, k), cur);
+    double yi[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yi[r] = 0.;
+    SweepRaw cur;
+    if (col) sweep_load<true, KE>(b, cid(b, 1, j, k), cur);
     for (int s = 0; s < nsteps; s++) {
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = q + 1;
             const int n = cid(b, i, j, k);
             const bool hjin = jj == 0 && j > 1, hkin = kk == 0 && k > 1;
-            unsigned long long hv[5], hw[5];
-            if (hjin) halo_issue(hj_slot(sw, b, tj, i, k), hv);
-            if (hkin) halo_issue(hk_slot(sw, b, tk, i, j), hw);
-            CorrRaw nxt;
-            if (q + 1 < b.imax) corr_load(b, n + 1, nxt);
+            unsigned long long hv[NV], hw[NV];
+            if (hjin) halo_issue<NV>(hj_slot(sw, b, tj, i, k), hv);
+            if (hkin) halo_issue<NV>(hk_slot(sw, b, tk, i, j), hw);
+            SweepRaw nxt;
+            if (q + 1 < b.imax) sweep_load<true, KE>(b, n + 1, nxt);
             SweepCell sc;
-            corr_prep(p, cur, sc);
+            corr_prep<KE>(p, cur, sc);
             const int rb = (s + 1) & 1;
-            double bj[5], ckc[5];
+            double bj[NV], ckc[NV];
             if (j == 1) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) bj[r] = 0.;
+                for (int r = 0; r < NV; r++) bj[r] = 0.;
             } else if (!hjin) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) bj[r] = xj[rb][r][lane - 1];
+                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - 1];
             } else {
-                halo_finish(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
+                halo_finish<NV>(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
             }
             if (k == 1) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) ckc[r] = 0.;
+                for (int r = 0; r < NV; r++) ckc[r] = 0.;
             } else if (!hkin) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) ckc[r] = xk[rb][r][lane - TJ];
+                for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane - TJ];
             } else {
-                halo_finish(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
+                halo_finish<NV>(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
             }
-            double dn[5];
+            double dn[NV];
 #pragma unroll
-            for (int r = 0; r < 5; r++) {
+            for (int r = 0; r < NV; r++) {
                 double cd = sc.dt1 * sc.wk * ckc[r] + sc.vol * sc.d[r];
                 cd = cd + sc.dt1 * (sc.wi * yi[r] + sc.wj * bj[r]);
-                dn[r] = r < 4 ? cd * sc.scinv : cd * sc.scinp;
+                dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
             }
             FLD(b, FF_DRO)[n] = dn[0] / sc.cp;
 #pragma unroll
             for (int r = 1; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
-            double oj[5], ok[5];
-            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fi, sc.ci.x1, sc.ci.x2, sc.ci.x5, yi);
-            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fj, sc.cj.x1, sc.cj.x2, sc.cj.x5, oj);
-            face_contrib_b(dn, sc.ro, sc.roc, sc.pc2, sc.proc, sc.fk, sc.ck.x1, sc.ck.x2, sc.ck.x5, ok);
+            if (KE) {
+                FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * sc.rho;
+                FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * sc.rho;
+            }
+            double oj[NV], ok[NV];
+            face_xfer<KE>(dn, sc, sc.fi, sc.ci, yi);
+            face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
+            face_xfer<KE>(dn, sc, sc.fk, sc.ck, ok);
             const int wb = s & 1;
             if (jj < TJ - 1) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) xj[wb][r][lane] = oj[r];
+                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
             } else if (j < b.jmax) {
-                halo_put(hj_slot(sw, b, tj + 1, i, k), oj);
+                halo_put<NV>(hj_slot(sw, b, tj + 1, i, k), oj);
             }
             if (kk < TK - 1) {
 #pragma unroll
-                for (int r = 0; r < 5; r++) xk[wb][r][lane] = ok[r];
+                for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
             } else if (k < b.kmax) {
-                halo_put(hk_slot(sw, b, tk + 1, i, j), ok);
+                halo_put<NV>(hk_slot(sw, b, tk + 1, i, j), ok);
             }
             if (q + 1 < b.imax) cur = nxt;
         }
@@ -929,6 +1389,8 @@ __global__ void __launch_bounds__(256) k_save_old(Blk b)
     FLD(b, FF_VOLD)[n] = FLD(b, FF_V)[n];
     FLD(b, FF_WOLD)[n] = FLD(b, FF_W)[n];
     FLD(b, FF_POLD)[n] = FLD(b, FF_PDIFF)[n];
+    FLD(b, FF_RKOLD)[n] = FLD(b, FF_RK)[n];
+    FLD(b, FF_EPSOLD)[n] = FLD(b, FF_REPS)[n];
 }
 
 // FIN (res3c.f:1122-1126) on interior cells
@@ -944,6 +1406,30 @@ __global__ void __launch_bounds__(256) k_fin(Blk b)
     FLD(b, FF_V)[n] = FLD(b, FF_VOLD)[n] + FLD(b, FF_DN)[n];
     FLD(b, FF_W)[n] = FLD(b, FF_WOLD)[n] + FLD(b, FF_DW)[n];
     FLD(b, FF_PDIFF)[n] = FLD(b, FF_POLD)[n] + FLD(b, FF_DE)[n];
+}
+
+// FIN for two-equation models (res3c.f:1182-1198): the mean flow as above plus the relaxed
+// update of rho*k and rho*omega
+__global__ void __launch_bounds__(256) k_fin_kw(Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    int n = cid(b, i, j, k);
+    FLD(b, FF_TEMP)[n] = FLD(b, FF_TOLD)[n] + FLD(b, FF_DRO)[n];
+    FLD(b, FF_U)[n] = FLD(b, FF_UOLD)[n] + FLD(b, FF_DM)[n];
+    FLD(b, FF_V)[n] = FLD(b, FF_VOLD)[n] + FLD(b, FF_DN)[n];
+    FLD(b, FF_W)[n] = FLD(b, FF_WOLD)[n] + FLD(b, FF_DW)[n];
+    FLD(b, FF_PDIFF)[n] = FLD(b, FF_POLD)[n] + FLD(b, FF_DE)[n];
+    double rko = FLD(b, FF_RKOLD)[n], epo = FLD(b, FF_EPSOLD)[n];
+    double drk = FLD(b, FF_DRK)[n], dep = FLD(b, FF_DEPS)[n];
+    double rkoldn = .45 * rko;
+    double epoldn = .90 * epo;
+    double absdrk = fabs(drk);
+    double absdep = fabs(dep);
+    FLD(b, FF_RK)[n] = rko + drk / (1. + absdrk / rkoldn);
+    FLD(b, FF_REPS)[n] = epo + dep / (1. + absdep / epoldn);
 }
 
 // Limiters of CYCLES (ns3c.f:1553-1596) on every cell: UPLIM acts on interior K-planes and
@@ -969,6 +1455,12 @@ __global__ void __launch_bounds__(256) k_limits(Prm p, Blk b)
         s = s > -5000. ? s : -5000.;
         x[n] = s;
     }
+    if (p.iturb == 6) {
+        // LOLIM of RK and REPS (FIN res3c.f:1202-1205; CYCLES ns3c.f:1575-1576)
+        double rk = FLD(b, FF_RK)[n], re = FLD(b, FF_REPS)[n];
+        FLD(b, FF_RK)[n] = rk > p.rklim ? rk : p.rklim;
+        FLD(b, FF_REPS)[n] = re > p.epslim ? re : p.epslim;
+    }
     double pd = FLD(b, FF_PDIFF)[n];
     double P = pd + p.frspre;
     double y = P - 0.01 * p.frspre;
@@ -989,6 +1481,8 @@ struct BlockRec {
     Sweep sw{};                // LU-SGS tile-wavefront state (halo slots, ticket, error word)
     void *sw_mem = nullptr;
     size_t nhj = 0, nhk = 0;   // halo slot counts
+    double *kw_scr = nullptr;  // k-omega: per-workgroup maxima of SEPS and their maximum
+    int kw_parts = 0;
 };
 
 struct PendingEv {
@@ -1041,7 +1535,10 @@ static Prm make_prm(const ff_params *p, int level)
     k.cfl = level >= 2 ? p->cfll : p->cfl;
     k.theta = p->theta; k.cdiff = p->cdiff; k.tuplim = p->tuplim; k.tlolim = p->tlolim;
     k.epslim = p->epslim;
+    k.rklim = p->rklim; k.frsvis = p->frsvis; k.re = p->re; k.csimps = p->csimps;
+    k.turlim = p->turlim;
     k.iturb = p->iturb;
+    k.kover = p->kover; k.katol = p->katol;
     for (int d = 0; d < 3; d++) k.inter[d] = level >= 2 ? 55 : p->inter[d];   // res3c RESIDI ns3c.f:2468-2472
     return k;
 }
@@ -1116,8 +1613,8 @@ static int sweep_alloc(ff_ctx *c, BlockRec &r)
     Sweep &sw = r.sw;
     sw.ntj = (b.jmax + TJ - 1) / TJ;
     sw.ntk = (b.kmax + TK - 1) / TK;
-    r.nhj = (size_t)5 * (sw.ntj + 1) * b.kmax * b.imax;
-    r.nhk = (size_t)5 * (sw.ntk + 1) * b.jmax * b.imax;
+    r.nhj = (size_t)HS * (sw.ntj + 1) * b.kmax * b.imax;
+    r.nhk = (size_t)HS * (sw.ntk + 1) * b.jmax * b.imax;
     size_t words = r.nhj + r.nhk + 2;
     if (hipMalloc(&r.sw_mem, words * sizeof(unsigned long long)) != hipSuccess)
         return fail(c, FF_E_NOMEM, "LU-SGS halo slots%s");
@@ -1127,6 +1624,11 @@ static int sweep_alloc(ff_ctx *c, BlockRec &r)
     sw.hj = m + 2;
     sw.hk = sw.hj + r.nhj;
     HIPCHK(c, hipMemsetAsync(r.sw_mem, 0, 2 * sizeof(unsigned long long), c->stream));
+    if (c->prm.iturb == 6) {
+        r.kw_parts = (b.ntot + 255) / 256;
+        if (hipMalloc(&r.kw_scr, sizeof(double) * (size_t)(r.kw_parts + 1)) != hipSuccess)
+            return fail(c, FF_E_NOMEM, "k-omega reduction scratch%s");
+    }
     size_t nh = r.nhj + r.nhk;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream,
                        sw.hj, nh, HALO_EMPTY);
@@ -1138,6 +1640,8 @@ static void sweep_free(BlockRec &r)
 {
     if (r.sw_mem) (void)hipFree(r.sw_mem);
     r.sw_mem = nullptr;
+    if (r.kw_scr) (void)hipFree(r.kw_scr);
+    r.kw_scr = nullptr;
 }
 
 // Read the error word of the block's sweeps; after a failed halo wait, refill the slots
@@ -1173,6 +1677,21 @@ static const int kNeeded[] = {
     FF_DRDP, FF_DRDH, FF_RM, FF_RN, FF_RW, FF_EPS2, FF_VIST, FF_DRO, FF_DM, FF_DN, FF_DW,
     FF_DE, FF_DTL, FF_TOLD, FF_UOLD, FF_VOLD, FF_WOLD, FF_POLD, FF_VOL, FF_A1, FF_A1X,
     FF_A1Y, FF_A1Z, FF_A2, FF_A2X, FF_A2Y, FF_A2Z, FF_A3, FF_A3X, FF_A3Y, FF_A3Z};
+// and those of the k-omega SST path
+static const int kNeededKw[] = {
+    FF_RK, FF_REPS, FF_DRK, FF_DEPS, FF_RKOLD, FF_EPSOLD, FF_D1, FF_D2, FF_D3, FF_DISTW,
+    FF_SRK, FF_SEPS, FF_PTUR, FF_FUN1, FF_OHMI, FF_W12, FF_W13, FF_W23, FF_S11, FF_S12,
+    FF_S13, FF_S22, FF_S23, FF_S33, FF_STRAIN, FF_TTS};
+
+static int check_fields(ff_ctx *c, const ff_block *v, const char *what)
+{
+    for (int f : kNeeded)
+        if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
+    if (c->prm.iturb == 6)
+        for (int f : kNeededKw)
+            if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
+    return FF_OK;
+}
 
 extern "C" {
 
@@ -1180,7 +1699,9 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
 {
     if (!out || !prm) return FF_E_ARG;
     *out = nullptr;
-    if (prm->iturb != 0) return FF_E_UNSUP;        // this build covers ITURB = 0
+    if (prm->iturb != 0 && prm->iturb != 6) return FF_E_UNSUP;   // laminar or k-omega SST
+    if (prm->iturb == 6 && (prm->kover != 0 && prm->kover != 5)) return FF_E_UNSUP;
+    if (prm->iturb == 6 && prm->iflux != 1) return FF_E_UNSUP;   // FLXCYC: Roe only
     if (prm->iflux != 1 && prm->iflux != 0) return FF_E_UNSUP;   // Roe FLUXKE, van Leer FLUXVL
     if (prm->idiff > 3) return FF_E_UNSUP;         // arithmetic FLUXDO average
     if (prm->expsu != 1.5) return FF_E_UNSUP;      // Sutherland sqrt branch of PERFGA
@@ -1193,7 +1714,7 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
     }
     c->own_stream = true;
     c->red_parts = 512;
-    if (hipMalloc(&c->red_part, sizeof(double) * 5 * c->red_parts) != hipSuccess ||
+    if (hipMalloc(&c->red_part, sizeof(double) * 7 * c->red_parts) != hipSuccess ||
         hipMalloc(&c->red_out, sizeof(double) * 8) != hipSuccess ||
         hipHostMalloc(&c->red_host, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess) {
         ff_ctx_destroy(c);
@@ -1286,8 +1807,7 @@ int ff_block_register(ff_ctx *c, int level, int block, const ff_block *host)
     if (!c) return FF_E_ARG;
     int rc = check_dims(c, host);
     if (rc) return rc;
-    for (int f : kNeeded)
-        if (!host->f[f]) return fail(c, FF_E_ARG, "host field %s is NULL", std::to_string(f).c_str());
+    if ((rc = check_fields(c, host, "host field %s is NULL"))) return rc;
     BlockRec r;
     r.host = *host;
     r.mirror = true;
@@ -1316,8 +1836,7 @@ int ff_block_register_device(ff_ctx *c, int level, int block, const ff_block *de
     if (!c) return FF_E_ARG;
     int rc = check_dims(c, dev);
     if (rc) return rc;
-    for (int f : kNeeded)
-        if (!dev->f[f]) return fail(c, FF_E_ARG, "device field %s is NULL", std::to_string(f).c_str());
+    if ((rc = check_fields(c, dev, "device field %s is NULL"))) return rc;
     BlockRec r;
     r.mirror = false;
     r.dev = make_blk(dev);
@@ -1332,7 +1851,7 @@ int ff_block_register_device(ff_ctx *c, int level, int block, const ff_block *de
     return FF_OK;
 }
 
-int ff_sync(ff_ctx *c, int level, int block, int direction, unsigned long long mask)
+int ff_sync(ff_ctx *c, int level, int block, int direction, const int *fields, int nfields)
 {
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
@@ -1343,8 +1862,15 @@ int ff_sync(ff_ctx *c, int level, int block, int direction, unsigned long long m
     }
     if (!r->mirror) return FF_OK;
     size_t bytes = sizeof(double) * (size_t)r->dev.ntot;
-    for (int f = 0; f < FF_NFIELDS; f++) {
-        if (!(mask >> f & 1ULL) || !r->host.f[f]) continue;
+    bool all = fields == nullptr || nfields < 0;
+    if (!all)
+        for (int t = 0; t < nfields; t++)
+            if (fields[t] < 0 || fields[t] >= FF_NFIELDS)
+                return fail(c, FF_E_ARG, "ff_sync: field index out of range%s");
+    int nf = all ? FF_NFIELDS : nfields;
+    for (int t = 0; t < nf; t++) {
+        int f = all ? t : fields[t];
+        if (!r->host.f[f]) continue;
         if (direction == 0)
             HIPCHK(c, hipMemcpyAsync(r->dev.f[f], r->host.f[f], bytes, hipMemcpyHostToDevice, c->stream));
         else
@@ -1377,6 +1903,11 @@ int ff_turb(ff_ctx *c, int level, int block)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    if (c->prm.iturb == 6) {
+        const Blk &b = r->dev;
+        dim3 g((b.imax + 2 + 255) / 256, b.jmax + 2, b.kmax + 2);
+        return launch(c, "k_turb_sst", k_turb_sst, g, dim3(256), make_prm(&c->prm, level), b);
+    }
     return launch(c, "k_turb_lam", k_turb_lam, cells_grid(r->dev, 256), dim3(256), r->dev);
 }
 
@@ -1384,13 +1915,29 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, double *resi)
 {
     const Blk &b = r->dev;
     int rc;
-    // ZEROZZ of DRO..DE on every cell (res3c.f:194-198): ghost residuals stay zero
-    for (int q = 0; q < 5; q++) {
+    const bool kw = c->prm.iturb == 6;
+    const int nq = kw ? 7 : 5;
+    // ZEROZZ of DRO..DE (DRK, DEPS) on every cell (res3c.f:194-198, 3668-3674): ghost
+    // residuals stay zero
+    for (int q = 0; q < nq; q++) {
         rc = launch(c, "k_zero", k_zero, all_grid(b), dim3(256), b.f[FF_DRO + q], b.ntot);
         if (rc) return rc;
     }
     const bool vl = c->prm.iflux == 0;
-    if (c->residual_mode == FF_RESIDUAL_TILES) {
+    if (kw) {
+        if (c->residual_mode != FF_RESIDUAL_TILES)
+            return fail(c, FF_E_UNSUP, "the k-omega residual has the tile form only%s");
+        Prm kp = make_prm(&c->prm, level);
+        // CROSKO, its rounding of RK/REPS, FUN1KO's floor, FUN1KO + PRODKE (res3c.f:3817-3994)
+        if ((rc = launch(c, "k_crosko", k_crosko, cells_grid(b, 256), dim3(256), kp, b))) return rc;
+        if ((rc = launch(c, "k_kw_round", k_kw_round, dim3(r->kw_parts), dim3(256), b, r->kw_scr))) return rc;
+        if ((rc = launch(c, "k_max_final", k_max_final, dim3(1), dim3(256), (const double *)r->kw_scr,
+                         r->kw_parts, r->kw_scr + r->kw_parts))) return rc;
+        if ((rc = launch(c, "k_fun1_prod", k_fun1_prod, cells_grid(b, 256), dim3(256), kp, b,
+                         (const double *)(r->kw_scr + r->kw_parts)))) return rc;
+        dim3 g((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
+        rc = launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b);
+    } else if (c->residual_mode == FF_RESIDUAL_TILES) {
         dim3 g((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ, (b.kmax + RKC - 1) / RKC);
         rc = launch(c, "k_residual_tile", vl ? k_residual_tile<0> : k_residual_tile<1>, g,
                     dim3(RI * RJ), make_prm(&c->prm, level), b);
@@ -1403,14 +1950,14 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, double *resi)
         int ncell = b.imax * b.jmax * b.kmax;
         int parts = (ncell + RED_THREADS - 1) / RED_THREADS;
         if (parts > c->red_parts) parts = c->red_parts;
-        rc = launch(c, "k_resl2_part", k_resl2_part, dim3(parts), dim3(RED_THREADS), b, c->red_part);
+        rc = kw ? launch(c, "k_resl2_part", k_resl2_part<7>, dim3(parts), dim3(RED_THREADS), b, c->red_part)
+                : launch(c, "k_resl2_part", k_resl2_part<5>, dim3(parts), dim3(RED_THREADS), b, c->red_part);
         if (rc) return rc;
-        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)c->red_part, parts, c->red_out);
+        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)c->red_part, parts, nq, c->red_out);
         if (rc) return rc;
-        HIPCHK(c, hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int q = 0; q < 5; q++) resi[q] = c->red_host[q];
-        resi[5] = resi[6] = 0.;
+        for (int q = 0; q < 7; q++) resi[q] = q < nq ? c->red_host[q] : 0.;
     }
     return FF_OK;
 }
@@ -1433,9 +1980,15 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
     if (c->lusgs_mode == FF_LUSGS_TILES) {
         // DRO/CP is applied inside the forward sweep
         dim3 g(r->sw.ntj * r->sw.ntk);
-        if ((rc = launch(c, "k_pred_tiles", k_pred_tiles, g, dim3(TW), kp, b, r->sw))) return rc;
-        return launch(c, "k_corr_tiles", k_corr_tiles, g, dim3(TW), kp, b, r->sw);
+        if (c->prm.iturb == 6) {
+            if ((rc = launch(c, "k_pred_tiles", k_pred_tiles<true>, g, dim3(TW), kp, b, r->sw))) return rc;
+            return launch(c, "k_corr_tiles", k_corr_tiles<true>, g, dim3(TW), kp, b, r->sw);
+        }
+        if ((rc = launch(c, "k_pred_tiles", k_pred_tiles<false>, g, dim3(TW), kp, b, r->sw))) return rc;
+        return launch(c, "k_corr_tiles", k_corr_tiles<false>, g, dim3(TW), kp, b, r->sw);
     }
+    if (c->prm.iturb == 6)
+        return fail(c, FF_E_UNSUP, "the k-omega LU-SGS has the tile-wavefront form only%s");
     const int mlo = 3, mhi = b.imax + b.jmax + b.kmax;
     // backward sweep: hyperplanes from the far corner down
     for (int m = mhi; m >= mlo; m--) {
@@ -1479,7 +2032,8 @@ int ff_implicit(ff_ctx *c, int level, int block, int m2)
 static int update_impl(ff_ctx *c, BlockRec *r, int level)
 {
     const Blk &b = r->dev;
-    int rc = launch(c, "k_fin", k_fin, cells_grid(b, 256), dim3(256), b);
+    int rc = c->prm.iturb == 6 ? launch(c, "k_fin", k_fin_kw, cells_grid(b, 256), dim3(256), b)
+                               : launch(c, "k_fin", k_fin, cells_grid(b, 256), dim3(256), b);
     if (rc) return rc;
     return launch(c, "k_limits", k_limits, all_grid(b), dim3(256), make_prm(&c->prm, level), b);
 }

@@ -24,8 +24,10 @@ struct Prm {
     double gamma, rgas, pr, prt, visu0, tsu0, rmult;
     double frspre, frsden, frstem;
     double cfl, theta, cdiff, tuplim, tlolim, epslim;
+    double rklim, frsvis, re, csimps, turlim;
     int iturb;
     int inter[3];
+    int kover, katol;
 };
 
 #define FLD(b, n) ((b).f[n])
@@ -97,16 +99,23 @@ __device__ __forceinline__ double efpro(double pp, double ro, double gamma)
     return (pp + 0.) / (gm1 * ro) - 0.;
 }
 
-struct Flux5 { double f[5]; };
+template <int NV> struct FluxN { double f[NV]; };
+using Flux5 = FluxN<5>;
+using Flux7 = FluxN<7>;
 
-// FLUXKE base3c.f:4753-5125 at one face (laminar: k and omega are zero), with DERIV's
-// I2 = I branch (base3c.f:719) on k-direction faces.
-template <bool SINGLE>
-__device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int S, int intem,
-                                          const double *__restrict__ an,
-                                          const double *__restrict__ anx,
-                                          const double *__restrict__ any,
-                                          const double *__restrict__ anz)
+// MUSCL switch of rho*k and rho*omega, INTETI/J/K of RESIKE res3c.f:3634-3641 (INTERTU = 0)
+__device__ __forceinline__ int intet_of(int inter) { return (inter > -5 && inter < 5) ? 1 : inter; }
+
+// FLUXKE base3c.f:4753-5125 at one face, with DERIV's I2 = I branch (base3c.f:719) on
+// k-direction faces.  NV = 5: the algebraic path of FLUCYC (k and omega zero); NV = 7: the
+// two-equation path of FLXCYC with k and omega reconstructed by the INTET switch and floored
+// at 0.001*RKLIM, 0.001*EPSLIM (base3c.f:4906-4909).
+template <bool SINGLE, int NV = 5>
+__device__ __forceinline__ FluxN<NV> roe_face(const Prm &p, const Blk &b, int L, int S, int intem,
+                                              const double *__restrict__ an,
+                                              const double *__restrict__ anx,
+                                              const double *__restrict__ any,
+                                              const double *__restrict__ anz)
 {
     const double *ro = FLD(b, FF_RO), *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W),
                  *P = FLD(b, FF_P), *c = FLD(b, FF_C), *drdp = FLD(b, FF_DRDP),
@@ -150,8 +159,19 @@ __device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int
     double ul = a11 * rmim + a12 * rnim + a13 * rwim;
     double vl = a21 * rmim + a22 * rnim + a23 * rwim;
     double wl = a31 * rmim + a32 * rnim + a33 * rwim;
-    // k and omega of the algebraic path: MAX(0.001*RKLIM, 0) with RKLIM = 0
-    const double rkipi = 0., rkimi = 0.;
+    // k and omega: MAX(0.001*RKLIM, RKIP) ...; zero on the algebraic path
+    double rkipi = 0., rkimi = 0., reipi = 0., reimi = 0.;
+    if (NV == 7) {
+        double rkip, rkim, reip, reim;
+        const int intet = intet_of(intem);
+        xxtrap(FLD(b, FF_RK), L, S, intet, rkip, rkim);
+        xxtrap(FLD(b, FF_REPS), L, S, intet, reip, reim);
+        double lk = 0.001 * p.rklim, le = 0.001 * p.epslim;
+        rkipi = lk > rkip ? lk : rkip;
+        reipi = le > reip ? le : reip;
+        rkimi = lk > rkim ? lk : rkim;
+        reimi = le > reim ? le : reim;
+    }
     double ypror = 1. / roip;
     double rkr = rkipi * ypror;
     double yprol = 1. / roim;
@@ -160,12 +180,14 @@ __device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int
     double eil = eim + .5 * (ul * ul + vl * vl + wl * wl) + rkl;
     double pr = ppi;
     double pkr = .6667 * rkipi;
+    double epsr = reipi * ypror;
     double hr = eir + pr * ypror;
     double er = eip;
     double rohat = sqrt(roip * roim);
     double yproh = 1. / rohat;
     double pl = pmi;
     double pkl = .6667 * rkimi;
+    double epsl = reimi * yprol;
     double hl = eil + yprol * pl;
     double el = eim;
     double sqrl = 1. / (roim + rohat);
@@ -177,6 +199,7 @@ __device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int
     double hhat = cqr * hr + cql * hl;
     double ehat = cqr * er + cql * el;
     double rkhat = cqr * rkr + cql * rkl;
+    double epshat = cqr * epsr + cql * epsl;
     double phat = rohat * (hhat - ehat - .5 * (uhat * uhat + vhat * vhat + what * what) - rkhat);
     double pmin = pr < pl ? pr : pl;
     double pmax = pr > pl ? pr : pl;
@@ -204,10 +227,11 @@ __device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int
     double alfa4 = rohat * (vr - vl);
     double alfa5 = rohat * (wr - wl);
     double alfa6 = rohat * dk;
+    double alfa7 = rohat * (epsr - epsl);
     double rl1 = fabs(uhat + chat);
     double rl2 = fabs(uhat - chat);
     double rl3 = fabs(uhat);
-    double rl4 = rl3, rl5 = rl3, rl6 = rl3;
+    double rl4 = rl3, rl5 = rl3, rl6 = rl3, rl7 = rl3;
     double roucr = roip * ur;
     double roucl = roim * ul;
     double damp = rl1 * alfa1 + rl2 * alfa2 + rl3 * alfa3;
@@ -221,12 +245,16 @@ __device__ __forceinline__ Flux5 roe_face(const Prm &p, const Blk &b, int L, int
                       - damp * hhat - ((rl1 * alfa1 - rl2 * alfa2) * uhat * chat
                       - rl3 * alfa3 * rohat * (chat * chat) * pdpde + rl4 * alfa4 * vhat
                       + rl5 * alfa5 * what) + rl6 * alfa6 * (1. - .6667 * rohat * pdpde));
-    Flux5 out;
+    FluxN<NV> out;
     out.f[0] = f1;
     out.f[1] = a11 * f2 + a21 * f3 + a31 * f4;
     out.f[2] = a12 * f2 + a22 * f3 + a32 * f4;
     out.f[3] = a13 * f2 + a23 * f3 + a33 * f4;
     out.f[4] = f5;
+    if (NV == 7) {
+        out.f[NV == 7 ? 5 : 0] = ha * (roucr * rkr + roucl * rkl - damp * rkhat - rl6 * alfa6);
+        out.f[NV == 7 ? 6 : 0] = ha * (roucr * epsr + roucl * epsl - damp * epshat - rl7 * alfa7);
+    }
     return out;
 }
 
@@ -254,6 +282,96 @@ __device__ __forceinline__ void visc_face(const Prm &p, const Blk &b, int L, int
     double f5 = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4);
     fm = f2; fn = f3; fw = f4;
     fe = f5 + surfe * (t[L] - t[m]);
+}
+
+// FLUXDO with the two energy terms it adds to FE returned apart (FE = FE + F5VIG + SURFE*dT)
+__device__ __forceinline__ void visc_face2(const Prm &p, const Blk &b, int L, int S,
+                                           const double *__restrict__ an,
+                                           const double *__restrict__ anx,
+                                           const double *__restrict__ any,
+                                           const double *__restrict__ anz, double &fm,
+                                           double &fn, double &fw, double &f5v, double &fh)
+{
+    const double *vol = FLD(b, FF_VOL), *eps2 = FLD(b, FF_EPS2), *vis = FLD(b, FF_VIS),
+                 *cp = FLD(b, FF_CP), *vist = FLD(b, FF_VIST), *ch = FLD(b, FF_CH),
+                 *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W),
+                 *t = FLD(b, FF_TEMP);
+    int m = L - S;
+    double ypd2 = an[L] / (vol[L] + vol[m]);
+    double surg = -an[L] * ypd2;
+    double surf = surg * (eps2[L] * vis[L] + eps2[m] * vis[m]);
+    double surfe = surg * (cp[L] * vist[L] / p.prt + ch[L] + cp[m] * vist[m] / p.prt + ch[m]);
+    double duc = .333333 * (anx[L] * (u[L] - u[m]) + any[L] * (v[L] - v[m]) + anz[L] * (w[L] - w[m]));
+    double f2 = surf * (u[L] - u[m] + anx[L] * duc);
+    double f3 = surf * (v[L] - v[m] + any[L] * duc);
+    double f4 = surf * (w[L] - w[m] + anz[L] * duc);
+    f5v = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4);
+    fm = f2; fn = f3; fw = f4;
+    fh = surfe * (t[L] - t[m]);
+}
+
+// SST diffusion constants of COEFKO turb3.f:8068-8108 (ITURB = 6)
+struct SstC {
+    double a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2;
+};
+__device__ __forceinline__ SstC coefko(int kover)
+{
+    SstC s;
+    s.a1kleb = 0.31; s.bstar = 0.09; s.ckappa = 0.41; s.beta1 = 0.075;
+    s.sigrk1 = kover < 5 ? 0.85 : 0.5;
+    s.sigom1 = 0.5; s.beta2 = 0.0828; s.sigrk2 = 1.0; s.sigom2 = 0.856;
+    return s;
+}
+
+// FLUDKE base3c.f:5547-5651 at one face: thin-layer k and omega diffusion with the SST
+// sigmas blended by FUN1 of the face's upper cell; returns F6VIG, F7VIG
+__device__ __forceinline__ void kw_diff_face(const Blk &b, int L, int S,
+                                             const double *__restrict__ an, const SstC &s,
+                                             double &f6, double &f7)
+{
+    const double *vol = FLD(b, FF_VOL), *eps2 = FLD(b, FF_EPS2), *vis = FLD(b, FF_VIS),
+                 *ro = FLD(b, FF_RO), *rk = FLD(b, FF_RK), *reps = FLD(b, FF_REPS),
+                 *fun1 = FLD(b, FF_FUN1);
+    int m = L - S;
+    double f1 = fun1[L];
+    double onemf1 = 0.0 > (1.0 - f1) ? 0.0 : (1.0 - f1);
+    double psigk = f1 * s.sigrk1 + onemf1 * s.sigrk2;
+    double psige = f1 * s.sigom1 + onemf1 * s.sigom2;
+    double ypd2 = an[L] / (vol[L] + vol[m]);
+    double surg = -an[L] * ypd2;
+    double rmut2 = (eps2[L] - 1.) * vis[L] + (eps2[m] - 1.) * vis[m];
+    double surfk = surg * (vis[L] + vis[m] + rmut2 * psigk);
+    double surfep = surg * (vis[L] + vis[m] + rmut2 * psige);
+    double ypro1 = 1. / ro[L];
+    double ypro2 = 1. / ro[m];
+    f6 = surfk * (rk[L] * ypro1 - rk[m] * ypro2);
+    f7 = surfep * (reps[L] * ypro1 - reps[m] * ypro2);
+}
+
+// FLXCYC face flux (ITURB = 6, IFLUX = 1): FLUDKE (FE = FRK = F6VIG, FEPS = F7VIG), SETV12,
+// FLUXDO (FE = FE + F5VIG + heat), then FLUXKE added (res3c.f:6817-6946)
+template <bool SINGLE>
+__device__ __forceinline__ Flux7 face_flux_kw(const Prm &p, const SstC &s, const Blk &b, int L,
+                                              int S, int intem,
+                                              const double *__restrict__ an,
+                                              const double *__restrict__ anx,
+                                              const double *__restrict__ any,
+                                              const double *__restrict__ anz)
+{
+    double f6, f7, fm, fn, fw, f5v, fh;
+    kw_diff_face(b, L, S, an, s, f6, f7);
+    visc_face2(p, b, L, S, an, anx, any, anz, fm, fn, fw, f5v, fh);
+    double fe = f6 + f5v + fh;
+    Flux7 r = roe_face<SINGLE, 7>(p, b, L, S, intem, an, anx, any, anz);
+    Flux7 o;
+    o.f[0] = r.f[0] + 0.;
+    o.f[1] = r.f[1] + fm;
+    o.f[2] = r.f[2] + fn;
+    o.f[3] = r.f[3] + fw;
+    o.f[4] = r.f[4] + fe;
+    o.f[5] = r.f[5] + f6;
+    o.f[6] = r.f[6] + f7;
+    return o;
 }
 
 // FLUXVL base3c.f:1733-1961 at one face: van Leer flux-vector splitting of the MUSCL states

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .layout import ALL_MASK, Block, BlockView, Params, mask
+from .layout import FID, Block, BlockView, Params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libffhip.so")
@@ -52,7 +52,7 @@ def lib():
         L.ff_kernel_stats_reset.argtypes = [vp]
         L.ff_block_register.argtypes = [vp, i, i, C.POINTER(BlockView)]
         L.ff_block_register_device.argtypes = [vp, i, i, C.POINTER(BlockView)]
-        L.ff_sync.argtypes = [vp, i, i, i, C.c_ulonglong]
+        L.ff_sync.argtypes = [vp, i, i, i, C.POINTER(C.c_int), i]
         L.ff_finish.argtypes = [vp]
         for fn in ("ff_state", "ff_turb", "ff_update", "ff_save_old"):
             getattr(L, fn).argtypes = [vp, i, i]
@@ -110,13 +110,20 @@ class Context:
         self._views[(level, block)] = (blk, v)
         self._chk(lib().ff_block_register(self.h, level, block, C.byref(v)), "ff_block_register")
 
+    @staticmethod
+    def _fields(fields):
+        if fields is None:
+            return None, -1
+        arr = (C.c_int * len(fields))(*[FID[f] for f in fields])
+        return arr, len(fields)
+
     def upload(self, level=1, block=1, fields=None):
-        m = ALL_MASK if fields is None else mask(fields)
-        self._chk(lib().ff_sync(self.h, level, block, 0, m), "ff_sync(h2d)")
+        arr, n = self._fields(fields)
+        self._chk(lib().ff_sync(self.h, level, block, 0, arr, n), "ff_sync(h2d)")
 
     def download(self, level=1, block=1, fields=None):
-        m = ALL_MASK if fields is None else mask(fields)
-        self._chk(lib().ff_sync(self.h, level, block, 1, m), "ff_sync(d2h)")
+        arr, n = self._fields(fields)
+        self._chk(lib().ff_sync(self.h, level, block, 1, arr, n), "ff_sync(d2h)")
 
     def finish(self):
         self._chk(lib().ff_finish(self.h), "ff_finish")

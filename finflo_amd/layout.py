@@ -26,7 +26,7 @@ FIELDS = [
 NF = len(FIELDS)
 FID = {name: i for i, name in enumerate(FIELDS)}
 
-# groups used by sync masks and tests
+# groups used by sync field lists and tests
 SOLVED = ["PDIFF", "TEMP", "U", "V", "W", "RK", "REPS"]
 DERIVED = ["P", "RO", "E", "C", "VIS", "CH", "CP", "DRDP", "DRDH", "RM", "RN", "RW",
            "EPS2", "VIST"]
@@ -35,15 +35,6 @@ OLD = ["TOLD", "UOLD", "VOLD", "WOLD", "POLD", "RKOLD", "EPSOLD"]
 GEOM = ["VOL", "A1", "A1X", "A1Y", "A1Z", "A2", "A2X", "A2Y", "A2Z", "A3", "A3X", "A3Y",
         "A3Z", "D1", "D2", "D3", "XC", "YC", "ZC", "DISTW"]
 
-
-def mask(names):
-    m = 0
-    for n in names:
-        m |= 1 << FID[n]
-    return m
-
-
-ALL_MASK = (1 << NF) - 1
 
 
 class Params(C.Structure):

@@ -137,8 +137,9 @@ int  ff_kernel_stats_reset(ff_ctx *ctx);
 int  ff_block_register(ff_ctx *ctx, int level, int block, const ff_block *host);
 /* Register device-resident arrays (caller-owned device memory, no mirror). */
 int  ff_block_register_device(ff_ctx *ctx, int level, int block, const ff_block *dev);
-/* direction 0: host -> device, 1: device -> host.  field_mask bit f selects field f. */
-int  ff_sync(ff_ctx *ctx, int level, int block, int direction, unsigned long long field_mask);
+/* direction 0: host -> device, 1: device -> host, for the nfields field slots listed in
+ * fields (enum ff_field values); fields == NULL or nfields < 0 moves every registered field. */
+int  ff_sync(ff_ctx *ctx, int level, int block, int direction, const int *fields, int nfields);
 /* Synchronise the context's stream. */
 int  ff_finish(ff_ctx *ctx);
 
