@@ -43,6 +43,28 @@ def _extend(x):
     return np.concatenate([[lo2, lo1], x, [hi1, hi2]])
 
 
+def _extend_axis(X, axis):
+    """Two linearly extrapolated ghost node layers at both ends of `axis` of a node array."""
+    X = np.moveaxis(X, axis, 0)
+    lo1 = 2 * X[0] - X[1]
+    lo2 = 2 * lo1 - X[0]
+    hi1 = 2 * X[-1] - X[-2]
+    hi2 = 2 * hi1 - X[-1]
+    X = np.concatenate([lo2[None], lo1[None], X, hi1[None], hi2[None]], axis=0)
+    return np.moveaxis(X, 0, axis)
+
+
+def coarsen_nodes(nodes):
+    """Node array of the next multigrid level: every other interior node of the finer level
+    (the reference's coarse levels take every second grid line, main.f:4600-4660), with two
+    ghost node layers extrapolated again."""
+    g = GHOST
+    inner = nodes[g:-g, g:-g, g:-g][::2, ::2, ::2]
+    for ax in range(3):
+        inner = _extend_axis(inner, ax)
+    return inner
+
+
 def node_grid(xi, yj, zk, distort=0.0, seed=0):
     """Nodes (KS+1, JS+1, IS+1, 3) of a box with ghost nodes; `distort` adds a smooth
     interior deformation (fraction of the local spacing) to exercise curvilinear metrics."""
@@ -155,6 +177,40 @@ def make_block(imax, jmax, kmax, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
     fill_geometry(blk, node_grid(xi, yj, zk, distort=distort, seed=seed + 1000))
     fill_state(blk, prm, seed=seed, amp=amp, kind=kind, turb=turb, mach_dir=mach_dir)
     return blk
+
+
+def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
+                   distort=0.0, seed=0, amp=0.01, kind="density", turb=None,
+                   mach_dir=(1.0, 0.0, 0.0), nodes=None):
+    """A block and its nlev-1 coarser multigrid levels (dimensions divisible by 2^(nlev-1)).
+    Every level has its own geometry; the state of the coarse levels is free stream (the
+    restriction overwrites what the V-cycle uses)."""
+    prm = prm or default_params()
+    f = 2 ** (nlev - 1)
+    if imax % f or jmax % f or kmax % f:
+        raise ValueError(f"dimensions must be divisible by {f} for {nlev} levels")
+    if nodes is None:
+        xi = stretched(imax, lengths[0])
+        yj = stretched(jmax, lengths[1], ratio=jratio)
+        zk = stretched(kmax, lengths[2])
+        nodes = node_grid(xi, yj, zk, distort=distort, seed=seed + 1000)
+    blocks = []
+    for m in range(nlev):
+        b = Block(imax >> m, jmax >> m, kmax >> m)
+        fill_geometry(b, nodes)
+        fill_state(b, prm, seed=seed + m, amp=amp if m == 0 else 0.0, kind=kind,
+                   turb=turb, mach_dir=mach_dir)
+        blocks.append(b)
+        if m < nlev - 1:
+            nodes = coarsen_nodes(nodes)
+    return blocks
+
+
+def flat_plate_nodes(imax, jmax, kmax, first=1e-6, ratio=1.15, length=1.0, width=None):
+    xi = cosine_clustered(imax, length)
+    yj = geometric_from_wall(jmax, first * length, ratio)
+    zk = np.linspace(0.0, width if width else 0.5 * length, kmax + 1)
+    return node_grid(xi, yj, zk)
 
 
 def flat_plate_block(imax, jmax, kmax, prm, first=1e-6, ratio=1.15, length=1.0, width=None,

@@ -22,6 +22,7 @@ FIELDS = [
     "SRK", "SEPS", "PTUR", "FUN1", "OHMI",
     "W12", "W13", "W23", "S11", "S12", "S13", "S22", "S23", "S33", "STRAIN", "TTS",
     "B11", "B12", "B13", "B22", "B23", "B33",
+    "ROP2H", "RMP2H", "RNP2H", "RWP2H", "EP2H", "RKP2H", "EPSP2H",
 ]
 NF = len(FIELDS)
 FID = {name: i for i, name in enumerate(FIELDS)}
@@ -43,8 +44,8 @@ class Params(C.Structure):
         "gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
         "frspre", "frsden", "frstem", "frsvel", "frsvis",
         "cfl", "cfll", "theta", "cdiff", "tuplim", "tlolim", "rklim", "epslim",
-        "turlim", "frsmut", "rmult", "re", "csimps")] + [
-        ("reserved", C.c_double * 5),
+        "turlim", "frsmut", "rmult", "re", "csimps", "cmgk", "cmgeps")] + [
+        ("reserved", C.c_double * 3),
         ("iturb", C.c_int), ("iflux", C.c_int), ("idiff", C.c_int),
         ("inter", C.c_int * 3), ("kover", C.c_int), ("katol", C.c_int),
         ("reserved_i", C.c_int * 8)]
@@ -52,7 +53,7 @@ class Params(C.Structure):
     DOUBLES = ("gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
                "frspre", "frsden", "frstem", "frsvel", "frsvis",
                "cfl", "cfll", "theta", "cdiff", "tuplim", "tlolim", "rklim", "epslim",
-               "turlim", "frsmut", "rmult", "re", "csimps")
+               "turlim", "frsmut", "rmult", "re", "csimps", "cmgk", "cmgeps")
 
     def dvec(self):
         """Parameters as the PRM vector of the reference driver (same order)."""
@@ -81,6 +82,7 @@ def default_params(**kw):
     p.rklim, p.epslim = 1e-10, 1e-12
     p.turlim, p.frsmut, p.rmult = 1e5, 1e-10, 1.0
     p.re, p.csimps = 1e5, 5.0
+    p.cmgk, p.cmgeps = 0.1, 0.2                            # main.f:5344-5345
     p.iturb, p.iflux, p.idiff = 0, 1, 1
     p.inter[0] = p.inter[1] = p.inter[2] = -3
     p.kover, p.katol = 0, 1

@@ -47,6 +47,9 @@ enum ff_field {
     FF_STRAIN, FF_TTS,
     /* EARSM extra anisotropy BIJ(:,1..6) (earsm.f:209-219) */
     FF_B11, FF_B12, FF_B13, FF_B22, FF_B23, FF_B33,
+    /* multigrid forcing of a coarse level, the reference's ROP2H..EP2H, RKP2H, EPSP2H at
+       IH(M,N) (TRANSF multi.f:245-270, FORCFU multi.f:1630-1648, ADDRES multi.f:1709-1725) */
+    FF_ROP2H, FF_RMP2H, FF_RNP2H, FF_RWP2H, FF_EP2H, FF_RKP2H, FF_EPSP2H,
     FF_NFIELDS
 };
 
@@ -77,7 +80,9 @@ typedef struct ff_params {
     double rmult;    /* RMULTV viscosity multiplier, 1 */
     double re;       /* Reynolds number RE of the reference length (FUN1KO's wall limit) */
     double csimps;   /* CSIMPS of the k-omega point-implicit sources (main.f:5346), 5 */
-    double reserved[5];
+    double cmgk;     /* CMGK, damping of the prolonged rho*k correction (main.f:5344), 0.1 */
+    double cmgeps;   /* CMGEPS, the same for rho*omega (main.f:5345), 0.2 */
+    double reserved[3];
     int iturb;       /* 0 laminar, 6 k-omega SST */
     int iflux;       /* 1 Roe (FLUXKE), 0 van Leer (FLUXVL) */
     int idiff;       /* FLUXDO averaging, 1 = arithmetic (main.f:5754) */

@@ -585,22 +585,25 @@ static inline double pow4(double x) { return x * x * x * x; }
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 
-/* DERFUN turb3.f:2476-2665 with IDERI = 1 (distance-weighted Green-Gauss) on level 1,
- * first ghost layer included: vorticity, strain-rate tensor, |Omega| (OHMI) and the strain
- * magnitude (VELLAP, used only by the SAS variant, is not formed). */
-static void derfun(ff_block *b)
+/* DERFUN turb3.f:2476-2665 with IDERI = 1 (distance-weighted Green-Gauss): vorticity,
+ * strain-rate tensor, |Omega| (OHMI) and the strain magnitude (VELLAP, used only by the SAS
+ * variant, is not formed).  Level 1 includes the first ghost layer (turb3.f:2520-2529);
+ * coarser levels form the interior and EXTEND it to the first ghost layer
+ * (turb3.f:2952-2965, 3342-3450). */
+static void derfun(ff_block *b, int level)
 {
     dims_t d = mkdims(b);
     long IS = d.is, IL = d.il;
+    const int g0 = level == 1 ? 1 : 0;   /* ghost layers included */
     const double *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W), *vol = F(b, FF_VOL),
                  *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
                  *d1 = F(b, FF_D1), *d2 = F(b, FF_D2), *d3 = F(b, FF_D3),
                  *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
                  *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
                  *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z);
-    for (int k = 0; k <= d.kmax + 1; k++)
-        for (int j = 0; j <= d.jmax + 1; j++)
-            for (int i = 0; i <= d.imax + 1; i++) {
+    for (int k = 1 - g0; k <= d.kmax + g0; k++)
+        for (int j = 1 - g0; j <= d.jmax + g0; j++)
+            for (int i = 1 - g0; i <= d.imax + g0; i++) {
                 long L = cid(&d, i, j, k);
                 double w1, w2;
                 w1 = 2.0 * (d1[L] / (d1[L] + d1[L - 1])); w2 = 2.0 - w1;
@@ -655,6 +658,29 @@ static void derfun(ff_block *b)
                 F(b, FF_S11)[L] = s11; F(b, FF_S12)[L] = s12; F(b, FF_S13)[L] = s13;
                 F(b, FF_S22)[L] = s22; F(b, FF_S23)[L] = s23; F(b, FF_S33)[L] = s33;
             }
+    if (level >= 2) {
+        /* EXTEND: the interior face-adjacent value into the first ghost layer */
+        static const int fld[] = {FF_OHMI, FF_W12, FF_W13, FF_W23, FF_S11, FF_S12, FF_S13,
+                                  FF_S22, FF_S23, FF_S33};
+        for (int q = 0; q < 10; q++) {
+            double *x = F(b, fld[q]);
+            for (int j = 1; j <= d.jmax; j++)
+                for (int i = 1; i <= d.imax; i++) {
+                    x[cid(&d, i, j, 0)] = x[cid(&d, i, j, 1)];
+                    x[cid(&d, i, j, d.kmax + 1)] = x[cid(&d, i, j, d.kmax)];
+                }
+            for (int k = 1; k <= d.kmax; k++)
+                for (int i = 1; i <= d.imax; i++) {
+                    x[cid(&d, i, 0, k)] = x[cid(&d, i, 1, k)];
+                    x[cid(&d, i, d.jmax + 1, k)] = x[cid(&d, i, d.jmax, k)];
+                }
+            for (int k = 1; k <= d.kmax; k++)
+                for (int j = 1; j <= d.jmax; j++) {
+                    x[cid(&d, 0, j, k)] = x[cid(&d, 1, j, k)];
+                    x[cid(&d, d.imax + 1, j, k)] = x[cid(&d, d.imax, j, k)];
+                }
+        }
+    }
 }
 
 /* TTIMES earsm.f:629-692 (ITURB = 6), first ghost layer included */
@@ -888,7 +914,7 @@ void ffo_turb(const ff_params *p, ff_block *b)
 {
     dims_t d = mkdims(b);
     if (p->iturb == 6) {
-        derfun(b);
+        derfun(b, 1);
         ttimes(b);
         eps2ko(p, b);
         return;
@@ -902,14 +928,26 @@ void ffo_turb(const ff_params *p, ff_block *b)
             }
 }
 
-/* RESIDU res3c.f:6-787 (laminar) or RESIKE res3c.f:3423-5387 (SST): zero the residuals,
- * k-, i- and j-direction fluxes with a SWEEP after each, the last one dividing by VOL; SST
- * adds CROSKO/FUN1KO/PRODKE before the fluxes (res3c.f:3817-3994) and SOURKO/ADDSKE after
- * (res3c.f:4957-5068); then RESL2 sums on level 1 (res3c.f:8964-9006). */
-void ffo_residual(const ff_params *p, ff_block *b, int level, double *resi)
+/* RESIDI ns3c.f:2355-2677 -> RESIDU res3c.f:6-787 (laminar) or RESIKE res3c.f:3423-5387
+ * (SST): zero the residuals, k-, i- and j-direction fluxes with a SWEEP after each, the last
+ * one dividing by VOL; SST adds CROSKO/FUN1KO/PRODKE before the fluxes (res3c.f:3817-3994)
+ * and SOURKO/ADDSKE after (res3c.f:4957-5068); then RESL2 sums on level 1
+ * (ns3c.f:2668-2674, res3c.f:8964-9006).  Coarse levels are first order (INTI = 55,
+ * ns3c.f:2467-2472); with M2 >= 2 RESIKE forms the velocity gradients itself (res3c.f:3749). */
+void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *resi)
 {
     dims_t d = mkdims(b);
+    ff_params pl = *p0;
+    if (level >= 2) pl.inter[0] = pl.inter[1] = pl.inter[2] = 55;
+    const ff_params *p = &pl;
     int kwm = p->iturb == 6;
+    if (kwm && m2 >= 2) {
+        /* ZEROZZ of OHMI, W12 and SIJ before DERFUN (res3c.f:3751-3762) */
+        static const int fld[] = {FF_OHMI, FF_W12, FF_W13, FF_W23, FF_S11, FF_S12, FF_S13,
+                                  FF_S22, FF_S23, FF_S33};
+        for (int q = 0; q < 10; q++) memset(F(b, fld[q]), 0, sizeof(double) * (size_t)d.ntot);
+        derfun(b, level);
+    }
     for (int q = FF_DRO; q <= FF_DEPS; q++)
         if (F(b, q)) memset(F(b, q), 0, sizeof(double) * (size_t)d.ntot);
     double sig[4];
@@ -1538,7 +1576,45 @@ void ffo_save_old(const ff_params *p, ff_block *b)
 }
 
 /* FIN res3c.f:1122-1126 (algebraic) and the limiters of CYCLES ns3c.f:1553-1596 */
-void ffo_update(const ff_params *p, ff_block *b)
+/* FIN res3c.f:1077 and the limiters CYCLES applies after it: on level 1 UPLIM/LOLIM of
+ * TEMP, U, V, W, PDTOP and LOLIMP (ns3c.f:1553-1596); on coarser levels UPLIM/LOLIM of
+ * TEMP and LOLIMP with the pressure of the level's last STATIM (ns3c.f:1936-1985). */
+static void lolimp(const ff_params *p, ff_block *b)
+{
+    dims_t d = mkdims(b);
+    double *P = F(b, FF_P), *pd = F(b, FF_PDIFF);
+    for (long n = 0; n < d.ntot; n++) {
+        double x = P[n] - 0.01 * p->frspre;
+        double prelim = 0. < x ? 0. : x;                   /* MIN(0., P-0.01*FRSPRE) */
+        P[n] = P[n] - prelim;
+        pd[n] = pd[n] - prelim;
+    }
+}
+
+/* UPLIM res3c.f:3401-3417 (interior K-planes and J-rows, every I) and LOLIM (every cell) */
+static void uplim(ff_block *b, double *x, double up)
+{
+    dims_t d = mkdims(b);
+    for (int k = 1; k <= d.kmax; k++) {
+        long ia = (long)(k + 1) * d.il + (long)G * d.is;
+        for (long i = 0; i < (long)d.jmax * d.is; i++) {
+            long n = ia + i;
+            x[n] = x[n] < up ? x[n] : up;
+        }
+    }
+}
+
+static void lolim(ff_block *b, double *x, double lo)
+{
+    dims_t d = mkdims(b);
+    for (long n = 0; n < d.ntot; n++) x[n] = x[n] > lo ? x[n] : lo;
+}
+
+void ffo_update_level(const ff_params *p, ff_block *b, int level);
+
+void ffo_update(const ff_params *p, ff_block *b) { ffo_update_level(p, b, 1); }
+
+void ffo_update_level(const ff_params *p, ff_block *b, int level)
 {
     dims_t d = mkdims(b);
     double *t = F(b, FF_TEMP), *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
@@ -1577,6 +1653,12 @@ void ffo_update(const ff_params *p, ff_block *b)
             reps[n] = reps[n] > p->epslim ? reps[n] : p->epslim;
         }
     }
+    if (level >= 2) {
+        uplim(b, t, p->tuplim);
+        lolim(b, t, p->tlolim);
+        lolimp(p, b);
+        return;
+    }
     /* UPLIM (res3c.f:3401: interior k-planes, interior j-rows, all i) and LOLIM (all) */
     double *lim[4] = {t, u, v, w};
     double up[4] = {p->tuplim, 5000., 5000., 5000.};
@@ -1600,6 +1682,221 @@ void ffo_update(const ff_params *p, ff_block *b)
         P[n] = P[n] - prelim;
         pd[n] = pd[n] - prelim;
     }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Multigrid: TRANSF/TRANKE, FORCFU/ADDRES, INTERP/INTERT and the CYCLES V-cycle          */
+/* ------------------------------------------------------------------------------------ */
+
+/* The volume-weighted sum of the 8 fine cells of coarse cell (I,J,K) in the order of
+ * TRANSF loop 1000 (multi.f:214-218): IPJPK, IJK, IPJK, IJPK, IPJPKP, IJKP, IPJKP, IJPKP,
+ * where IJK is the fine cell (2I-1, 2J-1, 2K-1) */
+static double restrict8(const dims_t *f, const double *vol, const double *x, int I, int J,
+                        int K, double pvol)
+{
+    long ijk = cid(f, 2 * I - 1, 2 * J - 1, 2 * K - 1);
+    long ipjk = ijk + 1, ijpk = ijk + f->is, ipjpk = ijk + f->is + 1;
+    long ijkp = ijk + f->il, ipjpkp = ipjpk + f->il, ijpkp = ijpk + f->il, ipjkp = ipjk + f->il;
+    return (vol[ipjpk] * x[ipjpk] + vol[ijk] * x[ijk] + vol[ipjk] * x[ipjk] +
+            vol[ijpk] * x[ijpk] + vol[ipjpkp] * x[ipjpkp] + vol[ijkp] * x[ijkp] +
+            vol[ipjkp] * x[ipjkp] + vol[ijpkp] * x[ijpkp]) * pvol;
+}
+
+/* The ghost treatment of TRANSF/TRANKE loops 2100-2600 (multi.f:283-425): residual
+ * transfers zero on both ghost layers, variables copied from the first ghost layer to the
+ * second, I faces first, then J, then K. */
+static void restrict_ghosts(const dims_t *c, double **zero, int nz, double **copy, int ncp)
+{
+    long IS = c->is, IL = c->il;
+    for (int k = -1; k <= c->kmax + 2; k++)
+        for (int j = -1; j <= c->jmax + 2; j++) {
+            long lo = cid(c, 0, j, k), hi = cid(c, c->imax + 1, j, k);
+            for (int q = 0; q < nz; q++) {
+                zero[q][lo] = 0.; zero[q][lo - 1] = 0.;
+            }
+            for (int q = 0; q < ncp; q++) copy[q][lo - 1] = copy[q][lo];
+            for (int q = 0; q < nz; q++) {
+                zero[q][hi] = 0.; zero[q][hi + 1] = 0.;
+            }
+            for (int q = 0; q < ncp; q++) copy[q][hi + 1] = copy[q][hi];
+        }
+    for (int k = -1; k <= c->kmax + 2; k++)
+        for (int i = -1; i <= c->imax + 2; i++) {
+            long hi = cid(c, i, c->jmax + 1, k);
+            for (int q = 0; q < nz; q++) {
+                zero[q][hi] = 0.; zero[q][hi + IS] = 0.;
+            }
+            for (int q = 0; q < ncp; q++) copy[q][hi + IS] = copy[q][hi];
+        }
+    for (int k = -1; k <= c->kmax + 2; k++)
+        for (int i = -1; i <= c->imax + 2; i++) {
+            long lo = cid(c, i, 0, k);
+            for (int q = 0; q < nz; q++) {
+                zero[q][lo] = 0.; zero[q][lo - IS] = 0.;
+            }
+            for (int q = 0; q < ncp; q++) copy[q][lo - IS] = copy[q][lo];
+        }
+    for (int j = -1; j <= c->jmax + 2; j++)
+        for (int i = -1; i <= c->imax + 2; i++) {
+            long lo = cid(c, i, j, 0);
+            for (int q = 0; q < nz; q++) {
+                zero[q][lo] = 0.; zero[q][lo - IL] = 0.;
+            }
+            for (int q = 0; q < ncp; q++) copy[q][lo - IL] = copy[q][lo];
+        }
+    for (int j = -1; j <= c->jmax + 2; j++)
+        for (int i = -1; i <= c->imax + 2; i++) {
+            long hi = cid(c, i, j, c->kmax + 1);
+            for (int q = 0; q < nz; q++) {
+                zero[q][hi] = 0.; zero[q][hi + IL] = 0.;
+            }
+            for (int q = 0; q < ncp; q++) copy[q][hi + IL] = copy[q][hi];
+        }
+}
+
+/* TRANSF multi.f:156-428 (+ TRANKE multi.f:434-620 for two-equation models): restriction of
+ * the state, EPS2, VIST and (as the coarse forcing store) the residuals from the fine level
+ * to the coarse level (cells 0..N+1 of the coarse level), OLD copies of the interior, then
+ * the ghost treatment.  The coarse level has half the fine cells in every direction. */
+void ffo_restrict(const ff_params *p, const ff_block *fb, ff_block *cb)
+{
+    dims_t f = mkdims(fb), c = mkdims(cb);
+    int ke = p->iturb == 6;
+    const double *vol = F(fb, FF_VOL);
+    static const int st[] = {FF_TEMP, FF_U, FF_V, FF_W, FF_PDIFF, FF_EPS2, FF_VIST};
+    static const int rs[] = {FF_DRO, FF_DM, FF_DN, FF_DW, FF_DE};
+    static const int fo[] = {FF_ROP2H, FF_RMP2H, FF_RNP2H, FF_RWP2H, FF_EP2H};
+    for (int K = 0; K <= c.kmax + 1; K++)
+        for (int J = 0; J <= c.jmax + 1; J++)
+            for (int I = 0; I <= c.imax + 1; I++) {
+                long n = cid(&c, I, J, K);
+                double pvol = 1. / F(cb, FF_VOL)[n];
+                for (int q = 0; q < 7; q++)
+                    F(cb, st[q])[n] = restrict8(&f, vol, F(fb, st[q]), I, J, K, pvol);
+                for (int q = 0; q < 5; q++)
+                    F(cb, fo[q])[n] = restrict8(&f, vol, F(fb, rs[q]), I, J, K, pvol);
+                if (ke) {
+                    F(cb, FF_RK)[n] = restrict8(&f, vol, F(fb, FF_RK), I, J, K, pvol);
+                    F(cb, FF_REPS)[n] = restrict8(&f, vol, F(fb, FF_REPS), I, J, K, pvol);
+                    F(cb, FF_RKP2H)[n] = restrict8(&f, vol, F(fb, FF_DRK), I, J, K, pvol);
+                    F(cb, FF_EPSP2H)[n] = restrict8(&f, vol, F(fb, FF_DEPS), I, J, K, pvol);
+                }
+            }
+    for (int K = 1; K <= c.kmax; K++)
+        for (int J = 1; J <= c.jmax; J++)
+            for (int I = 1; I <= c.imax; I++) {
+                long n = cid(&c, I, J, K);
+                F(cb, FF_TOLD)[n] = F(cb, FF_TEMP)[n];
+                F(cb, FF_UOLD)[n] = F(cb, FF_U)[n];
+                F(cb, FF_VOLD)[n] = F(cb, FF_V)[n];
+                F(cb, FF_WOLD)[n] = F(cb, FF_W)[n];
+                F(cb, FF_POLD)[n] = F(cb, FF_PDIFF)[n];
+                if (ke) {
+                    F(cb, FF_RKOLD)[n] = F(cb, FF_RK)[n];
+                    F(cb, FF_EPSOLD)[n] = F(cb, FF_REPS)[n];
+                }
+            }
+    double *zero[7], *copy[9];
+    int nz = 0, ncp = 0;
+    for (int q = 0; q < 5; q++) zero[nz++] = F(cb, fo[q]);
+    for (int q = 0; q < 7; q++) copy[ncp++] = F(cb, st[q]);
+    if (ke) {
+        zero[nz++] = F(cb, FF_RKP2H); zero[nz++] = F(cb, FF_EPSP2H);
+        copy[ncp++] = F(cb, FF_RK); copy[ncp++] = F(cb, FF_REPS);
+    }
+    restrict_ghosts(&c, zero, nz, copy, ncp);
+}
+
+/* FORCFU multi.f:1630-1648 (phase 0: P = P - D) and ADDRES multi.f:1707-1725 (phase 1:
+ * D = P + D), every cell */
+void ffo_forcing(const ff_params *p, ff_block *b, int phase)
+{
+    dims_t d = mkdims(b);
+    int nq = p->iturb == 6 ? 7 : 5;
+    for (int q = 0; q < nq; q++) {
+        double *P = F(b, FF_ROP2H + q), *D = F(b, FF_DRO + q);
+        for (long n = 0; n < d.ntot; n++) {
+            if (phase == 0) P[n] = P[n] - D[n];
+            else D[n] = P[n] + D[n];
+        }
+    }
+}
+
+/* INTERP multi.f:1763-1822 (injection of the coarse correction into TEMP, U, V, W, P, PDIFF
+ * of the interior fine cells); for two-equation models INTERT/INTERO multi.f:2007-2077
+ * (damped injection into RK, REPS with CMGK, CMGEPS) and the limiters of CYCLES
+ * (ns3c.f:2073-2093); then LOLIMP on the fine level (ns3c.f:2116-2118). */
+void ffo_prolong(const ff_params *p, const ff_block *cb, ff_block *fb)
+{
+    dims_t f = mkdims(fb), c = mkdims(cb);
+    int ke = p->iturb == 6;
+    for (int k = 1; k <= f.kmax; k++)
+        for (int j = 1; j <= f.jmax; j++)
+            for (int i = 1; i <= f.imax; i++) {
+                long ijk = cid(&f, i, j, k);
+                long n = cid(&c, (i + 1) / 2, (j + 1) / 2, (k + 1) / 2);
+                double dtok = F(cb, FF_TEMP)[n] - F(cb, FF_TOLD)[n];
+                double duk = F(cb, FF_U)[n] - F(cb, FF_UOLD)[n];
+                double dvk = F(cb, FF_V)[n] - F(cb, FF_VOLD)[n];
+                double dwk = F(cb, FF_W)[n] - F(cb, FF_WOLD)[n];
+                double dpk = F(cb, FF_PDIFF)[n] - F(cb, FF_POLD)[n];
+                F(fb, FF_TEMP)[ijk] = F(fb, FF_TEMP)[ijk] + dtok;
+                F(fb, FF_U)[ijk] = F(fb, FF_U)[ijk] + duk;
+                F(fb, FF_V)[ijk] = F(fb, FF_V)[ijk] + dvk;
+                F(fb, FF_W)[ijk] = F(fb, FF_W)[ijk] + dwk;
+                F(fb, FF_P)[ijk] = F(fb, FF_P)[ijk] + dpk;
+                F(fb, FF_PDIFF)[ijk] = F(fb, FF_PDIFF)[ijk] + dpk;
+            }
+    if (ke) {
+        const double eps = 1.E-7;
+        const int src[2] = {FF_RK, FF_REPS}, old[2] = {FF_RKOLD, FF_EPSOLD};
+        const double cc1v[2] = {p->cmgk, p->cmgeps};
+        for (int q = 0; q < 2; q++) {
+            double cc1 = cc1v[q], cc2 = cc1 + eps;
+            double *ro = F(fb, src[q]);
+            for (int k = 1; k <= f.kmax; k++)
+                for (int j = 1; j <= f.jmax; j++)
+                    for (int i = 1; i <= f.imax; i++) {
+                        long ijk = cid(&f, i, j, k);
+                        long n = cid(&c, (i + 1) / 2, (j + 1) / 2, (k + 1) / 2);
+                        double drok = F(cb, src[q])[n] - F(cb, old[q])[n];
+                        double alpo = fabs(drok);
+                        ro[ijk] = ro[ijk] + cc1 * ro[ijk] * drok / (cc2 * ro[ijk] + alpo);
+                    }
+        }
+        uplim(fb, F(fb, FF_TEMP), p->tuplim);
+        lolim(fb, F(fb, FF_TEMP), p->tlolim);
+        lolim(fb, F(fb, FF_RK), p->rklim);
+        lolim(fb, F(fb, FF_REPS), p->epslim);
+    }
+    lolimp(p, fb);
+}
+
+/* One CYCLES iteration (ns3c.f:1400-2137) of a block with nlev multigrid levels
+ * b[0] (finest) .. b[nlev-1]: the level-1 stages, the pre-smoothing-only FAS down-loop and
+ * the prolongation loop.  resi receives the RESL2 sums of the last level-1 residual. */
+void ffo_vcycle(const ff_params *p, ff_block **b, int nlev, double *resi)
+{
+    ffo_save_old(p, b[0]);
+    ffo_state(p, b[0]);
+    ffo_turb(p, b[0]);
+    ffo_residual(p, b[0], 1, 1, resi);
+    ffo_implicit(p, b[0], 1);
+    ffo_update_level(p, b[0], 1);
+    for (int m = 2; m <= nlev; m++) {
+        ff_block *fine = b[m - 2], *coarse = b[m - 1];
+        ffo_state(p, fine);
+        ffo_residual(p, fine, m - 1, m, m - 1 == 1 ? resi : NULL);
+        if (m > 2) ffo_forcing(p, fine, 1);
+        ffo_restrict(p, fine, coarse);
+        ffo_state(p, coarse);
+        ffo_residual(p, coarse, m, m, NULL);
+        ffo_forcing(p, coarse, 0);
+        ffo_forcing(p, coarse, 1);
+        ffo_implicit(p, coarse, m);
+        ffo_update_level(p, coarse, m);
+    }
+    for (int m = nlev; m >= 2; m--) ffo_prolong(p, b[m - 1], b[m - 2]);
 }
 
 /* ------------------------------------------------------------------------------------ */

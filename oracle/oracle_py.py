@@ -35,8 +35,18 @@ def oracle():
         for fn in ("ffo_state", "ffo_turb", "ffo_save_old", "ffo_update"):
             getattr(_ora, fn).argtypes = [C.c_void_p, C.c_void_p]
             getattr(_ora, fn).restype = None
-        _ora.ffo_residual.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _ora.ffo_residual.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         _ora.ffo_residual.restype = None
+        _ora.ffo_update_level.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _ora.ffo_update_level.restype = None
+        _ora.ffo_restrict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _ora.ffo_restrict.restype = None
+        _ora.ffo_prolong.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _ora.ffo_prolong.restype = None
+        _ora.ffo_forcing.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _ora.ffo_forcing.restype = None
+        _ora.ffo_vcycle.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _ora.ffo_vcycle.restype = None
         _ora.ffo_implicit.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         _ora.ffo_implicit.restype = None
     return _ora
@@ -76,9 +86,9 @@ class OracleBlock:
     def turb(self):
         oracle().ffo_turb(*self._args())
 
-    def residual(self):
+    def residual(self, m2=1):
         resi = np.zeros(7)
-        oracle().ffo_residual(*self._args(), self.level, resi.ctypes.data)
+        oracle().ffo_residual(*self._args(), self.level, m2, resi.ctypes.data)
         return resi
 
     def implicit(self):
@@ -88,7 +98,11 @@ class OracleBlock:
         oracle().ffo_save_old(*self._args())
 
     def update(self):
-        oracle().ffo_update(*self._args())
+        oracle().ffo_update_level(*self._args(), self.level)
+
+    def forcing(self, phase):
+        """FORCFU (phase 0) or ADDRES (phase 1) on this level."""
+        oracle().ffo_forcing(*self._args(), phase)
 
     def cycle(self):
         self.save_old()
@@ -100,15 +114,41 @@ class OracleBlock:
         return r
 
 
+class OracleHierarchy:
+    """The oracle on a block with its multigrid levels (finest first)."""
+
+    def __init__(self, prm, blocks):
+        self.prm = prm
+        self.levels = [OracleBlock(prm, b, level=m + 1) for m, b in enumerate(blocks)]
+
+    def restrict(self, m):
+        """TRANSF/TRANKE from level m to m+1 (1-based levels)."""
+        oracle().ffo_restrict(C.byref(self.prm), C.byref(self.levels[m - 1]._view),
+                              C.byref(self.levels[m]._view))
+
+    def prolong(self, m):
+        """INTERP/INTERT from level m to m-1."""
+        oracle().ffo_prolong(C.byref(self.prm), C.byref(self.levels[m - 1]._view),
+                             C.byref(self.levels[m - 2]._view))
+
+    def vcycle(self):
+        n = len(self.levels)
+        arr = (C.c_void_p * n)(*[C.addressof(l._view) for l in self.levels])
+        resi = np.zeros(7)
+        oracle().ffo_vcycle(C.byref(self.prm), arr, n, resi.ctypes.data)
+        return resi
+
+
 class RefBlock:
     """Runs the reference's Fortran (through ref_driver.f90) on a Block in place."""
 
     def __init__(self, prm, blk, level=1):
         self.prm, self.blk, self.level = prm, blk, level
         self.pv = prm.dvec()
+        # RESIDI takes first order (INTI = 55) on the coarse levels (ns3c.f:2467-2472)
+        inter = list(prm.inter) if level == 1 else [55, 55, 55]
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
-                            prm.inter[0], prm.inter[1], prm.inter[2], level,
-                            prm.kover, prm.katol],
+                            inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1],
                            dtype=np.int32)
 
     def _a(self):
@@ -121,8 +161,9 @@ class RefBlock:
     def turb(self):
         ref().ref_turb(*self._a())
 
-    def residual(self):
+    def residual(self, m2=1):
         resi = np.zeros(7)
+        self.iv[12] = m2
         ref().ref_residual(*self._a(), resi.ctypes.data)
         return resi
 

@@ -34,6 +34,8 @@ module ffref_ids
   integer, parameter :: IW12=62, IW13=63, IW23=64, IS11=65, IS12=66, IS13=67, IS22=68
   integer, parameter :: IS23=69, IS33=70, ISTRAIN=71, ITTS=72
   integer, parameter :: IB11=73, IB12=74, IB13=75, IB22=76, IB23=77, IB33=78
+  integer, parameter :: IROP2H=79, IRMP2H=80, IRNP2H=81, IRWP2H=82, IEP2H=83, IRKP2H=84
+  integer, parameter :: IEPSP2H=85
   ! parameter vector PRM (same order as the doubles of ff_params)
   integer, parameter :: PGAMMA=1, PRGAS=2, PPR=3, PPRT=4, PVISU0=5, PEXPSU=6, PTSU0=7
   integer, parameter :: PFRSPRE=8, PFRSDEN=9, PFRSTEM=10, PFRSVEL=11, PFRSVIS=12, PCFL=13
@@ -41,7 +43,7 @@ module ffref_ids
   integer, parameter :: PEPSLIM=20, PTURLIM=21, PFRSMUT=22, PRMULT=23, PRE=24, PCSIMPS=25
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
-  integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12
+  integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12, QM2=13
 end module ffref_ids
 
 subroutine ref_state(f, ntot, prm, ipr) bind(C, name="ref_state")
@@ -328,6 +330,15 @@ subroutine ref_residual_kw(f, ntot, prm, ipr, resi)
   call ZEROZZ(f(1,IDRO), ntot); call ZEROZZ(f(1,IDM), ntot); call ZEROZZ(f(1,IDN), ntot)
   call ZEROZZ(f(1,IDW), ntot); call ZEROZZ(f(1,IDE), ntot); call ZEROZZ(f(1,IDRK), ntot)
   call ZEROZZ(f(1,IDEPS), ntot); call ZEROZZ(f(1,ISRK), ntot); call ZEROZZ(f(1,IFUN1), ntot)
+  ! with M2 >= 2 RESIKE forms the velocity gradients itself (res3c.f:3749-3769)
+  if (ipr(QM2) >= 2) then
+     call DERFUN(f(1,IOHMI), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), &
+          f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IU), f(1,IV), f(1,IW), f(1,IA1), &
+          f(1,IA2), f(1,IA3), f(1,IVOL), f(1,ID1), f(1,ID2), f(1,ID3), f(1,IA1X), f(1,IA1Y), &
+          f(1,IA1Z), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3X), f(1,IA3Y), f(1,IA3Z), &
+          f(1,IXC), f(1,IYC), f(1,IZC), imax, jmax, kmax, 2, 2, 2, 1, ipr(QLEVEL), &
+          f(1,ISTRAIN), vellap)
+  end if
   ! COEFKO and the diffusion sigmas (res3c.f:3788-3806)
   call COEFKO(6, a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2, &
        sigphi, zeta2, kover)
@@ -500,6 +511,13 @@ subroutine ref_update(f, ntot, prm, ipr) bind(C, name="ref_update")
        kmax, ntot, 0, 0, ipr(QITURB), 0, prm(PTURLIM), f(1,IVIS), .false., trm, blks, 1, 1)
   call UPLIM(f(1,ITEMP), imax, jmax, kmax, 2, 2, 2, prm(PTUPLIM))
   call LOLIM(f(1,ITEMP), imax, jmax, kmax, 2, 2, 2, prm(PTLOLIM))
+  if (ipr(QLEVEL) >= 2) then
+     ! coarse levels (ns3c.f:1945-1985): TEMP limits and LOLIMP with the level's pressure
+     call LOLIMP(f(1,IP), f(1,IPDIFF), prm(PFRSDEN), prm(PFRSPRE), 0._c_double, 0._c_double, &
+          f(1,IXC), f(1,IYC), f(1,IZC), imax, jmax, kmax, 2, 2, 2)
+     deallocate(fi, dfi, fiold)
+     return
+  end if
   call UPLIM(f(1,IU), imax, jmax, kmax, 2, 2, 2, 5000._c_double)
   call LOLIM(f(1,IU), imax, jmax, kmax, 2, 2, 2, -5000._c_double)
   call UPLIM(f(1,IV), imax, jmax, kmax, 2, 2, 2, 5000._c_double)

@@ -1,0 +1,170 @@
+"""Closed-form checks of the oracle's multigrid transfers (TRANSF/TRANKE, FORCFU/ADDRES,
+INTERP/INTERT, multi.f).  The reference's multi.f cannot be built here, so these pin the
+restatement: a constant restricts to itself, the restriction conserves the volume integral,
+the fine cells each coarse cell averages are the reference's (checked with a field that
+encodes the fine cell index), the ghost treatment zeroes the transferred residual and copies
+the variables outwards, FORCFU/ADDRES reproduce the restricted residual, a zero coarse
+correction prolongs to nothing and a constant one to itself, and the damped k/omega
+prolongation follows its closed form."""
+import numpy as np
+import pytest
+
+from finflo_amd.cases import make_hierarchy
+from finflo_amd.layout import GHOST, default_params
+from oracle.oracle_py import OracleBlock, OracleHierarchy
+
+
+def uniform_hierarchy(nlev=2, iturb=0, dims=(8, 4, 6)):
+    # dyadic spacings: every coarse volume is exactly the sum of its 8 fine volumes
+    prm = default_params(iturb=iturb)
+    blocks = make_hierarchy(*dims, nlev, prm, lengths=(2.0, 1.0, 1.5), kind="all", amp=0.02,
+                            seed=5, turb=(1.0, 100.0) if iturb else None)
+    return prm, blocks
+
+
+def fine_of(c, I, J, K):
+    """The 8 fine cells (1-based) of coarse cell (I, J, K)."""
+    return [(2 * I - 1 + a, 2 * J - 1 + b, 2 * K - 1 + g) for g in (0, 1) for b in (0, 1)
+            for a in (0, 1)]
+
+
+@pytest.mark.parametrize("iturb", [0, 6])
+def test_restrict_constant_and_ghosts(iturb):
+    prm, (f, c) = uniform_hierarchy(iturb=iturb)
+    for name in ("TEMP", "U", "V", "W", "PDIFF", "EPS2", "VIST", "RK", "REPS"):
+        f[name] = 3.25
+    for name in ("DRO", "DM", "DN", "DW", "DE", "DRK", "DEPS"):
+        f[name] = -1.5
+    h = OracleHierarchy(prm, [f, c])
+    h.restrict(1)
+    names = ["TEMP", "U", "V", "W", "PDIFF", "EPS2", "VIST"] + (["RK", "REPS"] if iturb else [])
+    for name in names:
+        np.testing.assert_allclose(c[name], 3.25, rtol=1e-14, err_msg=name)
+    forc = ["ROP2H", "RMP2H", "RNP2H", "RWP2H", "EP2H"] + (["RKP2H", "EPSP2H"] if iturb else [])
+    for name in forc:
+        v = c.view3(name)
+        g = GHOST
+        np.testing.assert_allclose(c.interior(name), -1.5, rtol=1e-14, err_msg=name)
+        inner = np.zeros_like(v, dtype=bool)
+        inner[g:-g, g:-g, g:-g] = True
+        assert np.all(v[~inner] == 0.0), f"{name}: ghost residual transfer not zero"
+    for old, new in (("TOLD", "TEMP"), ("UOLD", "U"), ("VOLD", "V"), ("WOLD", "W"),
+                     ("POLD", "PDIFF")):
+        np.testing.assert_array_equal(c.interior(old), c.interior(new))
+
+
+def test_restrict_maps_the_reference_fine_cells():
+    """A fine field that encodes each fine cell's index: the coarse value must be the
+    volume-weighted mean over exactly the 8 fine cells (2I-1..2I, 2J-1..2J, 2K-1..2K)."""
+    prm, (f, c) = uniform_hierarchy(dims=(8, 8, 4))
+    ks, js, is_ = f.kstrid, f.jstrid, f.istrid
+    K, J, I = np.meshgrid(np.arange(ks) - 1, np.arange(js) - 1, np.arange(is_) - 1, indexing="ij")
+    code = (I + 100.0 * J + 10000.0 * K).ravel()       # 1-based fine cell index code
+    f["TEMP"] = code
+    h = OracleHierarchy(prm, [f, c])
+    h.restrict(1)
+    vol = f["VOL"]
+    for (I2, J2, K2) in [(1, 1, 1), (4, 2, 1), (0, 3, 2), (5, 5, 3), (2, 0, 0)]:
+        cells = fine_of(c, I2, J2, K2)
+        offs = [f.cell(i, j, k) for i, j, k in cells]
+        want = sum(vol[o] * code[o] for o in offs) / c["VOL"][c.cell(I2, J2, K2)]
+        got = c["TEMP"][c.cell(I2, J2, K2)]
+        assert abs(got - want) <= 1e-12 * abs(want), (I2, J2, K2, got, want)
+
+
+def test_restrict_conserves_the_volume_integral():
+    prm, (f, c) = uniform_hierarchy(dims=(16, 8, 8))
+    h = OracleHierarchy(prm, [f, c])
+    rng = np.random.default_rng(3)
+    f["DRO"] = rng.uniform(-1, 1, f.ntot)
+    h.restrict(1)
+    fine = np.sum(f.interior("VOL") * f.interior("DRO"))
+    coarse = np.sum(c.interior("VOL") * c.interior("ROP2H"))
+    assert abs(fine - coarse) <= 1e-13 * np.sum(f.interior("VOL") * np.abs(f.interior("DRO")))
+
+
+def test_outer_ghosts_copy_the_first_layer():
+    prm, (f, c) = uniform_hierarchy(dims=(8, 8, 4))
+    h = OracleHierarchy(prm, [f, c])
+    h.restrict(1)
+    v = c.view3("TEMP")
+    # the second ghost layer equals the first (I, then J, then K faces)
+    np.testing.assert_array_equal(v[:, :, 0], v[:, :, 1])
+    np.testing.assert_array_equal(v[:, :, -1], v[:, :, -2])
+    np.testing.assert_array_equal(v[:, 0, :], v[:, 1, :])
+    np.testing.assert_array_equal(v[:, -1, :], v[:, -2, :])
+    np.testing.assert_array_equal(v[0], v[1])
+    np.testing.assert_array_equal(v[-1], v[-2])
+
+
+@pytest.mark.parametrize("iturb", [0, 6])
+def test_forcing_reproduces_the_restricted_residual(iturb):
+    prm, (f, c) = uniform_hierarchy(iturb=iturb)
+    rng = np.random.default_rng(7)
+    nq = 7 if iturb else 5
+    res = ["DRO", "DM", "DN", "DW", "DE", "DRK", "DEPS"][:nq]
+    frc = ["ROP2H", "RMP2H", "RNP2H", "RWP2H", "EP2H", "RKP2H", "EPSP2H"][:nq]
+    for r, p_ in zip(res, frc):
+        c[p_] = rng.uniform(-1, 1, c.ntot)
+        c[r] = rng.uniform(-1, 1, c.ntot)
+    p0 = {p_: c[p_].copy() for p_ in frc}
+    d0 = {r: c[r].copy() for r in res}
+    oc = OracleBlock(prm, c, level=2)
+    oc.forcing(0)
+    for r, p_ in zip(res, frc):
+        np.testing.assert_array_equal(c[p_], p0[p_] - d0[r])
+    oc.forcing(1)
+    for r, p_ in zip(res, frc):
+        np.testing.assert_allclose(c[r], p0[p_], rtol=0, atol=4e-16)   # (P - D) + D
+
+
+@pytest.mark.parametrize("iturb", [0, 6])
+def test_zero_and_constant_corrections(iturb):
+    prm, (f, c) = uniform_hierarchy(iturb=iturb)
+    h = OracleHierarchy(prm, [f, c])
+    h.restrict(1)
+    before = f.copy()
+    h.prolong(2)                                  # coarse state == OLD: no correction
+    for name in ("TEMP", "U", "V", "W", "PDIFF", "RK", "REPS"):
+        np.testing.assert_array_equal(f[name], before[name])
+    # a constant correction dT = 2.5 on every coarse cell reaches every interior fine cell
+    c["TEMP"] = c["TOLD"] + 2.5
+    c["U"] = c["UOLD"] - 1.0
+    t0, u0 = f.interior("TEMP").copy(), f.interior("U").copy()
+    h.prolong(2)
+    np.testing.assert_allclose(f.interior("TEMP") - t0, 2.5, rtol=1e-12)
+    np.testing.assert_allclose(f.interior("U") - u0, -1.0, rtol=1e-12)
+
+
+def test_damped_turbulence_prolongation_closed_form():
+    """INTERO: phi += C*phi*d/((C+1E-7)*phi + |d|) with C = CMGK for rho*k."""
+    prm, (f, c) = uniform_hierarchy(iturb=6)
+    h = OracleHierarchy(prm, [f, c])
+    h.restrict(1)
+    rng = np.random.default_rng(11)
+    d = rng.uniform(-0.5, 0.5, c.ntot)
+    c["RK"] = c["RKOLD"] + d
+    rk0 = f["RK"].copy()
+    h.prolong(2)
+    cc1 = prm.cmgk
+    for (i, j, k) in [(1, 1, 1), (3, 2, 4), (8, 4, 6), (5, 1, 2)]:
+        n = f.cell(i, j, k)
+        nc = c.cell((i + 1) // 2, (j + 1) // 2, (k + 1) // 2)
+        dd = c["RK"][nc] - c["RKOLD"][nc]
+        want = rk0[n] + cc1 * rk0[n] * dd / ((cc1 + 1e-7) * rk0[n] + abs(dd))
+        want = max(want, prm.rklim)
+        assert abs(f["RK"][n] - want) <= 1e-15 * abs(want)
+        # the damped increment never exceeds C*phi in magnitude
+        assert abs(f["RK"][n] - rk0[n]) <= cc1 * rk0[n]
+
+
+@pytest.mark.parametrize("iturb", [0, 6])
+def test_single_level_vcycle_is_the_plain_cycle(iturb):
+    prm = default_params(iturb=iturb)
+    (b,) = make_hierarchy(8, 6, 4, 1, prm, distort=0.1, kind="all", amp=0.03, seed=9,
+                          turb=(1.0, 100.0) if iturb else None)
+    a = b.copy()
+    OracleBlock(prm, a).cycle()
+    h = OracleHierarchy(prm, [b])
+    h.vcycle()
+    np.testing.assert_array_equal(a.F, b.F)

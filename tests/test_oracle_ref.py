@@ -72,3 +72,58 @@ def test_sst_cycles_bit_exact(name, dims, pover, ckw):
         r1, r2 = oa.cycle(), rb.cycle()
         assert np.array_equal(r1, r2)
     assert_fields_equal(a, b, SST_PRIM, interior=False, what="4 cycles")
+
+
+# coarse multigrid levels: RESIDI with INTI = 55 and, for SST, RESIKE's own DERFUN (M2 >= 2,
+# interior + EXTEND); IMPSDI with CFLL; FIN with the coarse-level limiters of CYCLES
+COARSE_CASES = [
+    ("lam_coarse", (6, 8, 4), dict(cfll=3.0), dict(kind="all", amp=0.05, distort=0.12, seed=51)),
+    ("sst_coarse", (8, 6, 6), dict(iturb=6, cfll=3.0),
+     dict(kind="all", amp=0.05, distort=0.12, seed=53, turb=(1.0, 60.0))),
+    ("sst_coarse_wall", (6, 6, 4), dict(iturb=6, re=1e14, cfll=2.0),
+     dict(kind="velocity", amp=0.03, jratio=1.3, seed=57, turb=(0.5, 300.0))),
+]
+
+
+@pytest.mark.parametrize("level", [2, 3])
+@pytest.mark.parametrize("name,dims,pover,ckw", COARSE_CASES, ids=[c[0] for c in COARSE_CASES])
+def test_coarse_level_stages_bit_exact(name, dims, pover, ckw, level):
+    prm, blk = make_case(dims, pover, ckw)
+    if prm.iturb == 6:
+        # on a coarse level EPS2 and VIST are the restricted fine values
+        rng = np.random.default_rng(61)
+        blk["EPS2"] = 1.0 + rng.uniform(5.0, 50.0, blk.ntot)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a, level=level), RefBlock(prm, b, level=level)
+    oa.save_old(); rb.save_old()
+    oa.state(); rb.state()
+    if prm.iturb == 6:
+        a["VIST"] = (a["EPS2"] - 1.0) * a["VIS"]
+        b["VIST"] = a["VIST"]
+    oa.residual(m2=level); rb.residual(m2=level)
+    if prm.iturb == 6:
+        assert_fields_equal(a, b, SST_TURB[:10], interior=False, what="DERFUN+EXTEND")
+        assert_fields_equal(a, b, SST_RES, what="RESIKE coarse")
+    else:
+        assert_fields_equal(a, b, RES, what="RESIDU coarse")
+    oa.implicit(); rb.implicit()
+    nres = SST_RES[:7] if prm.iturb == 6 else RES
+    assert_fields_equal(a, b, ["DTL"] + nres, what="IMPS coarse")
+    oa.update(); rb.update()
+    prim = SST_PRIM if prm.iturb == 6 else PRIM
+    assert_fields_equal(a, b, prim, interior=False, what="FIN coarse")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", SST_CASES[:2], ids=[c[0] for c in SST_CASES[:2]])
+def test_sst_fine_level_second_residual(name, dims, pover, ckw):
+    """The level-1 residual the V-cycle re-evaluates after the update (RESIDI(1,N,2)):
+    RESIKE forms the velocity gradients itself, first ghost layer included."""
+    prm, blk = make_case(dims, pover, ckw)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    for o in (oa, rb):
+        o.save_old(); o.state(); o.turb(); o.residual(); o.implicit(); o.update(); o.state()
+    r1, r2 = oa.residual(m2=2), rb.residual(m2=2)
+    assert_fields_equal(a, b, SST_TURB[:11], interior=False, what="DERFUN level 1")
+    assert_fields_equal(a, b, SST_RES, what="RESIKE M2=2")
+    assert np.array_equal(r1, r2)
