@@ -103,18 +103,11 @@ __global__ void __launch_bounds__(256) k_turb_lam(Blk b)
 
 __device__ __forceinline__ double pow4(double x) { return x * x * x * x; }   // ((x*x)*x)*x
 
-// TURBEQ (res3c.f:5951-6371) on the cells of level 1 plus one ghost layer: DERFUN
-// (turb3.f:2476-2665, IDERI = 1: distance-weighted Green-Gauss) and TTIMES (earsm.f:671-684)
-// everywhere there, EPS2KO (turb3.f:7454-7537, limiter argument STRAIN, FRSMUT =
-// 1E-3*FRSVIS) on the interior.  Thread per cell, I fastest.
-__global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
+// DERFUN (turb3.f:2539-2665, IDERI = 1) at cell L: velocity gradients by distance-weighted
+// Green-Gauss, stored as vorticity, strain rate, |Omega| (OHMI) and strain magnitude
+__device__ __forceinline__ double derfun_cell(const Blk &b, int L)
 {
-    const int ni = b.imax + 2;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= ni) return;
     const int IS = b.is, IL = b.il;
-    const int L = cid(b, i, j, k);
     const double *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W);
     const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3);
     const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
@@ -167,6 +160,21 @@ __global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
     FLD(b, FF_STRAIN)[L] = strain;
     FLD(b, FF_S11)[L] = s11; FLD(b, FF_S12)[L] = s12; FLD(b, FF_S13)[L] = s13;
     FLD(b, FF_S22)[L] = s22; FLD(b, FF_S23)[L] = s23; FLD(b, FF_S33)[L] = s33;
+    return strain;
+}
+
+// TURBEQ (res3c.f:5951-6371) on the cells of level 1 plus one ghost layer: DERFUN
+// (turb3.f:2476-2665, IDERI = 1: distance-weighted Green-Gauss) and TTIMES (earsm.f:671-684)
+// everywhere there, EPS2KO (turb3.f:7454-7537, limiter argument STRAIN, FRSMUT =
+// 1E-3*FRSVIS) on the interior.  Thread per cell, I fastest.
+__global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
+{
+    const int ni = b.imax + 2;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= ni) return;
+    const int L = cid(b, i, j, k);
+    const double strain = derfun_cell(b, L);
     // TTIMES
     const double ro = FLD(b, FF_RO)[L], vis = FLD(b, FF_VIS)[L];
     const double rk = FLD(b, FF_RK)[L], rom = FLD(b, FF_REPS)[L];
@@ -204,6 +212,220 @@ __global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
     FLD(b, FF_VIST)[L] = rmutrb;
     double rmutpv = rmutrb / vis;
     FLD(b, FF_EPS2)[L] = 1.0 + rmutpv;
+}
+
+// RESIKE with M2 >= 2 (res3c.f:3749-3769): OHMI, W12 and SIJ zeroed on every cell, then
+// DERFUN on level 1 with the first ghost layer (g0 = 1) or on a coarse level's interior
+// (g0 = 0).  Thread per cell of the range.
+constexpr int NGRAD = 10;
+__global__ void __launch_bounds__(256) k_grad_zero(Blk b)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    FLD(b, FF_OHMI)[n] = 0.;
+#pragma unroll
+    for (int q = 0; q < 9; q++) FLD(b, FF_W12 + q)[n] = 0.;
+}
+
+__global__ void __launch_bounds__(256) k_derfun(Blk b, int g0)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1 - g0;
+    int j = blockIdx.y + 1 - g0, k = blockIdx.z + 1 - g0;
+    if (i > b.imax + g0) return;
+    derfun_cell(b, cid(b, i, j, k));
+}
+
+// EXTEND (turb3.f:3342-3450) of OHMI, W12 and SIJ on a coarse level: the interior value
+// next to each face copied into the first ghost layer (no edges or corners)
+__global__ void __launch_bounds__(256) k_extend(Blk b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > b.imax) return;
+    const int n = cid(b, i, j, k);
+    const int IS = b.is, IL = b.il;
+    int t[6], nt = 0;
+    if (k == 1) t[nt++] = n - IL;
+    if (k == b.kmax) t[nt++] = n + IL;
+    if (j == 1) t[nt++] = n - IS;
+    if (j == b.jmax) t[nt++] = n + IS;
+    if (i == 1) t[nt++] = n - 1;
+    if (i == b.imax) t[nt++] = n + 1;
+    if (nt == 0) return;
+    for (int q = 0; q < NGRAD; q++) {
+        double *x = FLD(b, q == 0 ? FF_OHMI : FF_W12 + q - 1);
+        double v = x[n];
+        for (int r = 0; r < nt; r++) x[t[r]] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// multigrid transfers (multi.f)
+// ------------------------------------------------------------------------------------
+
+// Volume-weighted sum of the 8 fine cells of a coarse cell in TRANSF's order
+// (multi.f:214-218): IPJPK, IJK, IPJK, IJPK, IPJPKP, IJKP, IPJKP, IJPKP
+__device__ __forceinline__ double restrict8(const Blk &f, const double *vol, const double *x,
+                                            int ijk, double pvol)
+{
+    const int ipjk = ijk + 1, ijpk = ijk + f.is, ipjpk = ijk + f.is + 1;
+    const int ijkp = ijk + f.il, ipjpkp = ipjpk + f.il, ijpkp = ijpk + f.il, ipjkp = ipjk + f.il;
+    return (vol[ipjpk] * x[ipjpk] + vol[ijk] * x[ijk] + vol[ipjk] * x[ipjk] +
+            vol[ijpk] * x[ijpk] + vol[ipjpkp] * x[ipjpkp] + vol[ijkp] * x[ijkp] +
+            vol[ipjkp] * x[ipjkp] + vol[ijpkp] * x[ijpkp]) * pvol;
+}
+
+// TRANSF (+ TRANKE, KW) loop 1000 and the OLD copies (multi.f:194-278, 453-510): coarse cell
+// per thread over the interior and the first ghost layer
+template <bool KW>
+__global__ void __launch_bounds__(256) k_restrict(Blk f, Blk c)
+{
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    int J = blockIdx.y, K = blockIdx.z;
+    if (I > c.imax + 1) return;
+    const int n = cid(c, I, J, K);
+    const int ijk = cid(f, 2 * I - 1, 2 * J - 1, 2 * K - 1);
+    const double *vol = FLD(f, FF_VOL);
+    const double pvol = 1. / FLD(c, FF_VOL)[n];
+    const int st[7] = {FF_TEMP, FF_U, FF_V, FF_W, FF_PDIFF, FF_EPS2, FF_VIST};
+    double x[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        x[q] = restrict8(f, vol, FLD(f, st[q]), ijk, pvol);
+        FLD(c, st[q])[n] = x[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) FLD(c, FF_ROP2H + q)[n] = restrict8(f, vol, FLD(f, FF_DRO + q), ijk, pvol);
+    double rk = 0., re = 0.;
+    if (KW) {
+        rk = restrict8(f, vol, FLD(f, FF_RK), ijk, pvol);
+        re = restrict8(f, vol, FLD(f, FF_REPS), ijk, pvol);
+        FLD(c, FF_RK)[n] = rk;
+        FLD(c, FF_REPS)[n] = re;
+        FLD(c, FF_RKP2H)[n] = restrict8(f, vol, FLD(f, FF_DRK), ijk, pvol);
+        FLD(c, FF_EPSP2H)[n] = restrict8(f, vol, FLD(f, FF_DEPS), ijk, pvol);
+    }
+    if (I >= 1 && I <= c.imax && J >= 1 && J <= c.jmax && K >= 1 && K <= c.kmax) {
+        FLD(c, FF_TOLD)[n] = x[0];
+        FLD(c, FF_UOLD)[n] = x[1];
+        FLD(c, FF_VOLD)[n] = x[2];
+        FLD(c, FF_WOLD)[n] = x[3];
+        FLD(c, FF_POLD)[n] = x[4];
+        if (KW) {
+            FLD(c, FF_RKOLD)[n] = rk;
+            FLD(c, FF_EPSOLD)[n] = re;
+        }
+    }
+}
+
+// The ghost treatment of TRANSF/TRANKE (loops 2100-2600, multi.f:283-425) for one pair of
+// opposite faces: face 0 = I, 1 = J, 2 = K.  Residual transfers are zeroed on both ghost
+// layers and the variables copied from the first to the second; the launches for I, J and
+// K run in that order, as the reference's loops do.
+template <bool KW>
+__global__ void __launch_bounds__(256) k_restrict_ghosts(Blk c, int face)
+{
+    const int na = (face == 0 ? c.jmax : c.imax) + 4;          // index a runs over -1..N+2
+    int a = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    int bb = blockIdx.y - 1;                                   // the other index
+    if (a > na - 2) return;
+    int lo, hi, st;
+    if (face == 0) {        // (J = a, K = bb)
+        lo = cid(c, 0, a, bb); hi = cid(c, c.imax + 1, a, bb); st = 1;
+    } else if (face == 1) { // (I = a, K = bb)
+        lo = cid(c, a, 0, bb); hi = cid(c, a, c.jmax + 1, bb); st = c.is;
+    } else {                // (I = a, J = bb)
+        lo = cid(c, a, bb, 0); hi = cid(c, a, bb, c.kmax + 1); st = c.il;
+    }
+    const int nz = KW ? 7 : 5;
+    for (int q = 0; q < nz; q++) {
+        double *x = FLD(c, FF_ROP2H + q);
+        x[lo] = 0.; x[lo - st] = 0.; x[hi] = 0.; x[hi + st] = 0.;
+    }
+    const int cp[9] = {FF_TEMP, FF_U, FF_V, FF_W, FF_PDIFF, FF_EPS2, FF_VIST, FF_RK, FF_REPS};
+    const int ncp = KW ? 9 : 7;
+    for (int q = 0; q < ncp; q++) {
+        double *x = FLD(c, cp[q]);
+        x[lo - st] = x[lo];
+        x[hi + st] = x[hi];
+    }
+}
+
+// FORCFU (phase 0, P = P - D, multi.f:1630-1648) or ADDRES (phase 1, D = P + D,
+// multi.f:1707-1725) on every cell
+__global__ void __launch_bounds__(256) k_forcing(Blk b, int nq, int phase)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    for (int q = 0; q < nq; q++) {
+        double *P = FLD(b, FF_ROP2H + q), *D = FLD(b, FF_DRO + q);
+        if (phase == 0) P[n] = P[n] - D[n];
+        else D[n] = P[n] + D[n];
+    }
+}
+
+// INTERP (multi.f:1790-1819) and, KW, INTERT/INTERO (multi.f:2007-2077): fine interior cell
+// per thread, the correction of coarse cell ((I+1)/2, (J+1)/2, (K+1)/2)
+template <bool KW>
+__global__ void __launch_bounds__(256) k_prolong(Prm p, Blk c, Blk f)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > f.imax) return;
+    const int ijk = cid(f, i, j, k);
+    const int n = cid(c, (i + 1) / 2, (j + 1) / 2, (k + 1) / 2);
+    double dtok = FLD(c, FF_TEMP)[n] - FLD(c, FF_TOLD)[n];
+    double duk = FLD(c, FF_U)[n] - FLD(c, FF_UOLD)[n];
+    double dvk = FLD(c, FF_V)[n] - FLD(c, FF_VOLD)[n];
+    double dwk = FLD(c, FF_W)[n] - FLD(c, FF_WOLD)[n];
+    double dpk = FLD(c, FF_PDIFF)[n] - FLD(c, FF_POLD)[n];
+    FLD(f, FF_TEMP)[ijk] = FLD(f, FF_TEMP)[ijk] + dtok;
+    FLD(f, FF_U)[ijk] = FLD(f, FF_U)[ijk] + duk;
+    FLD(f, FF_V)[ijk] = FLD(f, FF_V)[ijk] + dvk;
+    FLD(f, FF_W)[ijk] = FLD(f, FF_W)[ijk] + dwk;
+    FLD(f, FF_P)[ijk] = FLD(f, FF_P)[ijk] + dpk;
+    FLD(f, FF_PDIFF)[ijk] = FLD(f, FF_PDIFF)[ijk] + dpk;
+    if (KW) {
+        const double eps = 1.E-7;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const double cc1 = q == 0 ? p.cmgk : p.cmgeps, cc2 = cc1 + eps;
+            double *ro = FLD(f, q == 0 ? FF_RK : FF_REPS);
+            double drok = FLD(c, q == 0 ? FF_RK : FF_REPS)[n] - FLD(c, q == 0 ? FF_RKOLD : FF_EPSOLD)[n];
+            double alpo = fabs(drok);
+            double r = ro[ijk];
+            ro[ijk] = r + cc1 * r * drok / (cc2 * r + alpo);
+        }
+    }
+}
+
+// The limiters CYCLES applies on a level other than the finest after FIN (ns3c.f:1936-1985:
+// UPLIM/LOLIM of TEMP, LOLIMP with the level's last pressure) or on a level after the
+// prolongation into it (ns3c.f:2073-2118: for two-equation models the TEMP limits and LOLIM of
+// RK and REPS, then LOLIMP).  Every cell.
+__global__ void __launch_bounds__(256) k_limits_mg(Prm p, Blk b, int temp_limits, int kw_limits)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= b.ntot) return;
+    if (temp_limits) {
+        int kk = n / b.il;
+        int jj = (n / b.is) % b.js;
+        bool up = kk >= 2 && kk < 2 + b.kmax && jj >= 2 && jj < 2 + b.jmax;
+        double t = FLD(b, FF_TEMP)[n];
+        if (up) t = t < p.tuplim ? t : p.tuplim;
+        t = t > p.tlolim ? t : p.tlolim;
+        FLD(b, FF_TEMP)[n] = t;
+    }
+    if (kw_limits) {
+        double rk = FLD(b, FF_RK)[n], re = FLD(b, FF_REPS)[n];
+        FLD(b, FF_RK)[n] = rk > p.rklim ? rk : p.rklim;
+        FLD(b, FF_REPS)[n] = re > p.epslim ? re : p.epslim;
+    }
+    double P = FLD(b, FF_P)[n];
+    double y = P - 0.01 * p.frspre;
+    double prelim = 0. < y ? 0. : y;
+    FLD(b, FF_P)[n] = P - prelim;
+    FLD(b, FF_PDIFF)[n] = FLD(b, FF_PDIFF)[n] - prelim;
 }
 
 // k = RK/RO and omega = REPS/RO as CROSKO forms them in place (turb3.f:7817-7821)
@@ -1539,6 +1761,7 @@ static Prm make_prm(const ff_params *p, int level)
     k.turlim = p->turlim;
     k.iturb = p->iturb;
     k.kover = p->kover; k.katol = p->katol;
+    k.cmgk = p->cmgk; k.cmgeps = p->cmgeps;
     for (int d = 0; d < 3; d++) k.inter[d] = level >= 2 ? 55 : p->inter[d];   // res3c RESIDI ns3c.f:2468-2472
     return k;
 }
@@ -1911,10 +2134,18 @@ int ff_turb(ff_ctx *c, int level, int block)
     return launch(c, "k_turb_lam", k_turb_lam, cells_grid(r->dev, 256), dim3(256), r->dev);
 }
 
-static int residual_impl(ff_ctx *c, BlockRec *r, int level, double *resi)
+static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi)
 {
     const Blk &b = r->dev;
     int rc;
+    if (c->prm.iturb == 6 && m2 >= 2) {
+        // RESIKE's own DERFUN (res3c.f:3749-3769), EXTEND on coarse levels
+        const int g0 = level == 1 ? 1 : 0;
+        if ((rc = launch(c, "k_grad_zero", k_grad_zero, all_grid(b), dim3(256), b))) return rc;
+        dim3 g((b.imax + 2 * g0 + 255) / 256, b.jmax + 2 * g0, b.kmax + 2 * g0);
+        if ((rc = launch(c, "k_derfun", k_derfun, g, dim3(256), b, g0))) return rc;
+        if (level >= 2 && (rc = launch(c, "k_extend", k_extend, cells_grid(b, 256), dim3(256), b))) return rc;
+    }
     const bool kw = c->prm.iturb == 6;
     const int nq = kw ? 7 : 5;
     // ZEROZZ of DRO..DE (DRK, DEPS) on every cell (res3c.f:194-198, 3668-3674): ghost
@@ -1968,7 +2199,7 @@ int ff_residual(ff_ctx *c, int level, int block, int m2, double *resi)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
-    return residual_impl(c, r, level, resi);
+    return residual_impl(c, r, level, m2, resi);
 }
 
 static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
@@ -2032,10 +2263,62 @@ int ff_implicit(ff_ctx *c, int level, int block, int m2)
 static int update_impl(ff_ctx *c, BlockRec *r, int level)
 {
     const Blk &b = r->dev;
-    int rc = c->prm.iturb == 6 ? launch(c, "k_fin", k_fin_kw, cells_grid(b, 256), dim3(256), b)
-                               : launch(c, "k_fin", k_fin, cells_grid(b, 256), dim3(256), b);
+    const bool kw = c->prm.iturb == 6;
+    int rc = kw ? launch(c, "k_fin", k_fin_kw, cells_grid(b, 256), dim3(256), b)
+                : launch(c, "k_fin", k_fin, cells_grid(b, 256), dim3(256), b);
     if (rc) return rc;
+    if (level >= 2)
+        return launch(c, "k_limits_mg", k_limits_mg, all_grid(b), dim3(256), make_prm(&c->prm, level),
+                      b, 1, kw ? 1 : 0);
     return launch(c, "k_limits", k_limits, all_grid(b), dim3(256), make_prm(&c->prm, level), b);
+}
+
+// TRANSF/TRANKE from level m to level m+1 of a block (multi.f:156-620)
+static int restrict_impl(ff_ctx *c, BlockRec *fr, BlockRec *cr)
+{
+    const Blk &f = fr->dev, &cb = cr->dev;
+    const bool kw = c->prm.iturb == 6;
+    int rc;
+    dim3 g((cb.imax + 2 + 255) / 256, cb.jmax + 2, cb.kmax + 2);
+    rc = kw ? launch(c, "k_restrict", k_restrict<true>, g, dim3(256), f, cb)
+            : launch(c, "k_restrict", k_restrict<false>, g, dim3(256), f, cb);
+    if (rc) return rc;
+    const int na[3] = {cb.jmax + 4, cb.imax + 4, cb.imax + 4};
+    const int nb[3] = {cb.kmax + 4, cb.kmax + 4, cb.jmax + 4};
+    for (int face = 0; face < 3; face++) {
+        dim3 gg((na[face] + 255) / 256, nb[face]);
+        rc = kw ? launch(c, "k_restrict_ghosts", k_restrict_ghosts<true>, gg, dim3(256), cb, face)
+                : launch(c, "k_restrict_ghosts", k_restrict_ghosts<false>, gg, dim3(256), cb, face);
+        if (rc) return rc;
+    }
+    return FF_OK;
+}
+
+static int forcing_impl(ff_ctx *c, BlockRec *r, int phase)
+{
+    return launch(c, "k_forcing", k_forcing, all_grid(r->dev), dim3(256), r->dev,
+                  c->prm.iturb == 6 ? 7 : 5, phase);
+}
+
+// INTERP/INTERT from level m to level m-1 and the limiters of CYCLES (ns3c.f:2017-2118)
+static int prolong_impl(ff_ctx *c, BlockRec *cr, BlockRec *fr, int level_fine)
+{
+    const Blk &f = fr->dev, &cb = cr->dev;
+    const bool kw = c->prm.iturb == 6;
+    Prm kp = make_prm(&c->prm, level_fine);
+    int rc = kw ? launch(c, "k_prolong", k_prolong<true>, cells_grid(f, 256), dim3(256), kp, cb, f)
+                : launch(c, "k_prolong", k_prolong<false>, cells_grid(f, 256), dim3(256), kp, cb, f);
+    if (rc) return rc;
+    return launch(c, "k_limits_mg", k_limits_mg, all_grid(f), dim3(256), kp, f, kw ? 1 : 0, kw ? 1 : 0);
+}
+
+static int check_pair(ff_ctx *c, BlockRec *fr, BlockRec *cr)
+{
+    if (!fr || !cr) return fail(c, FF_E_STATE, "multigrid level not registered%s");
+    const Blk &f = fr->dev, &cb = cr->dev;
+    if (f.imax != 2 * cb.imax || f.jmax != 2 * cb.jmax || f.kmax != 2 * cb.kmax)
+        return fail(c, FF_E_ARG, "a coarse level must have half the cells of the finer level in each direction%s");
+    return FF_OK;
 }
 
 int ff_update(ff_ctx *c, int level, int block)
@@ -2065,11 +2348,84 @@ int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
         if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev))) return rc;
         if ((rc = ff_state(c, 1, block))) return rc;
         if ((rc = ff_turb(c, 1, block))) return rc;
-        if ((rc = residual_impl(c, r, 1, last ? resi : nullptr))) return rc;
+        if ((rc = residual_impl(c, r, 1, 1, last ? resi : nullptr))) return rc;
         if ((rc = implicit_impl(c, r, 1, true))) return rc;
         if ((rc = update_impl(c, r, 1))) return rc;
     }
     return c->lusgs_mode == FF_LUSGS_TILES ? sweep_check(c, *r) : FF_OK;
+}
+
+int ff_restrict(ff_ctx *c, int mfine, int block)
+{
+    if (!c) return FF_E_ARG;
+    BlockRec *fr = find(c, mfine, block), *cr = find(c, mfine + 1, block);
+    int rc = check_pair(c, fr, cr);
+    if (rc) return rc;
+    return restrict_impl(c, fr, cr);
+}
+
+int ff_forcing(ff_ctx *c, int level, int block, int phase)
+{
+    if (!c || (phase != 0 && phase != 1)) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    return forcing_impl(c, r, phase);
+}
+
+int ff_prolong(ff_ctx *c, int mcoarse, int block)
+{
+    if (!c || mcoarse < 2) return FF_E_ARG;
+    BlockRec *fr = find(c, mcoarse - 1, block), *cr = find(c, mcoarse, block);
+    int rc = check_pair(c, fr, cr);
+    if (rc) return rc;
+    return prolong_impl(c, cr, fr, mcoarse - 1);
+}
+
+int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
+{
+    if (!c || ncycles < 0 || nlev < 1) return FF_E_ARG;
+    std::vector<BlockRec *> lv(nlev);
+    for (int m = 1; m <= nlev; m++) {
+        lv[m - 1] = find(c, m, block);
+        if (!lv[m - 1]) return fail(c, FF_E_STATE, "multigrid level not registered%s");
+        if (m >= 2) {
+            int rc = check_pair(c, lv[m - 2], lv[m - 1]);
+            if (rc) return rc;
+        }
+    }
+    int rc;
+    for (int it = 0; it < ncycles; it++) {
+        double *rs = it == ncycles - 1 ? resi : nullptr;
+        BlockRec *r1 = lv[0];
+        // level 1: CYCLES ns3c.f:1450-1596
+        if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev))) return rc;
+        if ((rc = ff_state(c, 1, block))) return rc;
+        if ((rc = ff_turb(c, 1, block))) return rc;
+        if ((rc = residual_impl(c, r1, 1, 1, rs))) return rc;
+        if ((rc = implicit_impl(c, r1, 1, true))) return rc;
+        if ((rc = update_impl(c, r1, 1))) return rc;
+        // FAS down-loop, pre-smoothing only (ns3c.f:1607-1999)
+        for (int m = 2; m <= nlev; m++) {
+            BlockRec *fr = lv[m - 2], *cr = lv[m - 1];
+            if ((rc = ff_state(c, m - 1, block))) return rc;
+            if ((rc = residual_impl(c, fr, m - 1, m, m - 1 == 1 ? rs : nullptr))) return rc;
+            if (m > 2 && (rc = forcing_impl(c, fr, 1))) return rc;
+            if ((rc = restrict_impl(c, fr, cr))) return rc;
+            if ((rc = ff_state(c, m, block))) return rc;
+            if ((rc = residual_impl(c, cr, m, m, nullptr))) return rc;
+            if ((rc = forcing_impl(c, cr, 0))) return rc;
+            if ((rc = forcing_impl(c, cr, 1))) return rc;
+            if ((rc = implicit_impl(c, cr, m, true))) return rc;
+            if ((rc = update_impl(c, cr, m))) return rc;
+        }
+        // prolongation (ns3c.f:2005-2137)
+        for (int m = nlev; m >= 2; m--)
+            if ((rc = prolong_impl(c, lv[m - 1], lv[m - 2], m - 1))) return rc;
+    }
+    if (c->lusgs_mode == FF_LUSGS_TILES)
+        for (int m = 0; m < nlev; m++)
+            if ((rc = sweep_check(c, *lv[m]))) return rc;
+    return FF_OK;
 }
 
 }  // extern "C"

@@ -24,7 +24,7 @@ struct Prm {
     double gamma, rgas, pr, prt, visu0, tsu0, rmult;
     double frspre, frsden, frstem;
     double cfl, theta, cdiff, tuplim, tlolim, epslim;
-    double rklim, frsvis, re, csimps, turlim;
+    double rklim, frsvis, re, csimps, turlim, cmgk, cmgeps;
     int iturb;
     int inter[3];
     int kover, katol;

@@ -20,7 +20,8 @@ EXPORTS = [
     "ff_last_error", "ff_ctx_profile",
     "ff_kernel_stats", "ff_kernel_stats_reset", "ff_block_register",
     "ff_block_register_device", "ff_sync", "ff_finish", "ff_state", "ff_turb", "ff_residual",
-    "ff_implicit", "ff_update", "ff_save_old", "ff_cycle",
+    "ff_implicit", "ff_update", "ff_save_old", "ff_cycle", "ff_restrict", "ff_forcing",
+    "ff_prolong", "ff_vcycle",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES = 1, 0, 1
@@ -59,6 +60,10 @@ def lib():
         L.ff_residual.argtypes = [vp, i, i, i, vp]
         L.ff_implicit.argtypes = [vp, i, i, i]
         L.ff_cycle.argtypes = [vp, i, i, vp]
+        L.ff_restrict.argtypes = [vp, i, i]
+        L.ff_forcing.argtypes = [vp, i, i, i]
+        L.ff_prolong.argtypes = [vp, i, i]
+        L.ff_vcycle.argtypes = [vp, i, i, i, vp]
         for fn in EXPORTS:
             if fn != "ff_last_error":
                 getattr(L, fn).restype = C.c_int
@@ -152,6 +157,26 @@ class Context:
     def cycle(self, block=1, ncycles=1):
         resi = np.zeros(7)
         self._chk(lib().ff_cycle(self.h, block, ncycles, resi.ctypes.data), "ff_cycle")
+        return resi
+
+    # ---- multigrid (TRANSF/TRANKE, FORCFU/ADDRES, INTERP/INTERT, the V-cycle) ----------
+    def register_levels(self, blocks, block=1):
+        """Register a block's multigrid levels, finest first."""
+        for m, b in enumerate(blocks):
+            self.register(b, level=m + 1, block=block)
+
+    def restrict(self, mfine, block=1):
+        self._chk(lib().ff_restrict(self.h, mfine, block), "ff_restrict")
+
+    def forcing(self, level, phase, block=1):
+        self._chk(lib().ff_forcing(self.h, level, block, phase), "ff_forcing")
+
+    def prolong(self, mcoarse, block=1):
+        self._chk(lib().ff_prolong(self.h, mcoarse, block), "ff_prolong")
+
+    def vcycle(self, nlev, ncycles=1, block=1):
+        resi = np.zeros(7)
+        self._chk(lib().ff_vcycle(self.h, block, nlev, ncycles, resi.ctypes.data), "ff_vcycle")
         return resi
 
     # ---- timing ------------------------------------------------------------------------

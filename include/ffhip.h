@@ -169,6 +169,24 @@ int  ff_implicit(ff_ctx *ctx, int level, int block, int m2);
 int  ff_update(ff_ctx *ctx, int level, int block);
 /* Copy the solved variables into their *OLD slots (the start-of-cycle values FIN reads). */
 int  ff_save_old(ff_ctx *ctx, int level, int block);
+/* Multigrid transfers between level M and M+1 of block N; the coarse level has half the cells
+ * of the finer one in each direction and carries the FAS forcing in its *P2H slots.
+ *   ff_restrict:  TRANSF multi.f:156 (+ TRANKE multi.f:434 for k-omega) from Mfine to Mfine+1:
+ *                 state, EPS2, VIST and the residual (into the forcing slots), OLD copies.
+ *   ff_forcing:   phase 0 FORCFU multi.f:1609 (P = P - D), phase 1 ADDRES multi.f:1686
+ *                 (D = P + D), level M.
+ *   ff_prolong:   INTERP multi.f:1763 (+ INTERT multi.f:2051) from Mcoarse to Mcoarse-1 with the
+ *                 limiters CYCLES applies after it (ns3c.f:2073-2118).
+ * ff_update on a level M >= 2 applies the coarse-level limiters (ns3c.f:1936-1985) and
+ * ff_residual with M2 >= 2 forms the k-omega velocity gradients itself, as RESIKE does. */
+int  ff_restrict(ff_ctx *ctx, int mfine, int block);
+int  ff_forcing(ff_ctx *ctx, int level, int block, int phase);
+int  ff_prolong(ff_ctx *ctx, int mcoarse, int block);
+/* ncycles device-resident CYCLES iterations (ns3c.f:1400-2137) of block N with nlev
+ * multigrid levels 1..nlev registered: the level-1 stages, the pre-smoothing-only FAS
+ * down-loop and the prolongation.  resi receives the RESL2 sums of the last level-1
+ * residual (the one evaluated after the update when nlev >= 2). */
+int  ff_vcycle(ff_ctx *ctx, int block, int nlev, int ncycles, double *resi);
 /* ncycles device-resident iterations of the finest-level sequence of CYCLES for block N:
  * save_old, STATIM, TURBDI, RESIDI, IMPSDI, FIN + limits.  Ghost cells are left as given
  * (the caller's boundary routines own them).  resi (may be NULL) receives the RESL2 sums

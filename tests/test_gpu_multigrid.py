@@ -1,0 +1,122 @@
+"""Multigrid on the HIP path (libffhip.so through its C-ABI) against the oracle: the
+transfers TRANSF/TRANKE, FORCFU/ADDRES and INTERP/INTERT one by one, the coarse-level
+stages, and whole V-cycles.  Laminar results are bit for bit; k-omega results are held to
+1e-12 of each field's largest magnitude where a tanh (FUN1KO, EPS2KO) lies upstream, since the
+device and host math libraries round tanh differently in the last bit."""
+import numpy as np
+import pytest
+
+from finflo_amd.cases import make_hierarchy
+from finflo_amd.layout import default_params
+from oracle.oracle_py import OracleBlock, OracleHierarchy
+from tests.parity import PRIM, RES, SST_PRIM, SST_RES, assert_fields_equal
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+FORC = ["ROP2H", "RMP2H", "RNP2H", "RWP2H", "EP2H"]
+
+
+def hierarchy(iturb, nlev=2, dims=(16, 8, 8), seed=71):
+    prm = default_params(iturb=iturb, cfll=3.0)
+    blocks = make_hierarchy(*dims, nlev, prm, distort=0.1, kind="all", amp=0.03, seed=seed,
+                            turb=(1.0, 150.0) if iturb else None)
+    return prm, blocks
+
+
+def context(prm, blocks):
+    from finflo_amd.ffhip import Context
+    ctx = Context(prm)
+    ctx.register_levels(blocks)
+    for m in range(len(blocks)):
+        ctx.upload(level=m + 1)
+    return ctx
+
+
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_transfers(hip_device, iturb):
+    print(f"case multigrid transfers iturb={iturb}", flush=True)
+    prm, blocks = hierarchy(iturb)
+    rng = np.random.default_rng(5)
+    for name in ("DRO", "DM", "DN", "DW", "DE", "DRK", "DEPS"):
+        blocks[0][name] = rng.uniform(-1, 1, blocks[0].ntot)
+    host = [b.copy() for b in blocks]
+    oh = OracleHierarchy(prm, host)
+    ctx = context(prm, blocks)
+    # restriction
+    oh.restrict(1); ctx.restrict(1); ctx.download(level=2)
+    names = ["TEMP", "U", "V", "W", "PDIFF", "EPS2", "VIST", "TOLD", "UOLD", "VOLD", "WOLD",
+             "POLD"] + FORC + (["RK", "REPS", "RKOLD", "EPSOLD", "RKP2H", "EPSP2H"] if iturb else [])
+    assert_fields_equal(blocks[1], host[1], names, interior=False, what="TRANSF")
+    # FORCFU / ADDRES
+    for name in ("DRO", "DM", "DN", "DW", "DE", "DRK", "DEPS"):
+        host[1][name] = rng.uniform(-1, 1, host[1].ntot)
+    blocks[1].F[:] = host[1].F
+    ctx.upload(level=2)
+    oc = oh.levels[1]
+    for phase in (0, 1):
+        oc.forcing(phase); ctx.forcing(2, phase); ctx.download(level=2)
+        assert_fields_equal(blocks[1], host[1], FORC + RES, interior=False, what=f"forcing {phase}")
+    # prolongation of a coarse correction
+    for name in ("TEMP", "U", "V", "W", "PDIFF") + (("RK", "REPS") if iturb else ()):
+        host[1][name] = host[1][name] * (1.0 + 0.01 * rng.uniform(-1, 1, host[1].ntot))
+    blocks[1].F[:] = host[1].F
+    ctx.upload(level=2)
+    oh.prolong(2); ctx.prolong(2); ctx.download(level=1)
+    prim = SST_PRIM if iturb else PRIM
+    assert_fields_equal(blocks[0], host[0], prim, interior=False, what="INTERP")
+    ctx.close()
+
+
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_coarse_level_stages(hip_device, iturb):
+    """Level 2: first-order residual (and RESIKE's own gradients), CFLL, coarse limiters."""
+    print(f"case coarse stages iturb={iturb}", flush=True)
+    prm, blocks = hierarchy(iturb)
+    c = blocks[1]
+    if iturb:
+        rng = np.random.default_rng(9)
+        c["EPS2"] = 1.0 + rng.uniform(5.0, 50.0, c.ntot)
+    a = c.copy()
+    oa = OracleBlock(prm, a, level=2)
+    ctx = context(prm, blocks)
+    ctx.upload(level=2)
+    oa.save_old(); ctx.save_old(level=2)
+    oa.state(); ctx.statim(level=2)
+    if iturb:
+        ctx.download(level=2)
+        a["VIST"] = (a["EPS2"] - 1.0) * a["VIS"]
+        c["VIST"] = a["VIST"]
+        ctx.upload(level=2, fields=["VIST"])
+    oa.residual(m2=2); ctx.residi(level=2, m2=2); ctx.download(level=2)
+    res = SST_RES if iturb else RES
+    assert_fields_equal(c, a, res, rtol=RTOL if iturb else 0.0, what="coarse residual")
+    c.F[:] = a.F
+    ctx.upload(level=2)
+    oa.implicit(); ctx.impsdi(level=2); ctx.download(level=2)
+    assert_fields_equal(c, a, ["DTL"] + res[:7], what="coarse implicit")
+    oa.update(); ctx.fin(level=2); ctx.download(level=2)
+    prim = SST_PRIM if iturb else PRIM
+    assert_fields_equal(c, a, prim, interior=False, what="coarse update")
+    ctx.close()
+
+
+@pytest.mark.parametrize("nlev", [2, 3])
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_vcycles(hip_device, iturb, nlev):
+    print(f"case V-cycles iturb={iturb} nlev={nlev}", flush=True)
+    prm, blocks = hierarchy(iturb, nlev=nlev, dims=(16, 8, 12), seed=73)
+    host = [b.copy() for b in blocks]
+    oh = OracleHierarchy(prm, host)
+    ctx = context(prm, blocks)
+    for _ in range(3):
+        r1 = oh.vcycle()
+    r2 = ctx.vcycle(nlev, ncycles=3)
+    for m in range(nlev):
+        ctx.download(level=m + 1)
+    prim = SST_PRIM if iturb else PRIM
+    for m in range(nlev):
+        assert_fields_equal(blocks[m], host[m], prim, interior=False,
+                            rtol=RTOL if iturb else 0.0, what=f"V-cycle level {m + 1}")
+    np.testing.assert_allclose(r2, r1, rtol=1e-10 if iturb else 1e-13)
+    ctx.close()
