@@ -1,8 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark of the FINFLO hot path on one MI355X.
 
-A step is one solver iteration of the finest-level CYCLES sequence over the block (STATIM,
-TURBDI, RESIDI, IMPSDI with LU-SGS, FIN + limiters), with every array resident in HBM.
+Default workload: the 192x96x96 flat-plate block with k-omega SST and a 3-level multigrid
+V-cycle (CYCLES with MGRID=3: the finest-level iteration STATIM, TURBDI, RESIDI, IMPSDI with
+LU-SGS, FIN + limiters, then the FAS down-loop over the coarse levels and the
+prolongation).  A step is one V-cycle; cell-updates count the finest level's cells once per
+V-cycle.  Every array is resident in HBM before the timed region starts.
+
 Prints one JSON line with the throughput, the roofline of the dominant kernel (algorithmic
 bytes per cell over its event-timed launch duration) and a CPU baseline timed on the host.
 """
@@ -17,80 +21,136 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from finflo_amd.cases import flat_plate_block   # noqa: E402
-from finflo_amd.layout import default_params    # noqa: E402
+from finflo_amd.cases import flat_plate_hierarchy   # noqa: E402
+from finflo_amd.layout import default_params        # noqa: E402
 
 IMAX, JMAX, KMAX = 192, 96, 96
-MACH, REYNOLDS = 0.2, 5.0e6
+MACH = 0.2
 HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # k-omega SST (ITURB=6), Re = 5e6 per plate length, MGRID = 3
+    "sst-mg": dict(iturb=6, nlev=3, reynolds=5.0e6,
+                   label="k-omega SST, 3-level multigrid V-cycle"),
+    # laminar ideal gas on the finest level only (MGRID = 1)
+    "laminar": dict(iturb=0, nlev=1, reynolds=5.0e6,
+                    label="laminar ideal gas, finest level only"),
+}
 
 # Algorithmic (compulsory, unique) FP64 bytes per cell of each kernel: every field the
 # kernel must read or write, once (neighbour values come from the same arrays).
 KERNEL_BYTES_PER_CELL = {
-    # flux state (RO U V W P C DRDP DRDH), viscous state (VIS EPS2 VIST CP CH TEMP), VOL,
-    # 3 owned faces x (A, nx, ny, nz), D out (5)
-    "k_residual_tile": 8 * (8 + 6 + 1 + 12 + 5),
-    "k_residual": 8 * (8 + 6 + 1 + 12 + 5),
-    # D in (5), state (U V W C RO VOL VIS VIST DTL), 3 owned faces x 4, D out (5)
-    "k_pred_tiles": 8 * (5 + 9 + 12 + 5),
-    "k_pred_plane": 8 * (5 + 9 + 12 + 5),
-    # the same plus CP (DRO/CP at the end of IMPS)
-    "k_corr_tiles": 8 * (5 + 10 + 12 + 5),
-    "k_corr_plane": 8 * (5 + 9 + 12 + 5),
-    # 3 faces x 4, VOL, 11 state fields, D in, D and DTL out
-    "k_time3_copr": 8 * (12 + 1 + 11 + 5 + 6),
-    # every cell incl. ghosts: 5 solved variables in, 12 derived fields out
-    "k_state": 8 * (5 + 12),
+    0: {
+        # flux state (RO U V W P C DRDP DRDH), viscous state (VIS EPS2 VIST CP CH TEMP), VOL,
+        # 3 owned faces x (A, nx, ny, nz), D out (5)
+        "k_residual_tile": 8 * (8 + 6 + 1 + 12 + 5),
+        "k_residual": 8 * (8 + 6 + 1 + 12 + 5),
+        # D in (5), state (U V W C RO VOL VIS VIST DTL), 3 owned faces x 4, D out (5)
+        "k_pred_tiles": 8 * (5 + 9 + 12 + 5),
+        "k_pred_plane": 8 * (5 + 9 + 12 + 5),
+        # the same plus CP (DRO/CP at the end of IMPS)
+        "k_corr_tiles": 8 * (5 + 10 + 12 + 5),
+        "k_corr_plane": 8 * (5 + 9 + 12 + 5),
+        "k_pred_km": 8 * (5 + 9 + 12 + 5),
+        "k_corr_km": 8 * (5 + 10 + 12 + 5),
+        # 3 faces x 4, VOL, 11 state fields, D in, D and DTL out
+        "k_time3_copr": 8 * (12 + 1 + 11 + 5 + 6),
+        # every cell incl. ghosts: 5 solved variables in, 12 derived fields out
+        "k_state": 8 * (5 + 12),
+    },
+    6: {
+        # flux state (RO U V W P C DRDP DRDH RK REPS), viscous (VIS EPS2 VIST CP CH TEMP),
+        # FUN1, VOL, 3 owned faces x 4, sources (PTUR, SEPS in); SRK, SEPS, D (7) out
+        "k_residual_kw": 8 * (10 + 6 + 1 + 1 + 12 + 2 + 9),
+        # D in (7), state (U V W C RO VOL VIS VIST DTL RK DRDP DRDH), 3 faces x 4, D out (7)
+        "k_pred_tiles": 8 * (7 + 12 + 12 + 7),
+        # the same plus CP
+        "k_corr_tiles": 8 * (7 + 13 + 12 + 7),
+        "k_pred_km": 8 * (7 + 12 + 12 + 7),
+        "k_corr_km": 8 * (7 + 13 + 12 + 7),
+    },
 }
-ALL_CELL_KERNELS = {"k_state"}          # launched over ghosts too
-# whole iteration, laminar ideal gas, every field touched once per phase (residual; backward
-# sweep; forward sweep + update): 88 doubles = 704 B per cell-update
-ITER_BYTES_PER_CELL = 704
+ALL_CELL_KERNELS = {"k_state", "k_save_old", "k_limits", "k_limits_mg", "k_forcing"}
+# whole finest-level iteration, every field touched once per phase (residual; backward
+# sweep; forward sweep + update): laminar 88 doubles = 704 B, k-omega 110 doubles = 880 B
+ITER_BYTES_PER_CELL = {0: 704, 6: 880}
+
+KERNELS = ("k_residual_kw", "k_residual_tile", "k_residual", "k_pred_tiles", "k_corr_tiles",
+           "k_pred_km", "k_corr_km", "k_pred_plane", "k_corr_plane", "k_time3_copr", "k_state", "k_turb_lam",
+           "k_turb_sst", "k_grad_zero", "k_derfun", "k_extend", "k_crosko", "k_kw_round",
+           "k_max_final", "k_fun1_prod", "k_limits", "k_limits_mg", "k_fin", "k_zero",
+           "k_save_old", "k_divcp", "k_resl2_part", "k_resl2_final", "k_restrict",
+           "k_restrict_ghosts", "k_forcing", "k_prolong")
 
 
-def workload_params():
-    prm = default_params()
+def workload_params(iturb, reynolds):
+    prm = default_params(iturb=iturb)
     c = np.sqrt(prm.gamma * prm.rgas * prm.frstem)
     prm.frsvel = MACH * c
     # Re = rho U L / mu with L = 1 m: scale Sutherland's viscosity (RMULTV) to reach it
     t = prm.frstem
     mu_suth = prm.visu0 * t * np.sqrt(t) / (t + prm.tsu0)
-    prm.rmult = prm.frsden * prm.frsvel / REYNOLDS / mu_suth
+    prm.rmult = prm.frsden * prm.frsvel / reynolds / mu_suth
     prm.frsvis = mu_suth * prm.rmult
+    prm.re = reynolds
     prm.cfl = prm.cfll = 5.0
     return prm
 
 
-def make_workload(seed=0):
-    prm = workload_params()
-    blk = flat_plate_block(IMAX, JMAX, KMAX, prm, first=1e-6, ratio=1.15, seed=seed, amp=0.005)
-    return prm, blk
+def freestream_turbulence(prm, tu=0.01, mut_ratio=1.0):
+    """(RK, REPS) = (rho k, rho omega) of a free stream with turbulence intensity `tu` and
+    eddy-to-molecular viscosity ratio `mut_ratio`."""
+    k = 1.5 * (tu * prm.frsvel) ** 2
+    rk = prm.frsden * k
+    omega = prm.frsden * k / (mut_ratio * prm.frsvis)
+    return rk, prm.frsden * omega
 
 
-def cpu_baseline(prm, blk, seconds=20.0):
-    """Time the reference's own Fortran (oracle/_ref) on the host, single thread, on a
-    bounded sample of the same workload: whole iterations of the full block."""
+def make_workload(name="sst-mg", seed=0):
+    w = WORKLOADS[name]
+    prm = workload_params(w["iturb"], w["reynolds"])
+    turb = freestream_turbulence(prm) if w["iturb"] == 6 else None
+    blocks = flat_plate_hierarchy(IMAX, JMAX, KMAX, w["nlev"], prm, first=1e-6, ratio=1.15,
+                                  seed=seed, amp=0.005, turb=turb)
+    return prm, blocks
+
+
+def run_steps(ctx, nlev, n):
+    return ctx.vcycle(nlev, ncycles=n) if nlev > 1 else ctx.cycle(ncycles=n)
+
+
+def cpu_baseline(prm, blocks, seconds=20.0):
+    """Time the CPU path on the host, single thread, on a bounded sample of the same
+    workload: whole V-cycles (or iterations) of the full block.  One level: the reference's
+    own Fortran (oracle/_ref) when built, else the C restatement; multigrid: the C
+    restatement (the reference's multigrid transfers are not compiled here)."""
     try:
-        from oracle.oracle_py import OracleBlock, RefBlock, have_ref
+        from oracle.oracle_py import OracleBlock, OracleHierarchy, RefBlock, have_ref
     except Exception as e:   # the oracle is optional for the bench
         return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "none",
                 "sample": f"oracle unavailable: {e}"}
-    kind = "reference" if have_ref() else "port"
-    cls = RefBlock if have_ref() else OracleBlock
-    b = blk.copy()
-    o = cls(prm, b)
-    ncell = blk.imax * blk.jmax * blk.kmax
+    bl = [b.copy() for b in blocks]
+    if len(bl) > 1:
+        kind = "port"
+        o = OracleHierarchy(prm, bl)
+        step, what = o.vcycle, "V-cycles"
+    else:
+        kind = "reference" if have_ref() else "port"
+        o = (RefBlock if have_ref() else OracleBlock)(prm, bl[0])
+        step, what = o.cycle, "iterations"
+    ncell = blocks[0].imax * blocks[0].jmax * blocks[0].kmax
     n = 0
     t0 = time.perf_counter()
     while True:
-        o.cycle()
+        step()
         n += 1
         el = time.perf_counter() - t0
         if el >= seconds or el / n * (n + 1) > 1.5 * seconds:
             break
+    b0 = blocks[0]
     return {"value": ncell * n / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{n} full iterations of the {blk.imax}x{blk.jmax}x{blk.kmax} workload "
-                      f"block ({el:.1f} s, one thread)"}
+            "sample": f"{n} full {what} of the {b0.imax}x{b0.jmax}x{b0.kmax} workload "
+                      f"({el:.1f} s, one thread)"}
 
 
 def main():
@@ -98,6 +158,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="sst-mg")
+    ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes"], default=None,
+                    help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -108,64 +171,89 @@ def main():
 
     from finflo_amd.ffhip import Context
 
-    prm, blk = make_workload()
+    w = WORKLOADS[args.workload]
+    nlev, iturb = w["nlev"], w["iturb"]
+    prm, blocks = make_workload(args.workload)
     ncell = IMAX * JMAX * KMAX
-    print(f"[bench] workload {IMAX}x{JMAX}x{KMAX} ({ncell} cells), building context", file=sys.stderr, flush=True)
+    print(f"[bench] workload {args.workload} {IMAX}x{JMAX}x{KMAX} ({ncell} cells, {nlev} "
+          f"level(s)), building context", file=sys.stderr, flush=True)
     ctx = Context(prm)
-    ctx.register(blk)
-    ctx.upload()
-    ctx.cycle(ncycles=args.warmup)
+    if args.lusgs:
+        ctx.lusgs_mode(args.lusgs)
+    ctx.register_levels(blocks)
+    for m in range(1, nlev + 1):
+        ctx.upload(level=m)
+    resi0 = run_steps(ctx, nlev, args.warmup)
     ctx.finish()
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
 
-    # timed region: K iterations, device-resident, bracketed by stream synchronisation
+    # timed region: K steps, device-resident, bracketed by stream synchronisation
     ctx.finish()
     t0 = time.perf_counter()
-    resi = ctx.cycle(ncycles=args.steps)
+    resi = run_steps(ctx, nlev, args.steps)
     ctx.finish()
     el = time.perf_counter() - t0
     ms_per_step = el * 1e3 / args.steps
     value = ncell * args.steps / el
     print(f"[bench] {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    if not np.all(np.isfinite(resi)):
+        print(f"[bench] non-finite residual norms {resi}", file=sys.stderr, flush=True)
+        sys.exit(1)
 
-    # the same K iterations again with a HIP event pair around every launch on the
-    # library's stream: per-kernel durations for the roofline (events add host overhead,
-    # so they stay out of the timed region above)
+    # the same K steps again with a HIP event pair around every launch on the library's
+    # stream: per-kernel durations for the roofline (events add host overhead, so they stay
+    # out of the timed region above)
     ctx.profile(True)
     ctx.reset_stats()
-    ctx.cycle(ncycles=args.steps)
+    run_steps(ctx, nlev, args.steps)
     ctx.finish()
     ctx.profile(False)
 
+    cells = {m: blocks[m - 1].imax * blocks[m - 1].jmax * blocks[m - 1].kmax
+             for m in range(1, nlev + 1)}
+    ntot = {m: blocks[m - 1].ntot for m in range(1, nlev + 1)}
     kstats = {}
-    for name in ("k_residual_tile", "k_residual", "k_pred_tiles", "k_corr_tiles",
-                 "k_pred_plane", "k_corr_plane", "k_time3_copr", "k_state", "k_limits",
-                 "k_fin", "k_zero", "k_save_old", "k_divcp", "k_turb_lam", "k_resl2_part",
-                 "k_resl2_final"):
+    for name in KERNELS:
         n, ms = ctx.kernel_stats(name)
-        if n:
-            kstats[name] = {"launches": n, "ms_total": ms, "ms_per_step": ms / args.steps}
-    # dominant kernel: the largest share of the step among those with a byte model
-    dom = max((k for k in kstats if k in KERNEL_BYTES_PER_CELL), key=lambda k: kstats[k]["ms_total"])
+        if not n:
+            continue
+        ent = {"launches": n, "ms_total": ms, "ms_per_step": ms / args.steps}
+        ncells = 0
+        for m in range(1, nlev + 1):
+            nm, msm = ctx.kernel_stats(f"{name}@{m}")
+            if nm:
+                ent[f"level{m}"] = {"launches": nm, "ms_total": msm}
+                ncells += nm * (ntot[m] if name in ALL_CELL_KERNELS else cells[m])
+        ent["cells_total"] = ncells
+        kstats[name] = ent
+    models = KERNEL_BYTES_PER_CELL[iturb]
+    # dominant kernel: the largest share of the step among those with a byte model; its
+    # algorithmic bytes per launch are the per-cell bytes times the mean cells per launch
+    # over all its launches (every level), which is what a kernel trace averages over
+    dom = max((k for k in kstats if k in models), key=lambda k: kstats[k]["ms_total"])
     ks = kstats[dom]
     avg_ms = ks["ms_total"] / ks["launches"]
-    cells_per_launch = blk.ntot if dom in ALL_CELL_KERNELS else ncell
-    achieved = KERNEL_BYTES_PER_CELL[dom] * cells_per_launch / (avg_ms * 1e-3) / 1e9
+    cells_per_launch = ks["cells_total"] / ks["launches"]
+    achieved = models[dom] * cells_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "avg_launch_ms": avg_ms,
-                "bytes_per_cell": KERNEL_BYTES_PER_CELL[dom], "cells_per_launch": cells_per_launch}
+                "avg_launch_ms": avg_ms, "bytes_per_cell": models[dom],
+                "cells_per_launch": cells_per_launch}
+    l1 = ks.get("level1")
+    if l1:
+        roofline["level1_GBps"] = models[dom] * cells[1] / (l1["ms_total"] / l1["launches"] * 1e-3) / 1e9
     sweep_ms = sum(kstats.get(k, {}).get("ms_total", 0.0)
-                   for k in ("k_pred_plane", "k_corr_plane", "k_pred_tiles", "k_corr_tiles")) / args.steps
-    iter_bw = ITER_BYTES_PER_CELL * ncell / (ms_per_step * 1e-3) / 1e9
+                   for k in ("k_pred_plane", "k_corr_plane", "k_pred_tiles", "k_corr_tiles",
+                             "k_pred_km", "k_corr_km")) / args.steps
+    iter_bw = ITER_BYTES_PER_CELL[iturb] * ncell / (ms_per_step * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(prm, blk, args.cpu_seconds)
+        cpu = cpu_baseline(prm, blocks, args.cpu_seconds)
 
     line = {
-        "metric": "cell-updates/s per MI355X (residual+LU-SGS)",
+        "metric": "cell-updates/s per MI355X (residual+LU-SGS) and time to 8-decade residual drop",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -177,13 +265,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"flat plate {IMAX}x{JMAX}x{KMAX} single block, laminar "
-                               f"ideal gas M={MACH} Re={REYNOLDS:g}, Roe+FLUXDO, LU-SGS "
-                               f"CFL=5, finest level only",
-                   "cells": ncell, "iteration": "STATIM+TURBDI+RESIDI+IMPSDI+FIN",
+        "config": {"workload": f"flat plate {IMAX}x{JMAX}x{KMAX} single block, {w['label']}, "
+                               f"M={MACH} Re={w['reynolds']:g}, Roe+FLUXDO, LU-SGS CFL=5; "
+                               f"a step is one {'V-cycle' if nlev > 1 else 'iteration'}",
+                   "cells": ncell, "levels": nlev,
                    "lusgs_sweep_ms_per_step": sweep_ms,
-                   "iteration_hbm_GBps_at_704B_per_cell": iter_bw,
-                   "density_resl2": float(np.sqrt(resi[0]) / ncell)},
+                   f"iteration_hbm_GBps_at_{ITER_BYTES_PER_CELL[iturb]}B_per_cell": iter_bw,
+                   "density_resl2_after_warmup": float(np.sqrt(resi0[0]) / ncell),
+                   "density_resl2": float(np.sqrt(resi[0]) / ncell),
+                   "time_to_8_decade_drop": None,
+                   "time_to_8_decade_drop_note": "not measured: the wall, far-field and "
+                   "symmetry ghost cells are the host's boundary routines, outside this path"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kstats,

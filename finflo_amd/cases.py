@@ -126,12 +126,13 @@ def fill_geometry(blk, nodes):
 
 
 def fill_state(blk, prm, seed=0, amp=0.01, mach_dir=(1.0, 0.0, 0.0), kind="density",
-               turb=None):
+               turb=None, turb_amp=None):
     """Free stream plus seeded perturbations on every cell (ghosts included).
 
     kind "density": T = T_inf / (1 + d), d ~ U(-amp, amp) at fixed p (the configs' density
     perturbation expressed in FINFLO's primitive variables); "velocity": u, v, w perturbed
-    by amp*U_inf; "all": every primitive perturbed (parity tests)."""
+    by amp*U_inf; "all": every primitive perturbed (parity tests).  `turb` = (RK, REPS) free
+    stream values, perturbed relatively by `turb_amp` (default `amp`)."""
     rng = np.random.default_rng(seed)
     n = blk.ntot
     d = np.asarray(mach_dir, float)
@@ -155,8 +156,9 @@ def fill_state(blk, prm, seed=0, amp=0.01, mach_dir=(1.0, 0.0, 0.0), kind="densi
         raise ValueError(kind)
     if turb is not None:
         rk, reps = turb
-        blk["RK"] = rk * (1.0 + amp * rng.uniform(-1, 1, n))
-        blk["REPS"] = reps * (1.0 + amp * rng.uniform(-1, 1, n))
+        ta = amp if turb_amp is None else turb_amp
+        blk["RK"] = rk * (1.0 + ta * rng.uniform(-1, 1, n))
+        blk["REPS"] = reps * (1.0 + ta * rng.uniform(-1, 1, n))
     blk["P"] = blk["PDIFF"] + prm.frspre
     # laminar eddy-viscosity ratio and eddy viscosity on every cell (the boundary routines
     # own the ghost values; TURBDI writes the interior)
@@ -181,7 +183,7 @@ def make_block(imax, jmax, kmax, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
 
 def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
                    distort=0.0, seed=0, amp=0.01, kind="density", turb=None,
-                   mach_dir=(1.0, 0.0, 0.0), nodes=None):
+                   mach_dir=(1.0, 0.0, 0.0), nodes=None, turb_amp=None):
     """A block and its nlev-1 coarser multigrid levels (dimensions divisible by 2^(nlev-1)).
     Every level has its own geometry; the state of the coarse levels is free stream (the
     restriction overwrites what the V-cycle uses)."""
@@ -199,7 +201,7 @@ def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jr
         b = Block(imax >> m, jmax >> m, kmax >> m)
         fill_geometry(b, nodes)
         fill_state(b, prm, seed=seed + m, amp=amp if m == 0 else 0.0, kind=kind,
-                   turb=turb, mach_dir=mach_dir)
+                   turb=turb, mach_dir=mach_dir, turb_amp=turb_amp if m == 0 else 0.0)
         blocks.append(b)
         if m < nlev - 1:
             nodes = coarsen_nodes(nodes)
@@ -224,3 +226,12 @@ def flat_plate_block(imax, jmax, kmax, prm, first=1e-6, ratio=1.15, length=1.0, 
     fill_geometry(blk, node_grid(xi, yj, zk))
     fill_state(blk, prm, seed=seed, amp=amp, kind="velocity")
     return blk
+
+
+def flat_plate_hierarchy(imax, jmax, kmax, nlev, prm, first=1e-6, ratio=1.15, length=1.0,
+                         width=None, seed=0, amp=0.005, turb=None):
+    """The flat-plate block with its nlev-1 coarser multigrid levels: velocity noise of
+    relative amplitude `amp` on the finest level, unperturbed (RK, REPS) = `turb`."""
+    nodes = flat_plate_nodes(imax, jmax, kmax, first, ratio, length, width)
+    return make_hierarchy(imax, jmax, kmax, nlev, prm, seed=seed, amp=amp, kind="velocity",
+                          turb=turb, nodes=nodes, turb_amp=0.0)

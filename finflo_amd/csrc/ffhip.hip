@@ -24,6 +24,7 @@
 // never receives the contribution of its I+1 neighbour.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1583,6 +1584,223 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
     }
 }
 
+// ------------------------------------------------------------------------------------
+// LU-SGS as a tile wavefront marching along K.  A wave owns a KI x KJ tile of (I, J)
+// columns (16 x 4: each field load of a step is four coalesced 128-byte rows) and walks
+// K; lane (ii, jj) runs ii + jj steps behind the tile's corner (fewer in the backward
+// sweep's mirrored order), so at every step its I- and J-upstream neighbours inside the
+// tile finished one step earlier and left their contributions in LDS.  The K-upstream
+// contribution stays in registers.  Contributions that cross a tile edge go through HBM
+// halo slots with the same empty-marker protocol as the J/K tiles above.
+// ------------------------------------------------------------------------------------
+constexpr int KI = 16, KJ = 4;
+static_assert(KI * KJ == TW, "one wave per tile");
+
+struct SweepK {
+    int nti, ntj;                   // tiles along I and J
+    unsigned long long *ticket;
+    unsigned int *err;
+    unsigned long long *hi, *hj;    // halo slots: I-faces (nti+1, KMAX, JMAX, HS), J-faces
+};
+
+__device__ __forceinline__ unsigned long long *hi_slot(const SweepK &s, const Blk &b, int f, int j,
+                                                       int k)
+{
+    return s.hi + HS * (((size_t)f * b.kmax + (k - 1)) * b.jmax + (j - 1));
+}
+
+__device__ __forceinline__ unsigned long long *hjk_slot(const SweepK &s, const Blk &b, int f, int i,
+                                                        int k)
+{
+    return s.hj + HS * (((size_t)f * b.kmax + (k - 1)) * b.imax + (i - 1));
+}
+
+__device__ __forceinline__ int tile_ticket_k(const SweepK &s)
+{
+    unsigned long long t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(s.ticket, 1ULL);
+    t = __shfl(t, 0);
+    return (int)(t % (unsigned long long)(s.nti * s.ntj));
+}
+
+// PREDGS / PRKEGS (imps4.f:345-598, 1462-1841) marching K downwards
+template <bool KE>
+__global__ void __launch_bounds__(TW) k_pred_km(Prm p, Blk b, SweepK sw)
+{
+    constexpr int NV = KE ? 7 : 5;
+    __shared__ double xi[2][NV][TW], xj[2][NV][TW];
+    const int lane = threadIdx.x, ii = lane % KI, jj = lane / KI;
+    const int t = tile_ticket_k(sw);
+    const int ti = sw.nti - 1 - t % sw.nti, tj = sw.ntj - 1 - t / sw.nti;
+    const int i = ti * KI + ii + 1, j = tj * KJ + jj + 1;
+    const bool col = i <= b.imax && j <= b.jmax;
+    const int lag = (KI - 1 - ii) + (KJ - 1 - jj);
+    const int nsteps = b.kmax + KI + KJ - 2;
+    double yk[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yk[r] = 0.;
+    SweepRaw cur;
+    if (col) sweep_load<false, KE>(b, cid(b, i, j, b.kmax), cur);
+    for (int s = 0; s < nsteps; s++) {
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.kmax) {
+            const int k = b.kmax - q;
+            const int n = cid(b, i, j, k);
+            const bool hiin = ii == KI - 1 && i < b.imax, hjin = jj == KJ - 1 && j < b.jmax;
+            unsigned long long hv[NV], hw[NV];
+            if (hiin) halo_issue<NV>(hi_slot(sw, b, ti + 1, j, k), hv);
+            if (hjin) halo_issue<NV>(hjk_slot(sw, b, tj + 1, i, k), hw);
+            SweepRaw nxt;
+            if (q + 1 < b.kmax) sweep_load<false, KE>(b, n - b.il, nxt);
+            SweepCell sc;
+            pred_prep<KE>(p, cur, i, j, k, sc);
+            const int rb = (s + 1) & 1;
+            double bi[NV], bj[NV];
+            if (i == b.imax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bi[r] = 0.;
+            } else if (!hiin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bi[r] = xi[rb][r][lane + 1];
+            } else {
+                halo_finish<NV>(hi_slot(sw, b, ti + 1, j, k), hv, sw.err, bi);
+            }
+            if (j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = 0.;
+            } else if (!hjin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + KI];
+            } else {
+                halo_finish<NV>(hjk_slot(sw, b, tj + 1, i, k), hw, sw.err, bj);
+            }
+            // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
+            // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
+            if (!KE && j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bi[r] = 0.;
+            }
+            double dn[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) {
+                double cd = -sc.dt1 * sc.wk * yk[r] + sc.vol * sc.d[r];
+                cd = cd - sc.dt1 * (sc.wi * bi[r] + sc.wj * bj[r]);
+                dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
+            }
+#pragma unroll
+            for (int r = 0; r < NV; r++) FLD(b, FF_DRO + r)[n] = dn[r];
+            double oi[NV], oj[NV];
+            face_xfer<KE>(dn, sc, sc.fi, sc.ci, oi);
+            face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
+            face_xfer<KE>(dn, sc, sc.fk, sc.ck, yk);
+            const int wb = s & 1;
+            if (ii > 0) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xi[wb][r][lane] = oi[r];
+            } else if (i > 1) {
+                halo_put<NV>(hi_slot(sw, b, ti, j, k), oi);
+            }
+            if (jj > 0) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
+            } else if (j > 1) {
+                halo_put<NV>(hjk_slot(sw, b, tj, i, k), oj);
+            }
+            if (q + 1 < b.kmax) cur = nxt;
+        }
+        wave_step_sync();
+    }
+}
+
+// DIAGGS + CORRGS / DIKEGS + COKEGS (imps4.f:604-664, 76-339, 1841-1933, 1078-1456)
+// marching K upwards, with the end of IMPS on the final increment (imps3.f:483, 497-500)
+template <bool KE>
+__global__ void __launch_bounds__(TW) k_corr_km(Prm p, Blk b, SweepK sw)
+{
+    constexpr int NV = KE ? 7 : 5;
+    __shared__ double xi[2][NV][TW], xj[2][NV][TW];
+    const int lane = threadIdx.x, ii = lane % KI, jj = lane / KI;
+    const int t = tile_ticket_k(sw);
+    const int ti = t % sw.nti, tj = t / sw.nti;
+    const int i = ti * KI + ii + 1, j = tj * KJ + jj + 1;
+    const bool col = i <= b.imax && j <= b.jmax;
+    const int lag = ii + jj;
+    const int nsteps = b.kmax + KI + KJ - 2;
+    double yk[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yk[r] = 0.;
+    SweepRaw cur;
+    if (col) sweep_load<true, KE>(b, cid(b, i, j, 1), cur);
+    for (int s = 0; s < nsteps; s++) {
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.kmax) {
+            const int k = q + 1;
+            const int n = cid(b, i, j, k);
+            const bool hiin = ii == 0 && i > 1, hjin = jj == 0 && j > 1;
+            unsigned long long hv[NV], hw[NV];
+            if (hiin) halo_issue<NV>(hi_slot(sw, b, ti, j, k), hv);
+            if (hjin) halo_issue<NV>(hjk_slot(sw, b, tj, i, k), hw);
+            SweepRaw nxt;
+            if (q + 1 < b.kmax) sweep_load<true, KE>(b, n + b.il, nxt);
+            SweepCell sc;
+            corr_prep<KE>(p, cur, sc);
+            const int rb = (s + 1) & 1;
+            double bi[NV], bj[NV];
+            if (i == 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bi[r] = 0.;
+            } else if (!hiin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bi[r] = xi[rb][r][lane - 1];
+            } else {
+                halo_finish<NV>(hi_slot(sw, b, ti, j, k), hv, sw.err, bi);
+            }
+            if (j == 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = 0.;
+            } else if (!hjin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - KI];
+            } else {
+                halo_finish<NV>(hjk_slot(sw, b, tj, i, k), hw, sw.err, bj);
+            }
+            double dn[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) {
+                double cd = sc.dt1 * sc.wk * yk[r] + sc.vol * sc.d[r];
+                cd = cd + sc.dt1 * (sc.wi * bi[r] + sc.wj * bj[r]);
+                dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
+            }
+            FLD(b, FF_DRO)[n] = dn[0] / sc.cp;
+#pragma unroll
+            for (int r = 1; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
+            if (KE) {
+                FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * sc.rho;
+                FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * sc.rho;
+            }
+            double oi[NV], oj[NV];
+            face_xfer<KE>(dn, sc, sc.fi, sc.ci, oi);
+            face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
+            face_xfer<KE>(dn, sc, sc.fk, sc.ck, yk);
+            const int wb = s & 1;
+            if (ii < KI - 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xi[wb][r][lane] = oi[r];
+            } else if (i < b.imax) {
+                halo_put<NV>(hi_slot(sw, b, ti + 1, j, k), oi);
+            }
+            if (jj < KJ - 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
+            } else if (j < b.jmax) {
+                halo_put<NV>(hjk_slot(sw, b, tj + 1, i, k), oj);
+            }
+            if (q + 1 < b.kmax) cur = nxt;
+        }
+        wave_step_sync();
+    }
+}
+
 // Fill 64-bit words (halo slots to HALO_EMPTY at set-up and after a failed sweep)
 __global__ void __launch_bounds__(256) k_fill_u64(unsigned long long *x, size_t n, unsigned long long v)
 {
@@ -1701,14 +1919,17 @@ struct BlockRec {
     bool mirror = false;
     double *arena = nullptr;   // device arena of a mirrored block
     Sweep sw{};                // LU-SGS tile-wavefront state (halo slots, ticket, error word)
+    SweepK swk{};              // the same for the K-marching (I, J) tiles (shares the slots)
     void *sw_mem = nullptr;
-    size_t nhj = 0, nhk = 0;   // halo slot counts
+    size_t nhalo = 0;          // halo slot words
     double *kw_scr = nullptr;  // k-omega: per-workgroup maxima of SEPS and their maximum
     int kw_parts = 0;
+    int level = 0;             // multigrid level M the slab was registered as
 };
 
 struct PendingEv {
     const char *name;
+    int level;
     hipEvent_t a, b;
 };
 
@@ -1727,7 +1948,9 @@ struct ff_ctx {
     int lusgs_mode = FF_LUSGS_TILES;
     int residual_mode = FF_RESIDUAL_TILES;
     std::vector<PendingEv> pending;
+    // launches and event-timed milliseconds per kernel name, and per "name@level"
     std::map<std::string, std::pair<long long, double>> stats;
+    int lev = 0;                  // level of the stage being launched (statistics only)
 };
 
 static long long key(int level, int block) { return ((long long)level << 32) | (unsigned)block; }
@@ -1780,7 +2003,7 @@ static int launch(ff_ctx *c, const char *name, K kern, dim3 g, dim3 t, A... args
     HIPCHK(c, hipGetLastError());
     if (c->profile) {
         HIPCHK(c, hipEventRecord(eb, c->stream));
-        c->pending.push_back({name, ea, eb});
+        c->pending.push_back({name, c->lev, ea, eb});
     }
     return FF_OK;
 }
@@ -1794,6 +2017,9 @@ static int drain_events(ff_ctx *c)
         auto &s = c->stats[e.name];
         s.first += 1;
         s.second += ms;
+        auto &sl = c->stats[std::string(e.name) + "@" + std::to_string(e.level)];
+        sl.first += 1;
+        sl.second += ms;
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
@@ -1834,25 +2060,36 @@ static int sweep_alloc(ff_ctx *c, BlockRec &r)
 {
     const Blk &b = r.dev;
     Sweep &sw = r.sw;
+    SweepK &sk = r.swk;
     sw.ntj = (b.jmax + TJ - 1) / TJ;
     sw.ntk = (b.kmax + TK - 1) / TK;
-    r.nhj = (size_t)HS * (sw.ntj + 1) * b.kmax * b.imax;
-    r.nhk = (size_t)HS * (sw.ntk + 1) * b.jmax * b.imax;
-    size_t words = r.nhj + r.nhk + 2;
-    if (hipMalloc(&r.sw_mem, words * sizeof(unsigned long long)) != hipSuccess)
+    sk.nti = (b.imax + KI - 1) / KI;
+    sk.ntj = (b.jmax + KJ - 1) / KJ;
+    // both tile forms leave every slot empty after a sweep, so they share the slots
+    const size_t nhj = (size_t)HS * (sw.ntj + 1) * b.kmax * b.imax;
+    const size_t nhk = (size_t)HS * (sw.ntk + 1) * b.jmax * b.imax;
+    const size_t nhi2 = (size_t)HS * (sk.nti + 1) * b.kmax * b.jmax;
+    const size_t nhj2 = (size_t)HS * (sk.ntj + 1) * b.kmax * b.imax;
+    r.nhalo = std::max(nhj + nhk, nhi2 + nhj2);
+    const size_t head = 4;          // ticket, error word, ticket of the K-marching form, pad
+    if (hipMalloc(&r.sw_mem, (r.nhalo + head) * sizeof(unsigned long long)) != hipSuccess)
         return fail(c, FF_E_NOMEM, "LU-SGS halo slots%s");
     unsigned long long *m = (unsigned long long *)r.sw_mem;
     sw.ticket = m;
     sw.err = (unsigned int *)(m + 1);
-    sw.hj = m + 2;
-    sw.hk = sw.hj + r.nhj;
-    HIPCHK(c, hipMemsetAsync(r.sw_mem, 0, 2 * sizeof(unsigned long long), c->stream));
+    sw.hj = m + head;
+    sw.hk = sw.hj + nhj;
+    sk.ticket = m + 2;
+    sk.err = sw.err;
+    sk.hi = m + head;
+    sk.hj = sk.hi + nhi2;
+    HIPCHK(c, hipMemsetAsync(r.sw_mem, 0, head * sizeof(unsigned long long), c->stream));
     if (c->prm.iturb == 6) {
         r.kw_parts = (b.ntot + 255) / 256;
         if (hipMalloc(&r.kw_scr, sizeof(double) * (size_t)(r.kw_parts + 1)) != hipSuccess)
             return fail(c, FF_E_NOMEM, "k-omega reduction scratch%s");
     }
-    size_t nh = r.nhj + r.nhk;
+    size_t nh = r.nhalo;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream,
                        sw.hj, nh, HALO_EMPTY);
     HIPCHK(c, hipGetLastError());
@@ -1877,7 +2114,7 @@ static int sweep_check(ff_ctx *c, BlockRec &r)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (*h == 0) return FF_OK;
     *h = 0;
-    size_t nh = r.nhj + r.nhk;
+    size_t nh = r.nhalo;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, c->stream,
                        r.sw.hj, nh, HALO_EMPTY);
     HIPCHK(c, hipMemsetAsync(r.sw.err, 0, sizeof(unsigned int), c->stream));
@@ -1982,7 +2219,7 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
 {
     if (!c) return FF_E_ARG;
     if (option == FF_OPT_LUSGS) {
-        if (value != FF_LUSGS_TILES && value != FF_LUSGS_PLANES)
+        if (value != FF_LUSGS_TILES && value != FF_LUSGS_PLANES && value != FF_LUSGS_KMARCH)
             return fail(c, FF_E_ARG, "FF_OPT_LUSGS: unknown value%s");
         c->lusgs_mode = value;
         return FF_OK;
@@ -2034,6 +2271,7 @@ int ff_block_register(ff_ctx *c, int level, int block, const ff_block *host)
     BlockRec r;
     r.host = *host;
     r.mirror = true;
+    r.level = level;
     Blk b = make_blk(host);
     size_t bytes = sizeof(double) * (size_t)b.ntot * FF_NFIELDS;
     if (hipMalloc(&r.arena, bytes) != hipSuccess) return fail(c, FF_E_NOMEM, "device arena%s");
@@ -2062,6 +2300,7 @@ int ff_block_register_device(ff_ctx *c, int level, int block, const ff_block *de
     if ((rc = check_fields(c, dev, "device field %s is NULL"))) return rc;
     BlockRec r;
     r.mirror = false;
+    r.level = level;
     r.dev = make_blk(dev);
     if ((rc = sweep_alloc(c, r))) return rc;
     BlockRec *old = find(c, level, block);
@@ -2118,6 +2357,7 @@ int ff_state(ff_ctx *c, int level, int block)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    c->lev = level;
     return launch(c, "k_state", k_state, all_grid(r->dev), dim3(256), make_prm(&c->prm, level), r->dev);
 }
 
@@ -2126,6 +2366,7 @@ int ff_turb(ff_ctx *c, int level, int block)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    c->lev = level;
     if (c->prm.iturb == 6) {
         const Blk &b = r->dev;
         dim3 g((b.imax + 2 + 255) / 256, b.jmax + 2, b.kmax + 2);
@@ -2137,6 +2378,7 @@ int ff_turb(ff_ctx *c, int level, int block)
 static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi)
 {
     const Blk &b = r->dev;
+    c->lev = level;
     int rc;
     if (c->prm.iturb == 6 && m2 >= 2) {
         // RESIKE's own DERFUN (res3c.f:3749-3769), EXTEND on coarse levels
@@ -2205,9 +2447,20 @@ int ff_residual(ff_ctx *c, int level, int block, int m2, double *resi)
 static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
 {
     const Blk &b = r->dev;
+    c->lev = level;
     Prm kp = make_prm(&c->prm, level);
     int rc = launch(c, "k_time3_copr", k_time3_copr, cells_grid(b, 128), dim3(128), kp, b);
     if (rc) return rc;
+    if (c->lusgs_mode == FF_LUSGS_KMARCH) {
+        // DRO/CP is applied inside the forward sweep
+        dim3 g(r->swk.nti * r->swk.ntj);
+        if (c->prm.iturb == 6) {
+            if ((rc = launch(c, "k_pred_km", k_pred_km<true>, g, dim3(TW), kp, b, r->swk))) return rc;
+            return launch(c, "k_corr_km", k_corr_km<true>, g, dim3(TW), kp, b, r->swk);
+        }
+        if ((rc = launch(c, "k_pred_km", k_pred_km<false>, g, dim3(TW), kp, b, r->swk))) return rc;
+        return launch(c, "k_corr_km", k_corr_km<false>, g, dim3(TW), kp, b, r->swk);
+    }
     if (c->lusgs_mode == FF_LUSGS_TILES) {
         // DRO/CP is applied inside the forward sweep
         dim3 g(r->sw.ntj * r->sw.ntk);
@@ -2257,12 +2510,13 @@ int ff_implicit(ff_ctx *c, int level, int block, int m2)
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
     int rc = implicit_impl(c, r, level, true);
     if (rc) return rc;
-    return c->lusgs_mode == FF_LUSGS_TILES ? sweep_check(c, *r) : FF_OK;
+    return c->lusgs_mode != FF_LUSGS_PLANES ? sweep_check(c, *r) : FF_OK;
 }
 
 static int update_impl(ff_ctx *c, BlockRec *r, int level)
 {
     const Blk &b = r->dev;
+    c->lev = level;
     const bool kw = c->prm.iturb == 6;
     int rc = kw ? launch(c, "k_fin", k_fin_kw, cells_grid(b, 256), dim3(256), b)
                 : launch(c, "k_fin", k_fin, cells_grid(b, 256), dim3(256), b);
@@ -2279,6 +2533,7 @@ static int restrict_impl(ff_ctx *c, BlockRec *fr, BlockRec *cr)
     const Blk &f = fr->dev, &cb = cr->dev;
     const bool kw = c->prm.iturb == 6;
     int rc;
+    c->lev = cr->level;
     dim3 g((cb.imax + 2 + 255) / 256, cb.jmax + 2, cb.kmax + 2);
     rc = kw ? launch(c, "k_restrict", k_restrict<true>, g, dim3(256), f, cb)
             : launch(c, "k_restrict", k_restrict<false>, g, dim3(256), f, cb);
@@ -2296,6 +2551,7 @@ static int restrict_impl(ff_ctx *c, BlockRec *fr, BlockRec *cr)
 
 static int forcing_impl(ff_ctx *c, BlockRec *r, int phase)
 {
+    c->lev = r->level;
     return launch(c, "k_forcing", k_forcing, all_grid(r->dev), dim3(256), r->dev,
                   c->prm.iturb == 6 ? 7 : 5, phase);
 }
@@ -2306,6 +2562,7 @@ static int prolong_impl(ff_ctx *c, BlockRec *cr, BlockRec *fr, int level_fine)
     const Blk &f = fr->dev, &cb = cr->dev;
     const bool kw = c->prm.iturb == 6;
     Prm kp = make_prm(&c->prm, level_fine);
+    c->lev = level_fine;
     int rc = kw ? launch(c, "k_prolong", k_prolong<true>, cells_grid(f, 256), dim3(256), kp, cb, f)
                 : launch(c, "k_prolong", k_prolong<false>, cells_grid(f, 256), dim3(256), kp, cb, f);
     if (rc) return rc;
@@ -2334,6 +2591,7 @@ int ff_save_old(ff_ctx *c, int level, int block)
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    c->lev = level;
     return launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev);
 }
 
@@ -2345,6 +2603,7 @@ int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
     int rc;
     for (int it = 0; it < ncycles; it++) {
         bool last = it == ncycles - 1;
+        c->lev = 1;
         if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev))) return rc;
         if ((rc = ff_state(c, 1, block))) return rc;
         if ((rc = ff_turb(c, 1, block))) return rc;
@@ -2352,7 +2611,7 @@ int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
         if ((rc = implicit_impl(c, r, 1, true))) return rc;
         if ((rc = update_impl(c, r, 1))) return rc;
     }
-    return c->lusgs_mode == FF_LUSGS_TILES ? sweep_check(c, *r) : FF_OK;
+    return c->lusgs_mode != FF_LUSGS_PLANES ? sweep_check(c, *r) : FF_OK;
 }
 
 int ff_restrict(ff_ctx *c, int mfine, int block)
@@ -2398,6 +2657,7 @@ int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
         double *rs = it == ncycles - 1 ? resi : nullptr;
         BlockRec *r1 = lv[0];
         // level 1: CYCLES ns3c.f:1450-1596
+        c->lev = 1;
         if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev))) return rc;
         if ((rc = ff_state(c, 1, block))) return rc;
         if ((rc = ff_turb(c, 1, block))) return rc;
@@ -2422,7 +2682,7 @@ int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
         for (int m = nlev; m >= 2; m--)
             if ((rc = prolong_impl(c, lv[m - 1], lv[m - 2], m - 1))) return rc;
     }
-    if (c->lusgs_mode == FF_LUSGS_TILES)
+    if (c->lusgs_mode != FF_LUSGS_PLANES)
         for (int m = 0; m < nlev; m++)
             if ((rc = sweep_check(c, *lv[m]))) return rc;
     return FF_OK;

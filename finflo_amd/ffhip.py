@@ -24,7 +24,7 @@ EXPORTS = [
     "ff_prolong", "ff_vcycle",
 ]
 
-FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES = 1, 0, 1
+FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH = 1, 0, 1, 2
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 
 _lib = None
@@ -102,8 +102,10 @@ class Context:
         self._chk(lib().ff_ctx_set_option(self.h, option, value), "ff_ctx_set_option")
 
     def lusgs_mode(self, mode):
-        """LU-SGS sweep form: "tiles" (persistent tile wavefront) or "planes"."""
-        self.set_option(FF_OPT_LUSGS, {"tiles": FF_LUSGS_TILES, "planes": FF_LUSGS_PLANES}[mode])
+        """LU-SGS sweep form: "tiles" ((J,K) tiles marching I), "kmarch" ((I,J) tiles
+        marching K) or "planes" (one launch per hyperplane)."""
+        self.set_option(FF_OPT_LUSGS, {"tiles": FF_LUSGS_TILES, "planes": FF_LUSGS_PLANES,
+                                       "kmarch": FF_LUSGS_KMARCH}[mode])
 
     def residual_mode(self, mode):
         """Residual form: "tiles" (each face once) or "cells" (both faces per cell)."""

@@ -118,11 +118,13 @@ int  ff_ctx_set_stream(ff_ctx *ctx, void *stream);
 const char *ff_last_error(const ff_ctx *ctx);
 /* Context options (no counterpart in the reference, which has one form of each stage):
  *   FF_OPT_LUSGS: FF_LUSGS_TILES (default) runs each LU-SGS sweep as one persistent
- *   tile-wavefront launch; FF_LUSGS_PLANES launches one kernel per I+J+K hyperplane.
- *   Both give the same results bit for bit. */
+ *   launch of (J,K) tiles marching along I; FF_LUSGS_KMARCH as one persistent launch of
+ *   (I,J) tiles marching along K; FF_LUSGS_PLANES launches one kernel per I+J+K
+ *   hyperplane (laminar only).  All give the same results bit for bit. */
 #define FF_OPT_LUSGS     1
 #define FF_LUSGS_TILES   0
 #define FF_LUSGS_PLANES  1
+#define FF_LUSGS_KMARCH  2
 /*   FF_OPT_RESIDUAL: FF_RESIDUAL_TILES (default) forms every face flux once in (I,J) tiles
  *   marching along K; FF_RESIDUAL_CELLS forms both faces of every direction per cell.
  *   Both give the same results bit for bit. */
@@ -132,7 +134,8 @@ const char *ff_last_error(const ff_ctx *ctx);
 int  ff_ctx_set_option(ff_ctx *ctx, int option, int value);
 /* Per-kernel event timing (0 off, 1 on); results via ff_kernel_stats. */
 int  ff_ctx_profile(ff_ctx *ctx, int enable);
-/* Copy the accumulated timing of kernel `name`: launches and total milliseconds. */
+/* Copy the accumulated timing of kernel `name`: launches and total milliseconds.  `name`
+ * is a kernel name (all levels) or "name@M" for the launches on multigrid level M. */
 int  ff_kernel_stats(const ff_ctx *ctx, const char *name, long long *launches, double *ms);
 int  ff_kernel_stats_reset(ff_ctx *ctx);
 

@@ -101,14 +101,16 @@ def test_coarse_level_stages(hip_device, iturb):
     ctx.close()
 
 
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
 @pytest.mark.parametrize("nlev", [2, 3])
 @pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
-def test_vcycles(hip_device, iturb, nlev):
-    print(f"case V-cycles iturb={iturb} nlev={nlev}", flush=True)
+def test_vcycles(hip_device, iturb, nlev, lusgs):
+    print(f"case V-cycles iturb={iturb} nlev={nlev} LU-SGS {lusgs}", flush=True)
     prm, blocks = hierarchy(iturb, nlev=nlev, dims=(16, 8, 12), seed=73)
     host = [b.copy() for b in blocks]
     oh = OracleHierarchy(prm, host)
     ctx = context(prm, blocks)
+    ctx.lusgs_mode(lusgs)
     for _ in range(3):
         r1 = oh.vcycle()
     r2 = ctx.vcycle(nlev, ncycles=3)

@@ -46,7 +46,7 @@ def test_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "planes"])
+@pytest.mark.parametrize("lusgs", ["tiles", "planes", "kmarch"])
 @pytest.mark.parametrize("name,dims,pover,ckw", CASES[:4], ids=[c[0] for c in CASES[:4]])
 def test_cycles(hip_device, name, dims, pover, ckw, lusgs):
     print(f"case {name} dims {dims} 5 cycles, LU-SGS {lusgs}", flush=True)
@@ -63,19 +63,20 @@ def test_cycles(hip_device, name, dims, pover, ckw, lusgs):
     ctx.close()
 
 
-# blocks spanning several 8x8 (J,K) tiles of the LU-SGS wavefront, with ragged last tiles
-# and a J-extent that is a multiple of the tile edge
+# blocks spanning several 8x8 (J,K) tiles and 16x4 (I,J) tiles of the LU-SGS wavefronts,
+# with ragged last tiles and extents that are multiples of the tile edges
 MULTI_TILE = [(40, 24, 20), (20, 19, 13), (9, 17, 26), (33, 8, 16)]
 
 
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
 @pytest.mark.parametrize("dims", MULTI_TILE, ids=["x".join(map(str, d)) for d in MULTI_TILE])
-def test_larger_block_cycle(hip_device, dims):
+def test_larger_block_cycle(hip_device, dims, lusgs):
     """Blocks with more cells than one workgroup per row/plane/tile."""
-    print(f"case large {dims}", flush=True)
+    print(f"case large {dims} LU-SGS {lusgs}", flush=True)
     prm, blk = make_case(dims, {}, dict(kind="all", amp=0.03, distort=0.1, seed=21))
     a, g = blk.copy(), blk.copy()
     oa = OracleBlock(prm, a)
-    ctx = _ctx(prm, g)
+    ctx = _ctx(prm, g, lusgs)
     for _ in range(2):
         oa.cycle()
     ctx.cycle(ncycles=2)
@@ -85,11 +86,11 @@ def test_larger_block_cycle(hip_device, dims):
 
 
 def test_implicit_stage_tiles_vs_planes(hip_device):
-    """The two LU-SGS forms give the same increments bit for bit, stage by stage."""
-    print("case tiles-vs-planes 26x21x18", flush=True)
+    """The three LU-SGS forms give the same increments bit for bit, stage by stage."""
+    print("case tiles-vs-planes-vs-kmarch 26x21x18", flush=True)
     prm, blk = make_case((26, 21, 18), {}, dict(kind="all", amp=0.04, distort=0.1, seed=31))
     out = {}
-    for mode in ("tiles", "planes"):
+    for mode in ("tiles", "planes", "kmarch"):
         g = blk.copy()
         ctx = _ctx(prm, g, mode)
         ctx.save_old(); ctx.statim(); ctx.turbdi(); ctx.residi(); ctx.impsdi()
@@ -97,6 +98,7 @@ def test_implicit_stage_tiles_vs_planes(hip_device):
         out[mode] = g
         ctx.close()
     assert_fields_equal(out["tiles"], out["planes"], ["DTL"] + RES, what="tiles vs planes")
+    assert_fields_equal(out["kmarch"], out["planes"], ["DTL"] + RES, what="kmarch vs planes")
 
 
 @pytest.mark.parametrize("dims", [(70, 9, 17), (5, 3, 2), (130, 6, 9)], ids=["70x9x17", "5x3x2", "130x6x9"])
@@ -157,13 +159,14 @@ def test_sst_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
 @pytest.mark.parametrize("name,dims,pover,ckw", SST_CASES[:2], ids=[c[0] for c in SST_CASES[:2]])
-def test_sst_cycles(hip_device, name, dims, pover, ckw):
-    print(f"case {name} dims {dims} 5 SST cycles", flush=True)
+def test_sst_cycles(hip_device, name, dims, pover, ckw, lusgs):
+    print(f"case {name} dims {dims} 5 SST cycles, LU-SGS {lusgs}", flush=True)
     prm, blk = make_case(dims, pover, ckw)
     a, g = blk.copy(), blk.copy()
     oa = OracleBlock(prm, a)
-    ctx = _ctx(prm, g)
+    ctx = _ctx(prm, g, lusgs)
     for _ in range(5):
         r1 = oa.cycle()
     r2 = ctx.cycle(ncycles=5)
@@ -173,15 +176,16 @@ def test_sst_cycles(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
 @pytest.mark.parametrize("dims", [(40, 24, 20), (21, 19, 13)], ids=["40x24x20", "21x19x13"])
-def test_sst_multi_tile(hip_device, dims):
+def test_sst_multi_tile(hip_device, dims, lusgs):
     """SST blocks spanning several LU-SGS tiles and residual tiles, with ragged last tiles."""
-    print(f"case SST large {dims}", flush=True)
+    print(f"case SST large {dims} LU-SGS {lusgs}", flush=True)
     prm, blk = make_case(dims, dict(iturb=6), dict(kind="all", amp=0.03, distort=0.1, seed=43,
                                                    turb=(1.0, 200.0)))
     a, g = blk.copy(), blk.copy()
     oa = OracleBlock(prm, a)
-    ctx = _ctx(prm, g)
+    ctx = _ctx(prm, g, lusgs)
     for _ in range(2):
         oa.cycle()
     ctx.cycle(ncycles=2)
