@@ -1334,8 +1334,10 @@ __device__ __forceinline__ void face_xfer(const double *dn, const SweepCell &s, 
         face_contrib_b(dn, s.ro, s.roc, s.pc2, s.proc, f, cf.x1, cf.x2, cf.x5, out);
 }
 
+// PREDGS's recurrence-independent coefficients of a cell (imps4.f:396-466): the
+// split-eigenvalue coefficients of its three lower faces, SCINV/SCINP, THETA*DT
 template <bool KE>
-__device__ __forceinline__ void pred_prep(const Prm &p, const SweepRaw &r, int i, int j, int k,
+__device__ __forceinline__ void pred_coef(const Prm &p, const SweepRaw &r, int i, int j, int k,
                                           SweepCell &s)
 {
     const double cdiff = 2.;
@@ -1352,19 +1354,35 @@ __device__ __forceinline__ void pred_prep(const Prm &p, const SweepRaw &r, int i
     s.scinp = 1. / (r.vol + s.dt1 * (s.ci.rmax * r.a1 + s.cj.rmax * r.a2 + s.ck.rmax * r.a3) + r.vol * 0.);
 #pragma unroll
     for (int q = 0; q < (KE ? 7 : 5); q++) s.d[q] = r.d[q];
-    cell_scalars<KE>(s, r);
     s.wi = r.a1u; s.wj = r.a2u; s.wk = r.a3u;
+}
+
+// PREDGS's face bases (the lower faces) and SYI/SYM1I cell scalars
+template <bool KE>
+__device__ __forceinline__ void pred_basis(const SweepRaw &r, SweepCell &s)
+{
+    cell_scalars<KE>(s, r);
     s.fi = face_basis(r.n1x, r.n1y, r.n1z);
     s.fj = face_basis(r.n2x, r.n2y, r.n2z);
     s.fk = face_basis(r.n3x, r.n3y, r.n3z);
 }
 
 template <bool KE>
-__device__ __forceinline__ void corr_prep(const Prm &p, const SweepRaw &r, SweepCell &s)
+__device__ __forceinline__ void pred_prep(const Prm &p, const SweepRaw &r, int i, int j, int k,
+                                          SweepCell &s)
+{
+    pred_coef<KE>(p, r, i, j, k, s);
+    pred_basis<KE>(r, s);
+}
+
+// DIAGGS + CORRGS's recurrence-independent coefficients of a cell: the diagonal factor
+// applied to the increments (imps4.f:621-659 / DIKEGS imps4.f:1877-1912), the coefficients
+// of its three upper faces, SCINV/SCINP, THETA*DT
+template <bool KE>
+__device__ __forceinline__ void corr_coef(const Prm &p, const SweepRaw &r, SweepCell &s)
 {
     constexpr int NV = KE ? 7 : 5;
     const double cdiff = 2.;
-    // DIAGGS imps4.f:621-659 / DIKEGS imps4.f:1877-1912
     double ucon;
     ucon = r.u1x * r.u + r.u1y * r.v + r.u1z * r.w - 0.;
     double r1n = 1.00 * (fabs(ucon) + r.c);
@@ -1394,13 +1412,26 @@ __device__ __forceinline__ void corr_prep(const Prm &p, const SweepRaw &r, Sweep
     s.vol = r.vol;
     s.scinv = 1. / (r.vol + s.dt1 * (s.ci.xa * r.a1u + s.cj.xa * r.a2u + s.ck.xa * r.a3u) + r.vol * 0.);
     s.scinp = 1. / (r.vol + s.dt1 * (s.ci.rmax * r.a1 + s.cj.rmax * r.a2 + s.ck.rmax * r.a3) + r.vol * 0.);
-    cell_scalars<KE>(s, r);
     s.wi = r.a1; s.wj = r.a2; s.wk = r.a3;
+}
+
+// CORRGS's face bases (the upper faces), cell scalars, CP and RO for the end of IMPS
+template <bool KE>
+__device__ __forceinline__ void corr_basis(const SweepRaw &r, SweepCell &s)
+{
+    cell_scalars<KE>(s, r);
     s.fi = face_basis(r.u1x, r.u1y, r.u1z);
     s.fj = face_basis(r.u2x, r.u2y, r.u2z);
     s.fk = face_basis(r.u3x, r.u3y, r.u3z);
     s.cp = r.cp;
     s.rho = r.ro;
+}
+
+template <bool KE>
+__device__ __forceinline__ void corr_prep(const Prm &p, const SweepRaw &r, SweepCell &s)
+{
+    corr_coef<KE>(p, r, s);
+    corr_basis<KE>(r, s);
 }
 
 // Backward sweep PREDGS (imps4.f:345-598) or PRKEGS (imps4.f:1462-1841, KE) as a tile
@@ -1623,181 +1654,429 @@ __device__ __forceinline__ int tile_ticket_k(const SweepK &s)
     return (int)(t % (unsigned long long)(s.nti * s.ntj));
 }
 
-// PREDGS / PRKEGS (imps4.f:345-598, 1462-1841) marching K downwards
-template <bool KE>
-__global__ void __launch_bounds__(TW) k_pred_km(Prm p, Blk b, SweepK sw)
+// Halo words a lane has loaded for its next cell (I- and J-face contributions)
+struct HaloIn {
+    unsigned long long i[HS], j[HS];
+};
+
+// Wait until at most n vector-memory operations of this wave are outstanding (they
+// complete in issue order).  The LDS rings below are filled by LDS-DMA loads, which the
+// compiler does not order against the LDS reads of the rings; the memory clobber keeps
+// those reads after the wait.
+template <int N>
+__device__ __forceinline__ void vm_wait()
 {
-    constexpr int NV = KE ? 7 : 5;
-    __shared__ double xi[2][NV][TW], xj[2][NV][TW];
-    const int lane = threadIdx.x, ii = lane % KI, jj = lane / KI;
-    const int t = tile_ticket_k(sw);
-    const int ti = sw.nti - 1 - t % sw.nti, tj = sw.ntj - 1 - t / sw.nti;
-    const int i = ti * KI + ii + 1, j = tj * KJ + jj + 1;
-    const bool col = i <= b.imax && j <= b.jmax;
-    const int lag = (KI - 1 - ii) + (KJ - 1 - jj);
-    const int nsteps = b.kmax + KI + KJ - 2;
-    double yk[NV];
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// One LDS-DMA load: every lane fetches the 4 bytes at its `src` into LDS byte address
+// lds + 4*lane (M0 holds the wave's base).  Issued from inline assembly so that the
+// compiler, which cannot tell which LDS bytes such a load writes, does not follow it with
+// a full vmcnt wait before every LDS read; the kernels wait for these loads themselves
+// (vm_wait).  SC1 selects the device-coherent form used for the halo slots.  The s_nop is
+// the wait state an LDS-DMA load needs after an SALU write of M0.
+template <bool SC1>
+__device__ __forceinline__ void lds_dma4(const void *src, unsigned lds)
+{
+    if (SC1)
+        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off sc1"
+                     ::"v"(src), "s"(lds) : "memory", "m0");
+    else
+        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
+                     ::"v"(src), "s"(lds) : "memory", "m0");
+}
+
+__device__ __forceinline__ unsigned lds_addr(const void *p)
+{
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+// Halo words of one face taken from the LDS copy an LDS-DMA load made of the slot (two
+// 32-bit halves per word, hb[2q][lane] and hb[2q+1][lane]).  The halves are separate
+// loads, so a copy taken while the producer stored the word can mix the empty marker with
+// the value: a copy with either half equal to the marker's is not used, and the word is
+// then polled in HBM with 64-bit loads, as is a word still empty.  Every word is emptied
+// again after use.
+template <int NV>
+__device__ __forceinline__ void halo_take(unsigned long long *slot, const unsigned int *hb,
+                                          int lane, unsigned int *err, double v[NV])
+{
 #pragma unroll
-    for (int r = 0; r < NV; r++) yk[r] = 0.;
-    SweepRaw cur;
-    if (col) sweep_load<false, KE>(b, cid(b, i, j, b.kmax), cur);
-    for (int s = 0; s < nsteps; s++) {
-        const int q = s - lag;
-        if (col && q >= 0 && q < b.kmax) {
-            const int k = b.kmax - q;
-            const int n = cid(b, i, j, k);
-            const bool hiin = ii == KI - 1 && i < b.imax, hjin = jj == KJ - 1 && j < b.jmax;
-            unsigned long long hv[NV], hw[NV];
-            if (hiin) halo_issue<NV>(hi_slot(sw, b, ti + 1, j, k), hv);
-            if (hjin) halo_issue<NV>(hjk_slot(sw, b, tj + 1, i, k), hw);
-            SweepRaw nxt;
-            if (q + 1 < b.kmax) sweep_load<false, KE>(b, n - b.il, nxt);
-            SweepCell sc;
-            pred_prep<KE>(p, cur, i, j, k, sc);
-            const int rb = (s + 1) & 1;
-            double bi[NV], bj[NV];
-            if (i == b.imax) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bi[r] = 0.;
-            } else if (!hiin) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bi[r] = xi[rb][r][lane + 1];
-            } else {
-                halo_finish<NV>(hi_slot(sw, b, ti + 1, j, k), hv, sw.err, bi);
+    for (int q = 0; q < NV; q++) {
+        const unsigned int lo = hb[(2 * q) * TW + lane], hi = hb[(2 * q + 1) * TW + lane];
+        unsigned long long x = (unsigned long long)lo | ((unsigned long long)hi << 32);
+        if (lo == (unsigned int)HALO_EMPTY || hi == (unsigned int)(HALO_EMPTY >> 32)) x = HALO_EMPTY;
+        int spins = 0;
+        while (x == HALO_EMPTY) {
+            if ((++spins & 255) == 0) {
+                if (spins > SPIN_LIMIT) {
+                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    x = 0;
+                    break;
+                }
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    x = 0;
+                    break;
+                }
             }
-            if (j == b.jmax) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = 0.;
-            } else if (!hjin) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + KI];
-            } else {
-                halo_finish<NV>(hjk_slot(sw, b, tj + 1, i, k), hw, sw.err, bj);
-            }
-            // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
-            // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
-            if (!KE && j == b.jmax) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bi[r] = 0.;
-            }
-            double dn[NV];
-#pragma unroll
-            for (int r = 0; r < NV; r++) {
-                double cd = -sc.dt1 * sc.wk * yk[r] + sc.vol * sc.d[r];
-                cd = cd - sc.dt1 * (sc.wi * bi[r] + sc.wj * bj[r]);
-                dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
-            }
-#pragma unroll
-            for (int r = 0; r < NV; r++) FLD(b, FF_DRO + r)[n] = dn[r];
-            double oi[NV], oj[NV];
-            face_xfer<KE>(dn, sc, sc.fi, sc.ci, oi);
-            face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
-            face_xfer<KE>(dn, sc, sc.fk, sc.ck, yk);
-            const int wb = s & 1;
-            if (ii > 0) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) xi[wb][r][lane] = oi[r];
-            } else if (i > 1) {
-                halo_put<NV>(hi_slot(sw, b, ti, j, k), oi);
-            }
-            if (jj > 0) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
-            } else if (j > 1) {
-                halo_put<NV>(hjk_slot(sw, b, tj, i, k), oj);
-            }
-            if (q + 1 < b.kmax) cur = nxt;
+            __builtin_amdgcn_s_sleep(1);
+            x = __hip_atomic_load(slot + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        wave_step_sync();
+        v[q] = __longlong_as_double((long long)x);
+        __hip_atomic_store(slot + q, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// DIAGGS + CORRGS / DIKEGS + COKEGS (imps4.f:604-664, 76-339, 1841-1933, 1078-1456)
-// marching K upwards, with the end of IMPS on the final increment (imps3.f:483, 497-500)
-template <bool KE>
-__global__ void __launch_bounds__(TW) k_corr_km(Prm p, Blk b, SweepK sw)
+// Raw values of a cell for the coefficient half of a sweep's preparation (pred_coef /
+// corr_coef) and for the basis half (pred_basis / corr_basis)
+template <bool FWD, bool KE>
+__device__ __forceinline__ void sweep_load_coef(const Blk &b, int n, SweepRaw &r)
+{
+    const int IS = b.is, IL = b.il;
+    constexpr int NV = KE ? 7 : 5;
+    const int s1 = FWD ? 1 : -1, s2 = FWD ? IS : -IS, s3 = FWD ? IL : -IL;
+#pragma unroll
+    for (int q = 0; q < NV; q++) r.d[q] = FLD(b, FF_DRO + q)[n];
+    r.u = FLD(b, FF_U)[n]; r.v = FLD(b, FF_V)[n]; r.w = FLD(b, FF_W)[n];
+    r.c = FLD(b, FF_C)[n]; r.ro = FLD(b, FF_RO)[n]; r.vol = FLD(b, FF_VOL)[n];
+    r.vis = FLD(b, FF_VIS)[n]; r.vist = FLD(b, FF_VIST)[n]; r.dtl = FLD(b, FF_DTL)[n];
+    r.vol1 = FLD(b, FF_VOL)[n + s1]; r.ro1 = FLD(b, FF_RO)[n + s1];
+    r.vis1 = FLD(b, FF_VIS)[n + s1]; r.vist1 = FLD(b, FF_VIST)[n + s1];
+    r.vol2 = FLD(b, FF_VOL)[n + s2]; r.ro2 = FLD(b, FF_RO)[n + s2];
+    r.vis2 = FLD(b, FF_VIS)[n + s2]; r.vist2 = FLD(b, FF_VIST)[n + s2];
+    r.vol3 = FLD(b, FF_VOL)[n + s3]; r.ro3 = FLD(b, FF_RO)[n + s3];
+    r.vis3 = FLD(b, FF_VIS)[n + s3]; r.vist3 = FLD(b, FF_VIST)[n + s3];
+    r.a1 = FLD(b, FF_A1)[n]; r.a2 = FLD(b, FF_A2)[n]; r.a3 = FLD(b, FF_A3)[n];
+    r.n1x = FLD(b, FF_A1X)[n]; r.n1y = FLD(b, FF_A1Y)[n]; r.n1z = FLD(b, FF_A1Z)[n];
+    r.n2x = FLD(b, FF_A2X)[n]; r.n2y = FLD(b, FF_A2Y)[n]; r.n2z = FLD(b, FF_A2Z)[n];
+    r.n3x = FLD(b, FF_A3X)[n]; r.n3y = FLD(b, FF_A3Y)[n]; r.n3z = FLD(b, FF_A3Z)[n];
+    r.a1u = FLD(b, FF_A1)[n + 1]; r.a2u = FLD(b, FF_A2)[n + IS]; r.a3u = FLD(b, FF_A3)[n + IL];
+    if (FWD) {
+        r.u1x = FLD(b, FF_A1X)[n + 1]; r.u1y = FLD(b, FF_A1Y)[n + 1]; r.u1z = FLD(b, FF_A1Z)[n + 1];
+        r.u2x = FLD(b, FF_A2X)[n + IS]; r.u2y = FLD(b, FF_A2Y)[n + IS]; r.u2z = FLD(b, FF_A2Z)[n + IS];
+        r.u3x = FLD(b, FF_A3X)[n + IL]; r.u3y = FLD(b, FF_A3Y)[n + IL]; r.u3z = FLD(b, FF_A3Z)[n + IL];
+    }
+}
+
+template <bool FWD, bool KE>
+__device__ __forceinline__ void sweep_load_basis(const Blk &b, int n, SweepRaw &r)
+{
+    const int IS = b.is, IL = b.il;
+    r.c = FLD(b, FF_C)[n]; r.ro = FLD(b, FF_RO)[n];
+    if (KE) {
+        r.rk = FLD(b, FF_RK)[n]; r.drdp = FLD(b, FF_DRDP)[n]; r.drdh = FLD(b, FF_DRDH)[n];
+    }
+    if (FWD) {
+        r.cp = FLD(b, FF_CP)[n];
+        r.u1x = FLD(b, FF_A1X)[n + 1]; r.u1y = FLD(b, FF_A1Y)[n + 1]; r.u1z = FLD(b, FF_A1Z)[n + 1];
+        r.u2x = FLD(b, FF_A2X)[n + IS]; r.u2y = FLD(b, FF_A2Y)[n + IS]; r.u2z = FLD(b, FF_A2Z)[n + IS];
+        r.u3x = FLD(b, FF_A3X)[n + IL]; r.u3y = FLD(b, FF_A3Y)[n + IL]; r.u3z = FLD(b, FF_A3Z)[n + IL];
+    } else {
+        r.n1x = FLD(b, FF_A1X)[n]; r.n1y = FLD(b, FF_A1Y)[n]; r.n1z = FLD(b, FF_A1Z)[n];
+        r.n2x = FLD(b, FF_A2X)[n]; r.n2y = FLD(b, FF_A2Y)[n]; r.n2z = FLD(b, FF_A2Z)[n];
+        r.n3x = FLD(b, FF_A3X)[n]; r.n3y = FLD(b, FF_A3Y)[n]; r.n3z = FLD(b, FF_A3Z)[n];
+    }
+}
+
+// Rows of the LDS cell buffer through which the preparing waves hand a SweepCell to the
+// recurrence wave (64 cells per row)
+enum {
+    CB_D = 0, CB_DT1 = 7, CB_VOL, CB_WI, CB_WJ, CB_WK, CB_SCINV, CB_SCINP,
+    CB_X = 14,      // x1, x2, x5 of the I, J, K faces
+    CB_B = 23,      // nx ny nz a21 a22 a23 a31 a32 a33 of the I, J, K faces
+    CB_S = 50,      // ro roc pc2 proc yp ydrp drhro ro667
+    CB_CP = 58, CB_RHO, CB_N
+};
+
+__device__ __forceinline__ void cb_put_coef(double *cb, int lane, const SweepCell &s, int nv)
+{
+    for (int q = 0; q < nv; q++) cb[(CB_D + q) * TW + lane] = s.d[q];
+    cb[CB_DT1 * TW + lane] = s.dt1;
+    cb[CB_VOL * TW + lane] = s.vol;
+    cb[CB_WI * TW + lane] = s.wi;
+    cb[CB_WJ * TW + lane] = s.wj;
+    cb[CB_WK * TW + lane] = s.wk;
+    cb[CB_SCINV * TW + lane] = s.scinv;
+    cb[CB_SCINP * TW + lane] = s.scinp;
+    const Coef *c3[3] = {&s.ci, &s.cj, &s.ck};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        cb[(CB_X + 3 * f + 0) * TW + lane] = c3[f]->x1;
+        cb[(CB_X + 3 * f + 1) * TW + lane] = c3[f]->x2;
+        cb[(CB_X + 3 * f + 2) * TW + lane] = c3[f]->x5;
+    }
+}
+
+__device__ __forceinline__ void cb_put_basis(double *cb, int lane, const SweepCell &s)
+{
+    const FaceBasis *b3[3] = {&s.fi, &s.fj, &s.fk};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const FaceBasis &B = *b3[f];
+        const double v[9] = {B.nx, B.ny, B.nz, B.a21, B.a22, B.a23, B.a31, B.a32, B.a33};
+#pragma unroll
+        for (int x = 0; x < 9; x++) cb[(CB_B + 9 * f + x) * TW + lane] = v[x];
+    }
+    const double sv[8] = {s.ro, s.roc, s.pc2, s.proc, s.yp, s.ydrp, s.drhro, s.ro667};
+#pragma unroll
+    for (int x = 0; x < 8; x++) cb[(CB_S + x) * TW + lane] = sv[x];
+    cb[CB_CP * TW + lane] = s.cp;
+    cb[CB_RHO * TW + lane] = s.rho;
+}
+
+__device__ __forceinline__ void cb_get(const double *cb, int lane, SweepCell &s, int nv)
+{
+    for (int q = 0; q < nv; q++) s.d[q] = cb[(CB_D + q) * TW + lane];
+    s.dt1 = cb[CB_DT1 * TW + lane];
+    s.vol = cb[CB_VOL * TW + lane];
+    s.wi = cb[CB_WI * TW + lane];
+    s.wj = cb[CB_WJ * TW + lane];
+    s.wk = cb[CB_WK * TW + lane];
+    s.scinv = cb[CB_SCINV * TW + lane];
+    s.scinp = cb[CB_SCINP * TW + lane];
+    Coef *c3[3] = {&s.ci, &s.cj, &s.ck};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        c3[f]->x1 = cb[(CB_X + 3 * f + 0) * TW + lane];
+        c3[f]->x2 = cb[(CB_X + 3 * f + 1) * TW + lane];
+        c3[f]->x5 = cb[(CB_X + 3 * f + 2) * TW + lane];
+    }
+    FaceBasis *b3[3] = {&s.fi, &s.fj, &s.fk};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        FaceBasis &B = *b3[f];
+        const double *r = cb + (CB_B + 9 * f) * TW + lane;
+        B.nx = r[0]; B.ny = r[TW]; B.nz = r[2 * TW];
+        B.a21 = r[3 * TW]; B.a22 = r[4 * TW]; B.a23 = r[5 * TW];
+        B.a31 = r[6 * TW]; B.a32 = r[7 * TW]; B.a33 = r[8 * TW];
+    }
+    const double *sv = cb + CB_S * TW + lane;
+    s.ro = sv[0]; s.roc = sv[TW]; s.pc2 = sv[2 * TW]; s.proc = sv[3 * TW];
+    s.yp = sv[4 * TW]; s.ydrp = sv[5 * TW]; s.drhro = sv[6 * TW]; s.ro667 = sv[7 * TW];
+    s.cp = cb[CB_CP * TW + lane];
+    s.rho = cb[CB_RHO * TW + lane];
+}
+
+// s_waitcnt immediate of gfx9 that waits for vmcnt = 0 only (expcnt 7, lgkmcnt 15)
+constexpr int VMCNT0_ONLY = 0x0F70;
+
+// End of a step of a warp-specialised workgroup: this wave's LDS writes are complete,
+// then the hardware barrier.  (A workgroup-scope fence would also wait for the waves'
+// outstanding global loads and stores, which must stay in flight across steps.)
+__device__ __forceinline__ void lds_step_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// One LU-SGS sweep marching K: FWD = false is PREDGS / PRKEGS (imps4.f:345-598, 1462-1841),
+// K descending; FWD = true is DIAGGS + CORRGS / DIKEGS + COKEGS (imps4.f:604-664, 76-339,
+// 1841-1933, 1078-1456), K ascending, with the end of IMPS on the final increment
+// (imps3.f:483, 497-500).
+//
+// A workgroup of four waves owns one 16 x 4 tile of (I, J) columns; its waves specialise
+// and meet at one LDS barrier per step:
+//   wave 0  the recurrence: the cell's increments from its prepared coefficients and its
+//           three upstream contributions, and the contributions it hands downstream
+//           (LDS inside the tile, halo slots across tile faces);
+//   wave 1  the coefficient half of the next cell's preparation (pred_coef / corr_coef);
+//   wave 2  the basis half (face bases, cell scalars);
+//   wave 3  the halo words of the step after next, copied into LDS by device-coherent
+//           LDS-DMA loads.
+// Waves 1 and 2 load the raw values of the step after next while they prepare the next
+// one (two register sets, exchanged by writing the loop body twice).  Wave 0 reads only
+// LDS, so no load latency sits on the recurrence.
+template <bool FWD, bool KE>
+__global__ void __launch_bounds__(4 * TW) k_sweep_km(Prm p, Blk b, SweepK sw)
 {
     constexpr int NV = KE ? 7 : 5;
-    __shared__ double xi[2][NV][TW], xj[2][NV][TW];
-    const int lane = threadIdx.x, ii = lane % KI, jj = lane / KI;
-    const int t = tile_ticket_k(sw);
-    const int ti = t % sw.nti, tj = t / sw.nti;
+    __shared__ double cbuf[2][CB_N * TW];
+    __shared__ unsigned int hbuf[3][2 * 2 * NV * TW];
+    __shared__ double xi[NV][TW], xj[NV][TW];
+    __shared__ int tshare;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / TW);
+    const int lane = threadIdx.x % TW, ii = lane % KI, jj = lane / KI;
+    if (threadIdx.x == 0)
+        tshare = (int)(atomicAdd(sw.ticket, 1ULL) % (unsigned long long)(sw.nti * sw.ntj));
+    __syncthreads();
+    const int t = tshare;
+    const int ti = FWD ? t % sw.nti : sw.nti - 1 - t % sw.nti;
+    const int tj = FWD ? t / sw.nti : sw.ntj - 1 - t / sw.nti;
     const int i = ti * KI + ii + 1, j = tj * KJ + jj + 1;
     const bool col = i <= b.imax && j <= b.jmax;
-    const int lag = ii + jj;
+    const int lag = FWD ? ii + jj : (KI - 1 - ii) + (KJ - 1 - jj);
     const int nsteps = b.kmax + KI + KJ - 2;
-    double yk[NV];
+    const int nit = (nsteps + 3) & ~1;       // steps -2 .. nsteps-1, rounded up to even
+    const bool hin_i = FWD ? (ii == 0 && i > 1) : (ii == KI - 1 && i < b.imax);
+    const bool hin_j = FWD ? (jj == 0 && j > 1) : (jj == KJ - 1 && j < b.jmax);
+    const int fi_in = FWD ? ti : ti + 1, fj_in = FWD ? tj : tj + 1;     // slot faces read
+    const int fi_out = FWD ? ti + 1 : ti, fj_out = FWD ? tj + 1 : tj;   // slot faces written
+    auto kof = [&](int q) { return FWD ? q + 1 : b.kmax - q; };
+    auto live = [&](int q) { return col && q >= 0 && q < b.kmax; };
+
+    if (wid == 0) {
+        // ---- the recurrence -----------------------------------------------------------
+        double yk[NV];
 #pragma unroll
-    for (int r = 0; r < NV; r++) yk[r] = 0.;
-    SweepRaw cur;
-    if (col) sweep_load<true, KE>(b, cid(b, i, j, 1), cur);
-    for (int s = 0; s < nsteps; s++) {
-        const int q = s - lag;
-        if (col && q >= 0 && q < b.kmax) {
-            const int k = q + 1;
-            const int n = cid(b, i, j, k);
-            const bool hiin = ii == 0 && i > 1, hjin = jj == 0 && j > 1;
-            unsigned long long hv[NV], hw[NV];
-            if (hiin) halo_issue<NV>(hi_slot(sw, b, ti, j, k), hv);
-            if (hjin) halo_issue<NV>(hjk_slot(sw, b, tj, i, k), hw);
-            SweepRaw nxt;
-            if (q + 1 < b.kmax) sweep_load<true, KE>(b, n + b.il, nxt);
-            SweepCell sc;
-            corr_prep<KE>(p, cur, sc);
-            const int rb = (s + 1) & 1;
-            double bi[NV], bj[NV];
-            if (i == 1) {
+        for (int r = 0; r < NV; r++) yk[r] = 0.;
+        // The data registers of a global store stay locked until the store completes; the
+        // values this step stores are kept live through the next step, so that the next
+        // step's LDS reads land in other registers and do not wait for the stores.
+        double kept[3 * NV], held[3 * NV];
 #pragma unroll
-                for (int r = 0; r < NV; r++) bi[r] = 0.;
-            } else if (!hiin) {
+        for (int r = 0; r < 3 * NV; r++) kept[r] = held[r] = 0.;
+        for (int it = 0; it < nit; it++) {
+            const int s = it - 2;
+            const int q = s - lag;
+            if (s >= 0 && live(q)) {
+                const int k = kof(q);
+                const int n = cid(b, i, j, k);
+                SweepCell sc;
+                cb_get(cbuf[s & 1], lane, sc, NV);
+                const unsigned int *hb = hbuf[s % 3];
+                double bi[NV], bj[NV];
+                if (FWD ? i == 1 : i == b.imax) {
 #pragma unroll
-                for (int r = 0; r < NV; r++) bi[r] = xi[rb][r][lane - 1];
-            } else {
-                halo_finish<NV>(hi_slot(sw, b, ti, j, k), hv, sw.err, bi);
+                    for (int r = 0; r < NV; r++) bi[r] = 0.;
+                } else if (!hin_i) {
+#pragma unroll
+                    for (int r = 0; r < NV; r++) bi[r] = xi[r][FWD ? lane - 1 : lane + 1];
+                } else {
+                    halo_take<NV>(hi_slot(sw, b, fi_in, j, k), hb, lane, sw.err, bi);
+                }
+                if (FWD ? j == 1 : j == b.jmax) {
+#pragma unroll
+                    for (int r = 0; r < NV; r++) bj[r] = 0.;
+                } else if (!hin_j) {
+#pragma unroll
+                    for (int r = 0; r < NV; r++) bj[r] = xj[r][FWD ? lane - KI : lane + KI];
+                } else {
+                    halo_take<NV>(hjk_slot(sw, b, fj_in, i, k), hb + 2 * NV * TW, lane, sw.err, bj);
+                }
+                double dn[NV];
+                if (FWD) {
+#pragma unroll
+                    for (int r = 0; r < NV; r++) {
+                        double cd = sc.dt1 * sc.wk * yk[r] + sc.vol * sc.d[r];
+                        cd = cd + sc.dt1 * (sc.wi * bi[r] + sc.wj * bj[r]);
+                        dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
+                    }
+                    kept[0] = dn[0] / sc.cp;
+#pragma unroll
+                    for (int r = 1; r < 5; r++) kept[r] = dn[r];
+                    if (KE) {
+                        kept[NV > 5 ? 5 : 0] = dn[NV > 5 ? 5 : 0] * sc.rho;
+                        kept[NV > 6 ? 6 : 0] = dn[NV > 6 ? 6 : 0] * sc.rho;
+                    }
+                } else {
+                    // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
+                    // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
+                    if (!KE && j == b.jmax) {
+#pragma unroll
+                        for (int r = 0; r < NV; r++) bi[r] = 0.;
+                    }
+#pragma unroll
+                    for (int r = 0; r < NV; r++) {
+                        double cd = -sc.dt1 * sc.wk * yk[r] + sc.vol * sc.d[r];
+                        cd = cd - sc.dt1 * (sc.wi * bi[r] + sc.wj * bj[r]);
+                        dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
+                    }
+#pragma unroll
+                    for (int r = 0; r < NV; r++) kept[r] = dn[r];
+                }
+#pragma unroll
+                for (int r = 0; r < NV; r++) FLD(b, FF_DRO + r)[n] = kept[r];
+                double *oi = kept + NV, *oj = kept + 2 * NV;
+                face_xfer<KE>(dn, sc, sc.fi, sc.ci, oi);
+                face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
+                face_xfer<KE>(dn, sc, sc.fk, sc.ck, yk);
+                // the neighbours' values of this step were read above, before these writes
+                if (FWD ? ii < KI - 1 : ii > 0) {
+#pragma unroll
+                    for (int r = 0; r < NV; r++) xi[r][lane] = oi[r];
+                } else if (FWD ? i < b.imax : i > 1) {
+                    halo_put<NV>(hi_slot(sw, b, fi_out, j, k), oi);
+                }
+                if (FWD ? jj < KJ - 1 : jj > 0) {
+#pragma unroll
+                    for (int r = 0; r < NV; r++) xj[r][lane] = oj[r];
+                } else if (FWD ? j < b.jmax : j > 1) {
+                    halo_put<NV>(hjk_slot(sw, b, fj_out, i, k), oj);
+                }
             }
-            if (j == 1) {
+            // the previous step's stored values end their life here
 #pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = 0.;
-            } else if (!hjin) {
+            for (int r = 0; r < 3 * NV; r++) asm volatile("" ::"v"(held[r]));
 #pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - KI];
-            } else {
-                halo_finish<NV>(hjk_slot(sw, b, tj, i, k), hw, sw.err, bj);
-            }
-            double dn[NV];
-#pragma unroll
-            for (int r = 0; r < NV; r++) {
-                double cd = sc.dt1 * sc.wk * yk[r] + sc.vol * sc.d[r];
-                cd = cd + sc.dt1 * (sc.wi * bi[r] + sc.wj * bj[r]);
-                dn[r] = r != 4 ? cd * sc.scinv : cd * sc.scinp;
-            }
-            FLD(b, FF_DRO)[n] = dn[0] / sc.cp;
-#pragma unroll
-            for (int r = 1; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
-            if (KE) {
-                FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * sc.rho;
-                FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * sc.rho;
-            }
-            double oi[NV], oj[NV];
-            face_xfer<KE>(dn, sc, sc.fi, sc.ci, oi);
-            face_xfer<KE>(dn, sc, sc.fj, sc.cj, oj);
-            face_xfer<KE>(dn, sc, sc.fk, sc.ck, yk);
-            const int wb = s & 1;
-            if (ii < KI - 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) xi[wb][r][lane] = oi[r];
-            } else if (i < b.imax) {
-                halo_put<NV>(hi_slot(sw, b, ti + 1, j, k), oi);
-            }
-            if (jj < KJ - 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
-            } else if (j < b.jmax) {
-                halo_put<NV>(hjk_slot(sw, b, tj + 1, i, k), oj);
-            }
-            if (q + 1 < b.kmax) cur = nxt;
+            for (int r = 0; r < 3 * NV; r++) held[r] = kept[r];
+            lds_step_barrier();
         }
-        wave_step_sync();
+    } else if (wid == 1) {
+        // ---- coefficients of the next cell --------------------------------------------
+        auto iter = [&](int s, const SweepRaw &use, SweepRaw &fill) __attribute__((always_inline)) {
+            const int q1 = s + 1 - lag, q2 = s + 2 - lag;
+            // the loads of the previous step (the values used now) had a whole step to land;
+            // an explicit wait tells the compiler so, and it adds none inside the step
+            __builtin_amdgcn_s_waitcnt(VMCNT0_ONLY);
+            if (live(q2)) sweep_load_coef<FWD, KE>(b, cid(b, i, j, kof(q2)), fill);
+            if (live(q1)) {
+                SweepCell sc;
+                if (FWD)
+                    corr_coef<KE>(p, use, sc);
+                else
+                    pred_coef<KE>(p, use, i, j, kof(q1), sc);
+                cb_put_coef(cbuf[(s + 1) & 1], lane, sc, NV);
+            }
+            lds_step_barrier();
+        };
+        SweepRaw ra, rb;
+        for (int it = 0; it < nit; it += 2) {
+            iter(it - 2, rb, ra);
+            iter(it - 1, ra, rb);
+        }
+    } else if (wid == 2) {
+        // ---- face bases and cell scalars of the next cell ------------------------------
+        auto iter = [&](int s, const SweepRaw &use, SweepRaw &fill) __attribute__((always_inline)) {
+            const int q1 = s + 1 - lag, q2 = s + 2 - lag;
+            __builtin_amdgcn_s_waitcnt(VMCNT0_ONLY);
+            if (live(q2)) sweep_load_basis<FWD, KE>(b, cid(b, i, j, kof(q2)), fill);
+            if (live(q1)) {
+                SweepCell sc;
+                if (FWD)
+                    corr_basis<KE>(use, sc);
+                else
+                    pred_basis<KE>(use, sc);
+                cb_put_basis(cbuf[(s + 1) & 1], lane, sc);
+            }
+            lds_step_barrier();
+        };
+        SweepRaw ra, rb;
+        for (int it = 0; it < nit; it += 2) {
+            iter(it - 2, rb, ra);
+            iter(it - 1, ra, rb);
+        }
+    } else {
+        // ---- halo words of the step after next -----------------------------------------
+        for (int it = 0; it < nit; it++) {
+            const int s2 = it;                     // step whose halo words are copied
+            const int q = s2 - lag;
+            const bool ok = live(q);
+            const int k = ok ? kof(q) : 1;
+            const unsigned long long *si = hin_i && ok ? hi_slot(sw, b, fi_in, j, k) : sw.hi;
+            const unsigned long long *sj = hin_j && ok ? hjk_slot(sw, b, fj_in, i, k) : sw.hi;
+            unsigned int *hb = hbuf[s2 % 3];
+#pragma unroll
+            for (int w = 0; w < 2 * NV; w++) {
+                lds_dma4<true>((const unsigned int *)si + w, lds_addr(hb + w * TW));
+                lds_dma4<true>((const unsigned int *)sj + w, lds_addr(hb + (2 * NV + w) * TW));
+            }
+            // the copies of step s2 - 1 are complete: 4*NV loads were issued after them
+            vm_wait<4 * NV>();
+            lds_step_barrier();
+        }
+        vm_wait<0>();
     }
 }
 
@@ -2455,11 +2734,11 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         // DRO/CP is applied inside the forward sweep
         dim3 g(r->swk.nti * r->swk.ntj);
         if (c->prm.iturb == 6) {
-            if ((rc = launch(c, "k_pred_km", k_pred_km<true>, g, dim3(TW), kp, b, r->swk))) return rc;
-            return launch(c, "k_corr_km", k_corr_km<true>, g, dim3(TW), kp, b, r->swk);
+            if ((rc = launch(c, "k_pred_km", k_sweep_km<false, true>, g, dim3(4 * TW), kp, b, r->swk))) return rc;
+            return launch(c, "k_corr_km", k_sweep_km<true, true>, g, dim3(4 * TW), kp, b, r->swk);
         }
-        if ((rc = launch(c, "k_pred_km", k_pred_km<false>, g, dim3(TW), kp, b, r->swk))) return rc;
-        return launch(c, "k_corr_km", k_corr_km<false>, g, dim3(TW), kp, b, r->swk);
+        if ((rc = launch(c, "k_pred_km", k_sweep_km<false, false>, g, dim3(4 * TW), kp, b, r->swk))) return rc;
+        return launch(c, "k_corr_km", k_sweep_km<true, false>, g, dim3(4 * TW), kp, b, r->swk);
     }
     if (c->lusgs_mode == FF_LUSGS_TILES) {
         // DRO/CP is applied inside the forward sweep
