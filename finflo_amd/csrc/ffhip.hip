@@ -164,6 +164,116 @@ __device__ __forceinline__ double derfun_cell(const Blk &b, int L)
     return strain;
 }
 
+// EARSM1 earsm.f:60-219 at one interior cell: the Wallin-Johansson extra anisotropy b_ij
+// from the time scale TTS, the strain-rate tensor SIJ and the vorticity tensor W12, W13, W23.
+// The curvature correction is off (PA0 = 0, earsm.f:47; its WIR term is 0*0), and the
+// routine's EPS2/VIST update is dead code (earsm.f:222), so only BIJ is written.  The
+// association of every sum and product is the routine's; x**4 is a left-to-right product,
+// x**P3 and x**P6 are pow.
+__device__ __forceinline__ void earsm_cell(const Blk &b, int L)
+{
+    const double p3 = 1.0 / 3.0, p6 = 0.5 * p3, tt = 2.0 * p3;
+    const double c1 = 1.8, c1p = 9.0 * (c1 - 1.0) / 4.0, c1p3 = c1p * p3, c1psq = c1p * c1p;
+    const double pa0 = 0., wir = 0.;
+    const double ts = FLD(b, FF_TTS)[L];
+    double s11 = ts * FLD(b, FF_S11)[L], s22 = ts * FLD(b, FF_S22)[L], s33 = ts * FLD(b, FF_S33)[L];
+    double s12 = ts * FLD(b, FF_S12)[L], s13 = ts * FLD(b, FF_S13)[L], s23 = ts * FLD(b, FF_S23)[L];
+    double w12 = ts * FLD(b, FF_W12)[L] - pa0 * wir;
+    double w13 = ts * FLD(b, FF_W13)[L] + pa0 * wir;
+    double w23 = ts * FLD(b, FF_W23)[L] - pa0 * wir;
+    double s11sq = s11 * s11, s22sq = s22 * s22, s33sq = s33 * s33;
+    double s12sq = s12 * s12, s13sq = s13 * s13, s23sq = s23 * s23;
+    double w12sq = w12 * w12, w13sq = w13 * w13, w23sq = w23 * w23;
+    double sii = s11sq + s22sq + s33sq + 2.0 * (s12sq + s13sq + s23sq);
+    double wii = -(2.0 * (w12sq + w13sq + w23sq));
+    double wiip3 = p3 * wii;
+    double swwiv = -(s11 * (w12sq + w13sq)) - s22 * (w12sq + w23sq) - s33 * (w13sq + w23sq)
+                 + 2.0 * (-(s12 * w13 * w23) + s13 * w12 * w23 - s23 * w12 * w13);
+    double swwivtt = tt * swwiv;
+    double sswwv = 2.0 * (-((s12 * s13 + s22 * s23 + s23 * s33) * w12 * w13)
+                          + (s11 * s13 + s12 * s23 + s13 * s33) * w12 * w23
+                          - (s11 * s12 + s12 * s22 + s13 * s23) * w13 * w23)
+                 - (s11sq + s12sq + s13sq) * (w12sq + w13sq)
+                 - (s12sq + s22sq + s23sq) * (w12sq + w23sq)
+                 - (s13sq + s23sq + s33sq) * (w13sq + w23sq);
+    double t3c11 = -w12sq - w13sq - wiip3;
+    double t3c22 = -w12sq - w23sq - wiip3;
+    double t3c12 = -(w13 * w23);
+    double t3c13 = w12 * w23;
+    double t3c23 = -(w12 * w13);
+    double t4c11 = -(2.0 * (s12 * w12 + s13 * w13));
+    double t4c22 = 2.0 * (s12 * w12 - s23 * w23);
+    double t4c12 = (s11 - s22) * w12 - s23 * w13 - s13 * w23;
+    double t4c13 = -(s23 * w12) + (s11 - s33) * w13 + s12 * w23;
+    double t4c23 = s13 * w12 + s12 * w13 + (s22 - s33) * w23;
+    double t6c11 = -(2.0 * ((s12 * w13 - s13 * w12) * w23 + s11 * (w12sq + w13sq)))
+                 - swwivtt - wii * s11;
+    double t6c22 = -(2.0 * ((s23 * w12 + s12 * w23) * w13 + s22 * (w12sq + w23sq)))
+                 - swwivtt - wii * s22;
+    double t6c12 = -(s12 * (2.0 * w12sq + w13sq + w23sq))
+                 - (s13 * w13 - s23 * w23) * w12 - (s11 + s22) * w13 * w23 - wii * s12;
+    double t6c13 = -(s13 * (w12sq + 2.0 * w13sq + w23sq))
+                 - (s12 * w12 + s23 * w23) * w13 + (s11 + s33) * w12 * w23 - wii * s13;
+    double t6c23 = -(s23 * (w12sq + w13sq + 2.0 * w23sq))
+                 + (s12 * w12 - s13 * w13) * w23 - (s22 + s33) * w12 * w13 - wii * s23;
+    double t9c11 = -(2.0 * ((s12 * w12 + s13 * w13 - s23 * w23) * w12sq
+                            + (s12 * w12 + s13 * w13 + s23 * w23) * w13sq
+                            + (s22 - s33) * w12 * w13 * w23));
+    double t9c22 = -(2.0 * ((-(s12 * w12) - s13 * w13 + s23 * w23) * w12sq
+                            + (-(s12 * w12) + s13 * w13 + s23 * w23) * w23sq
+                            + (-s11 + s33) * w12 * w13 * w23));
+    double t9c12 = ((s11 - s22) * w12 - 2.0 * (s13 * w23 + s23 * w13)) * w12sq
+                 + ((s11 - s33) * w12 - 2.0 * s13 * w23) * w13sq
+                 + ((s33 - s22) * w12 - 2.0 * s23 * w13) * w23sq;
+    double t9c13 = ((s11 - s22) * w13 + 2.0 * s12 * w23) * w12sq
+                 + ((s11 - s33) * w13 + 2.0 * (s12 * w23 - s23 * w12)) * w13sq
+                 + ((s22 - s33) * w13 - 2.0 * s23 * w12) * w23sq;
+    double t9c23 = ((s22 - s11) * w23 + 2.0 * s12 * w13) * w12sq
+                 + ((s11 - s33) * w23 + 2.0 * s13 * w12) * w13sq
+                 + ((s22 - s33) * w23 + 2.0 * (s12 * w13 + s13 * w12)) * w23sq;
+    // N_c from the cubic (earsm.f:170-184)
+    double p1 = (c1psq / 27.0 + 0.45 * sii - tt * wii) * c1p;
+    double cb = c1psq * (p3 * p3) + 0.9 * sii + tt * wii;
+    double p2 = p1 * p1 - cb * cb * cb;
+    double fnc;
+    if (p2 >= 0.0) {
+        double sqp2 = sqrt(p2);
+        double pm = p1 - sqp2;
+        double pmp = pow(fabs(pm), p3);
+        fnc = c1p3 + pow(p1 + sqp2, p3) + copysign(fabs(pmp), pm);
+    } else {
+        double pm = p1 * p1 - p2;
+        double sqpm = sqrt(pm);
+        double alpo = p1 / sqpm;
+        alpo = dmin(alpo, 1.);
+        alpo = dmax(alpo, -1.);
+        double facos = p3 * acos(alpo);
+        fnc = c1p3 + 2.0 * pow(pm, p6) * cos(facos);
+    }
+    // the N correction and the betas (earsm.f:187-205)
+    double dd = 20.0 * pow4(fnc) * (fnc - 0.5 * c1p)
+              - wii * (10.0 * fnc + 15.0 * c1p) * (fnc * fnc)
+              + 10.0 * c1p * (wii * wii);
+    double fii1 = swwiv * swwiv;
+    double fii2 = sswwv - 0.5 * sii * wii;
+    double fn = fnc + 162.0 * (fii1 + fii2 * (fnc * fnc)) / dd;
+    double fnsq = fn * fn;
+    double q = 5.0 * p6 * (fnsq - 2.0 * wii) * (2.0 * fnsq - wii);
+    double pq = 1.0 / q;
+    double beta3 = -(pq * 12.0 * swwiv / fn);
+    double beta4 = -(pq * 2.0 * (fnsq - 2.0 * wii));
+    double beta6 = -(pq * 6.0 * fn);
+    double beta9 = pq * 6.0;
+    double b1 = 0.5 * (beta3 * t3c11 + beta4 * t4c11 + beta6 * t6c11 + beta9 * t9c11);
+    double b4 = 0.5 * (beta3 * t3c22 + beta4 * t4c22 + beta6 * t6c22 + beta9 * t9c22);
+    FLD(b, FF_B11)[L] = b1;
+    FLD(b, FF_B12)[L] = 0.5 * (beta3 * t3c12 + beta4 * t4c12 + beta6 * t6c12 + beta9 * t9c12);
+    FLD(b, FF_B13)[L] = 0.5 * (beta3 * t3c13 + beta4 * t4c13 + beta6 * t6c13 + beta9 * t9c13);
+    FLD(b, FF_B22)[L] = b4;
+    FLD(b, FF_B23)[L] = 0.5 * (beta3 * t3c23 + beta4 * t4c23 + beta6 * t6c23 + beta9 * t9c23);
+    FLD(b, FF_B33)[L] = -b1 - b4;
+}
+
 // TURBEQ (res3c.f:5951-6371) on the cells of level 1 plus one ghost layer: DERFUN
 // (turb3.f:2476-2665, IDERI = 1: distance-weighted Green-Gauss) and TTIMES (earsm.f:671-684)
 // everywhere there, EPS2KO (turb3.f:7454-7537, limiter argument STRAIN, FRSMUT =
@@ -187,6 +297,7 @@ __global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
         FLD(b, FF_TTS)[L] = dmax(tstur, tsvis);
     }
     if (i < 1 || i > b.imax || j < 1 || j > b.jmax || k < 1 || k > b.kmax) return;
+    if (p.istres == 1) earsm_cell(b, L);   // EARSM1 after TTIMES (res3c.f:5987-5990)
     // EPS2KO
     const SstC s = coefko(p.kover);
     const double eps = 1.E-20;
@@ -537,6 +648,26 @@ __global__ void __launch_bounds__(256) k_fun1_prod(Prm p, Blk b, const double *c
     double dislim = .40559E-7 * sqrt(p.re);
     if (distw < dislim) f1 = 1.;
     FLD(b, FF_FUN1)[L] = f1;
+    if (p.istres >= 1) {
+        // PEEKOO earsm.f:592-622 (res3c.f:3976-3978): production with the EARSM stresses
+        const double p3 = 1.0 / 3.0, epsc = 1.E-20;
+        double s11 = FLD(b, FF_S11)[L], s22 = FLD(b, FF_S22)[L], s33 = FLD(b, FF_S33)[L];
+        double s12 = FLD(b, FF_S12)[L], s13 = FLD(b, FF_S13)[L], s23 = FLD(b, FF_S23)[L];
+        double b1 = FLD(b, FF_B11)[L], b2 = FLD(b, FF_B12)[L], b3 = FLD(b, FF_B13)[L];
+        double b4 = FLD(b, FF_B22)[L], b5 = FLD(b, FF_B23)[L];
+        double rmut = dmax(FLD(b, FF_EPS2)[L] - 1.0, epsc) * vis;
+        double divvp3 = p3 * (s11 + s22 + s33);
+        double huu = dmax((rk * (p3 + b1) - rmut * (s11 - divvp3)), 0.0);
+        double hvv = dmax((rk * (p3 + b4) - rmut * (s22 - divvp3)), 0.0);
+        double hww = dmax((rk * (p3 - b1 - b4) - rmut * (s33 - divvp3)), 0.0);
+        double huv = rk * b2 - rmut * s12;
+        double huw = rk * b3 - rmut * s13;
+        double hvw = rk * b5 - rmut * s23;
+        double pt = -(2.0 * (huu * s11 + hvv * s22 + hww * s33
+                             + 2.0 * (huv * s12 + huw * s13 + hvw * s23)));
+        FLD(b, FF_PTUR)[L] = pt * 1.0;
+        return;
+    }
     // PRODKE
     const double r2o3 = 2. / 3.;
     double s11 = FLD(b, FF_S11)[L], s12 = FLD(b, FF_S12)[L], s13 = FLD(b, FF_S13)[L];
@@ -2262,7 +2393,7 @@ static Prm make_prm(const ff_params *p, int level)
     k.rklim = p->rklim; k.frsvis = p->frsvis; k.re = p->re; k.csimps = p->csimps;
     k.turlim = p->turlim;
     k.iturb = p->iturb;
-    k.kover = p->kover; k.katol = p->katol;
+    k.kover = p->kover; k.katol = p->katol; k.istres = p->istres;
     k.cmgk = p->cmgk; k.cmgeps = p->cmgeps;
     for (int d = 0; d < 3; d++) k.inter[d] = level >= 2 ? 55 : p->inter[d];   // res3c RESIDI ns3c.f:2468-2472
     return k;
@@ -2417,6 +2548,7 @@ static const int kNeeded[] = {
     FF_DE, FF_DTL, FF_TOLD, FF_UOLD, FF_VOLD, FF_WOLD, FF_POLD, FF_VOL, FF_A1, FF_A1X,
     FF_A1Y, FF_A1Z, FF_A2, FF_A2X, FF_A2Y, FF_A2Z, FF_A3, FF_A3X, FF_A3Y, FF_A3Z};
 // and those of the k-omega SST path
+static const int kNeededEarsm[] = {FF_TTS, FF_B11, FF_B12, FF_B13, FF_B22, FF_B23, FF_B33};
 static const int kNeededKw[] = {
     FF_RK, FF_REPS, FF_DRK, FF_DEPS, FF_RKOLD, FF_EPSOLD, FF_D1, FF_D2, FF_D3, FF_DISTW,
     FF_SRK, FF_SEPS, FF_PTUR, FF_FUN1, FF_OHMI, FF_W12, FF_W13, FF_W23, FF_S11, FF_S12,
@@ -2428,6 +2560,9 @@ static int check_fields(ff_ctx *c, const ff_block *v, const char *what)
         if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
     if (c->prm.iturb == 6)
         for (int f : kNeededKw)
+            if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
+    if (c->prm.istres >= 1)
+        for (int f : kNeededEarsm)
             if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
     return FF_OK;
 }
@@ -2441,6 +2576,7 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
     if (prm->iturb != 0 && prm->iturb != 6) return FF_E_UNSUP;   // laminar or k-omega SST
     if (prm->iturb == 6 && (prm->kover != 0 && prm->kover != 5)) return FF_E_UNSUP;
     if (prm->iturb == 6 && prm->iflux != 1) return FF_E_UNSUP;   // FLXCYC: Roe only
+    if (prm->istres != 0 && (prm->iturb != 6 || prm->istres != 1)) return FF_E_UNSUP;  // EARSM1 on SST
     if (prm->iflux != 1 && prm->iflux != 0) return FF_E_UNSUP;   // Roe FLUXKE, van Leer FLUXVL
     if (prm->idiff > 3) return FF_E_UNSUP;         // arithmetic FLUXDO average
     if (prm->expsu != 1.5) return FF_E_UNSUP;      // Sutherland sqrt branch of PERFGA

@@ -27,7 +27,7 @@ struct Prm {
     double rklim, frsvis, re, csimps, turlim, cmgk, cmgeps;
     int iturb;
     int inter[3];
-    int kover, katol;
+    int kover, katol, istres;
 };
 
 #define FLD(b, n) ((b).f[n])
@@ -362,6 +362,27 @@ __device__ __forceinline__ Flux7 face_flux_kw(const Prm &p, const SstC &s, const
     kw_diff_face(b, L, S, an, s, f6, f7);
     visc_face2(p, b, L, S, an, anx, any, anz, fm, fn, fw, f5v, fh);
     double fe = f6 + f5v + fh;
+    if (p.istres >= 1) {
+        // ANISFLUX earsm.f:278-307 (FLXCYC res3c.f:6933-6937): the EARSM Reynolds stresses
+        // added to the momentum and energy fluxes ahead of the inviscid flux
+        const int M = L - S;
+        const double *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W), *rk = FLD(b, FF_RK);
+        const double *b1 = FLD(b, FF_B11), *b2 = FLD(b, FF_B12), *b3 = FLD(b, FF_B13),
+                     *b4 = FLD(b, FF_B22), *b5 = FLD(b, FF_B23);
+        double ucm = 0.5 * (u[L] + u[M]), vcm = 0.5 * (v[L] + v[M]), wcm = 0.5 * (w[L] + w[M]);
+        double b11 = b1[L] + b1[M], b12 = b2[L] + b2[M], b13 = b3[L] + b3[M];
+        double b22 = b4[L] + b4[M], b23 = b5[L] + b5[M];
+        double b33 = -b1[L] - b4[L] - b1[M] - b4[M];
+        double rka = 0.5 * (rk[L] + rk[M]) * an[L];
+        double f2ex = rka * (anx[L] * b11 + any[L] * b12 + anz[L] * b13);
+        double f3ex = rka * (anx[L] * b12 + any[L] * b22 + anz[L] * b23);
+        double f4ex = rka * (anx[L] * b13 + any[L] * b23 + anz[L] * b33);
+        double f5ex = ucm * f2ex + vcm * f3ex + wcm * f4ex;
+        fm = fm + f2ex;
+        fn = fn + f3ex;
+        fw = fw + f4ex;
+        fe = fe + f5ex;
+    }
     Flux7 r = roe_face<SINGLE, 7>(p, b, L, S, intem, an, anx, any, anz);
     Flux7 o;
     o.f[0] = r.f[0] + 0.;

@@ -48,7 +48,7 @@ class Params(C.Structure):
         ("reserved", C.c_double * 3),
         ("iturb", C.c_int), ("iflux", C.c_int), ("idiff", C.c_int),
         ("inter", C.c_int * 3), ("kover", C.c_int), ("katol", C.c_int),
-        ("reserved_i", C.c_int * 8)]
+        ("istres", C.c_int), ("reserved_i", C.c_int * 7)]
 
     DOUBLES = ("gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
                "frspre", "frsden", "frstem", "frsvel", "frsvis",
@@ -86,6 +86,7 @@ def default_params(**kw):
     p.iturb, p.iflux, p.idiff = 0, 1, 1
     p.inter[0] = p.inter[1] = p.inter[2] = -3
     p.kover, p.katol = 0, 1
+    p.istres = 0
     for k, v in kw.items():
         if k == "inter":
             for i in range(3):

@@ -89,7 +89,9 @@ typedef struct ff_params {
     int inter[3];    /* MUSCL switch per direction (INTERI/J/K, main.f:5648-5650), -3 */
     int kover;       /* KOVER of the SST variant */
     int katol;       /* Kato-Launder production switch */
-    int reserved_i[8];
+    int istres;      /* ISTRES: 1 = Wallin-Johansson EARSM on top of the SST (EARSM1, ANISFLUX,
+                        PEEKOO), 0 = eddy-viscosity SST */
+    int reserved_i[7];
 } ff_params;
 
 /* One (level M, block N) slab: pointers to the first element of each field, i.e. the

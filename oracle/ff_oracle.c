@@ -530,6 +530,28 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last, 
                 visc_face2(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &f5, &fh);
                 ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
                 ff[4][L] = ff[4][L] + f5 + fh;
+                if (kw && p->istres >= 1) {
+                    /* ANISFLUX earsm.f:278-307 (FLXCYC res3c.f:6933-6937): the EARSM stresses
+                       added to the momentum and energy fluxes before the inviscid flux */
+                    long M = L - S;
+                    const double *u = F(b, FF_U), *v = F(b, FF_V), *w = F(b, FF_W),
+                                 *rk = F(b, FF_RK), *b1 = F(b, FF_B11), *b2 = F(b, FF_B12),
+                                 *b3 = F(b, FF_B13), *b4 = F(b, FF_B22), *b5 = F(b, FF_B23);
+                    double ucm = 0.5 * (u[L] + u[M]), vcm = 0.5 * (v[L] + v[M]);
+                    double wcm = 0.5 * (w[L] + w[M]);
+                    double b11 = b1[L] + b1[M], b12 = b2[L] + b2[M], b13 = b3[L] + b3[M];
+                    double b22 = b4[L] + b4[M], b23 = b5[L] + b5[M];
+                    double b33 = -b1[L] - b4[L] - b1[M] - b4[M];
+                    double rka = 0.5 * (rk[L] + rk[M]) * an[L];
+                    double f2ex = rka * (anx[L] * b11 + any[L] * b12 + anz[L] * b13);
+                    double f3ex = rka * (anx[L] * b12 + any[L] * b22 + anz[L] * b23);
+                    double f4ex = rka * (anx[L] * b13 + any[L] * b23 + anz[L] * b33);
+                    double f5ex = ucm * f2ex + vcm * f3ex + wcm * f4ex;
+                    ff[1][L] = ff[1][L] + f2ex;
+                    ff[2][L] = ff[2][L] + f3ex;
+                    ff[3][L] = ff[3][L] + f4ex;
+                    ff[4][L] = ff[4][L] + f5ex;
+                }
                 /* then FLUXKE / FLUXVL (res3c.f:7892, 7907; 6946) adds the inviscid flux */
                 flux7 r = kw ? roe_face(p, b, L, S, intem, an, anx, any, anz, dir == 2,
                                         intet_of(intem), p->rklim, p->epslim)
@@ -697,6 +719,126 @@ static void ttimes(ff_block *b)
                 double tstur = ro[L] * bnum;
                 double tsvis = ct * sqrt(ro[L] * vis[L] * bnum / fabs(rk[L] + eps));
                 F(b, FF_TTS)[L] = dmax(tstur, tsvis);
+            }
+}
+
+/* EARSM1 earsm.f:1-247: the Wallin-Johansson extra anisotropy b_ij of the interior cells
+ * from the time scale TTS, the strain-rate tensor SIJ and the vorticity tensor W12, W13,
+ * W23 (WIJ).  The curvature correction is off (PA0 = 0, earsm.f:47), and the EPS2/VIST
+ * update of the routine is dead code (the `imax < 0` branch, earsm.f:222), so only BIJ is
+ * written.  Literal constants, powers (x**4 a left-to-right product, x**P3 and x**P6 pow)
+ * and the association of every sum are the routine's. */
+static void earsm1(ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double p3 = 1.0 / 3.0, p6 = 0.5 * p3, tt = 2.0 * p3;
+    const double c1 = 1.8, c1p = 9.0 * (c1 - 1.0) / 4.0, c1p3 = c1p * p3, c1psq = c1p * c1p;
+    const double pa0 = 0., wir = 0.;
+    const double *tts = F(b, FF_TTS), *sij1 = F(b, FF_S11), *sij2 = F(b, FF_S12),
+                 *sij3 = F(b, FF_S13), *sij4 = F(b, FF_S22), *sij5 = F(b, FF_S23),
+                 *sij6 = F(b, FF_S33), *wij1 = F(b, FF_W12), *wij2 = F(b, FF_W13),
+                 *wij3 = F(b, FF_W23);
+    double *b1 = F(b, FF_B11), *b2 = F(b, FF_B12), *b3 = F(b, FF_B13), *b4 = F(b, FF_B22),
+           *b5 = F(b, FF_B23), *b6 = F(b, FF_B33);
+    for (int k = 1; k <= d.kmax; k++)
+        for (int j = 1; j <= d.jmax; j++)
+            for (int i = 1; i <= d.imax; i++) {
+                long L = cid(&d, i, j, k);
+                double s11 = tts[L] * sij1[L], s22 = tts[L] * sij4[L], s33 = tts[L] * sij6[L];
+                double s12 = tts[L] * sij2[L], s13 = tts[L] * sij3[L], s23 = tts[L] * sij5[L];
+                double w12 = tts[L] * wij1[L] - pa0 * wir;
+                double w13 = tts[L] * wij2[L] + pa0 * wir;
+                double w23 = tts[L] * wij3[L] - pa0 * wir;
+                double s11sq = s11 * s11, s22sq = s22 * s22, s33sq = s33 * s33;
+                double s12sq = s12 * s12, s13sq = s13 * s13, s23sq = s23 * s23;
+                double w12sq = w12 * w12, w13sq = w13 * w13, w23sq = w23 * w23;
+                double sii = s11sq + s22sq + s33sq + 2.0 * (s12sq + s13sq + s23sq);
+                double wii = -(2.0 * (w12sq + w13sq + w23sq));
+                double wiip3 = p3 * wii;
+                double swwiv = -(s11 * (w12sq + w13sq)) - s22 * (w12sq + w23sq)
+                             - s33 * (w13sq + w23sq)
+                             + 2.0 * (-(s12 * w13 * w23) + s13 * w12 * w23 - s23 * w12 * w13);
+                double swwivtt = tt * swwiv;
+                double sswwv = 2.0 * (-((s12 * s13 + s22 * s23 + s23 * s33) * w12 * w13)
+                                      + (s11 * s13 + s12 * s23 + s13 * s33) * w12 * w23
+                                      - (s11 * s12 + s12 * s22 + s13 * s23) * w13 * w23)
+                             - (s11sq + s12sq + s13sq) * (w12sq + w13sq)
+                             - (s12sq + s22sq + s23sq) * (w12sq + w23sq)
+                             - (s13sq + s23sq + s33sq) * (w13sq + w23sq);
+                double t3c11 = -w12sq - w13sq - wiip3;
+                double t3c22 = -w12sq - w23sq - wiip3;
+                double t3c12 = -(w13 * w23);
+                double t3c13 = w12 * w23;
+                double t3c23 = -(w12 * w13);
+                double t4c11 = -(2.0 * (s12 * w12 + s13 * w13));
+                double t4c22 = 2.0 * (s12 * w12 - s23 * w23);
+                double t4c12 = (s11 - s22) * w12 - s23 * w13 - s13 * w23;
+                double t4c13 = -(s23 * w12) + (s11 - s33) * w13 + s12 * w23;
+                double t4c23 = s13 * w12 + s12 * w13 + (s22 - s33) * w23;
+                double t6c11 = -(2.0 * ((s12 * w13 - s13 * w12) * w23 + s11 * (w12sq + w13sq)))
+                             - swwivtt - wii * s11;
+                double t6c22 = -(2.0 * ((s23 * w12 + s12 * w23) * w13 + s22 * (w12sq + w23sq)))
+                             - swwivtt - wii * s22;
+                double t6c12 = -(s12 * (2.0 * w12sq + w13sq + w23sq))
+                             - (s13 * w13 - s23 * w23) * w12 - (s11 + s22) * w13 * w23 - wii * s12;
+                double t6c13 = -(s13 * (w12sq + 2.0 * w13sq + w23sq))
+                             - (s12 * w12 + s23 * w23) * w13 + (s11 + s33) * w12 * w23 - wii * s13;
+                double t6c23 = -(s23 * (w12sq + w13sq + 2.0 * w23sq))
+                             + (s12 * w12 - s13 * w13) * w23 - (s22 + s33) * w12 * w13 - wii * s23;
+                double t9c11 = -(2.0 * ((s12 * w12 + s13 * w13 - s23 * w23) * w12sq
+                                        + (s12 * w12 + s13 * w13 + s23 * w23) * w13sq
+                                        + (s22 - s33) * w12 * w13 * w23));
+                double t9c22 = -(2.0 * ((-(s12 * w12) - s13 * w13 + s23 * w23) * w12sq
+                                        + (-(s12 * w12) + s13 * w13 + s23 * w23) * w23sq
+                                        + (-s11 + s33) * w12 * w13 * w23));
+                double t9c12 = ((s11 - s22) * w12 - 2.0 * (s13 * w23 + s23 * w13)) * w12sq
+                             + ((s11 - s33) * w12 - 2.0 * s13 * w23) * w13sq
+                             + ((s33 - s22) * w12 - 2.0 * s23 * w13) * w23sq;
+                double t9c13 = ((s11 - s22) * w13 + 2.0 * s12 * w23) * w12sq
+                             + ((s11 - s33) * w13 + 2.0 * (s12 * w23 - s23 * w12)) * w13sq
+                             + ((s22 - s33) * w13 - 2.0 * s23 * w12) * w23sq;
+                double t9c23 = ((s22 - s11) * w23 + 2.0 * s12 * w13) * w12sq
+                             + ((s11 - s33) * w23 + 2.0 * s13 * w12) * w13sq
+                             + ((s22 - s33) * w23 + 2.0 * (s12 * w13 + s13 * w12)) * w23sq;
+                /* N_c from the cubic (earsm.f:170-184) */
+                double p1 = (c1psq / 27.0 + 0.45 * sii - tt * wii) * c1p;
+                double cb = c1psq * (p3 * p3) + 0.9 * sii + tt * wii;
+                double p2 = p1 * p1 - cb * cb * cb;
+                double fnc;
+                if (p2 >= 0.0) {
+                    double sqp2 = sqrt(p2);
+                    double pm = p1 - sqp2;
+                    double pmp = pow(fabs(pm), p3);
+                    fnc = c1p3 + pow(p1 + sqp2, p3) + copysign(fabs(pmp), pm);
+                } else {
+                    double pm = p1 * p1 - p2;
+                    double sqpm = sqrt(pm);
+                    double alpo = p1 / sqpm;
+                    alpo = dmin(alpo, 1.);
+                    alpo = dmax(alpo, -1.);
+                    double facos = p3 * acos(alpo);
+                    fnc = c1p3 + 2.0 * pow(pm, p6) * cos(facos);
+                }
+                /* the N correction (earsm.f:187-192) */
+                double dd = 20.0 * pow4(fnc) * (fnc - 0.5 * c1p)
+                          - wii * (10.0 * fnc + 15.0 * c1p) * (fnc * fnc)
+                          + 10.0 * c1p * (wii * wii);
+                double fii1 = swwiv * swwiv;
+                double fii2 = sswwv - 0.5 * sii * wii;
+                double fn = fnc + 162.0 * (fii1 + fii2 * (fnc * fnc)) / dd;
+                double fnsq = fn * fn;
+                double q = 5.0 * p6 * (fnsq - 2.0 * wii) * (2.0 * fnsq - wii);
+                double pq = 1.0 / q;
+                double beta3 = -(pq * 12.0 * swwiv / fn);
+                double beta4 = -(pq * 2.0 * (fnsq - 2.0 * wii));
+                double beta6 = -(pq * 6.0 * fn);
+                double beta9 = pq * 6.0;
+                b1[L] = 0.5 * (beta3 * t3c11 + beta4 * t4c11 + beta6 * t6c11 + beta9 * t9c11);
+                b2[L] = 0.5 * (beta3 * t3c12 + beta4 * t4c12 + beta6 * t6c12 + beta9 * t9c12);
+                b3[L] = 0.5 * (beta3 * t3c13 + beta4 * t4c13 + beta6 * t6c13 + beta9 * t9c13);
+                b4[L] = 0.5 * (beta3 * t3c22 + beta4 * t4c22 + beta6 * t6c22 + beta9 * t9c22);
+                b5[L] = 0.5 * (beta3 * t3c23 + beta4 * t4c23 + beta6 * t6c23 + beta9 * t9c23);
+                b6[L] = -b1[L] - b4[L];
             }
 }
 
@@ -873,6 +1015,40 @@ static void prodke(const ff_params *p, ff_block *b, const double *trueps)
     }
 }
 
+/* PEEKOO earsm.f:553-626: the production with the EARSM Reynolds stresses, PHITUR = VTRAN
+ * = 1.  Like the reference's loop it covers, in each interior K-plane, the cells from
+ * (1,1) to (IMAX,JMAX) in storage order (ghost columns between the rows included). */
+static void peekoo(ff_block *b)
+{
+    dims_t d = mkdims(b);
+    const double p3 = 1.0 / 3.0, eps = 1.E-20;
+    const double *s1 = F(b, FF_S11), *s2 = F(b, FF_S12), *s3 = F(b, FF_S13),
+                 *s4 = F(b, FF_S22), *s5 = F(b, FF_S23), *s6 = F(b, FF_S33),
+                 *b1 = F(b, FF_B11), *b2 = F(b, FF_B12), *b3 = F(b, FF_B13),
+                 *b4 = F(b, FF_B22), *b5 = F(b, FF_B23), *eps2 = F(b, FF_EPS2),
+                 *vis = F(b, FF_VIS), *rk = F(b, FF_RK);
+    double *ptur = F(b, FF_PTUR);
+    const long ixcl = (long)G * d.is + G;
+    for (int k = 1; k <= d.kmax; k++) {
+        long ia = (long)(k + 1) * d.il;
+        for (long ij = ixcl; ij <= d.il - ixcl - 1; ij++) {
+            long L = ia + ij;
+            double s11 = s1[L], s22 = s4[L], s33 = s6[L], s12 = s2[L], s13 = s3[L], s23 = s5[L];
+            double rmut = dmax(eps2[L] - 1.0, eps) * vis[L];
+            double divvp3 = p3 * (s11 + s22 + s33);
+            double huu = dmax((rk[L] * (p3 + b1[L]) - rmut * (s11 - divvp3)), 0.0);
+            double hvv = dmax((rk[L] * (p3 + b4[L]) - rmut * (s22 - divvp3)), 0.0);
+            double hww = dmax((rk[L] * (p3 - b1[L] - b4[L]) - rmut * (s33 - divvp3)), 0.0);
+            double huv = rk[L] * b2[L] - rmut * s12;
+            double huw = rk[L] * b3[L] - rmut * s13;
+            double hvw = rk[L] * b5[L] - rmut * s23;
+            double pt = -(2.0 * (huu * s11 + hvv * s22 + hww * s33
+                                 + 2.0 * (huv * s12 + huw * s13 + hvw * s23)));
+            ptur[L] = pt * 1.0;
+        }
+    }
+}
+
 /* SOURKO turb3.f:7624-7784 (RANS branch, KOVER 0/5: no rotation correction, FCO = 1;
  * TUR_FRS_SOURCE = no, main.f:5267) on the interior: SRK, and SEPS = (1-F1)*SEPS + the
  * omega source, SEPS holding the cross-diffusion on entry */
@@ -916,6 +1092,7 @@ void ffo_turb(const ff_params *p, ff_block *b)
     if (p->iturb == 6) {
         derfun(b, 1);
         ttimes(b);
+        if (p->istres == 1) earsm1(b);   /* res3c.f:5987-5990 */
         eps2ko(p, b);
         return;
     }
@@ -960,7 +1137,10 @@ void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *r
         fun1ko(p, &s, b);
         double *trueps = malloc(sizeof(double) * (size_t)d.ntot);
         make_eps(b, trueps);
-        prodke(p, b, trueps);
+        if (p->istres >= 1)
+            peekoo(b);                    /* res3c.f:3976-3978 */
+        else
+            prodke(p, b, trueps);
         free(trueps);
     }
     direction_sweep(p, b, 2, 0, kwm ? sig : NULL);   /* res3c.f:270 / 4018, SWEEP(K) */

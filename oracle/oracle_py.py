@@ -148,7 +148,8 @@ class RefBlock:
         # RESIDI takes first order (INTI = 55) on the coarse levels (ns3c.f:2467-2472)
         inter = list(prm.inter) if level == 1 else [55, 55, 55]
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
-                            inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1],
+                            inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1,
+                            prm.istres],
                            dtype=np.int32)
 
     def _a(self):

@@ -5,13 +5,14 @@
 ! product links it.  Each entry point follows the reference's call sequence for one stage:
 !   ref_state     STATIM ns3c.f:10318-10447: PDTOP, STATE->PERFGA (state.f:159), CONSVA
 !   ref_turb      TURBDI ns3c.f:2987-2991 (ITURB=0): EPS2=1, VIST=0; ITURB=6: TURBEQ
-!                 res3c.f:5951-6371: DERFUN, TTIMES, EPS2KO
+!                 res3c.f:5951-6371: DERFUN, TTIMES, (ISTRES=1) EARSM1, EPS2KO
 !   ref_residual  RESIDU res3c.f:194-559: ZEROZZ, then per direction k,i,j the FLUCYC
 !                 sequence res3c.f:7705-7907 (ZEROZZ of F1*, FLUXDO, FLUXKE or FLUXVL)
 !                 and SWEEP;
 !                 RESL2 res3c.f:8920 on level 1; ITURB=6: the RESIKE sequence
 !                 res3c.f:3700-5068 (CROSKO, FUN1KO, MAKE_EPS, PRODKE, FLXCYC with FLUDKE,
-!                 SETV12, FLUXDO, FLUXKE; SWEEPK; SOURKO, ADDSKE)
+!                 SETV12, FLUXDO, FLUXKE; SWEEPK; SOURKO, ADDSKE); with ISTRES >= 1 PEEKOO
+!                 in place of PRODKE and ANISFLUX after FLUXDO
 !   ref_implicit  IMPS imps3.f:111-483 with LUSGS=1: TIME3, LOCDT, COPR, PREDGS, DIAGGS,
 !                 CORRGS, DIVV12; ITURB=6: LOCDTK, IMSOKO, COPRKE, PRKEGS, DIKEGS,
 !                 COKEGS, DIVV12, MULV12
@@ -44,6 +45,7 @@ module ffref_ids
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
   integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12, QM2=13
+  integer, parameter :: QISTRES=14
 end module ffref_ids
 
 subroutine ref_state(f, ntot, prm, ipr) bind(C, name="ref_state")
@@ -86,7 +88,8 @@ subroutine ref_turb(f, ntot, prm, ipr) bind(C, name="ref_turb")
   integer(c_int) :: ipr(*)
   real(c_double), allocatable :: vellap(:), rksi(:)
   real(c_double) :: a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2
-  real(c_double) :: sigphi, zeta2, frsmut
+  real(c_double) :: sigphi, zeta2, frsmut, eps4
+  real(c_double), allocatable :: wir(:)
   integer :: kover
   if (ipr(QITURB) == 6) then
      ! TURBEQ (res3c.f:5922, 5951-5954, 5984, 6362-6371), KOVER = 0: EPS2KO with STRAIN
@@ -102,6 +105,16 @@ subroutine ref_turb(f, ntot, prm, ipr) bind(C, name="ref_turb")
           f(1,ISTRAIN), vellap)
      call TTIMES(f(1,IRO), f(1,IVIS), f(1,IRK), f(1,IREPS), 6, ipr(QIMAX), ipr(QJMAX), &
           ipr(QKMAX), f(1,ITTS))
+     if (ipr(QISTRES) == 1) then
+        ! res3c.f:5987-5990; WIR is never given a value (PA0 = 0, earsm.f:47)
+        allocate(wir(3*ntot))
+        wir = 0.
+        eps4 = 0.
+        call EARSM1(f(1,IVIS), f(1,IVIST), f(1,IEPS2), eps4, f(1,IRK), wir, f(1,IW12), &
+             f(1,IS11), f(1,IB11), ipr(QIMAX), ipr(QJMAX), ipr(QKMAX), ntot, ntot, f(1,ITTS), &
+             prm(PPR), prm(PPRT), prm(PTURLIM), frsmut)
+        deallocate(wir)
+     end if
      call COEFKO(6, a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2, &
           sigphi, zeta2, kover)
      call EPS2KO(f(1,IEPS2), f(1,IVIST), f(1,IVIS), f(1,ISTRAIN), f(1,IRO), f(1,IRK), &
@@ -247,6 +260,7 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
   type(PRE_COR) :: prc(1)
   real(c_double) :: f1h(1), wave(1)
   integer :: imax, jmax, kmax, is, js, kstr, kmaxp, ia, iax, iay, iaz, inter, intet, ndi, ido
+  integer :: jadd, kadd
   logical :: xxtral, inchiml, entropy_fix, transl, mulphl, fresul
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
   is = imax + 4; js = jmax + 4
@@ -276,6 +290,15 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
        f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), prm(PFRSPRE), &
        hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, mulphl, 1, fresul, 0._c_double, f1h, &
        wave, ipr(QLEVEL), ipr(QIDIFF))
+  if (ipr(QISTRES) >= 1) then
+     ! FLXCYC res3c.f:6933-6937 (JADD/KADD of res3c.f:6565-6579)
+     jadd = 0; kadd = 0
+     if (idir == 2) jadd = 1
+     if (idir == 3) kadd = 1
+     call ANISFLUX(imax, jmax, kmax, jadd, kadd, kstr, f(1,IU), f(1,IV), f(1,IW), f(1,IRK), &
+          f(1,IB11), f(1,ia), f(1,iax), f(1,iay), f(1,iaz), fl(1,2), fl(1,3), fl(1,4), &
+          fl(1,5), ntot)
+  end if
   hat = 0.
   call FLUXKE(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), &
        f(1,ia), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), f(1,IP), &
@@ -352,10 +375,16 @@ subroutine ref_residual_kw(f, ntot, prm, ipr, resi)
   ! MAKE_EPS, PRODKE, ZEROZZ(DDEPS) (res3c.f:3949-3994)
   trueps = 0.
   call MAKE_EPS(f(1,IRO), f(1,IRK), f(1,IREPS), ddeps, trueps, 6, ntot)
-  call PRODKE(f(1,IOHMI), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), &
-       f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IPTUR), f(1,IRK), f(1,IREPS), ddeps, &
-       trueps, f(1,IEPS2), f(1,IVIST), f(1,IVIS), rst, rst, rst, rst, rst, rst, vtran, &
-       imax, jmax, kmax, 2, 2, 2, 1, 6, katol)
+  if (ipr(QISTRES) >= 1) then
+     ! res3c.f:3976-3978
+     call PEEKOO(f(1,IVIS), f(1,IEPS2), f(1,IS11), f(1,IB11), vtran, imax, jmax, kmax, &
+          f(1,IRK), f(1,IPTUR), ntot, ntot)
+  else
+     call PRODKE(f(1,IOHMI), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), &
+          f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IPTUR), f(1,IRK), f(1,IREPS), ddeps, &
+          trueps, f(1,IEPS2), f(1,IVIST), f(1,IVIS), rst, rst, rst, rst, rst, rst, vtran, &
+          imax, jmax, kmax, 2, 2, 2, 1, 6, katol)
+  end if
   call ZEROZZ(ddeps, ntot)
   call ref_direction_kw(f, ntot, prm, ipr, 3, fl, zzz, maw, maxw, rksi, hat, zero, trm, sig)
   call ref_direction_kw(f, ntot, prm, ipr, 1, fl, zzz, maw, maxw, rksi, hat, zero, trm, sig)

@@ -36,6 +36,35 @@ SST_CASES = [
      dict(kind="all", amp=0.08, distort=0.12, seed=37, turb=(0.3, 80.0))),
 ]
 
+# Wallin-Johansson EARSM on top of the SST (ISTRES = 1): EARSM1's cubic takes both of its
+# branches on these cases (P2 >= 0 where rotation dominates, P2 < 0 under strain), and the
+# ghost cells carry seeded b_ij so that ANISFLUX's boundary faces see non-zero stresses
+EARSM_CASES = [
+    ("earsm_curv", (10, 7, 6), dict(iturb=6, istres=1),
+     dict(kind="all", amp=0.05, distort=0.15, seed=61, turb=(1.0, 50.0))),
+    ("earsm_wall", (9, 8, 5), dict(iturb=6, istres=1, re=1e14, cfl=2.0),
+     dict(kind="velocity", amp=0.03, jratio=1.3, seed=67, turb=(0.5, 400.0))),
+    ("earsm_fullprod", (7, 6, 8), dict(iturb=6, istres=1, inter=(3, 2, 55)),
+     dict(kind="all", amp=0.08, distort=0.12, seed=71, turb=(0.3, 80.0))),
+]
+BIJ = ["B11", "B12", "B13", "B22", "B23", "B33"]
+
+
+def seed_bij(blk, seed=0, amp=0.05):
+    """Seeded b_ij on every cell (trace-free: B33 = -B11 - B22)."""
+    rng = np.random.default_rng(seed)
+    for name in BIJ[:5]:
+        blk[name] = amp * rng.uniform(-1, 1, blk.ntot)
+    blk["B33"] = -blk["B11"] - blk["B22"]
+    return blk
+
+
+def make_earsm_case(dims, pover, ckw):
+    prm, blk = make_case(dims, pover, ckw)
+    seed_bij(blk, seed=ckw.get("seed", 0) + 500)
+    return prm, blk
+
+
 # fields each stage writes, for the stage-by-stage comparisons
 STATE = ["P", "RO", "E", "C", "VIS", "CH", "CP", "DRDP", "DRDH", "RM", "RN", "RW"]
 RES = ["DRO", "DM", "DN", "DW", "DE"]

@@ -8,8 +8,8 @@ import numpy as np
 import pytest
 
 from oracle.oracle_py import OracleBlock
-from tests.parity import (CASES, PRIM, RES, SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE,
-                          assert_fields_equal, make_case)
+from tests.parity import (BIJ, CASES, EARSM_CASES, PRIM, RES, SST_CASES, SST_PRIM, SST_RES,
+                          SST_TURB, STATE, assert_fields_equal, make_case, make_earsm_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -191,4 +191,47 @@ def test_sst_multi_tile(hip_device, dims, lusgs):
     ctx.cycle(ncycles=2)
     ctx.download()
     assert_fields_equal(g, a, SST_PRIM, interior=False, rtol=SST_RTOL, what=f"SST {dims}")
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", EARSM_CASES, ids=[c[0] for c in EARSM_CASES])
+def test_earsm_stages(hip_device, name, dims, pover, ckw):
+    """SST + Wallin-Johansson EARSM stage by stage; each GPU stage starts from the oracle's
+    state of the previous stage.  EARSM1's cube roots (pow), ACOS and COS, like the SST's
+    tanh, round differently on the device in the last bit: compared to 1e-12."""
+    print(f"case {name} dims {dims}", flush=True)
+    prm, blk = make_earsm_case(dims, pover, ckw)
+    a, g = blk.copy(), blk.copy()
+    oa = OracleBlock(prm, a)
+    ctx = _ctx(prm, g)
+    oa.save_old(); ctx.save_old()
+    oa.state(); ctx.statim()
+    oa.turb(); ctx.turbdi(); ctx.download()
+    assert_fields_equal(g, a, BIJ, interior=False, rtol=SST_RTOL, what="EARSM1")
+    g.F[:] = a.F
+    ctx.upload()
+    r1 = oa.residual(); r2 = ctx.residi(); ctx.download()
+    assert_fields_equal(g, a, SST_RES, rtol=SST_RTOL, what="RESIKE with PEEKOO and ANISFLUX")
+    np.testing.assert_allclose(r2, r1, rtol=1e-11)
+    g.F[:] = a.F
+    ctx.upload()
+    oa.implicit(); ctx.impsdi(); ctx.download()
+    assert_fields_equal(g, a, SST_RES[:7], what="IMPS")
+    ctx.close()
+
+
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
+@pytest.mark.parametrize("name,dims,pover,ckw", EARSM_CASES[:2], ids=[c[0] for c in EARSM_CASES[:2]])
+def test_earsm_cycles(hip_device, name, dims, pover, ckw, lusgs):
+    print(f"case {name} dims {dims} 5 EARSM cycles, LU-SGS {lusgs}", flush=True)
+    prm, blk = make_earsm_case(dims, pover, ckw)
+    a, g = blk.copy(), blk.copy()
+    oa = OracleBlock(prm, a)
+    ctx = _ctx(prm, g, lusgs)
+    for _ in range(5):
+        r1 = oa.cycle()
+    r2 = ctx.cycle(ncycles=5)
+    ctx.download()
+    assert_fields_equal(g, a, SST_PRIM + BIJ, interior=False, rtol=SST_RTOL, what="5 EARSM cycles")
+    np.testing.assert_allclose(r2, r1, rtol=1e-10)
     ctx.close()

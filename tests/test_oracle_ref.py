@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 from oracle.oracle_py import OracleBlock, RefBlock, have_ref
-from tests.parity import (CASES, PRIM, RES, SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE,
+from tests.parity import (BIJ, CASES, EARSM_CASES, PRIM, RES, SST_CASES, SST_PRIM, SST_RES,
+                          SST_TURB, STATE, make_earsm_case,
                           assert_fields_equal, make_case)
 
 pytestmark = pytest.mark.skipif(not have_ref(), reason="oracle/_ref not built (needs /root/reference)")
@@ -127,3 +128,36 @@ def test_sst_fine_level_second_residual(name, dims, pover, ckw):
     assert_fields_equal(a, b, SST_TURB[:11], interior=False, what="DERFUN level 1")
     assert_fields_equal(a, b, SST_RES, what="RESIKE M2=2")
     assert np.array_equal(r1, r2)
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", EARSM_CASES, ids=[c[0] for c in EARSM_CASES])
+def test_earsm_stages_bit_exact(name, dims, pover, ckw):
+    """SST with the Wallin-Johansson EARSM (ISTRES = 1): EARSM1 in TURBEQ, PEEKOO and
+    ANISFLUX in RESIKE, then IMPS and FIN, stage by stage against the reference's Fortran."""
+    prm, blk = make_earsm_case(dims, pover, ckw)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    oa.save_old(); rb.save_old()
+    oa.state(); rb.state()
+    oa.turb(); rb.turb()
+    assert_fields_equal(a, b, SST_TURB, what="TURBEQ")
+    assert_fields_equal(a, b, BIJ, interior=False, what="EARSM1")
+    assert np.abs(a.interior("B12")).max() > 0, "EARSM1 wrote no anisotropy"
+    r1, r2 = oa.residual(), rb.residual()
+    assert_fields_equal(a, b, SST_RES, what="RESIKE with PEEKOO and ANISFLUX")
+    assert np.array_equal(r1, r2), "RESL2 sums differ"
+    oa.implicit(); rb.implicit()
+    assert_fields_equal(a, b, ["DTL"] + SST_RES[:7], what="IMPS")
+    oa.update(); rb.update()
+    assert_fields_equal(a, b, SST_PRIM, interior=False, what="FIN")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", EARSM_CASES[:2], ids=[c[0] for c in EARSM_CASES[:2]])
+def test_earsm_cycles_bit_exact(name, dims, pover, ckw):
+    prm, blk = make_earsm_case(dims, pover, ckw)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    for _ in range(4):
+        r1, r2 = oa.cycle(), rb.cycle()
+        assert np.array_equal(r1, r2)
+    assert_fields_equal(a, b, SST_PRIM + BIJ, interior=False, what="4 EARSM cycles")
