@@ -158,16 +158,22 @@ int  ff_finish(ff_ctx *ctx);
  * Sutherland viscosity (PERFGA state.f:159), sound speed and conservative variables
  * (CONSVA base3c.f:489), on every cell including ghosts. */
 int  ff_state(ff_ctx *ctx, int level, int block);
-/* TURBDI (ns3c.f:2874) for ITURB=0: EPS2 = 1, VIST = 0 (ADJUIN ns3c.f:2987-2991). */
+/* TURBDI (ns3c.f:2874).  ITURB=0: EPS2 = 1, VIST = 0 (ADJUIN ns3c.f:2987-2991).  ITURB=6:
+ * TURBEQ (res3c.f:5845) = DERFUN velocity gradients (turb3.f:2476), TTIMES (earsm.f:629),
+ * with ISTRES=1 the EARSM anisotropy BIJ (EARSM1 earsm.f:1), and the SST eddy viscosity
+ * EPS2KO (turb3.f:7454). */
 int  ff_turb(ff_ctx *ctx, int level, int block);
 /* RESIDI (ns3c.f:2355) -> RESIDU (res3c.f:6): viscous FLUXDO plus the inviscid flux of
  * params.iflux (1: Roe FLUXKE base3c.f:4753; 0: van Leer FLUXVL base3c.f:1733) in the k, i,
- * j directions, SWEEP differences, division by VOL.  With level == 1 the L2
+ * j directions, SWEEP differences, division by VOL.  ITURB=6: RESIKE (res3c.f:3423) with
+ * CROSKO, FUN1KO, PRODKE (PEEKOO and the ANISFLUX stress fluxes with ISTRES=1), FLXCYC's
+ * FLUDKE/FLUXDO/FLUXKE, SWEEPK, SOURKO and ADDSKE.  With level == 1 the L2
  * sums of RESL2 (res3c.f:8920) are returned in resi[0..6] (DRO, DM, DN, DW, DE, DRK, DEPS
  * squared sums; the caller takes sqrt()/N as SUBIN does); resi may be NULL. */
 int  ff_residual(ff_ctx *ctx, int level, int block, int m2, double *resi);
 /* IMPSDI (ns3c.f:2683) -> IMPS (imps3.f:6) with LUSGS=1: TIME3, LOCDT, COPR, the
- * backward PREDGS, DIAGGS and forward CORRGS sweeps, DRO/CP. */
+ * backward PREDGS, DIAGGS and forward CORRGS sweeps, DRO/CP.  ITURB=6: LOCDTK, IMSOKO,
+ * COPRKE, PRKEGS/DIKEGS/COKEGS and D(rho k), D(rho omega) times RO. */
 int  ff_implicit(ff_ctx *ctx, int level, int block, int m2);
 /* FIN (res3c.f:1077) + the limiters of CYCLES (ns3c.f:1553-1596): UPLIM/LOLIM of TEMP,
  * U, V, W, PDTOP, LOLIMP. */
