@@ -347,6 +347,65 @@ __global__ void __launch_bounds__(256) k_derfun(Blk b, int g0)
     derfun_cell(b, cid(b, i, j, k));
 }
 
+// TEMDER turb3.f:3456-3559, IDERI = 1, of the cell scalar q (gq picks it) at cell L: the
+// distance-weighted Green-Gauss gradient
+template <typename Q>
+__device__ __forceinline__ void temder_cell(const Blk &b, int L, Q gq, double &gx, double &gy,
+                                            double &gz)
+{
+    const int IS = b.is, IL = b.il;
+    const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
+    const double *a1 = FLD(b, FF_A1), *a2 = FLD(b, FF_A2), *a3 = FLD(b, FF_A3);
+    const double qc = gq(L);
+    double w1, w2;
+    w1 = 2.0 * (d1[L] / (d1[L] + d1[L - 1])); w2 = 2.0 - w1;
+    const double t1m = w2 * qc + w1 * gq(L - 1);
+    w1 = 2.0 * (d1[L] / (d1[L] + d1[L + 1])); w2 = 2.0 - w1;
+    const double t1p = w2 * qc + w1 * gq(L + 1);
+    w1 = 2.0 * (d2[L] / (d2[L] + d2[L - IS])); w2 = 2.0 - w1;
+    const double t2m = w2 * qc + w1 * gq(L - IS);
+    w1 = 2.0 * (d2[L] / (d2[L] + d2[L + IS])); w2 = 2.0 - w1;
+    const double t2p = w2 * qc + w1 * gq(L + IS);
+    w1 = 2.0 * (d3[L] / (d3[L] + d3[L - IL])); w2 = 2.0 - w1;
+    const double t3m = w2 * qc + w1 * gq(L - IL);
+    w1 = 2.0 * (d3[L] / (d3[L] + d3[L + IL])); w2 = 2.0 - w1;
+    const double t3p = w2 * qc + w1 * gq(L + IL);
+    const double pvol = .5 / FLD(b, FF_VOL)[L];
+    double g[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double *n1 = FLD(b, FF_A1X + c), *n2 = FLD(b, FF_A2X + c), *n3 = FLD(b, FF_A3X + c);
+        g[c] = pvol * (a3[L + IL] * n3[L + IL] * t3p - a3[L] * n3[L] * t3m
+                       + a2[L + IS] * n2[L + IS] * t2p - a2[L] * n2[L] * t2m
+                       + a1[L + 1] * n1[L + 1] * t1p - a1[L] * n1[L] * t1m);
+    }
+    gx = g[0]; gy = g[1]; gz = g[2];
+}
+
+// The FULLNS gradients on level 1, first ghost layer included (the K ghost layer only when
+// KMAX > 1, turb3.f:3498-3507): TEMDER of T (RESIDU res3c.f:257-259, RESIKE
+// res3c.f:3991-3993) and, KW, of k/rho and omega/rho (DIVV12 + TEMDER res3c.f:3995-4005;
+// the quotients are formed per neighbour, the same division as DIVV12's).  Thread per cell.
+template <bool KW>
+__global__ void __launch_bounds__(256) k_temder(Blk b, int gk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z + 1 - gk;
+    if (i > b.imax + 1) return;
+    const int L = cid(b, i, j, k);
+    const double *t = FLD(b, FF_TEMP);
+    double gx, gy, gz;
+    temder_cell(b, L, [=](int n) { return t[n]; }, gx, gy, gz);
+    FLD(b, FF_DTDX)[L] = gx; FLD(b, FF_DTDY)[L] = gy; FLD(b, FF_DTDZ)[L] = gz;
+    if (KW) {
+        const double *ro = FLD(b, FF_RO), *rk = FLD(b, FF_RK), *reps = FLD(b, FF_REPS);
+        temder_cell(b, L, [=](int n) { return rk[n] / ro[n]; }, gx, gy, gz);
+        FLD(b, FF_DKDX)[L] = gx; FLD(b, FF_DKDY)[L] = gy; FLD(b, FF_DKDZ)[L] = gz;
+        temder_cell(b, L, [=](int n) { return reps[n] / ro[n]; }, gx, gy, gz);
+        FLD(b, FF_DEDX)[L] = gx; FLD(b, FF_DEDY)[L] = gy; FLD(b, FF_DEDZ)[L] = gz;
+    }
+}
+
 // EXTEND (turb3.f:3342-3450) of OHMI, W12 and SIJ on a coarse level: the interior value
 // next to each face copied into the first ghost layer (no edges or corners)
 __global__ void __launch_bounds__(256) k_extend(Blk b)
@@ -724,8 +783,8 @@ __device__ __forceinline__ void sourko_cell(const Prm &p, const Blk &b, int L, d
 
 // RESIKE face fluxes (FLXCYC res3c.f:6424-7566 for k, i, j with SWEEPK) and SOURKO/ADDSKE
 // (res3c.f:4957-5068), with every face flux formed once: the layout of k_residual_tile with
-// seven components.
-template <int RIW, int RJW, int RKW>
+// seven components.  FNS: the FULLNS face fluxes FLKEBL + FLUXBL (level 1).
+template <int RIW, int RJW, int RKW, bool FNS = false>
 __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
 {
     __shared__ double fi[7][RJW][RIW + 1];
@@ -740,26 +799,27 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
     const double *a3 = FLD(b, FF_A3), *a3x = FLD(b, FF_A3X), *a3y = FLD(b, FF_A3Y), *a3z = FLD(b, FF_A3Z);
     const double *a1 = FLD(b, FF_A1), *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z);
     const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
+    const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
     Flux7 klo;
-    if (cell) klo = face_flux_kw<true>(p, sc, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z);
+    if (cell) klo = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
     for (int k = k0; k <= k1; k++) {
         const int n = cid(b, i, j, k);
         Flux7 khi;
         if (cell) {
-            khi = face_flux_kw<true>(p, sc, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z);
-            Flux7 f = face_flux_kw<false>(p, sc, b, n, 1, p.inter[0], a1, a1x, a1y, a1z);
+            khi = face_flux_kw<true, FNS>(p, sc, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, n, 1, p.inter[0], a1, a1x, a1y, a1z, d1);
 #pragma unroll
             for (int q = 0; q < 7; q++) fi[q][tj][ti] = f.f[q];
-            f = face_flux_kw<false>(p, sc, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z);
+            f = face_flux_kw<false, FNS>(p, sc, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
 #pragma unroll
             for (int q = 0; q < 7; q++) fj[q][tj][ti] = f.f[q];
         }
         if (tj == 0 && ti < il) {
-            Flux7 f = face_flux_kw<false>(p, sc, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z);
+            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
 #pragma unroll
             for (int q = 0; q < 7; q++) fj[q][jl][ti] = f.f[q];
         } else if (tj == 1 && ti < jl) {
-            Flux7 f = face_flux_kw<false>(p, sc, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z);
+            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z, d1);
 #pragma unroll
             for (int q = 0; q < 7; q++) fi[q][ti][il] = f.f[q];
         }
@@ -793,7 +853,7 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
 // in the reference's order (SWEEP after the k, i and j directions).  Both faces of the
 // cell are evaluated by the same code as the neighbour evaluates them, so the shared face
 // values agree bit for bit.
-template <int IFLUX>
+template <int IFLUX, bool FNS = false>
 __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -804,28 +864,28 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
     double d[5] = {0., 0., 0., 0., 0.};
     {   // k-direction (res3c.f:270, SWEEP res3c.f:330)
         const int S = b.il;
-        Flux5 lo = face_flux<true, IFLUX>(p, b, n, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
-                                   FLD(b, FF_A3Y), FLD(b, FF_A3Z));
-        Flux5 hi = face_flux<true, IFLUX>(p, b, n + S, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
-                                   FLD(b, FF_A3Y), FLD(b, FF_A3Z));
+        Flux5 lo = face_flux<true, IFLUX, FNS>(p, b, n, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
+                                   FLD(b, FF_A3Y), FLD(b, FF_A3Z), FLD(b, FF_D3));
+        Flux5 hi = face_flux<true, IFLUX, FNS>(p, b, n + S, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
+                                   FLD(b, FF_A3Y), FLD(b, FF_A3Z), FLD(b, FF_D3));
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
     }
     {   // i-direction (res3c.f:402, SWEEP res3c.f:448)
         const int S = 1;
-        Flux5 lo = face_flux<false, IFLUX>(p, b, n, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
-                                    FLD(b, FF_A1Y), FLD(b, FF_A1Z));
-        Flux5 hi = face_flux<false, IFLUX>(p, b, n + S, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
-                                    FLD(b, FF_A1Y), FLD(b, FF_A1Z));
+        Flux5 lo = face_flux<false, IFLUX, FNS>(p, b, n, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
+                                    FLD(b, FF_A1Y), FLD(b, FF_A1Z), FLD(b, FF_D1));
+        Flux5 hi = face_flux<false, IFLUX, FNS>(p, b, n + S, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
+                                    FLD(b, FF_A1Y), FLD(b, FF_A1Z), FLD(b, FF_D1));
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
     }
     {   // j-direction, last: divide by VOL (res3c.f:515, SWEEP res3c.f:559)
         const int S = b.is;
-        Flux5 lo = face_flux<false, IFLUX>(p, b, n, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
-                                    FLD(b, FF_A2Y), FLD(b, FF_A2Z));
-        Flux5 hi = face_flux<false, IFLUX>(p, b, n + S, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
-                                    FLD(b, FF_A2Y), FLD(b, FF_A2Z));
+        Flux5 lo = face_flux<false, IFLUX, FNS>(p, b, n, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
+                                    FLD(b, FF_A2Y), FLD(b, FF_A2Z), FLD(b, FF_D2));
+        Flux5 hi = face_flux<false, IFLUX, FNS>(p, b, n + S, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
+                                    FLD(b, FF_A2Y), FLD(b, FF_A2Z), FLD(b, FF_D2));
         double dt1 = 1. / FLD(b, FF_VOL)[n];
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = (d[q] - (hi.f[q] - lo.f[q])) * dt1;
@@ -847,7 +907,7 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
 constexpr int RI = 64, RJ = 4, RKC = 8;
 static_assert(RJ >= 2 && RJ <= RI, "the tile's upper I-faces are formed by the lanes of row 1");
 
-template <int IFLUX>
+template <int IFLUX, bool FNS = false>
 __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
 {
     __shared__ double fi[5][RJ][RI + 1];
@@ -861,28 +921,29 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
     const double *a3 = FLD(b, FF_A3), *a3x = FLD(b, FF_A3X), *a3y = FLD(b, FF_A3Y), *a3z = FLD(b, FF_A3Z);
     const double *a1 = FLD(b, FF_A1), *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z);
     const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
+    const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
     Flux5 klo;
-    if (cell) klo = face_flux<true, IFLUX>(p, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z);
+    if (cell) klo = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
     for (int k = k0; k <= k1; k++) {
         const int n = cid(b, i, j, k);
         Flux5 khi;
         if (cell) {
-            khi = face_flux<true, IFLUX>(p, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z);
-            Flux5 f = face_flux<false, IFLUX>(p, b, n, 1, p.inter[0], a1, a1x, a1y, a1z);
+            khi = face_flux<true, IFLUX, FNS>(p, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, n, 1, p.inter[0], a1, a1x, a1y, a1z, d1);
 #pragma unroll
             for (int q = 0; q < 5; q++) fi[q][tj][ti] = f.f[q];
-            f = face_flux<false, IFLUX>(p, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z);
+            f = face_flux<false, IFLUX, FNS>(p, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
 #pragma unroll
             for (int q = 0; q < 5; q++) fj[q][tj][ti] = f.f[q];
         }
         // upper faces of the tile: the J-face above row jl-1 (wave 0), the I-face after
         // column il-1 of every row (wave 1)
         if (tj == 0 && ti < il) {
-            Flux5 f = face_flux<false, IFLUX>(p, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z);
+            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
 #pragma unroll
             for (int q = 0; q < 5; q++) fj[q][jl][ti] = f.f[q];
         } else if (tj == 1 && ti < jl) {
-            Flux5 f = face_flux<false, IFLUX>(p, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z);
+            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z, d1);
 #pragma unroll
             for (int q = 0; q < 5; q++) fi[q][ti][il] = f.f[q];
         }
@@ -2394,6 +2455,8 @@ static Prm make_prm(const ff_params *p, int level)
     k.turlim = p->turlim;
     k.iturb = p->iturb;
     k.kover = p->kover; k.katol = p->katol; k.istres = p->istres;
+    k.fullns = level == 1 ? p->fullns : 0;       // FLUCYC/FLXCYC: FULLNS .AND. M == 1
+    k.idiff = p->idiff;
     k.cmgk = p->cmgk; k.cmgeps = p->cmgeps;
     for (int d = 0; d < 3; d++) k.inter[d] = level >= 2 ? 55 : p->inter[d];   // res3c RESIDI ns3c.f:2468-2472
     return k;
@@ -2553,6 +2616,11 @@ static const int kNeededKw[] = {
     FF_RK, FF_REPS, FF_DRK, FF_DEPS, FF_RKOLD, FF_EPSOLD, FF_D1, FF_D2, FF_D3, FF_DISTW,
     FF_SRK, FF_SEPS, FF_PTUR, FF_FUN1, FF_OHMI, FF_W12, FF_W13, FF_W23, FF_S11, FF_S12,
     FF_S13, FF_S22, FF_S23, FF_S33, FF_STRAIN, FF_TTS};
+// and those of FULLNS (DERFUN, TEMDER, FLUXBL; FLKEBL adds GRADK/GREPS)
+static const int kNeededFullns[] = {
+    FF_D1, FF_D2, FF_D3, FF_XC, FF_YC, FF_ZC, FF_OHMI, FF_W12, FF_W13, FF_W23, FF_S11, FF_S12,
+    FF_S13, FF_S22, FF_S23, FF_S33, FF_STRAIN, FF_DTDX, FF_DTDY, FF_DTDZ};
+static const int kNeededFullnsKw[] = {FF_DKDX, FF_DKDY, FF_DKDZ, FF_DEDX, FF_DEDY, FF_DEDZ};
 
 static int check_fields(ff_ctx *c, const ff_block *v, const char *what)
 {
@@ -2564,6 +2632,13 @@ static int check_fields(ff_ctx *c, const ff_block *v, const char *what)
     if (c->prm.istres >= 1)
         for (int f : kNeededEarsm)
             if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
+    if (c->prm.fullns) {
+        for (int f : kNeededFullns)
+            if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
+        if (c->prm.iturb == 6)
+            for (int f : kNeededFullnsKw)
+                if (!v->f[f]) return fail(c, FF_E_ARG, what, std::to_string(f).c_str());
+    }
     return FF_OK;
 }
 
@@ -2578,7 +2653,8 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
     if (prm->iturb == 6 && prm->iflux != 1) return FF_E_UNSUP;   // FLXCYC: Roe only
     if (prm->istres != 0 && (prm->iturb != 6 || prm->istres != 1)) return FF_E_UNSUP;  // EARSM1 on SST
     if (prm->iflux != 1 && prm->iflux != 0) return FF_E_UNSUP;   // Roe FLUXKE, van Leer FLUXVL
-    if (prm->idiff > 3) return FF_E_UNSUP;         // arithmetic FLUXDO average
+    if (prm->idiff > 3) return FF_E_UNSUP;         // arithmetic FLUXDO/FLUXBL average
+    if (prm->fullns != 0 && prm->fullns != 1) return FF_E_UNSUP;
     if (prm->expsu != 1.5) return FF_E_UNSUP;      // Sutherland sqrt branch of PERFGA
     ff_ctx *c = new ff_ctx();
     c->prm = *prm;
@@ -2805,6 +2881,20 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
     }
     const bool kw = c->prm.iturb == 6;
     const int nq = kw ? 7 : 5;
+    const bool fns = level == 1 && c->prm.fullns;   // FLUXBL/FLKEBL on M = 1 only
+    // FULLNS gradients: RESIDU forms DERFUN itself (res3c.f:250-256; the SST has TURBEQ's);
+    // TEMDER after RESIKE's CROSKO rounding of RK/REPS and PRODKE (res3c.f:3989-4007)
+    auto gradients = [&]() -> int {
+        int e;
+        if (!kw) {
+            dim3 g((b.imax + 2 + 255) / 256, b.jmax + 2, b.kmax + 2);
+            if ((e = launch(c, "k_derfun", k_derfun, g, dim3(256), b, 1))) return e;
+        }
+        const int gk = b.kmax > 1 ? 1 : 0;
+        dim3 g((b.imax + 2 + 255) / 256, b.jmax + 2, b.kmax + 2 * gk);
+        return kw ? launch(c, "k_temder", k_temder<true>, g, dim3(256), b, gk)
+                  : launch(c, "k_temder", k_temder<false>, g, dim3(256), b, gk);
+    };
     // ZEROZZ of DRO..DE (DRK, DEPS) on every cell (res3c.f:194-198, 3668-3674): ghost
     // residuals stay zero
     for (int q = 0; q < nq; q++) {
@@ -2823,15 +2913,24 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
                          r->kw_parts, r->kw_scr + r->kw_parts))) return rc;
         if ((rc = launch(c, "k_fun1_prod", k_fun1_prod, cells_grid(b, 256), dim3(256), kp, b,
                          (const double *)(r->kw_scr + r->kw_parts)))) return rc;
+        if (fns && (rc = gradients())) return rc;
         dim3 g((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
-        rc = launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b);
-    } else if (c->residual_mode == FF_RESIDUAL_TILES) {
-        dim3 g((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ, (b.kmax + RKC - 1) / RKC);
-        rc = launch(c, "k_residual_tile", vl ? k_residual_tile<0> : k_residual_tile<1>, g,
-                    dim3(RI * RJ), make_prm(&c->prm, level), b);
+        rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b)
+                 : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b);
     } else {
-        rc = launch(c, "k_residual", vl ? k_residual<0> : k_residual<1>, cells_grid(b, 128),
-                    dim3(128), make_prm(&c->prm, level), b);
+        if (fns && (rc = gradients())) return rc;
+        if (c->residual_mode == FF_RESIDUAL_TILES) {
+            dim3 g((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ, (b.kmax + RKC - 1) / RKC);
+            auto kern = fns ? (vl ? k_residual_tile<0, true> : k_residual_tile<1, true>)
+                            : (vl ? k_residual_tile<0> : k_residual_tile<1>);
+            rc = launch(c, fns ? "k_residual_tile_ns" : "k_residual_tile", kern, g,
+                        dim3(RI * RJ), make_prm(&c->prm, level), b);
+        } else {
+            auto kern = fns ? (vl ? k_residual<0, true> : k_residual<1, true>)
+                            : (vl ? k_residual<0> : k_residual<1>);
+            rc = launch(c, fns ? "k_residual_ns" : "k_residual", kern, cells_grid(b, 128),
+                        dim3(128), make_prm(&c->prm, level), b);
+        }
     }
     if (rc) return rc;
     if (level == 1 && resi) {

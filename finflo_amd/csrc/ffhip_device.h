@@ -28,6 +28,8 @@ struct Prm {
     int iturb;
     int inter[3];
     int kover, katol, istres;
+    int fullns;     // full viscous fluxes (FULLNS) on this level: level 1 only
+    int idiff;
 };
 
 #define FLD(b, n) ((b).f[n])
@@ -310,6 +312,86 @@ __device__ __forceinline__ void visc_face2(const Prm &p, const Blk &b, int L, in
     fh = surfe * (t[L] - t[m]);
 }
 
+// FLUXBL base3c.f:4085-4242 (IDIFF <= 3) at one face: the full viscous flux from the
+// distance-weighted face average of the cell gradients SIJ, W12 and GRADT, with the Blazek
+// correction along the cell-centre connection when ALFA = SXI.n > 0 and the lambda term
+// unless IDIFF = 2.  dd is the direction's D1/D2/D3.  fm, fn, fw are what FLUXBL stores
+// (-F2VIB..-F4VIB); f5 is the F5VIB it subtracts from FE.
+__device__ __forceinline__ void visc_face_bl(const Prm &p, const Blk &b, int L, int S,
+                                             const double *__restrict__ dd,
+                                             const double *__restrict__ an,
+                                             const double *__restrict__ anx,
+                                             const double *__restrict__ any,
+                                             const double *__restrict__ anz, double &fm,
+                                             double &fn, double &fw, double &f5)
+{
+    const double *eps2 = FLD(b, FF_EPS2), *vis = FLD(b, FF_VIS), *cp = FLD(b, FF_CP),
+                 *vist = FLD(b, FF_VIST), *ch = FLD(b, FF_CH), *u = FLD(b, FF_U),
+                 *v = FLD(b, FF_V), *w = FLD(b, FF_W), *t = FLD(b, FF_TEMP),
+                 *xc = FLD(b, FF_XC), *yc = FLD(b, FF_YC), *zc = FLD(b, FF_ZC),
+                 *s11 = FLD(b, FF_S11), *s12 = FLD(b, FF_S12), *s13 = FLD(b, FF_S13),
+                 *s22 = FLD(b, FF_S22), *s23 = FLD(b, FF_S23), *s33 = FLD(b, FF_S33),
+                 *w12 = FLD(b, FF_W12), *w13 = FLD(b, FF_W13), *w23 = FLD(b, FF_W23),
+                 *gx = FLD(b, FF_DTDX), *gy = FLD(b, FF_DTDY), *gz = FLD(b, FF_DTDZ);
+    const double r23 = .6666667;
+    const int m = L - S;
+    double w1 = 2.0 * (dd[L] / (dd[L] + dd[m]));
+    double w2 = 2.0 - w1;
+    double w1p = .5 * w1, w2p = .5 * w2;
+    double dudx = w2p * s11[L] + w1p * s11[m];
+    double dvdy = w2p * s22[L] + w1p * s22[m];
+    double dwdz = w2p * s33[L] + w1p * s33[m];
+    double dudy = w2p * (s12[L] + w12[L]) + w1p * (s12[m] + w12[m]);
+    double dudz = w2p * (s13[L] + w13[L]) + w1p * (s13[m] + w13[m]);
+    double dvdx = w2p * (s12[L] - w12[L]) + w1p * (s12[m] - w12[m]);
+    double dvdz = w2p * (s23[L] + w23[L]) + w1p * (s23[m] + w23[m]);
+    double dwdx = w2p * (s13[L] - w13[L]) + w1p * (s13[m] - w13[m]);
+    double dwdy = w2p * (s23[L] - w23[L]) + w1p * (s23[m] - w23[m]);
+    double s11i = w2 * s11[L] + w1 * s11[m], s12i = w2 * s12[L] + w1 * s12[m];
+    double s13i = w2 * s13[L] + w1 * s13[m], s22i = w2 * s22[L] + w1 * s22[m];
+    double s23i = w2 * s23[L] + w1 * s23[m], s33i = w2 * s33[L] + w1 * s33[m];
+    double dtdxi = w2p * gx[L] + w1p * gx[m];
+    double dtdyi = w2p * gy[L] + w1p * gy[m];
+    double dtdzi = w2p * gz[L] + w1p * gz[m];
+    double sxi = xc[L] - xc[m], syi = yc[L] - yc[m], szi = zc[L] - zc[m];
+    double slen = sqrt(sxi * sxi + syi * syi + szi * szi) + 1.E-21;
+    sxi = sxi / slen; syi = syi / slen; szi = szi / slen;
+    const double nx = anx[L], ny = any[L], nz = anz[L];
+    double alfa = sxi * nx + syi * ny + szi * nz;
+    double surf = 0.5 * an[L] * (eps2[L] * vis[L] + eps2[m] * vis[m]);
+    double surfe = 0.5 * an[L] * (cp[L] * vist[L] / p.prt + ch[L] + cp[m] * vist[m] / p.prt + ch[m]);
+    double dudxsj = dudx * sxi + dudy * syi + dudz * szi;
+    double dvdxsj = dvdx * sxi + dvdy * syi + dvdz * szi;
+    double dwdxsj = dwdx * sxi + dwdy * syi + dwdz * szi;
+    double duvwsj = nx * dudxsj + ny * dvdxsj + nz * dwdxsj;
+    double adotu = nx * (u[L] - u[m]) + ny * (v[L] - v[m]) + nz * (w[L] - w[m]);
+    double f2 = nx * s11i + ny * s12i + nz * s13i;
+    double f3 = nx * s12i + ny * s22i + nz * s23i;
+    double f4 = nx * s13i + ny * s23i + nz * s33i;
+    if (alfa > 0.) {
+        double sdotdu = sxi * dudx + syi * dudy + szi * dudz;
+        double sdotdv = sxi * dvdx + syi * dvdy + szi * dvdz;
+        double sdotdw = sxi * dwdx + syi * dwdy + szi * dwdz;
+        f2 = alfa * ((u[L] - u[m]) / slen - sdotdu) + sxi * (adotu / slen - duvwsj) + f2;
+        f3 = alfa * ((v[L] - v[m]) / slen - sdotdv) + syi * (adotu / slen - duvwsj) + f3;
+        f4 = alfa * ((w[L] - w[m]) / slen - sdotdw) + szi * (adotu / slen - duvwsj) + f4;
+    }
+    if (p.idiff != 2 && p.idiff != 5) {
+        f2 = f2 - r23 * nx * (dudx + dvdy + dwdz);
+        f3 = f3 - r23 * ny * (dudx + dvdy + dwdz);
+        f4 = f4 - r23 * nz * (dudx + dvdy + dwdz);
+    }
+    f2 = surf * f2;
+    f3 = surf * f3;
+    f4 = surf * f4;
+    double e5 = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4)
+              + surfe * (dtdxi * nx + dtdyi * ny + dtdzi * nz);
+    if (alfa > 0.)
+        e5 = surfe * alfa * ((t[L] - t[m]) / slen - sxi * dtdxi - syi * dtdyi - szi * dtdzi) + e5;
+    fm = -f2; fn = -f3; fw = -f4;
+    f5 = e5;
+}
+
 // SST diffusion constants of COEFKO turb3.f:8068-8108 (ITURB = 6)
 struct SstC {
     double a1kleb, bstar, ckappa, beta1, sigrk1, sigom1, beta2, sigrk2, sigom2;
@@ -348,20 +430,78 @@ __device__ __forceinline__ void kw_diff_face(const Blk &b, int L, int S,
     f7 = surfep * (reps[L] * ypro1 - reps[m] * ypro2);
 }
 
+// FLKEBL base3c.f:5727-5830 at one face: k and omega diffusion from the distance-weighted face
+// average of the gradients of k/rho and omega/rho, corrected along the cell-centre connection
+// with ALFA = MAX(SXI.n, 0); returns FRK and FEPS
+__device__ __forceinline__ void kw_diff_face_bl(const Blk &b, int L, int S,
+                                                const double *__restrict__ dd,
+                                                const double *__restrict__ an,
+                                                const double *__restrict__ anx,
+                                                const double *__restrict__ any,
+                                                const double *__restrict__ anz, const SstC &s,
+                                                double &f6, double &f7)
+{
+    const double *vis = FLD(b, FF_VIS), *vist = FLD(b, FF_VIST), *ro = FLD(b, FF_RO),
+                 *rk = FLD(b, FF_RK), *reps = FLD(b, FF_REPS), *fun1 = FLD(b, FF_FUN1),
+                 *xc = FLD(b, FF_XC), *yc = FLD(b, FF_YC), *zc = FLD(b, FF_ZC),
+                 *kx = FLD(b, FF_DKDX), *ky = FLD(b, FF_DKDY), *kz = FLD(b, FF_DKDZ),
+                 *ex = FLD(b, FF_DEDX), *ey = FLD(b, FF_DEDY), *ez = FLD(b, FF_DEDZ);
+    const int m = L - S;
+    double f1 = fun1[L];
+    double onemf1 = 0.0 > (1.0 - f1) ? 0.0 : (1.0 - f1);
+    double psigk = f1 * s.sigrk1 + onemf1 * s.sigrk2;
+    double psige = f1 * s.sigom1 + onemf1 * s.sigom2;
+    double w1 = 2.0 * (dd[L] / (dd[L] + dd[m]));
+    double w2 = 2.0 - w1;
+    double dkdxi = 0.5 * (w2 * kx[L] + w1 * kx[m]);
+    double dkdyi = 0.5 * (w2 * ky[L] + w1 * ky[m]);
+    double dkdzi = 0.5 * (w2 * kz[L] + w1 * kz[m]);
+    double dedxi = 0.5 * (w2 * ex[L] + w1 * ex[m]);
+    double dedyi = 0.5 * (w2 * ey[L] + w1 * ey[m]);
+    double dedzi = 0.5 * (w2 * ez[L] + w1 * ez[m]);
+    double sxi = xc[L] - xc[m], syi = yc[L] - yc[m], szi = zc[L] - zc[m];
+    double slen = sqrt(sxi * sxi + syi * syi + szi * szi) + 1.E-21;
+    sxi = sxi / slen; syi = syi / slen; szi = szi / slen;
+    const double nx = anx[L], ny = any[L], nz = anz[L];
+    double alfa = sxi * nx + syi * ny + szi * nz;
+    alfa = dmax(alfa, 0.);
+    double rmut2 = vist[L] + vist[m];
+    double surfk = 0.5 * an[L] * (vis[L] + vis[m] + rmut2 * psigk);
+    double surfep = 0.5 * an[L] * (vis[L] + vis[m] + rmut2 * psige);
+    double ypro1 = 1. / ro[L];
+    double ypro2 = 1. / ro[m];
+    double tx = nx - alfa * sxi, ty = ny - alfa * syi, tz = nz - alfa * szi;
+    f6 = -surfk * (alfa * (rk[L] * ypro1 - rk[m] * ypro2) / slen + dkdxi * tx + dkdyi * ty
+                   + dkdzi * tz);
+    f7 = -surfep * (alfa * (reps[L] * ypro1 - reps[m] * ypro2) / slen + dedxi * tx
+                    + dedyi * ty + dedzi * tz);
+}
+
 // FLXCYC face flux (ITURB = 6, IFLUX = 1): FLUDKE (FE = FRK = F6VIG, FEPS = F7VIG), SETV12,
-// FLUXDO (FE = FE + F5VIG + heat), then FLUXKE added (res3c.f:6817-6946)
-template <bool SINGLE>
+// FLUXDO (FE = FE + F5VIG + heat), then FLUXKE added (res3c.f:6817-6946).  FNS: FULLNS on
+// level 1, FLKEBL and FLUXBL (FE = FRK - F5VIB) in place of FLUDKE and FLUXDO
+// (res3c.f:6802-6848).
+template <bool SINGLE, bool FNS = false>
 __device__ __forceinline__ Flux7 face_flux_kw(const Prm &p, const SstC &s, const Blk &b, int L,
                                               int S, int intem,
                                               const double *__restrict__ an,
                                               const double *__restrict__ anx,
                                               const double *__restrict__ any,
-                                              const double *__restrict__ anz)
+                                              const double *__restrict__ anz,
+                                              const double *__restrict__ dd = nullptr)
 {
-    double f6, f7, fm, fn, fw, f5v, fh;
-    kw_diff_face(b, L, S, an, s, f6, f7);
-    visc_face2(p, b, L, S, an, anx, any, anz, fm, fn, fw, f5v, fh);
-    double fe = f6 + f5v + fh;
+    double f6, f7, fm, fn, fw, fe;
+    if (FNS) {
+        double f5;
+        kw_diff_face_bl(b, L, S, dd, an, anx, any, anz, s, f6, f7);
+        visc_face_bl(p, b, L, S, dd, an, anx, any, anz, fm, fn, fw, f5);
+        fe = f6 - f5;
+    } else {
+        double f5v, fh;
+        kw_diff_face(b, L, S, an, s, f6, f7);
+        visc_face2(p, b, L, S, an, anx, any, anz, fm, fn, fw, f5v, fh);
+        fe = f6 + f5v + fh;
+    }
     if (p.istres >= 1) {
         // ANISFLUX earsm.f:278-307 (FLXCYC res3c.f:6933-6937): the EARSM Reynolds stresses
         // added to the momentum and energy fluxes ahead of the inviscid flux
@@ -492,17 +632,25 @@ __device__ __forceinline__ Flux5 vl_face(const Prm &p, const Blk &b, int L, int 
     return out;
 }
 
-// FLUCYC face flux: FLUXDO, then the inviscid flux of the scheme IFLUX selects added
-// (res3c.f:7856, 7892 Roe FLUXKE, 7907 van Leer FLUXVL)
-template <bool SINGLE, int IFLUX = 1>
+// FLUCYC face flux: FLUXDO (FNS: FLUXBL, FULLNS on level 1, res3c.f:7839), then the
+// inviscid flux of the scheme IFLUX selects added (res3c.f:7856, 7892 Roe FLUXKE, 7907 van
+// Leer FLUXVL)
+template <bool SINGLE, int IFLUX = 1, bool FNS = false>
 __device__ __forceinline__ Flux5 face_flux(const Prm &p, const Blk &b, int L, int S, int intem,
                                            const double *__restrict__ an,
                                            const double *__restrict__ anx,
                                            const double *__restrict__ any,
-                                           const double *__restrict__ anz)
+                                           const double *__restrict__ anz,
+                                           const double *__restrict__ dd = nullptr)
 {
     double fm, fn, fw, fe;
-    visc_face(p, b, L, S, an, anx, any, anz, fm, fn, fw, fe);
+    if (FNS) {
+        double f5;
+        visc_face_bl(p, b, L, S, dd, an, anx, any, anz, fm, fn, fw, f5);
+        fe = 0. - f5;                                   // FE = FE - F5VIB on the zeroed FE
+    } else {
+        visc_face(p, b, L, S, an, anx, any, anz, fm, fn, fw, fe);
+    }
     Flux5 r = IFLUX == 0 ? vl_face(p, b, L, S, intem, an, anx, any, anz)
                          : roe_face<SINGLE>(p, b, L, S, intem, an, anx, any, anz);
     Flux5 o;

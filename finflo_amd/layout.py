@@ -23,6 +23,7 @@ FIELDS = [
     "W12", "W13", "W23", "S11", "S12", "S13", "S22", "S23", "S33", "STRAIN", "TTS",
     "B11", "B12", "B13", "B22", "B23", "B33",
     "ROP2H", "RMP2H", "RNP2H", "RWP2H", "EP2H", "RKP2H", "EPSP2H",
+    "DTDX", "DTDY", "DTDZ", "DKDX", "DKDY", "DKDZ", "DEDX", "DEDY", "DEDZ",
 ]
 NF = len(FIELDS)
 FID = {name: i for i, name in enumerate(FIELDS)}
@@ -48,7 +49,7 @@ class Params(C.Structure):
         ("reserved", C.c_double * 3),
         ("iturb", C.c_int), ("iflux", C.c_int), ("idiff", C.c_int),
         ("inter", C.c_int * 3), ("kover", C.c_int), ("katol", C.c_int),
-        ("istres", C.c_int), ("reserved_i", C.c_int * 7)]
+        ("istres", C.c_int), ("fullns", C.c_int), ("reserved_i", C.c_int * 6)]
 
     DOUBLES = ("gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
                "frspre", "frsden", "frstem", "frsvel", "frsvis",
@@ -87,6 +88,7 @@ def default_params(**kw):
     p.inter[0] = p.inter[1] = p.inter[2] = -3
     p.kover, p.katol = 0, 1
     p.istres = 0
+    p.fullns = 0
     for k, v in kw.items():
         if k == "inter":
             for i in range(3):

@@ -50,6 +50,10 @@ enum ff_field {
     /* multigrid forcing of a coarse level, the reference's ROP2H..EP2H, RKP2H, EPSP2H at
        IH(M,N) (TRANSF multi.f:245-270, FORCFU multi.f:1630-1648, ADDRES multi.f:1709-1725) */
     FF_ROP2H, FF_RMP2H, FF_RNP2H, FF_RWP2H, FF_EP2H, FF_RKP2H, FF_EPSP2H,
+    /* cell gradients of TEMDER (turb3.f:3456) for the full viscous fluxes (FULLNS): of T
+       (GRADT, RESIDU res3c.f:257), of k/rho (GRADK) and omega/rho (GREPS, RESIKE
+       res3c.f:3991-4005) */
+    FF_DTDX, FF_DTDY, FF_DTDZ, FF_DKDX, FF_DKDY, FF_DKDZ, FF_DEDX, FF_DEDY, FF_DEDZ,
     FF_NFIELDS
 };
 
@@ -91,7 +95,9 @@ typedef struct ff_params {
     int katol;       /* Kato-Launder production switch */
     int istres;      /* ISTRES: 1 = Wallin-Johansson EARSM on top of the SST (EARSM1, ANISFLUX,
                         PEEKOO), 0 = eddy-viscosity SST */
-    int reserved_i[7];
+    int fullns;      /* FULLNS: 1 = full viscous fluxes from cell gradients on level 1 (FLUXBL,
+                        FLKEBL with TEMDER), 0 = the thin-layer FLUXDO/FLUDKE */
+    int reserved_i[6];
 } ff_params;
 
 /* One (level M, block N) slab: pointers to the first element of each field, i.e. the

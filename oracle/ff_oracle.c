@@ -107,6 +107,9 @@ void ffo_state(const ff_params *p, ff_block *b)
 /* Face fluxes                                                                           */
 /* ------------------------------------------------------------------------------------ */
 
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
 /* MUSCL kappa-scheme of XXTRAP base3c.f:750-850 for the face between cells L-S and L.
  * Returns right (ROIP, cell-L side) and left (ROIM) states. */
 static void xxtrap(const double *q, long L, long S, int intem, double *qr, double *ql)
@@ -493,6 +496,126 @@ static void kw_diff_face(const ff_block *b, long L, long S, const double *an, do
     *f7 = surfep * (reps[L] * ypro1 - reps[m] * ypro2);
 }
 
+/* FLUXBL base3c.f:4085-4242 at one face, IDIFF <= 3: the full viscous flux from the
+ * distance-weighted face average of the cell gradients (SIJ, W12, GRADT), with the Blazek
+ * correction along the cell-centre connection SXI when ALFA = SXI.n > 0 (base3c.f:4206-4229;
+ * the IDIFF /= 3 .OR. IDIFF /= 6 test is always true) and the lambda term unless IDIFF = 2.
+ * `dd` is the direction's D1/D2/D3.  Returns the momentum fluxes FLUXBL stores (FRM = -F2VIB)
+ * and F5VIB, which it subtracts from FE. */
+static void visc_face_bl(const ff_params *p, const ff_block *b, long L, long S,
+                         const double *dd, const double *an, const double *anx,
+                         const double *any, const double *anz, double *fm, double *fn,
+                         double *fw, double *f5o)
+{
+    const double *eps2 = F(b, FF_EPS2), *vis = F(b, FF_VIS), *cp = F(b, FF_CP),
+                 *vist = F(b, FF_VIST), *ch = F(b, FF_CH), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *t = F(b, FF_TEMP), *xc = F(b, FF_XC), *yc = F(b, FF_YC),
+                 *zc = F(b, FF_ZC), *s11 = F(b, FF_S11), *s12 = F(b, FF_S12),
+                 *s13 = F(b, FF_S13), *s22 = F(b, FF_S22), *s23 = F(b, FF_S23),
+                 *s33 = F(b, FF_S33), *w12 = F(b, FF_W12), *w13 = F(b, FF_W13),
+                 *w23 = F(b, FF_W23), *gx = F(b, FF_DTDX), *gy = F(b, FF_DTDY),
+                 *gz = F(b, FF_DTDZ);
+    const double r23 = .6666667;
+    long m = L - S;
+    double w1 = 2.0 * (dd[L] / (dd[L] + dd[m]));
+    double w2 = 2.0 - w1;
+    double w1p = .5 * w1, w2p = .5 * w2;
+    double dudx = w2p * s11[L] + w1p * s11[m];
+    double dvdy = w2p * s22[L] + w1p * s22[m];
+    double dwdz = w2p * s33[L] + w1p * s33[m];
+    double dudy = w2p * (s12[L] + w12[L]) + w1p * (s12[m] + w12[m]);
+    double dudz = w2p * (s13[L] + w13[L]) + w1p * (s13[m] + w13[m]);
+    double dvdx = w2p * (s12[L] - w12[L]) + w1p * (s12[m] - w12[m]);
+    double dvdz = w2p * (s23[L] + w23[L]) + w1p * (s23[m] + w23[m]);
+    double dwdx = w2p * (s13[L] - w13[L]) + w1p * (s13[m] - w13[m]);
+    double dwdy = w2p * (s23[L] - w23[L]) + w1p * (s23[m] - w23[m]);
+    double s11i = w2 * s11[L] + w1 * s11[m], s12i = w2 * s12[L] + w1 * s12[m];
+    double s13i = w2 * s13[L] + w1 * s13[m], s22i = w2 * s22[L] + w1 * s22[m];
+    double s23i = w2 * s23[L] + w1 * s23[m], s33i = w2 * s33[L] + w1 * s33[m];
+    double dtdxi = w2p * gx[L] + w1p * gx[m];
+    double dtdyi = w2p * gy[L] + w1p * gy[m];
+    double dtdzi = w2p * gz[L] + w1p * gz[m];
+    double sxi = xc[L] - xc[m], syi = yc[L] - yc[m], szi = zc[L] - zc[m];
+    double slen = sqrt(sxi * sxi + syi * syi + szi * szi) + 1.E-21;
+    sxi = sxi / slen; syi = syi / slen; szi = szi / slen;
+    double nx = anx[L], ny = any[L], nz = anz[L];
+    double alfa = sxi * nx + syi * ny + szi * nz;
+    double surf = 0.5 * an[L] * (eps2[L] * vis[L] + eps2[m] * vis[m]);
+    double surfe = 0.5 * an[L] * (cp[L] * vist[L] / p->prt + ch[L] + cp[m] * vist[m] / p->prt + ch[m]);
+    double dudxsj = dudx * sxi + dudy * syi + dudz * szi;
+    double dvdxsj = dvdx * sxi + dvdy * syi + dvdz * szi;
+    double dwdxsj = dwdx * sxi + dwdy * syi + dwdz * szi;
+    double duvwsj = nx * dudxsj + ny * dvdxsj + nz * dwdxsj;
+    double sdotdu = sxi * dudx + syi * dudy + szi * dudz;
+    double sdotdv = sxi * dvdx + syi * dvdy + szi * dvdz;
+    double sdotdw = sxi * dwdx + syi * dwdy + szi * dwdz;
+    double adotu = nx * (u[L] - u[m]) + ny * (v[L] - v[m]) + nz * (w[L] - w[m]);
+    double f2 = nx * s11i + ny * s12i + nz * s13i;
+    double f3 = nx * s12i + ny * s22i + nz * s23i;
+    double f4 = nx * s13i + ny * s23i + nz * s33i;
+    if (alfa > 0.) {
+        f2 = alfa * ((u[L] - u[m]) / slen - sdotdu) + sxi * (adotu / slen - duvwsj) + f2;
+        f3 = alfa * ((v[L] - v[m]) / slen - sdotdv) + syi * (adotu / slen - duvwsj) + f3;
+        f4 = alfa * ((w[L] - w[m]) / slen - sdotdw) + szi * (adotu / slen - duvwsj) + f4;
+    }
+    if (p->idiff != 2 && p->idiff != 5) {
+        f2 = f2 - r23 * nx * (dudx + dvdy + dwdz);
+        f3 = f3 - r23 * ny * (dudx + dvdy + dwdz);
+        f4 = f4 - r23 * nz * (dudx + dvdy + dwdz);
+    }
+    f2 = surf * f2;
+    f3 = surf * f3;
+    f4 = surf * f4;
+    double f5 = .5 * ((u[L] + u[m]) * f2 + (v[L] + v[m]) * f3 + (w[L] + w[m]) * f4)
+              + surfe * (dtdxi * nx + dtdyi * ny + dtdzi * nz);
+    if (alfa > 0.)
+        f5 = surfe * alfa * ((t[L] - t[m]) / slen - sxi * dtdxi - syi * dtdyi - szi * dtdzi) + f5;
+    *fm = -f2; *fn = -f3; *fw = -f4;
+    *f5o = f5;
+}
+
+/* FLKEBL base3c.f:5727-5830 at one face: k and omega diffusion from the distance-weighted
+ * face average of the gradients of k/rho and omega/rho (GRADK, GREPS), corrected along the
+ * cell-centre connection with ALFA = MAX(SXI.n, 0).  Returns FRK and FEPS. */
+static void kw_diff_face_bl(const ff_block *b, long L, long S, const double *dd,
+                            const double *an, const double *anx, const double *any,
+                            const double *anz, double psigk1, double psige1, double psigk2,
+                            double psige2, double *f6, double *f7)
+{
+    const double *vis = F(b, FF_VIS), *vist = F(b, FF_VIST), *ro = F(b, FF_RO),
+                 *rk = F(b, FF_RK), *reps = F(b, FF_REPS), *fun1 = F(b, FF_FUN1),
+                 *xc = F(b, FF_XC), *yc = F(b, FF_YC), *zc = F(b, FF_ZC),
+                 *kx = F(b, FF_DKDX), *ky = F(b, FF_DKDY), *kz = F(b, FF_DKDZ),
+                 *ex = F(b, FF_DEDX), *ey = F(b, FF_DEDY), *ez = F(b, FF_DEDZ);
+    long m = L - S;
+    double onemf1 = dmax(0.0, 1.0 - fun1[L]);
+    double psigk = fun1[L] * psigk1 + onemf1 * psigk2;
+    double psige = fun1[L] * psige1 + onemf1 * psige2;
+    double w1 = 2.0 * (dd[L] / (dd[L] + dd[m]));
+    double w2 = 2.0 - w1;
+    double dkdxi = 0.5 * (w2 * kx[L] + w1 * kx[m]);
+    double dkdyi = 0.5 * (w2 * ky[L] + w1 * ky[m]);
+    double dkdzi = 0.5 * (w2 * kz[L] + w1 * kz[m]);
+    double dedxi = 0.5 * (w2 * ex[L] + w1 * ex[m]);
+    double dedyi = 0.5 * (w2 * ey[L] + w1 * ey[m]);
+    double dedzi = 0.5 * (w2 * ez[L] + w1 * ez[m]);
+    double sxi = xc[L] - xc[m], syi = yc[L] - yc[m], szi = zc[L] - zc[m];
+    double slen = sqrt(sxi * sxi + syi * syi + szi * szi) + 1.E-21;
+    sxi = sxi / slen; syi = syi / slen; szi = szi / slen;
+    double nx = anx[L], ny = any[L], nz = anz[L];
+    double alfa = sxi * nx + syi * ny + szi * nz;
+    alfa = dmax(alfa, 0.);
+    double rmut2 = vist[L] + vist[m];
+    double surfk = 0.5 * an[L] * (vis[L] + vis[m] + rmut2 * psigk);
+    double surfep = 0.5 * an[L] * (vis[L] + vis[m] + rmut2 * psige);
+    double ypro1 = 1. / ro[L];
+    double ypro2 = 1. / ro[m];
+    *f6 = -surfk * (alfa * (rk[L] * ypro1 - rk[m] * ypro2) / slen + dkdxi * (nx - alfa * sxi)
+                    + dkdyi * (ny - alfa * syi) + dkdzi * (nz - alfa * szi));
+    *f7 = -surfep * (alfa * (reps[L] * ypro1 - reps[m] * ypro2) / slen + dedxi * (nx - alfa * sxi)
+                     + dedyi * (ny - alfa * syi) + dedzi * (nz - alfa * szi));
+}
+
 /* FLUCYC (laminar, res3c.f:7572) or FLXCYC (two-equation, res3c.f:6424) for one direction
  * (dir 0=i, 1=j, 2=k) followed by SWEEP/SWEEPK.  Face fluxes are formed for the faces
  * adjacent to interior cells only; the reference also forms fluxes on ghost rows and
@@ -507,6 +630,7 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last, 
                  *any = F(b, FF_A1Y + 4 * dir), *anz = F(b, FF_A1Z + 4 * dir);
     int intem = p->inter[dir];
     int turb = !(p->iturb < 3 || p->iturb == 8);
+    const double *dd = F(b, FF_D1 + dir);   /* D1, D2 or D3 of the direction */
     int nv = kw ? 7 : 5;
     /* face flux arrays F1R..F1E (F1RK, F1EPS): zeroed (res3c.f:7705-7710, 6740-6748) */
     double *ff[7];
@@ -519,17 +643,29 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last, 
                 long L = cid(&d, i, j, k);
                 double fm, fn, fw, f5, fh;
                 if (kw) {
-                    /* FLUDKE (res3c.f:6817): FRK = FE = F6VIG, FEPS = F7VIG; then
-                       SETV12(F1E,F1RK) (res3c.f:6837) */
+                    /* FLUDKE (res3c.f:6817) or, with FULLNS on level 1, FLKEBL
+                       (res3c.f:6808): FRK = FE = F6, FEPS = F7; then SETV12(F1E,F1RK)
+                       (res3c.f:6837) */
                     double f6, f7;
-                    kw_diff_face(b, L, S, an, kw[0], kw[1], kw[2], kw[3], &f6, &f7);
+                    if (p->fullns)
+                        kw_diff_face_bl(b, L, S, dd, an, anx, any, anz, kw[0], kw[1], kw[2],
+                                        kw[3], &f6, &f7);
+                    else
+                        kw_diff_face(b, L, S, an, kw[0], kw[1], kw[2], kw[3], &f6, &f7);
                     ff[5][L] = f6; ff[6][L] = f7; ff[4][L] = f6;
                     ff[4][L] = ff[5][L];
                 }
-                /* FLUXDO (res3c.f:7856, 6867), FRO untouched by FLUXDO */
-                visc_face2(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &f5, &fh);
-                ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
-                ff[4][L] = ff[4][L] + f5 + fh;
+                if (p->fullns) {
+                    /* FLUXBL (res3c.f:7839, 6845): FRM..FRW stored, FE = FE - F5VIB */
+                    visc_face_bl(p, b, L, S, dd, an, anx, any, anz, &fm, &fn, &fw, &f5);
+                    ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
+                    ff[4][L] = ff[4][L] - f5;
+                } else {
+                    /* FLUXDO (res3c.f:7856, 6867), FRO untouched by FLUXDO */
+                    visc_face2(p, b, L, S, an, anx, any, anz, &fm, &fn, &fw, &f5, &fh);
+                    ff[1][L] = fm; ff[2][L] = fn; ff[3][L] = fw;
+                    ff[4][L] = ff[4][L] + f5 + fh;
+                }
                 if (kw && p->istres >= 1) {
                     /* ANISFLUX earsm.f:278-307 (FLXCYC res3c.f:6933-6937): the EARSM stresses
                        added to the momentum and energy fluxes before the inviscid flux */
@@ -604,8 +740,6 @@ static sst_t coefko(int kover)
 
 /* X**4 as the reference build evaluates it: ((X*X)*X)*X */
 static inline double pow4(double x) { return x * x * x * x; }
-static inline double dmax(double a, double b) { return a > b ? a : b; }
-static inline double dmin(double a, double b) { return a < b ? a : b; }
 
 /* DERFUN turb3.f:2476-2665 with IDERI = 1 (distance-weighted Green-Gauss): vorticity,
  * strain-rate tensor, |Omega| (OHMI) and the strain magnitude (VELLAP, used only by the SAS
@@ -703,6 +837,67 @@ static void derfun(ff_block *b, int level)
                 }
         }
     }
+}
+
+/* TEMDER turb3.f:3456-3559 with IDERI = 1: the distance-weighted Green-Gauss gradient of the
+ * cell scalar q into the fields out, out+1, out+2.  On level 1 the first ghost layer is
+ * included (the k ghost layer only when KMAX > 1, turb3.f:3498-3507); it is formed on level 1
+ * only, where FLUXBL and FLKEBL read it. */
+static void temder(ff_block *b, const double *q, int out)
+{
+    dims_t d = mkdims(b);
+    long IS = d.is, IL = d.il;
+    const int gk = d.kmax > 1 ? 1 : 0;
+    const double *vol = F(b, FF_VOL), *a1 = F(b, FF_A1), *a2 = F(b, FF_A2), *a3 = F(b, FF_A3),
+                 *d1 = F(b, FF_D1), *d2 = F(b, FF_D2), *d3 = F(b, FF_D3),
+                 *a1x = F(b, FF_A1X), *a1y = F(b, FF_A1Y), *a1z = F(b, FF_A1Z),
+                 *a2x = F(b, FF_A2X), *a2y = F(b, FF_A2Y), *a2z = F(b, FF_A2Z),
+                 *a3x = F(b, FF_A3X), *a3y = F(b, FF_A3Y), *a3z = F(b, FF_A3Z);
+    double *gx = F(b, out), *gy = F(b, out + 1), *gz = F(b, out + 2);
+    for (int k = 1 - gk; k <= d.kmax + gk; k++)
+        for (int j = 0; j <= d.jmax + 1; j++)
+            for (int i = 0; i <= d.imax + 1; i++) {
+                long L = cid(&d, i, j, k);
+                double w1, w2;
+                w1 = 2.0 * (d1[L] / (d1[L] + d1[L - 1])); w2 = 2.0 - w1;
+                double t1m = w2 * q[L] + w1 * q[L - 1];
+                w1 = 2.0 * (d1[L] / (d1[L] + d1[L + 1])); w2 = 2.0 - w1;
+                double t1p = w2 * q[L] + w1 * q[L + 1];
+                w1 = 2.0 * (d2[L] / (d2[L] + d2[L - IS])); w2 = 2.0 - w1;
+                double t2m = w2 * q[L] + w1 * q[L - IS];
+                w1 = 2.0 * (d2[L] / (d2[L] + d2[L + IS])); w2 = 2.0 - w1;
+                double t2p = w2 * q[L] + w1 * q[L + IS];
+                w1 = 2.0 * (d3[L] / (d3[L] + d3[L - IL])); w2 = 2.0 - w1;
+                double t3m = w2 * q[L] + w1 * q[L - IL];
+                w1 = 2.0 * (d3[L] / (d3[L] + d3[L + IL])); w2 = 2.0 - w1;
+                double t3p = w2 * q[L] + w1 * q[L + IL];
+                double pvol = .5 / vol[L];
+                gx[L] = pvol * (a3[L + IL] * a3x[L + IL] * t3p - a3[L] * a3x[L] * t3m
+                                + a2[L + IS] * a2x[L + IS] * t2p - a2[L] * a2x[L] * t2m
+                                + a1[L + 1] * a1x[L + 1] * t1p - a1[L] * a1x[L] * t1m);
+                gy[L] = pvol * (a3[L + IL] * a3y[L + IL] * t3p - a3[L] * a3y[L] * t3m
+                                + a2[L + IS] * a2y[L + IS] * t2p - a2[L] * a2y[L] * t2m
+                                + a1[L + 1] * a1y[L + 1] * t1p - a1[L] * a1y[L] * t1m);
+                gz[L] = pvol * (a3[L + IL] * a3z[L + IL] * t3p - a3[L] * a3z[L] * t3m
+                                + a2[L + IS] * a2z[L + IS] * t2p - a2[L] * a2z[L] * t2m
+                                + a1[L + 1] * a1z[L + 1] * t1p - a1[L] * a1z[L] * t1m);
+            }
+}
+
+/* TEMDER of T (RESIDU res3c.f:257-259, RESIKE res3c.f:3991-3993) and, for the SST, of
+ * DDEPS = RK/RO and DDEPS = REPS/RO (DIVV12, res3c.f:3995-4005) */
+static void fullns_gradients(ff_block *b, int kwm)
+{
+    dims_t d = mkdims(b);
+    temder(b, F(b, FF_TEMP), FF_DTDX);
+    if (!kwm) return;
+    double *q = malloc(sizeof(double) * (size_t)d.ntot);
+    const double *ro = F(b, FF_RO), *rk = F(b, FF_RK), *reps = F(b, FF_REPS);
+    for (long n = 0; n < d.ntot; n++) q[n] = rk[n] / ro[n];
+    temder(b, q, FF_DKDX);
+    for (long n = 0; n < d.ntot; n++) q[n] = reps[n] / ro[n];
+    temder(b, q, FF_DEDX);
+    free(q);
 }
 
 /* TTIMES earsm.f:629-692 (ITURB = 6), first ghost layer included */
@@ -1116,6 +1311,7 @@ void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *r
     dims_t d = mkdims(b);
     ff_params pl = *p0;
     if (level >= 2) pl.inter[0] = pl.inter[1] = pl.inter[2] = 55;
+    pl.fullns = level == 1 ? p0->fullns : 0;     /* FLUXBL/FLKEBL: FULLNS .AND. M == 1 */
     const ff_params *p = &pl;
     int kwm = p->iturb == 6;
     if (kwm && m2 >= 2) {
@@ -1142,6 +1338,11 @@ void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *r
         else
             prodke(p, b, trueps);
         free(trueps);
+    }
+    if (p->fullns) {
+        /* RESIDU forms DERFUN itself (res3c.f:250-256); RESIKE uses TURBEQ's */
+        if (!kwm) derfun(b, 1);
+        fullns_gradients(b, kwm);
     }
     direction_sweep(p, b, 2, 0, kwm ? sig : NULL);   /* res3c.f:270 / 4018, SWEEP(K) */
     direction_sweep(p, b, 0, 0, kwm ? sig : NULL);   /* res3c.f:402 / 4274 */

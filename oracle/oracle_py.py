@@ -149,7 +149,7 @@ class RefBlock:
         inter = list(prm.inter) if level == 1 else [55, 55, 55]
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
                             inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1,
-                            prm.istres],
+                            prm.istres, prm.fullns],
                            dtype=np.int32)
 
     def _a(self):

@@ -12,7 +12,10 @@
 !                 RESL2 res3c.f:8920 on level 1; ITURB=6: the RESIKE sequence
 !                 res3c.f:3700-5068 (CROSKO, FUN1KO, MAKE_EPS, PRODKE, FLXCYC with FLUDKE,
 !                 SETV12, FLUXDO, FLUXKE; SWEEPK; SOURKO, ADDSKE); with ISTRES >= 1 PEEKOO
-!                 in place of PRODKE and ANISFLUX after FLUXDO
+!                 in place of PRODKE and ANISFLUX after FLUXDO.  FULLNS: RESIDU adds DERFUN
+!                 and TEMDER of T (res3c.f:250-261), RESIKE TEMDER of T, k/rho, omega/rho
+!                 (res3c.f:3989-4007); on level 1 FLUXBL replaces FLUXDO and FLKEBL
+!                 replaces FLUDKE (res3c.f:6802-6848, 7839-7846)
 !   ref_implicit  IMPS imps3.f:111-483 with LUSGS=1: TIME3, LOCDT, COPR, PREDGS, DIAGGS,
 !                 CORRGS, DIVV12; ITURB=6: LOCDTK, IMSOKO, COPRKE, PRKEGS, DIKEGS,
 !                 COKEGS, DIVV12, MULV12
@@ -36,7 +39,8 @@ module ffref_ids
   integer, parameter :: IS23=69, IS33=70, ISTRAIN=71, ITTS=72
   integer, parameter :: IB11=73, IB12=74, IB13=75, IB22=76, IB23=77, IB33=78
   integer, parameter :: IROP2H=79, IRMP2H=80, IRNP2H=81, IRWP2H=82, IEP2H=83, IRKP2H=84
-  integer, parameter :: IEPSP2H=85
+  integer, parameter :: IEPSP2H=85, IDTDX=86, IDTDY=87, IDTDZ=88, IDKDX=89, IDKDY=90
+  integer, parameter :: IDKDZ=91, IDEDX=92, IDEDY=93, IDEDZ=94
   ! parameter vector PRM (same order as the doubles of ff_params)
   integer, parameter :: PGAMMA=1, PRGAS=2, PPR=3, PPRT=4, PVISU0=5, PEXPSU=6, PTSU0=7
   integer, parameter :: PFRSPRE=8, PFRSDEN=9, PFRSTEM=10, PFRSVEL=11, PFRSVIS=12, PCFL=13
@@ -45,7 +49,7 @@ module ffref_ids
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
   integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12, QM2=13
-  integer, parameter :: QISTRES=14
+  integer, parameter :: QISTRES=14, QFULLNS=15
 end module ffref_ids
 
 subroutine ref_state(f, ntot, prm, ipr) bind(C, name="ref_state")
@@ -144,6 +148,7 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
   type(MPHASE_VARIABLES) :: var(1)
   real(c_double) :: f1h(1), wave(1)
   integer :: imax, jmax, kmax, is, js, kstr, kmaxp, ia, iax, iay, iaz, inter, ndi, istr, ido
+  integer :: id
   logical :: xxtral, inchiml, entropy_fix, transl, mulphl, fresul
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
   is = imax + 4; js = jmax + 4
@@ -152,16 +157,28 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
   select case (idir)
   case (1)
      kstr = 1; kmaxp = kmax; ia = IA1; inter = ipr(QINTI); ndi = imax + 1
-     istr = 1
+     istr = 1; id = ID1
   case (2)
      kstr = is; kmaxp = kmax; ia = IA2; inter = ipr(QINTJ); ndi = jmax + 1
-     istr = is
+     istr = is; id = ID2
   case default
      kstr = is*js; kmaxp = kmax + 1; ia = IA3; inter = ipr(QINTK); ndi = kmax + 1
-     istr = is*js
+     istr = is*js; id = ID3
   end select
   iax = ia + 1; iay = ia + 2; iaz = ia + 3
   fl = 0.
+  if (ipr(QFULLNS) /= 0 .and. ipr(QLEVEL) == 1) then
+  ! FLUXBL (res3c.f:7839): full viscous flux from the cell gradients
+  call FLUXBL(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IVOL), f(1,ia), f(1,IXC), &
+       f(1,IYC), f(1,IZC), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IP), f(1,IU), f(1,IV), &
+       f(1,IW), f(1,IE), f(1,IC), imax, jmax, kmax, 1, inter, ndi, f(1,IVIS), f(1,IEPS2), &
+       f(1,IVIST), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), f(1,IS13), &
+       f(1,IS22), f(1,IS23), f(1,IS33), f(1,IDTDX), f(1,IDTDY), f(1,IDTDZ), f(1,id), &
+       f(1,iax), f(1,iay), f(1,iaz), f(1,ICP), prm(PPR), prm(PPRT), zero, pro, var, &
+       f(1,ITEMP), f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), &
+       prm(PFRSPRE), hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, ipr(QITURB), mulphl, &
+       ipr(QIDIFF))
+  else
   ! FLUXDO (res3c.f:7856): thin-layer viscous flux, arithmetic average
   call FLUXDO(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IVOL), f(1,ia), f(1,IRO), &
        f(1,IRM), f(1,IRN), f(1,IRW), f(1,IP), f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), &
@@ -170,6 +187,7 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
        f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), prm(PFRSPRE), &
        hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, mulphl, 1, fresul, 0._c_double, f1h, &
        wave, ipr(QLEVEL), ipr(QIDIFF))
+  end if
   if (ipr(QIFLUX) == 0) then
   ! FLUXVL (res3c.f:7907): van Leer flux-vector splitting
   call FLUXVL(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), &
@@ -198,6 +216,21 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
        fl(1,4), fl(1,5), f(1,IVOL), imax, jmax, kmax, istr, 2, 2, 2, 0._c_double, ido)
 end subroutine ref_direction
 
+! TEMDER turb3.f:3456-3703 of the scalar Q into the three fields from IOUT on, with
+! IDERI = IDER = 1 (main.f:5653) and the level of IPR
+subroutine ref_temder(f, ntot, q, iout, ipr)
+  use iso_c_binding
+  use ffref_ids
+  implicit none
+  integer :: ntot, iout
+  real(c_double) :: f(ntot, *), q(ntot)
+  integer(c_int) :: ipr(*)
+  call TEMDER(q, f(1,iout), f(1,iout+1), f(1,iout+2), f(1,IA1), f(1,IA2), f(1,IA3), f(1,IVOL), &
+       f(1,ID1), f(1,ID2), f(1,ID3), f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2X), f(1,IA2Y), &
+       f(1,IA2Z), f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IXC), f(1,IYC), f(1,IZC), ipr(QIMAX), &
+       ipr(QJMAX), ipr(QKMAX), 2, 2, 2, 1, ipr(QLEVEL))
+end subroutine ref_temder
+
 subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   use iso_c_binding
   use ffref_ids
@@ -208,6 +241,7 @@ subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   real(c_double) :: f(ntot, *), prm(*), resi(*)
   integer(c_int) :: ipr(*)
   real(c_double), allocatable :: fl(:,:), zzz(:), rksi(:), hat(:,:), zero(:), dfi(:)
+  real(c_double), allocatable :: vellap(:)
   type(INTERMITTENCY) :: trm(1)
   type(MPHASE_VARIABLES) :: var(1)
   type(PRE_COR) :: prc(1)
@@ -226,6 +260,19 @@ subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   ! ZEROZZ of the residuals (res3c.f:194-199)
   call ZEROZZ(f(1,IDRO), ntot); call ZEROZZ(f(1,IDM), ntot); call ZEROZZ(f(1,IDN), ntot)
   call ZEROZZ(f(1,IDW), ntot); call ZEROZZ(f(1,IDE), ntot); call ZEROZZ(f(1,IDEPS), ntot)
+  if (ipr(QFULLNS) /= 0) then
+     ! RESIDU res3c.f:250-261: DERFUN and TEMDER of T, on every level
+     allocate(vellap(ntot))
+     vellap = 0.
+     call DERFUN(f(1,IOHMI), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), &
+          f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IU), f(1,IV), f(1,IW), f(1,IA1), &
+          f(1,IA2), f(1,IA3), f(1,IVOL), f(1,ID1), f(1,ID2), f(1,ID3), f(1,IA1X), f(1,IA1Y), &
+          f(1,IA1Z), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3X), f(1,IA3Y), f(1,IA3Z), &
+          f(1,IXC), f(1,IYC), f(1,IZC), imax, jmax, kmax, 2, 2, 2, 1, ipr(QLEVEL), &
+          f(1,ISTRAIN), vellap)
+     call ref_temder(f, ntot, f(1,ITEMP), IDTDX, ipr)
+     deallocate(vellap)
+  end if
   call ref_direction(f, ntot, prm, ipr, 3, fl, zzz, maw, maxw, rksi, hat, zero, trm)
   call ref_direction(f, ntot, prm, ipr, 1, fl, zzz, maw, maxw, rksi, hat, zero, trm)
   call ref_direction(f, ntot, prm, ipr, 2, fl, zzz, maw, maxw, rksi, hat, zero, trm)
@@ -260,7 +307,8 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
   type(PRE_COR) :: prc(1)
   real(c_double) :: f1h(1), wave(1)
   integer :: imax, jmax, kmax, is, js, kstr, kmaxp, ia, iax, iay, iaz, inter, intet, ndi, ido
-  integer :: jadd, kadd
+  integer :: jadd, kadd, id
+  logical :: fullns
   logical :: xxtral, inchiml, entropy_fix, transl, mulphl, fresul
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
   is = imax + 4; js = jmax + 4
@@ -268,21 +316,43 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
   mulphl = .false.; fresul = .false.
   select case (idir)
   case (1)
-     kstr = 1; kmaxp = kmax; ia = IA1; inter = ipr(QINTI); ndi = imax + 1
+     kstr = 1; kmaxp = kmax; ia = IA1; inter = ipr(QINTI); ndi = imax + 1; id = ID1
   case (2)
-     kstr = is; kmaxp = kmax; ia = IA2; inter = ipr(QINTJ); ndi = jmax + 1
+     kstr = is; kmaxp = kmax; ia = IA2; inter = ipr(QINTJ); ndi = jmax + 1; id = ID2
   case default
-     kstr = is*js; kmaxp = kmax + 1; ia = IA3; inter = ipr(QINTK); ndi = kmax + 1
+     kstr = is*js; kmaxp = kmax + 1; ia = IA3; inter = ipr(QINTK); ndi = kmax + 1; id = ID3
   end select
+  fullns = ipr(QFULLNS) /= 0 .and. ipr(QLEVEL) == 1
   ! INTETI/J/K of RESIKE (res3c.f:3634-3641, INTERTU = 0)
   intet = inter
   if (inter > -5 .and. inter < 5) intet = +1
   iax = ia + 1; iay = ia + 2; iaz = ia + 3
   fl = 0.
+  if (fullns) then
+  ! FLKEBL (res3c.f:6808): k and omega diffusion from the cell gradients of k/rho, omega/rho
+  call FLKEBL(fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), f(1,ia), f(1,iax), f(1,iay), f(1,iaz), &
+       f(1,id), f(1,IRO), f(1,IRK), f(1,IREPS), imax, jmax, kmax, ndi, f(1,IVIS), f(1,IEPS2), &
+       f(1,IVIST), f(1,IFUN1), f(1,IDKDX), f(1,IDKDY), f(1,IDKDZ), f(1,IDEDX), f(1,IDEDY), &
+       f(1,IDEDZ), f(1,IXC), f(1,IYC), f(1,IZC), sig(1), sig(2), sig(3), sig(4), kstr, idir, &
+       ipr(QIDIFF))
+  else
   call FLUDKE(fl(1,5), fl(1,6), fl(1,7), f(1,IVOL), f(1,ia), f(1,IRO), f(1,IRK), f(1,IREPS), &
        imax, jmax, kmax, ndi, f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,IFUN1), sig(1), sig(2), &
        sig(3), sig(4), kstr, idir, ipr(QIDIFF), transl, trm)
+  end if
   call SETV12(fl(1,5), fl(1,6), ntot)
+  if (fullns) then
+  ! FLUXBL (res3c.f:6845)
+  call FLUXBL(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IVOL), f(1,ia), f(1,IXC), &
+       f(1,IYC), f(1,IZC), f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IP), f(1,IU), f(1,IV), &
+       f(1,IW), f(1,IE), f(1,IC), imax, jmax, kmax, 1, inter, ndi, f(1,IVIS), f(1,IEPS2), &
+       f(1,IVIST), f(1,IW12), f(1,IW13), f(1,IW23), f(1,IS11), f(1,IS12), f(1,IS13), &
+       f(1,IS22), f(1,IS23), f(1,IS33), f(1,IDTDX), f(1,IDTDY), f(1,IDTDZ), f(1,id), &
+       f(1,iax), f(1,iay), f(1,iaz), f(1,ICP), prm(PPR), prm(PPRT), zero, pro, var, &
+       f(1,ITEMP), f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), &
+       prm(PFRSPRE), hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, ipr(QITURB), mulphl, &
+       ipr(QIDIFF))
+  else
   call FLUXDO(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IVOL), f(1,ia), f(1,IRO), &
        f(1,IRM), f(1,IRN), f(1,IRW), f(1,IP), f(1,IU), f(1,IV), f(1,IW), f(1,IE), f(1,IC), &
        imax, jmax, kmax, 1, inter, ndi, f(1,IVIS), f(1,IEPS2), f(1,IVIST), f(1,iax), &
@@ -290,6 +360,7 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
        f(1,ICH), f(1,IDRDH), f(1,IDRDP), 1, prm(PGAMMA), prm(PFRSDEN), prm(PFRSPRE), &
        hat(1,1), hat(1,2), hat(1,3), hat(1,4), kstr, idir, mulphl, 1, fresul, 0._c_double, f1h, &
        wave, ipr(QLEVEL), ipr(QIDIFF))
+  end if
   if (ipr(QISTRES) >= 1) then
      ! FLXCYC res3c.f:6933-6937 (JADD/KADD of res3c.f:6565-6579)
      jadd = 0; kadd = 0
@@ -384,6 +455,14 @@ subroutine ref_residual_kw(f, ntot, prm, ipr, resi)
           f(1,IS13), f(1,IS22), f(1,IS23), f(1,IS33), f(1,IPTUR), f(1,IRK), f(1,IREPS), ddeps, &
           trueps, f(1,IEPS2), f(1,IVIST), f(1,IVIS), rst, rst, rst, rst, rst, rst, vtran, &
           imax, jmax, kmax, 2, 2, 2, 1, 6, katol)
+  end if
+  if (ipr(QFULLNS) /= 0) then
+     ! res3c.f:3989-4007: TEMDER of T, of DDEPS = RK/RO and of DDEPS = REPS/RO
+     call ref_temder(f, ntot, f(1,ITEMP), IDTDX, ipr)
+     call DIVV12(ddeps, f(1,IRK), f(1,IRO), ntot)
+     call ref_temder(f, ntot, ddeps, IDKDX, ipr)
+     call DIVV12(ddeps, f(1,IREPS), f(1,IRO), ntot)
+     call ref_temder(f, ntot, ddeps, IDEDX, ipr)
   end if
   call ZEROZZ(ddeps, ntot)
   call ref_direction_kw(f, ntot, prm, ipr, 3, fl, zzz, maw, maxw, rksi, hat, zero, trm, sig)

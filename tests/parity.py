@@ -65,6 +65,49 @@ def make_earsm_case(dims, pover, ckw):
     return prm, blk
 
 
+# Full viscous fluxes (FULLNS): FLUXBL with the TEMDER gradients, laminar and SST (FLKEBL),
+# with and without the lambda term (IDIFF = 2 drops it), with the van Leer flux, and under
+# the EARSM; curvilinear cells so that the cell-centre connection is not along the face
+# normal and the Blazek correction contributes
+FULLNS_CASES = [
+    ("ns_curv", (10, 7, 6), dict(fullns=1, rmult=3000.0),
+     dict(kind="all", amp=0.05, distort=0.15, seed=81)),
+    ("ns_idiff2", (7, 9, 6), dict(fullns=1, idiff=2, rmult=3000.0),
+     dict(kind="all", amp=0.03, distort=0.1, seed=83)),
+    ("ns_vanleer", (9, 7, 6), dict(fullns=1, iflux=0, rmult=1000.0),
+     dict(kind="all", amp=0.05, distort=0.12, seed=85)),
+    ("ns_sst", (10, 7, 6), dict(iturb=6, fullns=1),
+     dict(kind="all", amp=0.05, distort=0.15, seed=87, turb=(1.0, 50.0))),
+    ("ns_sst_wall", (9, 8, 5), dict(iturb=6, fullns=1, re=1e14, cfl=2.0),
+     dict(kind="velocity", amp=0.03, jratio=1.3, seed=89, turb=(0.5, 400.0))),
+    ("ns_earsm", (8, 7, 6), dict(iturb=6, istres=1, fullns=1, inter=(3, 2, 55)),
+     dict(kind="all", amp=0.06, distort=0.12, seed=91, turb=(0.3, 80.0))),
+    # ghost-cell centres moved past the first interior cell on the i- and k-low sides: the
+    # boundary faces there see ALFA < 0 (no correction in FLUXBL, ALFA = 0 in FLKEBL)
+    ("ns_sst_backghost", (8, 6, 6), dict(iturb=6, fullns=1),
+     dict(kind="all", amp=0.05, distort=0.1, seed=93, turb=(1.0, 50.0), back_ghost=True)),
+    # one K-plane: TEMDER leaves out the K ghost layer when KMAX = 1, DERFUN does not
+    ("ns_kplane", (10, 8, 1), dict(fullns=1, rmult=2000.0),
+     dict(kind="all", amp=0.05, distort=0.12, seed=95)),
+]
+GRADT = ["DTDX", "DTDY", "DTDZ"]
+GRADKW = ["DKDX", "DKDY", "DKDZ", "DEDX", "DEDY", "DEDZ"]
+
+
+def make_fullns_case(dims, pover, ckw):
+    ckw = dict(ckw)
+    back = ckw.pop("back_ghost", False)
+    prm, blk = make_case(dims, pover, ckw)
+    if prm.istres:
+        seed_bij(blk, seed=ckw.get("seed", 0) + 500)
+    if back:
+        x = {c: blk.view3(c) for c in ("XC", "YC", "ZC")}   # (k, j, i), ghosts included
+        for c in x:
+            x[c][:, :, 1] = x[c][:, :, 2] + 0.6 * (x[c][:, :, 3] - x[c][:, :, 2])
+            x[c][1, :, :] = x[c][2, :, :] + 0.6 * (x[c][3, :, :] - x[c][2, :, :])
+    return prm, blk
+
+
 # fields each stage writes, for the stage-by-stage comparisons
 STATE = ["P", "RO", "E", "C", "VIS", "CH", "CP", "DRDP", "DRDH", "RM", "RN", "RW"]
 RES = ["DRO", "DM", "DN", "DW", "DE"]

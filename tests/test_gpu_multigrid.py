@@ -17,8 +17,8 @@ RTOL = 1e-12
 FORC = ["ROP2H", "RMP2H", "RNP2H", "RWP2H", "EP2H"]
 
 
-def hierarchy(iturb, nlev=2, dims=(16, 8, 8), seed=71):
-    prm = default_params(iturb=iturb, cfll=3.0)
+def hierarchy(iturb, nlev=2, dims=(16, 8, 8), seed=71, fullns=0):
+    prm = default_params(iturb=iturb, cfll=3.0, fullns=fullns)
     blocks = make_hierarchy(*dims, nlev, prm, distort=0.1, kind="all", amp=0.03, seed=seed,
                             turb=(1.0, 150.0) if iturb else None)
     return prm, blocks
@@ -120,5 +120,27 @@ def test_vcycles(hip_device, iturb, nlev, lusgs):
     for m in range(nlev):
         assert_fields_equal(blocks[m], host[m], prim, interior=False,
                             rtol=RTOL if iturb else 0.0, what=f"V-cycle level {m + 1}")
+    np.testing.assert_allclose(r2, r1, rtol=1e-10 if iturb else 1e-13)
+    ctx.close()
+
+
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_vcycles_fullns(hip_device, iturb):
+    """FULLNS V-cycles: FLUXBL (and FLKEBL) with the TEMDER gradients on level 1, the
+    thin-layer fluxes on the coarse levels."""
+    print(f"case FULLNS V-cycles iturb={iturb} nlev=3", flush=True)
+    prm, blocks = hierarchy(iturb, nlev=3, dims=(16, 8, 12), seed=75, fullns=1)
+    host = [b.copy() for b in blocks]
+    oh = OracleHierarchy(prm, host)
+    ctx = context(prm, blocks)
+    for _ in range(3):
+        r1 = oh.vcycle()
+    r2 = ctx.vcycle(3, ncycles=3)
+    for m in range(3):
+        ctx.download(level=m + 1)
+    prim = SST_PRIM if iturb else PRIM
+    for m in range(3):
+        assert_fields_equal(blocks[m], host[m], prim, interior=False,
+                            rtol=RTOL if iturb else 0.0, what=f"FULLNS V-cycle level {m + 1}")
     np.testing.assert_allclose(r2, r1, rtol=1e-10 if iturb else 1e-13)
     ctx.close()

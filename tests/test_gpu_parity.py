@@ -8,8 +8,9 @@ import numpy as np
 import pytest
 
 from oracle.oracle_py import OracleBlock
-from tests.parity import (BIJ, CASES, EARSM_CASES, PRIM, RES, SST_CASES, SST_PRIM, SST_RES,
-                          SST_TURB, STATE, assert_fields_equal, make_case, make_earsm_case)
+from tests.parity import (BIJ, CASES, EARSM_CASES, FULLNS_CASES, GRADKW, GRADT, PRIM, RES,
+                          SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE, assert_fields_equal,
+                          make_case, make_earsm_case, make_fullns_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -235,3 +236,80 @@ def test_earsm_cycles(hip_device, name, dims, pover, ckw, lusgs):
     assert_fields_equal(g, a, SST_PRIM + BIJ, interior=False, rtol=SST_RTOL, what="5 EARSM cycles")
     np.testing.assert_allclose(r2, r1, rtol=1e-10)
     ctx.close()
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", FULLNS_CASES, ids=[c[0] for c in FULLNS_CASES])
+def test_fullns_stages(hip_device, name, dims, pover, ckw):
+    """FULLNS on level 1: the DERFUN and TEMDER gradients (first ghost layer included) bit
+    for bit; the FLUXBL residual bit for bit (laminar) or, with FLKEBL and the SST's tanh
+    upstream, to 1e-12; then IMPS and FIN.  Each GPU stage starts from the oracle's state."""
+    print(f"case {name} dims {dims}", flush=True)
+    prm, blk = make_fullns_case(dims, pover, ckw)
+    kw = prm.iturb == 6
+    a, g = blk.copy(), blk.copy()
+    oa = OracleBlock(prm, a)
+    ctx = _ctx(prm, g)
+    oa.save_old(); ctx.save_old()
+    oa.state(); ctx.statim()
+    oa.turb(); ctx.turbdi(); ctx.download()
+    g.F[:] = a.F
+    ctx.upload()
+    r1 = oa.residual(); r2 = ctx.residi(); ctx.download()
+    assert_fields_equal(g, a, SST_TURB[:10] + GRADT + (GRADKW if kw else []), interior=False,
+                        what="DERFUN+TEMDER")
+    assert_fields_equal(g, a, SST_RES if kw else RES, rtol=SST_RTOL if kw else 0.0,
+                        what="residual with FLUXBL")
+    np.testing.assert_allclose(r2, r1, rtol=1e-11 if kw else 1e-13)
+    g.F[:] = a.F
+    ctx.upload()
+    oa.implicit(); ctx.impsdi(); ctx.download()
+    assert_fields_equal(g, a, ["DTL"] + (SST_RES[:7] if kw else RES), what="IMPS")
+    oa.update(); ctx.fin(); ctx.download()
+    assert_fields_equal(g, a, SST_PRIM if kw else PRIM, interior=False, what="FIN")
+    ctx.close()
+
+
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
+@pytest.mark.parametrize("name,dims,pover,ckw", [FULLNS_CASES[i] for i in (0, 2, 3, 5)],
+                         ids=[FULLNS_CASES[i][0] for i in (0, 2, 3, 5)])
+def test_fullns_cycles(hip_device, name, dims, pover, ckw, lusgs):
+    print(f"case {name} dims {dims} 5 FULLNS cycles, LU-SGS {lusgs}", flush=True)
+    prm, blk = make_fullns_case(dims, pover, ckw)
+    kw = prm.iturb == 6
+    a, g = blk.copy(), blk.copy()
+    oa = OracleBlock(prm, a)
+    ctx = _ctx(prm, g, lusgs)
+    for _ in range(5):
+        r1 = oa.cycle()
+    r2 = ctx.cycle(ncycles=5)
+    ctx.download()
+    assert_fields_equal(g, a, SST_PRIM if kw else PRIM, interior=False,
+                        rtol=SST_RTOL if kw else 0.0, what="5 FULLNS cycles")
+    np.testing.assert_allclose(r2, r1, rtol=1e-10 if kw else 1e-13)
+    ctx.close()
+
+
+@pytest.mark.parametrize("dims", [(70, 9, 17), (5, 3, 2), (130, 6, 9), (40, 24, 1)],
+                         ids=["70x9x17", "5x3x2", "130x6x9", "40x24x1"])
+def test_fullns_residual_tiles_vs_cells_vs_oracle(hip_device, dims):
+    """The FULLNS residual in its tile and per-cell forms against the oracle on ragged
+    tiles, including a single K-plane (TEMDER then leaves out the K ghost layer)."""
+    print(f"case FULLNS residual forms {dims}", flush=True)
+    prm, blk = make_case(dims, dict(fullns=1, rmult=2000.0),
+                         dict(kind="all", amp=0.05, distort=0.12, seed=45))
+    a = blk.copy()
+    oa = OracleBlock(prm, a)
+    oa.save_old(); oa.state(); oa.turb(); r0 = oa.residual()
+    for mode in ("tiles", "cells"):
+        g = blk.copy()
+        from finflo_amd.ffhip import Context
+        ctx = Context(prm)
+        ctx.residual_mode(mode)
+        ctx.register(g)
+        ctx.upload()
+        ctx.save_old(); ctx.statim(); ctx.turbdi(); r = ctx.residi()
+        ctx.download()
+        assert_fields_equal(g, a, GRADT + SST_TURB[:10], interior=False, what=f"gradients ({mode})")
+        assert_fields_equal(g, a, RES, what=f"RESIDU FLUXBL ({mode})")
+        np.testing.assert_allclose(r[:5], r0[:5], rtol=1e-13)
+        ctx.close()

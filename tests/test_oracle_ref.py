@@ -5,9 +5,9 @@ import numpy as np
 import pytest
 
 from oracle.oracle_py import OracleBlock, RefBlock, have_ref
-from tests.parity import (BIJ, CASES, EARSM_CASES, PRIM, RES, SST_CASES, SST_PRIM, SST_RES,
-                          SST_TURB, STATE, make_earsm_case,
-                          assert_fields_equal, make_case)
+from tests.parity import (BIJ, CASES, EARSM_CASES, FULLNS_CASES, GRADKW, GRADT, PRIM, RES,
+                          SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE, make_earsm_case,
+                          make_fullns_case, assert_fields_equal, make_case)
 
 pytestmark = pytest.mark.skipif(not have_ref(), reason="oracle/_ref not built (needs /root/reference)")
 
@@ -161,3 +161,59 @@ def test_earsm_cycles_bit_exact(name, dims, pover, ckw):
         r1, r2 = oa.cycle(), rb.cycle()
         assert np.array_equal(r1, r2)
     assert_fields_equal(a, b, SST_PRIM + BIJ, interior=False, what="4 EARSM cycles")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", FULLNS_CASES, ids=[c[0] for c in FULLNS_CASES])
+def test_fullns_stages_bit_exact(name, dims, pover, ckw):
+    """FULLNS on level 1: DERFUN (laminar: inside RESIDU) and TEMDER with the first ghost
+    layer, FLUXBL in place of FLUXDO and, for the SST, FLKEBL in place of FLUDKE."""
+    prm, blk = make_fullns_case(dims, pover, ckw)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    kw = prm.iturb == 6
+    for o in (oa, rb):
+        o.save_old(); o.state(); o.turb()
+    r1, r2 = oa.residual(), rb.residual()
+    assert_fields_equal(a, b, SST_TURB[:10] + GRADT + (GRADKW if kw else []), interior=False,
+                        what="DERFUN+TEMDER")
+    assert max(np.abs(a.interior(g)).max() for g in GRADT + (GRADKW if kw else [])) > 0, \
+        "TEMDER wrote no gradient"
+    assert_fields_equal(a, b, SST_RES if kw else RES, what="RESIDU/RESIKE with FLUXBL")
+    assert np.array_equal(r1, r2), "RESL2 sums differ"
+    oa.implicit(); rb.implicit()
+    assert_fields_equal(a, b, ["DTL"] + (SST_RES[:7] if kw else RES), what="IMPS")
+    oa.update(); rb.update()
+    assert_fields_equal(a, b, SST_PRIM if kw else PRIM, interior=False, what="FIN")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", FULLNS_CASES[::3], ids=[c[0] for c in FULLNS_CASES[::3]])
+def test_fullns_cycles_bit_exact(name, dims, pover, ckw):
+    prm, blk = make_fullns_case(dims, pover, ckw)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    for _ in range(4):
+        r1, r2 = oa.cycle(), rb.cycle()
+        assert np.array_equal(r1, r2)
+    prim = SST_PRIM if prm.iturb == 6 else PRIM
+    assert_fields_equal(a, b, prim, interior=False, what="4 FULLNS cycles")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw", [FULLNS_CASES[0], FULLNS_CASES[3]],
+                         ids=[FULLNS_CASES[0][0], FULLNS_CASES[3][0]])
+def test_fullns_coarse_level_is_thin_layer(name, dims, pover, ckw):
+    """On a coarse level FULLNS leaves the thin-layer FLUXDO/FLUDKE in place (FLUXBL and
+    FLKEBL run on M = 1 only); the reference still forms the gradients there, which no
+    coarse-level flux reads."""
+    prm, blk = make_fullns_case(dims, pover, ckw)
+    if prm.iturb == 6:
+        rng = np.random.default_rng(61)
+        blk["EPS2"] = 1.0 + rng.uniform(5.0, 50.0, blk.ntot)
+    a, b = blk.copy(), blk.copy()
+    oa, rb = OracleBlock(prm, a, level=2), RefBlock(prm, b, level=2)
+    for o in (oa, rb):
+        o.save_old(); o.state()
+    if prm.iturb == 6:
+        a["VIST"] = (a["EPS2"] - 1.0) * a["VIS"]
+        b["VIST"] = a["VIST"]
+    oa.residual(m2=2); rb.residual(m2=2)
+    assert_fields_equal(a, b, SST_RES if prm.iturb == 6 else RES, what="coarse residual")
