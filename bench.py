@@ -153,6 +153,21 @@ def cpu_baseline(prm, blocks, seconds=20.0):
                       f"({el:.1f} s, one thread)"}
 
 
+def copy_probe(mib):
+    """A known-size HBM copy for counter calibration: an elementwise kernel that reads
+    MIB MiB and writes MIB MiB (y = x * 1), launched three times."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,7 +178,14 @@ def main():
                     help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="instead of the bench: run a plain device copy of MIB MiB three times "
+                         "(the known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE "
+                         "counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(blocks would shard across GPUs; this build measures one GPU)", file=sys.stderr)

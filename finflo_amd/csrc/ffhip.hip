@@ -800,28 +800,38 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
     const double *a1 = FLD(b, FF_A1), *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z);
     const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
     const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
-    Flux7 klo;
-    if (cell) klo = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
-    for (int k = k0; k <= k1; k++) {
-        const int n = cid(b, i, j, k);
-        Flux7 khi;
-        if (cell) {
-            khi = face_flux_kw<true, FNS>(p, sc, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
-            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, n, 1, p.inter[0], a1, a1x, a1y, a1z, d1);
-#pragma unroll
-            for (int q = 0; q < 7; q++) fi[q][tj][ti] = f.f[q];
-            f = face_flux_kw<false, FNS>(p, sc, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
-#pragma unroll
-            for (int q = 0; q < 7; q++) fj[q][tj][ti] = f.f[q];
+    // One call site per face kind keeps the kernel's code within the instruction cache: the
+    // plane loop starts one plane early to form the tile's lowest K-faces, and a plane's I-
+    // and J-faces go through one rolled loop.
+    Flux7 klo, khi;
+    for (int k = k0 - 1; k <= k1; k++) {
+        if (cell) khi = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+        if (k < k0) {
+            klo = khi;
+            continue;
         }
-        if (tj == 0 && ti < il) {
-            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
+        const int n = cid(b, i, j, k);
+        // t = 0: the cell's lower I-face, 1: its lower J-face, 2: the tile's upper J-face
+        // (row 0, one wave) or upper I-face (row 1)
+#pragma unroll 1
+        for (int t = 0; t < 3; t++) {
+            bool act = cell;
+            int L = n, dir = t;
+            double *dst = t == 0 ? &fi[0][tj][ti] : &fj[0][tj][ti];
+            if (t == 2) {
+                act = (tj == 0 && ti < il) || (tj == 1 && ti < jl);
+                dir = tj == 0 ? 1 : 0;
+                L = tj == 0 ? cid(b, i, j0 + jl, k) : cid(b, i0 + il, j0 + ti, k);
+                dst = tj == 0 ? &fj[0][jl][ti] : &fi[0][ti][il];
+            }
+            if (!act) continue;
+            const bool di = dir == 0;
+            const int qs = di ? RJW * (RIW + 1) : (RJW + 1) * RIW;
+            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, L, di ? 1 : b.is, di ? p.inter[0] : p.inter[1],
+                                               di ? a1 : a2, di ? a1x : a2x, di ? a1y : a2y,
+                                               di ? a1z : a2z, di ? d1 : d2);
 #pragma unroll
-            for (int q = 0; q < 7; q++) fj[q][jl][ti] = f.f[q];
-        } else if (tj == 1 && ti < jl) {
-            Flux7 f = face_flux_kw<false, FNS>(p, sc, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z, d1);
-#pragma unroll
-            for (int q = 0; q < 7; q++) fi[q][ti][il] = f.f[q];
+            for (int q = 0; q < 7; q++) dst[q * qs] = f.f[q];
         }
         __syncthreads();
         if (cell) {
@@ -922,30 +932,38 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
     const double *a1 = FLD(b, FF_A1), *a1x = FLD(b, FF_A1X), *a1y = FLD(b, FF_A1Y), *a1z = FLD(b, FF_A1Z);
     const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
     const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
-    Flux5 klo;
-    if (cell) klo = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k0), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
-    for (int k = k0; k <= k1; k++) {
-        const int n = cid(b, i, j, k);
-        Flux5 khi;
-        if (cell) {
-            khi = face_flux<true, IFLUX, FNS>(p, b, n + b.il, b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
-            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, n, 1, p.inter[0], a1, a1x, a1y, a1z, d1);
-#pragma unroll
-            for (int q = 0; q < 5; q++) fi[q][tj][ti] = f.f[q];
-            f = face_flux<false, IFLUX, FNS>(p, b, n, b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
-#pragma unroll
-            for (int q = 0; q < 5; q++) fj[q][tj][ti] = f.f[q];
+    // one call site per face kind (the instruction cache): the plane loop starts one plane
+    // early to form the tile's lowest K-faces; a plane's I- and J-faces share a rolled loop
+    Flux5 klo, khi;
+    for (int k = k0 - 1; k <= k1; k++) {
+        if (cell) khi = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+        if (k < k0) {
+            klo = khi;
+            continue;
         }
-        // upper faces of the tile: the J-face above row jl-1 (wave 0), the I-face after
-        // column il-1 of every row (wave 1)
-        if (tj == 0 && ti < il) {
-            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, cid(b, i, j0 + jl, k), b.is, p.inter[1], a2, a2x, a2y, a2z, d2);
+        const int n = cid(b, i, j, k);
+        // t = 0: the cell's lower I-face, 1: its lower J-face, 2: the tile's upper faces,
+        // the J-face above row jl-1 (wave 0) and the I-face after column il-1 of every row
+        // (wave 1)
+#pragma unroll 1
+        for (int t = 0; t < 3; t++) {
+            bool act = cell;
+            int L = n, dir = t;
+            double *dst = t == 0 ? &fi[0][tj][ti] : &fj[0][tj][ti];
+            if (t == 2) {
+                act = (tj == 0 && ti < il) || (tj == 1 && ti < jl);
+                dir = tj == 0 ? 1 : 0;
+                L = tj == 0 ? cid(b, i, j0 + jl, k) : cid(b, i0 + il, j0 + ti, k);
+                dst = tj == 0 ? &fj[0][jl][ti] : &fi[0][ti][il];
+            }
+            if (!act) continue;
+            const bool di = dir == 0;
+            const int qs = di ? RJ * (RI + 1) : (RJ + 1) * RI;
+            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, L, di ? 1 : b.is, di ? p.inter[0] : p.inter[1],
+                                                   di ? a1 : a2, di ? a1x : a2x, di ? a1y : a2y,
+                                                   di ? a1z : a2z, di ? d1 : d2);
 #pragma unroll
-            for (int q = 0; q < 5; q++) fj[q][jl][ti] = f.f[q];
-        } else if (tj == 1 && ti < jl) {
-            Flux5 f = face_flux<false, IFLUX, FNS>(p, b, cid(b, i0 + il, j0 + ti, k), 1, p.inter[0], a1, a1x, a1y, a1z, d1);
-#pragma unroll
-            for (int q = 0; q < 5; q++) fi[q][ti][il] = f.f[q];
+            for (int q = 0; q < 5; q++) dst[q * qs] = f.f[q];
         }
         __syncthreads();
         if (cell) {
@@ -1331,40 +1349,51 @@ struct Sweep {
     int ntj, ntk;                   // tiles along J and K
     unsigned long long *ticket;     // monotonic; ticket % (ntj*ntk) orders the tiles
     unsigned int *err;              // non-zero after a halo wait gave up
-    unsigned long long *hj, *hk;    // halo slots: J-faces (ntj+1, KMAX, IMAX, HS), K-faces
+    unsigned long long *hj, *hk;    // halo slots of the J-faces and K-faces between tiles
 };
+
+// Halo slots laid out by anti-diagonal.  The edge lanes that exchange across a tile face on
+// one step hold cells whose I plus offset along the face is the same (the step's hyperplane),
+// so with the layout [face f][tile along the face][I + offset][word q][offset] the eight
+// lanes' values of one word are one contiguous, 64-byte aligned run: a poll or a put of a
+// word is a single memory request instead of eight.
+constexpr int HQS = TK;             // stride between the words of one slot
+static_assert(TJ == TK, "one slot stride for both faces");
 
 __device__ __forceinline__ unsigned long long *hj_slot(const Sweep &s, const Blk &b, int f, int i,
                                                        int k)
 {
-    return s.hj + HS * (((size_t)f * b.kmax + (k - 1)) * b.imax + (i - 1));
+    const int kk = (k - 1) % TK, t = (k - 1) / TK;
+    return s.hj + (((size_t)f * s.ntk + t) * (b.imax + TK) + (i + kk - 1)) * (HS * TK) + kk;
 }
 
 __device__ __forceinline__ unsigned long long *hk_slot(const Sweep &s, const Blk &b, int f, int i,
                                                        int j)
 {
-    return s.hk + HS * (((size_t)f * b.jmax + (j - 1)) * b.imax + (i - 1));
+    const int jj = (j - 1) % TJ, t = (j - 1) / TJ;
+    return s.hk + (((size_t)f * s.ntj + t) * (b.imax + TJ) + (i + jj - 1)) * (HS * TJ) + jj;
 }
 
-template <int NV>
+// slot words are QS apart
+template <int NV, int QS = 1>
 __device__ __forceinline__ void halo_put(unsigned long long *slot, const double v[NV])
 {
 #pragma unroll
     for (int q = 0; q < NV; q++)
-        __hip_atomic_store(slot + q, (unsigned long long)__double_as_longlong(v[q]),
+        __hip_atomic_store(slot + q * QS, (unsigned long long)__double_as_longlong(v[q]),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int NV>
+template <int NV, int QS = 1>
 __device__ __forceinline__ void halo_issue(unsigned long long *slot, unsigned long long h[NV])
 {
 #pragma unroll
     for (int q = 0; q < NV; q++)
-        h[q] = __hip_atomic_load(slot + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        h[q] = __hip_atomic_load(slot + q * QS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Complete a halo read begun by halo_issue: poll empty slots, then empty them.
-template <int NV>
+template <int NV, int QS = 1>
 __device__ __forceinline__ void halo_finish(unsigned long long *slot, unsigned long long h[NV],
                                             unsigned int *err, double v[NV])
 {
@@ -1385,10 +1414,10 @@ __device__ __forceinline__ void halo_finish(unsigned long long *slot, unsigned l
                 }
             }
             __builtin_amdgcn_s_sleep(1);
-            x = __hip_atomic_load(slot + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x = __hip_atomic_load(slot + q * QS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         v[q] = __longlong_as_double((long long)x);
-        __hip_atomic_store(slot + q, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(slot + q * QS, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1652,8 +1681,8 @@ __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
             const int n = cid(b, i, j, k);
             const bool hjin = jj == TJ - 1 && j < b.jmax, hkin = kk == TK - 1 && k < b.kmax;
             unsigned long long hv[NV], hw[NV];
-            if (hjin) halo_issue<NV>(hj_slot(sw, b, tj + 1, i, k), hv);
-            if (hkin) halo_issue<NV>(hk_slot(sw, b, tk + 1, i, j), hw);
+            if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), hv);
+            if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), hw);
             SweepRaw nxt;
             if (q + 1 < b.imax) sweep_load<false, KE>(b, n - 1, nxt);
             SweepCell sc;
@@ -1667,7 +1696,7 @@ __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
 #pragma unroll
                 for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + 1];
             } else {
-                halo_finish<NV>(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
+                halo_finish<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
             }
             if (k == b.kmax) {
 #pragma unroll
@@ -1676,7 +1705,7 @@ __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
 #pragma unroll
                 for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane + TJ];
             } else {
-                halo_finish<NV>(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
+                halo_finish<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
             }
             // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
             // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
@@ -1702,13 +1731,13 @@ __global__ void __launch_bounds__(TW) k_pred_tiles(Prm p, Blk b, Sweep sw)
 #pragma unroll
                 for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
             } else if (j > 1) {
-                halo_put<NV>(hj_slot(sw, b, tj, i, k), oj);
+                halo_put<NV, HQS>(hj_slot(sw, b, tj, i, k), oj);
             }
             if (kk > 0) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
             } else if (k > 1) {
-                halo_put<NV>(hk_slot(sw, b, tk, i, j), ok);
+                halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), ok);
             }
             if (q + 1 < b.imax) cur = nxt;
         }
@@ -1744,8 +1773,8 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
             const int n = cid(b, i, j, k);
             const bool hjin = jj == 0 && j > 1, hkin = kk == 0 && k > 1;
             unsigned long long hv[NV], hw[NV];
-            if (hjin) halo_issue<NV>(hj_slot(sw, b, tj, i, k), hv);
-            if (hkin) halo_issue<NV>(hk_slot(sw, b, tk, i, j), hw);
+            if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj, i, k), hv);
+            if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk, i, j), hw);
             SweepRaw nxt;
             if (q + 1 < b.imax) sweep_load<true, KE>(b, n + 1, nxt);
             SweepCell sc;
@@ -1759,7 +1788,7 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
 #pragma unroll
                 for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - 1];
             } else {
-                halo_finish<NV>(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
+                halo_finish<NV, HQS>(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
             }
             if (k == 1) {
 #pragma unroll
@@ -1768,7 +1797,7 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
 #pragma unroll
                 for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane - TJ];
             } else {
-                halo_finish<NV>(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
+                halo_finish<NV, HQS>(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
             }
             double dn[NV];
 #pragma unroll
@@ -1793,13 +1822,13 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
 #pragma unroll
                 for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
             } else if (j < b.jmax) {
-                halo_put<NV>(hj_slot(sw, b, tj + 1, i, k), oj);
+                halo_put<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), oj);
             }
             if (kk < TK - 1) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
             } else if (k < b.kmax) {
-                halo_put<NV>(hk_slot(sw, b, tk + 1, i, j), ok);
+                halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
             }
             if (q + 1 < b.imax) cur = nxt;
         }
@@ -2539,12 +2568,14 @@ static int sweep_alloc(ff_ctx *c, BlockRec &r)
     sk.nti = (b.imax + KI - 1) / KI;
     sk.ntj = (b.jmax + KJ - 1) / KJ;
     // both tile forms leave every slot empty after a sweep, so they share the slots
-    const size_t nhj = (size_t)HS * (sw.ntj + 1) * b.kmax * b.imax;
-    const size_t nhk = (size_t)HS * (sw.ntk + 1) * b.jmax * b.imax;
+    const size_t nhj = (size_t)HS * TK * (sw.ntj + 1) * sw.ntk * (b.imax + TK);
+    const size_t nhk = (size_t)HS * TJ * (sw.ntk + 1) * sw.ntj * (b.imax + TJ);
     const size_t nhi2 = (size_t)HS * (sk.nti + 1) * b.kmax * b.jmax;
     const size_t nhj2 = (size_t)HS * (sk.ntj + 1) * b.kmax * b.imax;
     r.nhalo = std::max(nhj + nhk, nhi2 + nhj2);
-    const size_t head = 4;          // ticket, error word, ticket of the K-marching form, pad
+    // ticket, error word, ticket of the K-marching form, pad to 64 bytes (the slot runs of the
+    // tile form are 64-byte aligned)
+    const size_t head = 8;
     if (hipMalloc(&r.sw_mem, (r.nhalo + head) * sizeof(unsigned long long)) != hipSuccess)
         return fail(c, FF_E_NOMEM, "LU-SGS halo slots%s");
     unsigned long long *m = (unsigned long long *)r.sw_mem;
