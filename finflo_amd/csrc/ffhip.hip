@@ -781,11 +781,423 @@ __device__ __forceinline__ void sourko_cell(const Prm &p, const Blk &b, int L, d
     som = som + rom * (omepro - omedis);
 }
 
+// ------------------------------------------------------------------------------------
+// Solid-wall patches: BOTSKE base3c.f:6147-7095.  Before the residual kernel, k_wall forms
+// per patch face the wall flux that replaces the face flux (loop 1000), the patch arrays,
+// PTUR next to the wall and, with TURCOR and the SST, the corrections of the second and
+// third faces (base3c.f:6763-6897).  The residual kernels then take these at the patch
+// faces (wall_fix).  The force sums (loop 2000) follow in the reference's order on a side
+// stream (k_wall_forces).
+// ------------------------------------------------------------------------------------
+constexpr int NWARR = 23;
+enum { WA_UWALL, WA_VWALL, WA_WWALL, WA_TWALL, WA_HFLUX, WA_RBK, WA_XCP, WA_YCP, WA_ZCP,
+       WA_CPWALL, WA_QWALL, WA_QWFRIC, WA_TAUW1, WA_TAUW2, WA_TAUWX, WA_TAUWY, WA_TAUWZ,
+       WA_SURFX, WA_SURFY, WA_SURFZ, WA_SURFPX, WA_SURFPY, WA_SURFPZ };
+
+// Per block level: the six block faces' cells stacked (face f = dir, dir + 3 for the high
+// face; in-plane index wall_idx), a flag per cell that lies on a wall patch, the wall flux
+// (7 components), the corrections of the faces two and three cells in (2 x 4 components)
+// and the loop-1000 terms of the QY, QZ sums (2).
+struct WallDev {
+    int on, corr_on, nt;
+    int off[6];
+    const unsigned char *flag;
+    double *flux;       // [7][nt]
+    double *corr;       // [2][4][nt]
+    double *qterm;      // [2][nt]
+};
+
+// One patch in BOTSKE's frame: face (0..5), direction, ILMO, KBOTOP, the KX/KY ranges
+// after the J-face swap, the rotated strides ISTR/JSTR/KSTR, IHEAT and the patch arrays
+struct WallPatchDev {
+    int face, dir, idir, kbot, a1, a2, b1, b2, na, sa, sb, sw, iheat, n;
+    double *arr[NWARR];
+};
+
+struct WallEnv {
+    double t0, difpre, xmom, ymom, zmom, gx, gy, gz, ground, altitude;
+};
+
+// in-plane index of cell (i, j, k) on the block faces normal to dir
+__device__ __forceinline__ int wall_idx(const Blk &b, int dir, int i, int j, int k)
+{
+    return dir == 0 ? (k - 1) * b.jmax + (j - 1)
+                    : (dir == 1 ? (k - 1) * b.imax + (i - 1) : (j - 1) * b.imax + (i - 1));
+}
+
+// The face fi (1..nd+1) along dir with in-plane index idx: a wall patch face takes the wall
+// flux; with TURCOR the second and third faces from a wall add its corrections (F3RM(IF2) =
+// F3RM(IF2) + F2VIG ..., base3c.f:6801-6804, 6849-6852).  Registration keeps nd >= 5 where
+// both faces of a direction carry corrections, so they never meet on one face.
+template <int NV>
+__device__ __forceinline__ void wall_fix(const WallDev &w, int dir, int fi, int nd, int idx,
+                                         FluxN<NV> &f)
+{
+    if (!w.on) return;
+    if (fi == 1 || fi == nd + 1) {
+        const int g = w.off[fi == 1 ? dir : dir + 3] + idx;
+        if (w.flag[g]) {
+#pragma unroll
+            for (int q = 0; q < NV; q++) f.f[q] = w.flux[q * w.nt + g];
+        }
+        return;
+    }
+    if (!w.corr_on) return;
+    int g = -1, s = 0;
+    if (fi == 2 || fi == 3) {
+        g = w.off[dir] + idx;
+        s = fi - 2;
+    } else if (fi == nd || fi == nd - 1) {
+        g = w.off[dir + 3] + idx;
+        s = nd - fi;
+    }
+    if (g >= 0 && w.flag[g]) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) f.f[1 + c] = f.f[1 + c] + w.corr[(s * 4 + c) * w.nt + g];
+    }
+}
+
+// BOTSKE loop 1000 (base3c.f:6322-6758) for one face of patch wp, IFLUX = 1, the viscous
+// branch, POROS = 0, no wall functions, UROT = 0, and with TURCOR and ITURB < 9 the
+// corrections of loop 6766-6897.  KW: the FLXCYC call (IDIS = 2, PSIGK1/PSIGE1 = SIGRK1/
+// SIGOM1, CMU = 0.09); otherwise FLUCYC's (IDIS = 0, PSIG = CMU = 0, TURCOR = .FALSE.).
+template <bool KW>
+__global__ void __launch_bounds__(256) k_wall(Prm p, Blk b, WallDev w, WallPatchDev wp,
+                                              WallEnv ev, int level)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= wp.n) return;
+    const int nai = wp.a2 - wp.a1 + 1;
+    const int I = wp.a1 + t % nai, J = wp.b1 + t / nai;
+    const int LN = 2;
+    const int II = (1 + I) * wp.sa + (1 + J) * wp.sb + (1 + wp.kbot) * wp.sw;
+    const int lstrid = wp.sw * wp.idir;
+    const int II2 = II + lstrid, IM1 = II - lstrid, IM2 = IM1 - lstrid;
+    const int IF1 = wp.idir == 1 ? II : II + wp.sw;
+    const int NP = (LN + J - wp.b1) * wp.na + (I - wp.a1 + LN);
+    // natural (i, j, k) of the wall cell and its in-plane index
+    int ci, cj, ck;
+    if (wp.dir == 0) { ci = wp.kbot; cj = I; ck = J; }
+    else if (wp.dir == 1) { ci = J; cj = wp.kbot; ck = I; }
+    else { ci = I; cj = J; ck = wp.kbot; }
+    const int g = w.off[wp.face] + wall_idx(b, wp.dir, ci, cj, ck);
+    const double *A3 = FLD(b, FF_A1 + 4 * wp.dir), *A3X = FLD(b, FF_A1X + 4 * wp.dir),
+                 *A3Y = FLD(b, FF_A1Y + 4 * wp.dir), *A3Z = FLD(b, FF_A1Z + 4 * wp.dir);
+    const double *P = FLD(b, FF_P), *U = FLD(b, FF_U), *V = FLD(b, FF_V), *W = FLD(b, FF_W),
+                 *VOL = FLD(b, FF_VOL), *VIS = FLD(b, FF_VIS), *EPS2 = FLD(b, FF_EPS2),
+                 *VIST = FLD(b, FF_VIST), *PDI = FLD(b, FF_PDIFF), *RO = FLD(b, FF_RO),
+                 *XC = FLD(b, FF_XC), *YC = FLD(b, FF_YC), *ZC = FLD(b, FF_ZC),
+                 *TEMP = FLD(b, FF_TEMP), *CH = FLD(b, FF_CH), *STRAIN = FLD(b, FF_STRAIN),
+                 *RK = FLD(b, FF_RK), *REPS = FLD(b, FF_REPS);
+    double *const *pa = wp.arr;
+    const int turcor = KW ? p.turcor : 0;
+    const double eps3 = 1.E-20, xkappa = 0.4;
+    double rc1 = 1.500, rc2 = rc1 - 1.;
+    if (p.inter[wp.dir] == 6) { rc1 = 1.; rc2 = rc1 - 1.; }
+    double psigk = 0., psige = 0., romfac = 0.0;
+    if (KW) {
+        const SstC sc = coefko(p.kover);
+        psigk = sc.sigrk1; psige = sc.sigom1; romfac = 1.0;
+    }
+    const double repfac = 1.0 - romfac;
+    const double cmu = KW ? 0.09 : 0.;
+    const double prs = p.pr / p.prt;
+    const double ymr = .166666667, yp1 = 1., yp4 = 0.;
+    const double rsum = level == 1 ? 1. : 0.;
+    const double eps = 1.E-3;
+    const double t0 = ev.t0;
+    int ihfl = 0, iht = 1;
+    if (wp.iheat == 2 || wp.iheat >= 12) ihfl = 1;
+    if (wp.iheat == 0) iht = 0;
+    const double vislim = 0.01 * p.frsvis, rolim = .001 * p.frsden;
+    const double xaux1 = KW ? 1. : 0.0, xaux2 = 0.0, xaux3 = 0.0;
+    const double stra2 = STRAIN[II2], strai = STRAIN[II];
+    const double dist = (A3[IF1] + A3[IF1 + lstrid]) / VOL[II];
+    const double surg = -dist * ymr;
+    const double eps2b = 1., vistb = 0.;
+    const double visb = dmax(vislim, rc1 * VIS[II] - rc2 * VIS[II2]);
+    const double chsb = dmax(eps, rc1 * CH[II] - rc2 * CH[II2]);
+    const double straib = dmax(eps, rc1 * strai - rc2 * stra2);
+    const double pb1 = dmax(0.01 * p.frspre, rc1 * P[II] - rc2 * P[II2]);
+    const double rob = dmax(rolim, rc1 * RO[II] - rc2 * RO[II2]);
+    const double epsur2 = dmax(eps, rc1 * REPS[II] - rc2 * REPS[II2]);
+    const double tempb = dmax(0.01 * t0, rc1 * TEMP[II] - rc2 * TEMP[II2]);
+    const double surf = surg * eps2b * visb;
+    const double surfe = surg * (prs * vistb / visb + 1.) * chsb;
+    const double rapub = dmax(1.0E-20, eps2b - 1.0);
+    const double surfrk = surg * (1.0 + psigk * rapub) * visb;
+    const double surfep = surg * (1.0 + psige * rapub) * visb;
+    const double ypro1 = 1. / RO[II], ypro2 = 1. / RO[II2];
+    double rksurf = 0.0;
+    const double sqrk = (sqrt(RK[II] * ypro1) - sqrt(rksurf / rob)) * dist;
+    const double epsur3 = 2. * visb * (sqrk * sqrk);
+    double epsurf = xaux2 * epsur2 / rob + xaux3 * epsur3 / rob;
+    const double rnub = visb / rob;
+    const double utau2 = rnub * straib;
+    const double plusfc = sqrt(utau2) / rnub;
+    const double yplus1 = 2.0 * plusfc / dist;
+    double skpmin = dmin(2.4 * pow(yplus1, 0.85), 8.0);
+    skpmin = dmax(skpmin, eps);
+    const double skplus = dmax(pa[WA_RBK][NP] * plusfc, skpmin);
+    const double pskp = 1.0 / skplus;
+    const double sr1 = 2500.0 * (pskp * pskp);
+    const double sr2 = 100.0 * pskp;
+    const double srcoef = dmax(sr1, sr2);
+    const double omsurf = straib * srcoef;
+    rksurf = 0.0;
+    epsurf = romfac * omsurf + repfac * epsurf;
+    const double usurf = pa[WA_UWALL][NP], vsurf = pa[WA_VWALL][NP], wsurf = pa[WA_WWALL][NP];
+    const double temp1 = TEMP[II], temp2 = TEMP[II + lstrid];
+    double tsurf = t0;
+    if (wp.iheat == 1) {
+        tsurf = pa[WA_TWALL][NP];
+    } else if (wp.iheat != 3 && wp.iheat != 4) {
+        tsurf = t0;
+        pa[WA_TWALL][NP] = tempb;
+    } else if (wp.iheat == 3) {
+        tsurf = tempb;
+        pa[WA_TWALL][NP] = tempb;
+    }
+    const double ihtd = (double)(iht * wp.idir), idd = (double)wp.idir;
+    const double cvi2 = ihtd * (9.000 * temp1 - 1.000 * temp2 - 8. * tsurf);
+    const double uvi2 = idd * (9.000 * U[II] - 1.000 * U[II2] - 8. * usurf);
+    const double vvi2 = idd * (9.000 * V[II] - 1.000 * V[II2] - 8. * vsurf);
+    const double wvi2 = idd * (9.000 * W[II] - 1.000 * W[II2] - 8. * wsurf);
+    const double cvi1 = ihtd * (temp1 - tsurf);
+    const double uvi1 = idd * (U[II] - usurf) * 6.;
+    const double vvi1 = idd * (V[II] - vsurf) * 6.;
+    const double wvi1 = idd * (W[II] - wsurf) * 6.;
+    const double pb4 = dmax(-.99 * p.frspre, rc1 * PDI[II] - rc2 * PDI[II2]);
+    const double pb = yp1 * pb1 + yp4 * pb4;
+    pa[WA_CPWALL][NP] = pb4;
+    double alpha = 1.;
+    const double yp = 1. / dist;
+    if (turcor) {
+        const double du = U[II] - usurf, dv = V[II] - vsurf, dw = W[II] - wsurf;
+        const double uvel = sqrt(du * du + dv * dv + dw * dw);
+        const double para = yp * uvel * RO[II] / VIS[II];
+        alpha = dmin(para / 140., 1.);
+    }
+    const double cvii = (1. - alpha) * cvi1 + alpha * cvi2;
+    const double uvii = (1. - alpha) * uvi1 + alpha * uvi2;
+    const double vvii = (1. - alpha) * vvi1 + alpha * vvi2;
+    const double wvii = (1. - alpha) * wvi1 + alpha * wvi2;
+    const double nx = A3X[IF1], ny = A3Y[IF1], nz = A3Z[IF1], an = A3[IF1];
+    const double duc = .33333 * (ny * vvii + nz * wvii + nx * uvii);
+    const double f2v = surf * (uvii + nx * duc);
+    const double f3v = surf * (vvii + ny * duc);
+    const double f4v = surf * (wvii + nz * duc);
+    const double f5vf = usurf * f2v + vsurf * f3v + wsurf * f4v;
+    const double f5vq = (double)(1 - ihfl) * surfe * cvii + (double)ihfl * pa[WA_HFLUX][NP];
+    const double f5v = f5vf + f5vq;
+    w.qterm[g] = an * rsum * f5v;
+    w.qterm[w.nt + g] = an * rsum * f5vq;
+    double evii = idd * (9.0 * REPS[II] * ypro1 - 1.0 * REPS[II2] * ypro2 - 8. * epsurf);
+    double rkii = idd * (9.000 * RK[II] * ypro1 - 1.000 * RK[II2] * ypro2 - 8. * rksurf);
+    rkii = idd * dmax(idd * rkii, 0.0);
+    const double utau = sqrt(sqrt(f2v * f2v + f3v * f3v + f4v * f4v) / rob);
+    double f6v, f7v;
+    if (!turcor) {
+        f6v = surfrk * rkii * dmax(romfac, xaux1);
+        f7v = surfep * evii * dmax(romfac, xaux1);
+    } else {
+        const double t1 = rob * (utau * utau * utau) / xkappa / (yp * yp);
+        const double t2 = 4. * visb * (utau * utau) / sqrt(cmu) / (yp * yp * yp);
+        const double evik = idd * 6. * yp * dmin(0., -t1 + t2) * (1. + VIST[II] / VIS[II] * psige);
+        f6v = surf * rkii * .5 * (1. + EPS2[II]);
+        f7v = surf * (romfac * evii + repfac * evik);
+        const double rou2 = RO[II] * (utau * utau);
+        const double vv = VIS[II] + VIST[II];
+        FLD(b, FF_PTUR)[II] = VIST[II] * (rou2 * rou2) / (vv * vv);
+    }
+    const double anxi = XC[II + wp.sa] - XC[II - wp.sa], anyi = YC[II + wp.sa] - YC[II - wp.sa];
+    const double anzi = ZC[II + wp.sa] - ZC[II - wp.sa];
+    const double xnori = 1. / sqrt(anxi * anxi + anyi * anyi + anzi * anzi + eps3);
+    const double anxj = XC[II + wp.sb] - XC[II - wp.sb], anyj = YC[II + wp.sb] - YC[II - wp.sb];
+    const double anzj = ZC[II + wp.sb] - ZC[II - wp.sb];
+    const double xnorj = 1. / sqrt(anxj * anxj + anyj * anyj + anzj * anzj + eps3);
+    pa[WA_TAUW1][NP] = -(anxi * f2v + anyi * f3v + anzi * f4v) * xnori;
+    pa[WA_TAUW2][NP] = -(anxj * f2v + anyj * f3v + anzj * f4v) * xnorj;
+    pa[WA_TAUWX][NP] = -f2v;
+    pa[WA_TAUWY][NP] = -f3v;
+    pa[WA_TAUWZ][NP] = -f4v;
+    double fx[7];
+    fx[0] = 0.;
+    fx[1] = an * (nx * pb + f2v);
+    fx[2] = an * (ny * pb + f3v);
+    fx[3] = an * (nz * pb + f4v);
+    fx[4] = an * (0. * pb1 + f5v + f6v);          // UROT = 0
+    fx[5] = an * f6v;
+    fx[6] = an * f7v;
+#pragma unroll
+    for (int q = 0; q < 7; q++) w.flux[q * w.nt + g] = fx[q];
+    pa[WA_QWFRIC][NP] = idd * (f5vf + 0. * pb);
+    pa[WA_QWALL][NP] = idd * f5vq;
+    if (KW && turcor && p.iturb < 9) {
+        // the second and third faces from the wall (base3c.f:6766-6897)
+        const int II3 = II + 2 * lstrid, IF2 = IF1 + lstrid, IF3 = IF1 + 2 * lstrid;
+        double ypd2 = A3[IF2] / (VOL[II2] + VOL[II]);
+        double sg = -A3[IF2] * ypd2;
+        double sf = sg * (-.33333 * VIST[II2] + VIST[II]);
+        double sfe = sg * ((-.33333 * prs * VIST[II2] / VIS[II2]) * CH[II2]
+                           + (prs * VIST[II] / VIS[II]) * CH[II]);
+        double dc = .333333 * (A3X[IF2] * (U[II2] - U[II]) + A3Y[IF2] * (V[II2] - V[II])
+                               + A3Z[IF2] * (W[II2] - W[II]));
+        double c2 = sf * (U[II2] - U[II] + A3X[IF2] * dc);
+        double c3 = sf * (V[II2] - V[II] + A3Y[IF2] * dc);
+        double c4 = sf * (W[II2] - W[II] + A3Z[IF2] * dc);
+        double c5 = .5 * ((U[II2] + U[II]) * c2 + (V[II2] + V[II]) * c3 + (W[II2] + W[II]) * c4)
+                  + sfe * (TEMP[II2] - TEMP[II]);
+        w.corr[0 * w.nt + g] = c2; w.corr[1 * w.nt + g] = c3;
+        w.corr[2 * w.nt + g] = c4; w.corr[3 * w.nt + g] = c5;
+        ypd2 = A3[IF3] / (VOL[II3] + VOL[II2]);
+        sg = -A3[IF3] * ypd2;
+        sf = sg * (-.2 * VIST[II3] + .3333 * VIST[II2]);
+        sfe = sg * ((-.2 * prs * VIST[II3] / VIS[II3]) * CH[II3]
+                    + (+.3333 * prs * VIST[II2] / VIS[II2]) * CH[II2]);
+        dc = .2 * (A3X[IF3] * (U[II3] - U[II2]) + A3Y[IF3] * (V[II3] - V[II2])
+                   + A3Z[IF3] * (W[II3] - W[II2]));
+        c2 = sf * (U[II3] - U[II2] + A3X[IF3] * dc);
+        c3 = sf * (V[II3] - V[II2] + A3Y[IF3] * dc);
+        c4 = sf * (W[II3] - W[II2] + A3Z[IF3] * dc);
+        c5 = .5 * ((U[II3] + U[II2]) * c2 + (V[II3] + V[II2]) * c3 + (W[II3] + W[II2]) * c4)
+           + sfe * (TEMP[II3] - TEMP[II2]);
+        w.corr[4 * w.nt + g] = c2; w.corr[5 * w.nt + g] = c3;
+        w.corr[6 * w.nt + g] = c4; w.corr[7 * w.nt + g] = c5;
+    }
+}
+
+// BOTSKE loop 2000 (base3c.f:6988-7093) on level 1: the force sums of every patch (one wave
+// per patch) in the reference's face order.  A chunk of 64 faces is prepared by the lanes
+// in parallel (moment-arm products, pressure parts), the sums run through them in order on
+// lane 0 (the reference's association, e.g. CMX = (CMX + BZ*YD) - BY*ZD), and the lanes then
+// form SURFPX.. from the running DX.. that lane 0 left in LDS.
+constexpr int WF_T = 29;
+__global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
+                                                    const WallPatchDev *wps, WallEnv ev,
+                                                    double *sums)
+{
+    const WallPatchDev wp = wps[blockIdx.x];
+    const int lane = threadIdx.x;
+    __shared__ double tm[WF_T][64];
+    __shared__ double run[3][64];
+    const double *A3 = FLD(b, FF_A1 + 4 * wp.dir), *A3X = FLD(b, FF_A1X + 4 * wp.dir),
+                 *A3Y = FLD(b, FF_A1Y + 4 * wp.dir), *A3Z = FLD(b, FF_A1Z + 4 * wp.dir);
+    const double *P = FLD(b, FF_P), *PDI = FLD(b, FF_PDIFF);
+    double *const *pa = wp.arr;
+    const int LN = 2, nai = wp.a2 - wp.a1 + 1;
+    const int lstrid = wp.sw * wp.idir;
+    double rc1 = 1.500, rc2 = rc1 - 1.;
+    if (p.inter[wp.dir] == 6) { rc1 = 1.; rc2 = rc1 - 1.; }
+    const double ompor = 1. - 0.0, ypg = ev.difpre, yp1 = 1., yp4 = 0.;
+    const double idd = (double)wp.idir;
+    double s[FF_NFORCE];
+#pragma unroll
+    for (int q = 0; q < FF_NFORCE; q++) s[q] = 0.;
+    for (int c0 = 0; c0 < wp.n; c0 += 64) {
+        const int t = c0 + lane;
+        int NP = 0, g = 0;
+        double an = 0.;
+        if (t < wp.n) {
+            const int I = wp.a1 + t % nai, J = wp.b1 + t / nai;
+            const int II = (1 + I) * wp.sa + (1 + J) * wp.sb + (1 + wp.kbot) * wp.sw;
+            const int II2 = II + lstrid;
+            const int IF1 = wp.idir == 1 ? II : II + wp.sw;
+            NP = (LN + J - wp.b1) * wp.na + (I - wp.a1 + LN);
+            int ci, cj, ck;
+            if (wp.dir == 0) { ci = wp.kbot; cj = I; ck = J; }
+            else if (wp.dir == 1) { ci = J; cj = wp.kbot; ck = I; }
+            else { ci = I; cj = J; ck = wp.kbot; }
+            g = w.off[wp.face] + wall_idx(b, wp.dir, ci, cj, ck);
+            an = A3[IF1];
+            const double nx = A3X[IF1], ny = A3Y[IF1], nz = A3Z[IF1];
+            const double xd = pa[WA_XCP][NP] - ev.xmom, yd = pa[WA_YCP][NP] - ev.ymom;
+            const double zd = pa[WA_ZCP][NP] - ev.zmom;
+            double deltap = p.frsden * (ev.gx * (pa[WA_XCP][NP] - ev.ground)
+                                        + ev.gy * (pa[WA_YCP][NP] - ev.ground)
+                                        + ev.gz * (pa[WA_ZCP][NP] - ev.ground));
+            if (ev.altitude >= ev.ground) deltap = 0.0;
+            const double frm = w.flux[1 * w.nt + g], frn = w.flux[2 * w.nt + g];
+            const double frw = w.flux[3 * w.nt + g], fe = w.flux[4 * w.nt + g];
+            const double frk = w.flux[5 * w.nt + g];
+            double bx = -idd * (frm - an * ompor * nx * ypg);
+            double by = -idd * (frn - an * ompor * ny * ypg);
+            double bz = -idd * (frw - an * ompor * nz * ypg);
+            const double bxdp = -idd * (-an * ompor * nx * (-deltap));
+            const double bydp = -idd * (-an * ompor * ny * (-deltap));
+            const double bzdp = -idd * (-an * ompor * nz * (-deltap));
+            bx = bx + bxdp;
+            by = by + bydp;
+            bz = bz + bzdp;
+            pa[WA_SURFX][NP] = bx / (an + 1.E-20);
+            pa[WA_SURFY][NP] = by / (an + 1.E-20);
+            pa[WA_SURFZ][NP] = bz / (an + 1.E-20);
+            const double pbb1 = dmax(0.01 * p.frspre, rc1 * P[II] - rc2 * P[II2]);
+            const double pbb4 = rc1 * PDI[II] - rc2 * PDI[II2];
+            const double pb = yp1 * pbb1 + yp4 * pbb4;
+            const double qx_ = an * ompor * nx * (pb - ypg), qy_ = an * ompor * ny * (pb - ypg);
+            const double qz_ = an * ompor * nz * (pb - ypg);
+            const double bx2 = bx + idd * qx_, by2 = by + idd * qy_, bz2 = bz + idd * qz_;
+            const double v[WF_T] = {bz * yd, by * zd, bx * zd, bz * xd, by * xd, bx * yd,
+                                    bx, by, bz, idd * (fe - frk), idd * qx_, idd * qy_, idd * qz_,
+                                    bz2 * yd, by2 * zd, bx2 * zd, bz2 * xd, by2 * xd, bx2 * yd,
+                                    bxdp, bydp, bzdp, idd * fe, w.qterm[g], w.qterm[w.nt + g],
+                                    -idd * qx_, -idd * qy_, -idd * qz_, 0.};
+#pragma unroll
+            for (int q = 0; q < WF_T; q++) tm[q][lane] = v[q];
+        }
+        __syncthreads();
+        const int m = min(64, wp.n - c0);
+        if (lane == 0) {
+            for (int r = 0; r < m; r++) {
+                s[FF_CMX] = s[FF_CMX] + tm[0][r]; s[FF_CMX] = s[FF_CMX] - tm[1][r];
+                s[FF_CMY] = s[FF_CMY] + tm[2][r]; s[FF_CMY] = s[FF_CMY] - tm[3][r];
+                s[FF_CMZ] = s[FF_CMZ] + tm[4][r]; s[FF_CMZ] = s[FF_CMZ] - tm[5][r];
+                s[FF_CX] = s[FF_CX] + tm[6][r];
+                s[FF_CY] = s[FF_CY] + tm[7][r];
+                s[FF_CZ] = s[FF_CZ] + tm[8][r];
+                s[FF_TOM] = s[FF_TOM] - tm[9][r];
+                s[FF_DX] = s[FF_DX] - tm[10][r];
+                s[FF_DY] = s[FF_DY] - tm[11][r];
+                s[FF_DZ] = s[FF_DZ] - tm[12][r];
+                run[0][r] = s[FF_DX]; run[1][r] = s[FF_DY]; run[2][r] = s[FF_DZ];
+                s[FF_VMX] = s[FF_VMX] + tm[13][r]; s[FF_VMX] = s[FF_VMX] - tm[14][r];
+                s[FF_VMY] = s[FF_VMY] + tm[15][r]; s[FF_VMY] = s[FF_VMY] - tm[16][r];
+                s[FF_VMZ] = s[FF_VMZ] + tm[17][r]; s[FF_VMZ] = s[FF_VMZ] - tm[18][r];
+                s[FF_BUX] = s[FF_BUX] + tm[19][r];
+                s[FF_BUY] = s[FF_BUY] + tm[20][r];
+                s[FF_BUZ] = s[FF_BUZ] + tm[21][r];
+                s[FF_QX] = s[FF_QX] - tm[22][r];
+                s[FF_QY] = s[FF_QY] + tm[23][r];
+                s[FF_QY] = s[FF_QY] + 0.;                    // + RSUM*UROT*PB, UROT = 0
+                s[FF_QZ] = s[FF_QZ] + tm[24][r];
+            }
+        }
+        __syncthreads();
+        if (t < wp.n) {
+            pa[WA_SURFPX][NP] = (run[0][lane] + tm[25][lane]) / (an + 1.E-20);
+            pa[WA_SURFPY][NP] = (run[1][lane] + tm[26][lane]) / (an + 1.E-20);
+            pa[WA_SURFPZ][NP] = (run[2][lane] + tm[27][lane]) / (an + 1.E-20);
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        s[FF_TOM] = s[FF_TOM] + idd * s[FF_QZ];
+        s[FF_QZ] = idd * s[FF_QZ];
+        s[FF_QY] = idd * s[FF_QY];
+        s[FF_DX] = s[FF_DX] + s[FF_BUX];
+        s[FF_DY] = s[FF_DY] + s[FF_BUY];
+        s[FF_DZ] = s[FF_DZ] + s[FF_BUZ];
+#pragma unroll
+        for (int q = 0; q < FF_NFORCE; q++) sums[blockIdx.x * FF_NFORCE + q] = s[q];
+    }
+}
+
 // RESIKE face fluxes (FLXCYC res3c.f:6424-7566 for k, i, j with SWEEPK) and SOURKO/ADDSKE
 // (res3c.f:4957-5068), with every face flux formed once: the layout of k_residual_tile with
 // seven components.  FNS: the FULLNS face fluxes FLKEBL + FLUXBL (level 1).
 template <int RIW, int RJW, int RKW, bool FNS = false>
-__global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
+__global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev wd)
 {
     __shared__ double fi[7][RJW][RIW + 1];
     __shared__ double fj[7][RJW + 1][RIW];
@@ -805,7 +1217,10 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
     // and J-faces go through one rolled loop.
     Flux7 klo, khi;
     for (int k = k0 - 1; k <= k1; k++) {
-        if (cell) khi = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+        if (cell) {
+            khi = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+            wall_fix<7>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), khi);
+        }
         if (k < k0) {
             klo = khi;
             continue;
@@ -816,12 +1231,14 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
 #pragma unroll 1
         for (int t = 0; t < 3; t++) {
             bool act = cell;
-            int L = n, dir = t;
+            int L = n, dir = t, fx = i, fy = j;
             double *dst = t == 0 ? &fi[0][tj][ti] : &fj[0][tj][ti];
             if (t == 2) {
                 act = (tj == 0 && ti < il) || (tj == 1 && ti < jl);
                 dir = tj == 0 ? 1 : 0;
-                L = tj == 0 ? cid(b, i, j0 + jl, k) : cid(b, i0 + il, j0 + ti, k);
+                fx = tj == 0 ? i : i0 + il;
+                fy = tj == 0 ? j0 + jl : j0 + ti;
+                L = cid(b, fx, fy, k);
                 dst = tj == 0 ? &fj[0][jl][ti] : &fi[0][ti][il];
             }
             if (!act) continue;
@@ -830,6 +1247,8 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
             Flux7 f = face_flux_kw<false, FNS>(p, sc, b, L, di ? 1 : b.is, di ? p.inter[0] : p.inter[1],
                                                di ? a1 : a2, di ? a1x : a2x, di ? a1y : a2y,
                                                di ? a1z : a2z, di ? d1 : d2);
+            wall_fix<7>(wd, dir, di ? fx : fy, di ? b.imax : b.jmax,
+                        di ? wall_idx(b, 0, 0, fy, k) : wall_idx(b, 1, fx, 0, k), f);
 #pragma unroll
             for (int q = 0; q < 7; q++) dst[q * qs] = f.f[q];
         }
@@ -864,7 +1283,7 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b)
 // cell are evaluated by the same code as the neighbour evaluates them, so the shared face
 // values agree bit for bit.
 template <int IFLUX, bool FNS = false>
-__global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
+__global__ void __launch_bounds__(128) k_residual(Prm p, Blk b, WallDev wd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y + 1;
@@ -878,6 +1297,8 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
                                    FLD(b, FF_A3Y), FLD(b, FF_A3Z), FLD(b, FF_D3));
         Flux5 hi = face_flux<true, IFLUX, FNS>(p, b, n + S, S, p.inter[2], FLD(b, FF_A3), FLD(b, FF_A3X),
                                    FLD(b, FF_A3Y), FLD(b, FF_A3Z), FLD(b, FF_D3));
+        wall_fix<5>(wd, 2, k, b.kmax, wall_idx(b, 2, i, j, 0), lo);
+        wall_fix<5>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), hi);
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
     }
@@ -887,6 +1308,8 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
                                     FLD(b, FF_A1Y), FLD(b, FF_A1Z), FLD(b, FF_D1));
         Flux5 hi = face_flux<false, IFLUX, FNS>(p, b, n + S, S, p.inter[0], FLD(b, FF_A1), FLD(b, FF_A1X),
                                     FLD(b, FF_A1Y), FLD(b, FF_A1Z), FLD(b, FF_D1));
+        wall_fix<5>(wd, 0, i, b.imax, wall_idx(b, 0, 0, j, k), lo);
+        wall_fix<5>(wd, 0, i + 1, b.imax, wall_idx(b, 0, 0, j, k), hi);
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = d[q] - (hi.f[q] - lo.f[q]);
     }
@@ -896,6 +1319,8 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b)
                                     FLD(b, FF_A2Y), FLD(b, FF_A2Z), FLD(b, FF_D2));
         Flux5 hi = face_flux<false, IFLUX, FNS>(p, b, n + S, S, p.inter[1], FLD(b, FF_A2), FLD(b, FF_A2X),
                                     FLD(b, FF_A2Y), FLD(b, FF_A2Z), FLD(b, FF_D2));
+        wall_fix<5>(wd, 1, j, b.jmax, wall_idx(b, 1, i, 0, k), lo);
+        wall_fix<5>(wd, 1, j + 1, b.jmax, wall_idx(b, 1, i, 0, k), hi);
         double dt1 = 1. / FLD(b, FF_VOL)[n];
 #pragma unroll
         for (int q = 0; q < 5; q++) d[q] = (d[q] - (hi.f[q] - lo.f[q])) * dt1;
@@ -918,7 +1343,7 @@ constexpr int RI = 64, RJ = 4, RKC = 8;
 static_assert(RJ >= 2 && RJ <= RI, "the tile's upper I-faces are formed by the lanes of row 1");
 
 template <int IFLUX, bool FNS = false>
-__global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
+__global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev wd)
 {
     __shared__ double fi[5][RJ][RI + 1];
     __shared__ double fj[5][RJ + 1][RI];
@@ -936,7 +1361,10 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
     // early to form the tile's lowest K-faces; a plane's I- and J-faces share a rolled loop
     Flux5 klo, khi;
     for (int k = k0 - 1; k <= k1; k++) {
-        if (cell) khi = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+        if (cell) {
+            khi = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+            wall_fix<5>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), khi);
+        }
         if (k < k0) {
             klo = khi;
             continue;
@@ -948,12 +1376,14 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
 #pragma unroll 1
         for (int t = 0; t < 3; t++) {
             bool act = cell;
-            int L = n, dir = t;
+            int L = n, dir = t, fx = i, fy = j;
             double *dst = t == 0 ? &fi[0][tj][ti] : &fj[0][tj][ti];
             if (t == 2) {
                 act = (tj == 0 && ti < il) || (tj == 1 && ti < jl);
                 dir = tj == 0 ? 1 : 0;
-                L = tj == 0 ? cid(b, i, j0 + jl, k) : cid(b, i0 + il, j0 + ti, k);
+                fx = tj == 0 ? i : i0 + il;
+                fy = tj == 0 ? j0 + jl : j0 + ti;
+                L = cid(b, fx, fy, k);
                 dst = tj == 0 ? &fj[0][jl][ti] : &fi[0][ti][il];
             }
             if (!act) continue;
@@ -962,6 +1392,8 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b)
             Flux5 f = face_flux<false, IFLUX, FNS>(p, b, L, di ? 1 : b.is, di ? p.inter[0] : p.inter[1],
                                                    di ? a1 : a2, di ? a1x : a2x, di ? a1y : a2y,
                                                    di ? a1z : a2z, di ? d1 : d2);
+            wall_fix<5>(wd, dir, di ? fx : fy, di ? b.imax : b.jmax,
+                        di ? wall_idx(b, 0, 0, fy, k) : wall_idx(b, 1, fx, 0, k), f);
 #pragma unroll
             for (int q = 0; q < 5; q++) dst[q * qs] = f.f[q];
         }
@@ -2413,6 +2845,24 @@ __global__ void __launch_bounds__(256) k_limits(Prm p, Blk b)
 // host side
 // ------------------------------------------------------------------------------------
 
+// The solid-wall patches of a block level: the caller's descriptors and host arrays, the
+// patches in BOTSKE's frame with device arrays, and the per-face scratch of WallDev
+struct WallRec {
+    std::vector<ff_wall_patch> host;
+    std::vector<WallPatchDev> dev;
+    WallPatchDev *dev_tab = nullptr;   // device copy of dev (the force pass)
+    double *parr = nullptr;            // device patch arrays
+    unsigned char *flag = nullptr;
+    double *scr = nullptr;             // flux, corrections, QY/QZ terms
+    double *sums = nullptr;            // FF_NFORCE per patch (device)
+    double *sums_host = nullptr;       // pinned copy
+    WallDev wd{};
+    WallEnv env{};
+    hipEvent_t fwd = nullptr;          // recorded on the main stream before the force pass
+    hipEvent_t done = nullptr;         // recorded on the side stream after it
+    bool pending = false;              // a force pass may still read the scratch
+};
+
 struct BlockRec {
     ff_block host{};
     Blk dev{};
@@ -2425,6 +2875,7 @@ struct BlockRec {
     double *kw_scr = nullptr;  // k-omega: per-workgroup maxima of SEPS and their maximum
     int kw_parts = 0;
     int level = 0;             // multigrid level M the slab was registered as
+    WallRec walls;
 };
 
 struct PendingEv {
@@ -2451,6 +2902,7 @@ struct ff_ctx {
     // launches and event-timed milliseconds per kernel name, and per "name@level"
     std::map<std::string, std::pair<long long, double>> stats;
     int lev = 0;                  // level of the stage being launched (statistics only)
+    hipStream_t wstream = nullptr;   // side stream of the wall force sums (default priority)
 };
 
 static long long key(int level, int block) { return ((long long)level << 32) | (unsigned)block; }
@@ -2486,6 +2938,7 @@ static Prm make_prm(const ff_params *p, int level)
     k.kover = p->kover; k.katol = p->katol; k.istres = p->istres;
     k.fullns = level == 1 ? p->fullns : 0;       // FLUCYC/FLXCYC: FULLNS .AND. M == 1
     k.idiff = p->idiff;
+    k.turcor = p->turcor;
     k.cmgk = p->cmgk; k.cmgeps = p->cmgeps;
     for (int d = 0; d < 3; d++) k.inter[d] = level >= 2 ? 55 : p->inter[d];   // res3c RESIDI ns3c.f:2468-2472
     return k;
@@ -2493,21 +2946,27 @@ static Prm make_prm(const ff_params *p, int level)
 
 // Launch helper with optional event timing on the context's stream
 template <typename K, typename... A>
-static int launch(ff_ctx *c, const char *name, K kern, dim3 g, dim3 t, A... args)
+static int launch_on(ff_ctx *c, hipStream_t st, const char *name, K kern, dim3 g, dim3 t, A... args)
 {
     hipEvent_t ea = nullptr, eb = nullptr;
     if (c->profile) {
         HIPCHK(c, hipEventCreate(&ea));
         HIPCHK(c, hipEventCreate(&eb));
-        HIPCHK(c, hipEventRecord(ea, c->stream));
+        HIPCHK(c, hipEventRecord(ea, st));
     }
-    hipLaunchKernelGGL(kern, g, t, 0, c->stream, args...);
+    hipLaunchKernelGGL(kern, g, t, 0, st, args...);
     HIPCHK(c, hipGetLastError());
     if (c->profile) {
-        HIPCHK(c, hipEventRecord(eb, c->stream));
+        HIPCHK(c, hipEventRecord(eb, st));
         c->pending.push_back({name, c->lev, ea, eb});
     }
     return FF_OK;
+}
+
+template <typename K, typename... A>
+static int launch(ff_ctx *c, const char *name, K kern, dim3 g, dim3 t, A... args)
+{
+    return launch_on(c, c->stream, name, kern, g, t, args...);
 }
 
 static int drain_events(ff_ctx *c)
@@ -2600,8 +3059,23 @@ static int sweep_alloc(ff_ctx *c, BlockRec &r)
     return FF_OK;
 }
 
+static void walls_free(WallRec &w)
+{
+    if (w.pending && w.done) (void)hipEventSynchronize(w.done);
+    if (w.dev_tab) (void)hipFree(w.dev_tab);
+    if (w.parr) (void)hipFree(w.parr);
+    if (w.flag) (void)hipFree(w.flag);
+    if (w.scr) (void)hipFree(w.scr);
+    if (w.sums) (void)hipFree(w.sums);
+    if (w.sums_host) (void)hipHostFree(w.sums_host);
+    if (w.fwd) (void)hipEventDestroy(w.fwd);
+    if (w.done) (void)hipEventDestroy(w.done);
+    w = WallRec();
+}
+
 static void sweep_free(BlockRec &r)
 {
+    walls_free(r.walls);
     if (r.sw_mem) (void)hipFree(r.sw_mem);
     r.sw_mem = nullptr;
     if (r.kw_scr) (void)hipFree(r.kw_scr);
@@ -2719,6 +3193,7 @@ int ff_ctx_destroy(ff_ctx *c)
     if (c->red_out) (void)hipFree(c->red_out);
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->wstream) (void)hipStreamDestroy(c->wstream);
     delete c;
     return FF_OK;
 }
@@ -2868,6 +3343,7 @@ int ff_finish(ff_ctx *c)
 {
     if (!c) return FF_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->wstream) HIPCHK(c, hipStreamSynchronize(c->wstream));
     return check_all(c);
 }
 
@@ -2926,6 +3402,39 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
         return kw ? launch(c, "k_temder", k_temder<true>, g, dim3(256), b, gk)
                   : launch(c, "k_temder", k_temder<false>, g, dim3(256), b, gk);
     };
+    // BOTSKE at the solid-wall patches: the wall fluxes, patch arrays, PTUR next to the wall
+    // and the near-wall corrections, per direction K, I, J in patch order (the order of the
+    // FLXCYC/FLUCYC calls, res3c.f:7075-7157), ahead of the residual kernel that uses them
+    WallRec &wr = r->walls;
+    auto walls = [&]() -> int {
+        if (wr.dev.empty()) return FF_OK;
+        int e;
+        if (wr.pending) HIPCHK(c, hipStreamWaitEvent(c->stream, wr.done, 0));
+        Prm kp = make_prm(&c->prm, level);
+        for (int dir : {2, 0, 1})
+            for (const WallPatchDev &wp : wr.dev) {
+                if (wp.dir != dir) continue;
+                dim3 g((wp.n + 255) / 256);
+                e = kw ? launch(c, "k_wall", k_wall<true>, g, dim3(256), kp, b, wr.wd, wp, wr.env, level)
+                       : launch(c, "k_wall", k_wall<false>, g, dim3(256), kp, b, wr.wd, wp, wr.env, level);
+                if (e) return e;
+            }
+        return FF_OK;
+    };
+    // the force sums of level 1 (BOTSKE loop 2000) on the side stream, overlapping the
+    // stages that follow; the next wall pass of this block waits for them
+    auto forces = [&]() -> int {
+        if (wr.dev.empty() || level != 1) return FF_OK;
+        HIPCHK(c, hipEventRecord(wr.fwd, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->wstream, wr.fwd, 0));
+        int e = launch_on(c, c->wstream, "k_wall_forces", k_wall_forces, dim3((unsigned)wr.dev.size()),
+                          dim3(64), make_prm(&c->prm, level), b, wr.wd,
+                          (const WallPatchDev *)wr.dev_tab, wr.env, wr.sums);
+        if (e) return e;
+        HIPCHK(c, hipEventRecord(wr.done, c->wstream));
+        wr.pending = true;
+        return FF_OK;
+    };
     // ZEROZZ of DRO..DE (DRK, DEPS) on every cell (res3c.f:194-198, 3668-3674): ghost
     // residuals stay zero
     for (int q = 0; q < nq; q++) {
@@ -2945,25 +3454,28 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
         if ((rc = launch(c, "k_fun1_prod", k_fun1_prod, cells_grid(b, 256), dim3(256), kp, b,
                          (const double *)(r->kw_scr + r->kw_parts)))) return rc;
         if (fns && (rc = gradients())) return rc;
+        if ((rc = walls())) return rc;
         dim3 g((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
-        rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b)
-                 : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b);
+        rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b, wr.wd)
+                 : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b, wr.wd);
     } else {
         if (fns && (rc = gradients())) return rc;
+        if ((rc = walls())) return rc;
         if (c->residual_mode == FF_RESIDUAL_TILES) {
             dim3 g((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ, (b.kmax + RKC - 1) / RKC);
             auto kern = fns ? (vl ? k_residual_tile<0, true> : k_residual_tile<1, true>)
                             : (vl ? k_residual_tile<0> : k_residual_tile<1>);
             rc = launch(c, fns ? "k_residual_tile_ns" : "k_residual_tile", kern, g,
-                        dim3(RI * RJ), make_prm(&c->prm, level), b);
+                        dim3(RI * RJ), make_prm(&c->prm, level), b, wr.wd);
         } else {
             auto kern = fns ? (vl ? k_residual<0, true> : k_residual<1, true>)
                             : (vl ? k_residual<0> : k_residual<1>);
             rc = launch(c, fns ? "k_residual_ns" : "k_residual", kern, cells_grid(b, 128),
-                        dim3(128), make_prm(&c->prm, level), b);
+                        dim3(128), make_prm(&c->prm, level), b, wr.wd);
         }
     }
     if (rc) return rc;
+    if ((rc = forces())) return rc;
     if (level == 1 && resi) {
         int ncell = b.imax * b.jmax * b.kmax;
         int parts = (ncell + RED_THREADS - 1) / RED_THREADS;
@@ -3233,4 +3745,166 @@ int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
     return FF_OK;
 }
 
+
+// ---- solid-wall patches ---------------------------------------------------------------
+
+static const int kNeededWall[] = {FF_XC, FF_YC, FF_ZC, FF_STRAIN, FF_RK, FF_REPS, FF_PTUR};
+
+// the patch arrays of a descriptor in the order of the WA_* slots
+static void wall_arrays(const ff_wall_patch &h, double *a[NWARR])
+{
+    double *const v[NWARR] = {h.uwall, h.vwall, h.wwall, h.twall, h.hflux, h.rbk, h.xcp, h.ycp,
+                              h.zcp, h.cpwall, h.qwall, h.qwfric, h.tauw1, h.tauw2, h.tauwx,
+                              h.tauwy, h.tauwz, h.surfx, h.surfy, h.surfz, h.surfpx, h.surfpy,
+                              h.surfpz};
+    for (int q = 0; q < NWARR; q++) a[q] = v[q];
+}
+
+int ff_wall_register(ff_ctx *c, int level, int block, const ff_wall_params *wpar,
+                     const ff_wall_patch *patches, int npatch)
+{
+    if (!c || npatch < 0 || (npatch > 0 && (!patches || !wpar))) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    walls_free(r->walls);
+    if (npatch == 0) return FF_OK;
+    const Blk &b = r->dev;
+    for (int f : kNeededWall)
+        if (!b.f[f]) return fail(c, FF_E_ARG, "wall patches need field %s", std::to_string(f).c_str());
+    const int nd[3] = {b.imax, b.jmax, b.kmax};
+    const int nf[3] = {b.jmax * b.kmax, b.imax * b.kmax, b.imax * b.jmax};
+    WallRec w;
+    WallDev &wd = w.wd;
+    wd.off[0] = 0;
+    for (int f = 1; f < 6; f++) wd.off[f] = wd.off[f - 1] + nf[(f - 1) % 3];
+    wd.nt = wd.off[5] + nf[2];
+    std::vector<unsigned char> flag((size_t)wd.nt, 0);
+    const bool corr = c->prm.iturb == 6 && c->prm.turcor;
+    bool has_face[6] = {false, false, false, false, false, false};
+    size_t nparr = 0;
+    for (int l = 0; l < npatch; l++) {
+        const ff_wall_patch &h = patches[l];
+        if (h.iface < 1 || h.iface > 6) return fail(c, FF_E_ARG, "wall patch face %s", std::to_string(h.iface).c_str());
+        if (h.iheat < 0 || h.iheat > 3) return fail(c, FF_E_UNSUP, "IHEAT %s (0-3 supported)", std::to_string(h.iheat).c_str());
+        const int face = h.iface - 1, dir = face % 3;
+        // the ICON ranges: (J, K) on I-faces, (I, K) on J-faces, (I, J) on K-faces
+        const int rx = dir == 0 ? b.jmax : b.imax, ry = dir == 2 ? b.jmax : b.kmax;
+        if (h.kx1 < 1 || h.kx2 > rx || h.kx1 > h.kx2 || h.ky1 < 1 || h.ky2 > ry || h.ky1 > h.ky2)
+            return fail(c, FF_E_ARG, "wall patch %s range outside its face", std::to_string(l).c_str());
+        if (nd[dir] < 3) return fail(c, FF_E_UNSUP, "wall patch on a block under 3 cells thick%s");
+        double *ha[NWARR];
+        wall_arrays(h, ha);
+        for (int q = 0; q < NWARR; q++)
+            if (!ha[q]) return fail(c, FF_E_ARG, "wall patch %s array missing", std::to_string(l).c_str());
+        WallPatchDev d{};
+        d.face = face; d.dir = dir; d.idir = h.iface <= 3 ? 1 : -1;
+        d.kbot = h.iface <= 3 ? 1 : nd[dir];
+        d.iheat = h.iheat;
+        if (dir == 0) { d.sa = b.is; d.sb = b.il; d.sw = 1; d.a1 = h.kx1; d.a2 = h.kx2; d.b1 = h.ky1; d.b2 = h.ky2; }
+        else if (dir == 1) { d.sa = b.il; d.sb = 1; d.sw = b.is; d.a1 = h.ky1; d.a2 = h.ky2; d.b1 = h.kx1; d.b2 = h.kx2; }
+        else { d.sa = 1; d.sb = b.is; d.sw = b.il; d.a1 = h.kx1; d.a2 = h.kx2; d.b1 = h.ky1; d.b2 = h.ky2; }
+        d.na = d.a2 - d.a1 + 1 + 4;
+        d.n = (d.a2 - d.a1 + 1) * (d.b2 - d.b1 + 1);
+        for (int J = d.b1; J <= d.b2; J++)
+            for (int I = d.a1; I <= d.a2; I++) {
+                int i, j, k;
+                if (dir == 0) { i = d.kbot; j = I; k = J; }
+                else if (dir == 1) { i = J; j = d.kbot; k = I; }
+                else { i = I; j = J; k = d.kbot; }
+                const int idx = dir == 0 ? (k - 1) * b.jmax + (j - 1)
+                                         : (dir == 1 ? (k - 1) * b.imax + (i - 1) : (j - 1) * b.imax + (i - 1));
+                unsigned char &fl = flag[(size_t)wd.off[face] + idx];
+                if (fl) return fail(c, FF_E_ARG, "wall patches %s overlap", std::to_string(l).c_str());
+                fl = 1;
+            }
+        has_face[face] = true;
+        nparr += (size_t)d.na * (d.b2 - d.b1 + 1 + 4);
+        w.dev.push_back(d);
+        w.host.push_back(h);
+    }
+    if (corr)
+        for (int dir = 0; dir < 3; dir++)
+            if (has_face[dir] && has_face[dir + 3] && nd[dir] < 5)
+                return fail(c, FF_E_UNSUP, "near-wall corrections of both faces meet (under 5 cells)%s");
+    wd.on = 1;
+    wd.corr_on = corr ? 1 : 0;
+    w.env = WallEnv{wpar->t0, wpar->difpre, wpar->xmom, wpar->ymom, wpar->zmom,
+                    wpar->gx, wpar->gy, wpar->gz, wpar->ground, wpar->altitude};
+    const size_t nscr = (size_t)wd.nt * (7 + 8 + 2);
+    if (hipMalloc(&w.parr, sizeof(double) * NWARR * nparr) != hipSuccess ||
+        hipMalloc(&w.flag, (size_t)wd.nt) != hipSuccess ||
+        hipMalloc(&w.scr, sizeof(double) * nscr) != hipSuccess ||
+        hipMalloc(&w.sums, sizeof(double) * FF_NFORCE * npatch) != hipSuccess ||
+        hipMalloc(&w.dev_tab, sizeof(WallPatchDev) * npatch) != hipSuccess ||
+        hipHostMalloc(&w.sums_host, sizeof(double) * FF_NFORCE * npatch, hipHostMallocDefault) != hipSuccess) {
+        walls_free(w);
+        return fail(c, FF_E_NOMEM, "wall patch buffers%s");
+    }
+    if (hipEventCreateWithFlags(&w.fwd, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) {
+        walls_free(w);
+        return fail(c, FF_E_HIP, "wall events%s");
+    }
+    if (!c->wstream && hipStreamCreateWithFlags(&c->wstream, hipStreamNonBlocking) != hipSuccess) {
+        walls_free(w);
+        return fail(c, FF_E_HIP, "wall force stream%s");
+    }
+    wd.flag = w.flag;
+    wd.flux = w.scr;
+    wd.corr = w.scr + (size_t)wd.nt * 7;
+    wd.qterm = w.scr + (size_t)wd.nt * 15;
+    size_t o = 0;
+    for (auto &d : w.dev) {
+        const size_t sz = (size_t)d.na * ((d.b2 - d.b1 + 1) + 4);
+        for (int q = 0; q < NWARR; q++) d.arr[q] = w.parr + (size_t)q * nparr + o;
+        o += sz;
+    }
+    HIPCHK(c, hipMemcpyAsync(w.flag, flag.data(), flag.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w.dev_tab, w.dev.data(), sizeof(WallPatchDev) * npatch,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(w.scr, 0, sizeof(double) * nscr, c->stream));
+    HIPCHK(c, hipMemsetAsync(w.sums, 0, sizeof(double) * FF_NFORCE * npatch, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    r->walls = w;
+    return ff_wall_sync(c, level, block, 0);
+}
+
+int ff_wall_sync(ff_ctx *c, int level, int block, int direction)
+{
+    if (!c || (direction != 0 && direction != 1)) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    WallRec &w = r->walls;
+    if (w.pending) HIPCHK(c, hipStreamWaitEvent(c->stream, w.done, 0));
+    for (size_t l = 0; l < w.dev.size(); l++) {
+        const WallPatchDev &d = w.dev[l];
+        const size_t sz = (size_t)d.na * ((d.b2 - d.b1 + 1) + 4) * sizeof(double);
+        double *ha[NWARR];
+        wall_arrays(w.host[l], ha);
+        for (int q = 0; q < NWARR; q++) {
+            double *hp = ha[q];
+            if (direction == 0)
+                HIPCHK(c, hipMemcpyAsync(d.arr[q], hp, sz, hipMemcpyHostToDevice, c->stream));
+            else
+                HIPCHK(c, hipMemcpyAsync(hp, d.arr[q], sz, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FF_OK;
+}
+
+int ff_wall_forces(ff_ctx *c, int block, int patch, double *sums)
+{
+    if (!c || !sums) return FF_E_ARG;
+    BlockRec *r = find(c, 1, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    WallRec &w = r->walls;
+    if (patch < 0 || patch >= (int)w.dev.size()) return fail(c, FF_E_ARG, "no wall patch %s", std::to_string(patch).c_str());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (w.pending) HIPCHK(c, hipEventSynchronize(w.done));
+    HIPCHK(c, hipMemcpy(w.sums_host, w.sums, sizeof(double) * FF_NFORCE * w.dev.size(),
+                        hipMemcpyDeviceToHost));
+    for (int q = 0; q < FF_NFORCE; q++) sums[q] = w.sums_host[(size_t)patch * FF_NFORCE + q];
+    return FF_OK;
+}
 }  // extern "C"

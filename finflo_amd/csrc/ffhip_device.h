@@ -30,6 +30,7 @@ struct Prm {
     int kover, katol, istres;
     int fullns;     // full viscous fluxes (FULLNS) on this level: level 1 only
     int idiff;
+    int turcor;     // BOTSKE's near-wall corrections (the SST path)
 };
 
 #define FLD(b, n) ((b).f[n])

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .layout import FID, Block, BlockView, Params
+from .layout import FID, Block, BlockView, Params, WallParams, wall_views
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libffhip.so")
@@ -21,7 +21,7 @@ EXPORTS = [
     "ff_kernel_stats", "ff_kernel_stats_reset", "ff_block_register",
     "ff_block_register_device", "ff_sync", "ff_finish", "ff_state", "ff_turb", "ff_residual",
     "ff_implicit", "ff_update", "ff_save_old", "ff_cycle", "ff_restrict", "ff_forcing",
-    "ff_prolong", "ff_vcycle",
+    "ff_prolong", "ff_vcycle", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH = 1, 0, 1, 2
@@ -64,6 +64,9 @@ def lib():
         L.ff_forcing.argtypes = [vp, i, i, i]
         L.ff_prolong.argtypes = [vp, i, i]
         L.ff_vcycle.argtypes = [vp, i, i, i, vp]
+        L.ff_wall_register.argtypes = [vp, i, i, C.POINTER(WallParams), vp, i]
+        L.ff_wall_sync.argtypes = [vp, i, i, i]
+        L.ff_wall_forces.argtypes = [vp, i, i, vp]
         for fn in EXPORTS:
             if fn != "ff_last_error":
                 getattr(L, fn).restype = C.c_int
@@ -180,6 +183,26 @@ class Context:
         resi = np.zeros(7)
         self._chk(lib().ff_vcycle(self.h, block, nlev, ncycles, resi.ctypes.data), "ff_vcycle")
         return resi
+
+    # ---- solid-wall patches (BOTSKE) ---------------------------------------------------------
+    def wall_register(self, wparams, patches, level=1, block=1):
+        """Register the solid-wall patches (finflo_amd.layout.WallPatch, reference order) of a
+        block level; their host arrays are mirrored on the device and stay the caller's."""
+        views = wall_views(patches)
+        self._walls = getattr(self, "_walls", {})
+        self._walls[(level, block)] = (wparams, patches, views)
+        self._chk(lib().ff_wall_register(self.h, level, block, C.byref(wparams), views,
+                                         len(patches)), "ff_wall_register")
+
+    def wall_sync(self, direction, level=1, block=1):
+        """direction 0: patch arrays host -> device, 1: device -> host."""
+        self._chk(lib().ff_wall_sync(self.h, level, block, direction), "ff_wall_sync")
+
+    def wall_forces(self, patch, block=1):
+        """The 19 force sums (CX .. BUZ) of a patch from the last level-1 residual."""
+        out = np.zeros(19)
+        self._chk(lib().ff_wall_forces(self.h, block, patch, out.ctypes.data), "ff_wall_forces")
+        return out
 
     # ---- timing ------------------------------------------------------------------------
     def profile(self, on=True):

@@ -49,7 +49,8 @@ class Params(C.Structure):
         ("reserved", C.c_double * 3),
         ("iturb", C.c_int), ("iflux", C.c_int), ("idiff", C.c_int),
         ("inter", C.c_int * 3), ("kover", C.c_int), ("katol", C.c_int),
-        ("istres", C.c_int), ("fullns", C.c_int), ("reserved_i", C.c_int * 6)]
+        ("istres", C.c_int), ("fullns", C.c_int), ("turcor", C.c_int),
+        ("reserved_i", C.c_int * 5)]
 
     DOUBLES = ("gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
                "frspre", "frsden", "frstem", "frsvel", "frsvis",
@@ -89,6 +90,7 @@ def default_params(**kw):
     p.kover, p.katol = 0, 1
     p.istres = 0
     p.fullns = 0
+    p.turcor = 1           # TURCOR = YES (main.f:5257)
     for k, v in kw.items():
         if k == "inter":
             for i in range(3):
@@ -142,3 +144,73 @@ class Block:
         for i in range(NF):
             v.f[i] = addr + i * self.ntot * itemsize
         return v
+
+
+# ---- solid-wall patches (ff_wall_params, ff_wall_patch of include/ffhip.h) ---------------
+
+class WallParams(C.Structure):
+    """ff_wall_params: the constants BOTSKE reads besides ff_params."""
+    _fields_ = [(n, C.c_double) for n in (
+        "t0", "difpre", "xmom", "ymom", "zmom", "gx", "gy", "gz", "ground", "altitude")] + [
+        ("reserved", C.c_double * 6)]
+
+    def dvec(self):
+        return np.array([getattr(self, n) for n, _ in self._fields_[:10]] + [0.0] * 6)
+
+
+WALL_IN = ["uwall", "vwall", "wwall", "twall", "hflux", "rbk", "xcp", "ycp", "zcp"]
+WALL_OUT = ["cpwall", "qwall", "qwfric", "tauw1", "tauw2", "tauwx", "tauwy", "tauwz",
+            "surfx", "surfy", "surfz", "surfpx", "surfpy", "surfpz"]
+WALL_ARRAYS = WALL_IN + WALL_OUT
+FORCES = ["CX", "CY", "CZ", "CMX", "CMY", "CMZ", "TOM", "DX", "DY", "DZ", "QX", "QY", "QZ",
+          "VMX", "VMY", "VMZ", "BUX", "BUY", "BUZ"]
+
+
+class WallPatchView(C.Structure):
+    _fields_ = [("iface", C.c_int), ("kx1", C.c_int), ("kx2", C.c_int), ("ky1", C.c_int),
+                ("ky2", C.c_int), ("iheat", C.c_int), ("reserved", C.c_int * 2)] + [
+        (n, C.c_void_p) for n in WALL_ARRAYS]
+
+
+class WallPatch:
+    """A solid-wall patch: ICON(3..7,L), ICON(20,L) and the patch arrays, each
+    (NA+4)*(NB+4) doubles with two ghost rows (na along the face's fastest axis: J on an
+    I-face, K on a J-face, I on a K-face)."""
+
+    LN = 2
+
+    def __init__(self, iface, kx1, kx2, ky1, ky2, iheat=0, arrays=None):
+        self.iface, self.kx1, self.kx2, self.ky1, self.ky2 = iface, kx1, kx2, ky1, ky2
+        self.iheat = iheat
+        self.dir = (iface - 3 if iface > 3 else iface) - 1
+        if self.dir == 1:      # J-face: K fastest (FLXCYC swaps KX and KY)
+            self.na, self.nb = ky2 - ky1 + 1, kx2 - kx1 + 1
+        else:
+            self.na, self.nb = kx2 - kx1 + 1, ky2 - ky1 + 1
+        self.size = (self.na + 2 * self.LN) * (self.nb + 2 * self.LN)
+        self.a = arrays if arrays is not None else {
+            n: np.zeros(self.size) for n in WALL_ARRAYS}
+
+    def copy(self):
+        return WallPatch(self.iface, self.kx1, self.kx2, self.ky1, self.ky2, self.iheat,
+                         {n: v.copy() for n, v in self.a.items()})
+
+    def interior(self, name):
+        """(NB, NA) view of a patch array without its ghost rows."""
+        L = self.LN
+        return self.a[name].reshape(self.nb + 2 * L, self.na + 2 * L)[L:L + self.nb, L:L + self.na]
+
+    def view(self):
+        v = WallPatchView()
+        v.iface, v.kx1, v.kx2, v.ky1, v.ky2, v.iheat = (self.iface, self.kx1, self.kx2,
+                                                        self.ky1, self.ky2, self.iheat)
+        for n in WALL_ARRAYS:
+            setattr(v, n, self.a[n].ctypes.data)
+        return v
+
+
+def wall_views(patches):
+    arr = (WallPatchView * max(len(patches), 1))()
+    for i, wp in enumerate(patches):
+        arr[i] = wp.view()
+    return arr

@@ -97,8 +97,50 @@ typedef struct ff_params {
                         PEEKOO), 0 = eddy-viscosity SST */
     int fullns;      /* FULLNS: 1 = full viscous fluxes from cell gradients on level 1 (FLUXBL,
                         FLKEBL with TEMDER), 0 = the thin-layer FLUXDO/FLUDKE */
-    int reserved_i[6];
+    int turcor;      /* TURCOR (main.f:5257, default YES): the near-wall blending and flux
+                        corrections of BOTSKE with the SST; the laminar FLUCYC passes .FALSE. */
+    int reserved_i[5];
 } ff_params;
+
+/* ---- solid-wall patches (BOTSKE base3c.f:6147-7095) ---------------------------------- */
+/* Constants BOTSKE reads besides ff_params (NS3CO, main.f:5367-5369) */
+typedef struct ff_wall_params {
+    double t0;       /* T0: wall temperature of IHEAT = 0 and 2, and the TEMPB floor 0.01*T0 */
+    double difpre;   /* DIFPRE: the pressure subtracted in the force sums (YPG) */
+    double xmom, ymom, zmom;   /* XMOM, YMOM, ZMOM: moment reference point */
+    double gx, gy, gz;         /* GX, GY, GZ: gravity of the hydrostatic force part */
+    double ground, altitude;   /* GROUND, ALTITUDE: ALTITUDE >= GROUND switches gravity off */
+    double reserved[6];
+} ff_wall_params;
+
+/* One solid-wall patch of a block level (ICON(1,L) = 8..10 or 15), in the reference's terms:
+ * face ICON(3,L) (1, 2, 3 = the I-, J-, K-low face; 4, 5, 6 = the high faces) and the cell
+ * ranges ICON(4..7,L) on it (KX1..KX2, KY1..KY2: J and K on an I-face, I and K on a J-face,
+ * I and J on a K-face).  The patch arrays are the reference's IHF(ICON(2,L)) slabs with two
+ * ghost rows each side, (NA+4) x (NB+4) doubles, fastest along J on an I-face, K on a J-face
+ * (FLXCYC swaps KX and KY there, res3c.f:7090-7098) and I on a K-face. */
+typedef struct ff_wall_patch {
+    int iface;                  /* ICON(3,L) */
+    int kx1, kx2, ky1, ky2;     /* ICON(4..7,L) */
+    int iheat;                  /* ICON(20,L): 0 adiabatic, 1 TWALL given, 2 HFLUX given,
+                                   3 wall temperature extrapolated */
+    int reserved[2];
+    /* inputs */
+    double *uwall, *vwall, *wwall;   /* wall velocity */
+    double *twall;                   /* wall temperature (IHEAT = 1; written otherwise) */
+    double *hflux;                   /* wall heat flux (IHEAT = 2) */
+    double *rbk;                     /* roughness of the k+ wall condition of omega */
+    double *xcp, *ycp, *zcp;         /* face centres of the moment arms */
+    /* outputs */
+    double *cpwall, *qwall, *qwfric, *tauw1, *tauw2, *tauwx, *tauwy, *tauwz;
+    double *surfx, *surfy, *surfz, *surfpx, *surfpy, *surfpz;
+} ff_wall_patch;
+
+/* The force sums of one patch (CX(L) .. of FLXCYC, zeroed per level-1 RESIDI ns3c.f:2376-2396) */
+enum ff_force {
+    FF_CX, FF_CY, FF_CZ, FF_CMX, FF_CMY, FF_CMZ, FF_TOM, FF_DX, FF_DY, FF_DZ,
+    FF_QX, FF_QY, FF_QZ, FF_VMX, FF_VMY, FF_VMZ, FF_BUX, FF_BUY, FF_BUZ, FF_NFORCE
+};
 
 /* One (level M, block N) slab: pointers to the first element of each field, i.e. the
  * reference's X(IG(M,N)).  Unused slots may be NULL. */
@@ -209,6 +251,21 @@ int  ff_vcycle(ff_ctx *ctx, int block, int nlev, int ncycles, double *resi);
  * (the caller's boundary routines own them).  resi (may be NULL) receives the RESL2 sums
  * of the last cycle. */
 int  ff_cycle(ff_ctx *ctx, int block, int ncycles, double *resi);
+
+/* Register (or replace) the solid-wall patches of block N on level M, in the reference's
+ * patch order.  The patch arrays are host arrays mirrored on the device (ff_wall_sync moves
+ * them).  From then on ff_residual applies BOTSKE (base3c.f:6147-7095) at the patch faces
+ * in each direction's FLUCYC/FLXCYC pass (res3c.f:8092, 7157): the wall fluxes replace the
+ * face fluxes there, with TURCOR the SST's near-wall flux corrections and production, and
+ * on level 1 the force sums.  Heat options IHEAT 0-3; no porosity, wall functions or
+ * rotating frame.  npatch = 0 removes the patches. */
+int  ff_wall_register(ff_ctx *ctx, int level, int block, const ff_wall_params *wp,
+                      const ff_wall_patch *patches, int npatch);
+/* direction 0: host -> device, 1: device -> host, for every patch array of the block level */
+int  ff_wall_sync(ff_ctx *ctx, int level, int block, int direction);
+/* The FF_NFORCE force sums of patch `patch` (0-based, registration order) of block N from the
+ * last level-1 ff_residual. */
+int  ff_wall_forces(ff_ctx *ctx, int block, int patch, double *sums);
 
 #ifdef __cplusplus
 }

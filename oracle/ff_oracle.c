@@ -616,12 +616,364 @@ static void kw_diff_face_bl(const ff_block *b, long L, long S, const double *dd,
                      + dedyi * (ny - alfa * syi) + dedzi * (nz - alfa * szi));
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* Solid-wall patches: BOTSKE base3c.f:6147-7095                                          */
+/* ------------------------------------------------------------------------------------ */
+
+#define FFO_MAXLEV 8
+#define FFO_MAXPATCH 64
+typedef struct {
+    ff_wall_params wp;
+    const ff_wall_patch *p;
+    int n;
+    double forces[FFO_MAXPATCH][FF_NFORCE];
+} walls_t;
+static walls_t g_walls[FFO_MAXLEV + 1];
+
+/* the patches of block level `level` (the caller keeps the arrays alive) */
+void ffo_wall_set(int level, const ff_wall_params *wp, const ff_wall_patch *p, int n)
+{
+    if (level < 1 || level > FFO_MAXLEV || n < 0 || n > FFO_MAXPATCH) return;
+    if (n == 0 || !wp || !p) {
+        memset(&g_walls[level], 0, sizeof(g_walls[level]));
+        return;
+    }
+    g_walls[level].wp = *wp;
+    g_walls[level].p = p;
+    g_walls[level].n = n;
+    memset(g_walls[level].forces, 0, sizeof(g_walls[level].forces));
+}
+
+void ffo_wall_forces(int level, int patch, double *out)
+{
+    for (int q = 0; q < FF_NFORCE; q++) out[q] = g_walls[level].forces[patch][q];
+}
+
+/* BOTSKE base3c.f:6147-7095 for patch wp in direction dir (0 = I, 1 = J, 2 = K) as FLUCYC/
+ * FLXCYC call it after the direction's face fluxes (res3c.f:8092, 7157): IFLUX = 1
+ * (YP1 = 1, YP4 = 0), the viscous branch (IDM3 > KBOT), POROS = 0, no wall functions, no
+ * rotating frame (UROT = 0), IHEAT 0-3.  kw: the SST call (IDIS = 2, PSIGK1/PSIGE1 = SIGRK1/
+ * SIGOM1, CMU = 0.09 of TURBCO turb3.f:543-561, TURCOR = p->turcor); otherwise the laminar
+ * call (IDIS = 0, PSIG = CMU = 0, TURCOR = .FALSE., res3c.f:7715-7722, 8109).  The loop
+ * strides are FLXCYC's rotated ISTR, JSTR, KSTR (res3c.f:4018, 4274, 4499) and KX/KY are
+ * swapped on J-faces (res3c.f:7090-7098).  ff[q] are the direction's face fluxes F1R..F1EPS,
+ * sig the SST sigmas (NULL: laminar), f the patch's 19 force sums. */
+static void botske(const ff_params *p, ff_block *b, int level, int dir, const ff_wall_patch *wp,
+                   const ff_wall_params *ev, double **ff, const double *sig, double *f)
+{
+    dims_t d = mkdims(b);
+    const int LN = 2;
+    const int kw = sig != NULL;
+    const int idir = wp->iface <= 3 ? 1 : -1;                 /* ILMO */
+    const int ndir = dir == 0 ? d.imax : (dir == 1 ? d.jmax : d.kmax);
+    const int kbot = wp->iface <= 3 ? 1 : ndir;               /* KBOTOP */
+    long istr, jstr, kstr;
+    int a1, a2, b1, b2;
+    if (dir == 0) { istr = d.is; jstr = d.il; kstr = 1; a1 = wp->kx1; a2 = wp->kx2; b1 = wp->ky1; b2 = wp->ky2; }
+    else if (dir == 1) { istr = d.il; jstr = 1; kstr = d.is; a1 = wp->ky1; a2 = wp->ky2; b1 = wp->kx1; b2 = wp->kx2; }
+    else { istr = 1; jstr = d.is; kstr = d.il; a1 = wp->kx1; a2 = wp->kx2; b1 = wp->ky1; b2 = wp->ky2; }
+    const int na = a2 - a1 + 1 + 2 * LN;
+    const int turcor = kw ? p->turcor : 0;
+    const int intem = p->inter[dir];
+    double *F3R = ff[0], *F3RM = ff[1], *F3RN = ff[2], *F3RW = ff[3], *F3E = ff[4],
+           *F3RK = ff[5], *F3EPS = ff[6];
+    const double *P = F(b, FF_P), *U = F(b, FF_U), *V = F(b, FF_V), *W = F(b, FF_W),
+                 *VOL = F(b, FF_VOL), *A3 = F(b, FF_A1 + 4 * dir), *A3X = F(b, FF_A1X + 4 * dir),
+                 *A3Y = F(b, FF_A1Y + 4 * dir), *A3Z = F(b, FF_A1Z + 4 * dir),
+                 *VIS = F(b, FF_VIS), *EPS2 = F(b, FF_EPS2), *VIST = F(b, FF_VIST),
+                 *PDI = F(b, FF_PDIFF), *CP = F(b, FF_CP), *RO = F(b, FF_RO),
+                 *XC = F(b, FF_XC), *YC = F(b, FF_YC), *ZC = F(b, FF_ZC),
+                 *TEMP = F(b, FF_TEMP), *CH = F(b, FF_CH);
+    /* the laminar FLUCYC passes its RK, REPS and STRAIN too; the terms they feed are then
+       multiplied by zero (PSIG = 0, ROMFAC = XAUX1 = 0) */
+    const double *STRAIN = F(b, FF_STRAIN), *RK = F(b, FF_RK), *REPS = F(b, FF_REPS);
+    double *PTUR = F(b, FF_PTUR);
+    (void)CP;
+    const double eps3 = 1.E-20;
+    const double poros = 0.0, ompor = 1. - poros;
+    const double xkappa = 0.4;
+    double rc1 = 1.500, rc2 = rc1 - 1.;
+    if (intem == 6) { rc1 = 1.; rc2 = rc1 - 1.; }
+    const int idis = kw ? 2 : 0;
+    double psigk, psige, romfac, repfac;
+    if (idis == 2) {
+        psigk = sig[0]; psige = sig[1]; romfac = 1.0; repfac = 1.0 - romfac;
+    } else {
+        psigk = 0.; psige = 0.; romfac = 0.0; repfac = 1.0 - romfac;
+    }
+    const double cmu = kw ? 0.09 : 0.;
+    const double prs = p->pr / p->prt;
+    const long lstrid = kstr * idir;
+    const long nfl = idir == 1 ? 0 : -lstrid;
+    const double ymr = .166666667, ypg = ev->difpre, yp1 = 1., yp4 = 0.;
+    const double rsum = level == 1 ? 1. : 0.;
+    const double eps = 1.E-3;
+    double tsurf = ev->t0;
+    const double t0 = ev->t0;
+    int ihfl = 0, iht = 1;
+    if (wp->iheat == 2 || wp->iheat >= 12) ihfl = 1;
+    if (wp->iheat == 0) iht = 0;
+    const double vislim = 0.01 * p->frsvis, rolim = .001 * p->frsden;
+    double xaux1 = 0.0, xaux2 = 0.0, xaux3 = 0.0;
+    if (idis == 2) xaux1 = 1.;
+    double *cx = &f[FF_CX], *cy = &f[FF_CY], *cz = &f[FF_CZ], *cmx = &f[FF_CMX],
+           *cmy = &f[FF_CMY], *cmz = &f[FF_CMZ], *tom = &f[FF_TOM], *dxs = &f[FF_DX],
+           *dys = &f[FF_DY], *dzs = &f[FF_DZ], *qx = &f[FF_QX], *qy = &f[FF_QY],
+           *qz = &f[FF_QZ], *vmx = &f[FF_VMX], *vmy = &f[FF_VMY], *vmz = &f[FF_VMZ],
+           *bux = &f[FF_BUX], *buy = &f[FF_BUY], *buz = &f[FF_BUZ];
+    /* viscous surface fluxes (loop 1000) */
+    for (int J = b1; J <= b2; J++)
+        for (int I = a1; I <= a2; I++) {
+            long II = (1 + I) * istr + (1 + J) * jstr + (1 + kbot) * kstr;
+            long II2 = II + lstrid, IM1 = II - lstrid, IM2 = IM1 - lstrid;
+            long IF1 = II + nfl;
+            long NP = (long)(LN + J - b1) * na + (I - a1 + LN);
+            double stra2 = STRAIN[II2], strai = STRAIN[II];
+            double dist = (A3[IF1] + A3[IF1 + lstrid]) / VOL[II];
+            double distg = (A3[IF1] + A3[IF1 - lstrid]) / VOL[IM1];
+            double surg = -dist * ymr;
+            double eps2b = 1.;
+            double vistb = 0.;
+            double visb = dmax(vislim, rc1 * VIS[II] - rc2 * VIS[II2]);
+            double chsb = dmax(eps, rc1 * CH[II] - rc2 * CH[II2]);
+            double straib = dmax(eps, rc1 * strai - rc2 * stra2);
+            double pb1 = dmax(0.01 * p->frspre, rc1 * P[II] - rc2 * P[II2]);
+            double rob = dmax(rolim, rc1 * RO[II] - rc2 * RO[II2]);
+            double epsur2 = dmax(eps, rc1 * REPS[II] - rc2 * REPS[II2]);
+            double tempb = dmax(0.01 * t0, rc1 * TEMP[II] - rc2 * TEMP[II2]);
+            double surf = surg * eps2b * visb;
+            double surfe = surg * (prs * vistb / visb + 1.) * chsb;
+            double rapub = dmax(1.0E-20, eps2b - 1.0);
+            double surfrk = surg * (1.0 + psigk * rapub) * visb;
+            double surfep = surg * (1.0 + psige * rapub) * visb;
+            double surfe1 = .5 * dist * (prs * vistb / visb + 1.) * chsb;
+            double ypro1 = 1. / RO[II];
+            double ypro2 = 1. / RO[II2];
+            double chs = dmax(eps, rc1 * CH[IM1] - rc2 * CH[IM2]);
+            double surge = 0.5 * distg * chs;
+            double rksurf = 0.0;
+            double sqrk = (sqrt(RK[II] * ypro1) - sqrt(rksurf / rob)) * dist;
+            double epsur3 = 2. * visb * (sqrk * sqrk);
+            double epsurf = xaux2 * epsur2 / rob + xaux3 * epsur3 / rob;
+            double rnub = visb / rob;
+            double utau2 = rnub * straib;
+            double plusfc = sqrt(utau2) / rnub;
+            double yplus1 = 2.0 * plusfc / dist;
+            double skpmin = dmin(2.4 * pow(yplus1, 0.85), 8.0);
+            skpmin = dmax(skpmin, eps);
+            double skplus = dmax(wp->rbk[NP] * plusfc, skpmin);
+            double pskp = 1.0 / skplus;
+            double sr1 = 2500.0 * (pskp * pskp);
+            double sr2 = 100.0 * pskp;
+            double srcoef = dmax(sr1, sr2);
+            double omsurf = straib * srcoef;
+            rksurf = 0.0;
+            epsurf = romfac * omsurf + repfac * epsurf;
+            double usurf = wp->uwall[NP], vsurf = wp->vwall[NP], wsurf = wp->wwall[NP];
+            double temp1 = TEMP[II], temp2 = TEMP[II + lstrid];
+            if (wp->iheat == 1) {
+                tsurf = wp->twall[NP];
+            } else if (wp->iheat != 3 && wp->iheat != 4) {
+                tsurf = t0;
+                wp->twall[NP] = tempb;
+            } else if (wp->iheat == 3) {
+                tsurf = tempb;
+                wp->twall[NP] = tempb;
+            }
+            (void)surfe1; (void)surge;
+            const double ihtd = (double)(iht * idir), idd = (double)idir;
+            double cvi2 = ihtd * (9.000 * temp1 - 1.000 * temp2 - 8. * tsurf);
+            double uvi2 = idd * (9.000 * U[II] - 1.000 * U[II2] - 8. * usurf);
+            double vvi2 = idd * (9.000 * V[II] - 1.000 * V[II2] - 8. * vsurf);
+            double wvi2 = idd * (9.000 * W[II] - 1.000 * W[II2] - 8. * wsurf);
+            double cvi1 = ihtd * (temp1 - tsurf);
+            double uvi1 = idd * (U[II] - usurf) * 6.;
+            double vvi1 = idd * (V[II] - vsurf) * 6.;
+            double wvi1 = idd * (W[II] - wsurf) * 6.;
+            double pb4 = dmax(-.99 * p->frspre, rc1 * PDI[II] - rc2 * PDI[II2]);
+            double pb = yp1 * pb1 + yp4 * pb4;
+            wp->cpwall[NP] = pb4;
+            double alpha = 1.;
+            double yp = 1. / dist;
+            if (turcor) {
+                double du = U[II] - usurf, dv = V[II] - vsurf, dw = W[II] - wsurf;
+                double uvel = sqrt(du * du + dv * dv + dw * dw);
+                double para = yp * uvel * RO[II] / VIS[II];
+                alpha = dmin(para / 140., 1.);
+            }
+            double cvii = (1. - alpha) * cvi1 + alpha * cvi2;
+            double uvii = (1. - alpha) * uvi1 + alpha * uvi2;
+            double vvii = (1. - alpha) * vvi1 + alpha * vvi2;
+            double wvii = (1. - alpha) * wvi1 + alpha * wvi2;
+            double duc = .33333 * (A3Y[IF1] * vvii + A3Z[IF1] * wvii + A3X[IF1] * uvii);
+            double f2v = surf * (uvii + A3X[IF1] * duc);
+            double f3v = surf * (vvii + A3Y[IF1] * duc);
+            double f4v = surf * (wvii + A3Z[IF1] * duc);
+            double f5vf = usurf * f2v + vsurf * f3v + wsurf * f4v;
+            double f5vq = (double)(1 - ihfl) * surfe * cvii + (double)ihfl * wp->hflux[NP];
+            double f5v = f5vf + f5vq;
+            wp->qwfric[NP] = f5vf;
+            wp->qwall[NP] = f5vq;
+            *qy = *qy + A3[IF1] * rsum * f5v;
+            *qz = *qz + A3[IF1] * rsum * f5vq;
+            double evii = idd * (9.0 * REPS[II] * ypro1 - 1.0 * REPS[II2] * ypro2 - 8. * epsurf);
+            double rkii = idd * (9.000 * RK[II] * ypro1 - 1.000 * RK[II2] * ypro2 - 8. * rksurf);
+            rkii = idd * dmax(idd * rkii, 0.0);
+            double f6v, f7v;
+            double utau = sqrt(sqrt(f2v * f2v + f3v * f3v + f4v * f4v) / rob);
+            if (!turcor) {
+                f6v = surfrk * rkii * dmax(romfac, xaux1);
+                f7v = surfep * evii * dmax(romfac, xaux1);
+            } else {
+                double t1 = rob * (utau * utau * utau) / xkappa / (yp * yp);
+                double t2 = 4. * visb * (utau * utau) / sqrt(cmu) / (yp * yp * yp);
+                double evik = idd * 6. * yp * dmin(0., -t1 + t2) * (1. + VIST[II] / VIS[II] * psige);
+                f6v = surf * rkii * .5 * (1. + EPS2[II]);
+                f7v = surf * (romfac * evii + repfac * evik);
+                double rou2 = RO[II] * (utau * utau);
+                double vv = VIS[II] + VIST[II];
+                PTUR[II] = VIST[II] * (rou2 * rou2) / (vv * vv);
+            }
+            double anxi = XC[II + istr] - XC[II - istr];
+            double anyi = YC[II + istr] - YC[II - istr];
+            double anzi = ZC[II + istr] - ZC[II - istr];
+            double xnori = 1. / sqrt(anxi * anxi + anyi * anyi + anzi * anzi + eps3);
+            double anxj = XC[II + jstr] - XC[II - jstr];
+            double anyj = YC[II + jstr] - YC[II - jstr];
+            double anzj = ZC[II + jstr] - ZC[II - jstr];
+            double xnorj = 1. / sqrt(anxj * anxj + anyj * anyj + anzj * anzj + eps3);
+            wp->tauw1[NP] = -(anxi * f2v + anyi * f3v + anzi * f4v) * xnori;
+            wp->tauw2[NP] = -(anxj * f2v + anyj * f3v + anzj * f4v) * xnorj;
+            wp->tauwx[NP] = -f2v;
+            wp->tauwy[NP] = -f3v;
+            wp->tauwz[NP] = -f4v;
+            F3R[IF1] = 0.;
+            F3RM[IF1] = A3[IF1] * (A3X[IF1] * pb + f2v);
+            F3RN[IF1] = A3[IF1] * (A3Y[IF1] * pb + f3v);
+            F3RW[IF1] = A3[IF1] * (A3Z[IF1] * pb + f4v);
+            F3E[IF1] = A3[IF1] * (0. * pb1 + f5v + f6v);          /* UROT = 0 */
+            wp->qwfric[NP] = wp->qwfric[NP] + 0. * pb;
+            *qy = *qy + rsum * 0. * pb;
+            F3RK[IF1] = A3[IF1] * f6v;
+            F3EPS[IF1] = A3[IF1] * f7v;
+            wp->qwfric[NP] = idd * wp->qwfric[NP];
+            wp->qwall[NP] = idd * wp->qwall[NP];
+        }
+    /* flux corrections next to the surface (TURCOR, ITURB < 9) */
+    if (turcor && p->iturb < 9) {
+        for (int J = b1; J <= b2; J++)
+            for (int I = a1; I <= a2; I++) {
+                long II = (1 + I) * istr + (1 + J) * jstr + (1 + kbot) * kstr;
+                long II2 = II + lstrid, II3 = II + 2 * lstrid;
+                long IF1 = II + nfl, IF2 = IF1 + lstrid, IF3 = IF1 + 2 * lstrid;
+                double ypd2 = A3[IF2] / (VOL[II2] + VOL[II]);
+                double surg = -A3[IF2] * ypd2;
+                double surf = surg * (-.33333 * VIST[II2] + VIST[II]);
+                double surfe = surg * ((-.33333 * prs * VIST[II2] / VIS[II2]) * CH[II2]
+                                       + (prs * VIST[II] / VIS[II]) * CH[II]);
+                double duc = .333333 * (A3X[IF2] * (U[II2] - U[II]) + A3Y[IF2] * (V[II2] - V[II])
+                                        + A3Z[IF2] * (W[II2] - W[II]));
+                double f2 = surf * (U[II2] - U[II] + A3X[IF2] * duc);
+                double f3 = surf * (V[II2] - V[II] + A3Y[IF2] * duc);
+                double f4 = surf * (W[II2] - W[II] + A3Z[IF2] * duc);
+                double f5 = .5 * ((U[II2] + U[II]) * f2 + (V[II2] + V[II]) * f3 + (W[II2] + W[II]) * f4)
+                          + surfe * (TEMP[II2] - TEMP[II]);
+                F3RM[IF2] = F3RM[IF2] + f2;
+                F3RN[IF2] = F3RN[IF2] + f3;
+                F3RW[IF2] = F3RW[IF2] + f4;
+                F3E[IF2] = F3E[IF2] + f5;
+                ypd2 = A3[IF3] / (VOL[II3] + VOL[II2]);
+                surg = -A3[IF3] * ypd2;
+                surf = surg * (-.2 * VIST[II3] + .3333 * VIST[II2]);
+                surfe = surg * ((-.2 * prs * VIST[II3] / VIS[II3]) * CH[II3]
+                                + (+.3333 * prs * VIST[II2] / VIS[II2]) * CH[II2]);
+                duc = .2 * (A3X[IF3] * (U[II3] - U[II2]) + A3Y[IF3] * (V[II3] - V[II2])
+                            + A3Z[IF3] * (W[II3] - W[II2]));
+                f2 = surf * (U[II3] - U[II2] + A3X[IF3] * duc);
+                f3 = surf * (V[II3] - V[II2] + A3Y[IF3] * duc);
+                f4 = surf * (W[II3] - W[II2] + A3Z[IF3] * duc);
+                f5 = .5 * ((U[II3] + U[II2]) * f2 + (V[II3] + V[II2]) * f3 + (W[II3] + W[II2]) * f4)
+                   + surfe * (TEMP[II3] - TEMP[II2]);
+                F3RM[IF3] = F3RM[IF3] + f2;
+                F3RN[IF3] = F3RN[IF3] + f3;
+                F3RW[IF3] = F3RW[IF3] + f4;
+                F3E[IF3] = F3E[IF3] + f5;
+            }
+    }
+    /* forces acting on the surface (loop 2000, level 1) */
+    if (level == 1) {
+        for (int J = b1; J <= b2; J++)
+            for (int I = a1; I <= a2; I++) {
+                long II = (1 + I) * istr + (1 + J) * jstr + (1 + kbot) * kstr;
+                long II2 = II + lstrid;
+                long IF1 = II + nfl;
+                long NP = (long)(LN + J - b1) * na + (I - a1 + LN);
+                const double idd = (double)idir;
+                double xd = wp->xcp[NP] - ev->xmom;
+                double yd = wp->ycp[NP] - ev->ymom;
+                double zd = wp->zcp[NP] - ev->zmom;
+                double deltap = p->frsden * (ev->gx * (wp->xcp[NP] - ev->ground)
+                                             + ev->gy * (wp->ycp[NP] - ev->ground)
+                                             + ev->gz * (wp->zcp[NP] - ev->ground));
+                if (ev->altitude >= ev->ground) deltap = 0.0;
+                double bx = -idd * (F3RM[IF1] - A3[IF1] * ompor * A3X[IF1] * ypg);
+                double by = -idd * (F3RN[IF1] - A3[IF1] * ompor * A3Y[IF1] * ypg);
+                double bz = -idd * (F3RW[IF1] - A3[IF1] * ompor * A3Z[IF1] * ypg);
+                double bxdp = -idd * (-A3[IF1] * ompor * A3X[IF1] * (-deltap));
+                double bydp = -idd * (-A3[IF1] * ompor * A3Y[IF1] * (-deltap));
+                double bzdp = -idd * (-A3[IF1] * ompor * A3Z[IF1] * (-deltap));
+                bx = bx + bxdp;
+                by = by + bydp;
+                bz = bz + bzdp;
+                wp->surfx[NP] = bx / (A3[IF1] + 1.E-20);
+                wp->surfy[NP] = by / (A3[IF1] + 1.E-20);
+                wp->surfz[NP] = bz / (A3[IF1] + 1.E-20);
+                *cmx = *cmx + bz * yd - by * zd;
+                *cmy = *cmy + bx * zd - bz * xd;
+                *cmz = *cmz + by * xd - bx * yd;
+                *cx = *cx + bx;
+                *cy = *cy + by;
+                *cz = *cz + bz;
+                *tom = *tom - idd * (F3E[IF1] - F3RK[IF1]);
+                double pb1 = dmax(0.01 * p->frspre, rc1 * P[II] - rc2 * P[II2]);
+                double pb4 = rc1 * PDI[II] - rc2 * PDI[II2];
+                double pb = yp1 * pb1 + yp4 * pb4;
+                double dxp = -idd * (A3[IF1] * ompor * A3X[IF1] * (pb - ypg));
+                double dyp = -idd * (A3[IF1] * ompor * A3Y[IF1] * (pb - ypg));
+                double dzp = -idd * (A3[IF1] * ompor * A3Z[IF1] * (pb - ypg));
+                *dxs = *dxs - idd * (A3[IF1] * ompor * A3X[IF1] * (pb - ypg));
+                *dys = *dys - idd * (A3[IF1] * ompor * A3Y[IF1] * (pb - ypg));
+                *dzs = *dzs - idd * (A3[IF1] * ompor * A3Z[IF1] * (pb - ypg));
+                wp->surfpx[NP] = (*dxs + dxp) / (A3[IF1] + 1.E-20);
+                wp->surfpy[NP] = (*dys + dyp) / (A3[IF1] + 1.E-20);
+                wp->surfpz[NP] = (*dzs + dzp) / (A3[IF1] + 1.E-20);
+                bx = bx + idd * (A3[IF1] * ompor * A3X[IF1] * (pb - ypg));
+                by = by + idd * (A3[IF1] * ompor * A3Y[IF1] * (pb - ypg));
+                bz = bz + idd * (A3[IF1] * ompor * A3Z[IF1] * (pb - ypg));
+                *vmx = *vmx + bz * yd - by * zd;
+                *vmy = *vmy + bx * zd - bz * xd;
+                *vmz = *vmz + by * xd - bx * yd;
+                *bux = *bux + bxdp;
+                *buy = *buy + bydp;
+                *buz = *buz + bzdp;
+                *qx = *qx - idd * F3E[IF1];
+            }
+        *tom = *tom + (double)idir * *qz;
+        *qz = (double)idir * *qz;
+        *qy = (double)idir * *qy;
+        *dxs = *dxs + *bux;
+        *dys = *dys + *buy;
+        *dzs = *dzs + *buz;
+    }
+}
+
 /* FLUCYC (laminar, res3c.f:7572) or FLXCYC (two-equation, res3c.f:6424) for one direction
  * (dir 0=i, 1=j, 2=k) followed by SWEEP/SWEEPK.  Face fluxes are formed for the faces
  * adjacent to interior cells only; the reference also forms fluxes on ghost rows and
  * columns (FLUXKE loop 1000), which no interior residual reads.  `kw` (SST) holds the
  * FLUDKE sigmas PSIGK1, PSIGE1, PSIGK, PSIGE or is NULL. */
-static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last, const double *kw)
+static void direction_sweep(const ff_params *p, ff_block *b, int level, int dir, int last,
+                            const double *kw)
 {
     dims_t d = mkdims(b);
     long S = dir == 0 ? 1 : (dir == 1 ? d.is : d.il);
@@ -702,6 +1054,15 @@ static void direction_sweep(const ff_params *p, ff_block *b, int dir, int last, 
                     ff[6][L] = r.f[6] + ff[6][L];
                 }
             }
+    /* the solid-wall patches of this direction, in patch order (res3c.f:7075-7157, 8010-8092) */
+    if (level >= 1 && level <= FFO_MAXLEV) {
+        walls_t *wl = &g_walls[level];
+        for (int l = 0; l < wl->n; l++) {
+            const ff_wall_patch *wp = &wl->p[l];
+            int pdir = (wp->iface > 3 ? wp->iface - 3 : wp->iface) - 1;
+            if (pdir == dir) botske(p, b, level, dir, wp, &wl->wp, ff, kw, wl->forces[l]);
+        }
+    }
     /* SWEEP res3c.f:992-1036 / SWEEPK res3c.f:1368-1426 */
     double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
                     F(b, FF_DRK), F(b, FF_DEPS)};
@@ -1344,9 +1705,11 @@ void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *r
         if (!kwm) derfun(b, 1);
         fullns_gradients(b, kwm);
     }
-    direction_sweep(p, b, 2, 0, kwm ? sig : NULL);   /* res3c.f:270 / 4018, SWEEP(K) */
-    direction_sweep(p, b, 0, 0, kwm ? sig : NULL);   /* res3c.f:402 / 4274 */
-    direction_sweep(p, b, 1, 1, kwm ? sig : NULL);   /* res3c.f:515 / 4499 */
+    if (level == 1 && level <= FFO_MAXLEV)     /* RESIDI zeroes the force sums (ns3c.f:2376) */
+        memset(g_walls[level].forces, 0, sizeof(g_walls[level].forces));
+    direction_sweep(p, b, level, 2, 0, kwm ? sig : NULL);   /* res3c.f:270 / 4018, SWEEP(K) */
+    direction_sweep(p, b, level, 0, 0, kwm ? sig : NULL);   /* res3c.f:402 / 4274 */
+    direction_sweep(p, b, level, 1, 1, kwm ? sig : NULL);   /* res3c.f:515 / 4499 */
     if (kwm) {
         sourko(p, &s, b);
         /* ADDSKE res3c.f:1494-1502 */

@@ -10,6 +10,8 @@ import os
 
 import numpy as np
 
+from finflo_amd.layout import wall_views
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libff_oracle.so")
 REFLIB = os.path.join(HERE, "_ref", "libfinflo_ref.so")
@@ -49,6 +51,10 @@ def oracle():
         _ora.ffo_vcycle.restype = None
         _ora.ffo_implicit.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         _ora.ffo_implicit.restype = None
+        _ora.ffo_wall_set.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        _ora.ffo_wall_set.restype = None
+        _ora.ffo_wall_forces.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        _ora.ffo_wall_forces.restype = None
     return _ora
 
 
@@ -67,6 +73,10 @@ def ref():
             getattr(_ref, fn).restype = None
         _ref.ref_residual.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _ref.ref_residual.restype = None
+        _ref.ref_wall_set.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _ref.ref_wall_set.restype = None
+        _ref.ref_wall_forces.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        _ref.ref_wall_forces.restype = None
     return _ref
 
 
@@ -76,6 +86,8 @@ class OracleBlock:
     def __init__(self, prm, blk, level=1):
         self.prm, self.blk, self.level = prm, blk, level
         self._view = blk.view_struct()
+        # the oracle keeps wall patches per level: a new block starts without any
+        oracle().ffo_wall_set(level, None, None, 0)
 
     def _args(self):
         return C.byref(self.prm), C.byref(self._view)
@@ -113,6 +125,18 @@ class OracleBlock:
         self.update()
         return r
 
+    def set_walls(self, wparams, patches):
+        """Solid-wall patches (finflo_amd.layout.WallPatch) of this level; the arrays stay
+        owned by the caller and are written in place."""
+        self._wp = wparams
+        self._wv = wall_views(patches)
+        oracle().ffo_wall_set(self.level, C.byref(self._wp), self._wv, len(patches))
+
+    def forces(self, patch):
+        out = np.zeros(19)
+        oracle().ffo_wall_forces(self.level, patch, out.ctypes.data)
+        return out
+
 
 class OracleHierarchy:
     """The oracle on a block with its multigrid levels (finest first)."""
@@ -149,8 +173,12 @@ class RefBlock:
         inter = list(prm.inter) if level == 1 else [55, 55, 55]
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
                             inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1,
-                            prm.istres, prm.fullns],
+                            prm.istres, prm.fullns, prm.turcor],
                            dtype=np.int32)
+        # the reference driver keeps wall patches per level: a new block starts without any
+        if have_ref():
+            z = np.zeros(32)
+            ref().ref_wall_set(level, 0, z.ctypes.data, z.ctypes.data, z.ctypes.data)
 
     def _a(self):
         return (self.blk.F.ctypes.data, self.blk.ntot, self.pv.ctypes.data,
@@ -188,3 +216,21 @@ class RefBlock:
         self.implicit()
         self.update()
         return r
+
+    def set_walls(self, wparams, patches):
+        """The same patches for the reference's BOTSKE (ref_wall_set)."""
+        from finflo_amd.layout import WALL_ARRAYS
+        n = len(patches)
+        self._wints = np.zeros((max(n, 1), 8), dtype=np.int32)
+        self._wptrs = np.zeros((max(n, 1), len(WALL_ARRAYS)), dtype=np.uint64)
+        for l, wp in enumerate(patches):
+            self._wints[l, :6] = [wp.iface, wp.kx1, wp.kx2, wp.ky1, wp.ky2, wp.iheat]
+            self._wptrs[l] = [wp.a[a].ctypes.data for a in WALL_ARRAYS]
+        self._wenv = wparams.dvec()
+        ref().ref_wall_set(self.level, n, self._wints.ctypes.data, self._wenv.ctypes.data,
+                           self._wptrs.ctypes.data)
+
+    def forces(self, patch):
+        out = np.zeros(19)
+        ref().ref_wall_forces(self.level, patch, out.ctypes.data)
+        return out

@@ -49,8 +49,149 @@ module ffref_ids
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
   integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12, QM2=13
-  integer, parameter :: QISTRES=14, QFULLNS=15
+  integer, parameter :: QISTRES=14, QFULLNS=15, QTURCOR=16
 end module ffref_ids
+
+! Solid-wall patches per level, set by ref_wall_set: ICON(3..7,L) and ICON(20,L), the
+! constants of ff_wall_params and the patch arrays (in the order of ff_wall_patch), plus the
+! force sums of the last level-1 residual
+module ffref_walls
+  use iso_c_binding
+  implicit none
+  integer, parameter :: MAXP = 64, NWA = 23, MAXL = 8
+  integer :: nwall(MAXL) = 0
+  integer :: wint(8, MAXP, MAXL) = 0
+  type(c_ptr) :: wptr(NWA, MAXP, MAXL)
+  real(c_double) :: wenv(16, MAXL) = 0.
+  real(c_double) :: wforce(19, MAXP, MAXL) = 0.
+end module ffref_walls
+
+subroutine ref_wall_set(level, n, ints, env, ptrs) bind(C, name="ref_wall_set")
+  use iso_c_binding
+  use ffref_walls
+  implicit none
+  integer(c_int), value :: level, n
+  integer(c_int) :: ints(8, *)
+  real(c_double) :: env(16)
+  type(c_ptr) :: ptrs(NWA, *)
+  integer :: l
+  if (level < 1 .or. level > MAXL .or. n < 0 .or. n > MAXP) return
+  nwall(level) = n
+  wenv(:, level) = env
+  do l = 1, n
+     wint(:, l, level) = ints(:, l)
+     wptr(:, l, level) = ptrs(:, l)
+  end do
+  wforce(:, :, level) = 0.
+end subroutine ref_wall_set
+
+subroutine ref_wall_forces(level, patch, out) bind(C, name="ref_wall_forces")
+  use iso_c_binding
+  use ffref_walls
+  implicit none
+  integer(c_int), value :: level, patch
+  real(c_double) :: out(19)
+  out = wforce(:, patch + 1, level)
+end subroutine ref_wall_forces
+
+! BOTSKE base3c.f:6147 for every solid-wall patch of the level in direction IDIR, in patch
+! order, as FLUCYC (res3c.f:8092, TURCOR = .FALSE., IDIS = 0, PSIG = CMU = 0) or FLXCYC
+! (res3c.f:7157, IDIS = 2, SST sigmas, CMU = 0.09) call it after the direction's fluxes:
+! FLXCYC's rotated strides (res3c.f:4018, 4274, 4499), KX/KY swapped on J-faces
+! (res3c.f:7090-7098), no wall functions, UROT = 0
+subroutine ref_walls(f, ntot, prm, ipr, idir, fl, zero, sig, kw)
+  use iso_c_binding
+  use ffref_ids
+  use ffref_walls
+  use TYPE_ARRAYS
+  use NS3CO, only : WALLFUNL, GX, GY, GZ, GROUND, ALTITUDE
+  implicit none
+  integer :: ntot, idir
+  real(c_double) :: f(ntot, *), prm(*), fl(ntot, 7), zero(ntot), sig(4)
+  integer(c_int) :: ipr(*)
+  logical :: kw
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(INTERMITTENCY) :: trm(1)
+  type(PRE_COR) :: prc(1)
+  type pa
+     real(c_double), pointer :: a(:)
+  end type pa
+  type(pa) :: w(NWA)
+  real(c_double), allocatable :: dum(:)
+  real(c_double) :: rcon(1), xco(1), f1h(1), wave(1), utaum(1), fvw(1)
+  real(c_double) :: psk1, pse1, psk2, pse2, cmu
+  integer :: iwaveb(1)
+  integer :: lev, l, q, iface, pdir, kx1, kx2, ky1, ky2, ilmo, kbotop, ndi, npa
+  integer :: imax, jmax, kmax, is, js, istr, jstr, kstr, ia, iax, iay, iaz, inter, intet, idis
+  logical :: turcor
+  lev = ipr(QLEVEL)
+  if (lev < 1 .or. lev > MAXL) return
+  if (nwall(lev) == 0) return
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  is = imax + 4; js = jmax + 4
+  WALLFUNL = .false.
+  GX = wenv(6, lev); GY = wenv(7, lev); GZ = wenv(8, lev)
+  GROUND = wenv(9, lev); ALTITUDE = wenv(10, lev)
+  select case (idir)
+  case (1)
+     istr = is; jstr = is*js; kstr = 1; ia = IA1; inter = ipr(QINTI); ndi = imax + 1
+  case (2)
+     istr = is*js; jstr = 1; kstr = is; ia = IA2; inter = ipr(QINTJ); ndi = jmax + 1
+  case default
+     istr = 1; jstr = is; kstr = is*js; ia = IA3; inter = ipr(QINTK); ndi = kmax + 1
+  end select
+  iax = ia + 1; iay = ia + 2; iaz = ia + 3
+  intet = inter
+  if (inter > -5 .and. inter < 5) intet = +1
+  if (kw) then
+     idis = 2; psk1 = sig(1); pse1 = sig(2); psk2 = sig(3); pse2 = sig(4); cmu = 0.09_c_double
+     turcor = ipr(QTURCOR) /= 0
+  else
+     idis = 0; psk1 = 0.; pse1 = 0.; psk2 = 0.; pse2 = 0.; cmu = 0.; turcor = .false.
+  end if
+  do l = 1, nwall(lev)
+     iface = wint(1, l, lev)
+     pdir = iface
+     if (iface > 3) pdir = iface - 3
+     if (pdir /= idir) cycle
+     kx1 = wint(2, l, lev); kx2 = wint(3, l, lev); ky1 = wint(4, l, lev); ky2 = wint(5, l, lev)
+     if (idir == 2) then
+        kx1 = wint(4, l, lev); kx2 = wint(5, l, lev); ky1 = wint(2, l, lev); ky2 = wint(3, l, lev)
+     end if
+     if (iface <= 3) then
+        ilmo = 1; kbotop = 1
+     else
+        ilmo = -1; kbotop = ndi - 1
+     end if
+     npa = (kx2 - kx1 + 5)*(ky2 - ky1 + 5)
+     do q = 1, NWA
+        call c_f_pointer(wptr(q, l, lev), w(q)%a, [npa])
+     end do
+     allocate(dum(npa))
+     dum = 0.
+     call BOTSKE(fl(1,1), fl(1,2), fl(1,3), fl(1,4), fl(1,5), f(1,IP), f(1,IU), f(1,IV), &
+          f(1,IW), f(1,IC), f(1,IE), f(1,IVOL), f(1,ia), f(1,iax), f(1,iay), f(1,iaz), &
+          f(1,IOHMI), f(1,IVIS), f(1,IEPS2), f(1,IVIST), is, js, ipr(QKMAX) + 4, ndi, kbotop, &
+          f(1,IPDIFF), pro, var, kx1, kx2, ky1, ky2, f(1,ICP), prm(PPR), prm(PPRT), f(1,IRO), &
+          wforce(7,l,lev), ipr(QIFLUX), wforce(1,l,lev), wforce(2,l,lev), wforce(3,l,lev), &
+          wforce(4,l,lev), wforce(5,l,lev), wforce(6,l,lev), lev, 2, 2, 2, f(1,IXC), &
+          f(1,IYC), f(1,IZC), prm(PRKLIM), wenv(3,lev), wenv(4,lev), wenv(5,lev), ilmo, &
+          wforce(8,l,lev), wforce(9,l,lev), wforce(10,l,lev), fl(1,6), fl(1,7), f(1,IRK), &
+          f(1,IREPS), f(1,IPTUR), zero, f(1,ITEMP), f(1,ICH), wforce(11,l,lev), &
+          wforce(12,l,lev), wforce(13,l,lev), prm(PFRSDEN), prm(PFRSPRE), prm(PFRSVEL), &
+          prm(PFRSVIS), wenv(1,lev), wenv(2,lev), ipr(QITURB), istr, jstr, kstr, &
+          wint(6,l,lev), rcon, idis, psk1, pse1, psk2, pse2, w(1)%a, w(2)%a, w(3)%a, &
+          w(10)%a, w(4)%a, w(11)%a, w(12)%a, w(5)%a, w(13)%a, w(14)%a, w(15)%a, w(16)%a, &
+          w(17)%a, dum, dum, dum, dum, dum, dum, w(6)%a, dum, dum, dum, w(18)%a, w(19)%a, &
+          w(20)%a, w(7)%a, w(8)%a, w(9)%a, ntot, cmu, xco, xco, xco, inter, intet, &
+          .false., 'FLUID', 1, turcor, 1, .false., f1h, iwaveb, wforce(17,l,lev), &
+          wforce(18,l,lev), wforce(19,l,lev), wforce(14,l,lev), wforce(15,l,lev), &
+          wforce(16,l,lev), f(1,ISTRAIN), .false., trm, wave, w(21)%a, w(22)%a, w(23)%a, &
+          prc, utaum, fvw, fvw, fvw, prm(PTURLIM))
+     deallocate(dum)
+  end do
+end subroutine ref_walls
 
 subroutine ref_state(f, ntot, prm, ipr) bind(C, name="ref_state")
   use iso_c_binding
@@ -149,6 +290,7 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
   real(c_double) :: f1h(1), wave(1)
   integer :: imax, jmax, kmax, is, js, kstr, kmaxp, ia, iax, iay, iaz, inter, ndi, istr, ido
   integer :: id
+  real(c_double) :: sig(4)
   logical :: xxtral, inchiml, entropy_fix, transl, mulphl, fresul
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
   is = imax + 4; js = jmax + 4
@@ -209,6 +351,9 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
        hat(1,1), hat(1,2), hat(1,3), hat(1,4), kmaxp, kstr, idir, ipr(QITURB), zzz, maw, &
        maxw, xxtral, rksi, inchiml, entropy_fix, transl, trm)
   end if
+  ! BOTSKE on the solid-wall patches of the direction (res3c.f:8092)
+  sig = 0.
+  call ref_walls(f, ntot, prm, ipr, idir, fl, zero, sig, .false.)
   ! SWEEP (res3c.f:330, 448, 559); the j-direction is last and divides by VOL
   ido = 0
   if (idir == 2) ido = 1
@@ -231,6 +376,13 @@ subroutine ref_temder(f, ntot, q, iout, ipr)
        ipr(QJMAX), ipr(QKMAX), 2, 2, 2, 1, ipr(QLEVEL))
 end subroutine ref_temder
 
+! RESIDI zeroes the force sums of the patches on level 1 (ns3c.f:2376-2396)
+subroutine ref_wall_zero()
+  use ffref_walls
+  implicit none
+  wforce(:, :, 1) = 0.
+end subroutine ref_wall_zero
+
 subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   use iso_c_binding
   use ffref_ids
@@ -248,6 +400,7 @@ subroutine ref_residual(f, ntot, prm, ipr, resi) bind(C, name="ref_residual")
   real(c_double) :: r(67)
   integer :: maw, maxw, imax, jmax, kmax
   imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  if (ipr(QLEVEL) == 1) call ref_wall_zero()
   if (ipr(QITURB) == 6) then
      call ref_residual_kw(f, ntot, prm, ipr, resi)
      return
@@ -379,6 +532,8 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
        1, prm(PRGAS), prm(PGAMMA), 0._c_double, 0._c_double, prm(PFRSDEN), prm(PFRSPRE), &
        prm(PRKLIM), prm(PEPSLIM), hat(1,1), hat(1,2), hat(1,3), hat(1,4), kmaxp, kstr, idir, &
        ipr(QITURB), zzz, maw, maxw, xxtral, rksi, inchiml, entropy_fix, transl, trm)
+  ! BOTSKE on the solid-wall patches of the direction (res3c.f:7157)
+  call ref_walls(f, ntot, prm, ipr, idir, fl, zero, sig, .true.)
   ido = 0
   if (idir == 2) ido = 1
   call SWEEPK(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IDRK), f(1,IDEPS), &

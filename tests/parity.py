@@ -2,7 +2,7 @@
 import numpy as np
 
 from finflo_amd.cases import make_block
-from finflo_amd.layout import default_params
+from finflo_amd.layout import WALL_ARRAYS, WALL_OUT, WallParams, WallPatch, default_params
 
 # (name, dims, params overrides, case kwargs)
 CASES = [
@@ -135,3 +135,71 @@ def assert_fields_equal(a, b, names, interior=True, rtol=0.0, what=""):
             scale = max(np.abs(y).max(), 1e-300)
             err = np.abs(x - y).max() / scale
             assert err <= rtol, f"{what} {n}: rel err {err:.3e} > {rtol}"
+
+
+# Solid-wall patches (BOTSKE): (iface, kx1, kx2, ky1, ky2, iheat) as ICON(3..7,L), ICON(20,L).
+# Faces low and high in every direction, partial and whole faces, the four heat options,
+# laminar and SST (with and without TURCOR), with FULLNS and with gravity in the forces
+WALL_CASES = [
+    ("wall_lam_jlow", (10, 8, 6), dict(rmult=500.0),
+     dict(kind="all", amp=0.05, distort=0.12, seed=101), [(2, 2, 9, 1, 6, 0)], False),
+    ("wall_lam_multi", (9, 8, 7), dict(rmult=500.0),
+     dict(kind="all", amp=0.05, distort=0.1, seed=103),
+     [(4, 1, 8, 2, 7, 1), (3, 1, 9, 1, 8, 2), (5, 2, 8, 1, 7, 3), (1, 3, 6, 1, 4, 0)], True),
+    ("wall_sst_jlow", (10, 8, 6), dict(iturb=6),
+     dict(kind="all", amp=0.05, distort=0.12, seed=105, turb=(1.0, 50.0)),
+     [(2, 1, 10, 1, 6, 0), (6, 2, 9, 1, 8, 1)], False),
+    ("wall_sst_noturcor", (9, 8, 7), dict(iturb=6, turcor=0),
+     dict(kind="all", amp=0.05, distort=0.1, seed=107, turb=(0.5, 200.0)),
+     [(2, 1, 9, 1, 7, 3), (4, 1, 8, 1, 7, 0)], True),
+    ("wall_sst_fullns", (8, 9, 6), dict(iturb=6, fullns=1, re=1e14),
+     dict(kind="velocity", amp=0.03, jratio=1.3, seed=109, turb=(0.5, 400.0)),
+     [(2, 1, 8, 1, 6, 2), (1, 1, 9, 1, 6, 1), (5, 1, 8, 1, 6, 0)], False),
+]
+
+
+def make_walls(blk, prm, specs, seed, gravity=False):
+    """Patches of the given ICON entries with seeded wall data (moving walls, wall
+    temperatures and heat fluxes, roughness, face centres) and the BOTSKE constants."""
+    rng = np.random.default_rng(seed)
+    wp = WallParams()
+    wp.t0 = 1.02 * prm.frstem
+    wp.difpre = 0.01 * prm.frspre
+    wp.xmom, wp.ymom, wp.zmom = 0.25, 0.1, -0.05
+    if gravity:
+        wp.gx, wp.gy, wp.gz, wp.ground, wp.altitude = 0.3, -9.81, 0.1, -0.2, -10000.0
+    else:
+        wp.ground, wp.altitude = 0.0, -10000.0
+    patches = []
+    for iface, kx1, kx2, ky1, ky2, iheat in specs:
+        p = WallPatch(iface, kx1, kx2, ky1, ky2, iheat)
+        n = p.size
+        p.a["uwall"][:] = rng.uniform(-1, 1, n)
+        p.a["vwall"][:] = rng.uniform(-1, 1, n)
+        p.a["wwall"][:] = rng.uniform(-1, 1, n)
+        p.a["twall"][:] = prm.frstem * rng.uniform(0.95, 1.1, n)
+        p.a["hflux"][:] = rng.uniform(-50.0, 50.0, n)
+        p.a["rbk"][:] = rng.uniform(0.0, 80.0, n)
+        for c in ("xcp", "ycp", "zcp"):
+            p.a[c][:] = rng.uniform(-1, 1, n)
+        patches.append(p)
+    return wp, patches
+
+
+def make_wall_case(dims, pover, ckw, specs, gravity):
+    prm, blk = make_case(dims, pover, ckw)
+    wp, patches = make_walls(blk, prm, specs, ckw.get("seed", 0) + 900, gravity)
+    return prm, blk, wp, patches
+
+
+def assert_patches_equal(pa, pb, rtol=0.0, what=""):
+    for l, (x, y) in enumerate(zip(pa, pb)):
+        for n in WALL_OUT + ["twall"]:
+            u, v = x.a[n], y.a[n]
+            if rtol == 0.0:
+                bad = np.count_nonzero(u != v)
+                assert bad == 0, f"{what} patch {l} {n}: {bad} values differ, max {np.abs(u - v).max():.3e}"
+            else:
+                scale = max(np.abs(v).max(), 1e-300)
+                err = np.abs(u - v).max() / scale
+                assert err <= rtol, f"{what} patch {l} {n}: rel err {err:.3e} > {rtol}"

@@ -9,8 +9,9 @@ import pytest
 
 from oracle.oracle_py import OracleBlock
 from tests.parity import (BIJ, CASES, EARSM_CASES, FULLNS_CASES, GRADKW, GRADT, PRIM, RES,
-                          SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE, assert_fields_equal,
-                          make_case, make_earsm_case, make_fullns_case)
+                          SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE, WALL_CASES,
+                          assert_fields_equal, assert_patches_equal, make_case,
+                          make_earsm_case, make_fullns_case, make_wall_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -313,3 +314,111 @@ def test_fullns_residual_tiles_vs_cells_vs_oracle(hip_device, dims):
         assert_fields_equal(g, a, RES, what=f"RESIDU FLUXBL ({mode})")
         np.testing.assert_allclose(r[:5], r0[:5], rtol=1e-13)
         ctx.close()
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw,specs,grav", WALL_CASES, ids=[c[0] for c in WALL_CASES])
+def test_walls_stages(hip_device, name, dims, pover, ckw, specs, grav):
+    """BOTSKE at solid-wall patches on the GPU against the oracle: the residual with the wall
+    fluxes, the patch arrays and the force sums (side stream), then IMPS and FIN.  Laminar is
+    bit for bit.  With the SST the residual carries FUN1KO's tanh (1e-12); the wall quantities
+    are bit for bit except F3EPS, whose surface omega takes a device pow."""
+    print(f"case {name} dims {dims} walls {specs}", flush=True)
+    prm, blk, wp, patches = make_wall_case(dims, pover, ckw, specs, grav)
+    kw = prm.iturb == 6
+    tol = SST_RTOL if kw else 0.0
+    a, g = blk.copy(), blk.copy()
+    pa, pg = [p.copy() for p in patches], [p.copy() for p in patches]
+    oa = OracleBlock(prm, a)
+    oa.set_walls(wp, pa)
+    ctx = _ctx(prm, g)
+    ctx.wall_register(wp, pg)
+    oa.save_old(); ctx.save_old()
+    oa.state(); ctx.statim()
+    oa.turb(); ctx.turbdi(); ctx.download()
+    g.F[:] = a.F
+    ctx.upload()
+    r1 = oa.residual(); r2 = ctx.residi(); ctx.download(); ctx.wall_sync(1)
+    assert_fields_equal(g, a, (SST_RES + ["PTUR"]) if kw else RES, rtol=tol,
+                        what="residual with BOTSKE")
+    np.testing.assert_allclose(r2, r1, rtol=1e-11 if kw else 1e-13)
+    assert_patches_equal(pg, pa, what="patch arrays")
+    for l in range(len(patches)):
+        f1, f2 = oa.forces(l), ctx.wall_forces(l)
+        np.testing.assert_array_equal(f2, f1, err_msg=f"forces of patch {l}")
+    g.F[:] = a.F
+    ctx.upload()
+    oa.implicit(); ctx.impsdi(); ctx.download()
+    assert_fields_equal(g, a, ["DTL"] + (SST_RES[:7] if kw else RES), what="IMPS")
+    oa.update(); ctx.fin(); ctx.download()
+    assert_fields_equal(g, a, SST_PRIM if kw else PRIM, interior=False, what="FIN")
+    ctx.close()
+
+
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
+@pytest.mark.parametrize("name,dims,pover,ckw,specs,grav", [WALL_CASES[i] for i in (1, 2)],
+                         ids=[WALL_CASES[i][0] for i in (1, 2)])
+def test_walls_cycles(hip_device, name, dims, pover, ckw, specs, grav, lusgs):
+    """Five cycles with wall patches: the residual form with the wall fluxes, the per-cell
+    form agreeing with it, and the force sums of the last residual."""
+    print(f"case {name} dims {dims} 5 cycles with walls, LU-SGS {lusgs}", flush=True)
+    prm, blk, wp, patches = make_wall_case(dims, pover, ckw, specs, grav)
+    kw = prm.iturb == 6
+    a, g = blk.copy(), blk.copy()
+    pa, pg = [p.copy() for p in patches], [p.copy() for p in patches]
+    oa = OracleBlock(prm, a)
+    oa.set_walls(wp, pa)
+    ctx = _ctx(prm, g, lusgs)
+    ctx.wall_register(wp, pg)
+    for _ in range(5):
+        r1 = oa.cycle()
+    r2 = ctx.cycle(ncycles=5)
+    ctx.download(); ctx.wall_sync(1)
+    assert_fields_equal(g, a, SST_PRIM if kw else PRIM, interior=False,
+                        rtol=SST_RTOL if kw else 0.0, what="5 cycles with walls")
+    np.testing.assert_allclose(r2, r1, rtol=1e-10 if kw else 1e-13)
+    assert_patches_equal(pg, pa, rtol=SST_RTOL if kw else 0.0, what="patch arrays")
+    for l in range(len(patches)):
+        f0 = oa.forces(l)
+        np.testing.assert_allclose(ctx.wall_forces(l), f0, rtol=1e-10 if kw else 0,
+                                   atol=1e-10 * np.abs(f0).max() if kw else 0)
+    ctx.close()
+
+
+def test_walls_residual_cells_form(hip_device):
+    """The per-cell residual form takes the wall fluxes like the tile form."""
+    name, dims, pover, ckw, specs, grav = WALL_CASES[1]
+    prm, blk, wp, patches = make_wall_case(dims, pover, ckw, specs, grav)
+    a = blk.copy()
+    pa = [p.copy() for p in patches]
+    oa = OracleBlock(prm, a)
+    oa.set_walls(wp, pa)
+    oa.save_old(); oa.state(); oa.turb(); oa.residual()
+    from finflo_amd.ffhip import Context
+    g = blk.copy()
+    pg = [p.copy() for p in patches]
+    ctx = Context(prm)
+    ctx.residual_mode("cells")
+    ctx.register(g)
+    ctx.upload()
+    ctx.wall_register(wp, pg)
+    ctx.save_old(); ctx.statim(); ctx.turbdi(); ctx.residi()
+    ctx.download(); ctx.wall_sync(1)
+    assert_fields_equal(g, a, RES, what="per-cell RESIDU with walls")
+    assert_patches_equal(pg, pa, what="patch arrays (cells form)")
+    ctx.close()
+
+
+def test_wall_register_rejects(hip_device):
+    """Patches outside the face, overlapping patches and unsupported heat options are refused
+    with an error, before any launch."""
+    from finflo_amd.ffhip import Context, FFError
+    from tests.parity import make_walls
+    prm, blk = make_case((8, 6, 6), {}, dict(kind="all", amp=0.02, seed=3))
+    ctx = Context(prm)
+    ctx.register(blk)
+    for specs in ([(2, 1, 9, 1, 6, 0)], [(2, 1, 8, 1, 6, 4)],
+                  [(2, 1, 4, 1, 6, 0), (2, 4, 8, 1, 6, 0)], [(7, 1, 2, 1, 2, 0)]):
+        wp, patches = make_walls(blk, prm, specs, 5)
+        with pytest.raises(FFError):
+            ctx.wall_register(wp, patches)
+    ctx.close()

@@ -270,3 +270,27 @@ def test_lusgs_single_cell_is_diagonal_scaling():
     for q in range(5):
         expect = rhs[q].item() * fac * ((sv1 * sv2) if q < 4 else (sp1 * sp2))
         assert np.isclose(out[q].item(), expect, rtol=1e-12), (q, out[q].item(), expect)
+
+
+def test_oracle_walls_do_not_outlive_their_block():
+    """Wall patches set on a level belong to that block: a block made afterwards on the same
+    level runs without them (the earlier patch arrays may be gone)."""
+    import gc
+    from oracle.oracle_py import OracleBlock
+    from tests.parity import WALL_CASES, make_wall_case, RES
+    name, dims, pover, ckw, specs, grav = WALL_CASES[0]
+    prm, blk, wp, patches = make_wall_case(dims, pover, ckw, specs, grav)
+    ow = OracleBlock(prm, blk.copy())
+    ow.set_walls(wp, [p.copy() for p in patches])
+    del ow, patches
+    gc.collect()
+    a, b = blk.copy(), blk.copy()
+    oa = OracleBlock(prm, a)
+    oa.save_old(); oa.state(); oa.turb(); oa.residual()
+    from oracle.oracle_py import oracle
+    import ctypes as C
+    oracle().ffo_wall_set(1, None, None, 0)
+    ob = OracleBlock(prm, b)
+    ob.save_old(); ob.state(); ob.turb(); ob.residual()
+    for n in RES:
+        assert np.array_equal(a[n], b[n])

@@ -6,8 +6,9 @@ import pytest
 
 from oracle.oracle_py import OracleBlock, RefBlock, have_ref
 from tests.parity import (BIJ, CASES, EARSM_CASES, FULLNS_CASES, GRADKW, GRADT, PRIM, RES,
-                          SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE, make_earsm_case,
-                          make_fullns_case, assert_fields_equal, make_case)
+                          SST_CASES, SST_PRIM, SST_RES, SST_TURB, STATE, WALL_CASES,
+                          assert_patches_equal, make_earsm_case, make_fullns_case,
+                          make_wall_case, assert_fields_equal, make_case)
 
 pytestmark = pytest.mark.skipif(not have_ref(), reason="oracle/_ref not built (needs /root/reference)")
 
@@ -217,3 +218,63 @@ def test_fullns_coarse_level_is_thin_layer(name, dims, pover, ckw):
         b["VIST"] = a["VIST"]
     oa.residual(m2=2); rb.residual(m2=2)
     assert_fields_equal(a, b, SST_RES if prm.iturb == 6 else RES, what="coarse residual")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw,specs,grav", WALL_CASES, ids=[c[0] for c in WALL_CASES])
+def test_walls_stages_bit_exact(name, dims, pover, ckw, specs, grav):
+    """BOTSKE on solid-wall patches: the wall fluxes inside RESIDU/RESIKE, the patch arrays
+    (Cp, T, heat fluxes, shear stresses, surface forces), PTUR next to the wall (TURCOR),
+    the force sums, then IMPS and FIN."""
+    prm, blk, wp, patches = make_wall_case(dims, pover, ckw, specs, grav)
+    kw = prm.iturb == 6
+    a, b = blk.copy(), blk.copy()
+    pa, pb = [p.copy() for p in patches], [p.copy() for p in patches]
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    oa.set_walls(wp, pa); rb.set_walls(wp, pb)
+    for o in (oa, rb):
+        o.save_old(); o.state(); o.turb()
+    r1, r2 = oa.residual(), rb.residual()
+    assert_fields_equal(a, b, (SST_RES if kw else RES), what="residual with BOTSKE")
+    assert np.array_equal(r1, r2), "RESL2 sums differ"
+    assert_patches_equal(pa, pb, what="patch arrays")
+    for l in range(len(patches)):
+        f1, f2 = oa.forces(l), rb.forces(l)
+        assert np.array_equal(f1, f2), f"forces of patch {l}: {f1 - f2}"
+        assert np.abs(f1[:3]).max() > 0, "no force"
+    oa.implicit(); rb.implicit()
+    assert_fields_equal(a, b, ["DTL"] + (SST_RES[:7] if kw else RES), what="IMPS")
+    oa.update(); rb.update()
+    assert_fields_equal(a, b, SST_PRIM if kw else PRIM, interior=False, what="FIN")
+
+
+@pytest.mark.parametrize("name,dims,pover,ckw,specs,grav", [WALL_CASES[i] for i in (1, 2)],
+                         ids=[WALL_CASES[i][0] for i in (1, 2)])
+def test_walls_cycles_and_coarse_level(name, dims, pover, ckw, specs, grav):
+    """Four whole cycles with walls, and a coarse-level residual (no forces there)."""
+    prm, blk, wp, patches = make_wall_case(dims, pover, ckw, specs, grav)
+    kw = prm.iturb == 6
+    a, b = blk.copy(), blk.copy()
+    pa, pb = [p.copy() for p in patches], [p.copy() for p in patches]
+    oa, rb = OracleBlock(prm, a), RefBlock(prm, b)
+    oa.set_walls(wp, pa); rb.set_walls(wp, pb)
+    for _ in range(4):
+        r1, r2 = oa.cycle(), rb.cycle()
+        assert np.array_equal(r1, r2)
+    assert_fields_equal(a, b, SST_PRIM if kw else PRIM, interior=False, what="4 cycles with walls")
+    assert_patches_equal(pa, pb, what="patch arrays after 4 cycles")
+    c, d = blk.copy(), blk.copy()
+    if kw:
+        rng = np.random.default_rng(61)
+        c["EPS2"] = 1.0 + rng.uniform(5.0, 50.0, c.ntot)
+        d.F[:] = c.F
+    qa, qb = [p.copy() for p in patches], [p.copy() for p in patches]
+    oc, rc = OracleBlock(prm, c, level=2), RefBlock(prm, d, level=2)
+    oc.set_walls(wp, qa); rc.set_walls(wp, qb)
+    for o in (oc, rc):
+        o.save_old(); o.state()
+    if kw:
+        c["VIST"] = (c["EPS2"] - 1.0) * c["VIS"]
+        d["VIST"] = c["VIST"]
+    oc.residual(m2=2); rc.residual(m2=2)
+    assert_fields_equal(c, d, SST_RES if kw else RES, what="coarse residual with walls")
+    assert_patches_equal(qa, qb, what="coarse patch arrays")
