@@ -153,6 +153,24 @@ def cpu_baseline(prm, blocks, seconds=20.0):
                       f"({el:.1f} s, one thread)"}
 
 
+def committed_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed counter summary
+    (profiles/hbm_traffic_*.md: FETCH_SIZE and WRITE_SIZE passes of this bench, each scaled
+    by what the counter reads for a copy kernel of known size), or None."""
+    tag = {"sst-mg": "sst_mg3", "laminar": "laminar"}[workload]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"hbm_traffic_{tag}_{IMAX}x{JMAX}x{KMAX}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("| `"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            name = cells[0].strip("`")
+            if name == kernel or name.startswith(kernel + "<"):
+                return float(cells[4]) * 1e6, os.path.relpath(path, os.path.dirname(path) + "/..")
+    return None, None
+
+
 def copy_probe(mib):
     """A known-size HBM copy for counter calibration: an elementwise kernel that reads
     MIB MiB and writes MIB MiB (y = x * 1), launched three times."""
@@ -257,10 +275,14 @@ def main():
     avg_ms = ks["ms_total"] / ks["launches"]
     cells_per_launch = ks["cells_total"] / ks["launches"]
     achieved = models[dom] * cells_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic, tsrc = committed_traffic(args.workload, dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "avg_launch_ms": avg_ms, "bytes_per_cell": models[dom],
                 "cells_per_launch": cells_per_launch}
+    if traffic:
+        roofline["traffic_source"] = tsrc
+        roofline["traffic_over_algorithmic"] = traffic / (models[dom] * cells_per_launch)
     l1 = ks.get("level1")
     if l1:
         roofline["level1_GBps"] = models[dom] * cells[1] / (l1["ms_total"] / l1["launches"] * 1e-3) / 1e9
