@@ -55,6 +55,12 @@ KERNEL_BYTES_PER_CELL = {
         "k_corr_km": 8 * (5 + 10 + 12 + 5),
         # 3 faces x 4, VOL, 11 state fields, D in, D and DTL out
         "k_time3_copr": 8 * (12 + 1 + 11 + 5 + 6),
+        # staged LU-SGS: the pass reads D (5), U V W C RO VOL VIS VIST DTL CP, A1..A3 and
+        # their normals (12) and writes the two 56-double records; the sweeps read a record
+        # (56) and write (predictor) or read and write (corrector) the 5 increments
+        "k_sweep_prep": 8 * (5 + 10 + 12 + 2 * 56),
+        "k_pred_st": 8 * (56 + 5),
+        "k_corr_st": 8 * (56 + 5 + 5),
         # every cell incl. ghosts: 5 solved variables in, 12 derived fields out
         "k_state": 8 * (5 + 12),
     },
@@ -68,6 +74,10 @@ KERNEL_BYTES_PER_CELL = {
         "k_corr_tiles": 8 * (7 + 13 + 12 + 7),
         "k_pred_km": 8 * (7 + 12 + 12 + 7),
         "k_corr_km": 8 * (7 + 13 + 12 + 7),
+        # staged LU-SGS (7 increments; RK DRDP DRDH read by the pass as well)
+        "k_sweep_prep": 8 * (7 + 13 + 12 + 2 * 56),
+        "k_pred_st": 8 * (56 + 7),
+        "k_corr_st": 8 * (56 + 7 + 7),
     },
 }
 ALL_CELL_KERNELS = {"k_state", "k_save_old", "k_limits", "k_limits_mg", "k_forcing"}
@@ -76,6 +86,8 @@ ALL_CELL_KERNELS = {"k_state", "k_save_old", "k_limits", "k_limits_mg", "k_forci
 ITER_BYTES_PER_CELL = {0: 704, 6: 880}
 
 KERNELS = ("k_residual_kw", "k_residual_tile", "k_residual", "k_pred_tiles", "k_corr_tiles",
+           "k_sweep_prep", "k_pred_st", "k_corr_st", "k_wall", "k_wall_forces", "k_temder",
+           "k_residual_kw_ns", "k_residual_tile_ns",
            "k_pred_km", "k_corr_km", "k_pred_plane", "k_corr_plane", "k_time3_copr", "k_state", "k_turb_lam",
            "k_turb_sst", "k_grad_zero", "k_derfun", "k_extend", "k_crosko", "k_kw_round",
            "k_max_final", "k_fun1_prod", "k_limits", "k_limits_mg", "k_fin", "k_zero",
@@ -192,7 +204,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="sst-mg")
-    ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes"], default=None,
+    ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -288,7 +300,8 @@ def main():
         roofline["level1_GBps"] = models[dom] * cells[1] / (l1["ms_total"] / l1["launches"] * 1e-3) / 1e9
     sweep_ms = sum(kstats.get(k, {}).get("ms_total", 0.0)
                    for k in ("k_pred_plane", "k_corr_plane", "k_pred_tiles", "k_corr_tiles",
-                             "k_pred_km", "k_corr_km")) / args.steps
+                             "k_pred_km", "k_corr_km", "k_sweep_prep", "k_pred_st",
+                             "k_corr_st")) / args.steps
     iter_bw = ITER_BYTES_PER_CELL[iturb] * ncell / (ms_per_step * 1e-3) / 1e9
 
     cpu = None

@@ -1923,6 +1923,7 @@ struct SweepCell {
     FaceBasis fi, fj, fk;           // faces the outgoing contributions cross
     Coef ci, cj, ck;
     double cp, rho;                 // CORRGS only: DRO/CP and D(rho k), D(rho omega) * RO
+    double fac;                     // CORRGS only: the DIAGGS factor of the increments
 };
 
 template <bool KE>
@@ -2052,6 +2053,7 @@ __device__ __forceinline__ void corr_coef(const Prm &p, const SweepRaw &r, Sweep
     double dtv = p.theta * r.dtl / r.vol;
     double sc = .5 * (r.a1 * r1p + r.a1u * r1n + r.a2 * r2p + r.a2u * r2n + r.a3 * r3p + r.a3u * r3n);
     double fac = 1. + dtv * sc + 0.;
+    s.fac = fac;
 #pragma unroll
     for (int q = 0; q < NV; q++) s.d[q] = r.d[q] * fac;
     // CORRGS / COKEGS own coefficients (upper faces, viscous area A(n))
@@ -2263,6 +2265,341 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
                 halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
             }
             if (q + 1 < b.imax) cur = nxt;
+        }
+        wave_step_sync();
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// LU-SGS with staged cell records.  The (J,K)-tile wavefront above spends most of a step
+// forming the cell's coefficients from some sixty scattered 8-byte loads per lane (each lane
+// its own cache line).  Here a parallel pass (k_sweep_prep) forms, for every cell, the
+// predictor's and the corrector's recurrence-independent values (the same operations as
+// pred_coef/pred_basis, corr_coef/corr_basis) and stores them in the order the sweeps
+// consume them: for tile t and corrector step s the 64 lanes' values of one record field are
+// one contiguous 512-byte row.  A lane (jj,kk) reaches the cell of corrector step s when
+// i = s - (jj + kk) + 1; the predictor reaches the same cell at step NSTEP - 1 - s, so both
+// sweeps read whole rows.  The sweeps then carry only the recurrence and the three outgoing
+// contributions per step.  The predictor's increments go to a row buffer the corrector reads
+// the same way.  Arithmetic is that of the tile kernels, operation for operation.
+// ------------------------------------------------------------------------------------
+constexpr int NRF = 56;            // record fields (doubles) per cell and sweep
+// predictor record: vol*d (7), -dt1*wk, dt1, wi, wj, SCINV, SCINP, 7 scalars, 3 bases (27),
+// 3 x (X1, X2, X5)
+enum { RP_VD = 0, RP_MK = 7, RP_DT1, RP_WI, RP_WJ, RP_SCINV, RP_SCINP, RP_SC = 13,
+       RP_B = 20, RP_X = 47 };
+// corrector record: DIAGGS factor, vol, dt1*wk, dt1, wi, wj, SCINV, SCINP, 7 scalars,
+// 3 bases, 3 x (X1, X2, X5), CP, RO
+enum { RC_FAC = 0, RC_VOL, RC_MK, RC_DT1, RC_WI, RC_WJ, RC_SCINV, RC_SCINP, RC_SC = 8,
+       RC_B = 15, RC_X = 42, RC_CP = 51, RC_RHO = 52 };
+
+struct SweepSt {
+    int nstep;                      // IMAX + TJ + TK - 2
+    double *recp, *recc;            // [tile][step][NRF][TW]
+    double *dn;                     // predictor increments [tile][step][7][TW]
+};
+
+__device__ __forceinline__ size_t st_row(const SweepSt &st, int tile, int s, int f)
+{
+    return (((size_t)tile * st.nstep + s) * NRF + f) * TW;
+}
+
+// the seven SYI/SYM1I (SYIKE/SYM1KE) scalars of a SweepCell in record order
+template <bool KE>
+__device__ __forceinline__ void sc_put(const SweepCell &c, double *v)
+{
+    if (KE) {
+        v[0] = c.yp; v[1] = c.roc; v[2] = c.pc2; v[3] = c.proc; v[4] = c.ydrp; v[5] = c.drhro;
+        v[6] = c.ro667;
+    } else {
+        v[0] = c.ro; v[1] = c.roc; v[2] = c.pc2; v[3] = c.proc; v[4] = v[5] = v[6] = 0.;
+    }
+}
+
+template <bool KE>
+__device__ __forceinline__ void sc_get(SweepCell &c, const double *v)
+{
+    if (KE) {
+        c.yp = v[0]; c.roc = v[1]; c.pc2 = v[2]; c.proc = v[3]; c.ydrp = v[4]; c.drhro = v[5];
+        c.ro667 = v[6];
+    } else {
+        c.ro = v[0]; c.roc = v[1]; c.pc2 = v[2]; c.proc = v[3];
+    }
+}
+
+__device__ __forceinline__ void basis_put(const FaceBasis &f, double *v)
+{
+    v[0] = f.nx; v[1] = f.ny; v[2] = f.nz; v[3] = f.a21; v[4] = f.a22; v[5] = f.a23;
+    v[6] = f.a31; v[7] = f.a32; v[8] = f.a33;
+}
+
+__device__ __forceinline__ FaceBasis basis_get(const double *v)
+{
+    FaceBasis f;
+    f.nx = v[0]; f.ny = v[1]; f.nz = v[2]; f.a21 = v[3]; f.a22 = v[4]; f.a23 = v[5];
+    f.a31 = v[6]; f.a32 = v[7]; f.a33 = v[8];
+    return f;
+}
+
+// Both records of every cell: thread (lane, step) of tile blockIdx.x, four steps per
+// workgroup, so that a workgroup's loads of a column run along I and its record stores are
+// whole rows.
+template <bool KE>
+__global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st)
+{
+    constexpr int NV = KE ? 7 : 5;
+    const int lane = threadIdx.x % TW, s = blockIdx.y * 4 + threadIdx.x / TW;
+    if (s >= st.nstep) return;
+    const int tile = blockIdx.x, tj = tile % sw.ntj, tk = tile / sw.ntj;
+    const int jj = lane % TJ, kk = lane / TJ;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1, i = s - (jj + kk) + 1;
+    if (j > b.jmax || k > b.kmax || i < 1 || i > b.imax) return;
+    const int n = cid(b, i, j, k);
+    double v[NRF];
+    {   // predictor (PREDGS / PRKEGS): lower faces
+        SweepRaw r;
+        sweep_load<false, KE>(b, n, r);
+        SweepCell c;
+        pred_prep<KE>(p, r, i, j, k, c);
+#pragma unroll
+        for (int q = 0; q < 7; q++) v[RP_VD + q] = q < NV ? c.vol * c.d[q] : 0.;
+        v[RP_MK] = -c.dt1 * c.wk;
+        v[RP_DT1] = c.dt1; v[RP_WI] = c.wi; v[RP_WJ] = c.wj;
+        v[RP_SCINV] = c.scinv; v[RP_SCINP] = c.scinp;
+        sc_put<KE>(c, v + RP_SC);
+        basis_put(c.fi, v + RP_B); basis_put(c.fj, v + RP_B + 9); basis_put(c.fk, v + RP_B + 18);
+        const Coef *cf[3] = {&c.ci, &c.cj, &c.ck};
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            v[RP_X + 3 * f] = cf[f]->x1; v[RP_X + 3 * f + 1] = cf[f]->x2; v[RP_X + 3 * f + 2] = cf[f]->x5;
+        }
+        static_assert(RP_X + 9 == NRF, "the predictor record fills NRF fields");
+#pragma unroll
+        for (int f = 0; f < NRF; f++) st.recp[st_row(st, tile, s, f) + lane] = v[f];
+    }
+    {   // corrector (DIAGGS + CORRGS / DIKEGS + COKEGS): upper faces
+        SweepRaw r;
+        sweep_load<true, KE>(b, n, r);
+        SweepCell c;
+        corr_prep<KE>(p, r, c);
+        v[RC_FAC] = c.fac; v[RC_VOL] = c.vol; v[RC_MK] = c.dt1 * c.wk; v[RC_DT1] = c.dt1;
+        v[RC_WI] = c.wi; v[RC_WJ] = c.wj; v[RC_SCINV] = c.scinv; v[RC_SCINP] = c.scinp;
+        sc_put<KE>(c, v + RC_SC);
+        basis_put(c.fi, v + RC_B); basis_put(c.fj, v + RC_B + 9); basis_put(c.fk, v + RC_B + 18);
+        const Coef *cf[3] = {&c.ci, &c.cj, &c.ck};
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            v[RC_X + 3 * f] = cf[f]->x1; v[RC_X + 3 * f + 1] = cf[f]->x2; v[RC_X + 3 * f + 2] = cf[f]->x5;
+        }
+        v[RC_CP] = c.cp; v[RC_RHO] = c.rho;
+        static_assert(RC_RHO < NRF, "the corrector record fits NRF fields");
+        for (int f = RC_RHO + 1; f < NRF; f++) v[f] = 0.;
+#pragma unroll
+        for (int f = 0; f < NRF; f++) st.recc[st_row(st, tile, s, f) + lane] = v[f];
+    }
+}
+
+// the record of a step: one 512-byte row per field
+__device__ __forceinline__ void rec_load(const double *rec, const SweepSt &st, int tile, int s,
+                                         int lane, double v[NRF])
+{
+    const double *base = rec + st_row(st, tile, s, 0) + lane;
+#pragma unroll
+    for (int f = 0; f < NRF; f++) v[f] = base[(size_t)f * TW];
+}
+
+// PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles
+template <bool KE>
+__global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt st)
+{
+    constexpr int NV = KE ? 7 : 5;
+    __shared__ double xj[2][NV][TW], xk[2][NV][TW];
+    const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
+    const int t = tile_ticket(sw);
+    const int tj = sw.ntj - 1 - t % sw.ntj, tk = sw.ntk - 1 - t / sw.ntj;
+    const int tile = tk * sw.ntj + tj;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
+    const bool col = j <= b.jmax && k <= b.kmax;
+    const int lag = (TJ - 1 - jj) + (TK - 1 - kk);
+    const int nsteps = st.nstep;
+    double yi[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yi[r] = 0.;
+    double cur[NRF], nxt[NRF];
+    rec_load(st.recp, st, tile, nsteps - 1, lane, cur);
+    for (int s = 0; s < nsteps; s++) {
+        const int row = nsteps - 1 - s;
+        if (s + 1 < nsteps) rec_load(st.recp, st, tile, row - 1, lane, nxt);
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.imax) {
+            const int i = b.imax - q;
+            const bool hjin = jj == TJ - 1 && j < b.jmax, hkin = kk == TK - 1 && k < b.kmax;
+            unsigned long long hv[NV], hw[NV];
+            if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), hv);
+            if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), hw);
+            const int rb = (s + 1) & 1;
+            double bj[NV], ckc[NV];
+            if (j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = 0.;
+            } else if (!hjin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + 1];
+            } else {
+                halo_finish<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
+            }
+            if (k == b.kmax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) ckc[r] = 0.;
+            } else if (!hkin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane + TJ];
+            } else {
+                halo_finish<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
+            }
+            // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
+            // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
+            if (!KE && j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) yi[r] = 0.;
+            }
+            double dn[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) {
+                double cd = cur[RP_MK] * ckc[r] + cur[RP_VD + r];
+                cd = cd - cur[RP_DT1] * (cur[RP_WI] * yi[r] + cur[RP_WJ] * bj[r]);
+                dn[r] = r != 4 ? cd * cur[RP_SCINV] : cd * cur[RP_SCINP];
+            }
+            double *dno = st.dn + (((size_t)tile * nsteps + row) * 7) * TW + lane;
+#pragma unroll
+            for (int r = 0; r < NV; r++) dno[(size_t)r * TW] = dn[r];
+            SweepCell sc;
+            sc_get<KE>(sc, cur + RP_SC);
+            Coef ci, cj, ck;
+            ci.x1 = cur[RP_X]; ci.x2 = cur[RP_X + 1]; ci.x5 = cur[RP_X + 2];
+            cj.x1 = cur[RP_X + 3]; cj.x2 = cur[RP_X + 4]; cj.x5 = cur[RP_X + 5];
+            ck.x1 = cur[RP_X + 6]; ck.x2 = cur[RP_X + 7]; ck.x5 = cur[RP_X + 8];
+            double oj[NV], ok[NV];
+            face_xfer<KE>(dn, sc, basis_get(cur + RP_B), ci, yi);
+            face_xfer<KE>(dn, sc, basis_get(cur + RP_B + 9), cj, oj);
+            face_xfer<KE>(dn, sc, basis_get(cur + RP_B + 18), ck, ok);
+            const int wb = s & 1;
+            if (jj > 0) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
+            } else if (j > 1) {
+                halo_put<NV, HQS>(hj_slot(sw, b, tj, i, k), oj);
+            }
+            if (kk > 0) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
+            } else if (k > 1) {
+                halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), ok);
+            }
+        }
+        if (s + 1 < nsteps) {
+#pragma unroll
+            for (int f = 0; f < NRF; f++) cur[f] = nxt[f];
+        }
+        wave_step_sync();
+    }
+}
+
+// DIAGGS + CORRGS / DIKEGS + COKEGS on staged records, the wavefront of k_corr_tiles, with
+// the end of IMPS (DRO/CP, D(rho k) and D(rho omega) times RO)
+template <bool KE>
+__global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt st)
+{
+    constexpr int NV = KE ? 7 : 5;
+    __shared__ double xj[2][NV][TW], xk[2][NV][TW];
+    const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
+    const int t = tile_ticket(sw);
+    const int tj = t % sw.ntj, tk = t / sw.ntj;
+    const int tile = tk * sw.ntj + tj;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
+    const bool col = j <= b.jmax && k <= b.kmax;
+    const int lag = jj + kk;
+    const int nsteps = st.nstep;
+    double yi[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yi[r] = 0.;
+    double cur[NRF], nxt[NRF];
+    rec_load(st.recc, st, tile, 0, lane, cur);
+    for (int s = 0; s < nsteps; s++) {
+        if (s + 1 < nsteps) rec_load(st.recc, st, tile, s + 1, lane, nxt);
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.imax) {
+            const int i = q + 1;
+            const int n = cid(b, i, j, k);
+            const bool hjin = jj == 0 && j > 1, hkin = kk == 0 && k > 1;
+            unsigned long long hv[NV], hw[NV];
+            if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj, i, k), hv);
+            if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk, i, j), hw);
+            const double *dni = st.dn + (((size_t)tile * nsteps + s) * 7) * TW + lane;
+            double d[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) d[r] = dni[(size_t)r * TW] * cur[RC_FAC];
+            const int rb = (s + 1) & 1;
+            double bj[NV], ckc[NV];
+            if (j == 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = 0.;
+            } else if (!hjin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - 1];
+            } else {
+                halo_finish<NV, HQS>(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
+            }
+            if (k == 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) ckc[r] = 0.;
+            } else if (!hkin) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane - TJ];
+            } else {
+                halo_finish<NV, HQS>(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
+            }
+            double dn[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) {
+                double cd = cur[RC_MK] * ckc[r] + cur[RC_VOL] * d[r];
+                cd = cd + cur[RC_DT1] * (cur[RC_WI] * yi[r] + cur[RC_WJ] * bj[r]);
+                dn[r] = r != 4 ? cd * cur[RC_SCINV] : cd * cur[RC_SCINP];
+            }
+            FLD(b, FF_DRO)[n] = dn[0] / cur[RC_CP];
+#pragma unroll
+            for (int r = 1; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
+            if (KE) {
+                FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * cur[RC_RHO];
+                FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * cur[RC_RHO];
+            }
+            SweepCell sc;
+            sc_get<KE>(sc, cur + RC_SC);
+            Coef ci, cj, ck;
+            ci.x1 = cur[RC_X]; ci.x2 = cur[RC_X + 1]; ci.x5 = cur[RC_X + 2];
+            cj.x1 = cur[RC_X + 3]; cj.x2 = cur[RC_X + 4]; cj.x5 = cur[RC_X + 5];
+            ck.x1 = cur[RC_X + 6]; ck.x2 = cur[RC_X + 7]; ck.x5 = cur[RC_X + 8];
+            double oj[NV], ok[NV];
+            face_xfer<KE>(dn, sc, basis_get(cur + RC_B), ci, yi);
+            face_xfer<KE>(dn, sc, basis_get(cur + RC_B + 9), cj, oj);
+            face_xfer<KE>(dn, sc, basis_get(cur + RC_B + 18), ck, ok);
+            const int wb = s & 1;
+            if (jj < TJ - 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
+            } else if (j < b.jmax) {
+                halo_put<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), oj);
+            }
+            if (kk < TK - 1) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
+            } else if (k < b.kmax) {
+                halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
+            }
+        }
+        if (s + 1 < nsteps) {
+#pragma unroll
+            for (int f = 0; f < NRF; f++) cur[f] = nxt[f];
         }
         wave_step_sync();
     }
@@ -2876,6 +3213,8 @@ struct BlockRec {
     int kw_parts = 0;
     int level = 0;             // multigrid level M the slab was registered as
     WallRec walls;
+    SweepSt st{};              // staged records of the (J,K)-tile sweeps (allocated on first use)
+    double *st_mem = nullptr;
 };
 
 struct PendingEv {
@@ -2896,7 +3235,7 @@ struct ff_ctx {
     double *red_host = nullptr;   // pinned host sums
     int red_parts = 0;
     bool profile = false;
-    int lusgs_mode = FF_LUSGS_TILES;
+    int lusgs_mode = FF_LUSGS_STAGED;
     int residual_mode = FF_RESIDUAL_TILES;
     std::vector<PendingEv> pending;
     // launches and event-timed milliseconds per kernel name, and per "name@level"
@@ -3076,6 +3415,8 @@ static void walls_free(WallRec &w)
 static void sweep_free(BlockRec &r)
 {
     walls_free(r.walls);
+    if (r.st_mem) (void)hipFree(r.st_mem);
+    r.st_mem = nullptr;
     if (r.sw_mem) (void)hipFree(r.sw_mem);
     r.sw_mem = nullptr;
     if (r.kw_scr) (void)hipFree(r.kw_scr);
@@ -3216,7 +3557,8 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
 {
     if (!c) return FF_E_ARG;
     if (option == FF_OPT_LUSGS) {
-        if (value != FF_LUSGS_TILES && value != FF_LUSGS_PLANES && value != FF_LUSGS_KMARCH)
+        if (value != FF_LUSGS_TILES && value != FF_LUSGS_PLANES && value != FF_LUSGS_KMARCH &&
+            value != FF_LUSGS_STAGED)
             return fail(c, FF_E_ARG, "FF_OPT_LUSGS: unknown value%s");
         c->lusgs_mode = value;
         return FF_OK;
@@ -3517,6 +3859,30 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         }
         if ((rc = launch(c, "k_pred_km", k_sweep_km<false, false>, g, dim3(4 * TW), kp, b, r->swk))) return rc;
         return launch(c, "k_corr_km", k_sweep_km<true, false>, g, dim3(4 * TW), kp, b, r->swk);
+    }
+    if (c->lusgs_mode == FF_LUSGS_STAGED) {
+        // records of both sweeps in step order, then the two wavefronts on them
+        if (!r->st_mem) {
+            SweepSt &st = r->st;
+            st.nstep = b.imax + TJ + TK - 2;
+            const size_t nt = (size_t)r->sw.ntj * r->sw.ntk;
+            const size_t nrec = nt * st.nstep * NRF * TW, ndn = nt * st.nstep * 7 * TW;
+            if (hipMalloc(&r->st_mem, sizeof(double) * (2 * nrec + ndn)) != hipSuccess)
+                return fail(c, FF_E_NOMEM, "LU-SGS staged records%s");
+            st.recp = r->st_mem;
+            st.recc = st.recp + nrec;
+            st.dn = st.recc + nrec;
+        }
+        dim3 g(r->sw.ntj * r->sw.ntk), gp(r->sw.ntj * r->sw.ntk, (r->st.nstep + 3) / 4);
+        const bool ke = c->prm.iturb == 6;
+        rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(4 * TW), kp, b, r->sw, r->st)
+                : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(4 * TW), kp, b, r->sw, r->st);
+        if (rc) return rc;
+        rc = ke ? launch(c, "k_pred_st", k_pred_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
+                : launch(c, "k_pred_st", k_pred_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
+        if (rc) return rc;
+        return ke ? launch(c, "k_corr_st", k_corr_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
+                  : launch(c, "k_corr_st", k_corr_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
     }
     if (c->lusgs_mode == FF_LUSGS_TILES) {
         // DRO/CP is applied inside the forward sweep

@@ -24,7 +24,7 @@ EXPORTS = [
     "ff_prolong", "ff_vcycle", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
 ]
 
-FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH = 1, 0, 1, 2
+FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 
 _lib = None
@@ -108,7 +108,8 @@ class Context:
         """LU-SGS sweep form: "tiles" ((J,K) tiles marching I), "kmarch" ((I,J) tiles
         marching K) or "planes" (one launch per hyperplane)."""
         self.set_option(FF_OPT_LUSGS, {"tiles": FF_LUSGS_TILES, "planes": FF_LUSGS_PLANES,
-                                       "kmarch": FF_LUSGS_KMARCH}[mode])
+                                       "kmarch": FF_LUSGS_KMARCH,
+                                       "staged": FF_LUSGS_STAGED}[mode])
 
     def residual_mode(self, mode):
         """Residual form: "tiles" (each face once) or "cells" (both faces per cell)."""

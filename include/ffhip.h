@@ -167,14 +167,19 @@ int  ff_ctx_destroy(ff_ctx *ctx);
 int  ff_ctx_set_stream(ff_ctx *ctx, void *stream);
 const char *ff_last_error(const ff_ctx *ctx);
 /* Context options (no counterpart in the reference, which has one form of each stage):
- *   FF_OPT_LUSGS: FF_LUSGS_TILES (default) runs each LU-SGS sweep as one persistent
- *   launch of (J,K) tiles marching along I; FF_LUSGS_KMARCH as one persistent launch of
- *   (I,J) tiles marching along K; FF_LUSGS_PLANES launches one kernel per I+J+K
- *   hyperplane (laminar only).  All give the same results bit for bit. */
+ *   FF_OPT_LUSGS: FF_LUSGS_STAGED (default, below); FF_LUSGS_TILES runs each LU-SGS sweep
+ *   as one persistent launch of (J,K) tiles marching along I that form their coefficients
+ *   on the way; FF_LUSGS_KMARCH as one persistent launch of (I,J) tiles marching along K;
+ *   FF_LUSGS_PLANES launches one kernel per I+J+K hyperplane (laminar only).  All give the
+ *   same results bit for bit. */
 #define FF_OPT_LUSGS     1
 #define FF_LUSGS_TILES   0
 #define FF_LUSGS_PLANES  1
 #define FF_LUSGS_KMARCH  2
+/*   FF_LUSGS_STAGED: the (J,K) tiles of FF_LUSGS_TILES on cell records that a parallel pass
+ *   forms beforehand in the sweeps' step order (coalesced, the recurrence-independent work
+ *   off the wavefront); needs 2 x 56 + 7 doubles per cell of device memory per level. */
+#define FF_LUSGS_STAGED  3
 /*   FF_OPT_RESIDUAL: FF_RESIDUAL_TILES (default) forms every face flux once in (I,J) tiles
  *   marching along K; FF_RESIDUAL_CELLS forms both faces of every direction per cell.
  *   Both give the same results bit for bit. */

@@ -102,7 +102,7 @@ def test_coarse_level_stages(hip_device, iturb):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
 @pytest.mark.parametrize("nlev", [2, 3])
 @pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
 def test_vcycles(hip_device, iturb, nlev, lusgs):
