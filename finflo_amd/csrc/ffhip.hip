@@ -782,6 +782,32 @@ __device__ __forceinline__ void sourko_cell(const Prm &p, const Blk &b, int L, d
 }
 
 // ------------------------------------------------------------------------------------
+// XCD-aware workgroup numbering.  The dispatcher deals workgroups to the eight XCDs round-
+// robin (XCD = linear id mod 8), each XCD with its own L2.  A grid padded to a multiple of 8
+// is renumbered so that every XCD runs one contiguous range of the logical tiles: tiles that
+// share halo cells (neighbours along I, J and the next K slab) then meet in the same L2.
+// ------------------------------------------------------------------------------------
+constexpr int NXCD = 8;
+struct TileGrid {
+    int nx, ny, nz;   // logical tiles along the three axes
+    int per;          // logical tiles per XCD (the launch has NXCD * per workgroups)
+};
+
+static inline TileGrid tile_grid(int nx, int ny, int nz)
+{
+    const int n = nx * ny * nz;
+    return TileGrid{nx, ny, nz, (n + NXCD - 1) / NXCD};
+}
+
+// logical tile of this workgroup, or -1 for the padding
+__device__ __forceinline__ int xcd_tile(const TileGrid &g)
+{
+    const int w = blockIdx.x;
+    const int l = (w % NXCD) * g.per + w / NXCD;
+    return l < g.nx * g.ny * g.nz ? l : -1;
+}
+
+// ------------------------------------------------------------------------------------
 // Solid-wall patches: BOTSKE base3c.f:6147-7095.  Before the residual kernel, k_wall forms
 // per patch face the wall flux that replaces the face flux (loop 1000), the patch arrays,
 // PTUR next to the wall and, with TURCOR and the SST, the corrections of the second and
@@ -1197,13 +1223,16 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
 // (res3c.f:4957-5068), with every face flux formed once: the layout of k_residual_tile with
 // seven components.  FNS: the FULLNS face fluxes FLKEBL + FLUXBL (level 1).
 template <int RIW, int RJW, int RKW, bool FNS = false>
-__global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev wd)
+__global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev wd, TileGrid tg)
 {
     __shared__ double fi[7][RJW][RIW + 1];
     __shared__ double fj[7][RJW + 1][RIW];
     const SstC sc = coefko(p.kover);
     const int ti = threadIdx.x % RIW, tj = threadIdx.x / RIW;
-    const int i0 = blockIdx.x * RIW + 1, j0 = blockIdx.y * RJW + 1, k0 = blockIdx.z * RKW + 1;
+    const int tl = xcd_tile(tg);
+    if (tl < 0) return;
+    const int bx = tl % tg.nx, by = (tl / tg.nx) % tg.ny, bz = tl / (tg.nx * tg.ny);
+    const int i0 = bx * RIW + 1, j0 = by * RJW + 1, k0 = bz * RKW + 1;
     const int i = i0 + ti, j = j0 + tj;
     const bool cell = i <= b.imax && j <= b.jmax;
     const int il = min(RIW, b.imax - i0 + 1), jl = min(RJW, b.jmax - j0 + 1);
@@ -1343,12 +1372,15 @@ constexpr int RI = 64, RJ = 4, RKC = 8;
 static_assert(RJ >= 2 && RJ <= RI, "the tile's upper I-faces are formed by the lanes of row 1");
 
 template <int IFLUX, bool FNS = false>
-__global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev wd)
+__global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev wd, TileGrid tg)
 {
     __shared__ double fi[5][RJ][RI + 1];
     __shared__ double fj[5][RJ + 1][RI];
     const int ti = threadIdx.x % RI, tj = threadIdx.x / RI;
-    const int i0 = blockIdx.x * RI + 1, j0 = blockIdx.y * RJ + 1, k0 = blockIdx.z * RKC + 1;
+    const int tl = xcd_tile(tg);
+    if (tl < 0) return;
+    const int bx = tl % tg.nx, by = (tl / tg.nx) % tg.ny, bz = tl / (tg.nx * tg.ny);
+    const int i0 = bx * RI + 1, j0 = by * RJ + 1, k0 = bz * RKC + 1;
     const int i = i0 + ti, j = j0 + tj;
     const bool cell = i <= b.imax && j <= b.jmax;
     const int il = min(RI, b.imax - i0 + 1), jl = min(RJ, b.jmax - j0 + 1);   // cells in the tile
@@ -2345,12 +2377,17 @@ __device__ __forceinline__ FaceBasis basis_get(const double *v)
 // workgroup, so that a workgroup's loads of a column run along I and its record stores are
 // whole rows.
 template <bool KE>
-__global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st)
+__global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st,
+                                                      TileGrid tg)
 {
     constexpr int NV = KE ? 7 : 5;
-    const int lane = threadIdx.x % TW, s = blockIdx.y * 4 + threadIdx.x / TW;
+    // logical tiles: (step chunk, tile), chunks of one tile consecutive, so that a tile's
+    // columns stay in one XCD's L2 while their cells are read
+    const int tl = xcd_tile(tg);
+    if (tl < 0) return;
+    const int lane = threadIdx.x % TW, s = (tl % tg.nx) * 4 + threadIdx.x / TW;
     if (s >= st.nstep) return;
-    const int tile = blockIdx.x, tj = tile % sw.ntj, tk = tile / sw.ntj;
+    const int tile = tl / tg.nx, tj = tile % sw.ntj, tk = tile / sw.ntj;
     const int jj = lane % TJ, kk = lane / TJ;
     const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1, i = s - (jj + kk) + 1;
     if (j > b.jmax || k > b.kmax || i < 1 || i > b.imax) return;
@@ -3797,18 +3834,21 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
                          (const double *)(r->kw_scr + r->kw_parts)))) return rc;
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
-        dim3 g((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
-        rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b, wr.wd)
-                 : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b, wr.wd);
+        const TileGrid tg = tile_grid((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
+        dim3 g(NXCD * tg.per);
+        rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b, wr.wd, tg)
+                 : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b, wr.wd, tg);
     } else {
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
         if (c->residual_mode == FF_RESIDUAL_TILES) {
-            dim3 g((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ, (b.kmax + RKC - 1) / RKC);
+            const TileGrid tg = tile_grid((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ,
+                                          (b.kmax + RKC - 1) / RKC);
+            dim3 g(NXCD * tg.per);
             auto kern = fns ? (vl ? k_residual_tile<0, true> : k_residual_tile<1, true>)
                             : (vl ? k_residual_tile<0> : k_residual_tile<1>);
             rc = launch(c, fns ? "k_residual_tile_ns" : "k_residual_tile", kern, g,
-                        dim3(RI * RJ), make_prm(&c->prm, level), b, wr.wd);
+                        dim3(RI * RJ), make_prm(&c->prm, level), b, wr.wd, tg);
         } else {
             auto kern = fns ? (vl ? k_residual<0, true> : k_residual<1, true>)
                             : (vl ? k_residual<0> : k_residual<1>);
@@ -3873,10 +3913,12 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
             st.recc = st.recp + nrec;
             st.dn = st.recc + nrec;
         }
-        dim3 g(r->sw.ntj * r->sw.ntk), gp(r->sw.ntj * r->sw.ntk, (r->st.nstep + 3) / 4);
+        dim3 g(r->sw.ntj * r->sw.ntk);
+        const TileGrid tp = tile_grid((r->st.nstep + 3) / 4, r->sw.ntj * r->sw.ntk, 1);
+        dim3 gp(NXCD * tp.per);
         const bool ke = c->prm.iturb == 6;
-        rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(4 * TW), kp, b, r->sw, r->st)
-                : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(4 * TW), kp, b, r->sw, r->st);
+        rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(4 * TW), kp, b, r->sw, r->st, tp)
+                : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(4 * TW), kp, b, r->sw, r->st, tp);
         if (rc) return rc;
         rc = ke ? launch(c, "k_pred_st", k_pred_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
                 : launch(c, "k_pred_st", k_pred_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
