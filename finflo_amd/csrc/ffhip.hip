@@ -2302,6 +2302,39 @@ __global__ void __launch_bounds__(TW) k_corr_tiles(Prm p, Blk b, Sweep sw)
     }
 }
 
+// Wait until at most n vector-memory operations of this wave are outstanding (they
+// complete in issue order).  The LDS rings below are filled by LDS-DMA loads, which the
+// compiler does not order against the LDS reads of the rings; the memory clobber keeps
+// those reads after the wait.
+template <int N>
+__device__ __forceinline__ void vm_wait()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// One LDS-DMA load: every lane fetches the 4 bytes at its `src` into LDS byte address
+// lds + 4*lane (M0 holds the wave's base).  Issued from inline assembly so that the
+// compiler, which cannot tell which LDS bytes such a load writes, does not follow it with
+// a full vmcnt wait before every LDS read; the kernels wait for these loads themselves
+// (vm_wait).  SC1 selects the device-coherent form used for the halo slots.  The s_nop is
+// the wait state an LDS-DMA load needs after an SALU write of M0.
+template <bool SC1>
+__device__ __forceinline__ void lds_dma4(const void *src, unsigned lds)
+{
+    if (SC1)
+        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off sc1"
+                     ::"v"(src), "s"(lds) : "memory", "m0");
+    else
+        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
+                     ::"v"(src), "s"(lds) : "memory", "m0");
+}
+
+__device__ __forceinline__ unsigned lds_addr(const void *p)
+{
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) void *)p;
+}
+
 // ------------------------------------------------------------------------------------
 // LU-SGS with staged cell records.  The (J,K)-tile wavefront above spends most of a step
 // forming the cell's coefficients from some sixty scattered 8-byte loads per lane (each lane
@@ -2325,10 +2358,11 @@ enum { RP_VD = 0, RP_MK = 7, RP_DT1, RP_WI, RP_WJ, RP_SCINV, RP_SCINP, RP_SC = 1
 enum { RC_FAC = 0, RC_VOL, RC_MK, RC_DT1, RC_WI, RC_WJ, RC_SCINV, RC_SCINP, RC_SC = 8,
        RC_B = 15, RC_X = 42, RC_CP = 51, RC_RHO = 52 };
 
+constexpr int NDN = 8;             // rows of predictor increments per step (7 used)
 struct SweepSt {
     int nstep;                      // IMAX + TJ + TK - 2
     double *recp, *recc;            // [tile][step][NRF][TW]
-    double *dn;                     // predictor increments [tile][step][7][TW]
+    double *dn;                     // predictor increments [tile][step][NDN][TW]
 };
 
 __device__ __forceinline__ size_t st_row(const SweepSt &st, int tile, int s, int f)
@@ -2436,13 +2470,30 @@ __global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, S
     }
 }
 
-// the record of a step: one 512-byte row per field
-__device__ __forceinline__ void rec_load(const double *rec, const SweepSt &st, int tile, int s,
-                                         int lane, double v[NRF])
+// The sweeps stream their records through an LDS ring: at step s the wave starts the copy of
+// the record of step s + RS - 1 (and, in the corrector, the predictor's increments of that
+// step) with 16-byte LDS-DMA loads, one instruction per 1 KiB of the contiguous record, and
+// reads the record of step s from its slot.  The copies are issued from inline assembly, so
+// the compiler does not wait for all of them before each LDS read; the sweep waits itself
+// until the copy of the slot it reads is complete (vm_wait: loads complete in issue order,
+// and the copies of the RS - 2 later steps may stay in flight).
+constexpr int RBYTES = NRF * TW * 8;            // one record: 28 KiB
+constexpr int DBYTES = NDN * TW * 8;            // one step of increments: 4 KiB
+
+__device__ __forceinline__ void lds_dma16(const char *src, unsigned lds)
 {
-    const double *base = rec + st_row(st, tile, s, 0) + lane;
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                 ::"v"(src), "s"(lds) : "memory", "m0");
+}
+
+// copy `bytes` (a multiple of 1 KiB) from global src to LDS dst, 16 bytes per lane per load
+template <int BYTES>
+__device__ __forceinline__ void ring_copy(const void *src, void *dst, int lane)
+{
+    const char *g = (const char *)src + lane * 16;
+    const unsigned l = lds_addr(dst);
 #pragma unroll
-    for (int f = 0; f < NRF; f++) v[f] = base[(size_t)f * TW];
+    for (int m = 0; m < BYTES / 1024; m++) lds_dma16(g + m * 1024, l + m * 1024);
 }
 
 // PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles
@@ -2450,6 +2501,10 @@ template <bool KE>
 __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
     constexpr int NV = KE ? 7 : 5;
+    constexpr int RS = 4;                               // ring slots: 3 steps of lead
+    constexpr int NLD = RBYTES / 1024;                  // loads per step
+    static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
+    __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
     __shared__ double xj[2][NV][TW], xk[2][NV][TW];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
     const int t = tile_ticket(sw);
@@ -2462,11 +2517,16 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
     double yi[NV];
 #pragma unroll
     for (int r = 0; r < NV; r++) yi[r] = 0.;
-    double cur[NRF], nxt[NRF];
-    rec_load(st.recp, st, tile, nsteps - 1, lane, cur);
+    // predictor step s reads record row nsteps - 1 - s
+    for (int s = 0; s < RS - 1 && s < nsteps; s++)
+        ring_copy<RBYTES>(st.recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
     for (int s = 0; s < nsteps; s++) {
         const int row = nsteps - 1 - s;
-        if (s + 1 < nsteps) rec_load(st.recp, st, tile, row - 1, lane, nxt);
+        // the copy of step s is complete once at most the RS - 2 later copies are in flight
+        if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
+        if (s + RS - 1 < nsteps)
+            ring_copy<RBYTES>(st.recp + st_row(st, tile, row - (RS - 1), 0), ring[(s + RS - 1) % RS], lane);
+        const double *cur = ring[s % RS] + lane;
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = b.imax - q;
@@ -2503,23 +2563,31 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
             double dn[NV];
 #pragma unroll
             for (int r = 0; r < NV; r++) {
-                double cd = cur[RP_MK] * ckc[r] + cur[RP_VD + r];
-                cd = cd - cur[RP_DT1] * (cur[RP_WI] * yi[r] + cur[RP_WJ] * bj[r]);
-                dn[r] = r != 4 ? cd * cur[RP_SCINV] : cd * cur[RP_SCINP];
+                double cd = cur[RP_MK * TW] * ckc[r] + cur[(RP_VD + r) * TW];
+                cd = cd - cur[RP_DT1 * TW] * (cur[RP_WI * TW] * yi[r] + cur[RP_WJ * TW] * bj[r]);
+                dn[r] = r != 4 ? cd * cur[RP_SCINV * TW] : cd * cur[RP_SCINP * TW];
             }
-            double *dno = st.dn + (((size_t)tile * nsteps + row) * 7) * TW + lane;
+            double *dno = st.dn + (((size_t)tile * nsteps + row) * NDN) * TW + lane;
 #pragma unroll
             for (int r = 0; r < NV; r++) dno[(size_t)r * TW] = dn[r];
             SweepCell sc;
-            sc_get<KE>(sc, cur + RP_SC);
+            double scv[7];
+#pragma unroll
+            for (int m = 0; m < 7; m++) scv[m] = cur[(RP_SC + m) * TW];
+            sc_get<KE>(sc, scv);
+            double bv[27], xv[9];
+#pragma unroll
+            for (int m = 0; m < 27; m++) bv[m] = cur[(RP_B + m) * TW];
+#pragma unroll
+            for (int m = 0; m < 9; m++) xv[m] = cur[(RP_X + m) * TW];
             Coef ci, cj, ck;
-            ci.x1 = cur[RP_X]; ci.x2 = cur[RP_X + 1]; ci.x5 = cur[RP_X + 2];
-            cj.x1 = cur[RP_X + 3]; cj.x2 = cur[RP_X + 4]; cj.x5 = cur[RP_X + 5];
-            ck.x1 = cur[RP_X + 6]; ck.x2 = cur[RP_X + 7]; ck.x5 = cur[RP_X + 8];
+            ci.x1 = xv[0]; ci.x2 = xv[1]; ci.x5 = xv[2];
+            cj.x1 = xv[3]; cj.x2 = xv[4]; cj.x5 = xv[5];
+            ck.x1 = xv[6]; ck.x2 = xv[7]; ck.x5 = xv[8];
             double oj[NV], ok[NV];
-            face_xfer<KE>(dn, sc, basis_get(cur + RP_B), ci, yi);
-            face_xfer<KE>(dn, sc, basis_get(cur + RP_B + 9), cj, oj);
-            face_xfer<KE>(dn, sc, basis_get(cur + RP_B + 18), ck, ok);
+            face_xfer<KE>(dn, sc, basis_get(bv), ci, yi);
+            face_xfer<KE>(dn, sc, basis_get(bv + 9), cj, oj);
+            face_xfer<KE>(dn, sc, basis_get(bv + 18), ck, ok);
             const int wb = s & 1;
             if (jj > 0) {
 #pragma unroll
@@ -2534,12 +2602,9 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
                 halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), ok);
             }
         }
-        if (s + 1 < nsteps) {
-#pragma unroll
-            for (int f = 0; f < NRF; f++) cur[f] = nxt[f];
-        }
         wave_step_sync();
     }
+    vm_wait<0>();
 }
 
 // DIAGGS + CORRGS / DIKEGS + COKEGS on staged records, the wavefront of k_corr_tiles, with
@@ -2548,6 +2613,11 @@ template <bool KE>
 __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
     constexpr int NV = KE ? 7 : 5;
+    constexpr int RS = 3;                               // ring slots: 2 steps of lead
+    constexpr int NLD = (RBYTES + DBYTES) / 1024;       // loads per step
+    static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
+    __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
+    __shared__ __attribute__((aligned(16))) double dring[RS][NDN * TW];
     __shared__ double xj[2][NV][TW], xk[2][NV][TW];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
     const int t = tile_ticket(sw);
@@ -2560,10 +2630,16 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
     double yi[NV];
 #pragma unroll
     for (int r = 0; r < NV; r++) yi[r] = 0.;
-    double cur[NRF], nxt[NRF];
-    rec_load(st.recc, st, tile, 0, lane, cur);
+    auto copy_step = [&](int s) {
+        ring_copy<RBYTES>(st.recc + st_row(st, tile, s, 0), ring[s % RS], lane);
+        ring_copy<DBYTES>(st.dn + (((size_t)tile * nsteps + s) * NDN) * TW, dring[s % RS], lane);
+    };
+    for (int s = 0; s < RS - 1 && s < nsteps; s++) copy_step(s);
     for (int s = 0; s < nsteps; s++) {
-        if (s + 1 < nsteps) rec_load(st.recc, st, tile, s + 1, lane, nxt);
+        if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
+        if (s + RS - 1 < nsteps) copy_step(s + RS - 1);
+        const double *cur = ring[s % RS] + lane;
+        const double *dni = dring[s % RS] + lane;
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = q + 1;
@@ -2572,10 +2648,9 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             unsigned long long hv[NV], hw[NV];
             if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj, i, k), hv);
             if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk, i, j), hw);
-            const double *dni = st.dn + (((size_t)tile * nsteps + s) * 7) * TW + lane;
             double d[NV];
 #pragma unroll
-            for (int r = 0; r < NV; r++) d[r] = dni[(size_t)r * TW] * cur[RC_FAC];
+            for (int r = 0; r < NV; r++) d[r] = dni[r * TW] * cur[RC_FAC * TW];
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
             if (j == 1) {
@@ -2599,27 +2674,35 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             double dn[NV];
 #pragma unroll
             for (int r = 0; r < NV; r++) {
-                double cd = cur[RC_MK] * ckc[r] + cur[RC_VOL] * d[r];
-                cd = cd + cur[RC_DT1] * (cur[RC_WI] * yi[r] + cur[RC_WJ] * bj[r]);
-                dn[r] = r != 4 ? cd * cur[RC_SCINV] : cd * cur[RC_SCINP];
+                double cd = cur[RC_MK * TW] * ckc[r] + cur[RC_VOL * TW] * d[r];
+                cd = cd + cur[RC_DT1 * TW] * (cur[RC_WI * TW] * yi[r] + cur[RC_WJ * TW] * bj[r]);
+                dn[r] = r != 4 ? cd * cur[RC_SCINV * TW] : cd * cur[RC_SCINP * TW];
             }
-            FLD(b, FF_DRO)[n] = dn[0] / cur[RC_CP];
+            FLD(b, FF_DRO)[n] = dn[0] / cur[RC_CP * TW];
 #pragma unroll
             for (int r = 1; r < 5; r++) FLD(b, FF_DRO + r)[n] = dn[r];
             if (KE) {
-                FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * cur[RC_RHO];
-                FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * cur[RC_RHO];
+                FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * cur[RC_RHO * TW];
+                FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * cur[RC_RHO * TW];
             }
             SweepCell sc;
-            sc_get<KE>(sc, cur + RC_SC);
+            double scv[7];
+#pragma unroll
+            for (int m = 0; m < 7; m++) scv[m] = cur[(RC_SC + m) * TW];
+            sc_get<KE>(sc, scv);
+            double bv[27], xv[9];
+#pragma unroll
+            for (int m = 0; m < 27; m++) bv[m] = cur[(RC_B + m) * TW];
+#pragma unroll
+            for (int m = 0; m < 9; m++) xv[m] = cur[(RC_X + m) * TW];
             Coef ci, cj, ck;
-            ci.x1 = cur[RC_X]; ci.x2 = cur[RC_X + 1]; ci.x5 = cur[RC_X + 2];
-            cj.x1 = cur[RC_X + 3]; cj.x2 = cur[RC_X + 4]; cj.x5 = cur[RC_X + 5];
-            ck.x1 = cur[RC_X + 6]; ck.x2 = cur[RC_X + 7]; ck.x5 = cur[RC_X + 8];
+            ci.x1 = xv[0]; ci.x2 = xv[1]; ci.x5 = xv[2];
+            cj.x1 = xv[3]; cj.x2 = xv[4]; cj.x5 = xv[5];
+            ck.x1 = xv[6]; ck.x2 = xv[7]; ck.x5 = xv[8];
             double oj[NV], ok[NV];
-            face_xfer<KE>(dn, sc, basis_get(cur + RC_B), ci, yi);
-            face_xfer<KE>(dn, sc, basis_get(cur + RC_B + 9), cj, oj);
-            face_xfer<KE>(dn, sc, basis_get(cur + RC_B + 18), ck, ok);
+            face_xfer<KE>(dn, sc, basis_get(bv), ci, yi);
+            face_xfer<KE>(dn, sc, basis_get(bv + 9), cj, oj);
+            face_xfer<KE>(dn, sc, basis_get(bv + 18), ck, ok);
             const int wb = s & 1;
             if (jj < TJ - 1) {
 #pragma unroll
@@ -2634,12 +2717,9 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
                 halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
             }
         }
-        if (s + 1 < nsteps) {
-#pragma unroll
-            for (int f = 0; f < NRF; f++) cur[f] = nxt[f];
-        }
         wave_step_sync();
     }
+    vm_wait<0>();
 }
 
 // ------------------------------------------------------------------------------------
@@ -2685,39 +2765,6 @@ __device__ __forceinline__ int tile_ticket_k(const SweepK &s)
 struct HaloIn {
     unsigned long long i[HS], j[HS];
 };
-
-// Wait until at most n vector-memory operations of this wave are outstanding (they
-// complete in issue order).  The LDS rings below are filled by LDS-DMA loads, which the
-// compiler does not order against the LDS reads of the rings; the memory clobber keeps
-// those reads after the wait.
-template <int N>
-__device__ __forceinline__ void vm_wait()
-{
-    static_assert(N >= 0 && N <= 63, "vmcnt range");
-    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
-}
-
-// One LDS-DMA load: every lane fetches the 4 bytes at its `src` into LDS byte address
-// lds + 4*lane (M0 holds the wave's base).  Issued from inline assembly so that the
-// compiler, which cannot tell which LDS bytes such a load writes, does not follow it with
-// a full vmcnt wait before every LDS read; the kernels wait for these loads themselves
-// (vm_wait).  SC1 selects the device-coherent form used for the halo slots.  The s_nop is
-// the wait state an LDS-DMA load needs after an SALU write of M0.
-template <bool SC1>
-__device__ __forceinline__ void lds_dma4(const void *src, unsigned lds)
-{
-    if (SC1)
-        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off sc1"
-                     ::"v"(src), "s"(lds) : "memory", "m0");
-    else
-        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
-                     ::"v"(src), "s"(lds) : "memory", "m0");
-}
-
-__device__ __forceinline__ unsigned lds_addr(const void *p)
-{
-    return (unsigned)(size_t)(const __attribute__((address_space(3))) void *)p;
-}
 
 // Halo words of one face taken from the LDS copy an LDS-DMA load made of the slot (two
 // 32-bit halves per word, hb[2q][lane] and hb[2q+1][lane]).  The halves are separate
@@ -3906,7 +3953,7 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
             SweepSt &st = r->st;
             st.nstep = b.imax + TJ + TK - 2;
             const size_t nt = (size_t)r->sw.ntj * r->sw.ntk;
-            const size_t nrec = nt * st.nstep * NRF * TW, ndn = nt * st.nstep * 7 * TW;
+            const size_t nrec = nt * st.nstep * NRF * TW, ndn = nt * st.nstep * NDN * TW;
             if (hipMalloc(&r->st_mem, sizeof(double) * (2 * nrec + ndn)) != hipSuccess)
                 return fail(c, FF_E_NOMEM, "LU-SGS staged records%s");
             st.recp = r->st_mem;
