@@ -302,6 +302,14 @@ def main():
                    for k in ("k_pred_plane", "k_corr_plane", "k_pred_tiles", "k_corr_tiles",
                              "k_pred_km", "k_corr_km", "k_sweep_prep", "k_pred_st",
                              "k_corr_st")) / args.steps
+    # the whole LU-SGS (with the staged form: the record pass and both sweeps) against the
+    # bytes the two sweeps must move without any staging (the tile-form byte models)
+    corr = next((kstats[k] for k in ("k_corr_st", "k_corr_tiles", "k_corr_km", "k_corr_plane")
+                 if k in kstats), None)
+    lusgs_gbps = None
+    if corr and sweep_ms > 0:
+        lusgs_bytes = (models["k_pred_tiles"] + models["k_corr_tiles"]) * corr["cells_total"]
+        lusgs_gbps = lusgs_bytes / args.steps / (sweep_ms * 1e-3) / 1e9
     iter_bw = ITER_BYTES_PER_CELL[iturb] * ncell / (ms_per_step * 1e-3) / 1e9
 
     cpu = None
@@ -327,6 +335,7 @@ def main():
                                f"a step is one {'V-cycle' if nlev > 1 else 'iteration'}",
                    "cells": ncell, "levels": nlev,
                    "lusgs_sweep_ms_per_step": sweep_ms,
+                   "lusgs_compulsory_GBps": lusgs_gbps,
                    f"iteration_hbm_GBps_at_{ITER_BYTES_PER_CELL[iturb]}B_per_cell": iter_bw,
                    "density_resl2_after_warmup": float(np.sqrt(resi0[0]) / ncell),
                    "density_resl2": float(np.sqrt(resi[0]) / ncell),

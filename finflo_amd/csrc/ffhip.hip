@@ -1446,11 +1446,12 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev
     }
 }
 
-// Zero a field on every cell (ZEROZZ res3c.f:9330)
-__global__ void __launch_bounds__(256) k_zero(double *x, int n)
+// ZEROZZ (res3c.f:9330) of the nq residual fields DRO.. on every cell, one launch for all
+__global__ void __launch_bounds__(256) k_zero(Blk b, int nq)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) x[t] = 0.;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.ntot) return;
+    for (int q = 0; q < nq; q++) FLD(b, FF_DRO + q)[t] = 0.;
 }
 
 // RESL2 stage 1: per-workgroup partial sums of D^2 over interior cells (res3c.f:8979-9006);
@@ -1856,31 +1857,41 @@ __device__ __forceinline__ void halo_issue(unsigned long long *slot, unsigned lo
         h[q] = __hip_atomic_load(slot + q * QS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Complete a halo read begun by halo_issue: poll empty slots, then empty them.
+// Complete a halo read begun by halo_issue: poll empty slots, then empty them.  The words
+// are tested together first, so a slot that is already filled costs no polling branches.
 template <int NV, int QS = 1>
 __device__ __forceinline__ void halo_finish(unsigned long long *slot, unsigned long long h[NV],
                                             unsigned int *err, double v[NV])
 {
+    bool empty = false;
+#pragma unroll
+    for (int q = 0; q < NV; q++) empty |= h[q] == HALO_EMPTY;
+    if (empty) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            unsigned long long x = h[q];
+            int spins = 0;
+            while (x == HALO_EMPTY) {
+                if ((++spins & 255) == 0) {
+                    if (spins > SPIN_LIMIT) {
+                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        x = 0;
+                        break;
+                    }
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        x = 0;
+                        break;
+                    }
+                }
+                __builtin_amdgcn_s_sleep(1);
+                x = __hip_atomic_load(slot + q * QS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            h[q] = x;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-        unsigned long long x = h[q];
-        int spins = 0;
-        while (x == HALO_EMPTY) {
-            if ((++spins & 255) == 0) {
-                if (spins > SPIN_LIMIT) {
-                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    x = 0;
-                    break;
-                }
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    x = 0;
-                    break;
-                }
-            }
-            __builtin_amdgcn_s_sleep(1);
-            x = __hip_atomic_load(slot + q * QS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        v[q] = __longlong_as_double((long long)x);
+        v[q] = __longlong_as_double((long long)h[q]);
         __hip_atomic_store(slot + q * QS, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -2476,35 +2487,129 @@ __global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, S
 // reads the record of step s from its slot.  The copies are issued from inline assembly, so
 // the compiler does not wait for all of them before each LDS read; the sweep waits itself
 // until the copy of the slot it reads is complete (vm_wait: loads complete in issue order,
-// and the copies of the RS - 2 later steps may stay in flight).
+// and the copies of the RS - 2 later steps may stay in flight).  For the same reason a step
+// loads its halo words before it issues its copy: the wait for them then covers only the
+// copies of earlier steps, which have had a step or more to arrive.
 constexpr int RBYTES = NRF * TW * 8;            // one record: 28 KiB
 constexpr int DBYTES = NDN * TW * 8;            // one step of increments: 4 KiB
 
-__device__ __forceinline__ void lds_dma16(const char *src, unsigned lds)
+// four 1 KiB loads from one address pair: the instruction offset advances the global and the
+// LDS address alike
+__device__ __forceinline__ void lds_dma16x4(const char *src, unsigned lds)
 {
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\t"
+                 "global_load_lds_dwordx4 %0, off\n\t"
+                 "global_load_lds_dwordx4 %0, off offset:1024\n\t"
+                 "global_load_lds_dwordx4 %0, off offset:2048\n\t"
+                 "global_load_lds_dwordx4 %0, off offset:3072"
                  ::"v"(src), "s"(lds) : "memory", "m0");
 }
 
-// copy `bytes` (a multiple of 1 KiB) from global src to LDS dst, 16 bytes per lane per load
+// copy `bytes` (a multiple of 4 KiB) from global src to LDS dst, 16 bytes per lane per load
 template <int BYTES>
 __device__ __forceinline__ void ring_copy(const void *src, void *dst, int lane)
 {
+    static_assert(BYTES % 4096 == 0, "whole groups of four loads");
     const char *g = (const char *)src + lane * 16;
     const unsigned l = lds_addr(dst);
 #pragma unroll
-    for (int m = 0; m < BYTES / 1024; m++) lds_dma16(g + m * 1024, l + m * 1024);
+    for (int m = 0; m < BYTES / 4096; m++) lds_dma16x4(g + m * 4096, l + m * 4096);
 }
 
-// PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles
+// Halo words through LDS as well.  At a step, the edge lanes that read across one tile face
+// hold cells on one anti-diagonal d = I + offset - 1, so the step's words of that face are
+// one contiguous block of the slot layout (7 words x 8 lanes, 448 bytes).  The sweeps copy
+// the block of step s + 2 to an LDS ring at step s (lanes 0-27, 16 bytes each, device-coherent
+// loads), so the wait for it covers loads issued two steps earlier.  A word that was still
+// empty when copied (either 32-bit half equal to the marker's, so a copy racing the
+// producer's store is caught too) is polled again with 64-bit loads.
+constexpr int HBLK = HS * TK;                   // words per block (448 bytes)
+constexpr int HR = 3;                           // LDS halo slots: steps s, s + 1, s + 2
+
+__device__ __forceinline__ const unsigned long long *hj_blk(const Sweep &s, const Blk &b, int f,
+                                                             int t, int d)
+{
+    return s.hj + (((size_t)f * s.ntk + t) * (b.imax + TK) + d) * HBLK;
+}
+
+__device__ __forceinline__ const unsigned long long *hk_blk(const Sweep &s, const Blk &b, int f,
+                                                             int t, int d)
+{
+    return s.hk + (((size_t)f * s.ntj + t) * (b.imax + TJ) + d) * HBLK;
+}
+
+__device__ __forceinline__ void halo_dma(const unsigned long long *blk, double *dst, int lane)
+{
+    static_assert(HBLK * 8 % 16 == 0 && HBLK * 8 / 16 <= TW, "one 16-byte load per lane");
+    if (lane < HBLK * 8 / 16)
+        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off sc1"
+                     ::"v"((const char *)blk + lane * 16), "s"(lds_addr(dst)) : "memory", "m0");
+}
+
+__device__ __forceinline__ bool half_empty(unsigned long long x)
+{
+    return (unsigned)x == (unsigned)HALO_EMPTY || (unsigned)(x >> 32) == (unsigned)(HALO_EMPTY >> 32);
+}
+
+// The halo words of one lane from its LDS copy (word q at lw[q * TK]); words copied while
+// still empty are polled from the slot; then the slot is emptied for the next sweep.
+template <int NV>
+__device__ __forceinline__ void halo_take(const double *lw, unsigned long long *slot,
+                                          unsigned int *err, double v[NV])
+{
+    unsigned long long h[NV];
+    bool empty = false;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        h[q] = (unsigned long long)__double_as_longlong(lw[q * TK]);
+        if (half_empty(h[q])) {
+            h[q] = HALO_EMPTY;
+            empty = true;
+        }
+    }
+    if (empty) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            unsigned long long x = h[q];
+            int spins = 0;
+            while (x == HALO_EMPTY) {
+                if ((++spins & 255) == 0) {
+                    if (spins > SPIN_LIMIT) {
+                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        x = 0;
+                        break;
+                    }
+                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        x = 0;
+                        break;
+                    }
+                }
+                __builtin_amdgcn_s_sleep(1);
+                x = __hip_atomic_load(slot + q * HQS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            h[q] = x;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        v[q] = __longlong_as_double((long long)h[q]);
+        __hip_atomic_store(slot + q * HQS, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles.  Issue order
+// within step s: the step's stores, the halo blocks of step s + 2, the record of step s + 3.
+// At the top of step s at least the two later record copies (56 loads) follow the halo
+// blocks of step s, which follow the record of step s, so vmcnt(56) covers both.
 template <bool KE>
 __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
     constexpr int NV = KE ? 7 : 5;
     constexpr int RS = 4;                               // ring slots: 3 steps of lead
-    constexpr int NLD = RBYTES / 1024;                  // loads per step
+    constexpr int NLD = RBYTES / 1024;                  // record loads per step
     static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
     __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
+    __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
     __shared__ double xj[2][NV][TW], xk[2][NV][TW];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
     const int t = tile_ticket(sw);
@@ -2512,47 +2617,53 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
     const int tile = tk * sw.ntj + tj;
     const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
     const bool col = j <= b.jmax && k <= b.kmax;
+    const bool hasj = tj + 1 < sw.ntj, hask = tk + 1 < sw.ntk;   // upstream tiles
     const int lag = (TJ - 1 - jj) + (TK - 1 - kk);
     const int nsteps = st.nstep;
+    // the edge lanes of step s read anti-diagonal d = IMAX + 6 - s of the upstream faces
+    auto halo_copy = [&](int s) {
+        const int d = b.imax + TK - 2 - s;
+        if (d < 0 || d > b.imax + TK - 2) return;
+        if (hasj) halo_dma(hj_blk(sw, b, tj + 1, tk, d), hring[s % HR][0], lane);
+        if (hask) halo_dma(hk_blk(sw, b, tk + 1, tj, d), hring[s % HR][1], lane);
+    };
+    auto rec_copy = [&](int s) {
+        ring_copy<RBYTES>(st.recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
+    };
     double yi[NV];
 #pragma unroll
     for (int r = 0; r < NV; r++) yi[r] = 0.;
-    // predictor step s reads record row nsteps - 1 - s
-    for (int s = 0; s < RS - 1 && s < nsteps; s++)
-        ring_copy<RBYTES>(st.recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
+    rec_copy(0);
+    halo_copy(0);
+    if (1 < nsteps) rec_copy(1);
+    halo_copy(1);
+    if (2 < nsteps) rec_copy(2);
     for (int s = 0; s < nsteps; s++) {
         const int row = nsteps - 1 - s;
-        // the copy of step s is complete once at most the RS - 2 later copies are in flight
         if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
-        if (s + RS - 1 < nsteps)
-            ring_copy<RBYTES>(st.recp + st_row(st, tile, row - (RS - 1), 0), ring[(s + RS - 1) % RS], lane);
         const double *cur = ring[s % RS] + lane;
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = b.imax - q;
-            const bool hjin = jj == TJ - 1 && j < b.jmax, hkin = kk == TK - 1 && k < b.kmax;
-            unsigned long long hv[NV], hw[NV];
-            if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), hv);
-            if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), hw);
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
             if (j == b.jmax) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) bj[r] = 0.;
-            } else if (!hjin) {
+            } else if (jj < TJ - 1) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + 1];
             } else {
-                halo_finish<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), hv, sw.err, bj);
+                halo_take<NV>(hring[s % HR][0] + kk, hj_slot(sw, b, tj + 1, i, k), sw.err, bj);
             }
             if (k == b.kmax) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) ckc[r] = 0.;
-            } else if (!hkin) {
+            } else if (kk < TK - 1) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane + TJ];
             } else {
-                halo_finish<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), hw, sw.err, ckc);
+                halo_take<NV>(hring[s % HR][1] + jj, hk_slot(sw, b, tk + 1, i, j), sw.err, ckc);
             }
             // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
             // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
@@ -2602,22 +2713,29 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
                 halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), ok);
             }
         }
+        // the LDS reads of this step are complete before its slots are refilled
         wave_step_sync();
+        halo_copy(s + 2);
+        if (s + RS - 1 < nsteps) rec_copy(s + RS - 1);
     }
     vm_wait<0>();
 }
 
 // DIAGGS + CORRGS / DIKEGS + COKEGS on staged records, the wavefront of k_corr_tiles, with
-// the end of IMPS (DRO/CP, D(rho k) and D(rho omega) times RO)
+// the end of IMPS (DRO/CP, D(rho k) and D(rho omega) times RO).  Issue order within step s:
+// the step's stores, the halo blocks of step s + 2, the record and increments of step s + 2;
+// at the top of step s the copy of step s + 1 (32 loads) follows that of step s, which
+// follows the halo blocks of step s, so vmcnt(32) covers both.
 template <bool KE>
 __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
     constexpr int NV = KE ? 7 : 5;
     constexpr int RS = 3;                               // ring slots: 2 steps of lead
-    constexpr int NLD = (RBYTES + DBYTES) / 1024;       // loads per step
+    constexpr int NLD = (RBYTES + DBYTES) / 1024;       // record loads per step
     static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
     __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
     __shared__ __attribute__((aligned(16))) double dring[RS][NDN * TW];
+    __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
     __shared__ double xj[2][NV][TW], xk[2][NV][TW];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
     const int t = tile_ticket(sw);
@@ -2625,52 +2743,57 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
     const int tile = tk * sw.ntj + tj;
     const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
     const bool col = j <= b.jmax && k <= b.kmax;
+    const bool hasj = tj > 0, hask = tk > 0;                  // upstream tiles
     const int lag = jj + kk;
     const int nsteps = st.nstep;
-    double yi[NV];
-#pragma unroll
-    for (int r = 0; r < NV; r++) yi[r] = 0.;
-    auto copy_step = [&](int s) {
+    // the edge lanes of step s read anti-diagonal d = s of the upstream faces
+    auto halo_copy = [&](int s) {
+        if (s > b.imax + TK - 2) return;
+        if (hasj) halo_dma(hj_blk(sw, b, tj, tk, s), hring[s % HR][0], lane);
+        if (hask) halo_dma(hk_blk(sw, b, tk, tj, s), hring[s % HR][1], lane);
+    };
+    auto rec_copy = [&](int s) {
         ring_copy<RBYTES>(st.recc + st_row(st, tile, s, 0), ring[s % RS], lane);
         ring_copy<DBYTES>(st.dn + (((size_t)tile * nsteps + s) * NDN) * TW, dring[s % RS], lane);
     };
-    for (int s = 0; s < RS - 1 && s < nsteps; s++) copy_step(s);
+    double yi[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) yi[r] = 0.;
+    halo_copy(0);
+    rec_copy(0);
+    halo_copy(1);
+    if (1 < nsteps) rec_copy(1);
     for (int s = 0; s < nsteps; s++) {
         if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
-        if (s + RS - 1 < nsteps) copy_step(s + RS - 1);
         const double *cur = ring[s % RS] + lane;
         const double *dni = dring[s % RS] + lane;
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = q + 1;
             const int n = cid(b, i, j, k);
-            const bool hjin = jj == 0 && j > 1, hkin = kk == 0 && k > 1;
-            unsigned long long hv[NV], hw[NV];
-            if (hjin) halo_issue<NV, HQS>(hj_slot(sw, b, tj, i, k), hv);
-            if (hkin) halo_issue<NV, HQS>(hk_slot(sw, b, tk, i, j), hw);
-            double d[NV];
-#pragma unroll
-            for (int r = 0; r < NV; r++) d[r] = dni[r * TW] * cur[RC_FAC * TW];
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
             if (j == 1) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) bj[r] = 0.;
-            } else if (!hjin) {
+            } else if (jj > 0) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - 1];
             } else {
-                halo_finish<NV, HQS>(hj_slot(sw, b, tj, i, k), hv, sw.err, bj);
+                halo_take<NV>(hring[s % HR][0] + kk, hj_slot(sw, b, tj, i, k), sw.err, bj);
             }
             if (k == 1) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) ckc[r] = 0.;
-            } else if (!hkin) {
+            } else if (kk > 0) {
 #pragma unroll
                 for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane - TJ];
             } else {
-                halo_finish<NV, HQS>(hk_slot(sw, b, tk, i, j), hw, sw.err, ckc);
+                halo_take<NV>(hring[s % HR][1] + jj, hk_slot(sw, b, tk, i, j), sw.err, ckc);
             }
+            double d[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) d[r] = dni[r * TW] * cur[RC_FAC * TW];
             double dn[NV];
 #pragma unroll
             for (int r = 0; r < NV; r++) {
@@ -2717,7 +2840,10 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
                 halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
             }
         }
+        // the LDS reads of this step are complete before its slots are refilled
         wave_step_sync();
+        halo_copy(s + 2);
+        if (s + RS - 1 < nsteps) rec_copy(s + RS - 1);
     }
     vm_wait<0>();
 }
@@ -3863,10 +3989,7 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
     };
     // ZEROZZ of DRO..DE (DRK, DEPS) on every cell (res3c.f:194-198, 3668-3674): ghost
     // residuals stay zero
-    for (int q = 0; q < nq; q++) {
-        rc = launch(c, "k_zero", k_zero, all_grid(b), dim3(256), b.f[FF_DRO + q], b.ntot);
-        if (rc) return rc;
-    }
+    if ((rc = launch(c, "k_zero", k_zero, all_grid(b), dim3(256), b, nq))) return rc;
     const bool vl = c->prm.iflux == 0;
     if (kw) {
         if (c->residual_mode != FF_RESIDUAL_TILES)

@@ -178,7 +178,8 @@ const char *ff_last_error(const ff_ctx *ctx);
 #define FF_LUSGS_KMARCH  2
 /*   FF_LUSGS_STAGED: the (J,K) tiles of FF_LUSGS_TILES on cell records that a parallel pass
  *   forms beforehand in the sweeps' step order (coalesced, the recurrence-independent work
- *   off the wavefront); needs 2 x 56 + 7 doubles per cell of device memory per level. */
+ *   off the wavefront); needs 2 x 56 + 8 doubles per cell and step of device memory per
+ *   level (about 1 KB per cell), allocated on the block's first implicit stage. */
 #define FF_LUSGS_STAGED  3
 /*   FF_OPT_RESIDUAL: FF_RESIDUAL_TILES (default) forms every face flux once in (I,J) tiles
  *   marching along K; FF_RESIDUAL_CELLS forms both faces of every direction per cell.
