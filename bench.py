@@ -90,7 +90,7 @@ KERNELS = ("k_residual_kw", "k_residual_tile", "k_residual", "k_pred_tiles", "k_
            "k_residual_kw_ns", "k_residual_tile_ns",
            "k_pred_km", "k_corr_km", "k_pred_plane", "k_corr_plane", "k_time3_copr", "k_state", "k_turb_lam",
            "k_turb_sst", "k_grad_zero", "k_derfun", "k_extend", "k_crosko", "k_kw_round",
-           "k_max_final", "k_fun1_prod", "k_limits", "k_limits_mg", "k_fin", "k_zero",
+           "k_max_final", "k_fun1_prod", "k_limits", "k_limits_mg", "k_fin", "k_zero", "k_fill_slots",
            "k_save_old", "k_divcp", "k_resl2_part", "k_resl2_final", "k_restrict",
            "k_restrict_ghosts", "k_forcing", "k_prolong")
 

@@ -2552,7 +2552,9 @@ __device__ __forceinline__ bool half_empty(unsigned long long x)
 }
 
 // The halo words of one lane from its LDS copy (word q at lw[q * TK]); words copied while
-// still empty are polled from the slot; then the slot is emptied for the next sweep.
+// still empty are polled from the slot.  The staged sweeps leave their slots filled: a fill
+// kernel empties all of them after the sweep, which costs less than a device-coherent
+// store per word on the wavefront's critical path.
 template <int NV>
 __device__ __forceinline__ void halo_take(const double *lw, unsigned long long *slot,
                                           unsigned int *err, double v[NV])
@@ -2591,10 +2593,7 @@ __device__ __forceinline__ void halo_take(const double *lw, unsigned long long *
         }
     }
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        v[q] = __longlong_as_double((long long)h[q]);
-        __hip_atomic_store(slot + q * HQS, HALO_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    for (int q = 0; q < NV; q++) v[q] = __longlong_as_double((long long)h[q]);
 }
 
 // PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles.  Issue order
@@ -3419,6 +3418,7 @@ struct BlockRec {
     SweepK swk{};              // the same for the K-marching (I, J) tiles (shares the slots)
     void *sw_mem = nullptr;
     size_t nhalo = 0;          // halo slot words
+    size_t ntile_slots = 0;    // words of the (J,K)-tile forms' slots (J faces, then K faces)
     double *kw_scr = nullptr;  // k-omega: per-workgroup maxima of SEPS and their maximum
     int kw_parts = 0;
     int level = 0;             // multigrid level M the slab was registered as
@@ -3581,6 +3581,7 @@ static int sweep_alloc(ff_ctx *c, BlockRec &r)
     const size_t nhi2 = (size_t)HS * (sk.nti + 1) * b.kmax * b.jmax;
     const size_t nhj2 = (size_t)HS * (sk.ntj + 1) * b.kmax * b.imax;
     r.nhalo = std::max(nhj + nhk, nhi2 + nhj2);
+    r.ntile_slots = nhj + nhk;
     // ticket, error word, ticket of the K-marching form, pad to 64 bytes (the slot runs of the
     // tile form are 64-byte aligned)
     const size_t head = 8;
@@ -4090,11 +4091,16 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(4 * TW), kp, b, r->sw, r->st, tp)
                 : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(4 * TW), kp, b, r->sw, r->st, tp);
         if (rc) return rc;
+        // each sweep leaves its halo slots filled; empty them for the next one
+        const dim3 gf((unsigned)((r->ntile_slots + 255) / 256));
         rc = ke ? launch(c, "k_pred_st", k_pred_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
                 : launch(c, "k_pred_st", k_pred_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
         if (rc) return rc;
-        return ke ? launch(c, "k_corr_st", k_corr_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
-                  : launch(c, "k_corr_st", k_corr_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
+        if ((rc = launch(c, "k_fill_slots", k_fill_u64, gf, dim3(256), r->sw.hj, r->ntile_slots, HALO_EMPTY))) return rc;
+        rc = ke ? launch(c, "k_corr_st", k_corr_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
+                : launch(c, "k_corr_st", k_corr_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
+        if (rc) return rc;
+        return launch(c, "k_fill_slots", k_fill_u64, gf, dim3(256), r->sw.hj, r->ntile_slots, HALO_EMPTY);
     }
     if (c->lusgs_mode == FF_LUSGS_TILES) {
         // DRO/CP is applied inside the forward sweep
