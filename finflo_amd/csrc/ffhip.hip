@@ -169,11 +169,17 @@ __device__ __forceinline__ double derfun_cell(const Blk &b, int L)
 // The curvature correction is off (PA0 = 0, earsm.f:47; its WIR term is 0*0), and the
 // routine's EPS2/VIST update is dead code (earsm.f:222), so only BIJ is written.  The
 // association of every sum and product is the routine's; x**4 is a left-to-right product,
-// x**P3 and x**P6 are pow.
-__device__ __forceinline__ void earsm_cell(const Blk &b, int L)
+// x**P3 and x**P6 are pow.  Three parts: the invariants and tensor terms (earsm.f:72-167),
+// the cubic solve, N correction and betas (earsm.f:170-205), and the b_ij sums
+// (earsm.f:209-219); the learned closure replaces the middle part only.
+struct EarsmT {
+    double sii, wii, swwiv, sswwv, iiis;
+    double t[4][5];               // terms of beta3, beta4, beta6, beta9: 11, 22, 12, 13, 23
+};
+
+__device__ __forceinline__ void earsm_terms(const Blk &b, int L, EarsmT &e)
 {
-    const double p3 = 1.0 / 3.0, p6 = 0.5 * p3, tt = 2.0 * p3;
-    const double c1 = 1.8, c1p = 9.0 * (c1 - 1.0) / 4.0, c1p3 = c1p * p3, c1psq = c1p * c1p;
+    const double p3 = 1.0 / 3.0, tt = 2.0 * p3;
     const double pa0 = 0., wir = 0.;
     const double ts = FLD(b, FF_TTS)[L];
     double s11 = ts * FLD(b, FF_S11)[L], s22 = ts * FLD(b, FF_S22)[L], s33 = ts * FLD(b, FF_S33)[L];
@@ -196,41 +202,68 @@ __device__ __forceinline__ void earsm_cell(const Blk &b, int L)
                  - (s11sq + s12sq + s13sq) * (w12sq + w13sq)
                  - (s12sq + s22sq + s23sq) * (w12sq + w23sq)
                  - (s13sq + s23sq + s33sq) * (w13sq + w23sq);
-    double t3c11 = -w12sq - w13sq - wiip3;
-    double t3c22 = -w12sq - w23sq - wiip3;
-    double t3c12 = -(w13 * w23);
-    double t3c13 = w12 * w23;
-    double t3c23 = -(w12 * w13);
-    double t4c11 = -(2.0 * (s12 * w12 + s13 * w13));
-    double t4c22 = 2.0 * (s12 * w12 - s23 * w23);
-    double t4c12 = (s11 - s22) * w12 - s23 * w13 - s13 * w23;
-    double t4c13 = -(s23 * w12) + (s11 - s33) * w13 + s12 * w23;
-    double t4c23 = s13 * w12 + s12 * w13 + (s22 - s33) * w23;
-    double t6c11 = -(2.0 * ((s12 * w13 - s13 * w12) * w23 + s11 * (w12sq + w13sq)))
-                 - swwivtt - wii * s11;
-    double t6c22 = -(2.0 * ((s23 * w12 + s12 * w23) * w13 + s22 * (w12sq + w23sq)))
-                 - swwivtt - wii * s22;
-    double t6c12 = -(s12 * (2.0 * w12sq + w13sq + w23sq))
-                 - (s13 * w13 - s23 * w23) * w12 - (s11 + s22) * w13 * w23 - wii * s12;
-    double t6c13 = -(s13 * (w12sq + 2.0 * w13sq + w23sq))
-                 - (s12 * w12 + s23 * w23) * w13 + (s11 + s33) * w12 * w23 - wii * s13;
-    double t6c23 = -(s23 * (w12sq + w13sq + 2.0 * w23sq))
-                 + (s12 * w12 - s13 * w13) * w23 - (s22 + s33) * w12 * w13 - wii * s23;
-    double t9c11 = -(2.0 * ((s12 * w12 + s13 * w13 - s23 * w23) * w12sq
-                            + (s12 * w12 + s13 * w13 + s23 * w23) * w13sq
-                            + (s22 - s33) * w12 * w13 * w23));
-    double t9c22 = -(2.0 * ((-(s12 * w12) - s13 * w13 + s23 * w23) * w12sq
-                            + (-(s12 * w12) + s13 * w13 + s23 * w23) * w23sq
-                            + (-s11 + s33) * w12 * w13 * w23));
-    double t9c12 = ((s11 - s22) * w12 - 2.0 * (s13 * w23 + s23 * w13)) * w12sq
-                 + ((s11 - s33) * w12 - 2.0 * s13 * w23) * w13sq
-                 + ((s33 - s22) * w12 - 2.0 * s23 * w13) * w23sq;
-    double t9c13 = ((s11 - s22) * w13 + 2.0 * s12 * w23) * w12sq
-                 + ((s11 - s33) * w13 + 2.0 * (s12 * w23 - s23 * w12)) * w13sq
-                 + ((s22 - s33) * w13 - 2.0 * s23 * w12) * w23sq;
-    double t9c23 = ((s22 - s11) * w23 + 2.0 * s12 * w13) * w12sq
-                 + ((s11 - s33) * w23 + 2.0 * s13 * w12) * w13sq
-                 + ((s22 - s33) * w23 + 2.0 * (s12 * w13 + s13 * w12)) * w23sq;
+    e.sii = sii; e.wii = wii; e.swwiv = swwiv; e.sswwv = sswwv;
+    // tr(S^3), an input of the learned closure only
+    e.iiis = s11 * s11sq + s22 * s22sq + s33 * s33sq
+           + 3.0 * (s11 * (s12sq + s13sq) + s22 * (s12sq + s23sq) + s33 * (s13sq + s23sq))
+           + 6.0 * s12 * s13 * s23;
+    e.t[0][0] = -w12sq - w13sq - wiip3;
+    e.t[0][1] = -w12sq - w23sq - wiip3;
+    e.t[0][2] = -(w13 * w23);
+    e.t[0][3] = w12 * w23;
+    e.t[0][4] = -(w12 * w13);
+    e.t[1][0] = -(2.0 * (s12 * w12 + s13 * w13));
+    e.t[1][1] = 2.0 * (s12 * w12 - s23 * w23);
+    e.t[1][2] = (s11 - s22) * w12 - s23 * w13 - s13 * w23;
+    e.t[1][3] = -(s23 * w12) + (s11 - s33) * w13 + s12 * w23;
+    e.t[1][4] = s13 * w12 + s12 * w13 + (s22 - s33) * w23;
+    e.t[2][0] = -(2.0 * ((s12 * w13 - s13 * w12) * w23 + s11 * (w12sq + w13sq)))
+              - swwivtt - wii * s11;
+    e.t[2][1] = -(2.0 * ((s23 * w12 + s12 * w23) * w13 + s22 * (w12sq + w23sq)))
+              - swwivtt - wii * s22;
+    e.t[2][2] = -(s12 * (2.0 * w12sq + w13sq + w23sq))
+              - (s13 * w13 - s23 * w23) * w12 - (s11 + s22) * w13 * w23 - wii * s12;
+    e.t[2][3] = -(s13 * (w12sq + 2.0 * w13sq + w23sq))
+              - (s12 * w12 + s23 * w23) * w13 + (s11 + s33) * w12 * w23 - wii * s13;
+    e.t[2][4] = -(s23 * (w12sq + w13sq + 2.0 * w23sq))
+              + (s12 * w12 - s13 * w13) * w23 - (s22 + s33) * w12 * w13 - wii * s23;
+    e.t[3][0] = -(2.0 * ((s12 * w12 + s13 * w13 - s23 * w23) * w12sq
+                         + (s12 * w12 + s13 * w13 + s23 * w23) * w13sq
+                         + (s22 - s33) * w12 * w13 * w23));
+    e.t[3][1] = -(2.0 * ((-(s12 * w12) - s13 * w13 + s23 * w23) * w12sq
+                         + (-(s12 * w12) + s13 * w13 + s23 * w23) * w23sq
+                         + (-s11 + s33) * w12 * w13 * w23));
+    e.t[3][2] = ((s11 - s22) * w12 - 2.0 * (s13 * w23 + s23 * w13)) * w12sq
+              + ((s11 - s33) * w12 - 2.0 * s13 * w23) * w13sq
+              + ((s33 - s22) * w12 - 2.0 * s23 * w13) * w23sq;
+    e.t[3][3] = ((s11 - s22) * w13 + 2.0 * s12 * w23) * w12sq
+              + ((s11 - s33) * w13 + 2.0 * (s12 * w23 - s23 * w12)) * w13sq
+              + ((s22 - s33) * w13 - 2.0 * s23 * w12) * w23sq;
+    e.t[3][4] = ((s22 - s11) * w23 + 2.0 * s12 * w13) * w12sq
+              + ((s11 - s33) * w23 + 2.0 * s13 * w12) * w13sq
+              + ((s22 - s33) * w23 + 2.0 * (s12 * w13 + s13 * w12)) * w23sq;
+}
+
+// beta3, beta4, beta6, beta9 from N (earsm.f:196-205)
+__device__ __forceinline__ void earsm_betas_n(const EarsmT &e, double fn, double beta[4])
+{
+    const double p6 = 0.5 * (1.0 / 3.0);
+    const double wii = e.wii, swwiv = e.swwiv;
+    double fnsq = fn * fn;
+    double q = 5.0 * p6 * (fnsq - 2.0 * wii) * (2.0 * fnsq - wii);
+    double pq = 1.0 / q;
+    beta[0] = -(pq * 12.0 * swwiv / fn);
+    beta[1] = -(pq * 2.0 * (fnsq - 2.0 * wii));
+    beta[2] = -(pq * 6.0 * fn);
+    beta[3] = pq * 6.0;
+}
+
+// the exact coefficients beta3, beta4, beta6, beta9 (earsm.f:170-205)
+__device__ __forceinline__ void earsm_betas(const EarsmT &e, double beta[4])
+{
+    const double p3 = 1.0 / 3.0, p6 = 0.5 * p3, tt = 2.0 * p3;
+    const double c1 = 1.8, c1p = 9.0 * (c1 - 1.0) / 4.0, c1p3 = c1p * p3, c1psq = c1p * c1p;
+    const double sii = e.sii, wii = e.wii, swwiv = e.swwiv, sswwv = e.sswwv;
     // N_c from the cubic (earsm.f:170-184)
     double p1 = (c1psq / 27.0 + 0.45 * sii - tt * wii) * c1p;
     double cb = c1psq * (p3 * p3) + 0.9 * sii + tt * wii;
@@ -257,21 +290,30 @@ __device__ __forceinline__ void earsm_cell(const Blk &b, int L)
     double fii1 = swwiv * swwiv;
     double fii2 = sswwv - 0.5 * sii * wii;
     double fn = fnc + 162.0 * (fii1 + fii2 * (fnc * fnc)) / dd;
-    double fnsq = fn * fn;
-    double q = 5.0 * p6 * (fnsq - 2.0 * wii) * (2.0 * fnsq - wii);
-    double pq = 1.0 / q;
-    double beta3 = -(pq * 12.0 * swwiv / fn);
-    double beta4 = -(pq * 2.0 * (fnsq - 2.0 * wii));
-    double beta6 = -(pq * 6.0 * fn);
-    double beta9 = pq * 6.0;
-    double b1 = 0.5 * (beta3 * t3c11 + beta4 * t4c11 + beta6 * t6c11 + beta9 * t9c11);
-    double b4 = 0.5 * (beta3 * t3c22 + beta4 * t4c22 + beta6 * t6c22 + beta9 * t9c22);
+    earsm_betas_n(e, fn, beta);
+}
+
+// b_ij from the betas and the tensor terms (earsm.f:209-219)
+__device__ __forceinline__ void earsm_store(const Blk &b, int L, const EarsmT &e, const double beta[4])
+{
+    const double beta3 = beta[0], beta4 = beta[1], beta6 = beta[2], beta9 = beta[3];
+    double b1 = 0.5 * (beta3 * e.t[0][0] + beta4 * e.t[1][0] + beta6 * e.t[2][0] + beta9 * e.t[3][0]);
+    double b4 = 0.5 * (beta3 * e.t[0][1] + beta4 * e.t[1][1] + beta6 * e.t[2][1] + beta9 * e.t[3][1]);
     FLD(b, FF_B11)[L] = b1;
-    FLD(b, FF_B12)[L] = 0.5 * (beta3 * t3c12 + beta4 * t4c12 + beta6 * t6c12 + beta9 * t9c12);
-    FLD(b, FF_B13)[L] = 0.5 * (beta3 * t3c13 + beta4 * t4c13 + beta6 * t6c13 + beta9 * t9c13);
+    FLD(b, FF_B12)[L] = 0.5 * (beta3 * e.t[0][2] + beta4 * e.t[1][2] + beta6 * e.t[2][2] + beta9 * e.t[3][2]);
+    FLD(b, FF_B13)[L] = 0.5 * (beta3 * e.t[0][3] + beta4 * e.t[1][3] + beta6 * e.t[2][3] + beta9 * e.t[3][3]);
     FLD(b, FF_B22)[L] = b4;
-    FLD(b, FF_B23)[L] = 0.5 * (beta3 * t3c23 + beta4 * t4c23 + beta6 * t6c23 + beta9 * t9c23);
+    FLD(b, FF_B23)[L] = 0.5 * (beta3 * e.t[0][4] + beta4 * e.t[1][4] + beta6 * e.t[2][4] + beta9 * e.t[3][4]);
     FLD(b, FF_B33)[L] = -b1 - b4;
+}
+
+__device__ __forceinline__ void earsm_cell(const Blk &b, int L)
+{
+    EarsmT e;
+    earsm_terms(b, L, e);
+    double beta[4];
+    earsm_betas(e, beta);
+    earsm_store(b, L, e, beta);
 }
 
 // TURBEQ (res3c.f:5951-6371) on the cells of level 1 plus one ghost layer: DERFUN
@@ -297,7 +339,8 @@ __global__ void __launch_bounds__(256) k_turb_sst(Prm p, Blk b)
         FLD(b, FF_TTS)[L] = dmax(tstur, tsvis);
     }
     if (i < 1 || i > b.imax || j < 1 || j > b.jmax || k < 1 || k > b.kmax) return;
-    if (p.istres == 1) earsm_cell(b, L);   // EARSM1 after TTIMES (res3c.f:5987-5990)
+    // EARSM1 after TTIMES (res3c.f:5987-5990); with the learned closure a later pass forms BIJ
+    if (p.istres == 1 && !p.earsm_mlp) earsm_cell(b, L);
     // EPS2KO
     const SstC s = coefko(p.kover);
     const double eps = 1.E-20;
@@ -3433,6 +3476,18 @@ struct PendingEv {
     hipEvent_t a, b;
 };
 
+// the learned EARSM closure of a context (kernels in ffmlp.hip)
+struct MlpRec {
+    float *p = nullptr, *m = nullptr, *v = nullptr;    // parameters and Adam moments
+    float *nrm = nullptr;                               // normalisation block
+    float *part = nullptr;                              // per-wave gradient partials
+    unsigned *idx = nullptr;                            // minibatch indices
+    float *grad = nullptr;                              // reduced gradient and loss
+    int part_waves = 0, idx_cap = 0;
+    int step = 0;                                       // Adam steps taken
+    bool ready = false;                                 // parameters and normalisation set
+};
+
 struct ff_ctx {
     ff_params prm{};
     Prm kp{};
@@ -3452,7 +3507,12 @@ struct ff_ctx {
     std::map<std::string, std::pair<long long, double>> stats;
     int lev = 0;                  // level of the stage being launched (statistics only)
     hipStream_t wstream = nullptr;   // side stream of the wall force sums (default priority)
+    int earsm_mode = FF_EARSM_EXACT;
+    MlpRec mlp;
 };
+
+static void mlp_free(MlpRec &r);
+static int earsm_mlp_pass(ff_ctx *c, const Blk &b);
 
 static long long key(int level, int block) { return ((long long)level << 32) | (unsigned)block; }
 
@@ -3746,6 +3806,7 @@ int ff_ctx_destroy(ff_ctx *c)
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     if (c->wstream) (void)hipStreamDestroy(c->wstream);
+    mlp_free(c->mlp);
     delete c;
     return FF_OK;
 }
@@ -3772,6 +3833,12 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
             value != FF_LUSGS_STAGED)
             return fail(c, FF_E_ARG, "FF_OPT_LUSGS: unknown value%s");
         c->lusgs_mode = value;
+        return FF_OK;
+    }
+    if (option == FF_OPT_EARSM) {
+        if (value != FF_EARSM_EXACT && value != FF_EARSM_MLP)
+            return fail(c, FF_E_ARG, "FF_OPT_EARSM: unknown value%s");
+        c->earsm_mode = value;
         return FF_OK;
     }
     if (option == FF_OPT_RESIDUAL) {
@@ -3921,7 +3988,12 @@ int ff_turb(ff_ctx *c, int level, int block)
     if (c->prm.iturb == 6) {
         const Blk &b = r->dev;
         dim3 g((b.imax + 2 + 255) / 256, b.jmax + 2, b.kmax + 2);
-        return launch(c, "k_turb_sst", k_turb_sst, g, dim3(256), make_prm(&c->prm, level), b);
+        Prm kp = make_prm(&c->prm, level);
+        kp.earsm_mlp = c->prm.istres == 1 && c->earsm_mode == FF_EARSM_MLP;
+        if (kp.earsm_mlp && !c->mlp.ready) return fail(c, FF_E_STATE, "FF_EARSM_MLP: learned closure not set%s");
+        int rc = launch(c, "k_turb_sst", k_turb_sst, g, dim3(256), kp, b);
+        if (rc || !kp.earsm_mlp) return rc;
+        return earsm_mlp_pass(c, b);
     }
     return launch(c, "k_turb_lam", k_turb_lam, cells_grid(r->dev, 256), dim3(256), r->dev);
 }
@@ -4492,3 +4564,5 @@ int ff_wall_forces(ff_ctx *c, int block, int patch, double *sums)
     return FF_OK;
 }
 }  // extern "C"
+
+#include "ffmlp.hip"

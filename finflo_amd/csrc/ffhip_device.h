@@ -31,6 +31,7 @@ struct Prm {
     int fullns;     // full viscous fluxes (FULLNS) on this level: level 1 only
     int idiff;
     int turcor;     // BOTSKE's near-wall corrections (the SST path)
+    int earsm_mlp;  // EARSM1's coefficients from the learned closure (a separate pass)
 };
 
 #define FLD(b, n) ((b).f[n])

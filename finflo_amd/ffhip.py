@@ -22,10 +22,14 @@ EXPORTS = [
     "ff_block_register_device", "ff_sync", "ff_finish", "ff_state", "ff_turb", "ff_residual",
     "ff_implicit", "ff_update", "ff_save_old", "ff_cycle", "ff_restrict", "ff_forcing",
     "ff_prolong", "ff_vcycle", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
+    "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
+    "ff_mlp_grad", "ff_mlp_train",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
+FF_OPT_EARSM, FF_EARSM_EXACT, FF_EARSM_MLP = 3, 0, 1
+FF_MLP_NPARAM, FF_MLP_NNORM = 4739, 26
 
 _lib = None
 
@@ -67,6 +71,13 @@ def lib():
         L.ff_wall_register.argtypes = [vp, i, i, C.POINTER(WallParams), vp, i]
         L.ff_wall_sync.argtypes = [vp, i, i, i]
         L.ff_wall_forces.argtypes = [vp, i, i, vp]
+        L.ff_mlp_set.argtypes = [vp, vp, vp]
+        L.ff_mlp_get.argtypes = [vp, vp, vp]
+        L.ff_earsm_samples.argtypes = [vp, i, i, vp, vp]
+        L.ff_mlp_fit_norm.argtypes = [vp, vp, vp, i]
+        L.ff_mlp_infer.argtypes = [vp, vp, i, vp]
+        L.ff_mlp_grad.argtypes = [vp, vp, vp, i, vp]
+        L.ff_mlp_train.argtypes = [vp, vp, vp, i, i, i, C.c_float, C.c_uint, vp]
         for fn in EXPORTS:
             if fn != "ff_last_error":
                 getattr(L, fn).restype = C.c_int
@@ -204,6 +215,72 @@ class Context:
         out = np.zeros(19)
         self._chk(lib().ff_wall_forces(self.h, block, patch, out.ctypes.data), "ff_wall_forces")
         return out
+
+    # ---- the learned EARSM closure (device sample arrays are torch tensors) -------------
+    def earsm_mode(self, mode):
+        """EARSM1's coefficients: "exact" or "mlp" (the learned closure in its box)."""
+        self.set_option(FF_OPT_EARSM, {"exact": FF_EARSM_EXACT, "mlp": FF_EARSM_MLP}[mode])
+
+    def mlp_set(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        q = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        self._chk(lib().ff_mlp_set(self.h, None if p is None else p.ctypes.data,
+                                   None if q is None else q.ctypes.data), "ff_mlp_set")
+
+    def mlp_get(self):
+        p = np.zeros(FF_MLP_NPARAM, dtype=np.float32)
+        q = np.zeros(FF_MLP_NNORM, dtype=np.float32)
+        self._chk(lib().ff_mlp_get(self.h, p.ctypes.data, q.ctypes.data), "ff_mlp_get")
+        return p, q
+
+    @staticmethod
+    def _dev(t, cols):
+        import torch
+        assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape[-1] == cols
+        return t.data_ptr()
+
+    def earsm_samples(self, level=1, block=1):
+        """Exact-EARSM samples of every interior cell: (x (n, 5), y (n, 3)) on the device."""
+        import torch
+        blk = self._views[(level, block)][0]
+        n = blk.imax * blk.jmax * blk.kmax
+        x = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+        y = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        self._chk(lib().ff_earsm_samples(self.h, level, block, x.data_ptr(), y.data_ptr()),
+                  "ff_earsm_samples")
+        return x, y
+
+    def mlp_fit_norm(self, x, y):
+        import torch
+        torch.cuda.synchronize()
+        self._chk(lib().ff_mlp_fit_norm(self.h, self._dev(x, 5), self._dev(y, 3), len(x)),
+                  "ff_mlp_fit_norm")
+
+    def mlp_infer(self, x):
+        import torch
+        y = torch.empty((len(x), 3), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        self._chk(lib().ff_mlp_infer(self.h, self._dev(x, 5), len(x), y.data_ptr()), "ff_mlp_infer")
+        return y
+
+    def mlp_grad(self, x, y):
+        """(loss, gradient) over all samples."""
+        import torch
+        g = np.zeros(FF_MLP_NPARAM + 1, dtype=np.float32)
+        torch.cuda.synchronize()
+        self._chk(lib().ff_mlp_grad(self.h, self._dev(x, 5), self._dev(y, 3), len(x),
+                                    g.ctypes.data), "ff_mlp_grad")
+        return float(g[-1]), g[:-1]
+
+    def mlp_train(self, x, y, steps, batch=4096, lr=1e-3, seed=0):
+        """Adam steps on minibatches; returns the minibatch losses."""
+        import torch
+        loss = np.zeros(max(steps, 1), dtype=np.float32)
+        torch.cuda.synchronize()
+        self._chk(lib().ff_mlp_train(self.h, self._dev(x, 5), self._dev(y, 3), len(x), steps,
+                                     batch, lr, seed, loss.ctypes.data), "ff_mlp_train")
+        return loss[:steps]
 
     # ---- timing ------------------------------------------------------------------------
     def profile(self, on=True):

@@ -187,6 +187,12 @@ const char *ff_last_error(const ff_ctx *ctx);
 #define FF_OPT_RESIDUAL    2
 #define FF_RESIDUAL_TILES  0
 #define FF_RESIDUAL_CELLS  1
+/*   FF_OPT_EARSM: FF_EARSM_EXACT (default) evaluates EARSM1 as the reference does;
+ *   FF_EARSM_MLP takes its coefficients from the learned closure (ff_mlp_*, below) in the
+ *   cells whose invariants lie in the training box and the exact EARSM1 everywhere else. */
+#define FF_OPT_EARSM       3
+#define FF_EARSM_EXACT     0
+#define FF_EARSM_MLP       1
 int  ff_ctx_set_option(ff_ctx *ctx, int option, int value);
 /* Per-kernel event timing (0 off, 1 on); results via ff_kernel_stats. */
 int  ff_ctx_profile(ff_ctx *ctx, int enable);
@@ -272,6 +278,37 @@ int  ff_wall_sync(ff_ctx *ctx, int level, int block, int direction);
 /* The FF_NFORCE force sums of patch `patch` (0-based, registration order) of block N from the
  * last level-1 ff_residual. */
 int  ff_wall_forces(ff_ctx *ctx, int block, int patch, double *sums);
+
+/* ---- the learned EARSM closure ------------------------------------------------------ */
+/* No counterpart in the reference: a 5 -> 64 -> 64 -> 3 SiLU network that replaces EARSM1's
+ * cubic solve, N correction and coefficients (earsm.f:170-205).  Inputs: the invariants
+ * II_S, II_W, III_S, IV, V of the T-scaled tensors (SII, WII, tr S^3, SWWIV, SSWWV of
+ * earsm.f:108-124); outputs: beta4, beta6, beta9 (beta3 = 2 IV beta9^2 / beta6).  BF16
+ * MFMA products with FP32 accumulation on the device.
+ * Parameters (FF_MLP_NPARAM floats): W1[64][5], b1[64], W2[64][64], b2[64], W3[3][64],
+ * b3[3], row = output.  Normalisation (FF_MLP_NNORM floats): input means (5) and scales
+ * (5), output means (3) and scales (3), then the training box of the normalised inputs:
+ * low (5), high (5).  Sample arrays are device memory: x[n][5], y[n][3] (float). */
+#define FF_MLP_NPARAM  4739
+#define FF_MLP_NNORM   26
+/* Host copies in and out; either pointer may be NULL.  Setting parameters resets Adam. */
+int  ff_mlp_set(ff_ctx *ctx, const float *params, const float *norm);
+int  ff_mlp_get(ff_ctx *ctx, float *params, float *norm);
+/* Exact-EARSM samples of every interior cell of a registered block (after ff_turb):
+ * cell (I,J,K) at (K-1)*JMAX*IMAX + (J-1)*IMAX + I-1.  Needs ITURB = 6, ISTRES = 1. */
+int  ff_earsm_samples(ff_ctx *ctx, int level, int block, float *x, float *y);
+/* Normalisation and training box from n samples. */
+int  ff_mlp_fit_norm(ff_ctx *ctx, const float *x, const float *y, int n);
+/* Physical outputs of n samples. */
+int  ff_mlp_infer(ff_ctx *ctx, const float *x, int n, float *y);
+/* Mean squared error of the normalised outputs over n samples and its gradient: host
+ * array of FF_MLP_NPARAM + 1 floats, the loss last. */
+int  ff_mlp_grad(ff_ctx *ctx, const float *x, const float *y, int n, float *grad_and_loss);
+/* `steps` Adam steps (learning rate lr) on minibatches of `batch` samples drawn from the
+ * n samples by a hash of (seed, step, slot); loss[s] (host, may be NULL) is the
+ * minibatch loss before step s. */
+int  ff_mlp_train(ff_ctx *ctx, const float *x, const float *y, int n, int steps, int batch,
+                  float lr, unsigned seed, float *loss);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,190 @@
+"""The learned EARSM closure on the GPU, through the C-ABI: BF16-MFMA inference and gradients
+against the CPU oracle (which emulates the kernels' BF16 operand rounding), exact-EARSM
+samples against the numpy restatement, training, and EARSM1 with the closure: exact in the
+cells outside the training box, within a stated error inside it."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from parity import BIJ, make_earsm_case
+
+pytestmark = pytest.mark.gpu
+
+EARSM = dict(iturb=6, istres=1)
+
+
+def _ctx(prm):
+    from finflo_amd.ffhip import Context
+    return Context(prm)
+
+
+def _rand_net(seed):
+    rng = np.random.default_rng(seed)
+    p = M.init_params(seed)
+    p[M.OFF_B1:M.OFF_W2] = rng.normal(0, 0.3, M.NH)
+    p[M.OFF_B2:M.OFF_W3] = rng.normal(0, 0.3, M.NH)
+    p[M.OFF_B3:] = rng.normal(0, 0.3, M.NOUT)
+    norm = np.concatenate([rng.normal(0, 1, 5), rng.uniform(0.5, 2, 5), rng.normal(0, 1, 3),
+                           rng.uniform(0.5, 2, 3), -3 * np.ones(5), 3 * np.ones(5)]).astype(np.float32)
+    return p.astype(np.float32), norm
+
+
+def test_mlp_infer_matches_oracle(hip_device):
+    import torch
+    prm, _ = make_earsm_case((4, 4, 4), EARSM, dict(kind="all", amp=0.01, seed=1))
+    ctx = _ctx(prm)
+    p, norm = _rand_net(11)
+    ctx.mlp_set(p, norm)
+    rng = np.random.default_rng(12)
+    n = 1003                                   # not a multiple of the 16-cell blocks
+    x = np.sinh(rng.normal(0, 1.5, (n, 5)) * norm[5:10] + norm[0:5]).astype(np.float32)
+    print(f"inference of {n} samples", flush=True)
+    y = ctx.mlp_infer(torch.from_numpy(x).cuda()).cpu().numpy()
+    ref = M.predict(p, norm, x, emulate=True)
+    # BF16 rounding of an activation can land on the other side of a tie when the FP32
+    # accumulation order (or asinhf) differs: bounded as a relative error of the outputs
+    err = np.abs(y - ref) / np.abs(ref)
+    print("max normalised error", err.max(), flush=True)
+    assert err.max() < 5e-3
+    p2, n2 = ctx.mlp_get()
+    np.testing.assert_array_equal(p2, p)
+    np.testing.assert_array_equal(n2, norm)
+    ctx.close()
+
+
+def test_mlp_gradient_matches_oracle(hip_device):
+    import torch
+    prm, _ = make_earsm_case((4, 4, 4), EARSM, dict(kind="all", amp=0.01, seed=1))
+    ctx = _ctx(prm)
+    p, norm = _rand_net(21)
+    ctx.mlp_set(p, norm)
+    rng = np.random.default_rng(22)
+    n = 777
+    x = np.sinh(rng.normal(0, 1.5, (n, 5)) * norm[5:10] + norm[0:5]).astype(np.float32)
+    t = (M.SIGN * np.exp(rng.normal(0, 1.0, (n, 3)) * norm[13:16] + norm[10:13])).astype(np.float32)
+    print(f"gradient over {n} samples", flush=True)
+    loss, g = ctx.mlp_grad(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+    lref, gref = M.loss_grad(p, norm, x, t, emulate=True)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print("loss", loss, lref, "relative gradient error", rel, flush=True)
+    assert abs(loss - lref) <= 1e-3 * lref
+    assert rel < 1e-2
+    # every parameter block is covered (a transposed or dropped block shows here)
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2),
+                 (M.OFF_B2, M.OFF_W3), (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.NP)):
+        rb = np.linalg.norm(g[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
+        assert rb < 3e-2, (a, b, rb)
+    ctx.close()
+
+
+def _earsm_blocks(ctx, specs):
+    """Register, upload and run STATIM + TURBDI on seeded EARSM blocks (block ids 1..)."""
+    blks = []
+    for bid, (dims, ckw) in enumerate(specs, start=1):
+        _, blk = make_earsm_case(dims, EARSM, ckw)
+        ctx.register(blk, block=bid)
+        ctx.upload(block=bid)
+        ctx.save_old(block=bid)
+        ctx.statim(block=bid)
+        ctx.turbdi(block=bid)
+        blks.append(blk)
+    return blks
+
+
+TRAIN = [((20, 14, 10), dict(kind="all", amp=0.06, distort=0.12, seed=301, turb=(1.0, 50.0))),
+         ((18, 12, 12), dict(kind="all", amp=0.08, distort=0.1, seed=302, turb=(0.5, 100.0))),
+         ((16, 16, 10), dict(kind="velocity", amp=0.05, jratio=1.2, seed=303, turb=(0.3, 80.0)))]
+HELD = ((18, 14, 10), dict(kind="all", amp=0.07, distort=0.11, seed=399, turb=(0.7, 70.0)))
+
+
+def test_earsm_samples_match_restatement(hip_device):
+    prm, _ = make_earsm_case((4, 4, 4), EARSM, dict(kind="all", amp=0.01, seed=1))
+    ctx = _ctx(prm)
+    (blk,) = _earsm_blocks(ctx, TRAIN[:1])
+    x, y = ctx.earsm_samples(block=1)
+    ctx.download(block=1)
+    S = np.stack([blk.interior(f).ravel() for f in ("S11", "S12", "S13", "S22", "S23", "S33")], 1)
+    W = np.stack([blk.interior(f).ravel() for f in ("W12", "W13", "W23")], 1)
+    inv, beta, _ = M.earsm_parts(S, W, blk.interior("TTS").ravel())
+    np.testing.assert_array_equal(x.cpu().numpy(), inv.astype(np.float32))
+    np.testing.assert_allclose(y.cpu().numpy(), beta[:, 1:].astype(np.float32), rtol=1e-6)
+    ctx.close()
+
+
+def test_training_and_learned_closure(hip_device):
+    import torch
+    prm, _ = make_earsm_case((4, 4, 4), EARSM, dict(kind="all", amp=0.01, seed=1))
+    ctx = _ctx(prm)
+    blks = _earsm_blocks(ctx, TRAIN + [HELD])
+    xs, ys = zip(*[ctx.earsm_samples(block=b) for b in range(1, len(TRAIN) + 1)])
+    x, y = torch.cat(xs), torch.cat(ys)
+    print(f"{len(x)} training samples", flush=True)
+    ctx.mlp_fit_norm(x, y)
+    ctx.mlp_set(M.init_params(5))
+    xh, yh = ctx.earsm_samples(block=len(TRAIN) + 1)
+    p0, nrm = ctx.mlp_get()
+    yhn = yh.cpu().numpy()
+    # mean |log error| = mean relative error of the coefficients
+    e0 = np.abs(np.log(M.predict(p0, nrm, xh.cpu().numpy()) / yhn)).mean(0)
+    loss = ctx.mlp_train(x, y, steps=4000, batch=2048, lr=3e-3, seed=17)
+    print("minibatch loss", loss[:3], loss[-3:], flush=True)
+    assert loss[-100:].mean() < 0.02 * loss[:10].mean()
+    # the trained surrogate on held-out cells, against the untrained network
+    yp = ctx.mlp_infer(xh).cpu().numpy()
+    e1 = np.abs(np.log(yp / yhn)).mean(0)
+    print("held-out mean |log error| of beta4, beta6, beta9: untrained", e0, "trained", e1,
+          flush=True)
+    assert np.all(e1 < 0.2 * e0)
+    # EARSM1 on the held-out block: exact, then with the closure
+    hb = len(TRAIN) + 1
+    ctx.turbdi(block=hb)
+    ctx.download(block=hb)
+    hblk = blks[-1]
+    exact = {f: hblk.interior(f).copy() for f in BIJ}
+    S = np.stack([hblk.interior(f).ravel() for f in ("S11", "S12", "S13", "S22", "S23", "S33")], 1)
+    W = np.stack([hblk.interior(f).ravel() for f in ("W12", "W13", "W23")], 1)
+    inv, _, terms = M.earsm_parts(S, W, hblk.interior("TTS").ravel())
+    ctx.earsm_mode("mlp")
+    ctx.turbdi(block=hb)
+    ctx.download(block=hb)
+    got = np.stack([hblk.interior(f).ravel() for f in BIJ], 1)
+    ref = np.stack([exact[f].ravel() for f in BIJ], 1)
+    p, nrm = ctx.mlp_get()
+    xn = (M.feature(xh.cpu().numpy()) - nrm[0:5]) / nrm[5:10]
+    margin = 1e-4
+    inside = np.all((xn >= nrm[16:21] + margin) & (xn <= nrm[21:26] - margin), axis=1)
+    outside = np.any((xn < nrm[16:21] - margin) | (xn > nrm[21:26] + margin), axis=1)
+    print(f"held-out cells: {inside.sum()} inside the box, {outside.sum()} outside", flush=True)
+    assert inside.sum() > 0
+    np.testing.assert_array_equal(got[outside], ref[outside])
+    # the kernel's closure is the network's N put through EARSM1's formulas
+    want = M.bij(M.betas_from_learned(yp, inv), terms)
+    assert np.all(np.abs(got[inside] - want[inside]) <= 1e-9 * np.abs(want).max(0))
+    for q, f in enumerate(BIJ):
+        a, b = got[inside, q], ref[inside, q]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"{f}: relative L2 error of the learned closure against the exact EARSM {rel:.3e}",
+              flush=True)
+        assert rel < 0.25
+    ctx.earsm_mode("exact")
+    ctx.close()
+
+
+def test_learned_closure_falls_back_outside_the_box(hip_device):
+    """With a training box that holds no cell, EARSM1 through the closure is the exact EARSM1
+    bit for bit."""
+    prm, _ = make_earsm_case((4, 4, 4), EARSM, dict(kind="all", amp=0.01, seed=1))
+    ctx = _ctx(prm)
+    (blk,) = _earsm_blocks(ctx, [HELD])
+    ctx.download()
+    exact = {f: blk.interior(f).copy() for f in BIJ}
+    p, norm = _rand_net(31)
+    norm[16:21] = 50.0                       # low = high = 50: no normalised input reaches it
+    norm[21:26] = 50.0
+    ctx.mlp_set(p, norm)
+    ctx.earsm_mode("mlp")
+    ctx.turbdi()
+    ctx.download()
+    for f in BIJ:
+        np.testing.assert_array_equal(blk.interior(f), exact[f])
+    ctx.close()
