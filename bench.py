@@ -198,12 +198,114 @@ def copy_probe(mib):
                       "launches": 3}))
 
 
+def earsm_mlp_bench(args):
+    """The learned EARSM closure on a 256x128x128 block (4.19M cells, k-omega SST with
+    EARSM): exact-EARSM samples of every cell, training (Adam minibatches), inference of
+    every cell, and EARSM1 through the closure against the exact EARSM1, timed with HIP
+    events and compared field by field.  Prints one JSON line."""
+    import torch
+    from finflo_amd.ffhip import Context
+    from finflo_amd.cases import make_hierarchy, flat_plate_nodes
+    ni, nj, nk = 256, 128, 128
+    prm = workload_params(6, 2.0e6)
+    prm.istres = 1
+    turb = freestream_turbulence(prm)
+    nodes = flat_plate_nodes(ni, nj, nk, 1e-6, 1.15, 1.0, None)
+    blk = make_hierarchy(ni, nj, nk, 1, prm, seed=4, amp=0.05, kind="velocity", turb=turb,
+                         nodes=nodes, turb_amp=0.3)[0]
+    ncell = ni * nj * nk
+    print(f"[bench] earsm-mlp {ni}x{nj}x{nk} ({ncell} cells), building context", file=sys.stderr, flush=True)
+    ctx = Context(prm)
+    ctx.register(blk)
+    ctx.upload()
+    ctx.save_old()
+    ctx.statim()
+    ctx.turbdi()                                   # exact EARSM1
+    ctx.finish()
+    ctx.download(fields=["B11", "B12", "B13", "B22", "B23", "B33"])
+    exact = np.stack([blk.interior(f).ravel().copy() for f in ("B11", "B12", "B13", "B22", "B23", "B33")], 1)
+    x, y = ctx.earsm_samples()
+    ctx.mlp_fit_norm(x, y)
+    from finflo_amd.mlp import init_params
+    ctx.mlp_set(init_params(5))
+    steps, batch = args.steps * 100, 8192
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = ctx.mlp_train(x, y, steps=steps, batch=batch, lr=3e-3, seed=11)
+    train_s = time.perf_counter() - t0
+    print(f"[bench] {steps} training steps in {train_s:.2f} s, loss {loss[0]:.3g} -> {loss[-1]:.3g}",
+          file=sys.stderr, flush=True)
+    # event-timed kernels: training steps, inference, EARSM1 both ways
+    ctx.profile(True)
+    ctx.reset_stats()
+    ctx.mlp_train(x, y, steps=args.steps, batch=batch, lr=1e-5, seed=12)
+    st_grad = ctx.kernel_stats("k_mlp_grad")
+    for _ in range(args.warmup):
+        ctx.mlp_infer(x)
+    ctx.reset_stats()
+    for _ in range(args.steps):
+        yp = ctx.mlp_infer(x)
+    ctx.finish()
+    st = {"k_mlp_infer": ctx.kernel_stats("k_mlp_infer"), "k_mlp_grad": st_grad}
+    ctx.earsm_mode("mlp")
+    ctx.reset_stats()
+    for _ in range(args.steps):
+        ctx.turbdi()
+    ctx.finish()
+    st["k_turb_sst without EARSM1"] = ctx.kernel_stats("k_turb_sst")
+    st["k_earsm_mlp"] = ctx.kernel_stats("k_earsm_mlp")
+    ctx.download(fields=["B11", "B12", "B13", "B22", "B23", "B33"])
+    got = np.stack([blk.interior(f).ravel() for f in ("B11", "B12", "B13", "B22", "B23", "B33")], 1)
+    ctx.earsm_mode("exact")
+    ctx.reset_stats()
+    for _ in range(args.steps):
+        ctx.turbdi()
+    ctx.finish()
+    st["k_turb_sst with EARSM1"] = ctx.kernel_stats("k_turb_sst")
+    ctx.profile(False)
+    nin, msin = st["k_mlp_infer"]
+    infer_ms = msin / nin
+    flops_cell = 2 * (5 * 64 + 64 * 64 + 64 * 3)
+    tflops = flops_cell * ncell / (infer_ms * 1e-3) / 1e12
+    ng, msg = st["k_mlp_grad"]
+    # the training kernels' flops per sample: forward, the two input-gradient products and
+    # the three weight-gradient products
+    grad_tflops = 3 * flops_cell * batch / (msg / ng * 1e-3) / 1e12
+    rel = [float(np.linalg.norm(got[:, q] - exact[:, q]) / np.linalg.norm(exact[:, q])) for q in range(6)]
+    yt = y.cpu().numpy()
+    logerr = np.abs(np.log(yp.cpu().numpy() / yt)).mean(0).tolist()
+    avg = {k: v[1] / max(v[0], 1) for k, v in st.items()}
+    line = {
+        "metric": "learned EARSM closure: inference TFLOP/s (BF16 MFMA) and b_ij error against the exact EARSM",
+        "value": tflops, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "dtype": "bf16 (FP32 accumulate)", "data": "synthetic",
+        "config": {"workload": f"flat plate {ni}x{nj}x{nk} single block, k-omega SST + EARSM, "
+                               "velocity noise 5 %, training samples = every cell's exact EARSM",
+                   "cells": ncell, "train_steps": steps, "train_batch": batch,
+                   "train_seconds": train_s, "loss_first": float(loss[0]), "loss_last": float(loss[-1])},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "kernel": "k_mlp_infer", "avg_launch_ms": infer_ms,
+                     "flops_per_cell": flops_cell},
+        "training_tflops": grad_tflops,
+        "mean_abs_log_error_beta4_beta6_beta9": logerr,
+        "bij_relative_l2_error": dict(zip(("B11", "B12", "B13", "B22", "B23", "B33"), rel)),
+        "earsm_pass_ms": {
+            "exact EARSM1 (k_turb_sst with it minus without it)":
+                avg["k_turb_sst with EARSM1"] - avg["k_turb_sst without EARSM1"],
+            "learned closure (k_earsm_mlp)": avg["k_earsm_mlp"],
+            "k_turb_sst with EARSM1": avg["k_turb_sst with EARSM1"],
+            "k_turb_sst without EARSM1": avg["k_turb_sst without EARSM1"]},
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="sst-mg")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp"], default="sst-mg")
     ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -220,6 +322,10 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(blocks would shard across GPUs; this build measures one GPU)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "earsm-mlp":
+        earsm_mlp_bench(args)
+        return
 
     from finflo_amd.ffhip import Context
 

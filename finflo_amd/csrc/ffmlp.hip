@@ -28,11 +28,10 @@ constexpr int MPP = MNP + 1;           // per-wave partial: gradient, then the l
 constexpr int MN_XM = 0, MN_XS = 5, MN_YM = 10, MN_YS = 13, MN_LO = 16, MN_HI = 21;
 static_assert(MN_HI + 5 == FF_MLP_NNORM, "normalisation block of the header");
 
+// round to nearest even (gfx950's conversion instruction)
 __device__ __forceinline__ short bf16_bits(float f)
 {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);           // round to nearest even
-    return (short)(u >> 16);
+    return __builtin_bit_cast(short, (__bf16)f);
 }
 
 __device__ __forceinline__ float bf16_val(short h)
@@ -40,11 +39,14 @@ __device__ __forceinline__ float bf16_val(short h)
     return __uint_as_float(((unsigned)(unsigned short)h) << 16);
 }
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + expf(-z)); }
+// SiLU z s(z) and its derivative s (1 + z (1 - s)), s = 1/(1 + e^-z), with the hardware's
+// exp2 and reciprocal (the activations are rounded to BF16 next)
+__device__ __forceinline__ float sigm_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_f(float z) { return z * sigm_f(z); }
 
 __device__ __forceinline__ float dsilu_f(float z)
 {
-    const float s = 1.0f / (1.0f + expf(-z));
+    const float s = sigm_f(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -56,16 +58,21 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
 // The weights of one wave as A operands (forward: W1, W2, W3; backward: W2^T, W3^T)
 struct MlpFrags {
     v4s w1[4], w2[4][4], w3[4], w2t[4][4], w3t[4];
+    float b1[4][4], b2[4][4], b3[4];          // this lane's rows of the biases
 };
 
 __device__ __forceinline__ void mlp_load_frags(const float *p, int lane, MlpFrags &f, bool bwd)
 {
     const int g = lane >> 4, c = lane & 15;
 #pragma unroll
+    for (int i = 0; i < 4; i++) f.b3[i] = 4 * g + i < MNOUT ? p[MOFF_B3 + 4 * g + i] : 0.f;
+#pragma unroll
     for (int ob = 0; ob < 4; ob++) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int k = 4 * g + i;
+            f.b1[ob][i] = p[MOFF_B1 + 16 * ob + k];
+            f.b2[ob][i] = p[MOFF_B2 + 16 * ob + k];
             f.w1[ob][i] = bf16_bits(k < MNIN ? p[MOFF_W1 + (16 * ob + c) * MNIN + k] : 0.f);
             f.w3[ob][i] = bf16_bits(c < MNOUT ? p[MOFF_W3 + c * MNH + 16 * ob + k] : 0.f);
             if (bwd) f.w3t[ob][i] = bf16_bits(k < MNOUT ? p[MOFF_W3 + k * MNH + 16 * ob + c] : 0.f);
@@ -89,8 +96,8 @@ struct MlpFwd {
     v4s x, h1[4], h2[4];
 };
 
-__device__ __forceinline__ void mlp_forward16(const float *p, const MlpFrags &f, int lane,
-                                              const float xn[4], MlpFwd &o)
+__device__ __forceinline__ void mlp_forward16(const MlpFrags &f, int lane, const float xn[4],
+                                              MlpFwd &o)
 {
     const int g = lane >> 4;
 #pragma unroll
@@ -101,7 +108,7 @@ __device__ __forceinline__ void mlp_forward16(const float *p, const MlpFrags &f,
         z = mfma(f.w1[ob], o.x, z);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            z[i] += p[MOFF_B1 + 16 * ob + 4 * g + i];
+            z[i] += f.b1[ob][i];
             o.h1[ob][i] = bf16_bits(silu_f(z[i]));
         }
         o.z1[ob] = z;
@@ -113,7 +120,7 @@ __device__ __forceinline__ void mlp_forward16(const float *p, const MlpFrags &f,
         for (int kb = 0; kb < 4; kb++) z = mfma(f.w2[ob][kb], o.h1[kb], z);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            z[i] += p[MOFF_B2 + 16 * ob + 4 * g + i];
+            z[i] += f.b2[ob][i];
             o.h2[ob][i] = bf16_bits(silu_f(z[i]));
         }
         o.z2[ob] = z;
@@ -122,14 +129,18 @@ __device__ __forceinline__ void mlp_forward16(const float *p, const MlpFrags &f,
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) y = mfma(f.w3[kb], o.h2[kb], y);
 #pragma unroll
-    for (int i = 0; i < 4; i++) y[i] += 4 * g + i < MNOUT ? p[MOFF_B3 + 4 * g + i] : 0.f;
+    for (int i = 0; i < 4; i++) y[i] += f.b3[i];
     o.y = y;
 }
 
 // Feature and target transforms: the invariants span several decades with either sign, so
 // the network sees asinh of each; the coefficients have fixed signs (beta4, beta6 < 0,
 // beta9 > 0) and span many decades, so it learns log(sign * beta).
-__device__ __forceinline__ float mlp_feature(float v) { return asinhf(v); }
+__device__ __forceinline__ float mlp_feature(float v)
+{
+    const float a = fabsf(v);
+    return copysignf(__logf(a + __fsqrt_rn(a * a + 1.f)), v);       // asinh
+}
 __device__ __forceinline__ float mlp_sign(int q) { return q < 2 ? -1.f : 1.f; }
 __device__ __forceinline__ float mlp_target(float beta, int q) { return logf(fmaxf(mlp_sign(q) * beta, 1e-37f)); }
 __device__ __forceinline__ float mlp_output(float yn, const float *nrm, int q)
@@ -163,7 +174,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *p, const float *
         float xn[4];
         mlp_inputs(x, nrm, cell, cell < n, g, xn);
         MlpFwd o;
-        mlp_forward16(p, f, lane, xn, o);
+        mlp_forward16(f, lane, xn, o);
         if (g == 0 && cell < n)
 #pragma unroll
             for (int i = 0; i < MNOUT; i++) y[(size_t)cell * MNOUT + i] = mlp_output(o.y[i], nrm, i);
@@ -232,7 +243,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float *p, const float *n
         float xn[4];
         mlp_inputs(x, nrm, cell, ok, g, xn);
         MlpFwd o;
-        mlp_forward16(p, f, lane, xn, o);
+        mlp_forward16(f, lane, xn, o);
         // output signal dL/dyn = 2 (yn - tn) / N on rows o < 3 (lanes g = 0)
         v4f dy = {0.f, 0.f, 0.f, 0.f};
         v4s dyb;
@@ -497,7 +508,7 @@ __global__ void __launch_bounds__(256) k_earsm_mlp(Blk b, const float *p, const 
 #pragma unroll
         for (int i = 0; i < 4; i++) xn[i] = 4 * g + i < MNIN ? xs[w][16 * cb + c][4 * g + i] : 0.f;
         MlpFwd o;
-        mlp_forward16(p, f, lane, xn, o);
+        mlp_forward16(f, lane, xn, o);
         if (g == 0)
 #pragma unroll
             for (int i = 0; i < MNOUT; i++) ys[w][16 * cb + c][i] = mlp_output(o.y[i], nrm, i);
