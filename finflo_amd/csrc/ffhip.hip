@@ -2463,7 +2463,8 @@ __device__ __forceinline__ FaceBasis basis_get(const double *v)
 
 // Both records of every cell: thread (lane, step) of tile blockIdx.x, four steps per
 // workgroup, so that a workgroup's loads of a column run along I and its record stores are
-// whole rows.
+// whole rows.  The records are read back only by the sweeps, long after: non-temporal
+// stores keep them from displacing the caches' working set on the way out.
 template <bool KE>
 __global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st,
                                                       TileGrid tg)
@@ -2500,7 +2501,7 @@ __global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, S
         }
         static_assert(RP_X + 9 == NRF, "the predictor record fills NRF fields");
 #pragma unroll
-        for (int f = 0; f < NRF; f++) st.recp[st_row(st, tile, s, f) + lane] = v[f];
+        for (int f = 0; f < NRF; f++) __builtin_nontemporal_store(v[f], &st.recp[st_row(st, tile, s, f) + lane]);
     }
     {   // corrector (DIAGGS + CORRGS / DIKEGS + COKEGS): upper faces
         SweepRaw r;
@@ -2520,7 +2521,7 @@ __global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, S
         static_assert(RC_RHO < NRF, "the corrector record fits NRF fields");
         for (int f = RC_RHO + 1; f < NRF; f++) v[f] = 0.;
 #pragma unroll
-        for (int f = 0; f < NRF; f++) st.recc[st_row(st, tile, s, f) + lane] = v[f];
+        for (int f = 0; f < NRF; f++) __builtin_nontemporal_store(v[f], &st.recc[st_row(st, tile, s, f) + lane]);
     }
 }
 
@@ -2537,14 +2538,14 @@ constexpr int RBYTES = NRF * TW * 8;            // one record: 28 KiB
 constexpr int DBYTES = NDN * TW * 8;            // one step of increments: 4 KiB
 
 // four 1 KiB loads from one address pair: the instruction offset advances the global and the
-// LDS address alike
+// LDS address alike.  The records are read once: non-temporal (nt), as they were written.
 __device__ __forceinline__ void lds_dma16x4(const char *src, unsigned lds)
 {
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\t"
-                 "global_load_lds_dwordx4 %0, off\n\t"
-                 "global_load_lds_dwordx4 %0, off offset:1024\n\t"
-                 "global_load_lds_dwordx4 %0, off offset:2048\n\t"
-                 "global_load_lds_dwordx4 %0, off offset:3072"
+                 "global_load_lds_dwordx4 %0, off nt\n\t"
+                 "global_load_lds_dwordx4 %0, off offset:1024 nt\n\t"
+                 "global_load_lds_dwordx4 %0, off offset:2048 nt\n\t"
+                 "global_load_lds_dwordx4 %0, off offset:3072 nt"
                  ::"v"(src), "s"(lds) : "memory", "m0");
 }
 
