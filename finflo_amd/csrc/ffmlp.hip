@@ -3,8 +3,9 @@
 // from which N = -beta6/beta9 replaces the cubic solve and N correction of earsm.f:170-193.  Cells whose normalised
 // invariants fall outside the box spanned by the training samples take the exact EARSM.
 //
-// Every product of the network runs on the matrix cores: v_mfma_f32_16x16x16_bf16 with BF16
-// operands and FP32 accumulation, FP32 biases and activations, FP32 master weights.  The
+// Every product of the network runs on the matrix cores: v_mfma_f32_16x16x32_bf16 for the
+// contractions over hidden features, v_mfma_f32_16x16x16_bf16 for the rest, BF16 operands and
+// FP32 accumulation, FP32 biases and activations, FP32 master weights.  The
 // network is evaluated transposed (features along M, cells along N), so a layer's 16x16
 // accumulator block is, lane for lane, the B operand of the next layer:
 //   A operand  a[i] = A[m = lane % 16][k = 4 (lane / 16) + i]
@@ -16,6 +17,8 @@
 // This file is part of the ffhip.hip translation unit (included at its end).
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int MNIN = 5, MNH = 64, MNOUT = 3;
@@ -55,9 +58,28 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// 16x16x32: a[j] = A[m = lane % 16][k = 8 (lane / 16) + j], b[j] = B[k = 8 (lane / 16) + j]
+// [n = lane % 16], j = 0..7.  The hidden layers contract over 64 features in two such steps.
+// Their B operand is two accumulator blocks of the previous layer side by side, so the
+// instruction's k = 8g + j stands for feature phi(kb, g, j) = 32 kb + 16 (j / 4) + 4 g + j % 4;
+// the weights are gathered in the same order and no lane exchanges values.
+__device__ __forceinline__ v4f mfma32(v8s a, v8s b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                   __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int phi(int kb, int g, int j) { return 32 * kb + 16 * (j >> 2) + 4 * g + (j & 3); }
+
+__device__ __forceinline__ v8s cat8(v4s lo, v4s hi)
+{
+    return v8s{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
 // The weights of one wave as A operands (forward: W1, W2, W3; backward: W2^T, W3^T)
 struct MlpFrags {
-    v4s w1[4], w2[4][4], w3[4], w2t[4][4], w3t[4];
+    v4s w1[4], w3t[4];                        // 16x16x16 operands
+    v8s w2[4][2], w3[2], w2t[4][2];           // 16x16x32 operands, k in phi order
     float b1[4][4], b2[4][4], b3[4];          // this lane's rows of the biases
 };
 
@@ -74,18 +96,22 @@ __device__ __forceinline__ void mlp_load_frags(const float *p, int lane, MlpFrag
             f.b1[ob][i] = p[MOFF_B1 + 16 * ob + k];
             f.b2[ob][i] = p[MOFF_B2 + 16 * ob + k];
             f.w1[ob][i] = bf16_bits(k < MNIN ? p[MOFF_W1 + (16 * ob + c) * MNIN + k] : 0.f);
-            f.w3[ob][i] = bf16_bits(c < MNOUT ? p[MOFF_W3 + c * MNH + 16 * ob + k] : 0.f);
             if (bwd) f.w3t[ob][i] = bf16_bits(k < MNOUT ? p[MOFF_W3 + k * MNH + 16 * ob + c] : 0.f);
         }
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++)
+        for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int k = 16 * kb + 4 * g + i;
-                f.w2[ob][kb][i] = bf16_bits(p[MOFF_W2 + (16 * ob + c) * MNH + k]);
-                if (bwd) f.w2t[ob][kb][i] = bf16_bits(p[MOFF_W2 + k * MNH + 16 * ob + c]);
+            for (int j = 0; j < 8; j++) {
+                const int k = phi(kb, g, j);
+                f.w2[ob][kb][j] = bf16_bits(p[MOFF_W2 + (16 * ob + c) * MNH + k]);
+                if (bwd) f.w2t[ob][kb][j] = bf16_bits(p[MOFF_W2 + k * MNH + 16 * ob + c]);
             }
     }
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            f.w3[kb][j] = bf16_bits(c < MNOUT ? p[MOFF_W3 + c * MNH + phi(kb, g, j)] : 0.f);
 }
 
 // Forward pass of 16 cells.  xn: this lane's normalised inputs 4g..4g+3 of cell lane%16.
@@ -117,7 +143,7 @@ __device__ __forceinline__ void mlp_forward16(const MlpFrags &f, int lane, const
     for (int ob = 0; ob < 4; ob++) {
         v4f z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) z = mfma(f.w2[ob][kb], o.h1[kb], z);
+        for (int kb = 0; kb < 2; kb++) z = mfma32(f.w2[ob][kb], cat8(o.h1[2 * kb], o.h1[2 * kb + 1]), z);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             z[i] += f.b2[ob][i];
@@ -127,7 +153,7 @@ __device__ __forceinline__ void mlp_forward16(const MlpFrags &f, int lane, const
     }
     v4f y = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) y = mfma(f.w3[kb], o.h2[kb], y);
+    for (int kb = 0; kb < 2; kb++) y = mfma32(f.w3[kb], cat8(o.h2[2 * kb], o.h2[2 * kb + 1]), y);
 #pragma unroll
     for (int i = 0; i < 4; i++) y[i] += f.b3[i];
     o.y = y;
@@ -278,7 +304,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const float *p, const float *n
         for (int fb = 0; fb < 4; fb++) {
             v4f dh = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) dh = mfma(f.w2t[fb][kb], dz2b[kb], dh);
+            for (int kb = 0; kb < 2; kb++) dh = mfma32(f.w2t[fb][kb], cat8(dz2b[2 * kb], dz2b[2 * kb + 1]), dh);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const float dz = dh[i] * dsilu_f(o.z1[fb][i]);
