@@ -4078,10 +4078,17 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
                          (const double *)(r->kw_scr + r->kw_parts)))) return rc;
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
-        const TileGrid tg = tile_grid((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
-        dim3 g(NXCD * tg.per);
-        rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b, wr.wd, tg)
-                 : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b, wr.wd, tg);
+        if (level >= 2) {
+            // the coarse levels' blocks are small: 32x4 tiles of two planes keep every CU busy
+            const TileGrid tg = tile_grid((b.imax + 31) / 32, (b.jmax + 3) / 4, (b.kmax + 1) / 2);
+            rc = launch(c, "k_residual_kw", k_residual_kw<32, 4, 2>, dim3(NXCD * tg.per), dim3(128),
+                        kp, b, wr.wd, tg);
+        } else {
+            const TileGrid tg = tile_grid((b.imax + 63) / 64, (b.jmax + 3) / 4, (b.kmax + 7) / 8);
+            dim3 g(NXCD * tg.per);
+            rc = fns ? launch(c, "k_residual_kw_ns", k_residual_kw<64, 4, 8, true>, g, dim3(256), kp, b, wr.wd, tg)
+                     : launch(c, "k_residual_kw", k_residual_kw<64, 4, 8>, g, dim3(256), kp, b, wr.wd, tg);
+        }
     } else {
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
