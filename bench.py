@@ -127,8 +127,14 @@ def make_workload(name="sst-mg", seed=0):
     return prm, blocks
 
 
-def run_steps(ctx, nlev, n):
-    return ctx.vcycle(nlev, ncycles=n) if nlev > 1 else ctx.cycle(ncycles=n)
+def run_steps(ctx, nlev, n, graph=False):
+    """n V-cycles (iterations with one level); with graph, the first n - 1 replay one captured
+    HIP graph and the last one runs eagerly to return the level-1 residual sums."""
+    eager = (lambda k: ctx.vcycle(nlev, ncycles=k)) if nlev > 1 else (lambda k: ctx.cycle(ncycles=k))
+    if graph and n > 1:
+        ctx.vcycle_graph(nlev, ncycles=n - 1)
+        return eager(1)
+    return eager(n)
 
 
 def cpu_baseline(prm, blocks, seconds=20.0):
@@ -308,6 +314,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp"], default="sst-mg")
     ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
+    ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
+                    help="replay the steps from a captured HIP graph")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -341,14 +349,14 @@ def main():
     ctx.register_levels(blocks)
     for m in range(1, nlev + 1):
         ctx.upload(level=m)
-    resi0 = run_steps(ctx, nlev, args.warmup)
+    resi0 = run_steps(ctx, nlev, args.warmup, args.graph)
     ctx.finish()
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
 
     # timed region: K steps, device-resident, bracketed by stream synchronisation
     ctx.finish()
     t0 = time.perf_counter()
-    resi = run_steps(ctx, nlev, args.steps)
+    resi = run_steps(ctx, nlev, args.steps, args.graph)
     ctx.finish()
     el = time.perf_counter() - t0
     ms_per_step = el * 1e3 / args.steps
@@ -440,6 +448,8 @@ def main():
                                f"M={MACH} Re={w['reynolds']:g}, Roe+FLUXDO, LU-SGS CFL=5; "
                                f"a step is one {'V-cycle' if nlev > 1 else 'iteration'}",
                    "cells": ncell, "levels": nlev,
+                   "launch": "HIP graph replay of K-1 steps + 1 eager step (residual sums)"
+                   if args.graph else "eager launches",
                    "lusgs_sweep_ms_per_step": sweep_ms,
                    "lusgs_compulsory_GBps": lusgs_gbps,
                    f"iteration_hbm_GBps_at_{ITER_BYTES_PER_CELL[iturb]}B_per_cell": iter_bw,

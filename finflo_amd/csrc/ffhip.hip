@@ -3510,6 +3510,12 @@ struct ff_ctx {
     hipStream_t wstream = nullptr;   // side stream of the wall force sums (default priority)
     int earsm_mode = FF_EARSM_EXACT;
     MlpRec mlp;
+    // the captured V-cycle (ff_vcycle_graph) and what it was captured for; gen counts the
+    // registrations and option changes that make a capture stale
+    hipGraph_t vg_graph = nullptr;
+    hipGraphExec_t vg_exec = nullptr;
+    long long vg_key[6] = {0, 0, 0, 0, 0, 0};
+    long long gen = 0;
 };
 
 static void mlp_free(MlpRec &r);
@@ -3808,6 +3814,8 @@ int ff_ctx_destroy(ff_ctx *c)
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     if (c->wstream) (void)hipStreamDestroy(c->wstream);
     mlp_free(c->mlp);
+    if (c->vg_exec) (void)hipGraphExecDestroy(c->vg_exec);
+    if (c->vg_graph) (void)hipGraphDestroy(c->vg_graph);
     delete c;
     return FF_OK;
 }
@@ -3834,18 +3842,21 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
             value != FF_LUSGS_STAGED)
             return fail(c, FF_E_ARG, "FF_OPT_LUSGS: unknown value%s");
         c->lusgs_mode = value;
+        c->gen++;
         return FF_OK;
     }
     if (option == FF_OPT_EARSM) {
         if (value != FF_EARSM_EXACT && value != FF_EARSM_MLP)
             return fail(c, FF_E_ARG, "FF_OPT_EARSM: unknown value%s");
         c->earsm_mode = value;
+        c->gen++;
         return FF_OK;
     }
     if (option == FF_OPT_RESIDUAL) {
         if (value != FF_RESIDUAL_TILES && value != FF_RESIDUAL_CELLS)
             return fail(c, FF_E_ARG, "FF_OPT_RESIDUAL: unknown value%s");
         c->residual_mode = value;
+        c->gen++;
         return FF_OK;
     }
     return fail(c, FF_E_ARG, "unknown option%s");
@@ -3883,6 +3894,7 @@ int ff_kernel_stats_reset(ff_ctx *c)
 int ff_block_register(ff_ctx *c, int level, int block, const ff_block *host)
 {
     if (!c) return FF_E_ARG;
+    c->gen++;
     int rc = check_dims(c, host);
     if (rc) return rc;
     if ((rc = check_fields(c, host, "host field %s is NULL"))) return rc;
@@ -3913,6 +3925,7 @@ int ff_block_register(ff_ctx *c, int level, int block, const ff_block *host)
 int ff_block_register_device(ff_ctx *c, int level, int block, const ff_block *dev)
 {
     if (!c) return FF_E_ARG;
+    c->gen++;
     int rc = check_dims(c, dev);
     if (rc) return rc;
     if ((rc = check_fields(c, dev, "device field %s is NULL"))) return rc;
@@ -4134,6 +4147,23 @@ int ff_residual(ff_ctx *c, int level, int block, int m2, double *resi)
     return residual_impl(c, r, level, m2, resi);
 }
 
+// the staged LU-SGS records of a block level, allocated on first use and kept
+static int staged_alloc(ff_ctx *c, BlockRec *r)
+{
+    if (r->st_mem) return FF_OK;
+    const Blk &b = r->dev;
+    SweepSt &st = r->st;
+    st.nstep = b.imax + TJ + TK - 2;
+    const size_t nt = (size_t)r->sw.ntj * r->sw.ntk;
+    const size_t nrec = nt * st.nstep * NRF * TW, ndn = nt * st.nstep * NDN * TW;
+    if (hipMalloc(&r->st_mem, sizeof(double) * (2 * nrec + ndn)) != hipSuccess)
+        return fail(c, FF_E_NOMEM, "LU-SGS staged records%s");
+    st.recp = r->st_mem;
+    st.recc = st.recp + nrec;
+    st.dn = st.recc + nrec;
+    return FF_OK;
+}
+
 static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
 {
     const Blk &b = r->dev;
@@ -4153,17 +4183,7 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
     }
     if (c->lusgs_mode == FF_LUSGS_STAGED) {
         // records of both sweeps in step order, then the two wavefronts on them
-        if (!r->st_mem) {
-            SweepSt &st = r->st;
-            st.nstep = b.imax + TJ + TK - 2;
-            const size_t nt = (size_t)r->sw.ntj * r->sw.ntk;
-            const size_t nrec = nt * st.nstep * NRF * TW, ndn = nt * st.nstep * NDN * TW;
-            if (hipMalloc(&r->st_mem, sizeof(double) * (2 * nrec + ndn)) != hipSuccess)
-                return fail(c, FF_E_NOMEM, "LU-SGS staged records%s");
-            st.recp = r->st_mem;
-            st.recc = st.recp + nrec;
-            st.dn = st.recc + nrec;
-        }
+        if ((rc = staged_alloc(c, r))) return rc;
         dim3 g(r->sw.ntj * r->sw.ntk);
         const TileGrid tp = tile_grid((r->st.nstep + 3) / 4, r->sw.ntj * r->sw.ntk, 1);
         dim3 gp(NXCD * tp.per);
@@ -4361,10 +4381,43 @@ int ff_prolong(ff_ctx *c, int mcoarse, int block)
     return prolong_impl(c, cr, fr, mcoarse - 1);
 }
 
-int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
+// One V-cycle (CYCLES ns3c.f:1450-2137 with MGRID = nlev): the finest-level iteration,
+// the FAS down-loop with pre-smoothing, the prolongation; RESL2 on level 1 into rs if given
+static int vcycle_once(ff_ctx *c, const std::vector<BlockRec *> &lv, int nlev, int block, double *rs)
 {
-    if (!c || ncycles < 0 || nlev < 1) return FF_E_ARG;
-    std::vector<BlockRec *> lv(nlev);
+    int rc;
+    BlockRec *r1 = lv[0];
+    // level 1: CYCLES ns3c.f:1450-1596
+    c->lev = 1;
+    if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev))) return rc;
+    if ((rc = ff_state(c, 1, block))) return rc;
+    if ((rc = ff_turb(c, 1, block))) return rc;
+    if ((rc = residual_impl(c, r1, 1, 1, rs))) return rc;
+    if ((rc = implicit_impl(c, r1, 1, true))) return rc;
+    if ((rc = update_impl(c, r1, 1))) return rc;
+    // FAS down-loop, pre-smoothing only (ns3c.f:1607-1999)
+    for (int m = 2; m <= nlev; m++) {
+        BlockRec *fr = lv[m - 2], *cr = lv[m - 1];
+        if ((rc = ff_state(c, m - 1, block))) return rc;
+        if ((rc = residual_impl(c, fr, m - 1, m, m - 1 == 1 ? rs : nullptr))) return rc;
+        if (m > 2 && (rc = forcing_impl(c, fr, 1))) return rc;
+        if ((rc = restrict_impl(c, fr, cr))) return rc;
+        if ((rc = ff_state(c, m, block))) return rc;
+        if ((rc = residual_impl(c, cr, m, m, nullptr))) return rc;
+        if ((rc = forcing_impl(c, cr, 0))) return rc;
+        if ((rc = forcing_impl(c, cr, 1))) return rc;
+        if ((rc = implicit_impl(c, cr, m, true))) return rc;
+        if ((rc = update_impl(c, cr, m))) return rc;
+    }
+    // prolongation (ns3c.f:2005-2137)
+    for (int m = nlev; m >= 2; m--)
+        if ((rc = prolong_impl(c, lv[m - 1], lv[m - 2], m - 1))) return rc;
+    return FF_OK;
+}
+
+static int vcycle_levels(ff_ctx *c, int block, int nlev, std::vector<BlockRec *> &lv)
+{
+    lv.assign(nlev, nullptr);
     for (int m = 1; m <= nlev; m++) {
         lv[m - 1] = find(c, m, block);
         if (!lv[m - 1]) return fail(c, FF_E_STATE, "multigrid level not registered%s");
@@ -4373,36 +4426,68 @@ int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
             if (rc) return rc;
         }
     }
-    int rc;
-    for (int it = 0; it < ncycles; it++) {
-        double *rs = it == ncycles - 1 ? resi : nullptr;
-        BlockRec *r1 = lv[0];
-        // level 1: CYCLES ns3c.f:1450-1596
-        c->lev = 1;
-        if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev))) return rc;
-        if ((rc = ff_state(c, 1, block))) return rc;
-        if ((rc = ff_turb(c, 1, block))) return rc;
-        if ((rc = residual_impl(c, r1, 1, 1, rs))) return rc;
-        if ((rc = implicit_impl(c, r1, 1, true))) return rc;
-        if ((rc = update_impl(c, r1, 1))) return rc;
-        // FAS down-loop, pre-smoothing only (ns3c.f:1607-1999)
-        for (int m = 2; m <= nlev; m++) {
-            BlockRec *fr = lv[m - 2], *cr = lv[m - 1];
-            if ((rc = ff_state(c, m - 1, block))) return rc;
-            if ((rc = residual_impl(c, fr, m - 1, m, m - 1 == 1 ? rs : nullptr))) return rc;
-            if (m > 2 && (rc = forcing_impl(c, fr, 1))) return rc;
-            if ((rc = restrict_impl(c, fr, cr))) return rc;
-            if ((rc = ff_state(c, m, block))) return rc;
-            if ((rc = residual_impl(c, cr, m, m, nullptr))) return rc;
-            if ((rc = forcing_impl(c, cr, 0))) return rc;
-            if ((rc = forcing_impl(c, cr, 1))) return rc;
-            if ((rc = implicit_impl(c, cr, m, true))) return rc;
-            if ((rc = update_impl(c, cr, m))) return rc;
+    return FF_OK;
+}
+
+int ff_vcycle(ff_ctx *c, int block, int nlev, int ncycles, double *resi)
+{
+    if (!c || ncycles < 0 || nlev < 1) return FF_E_ARG;
+    std::vector<BlockRec *> lv;
+    int rc = vcycle_levels(c, block, nlev, lv);
+    if (rc) return rc;
+    for (int it = 0; it < ncycles; it++)
+        if ((rc = vcycle_once(c, lv, nlev, block, it == ncycles - 1 ? resi : nullptr))) return rc;
+    if (c->lusgs_mode != FF_LUSGS_PLANES)
+        for (int m = 0; m < nlev; m++)
+            if ((rc = sweep_check(c, *lv[m]))) return rc;
+    return FF_OK;
+}
+
+static void vgraph_free(ff_ctx *c)
+{
+    if (c->vg_exec) (void)hipGraphExecDestroy(c->vg_exec);
+    if (c->vg_graph) (void)hipGraphDestroy(c->vg_graph);
+    c->vg_exec = nullptr;
+    c->vg_graph = nullptr;
+}
+
+int ff_vcycle_graph(ff_ctx *c, int block, int nlev, int ncycles)
+{
+    if (!c || ncycles < 0 || nlev < 1) return FF_E_ARG;
+    std::vector<BlockRec *> lv;
+    int rc = vcycle_levels(c, block, nlev, lv);
+    if (rc) return rc;
+    if (c->profile) return fail(c, FF_E_STATE, "ff_vcycle_graph: per-kernel timing is on%s");
+    if (!c->stream) return fail(c, FF_E_STATE, "ff_vcycle_graph: the null stream cannot be captured%s");
+    for (BlockRec *r : lv)
+        if (!r->walls.dev.empty())
+            return fail(c, FF_E_UNSUP, "ff_vcycle_graph: blocks with wall patches use ff_vcycle%s");
+    const long long key[6] = {block, nlev, c->lusgs_mode, c->residual_mode, c->earsm_mode, c->gen};
+    bool same = c->vg_exec != nullptr;
+    for (int q = 0; q < 6; q++) same = same && c->vg_key[q] == key[q];
+    if (!same) {
+        vgraph_free(c);
+        if (c->lusgs_mode == FF_LUSGS_STAGED)
+            for (BlockRec *r : lv)
+                if ((rc = staged_alloc(c, r))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        rc = vcycle_once(c, lv, nlev, block, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
         }
-        // prolongation (ns3c.f:2005-2137)
-        for (int m = nlev; m >= 2; m--)
-            if ((rc = prolong_impl(c, lv[m - 1], lv[m - 2], m - 1))) return rc;
+        if (ec != hipSuccess || !g) return fail(c, FF_E_HIP, "%s", hipGetErrorString(ec));
+        c->vg_graph = g;
+        if (hipGraphInstantiate(&c->vg_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            vgraph_free(c);
+            return fail(c, FF_E_HIP, "ff_vcycle_graph: instantiation failed%s");
+        }
+        for (int q = 0; q < 6; q++) c->vg_key[q] = key[q];
     }
+    for (int it = 0; it < ncycles; it++) HIPCHK(c, hipGraphLaunch(c->vg_exec, c->stream));
     if (c->lusgs_mode != FF_LUSGS_PLANES)
         for (int m = 0; m < nlev; m++)
             if ((rc = sweep_check(c, *lv[m]))) return rc;
@@ -4427,6 +4512,7 @@ static void wall_arrays(const ff_wall_patch &h, double *a[NWARR])
 int ff_wall_register(ff_ctx *c, int level, int block, const ff_wall_params *wpar,
                      const ff_wall_patch *patches, int npatch)
 {
+    if (c) c->gen++;
     if (!c || npatch < 0 || (npatch > 0 && (!patches || !wpar))) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");

@@ -21,7 +21,7 @@ EXPORTS = [
     "ff_kernel_stats", "ff_kernel_stats_reset", "ff_block_register",
     "ff_block_register_device", "ff_sync", "ff_finish", "ff_state", "ff_turb", "ff_residual",
     "ff_implicit", "ff_update", "ff_save_old", "ff_cycle", "ff_restrict", "ff_forcing",
-    "ff_prolong", "ff_vcycle", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
+    "ff_prolong", "ff_vcycle", "ff_vcycle_graph", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
     "ff_mlp_grad", "ff_mlp_train",
 ]
@@ -68,6 +68,7 @@ def lib():
         L.ff_forcing.argtypes = [vp, i, i, i]
         L.ff_prolong.argtypes = [vp, i, i]
         L.ff_vcycle.argtypes = [vp, i, i, i, vp]
+        L.ff_vcycle_graph.argtypes = [vp, i, i, i]
         L.ff_wall_register.argtypes = [vp, i, i, C.POINTER(WallParams), vp, i]
         L.ff_wall_sync.argtypes = [vp, i, i, i]
         L.ff_wall_forces.argtypes = [vp, i, i, vp]
@@ -195,6 +196,10 @@ class Context:
         resi = np.zeros(7)
         self._chk(lib().ff_vcycle(self.h, block, nlev, ncycles, resi.ctypes.data), "ff_vcycle")
         return resi
+
+    def vcycle_graph(self, nlev, ncycles=1, block=1):
+        """ncycles V-cycles replayed from one captured HIP graph (no residual sums)."""
+        self._chk(lib().ff_vcycle_graph(self.h, block, nlev, ncycles), "ff_vcycle_graph")
 
     # ---- solid-wall patches (BOTSKE) ---------------------------------------------------------
     def wall_register(self, wparams, patches, level=1, block=1):

@@ -258,6 +258,13 @@ int  ff_prolong(ff_ctx *ctx, int mcoarse, int block);
  * down-loop and the prolongation.  resi receives the RESL2 sums of the last level-1
  * residual (the one evaluated after the update when nlev >= 2). */
 int  ff_vcycle(ff_ctx *ctx, int block, int nlev, int ncycles, double *resi);
+/* ff_vcycle without residual sums, as one HIP graph: the first call captures one V-cycle
+ * of the context's stream (every kernel of ff_vcycle, same launches, same results bit for
+ * bit) and each call replays it ncycles times; nlev = 1 is the iteration of ff_cycle.  A registration or option change makes the
+ * next call capture again.  Not for blocks with wall patches (FF_E_UNSUP), with per-kernel
+ * timing on, or on the null stream (FF_E_STATE).  No counterpart in the reference: CYCLES
+ * (ns3c.f:1400-2137) launches nothing; this replaces the per-stage launches of ff_vcycle. */
+int  ff_vcycle_graph(ff_ctx *ctx, int block, int nlev, int ncycles);
 /* ncycles device-resident iterations of the finest-level sequence of CYCLES for block N:
  * save_old, STATIM, TURBDI, RESIDI, IMPSDI, FIN + limits.  Ghost cells are left as given
  * (the caller's boundary routines own them).  resi (may be NULL) receives the RESL2 sums

@@ -183,3 +183,53 @@ def test_vcycles_walls(hip_device, iturb):
         np.testing.assert_allclose(ctx.wall_forces(l), f0, rtol=1e-10 if iturb else 0,
                                    atol=1e-10 * np.abs(f0).max() if iturb else 0)
     ctx.close()
+
+
+@pytest.mark.parametrize("nlev", [1, 3])
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_vcycle_graph_replays(hip_device, iturb, nlev):
+    """The captured V-cycle against eager V-cycles of a twin context, bit for bit in every
+    field of every level after each replay, with eager cycles, downloads and an option change
+    (which captures again) between the replays."""
+    print(f"case V-cycle graph iturb={iturb} nlev={nlev}", flush=True)
+    prm, blocks = hierarchy(iturb, nlev=nlev, dims=(16, 8, 12), seed=79)
+    twin = [b.copy() for b in blocks]
+    cg, ce = context(prm, blocks), context(prm, twin)
+
+    def check(what):
+        for m in range(nlev):
+            cg.download(level=m + 1)
+            ce.download(level=m + 1)
+            np.testing.assert_array_equal(blocks[m].F, twin[m].F, err_msg=f"{what} level {m + 1}")
+
+    for rep in range(3):
+        ce.vcycle(nlev)
+        cg.vcycle_graph(nlev)
+        check(f"replay {rep}")
+    # eager work on the same context between replays
+    ce.vcycle(nlev, ncycles=2)
+    cg.vcycle(nlev, ncycles=2)
+    check("eager cycles")
+    ce.vcycle(nlev, ncycles=2)
+    cg.vcycle_graph(nlev, ncycles=2)
+    check("two replays in one call")
+    for mode in ("tiles", "staged"):
+        ce.lusgs_mode(mode)
+        cg.lusgs_mode(mode)
+        for rep in range(2):
+            ce.vcycle(nlev)
+            cg.vcycle_graph(nlev)
+            check(f"LU-SGS {mode} replay {rep}")
+    cg.close()
+    ce.close()
+
+
+def test_vcycle_graph_refuses_walls(hip_device):
+    from finflo_amd.ffhip import FFError
+    prm, blocks = hierarchy(0, nlev=2, dims=(16, 8, 8), seed=81)
+    ctx = context(prm, blocks)
+    wp, patches = make_walls(blocks[0], prm, [(2, 1, blocks[0].imax, 1, blocks[0].kmax, 0)], 5)
+    ctx.wall_register(wp, patches, level=1)
+    with pytest.raises(FFError, match="wall patches"):
+        ctx.vcycle_graph(2)
+    ctx.close()
