@@ -2595,28 +2595,36 @@ __device__ __forceinline__ bool half_empty(unsigned long long x)
     return (unsigned)x == (unsigned)HALO_EMPTY || (unsigned)(x >> 32) == (unsigned)(HALO_EMPTY >> 32);
 }
 
-// The halo words of one lane from its LDS copy (word q at lw[q * TK]); words copied while
-// still empty are polled from the slot.  The staged sweeps leave their slots filled: a fill
-// kernel empties all of them after the sweep, which costs less than a device-coherent
-// store per word on the wavefront's critical path.
-template <int NV>
-__device__ __forceinline__ void halo_take(const double *lw, unsigned long long *slot,
-                                          unsigned int *err, double v[NV])
+// The NV words a lane takes across one tile face at a step, all from LDS: the in-tile
+// neighbour's output of the previous step (words TW apart), the upstream tile's halo block
+// (words TK apart) or a zero block at the block edge (words TK apart).  One address and
+// stride per lane replace a branch per source.  A halo word copied while still empty
+// (either 32-bit half equal to the marker's) is polled from its slot (slotf gives the
+// address) in one branch that is rarely taken; marker halves in words from the other two
+// sources are ordinary values and are not polled.  The staged sweeps leave their slots
+// filled: a fill kernel empties all of them after the sweep, which costs less than a
+// device-coherent store per word on the wavefront's critical path.
+template <int NV, typename SlotF>
+__device__ __forceinline__ void face_in(const double *src, int stride, bool halo, SlotF slotf,
+                                        unsigned int *err, double v[NV])
 {
     unsigned long long h[NV];
-    bool empty = false;
+    unsigned hit = 0xffffffffu;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-        h[q] = (unsigned long long)__double_as_longlong(lw[q * TK]);
-        if (half_empty(h[q])) {
-            h[q] = HALO_EMPTY;
-            empty = true;
-        }
+        h[q] = (unsigned long long)__double_as_longlong(src[q * stride]);
+        const unsigned lo = (unsigned)h[q] ^ (unsigned)HALO_EMPTY;
+        const unsigned hi = (unsigned)(h[q] >> 32) ^ (unsigned)(HALO_EMPTY >> 32);
+        hit = min(hit, min(lo, hi));
     }
-    if (empty) {
+    const bool poll = halo && hit == 0;
+    // a wave-uniform test first, so that the common case skips the region in one branch
+    if (__builtin_amdgcn_ballot_w64(poll) && poll) {
+        unsigned long long *slot = slotf();
 #pragma unroll
         for (int q = 0; q < NV; q++) {
-            unsigned long long x = h[q];
+            if (!half_empty(h[q])) continue;
+            unsigned long long x = HALO_EMPTY;
             int spins = 0;
             while (x == HALO_EMPTY) {
                 if ((++spins & 255) == 0) {
@@ -2630,8 +2638,8 @@ __device__ __forceinline__ void halo_take(const double *lw, unsigned long long *
                         break;
                     }
                 }
-                __builtin_amdgcn_s_sleep(1);
                 x = __hip_atomic_load(slot + q * HQS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (x == HALO_EMPTY) __builtin_amdgcn_s_sleep(1);
             }
             h[q] = x;
         }
@@ -2653,8 +2661,10 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
     static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
     __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
     __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
-    __shared__ double xj[2][NV][TW], xk[2][NV][TW];
+    __shared__ double xj[2][NV][TW], xk[2][NV][TW], zx[NV * TK];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
+    if (lane < NV * TK) zx[lane] = 0.;
+    wave_step_sync();
     const int t = tile_ticket(sw);
     const int tj = sw.ntj - 1 - t % sw.ntj, tk = sw.ntk - 1 - t / sw.ntj;
     const int tile = tk * sw.ntj + tj;
@@ -2690,23 +2700,17 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
             const int i = b.imax - q;
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
-            if (j == b.jmax) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = 0.;
-            } else if (jj < TJ - 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane + 1];
-            } else {
-                halo_take<NV>(hring[s % HR][0] + kk, hj_slot(sw, b, tj + 1, i, k), sw.err, bj);
+            {
+                const bool edge = j == b.jmax, in = jj < TJ - 1;
+                const double *src = edge ? zx + kk : in ? &xj[rb][0][lane + 1] : hring[s % HR][0] + kk;
+                face_in<NV>(src, !edge && in ? TW : TK, !edge && !in,
+                            [&] { return hj_slot(sw, b, tj + 1, i, k); }, sw.err, bj);
             }
-            if (k == b.kmax) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) ckc[r] = 0.;
-            } else if (kk < TK - 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane + TJ];
-            } else {
-                halo_take<NV>(hring[s % HR][1] + jj, hk_slot(sw, b, tk + 1, i, j), sw.err, ckc);
+            {
+                const bool edge = k == b.kmax, in = kk < TK - 1;
+                const double *src = edge ? zx + jj : in ? &xk[rb][0][lane + TJ] : hring[s % HR][1] + jj;
+                face_in<NV>(src, !edge && in ? TW : TJ, !edge && !in,
+                            [&] { return hk_slot(sw, b, tk + 1, i, j); }, sw.err, ckc);
             }
             // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
             // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
@@ -2742,19 +2746,15 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
             face_xfer<KE>(dn, sc, basis_get(bv), ci, yi);
             face_xfer<KE>(dn, sc, basis_get(bv + 9), cj, oj);
             face_xfer<KE>(dn, sc, basis_get(bv + 18), ck, ok);
+            // every lane leaves its outputs in LDS (an edge lane's are never read there)
             const int wb = s & 1;
-            if (jj > 0) {
 #pragma unroll
-                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
-            } else if (j > 1) {
-                halo_put<NV, HQS>(hj_slot(sw, b, tj, i, k), oj);
+            for (int r = 0; r < NV; r++) {
+                xj[wb][r][lane] = oj[r];
+                xk[wb][r][lane] = ok[r];
             }
-            if (kk > 0) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
-            } else if (k > 1) {
-                halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), ok);
-            }
+            if (jj == 0 && j > 1) halo_put<NV, HQS>(hj_slot(sw, b, tj, i, k), oj);
+            if (kk == 0 && k > 1) halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), ok);
         }
         // the LDS reads of this step are complete before its slots are refilled
         wave_step_sync();
@@ -2779,8 +2779,10 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
     __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
     __shared__ __attribute__((aligned(16))) double dring[RS][NDN * TW];
     __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
-    __shared__ double xj[2][NV][TW], xk[2][NV][TW];
+    __shared__ double xj[2][NV][TW], xk[2][NV][TW], zx[NV * TK];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
+    if (lane < NV * TK) zx[lane] = 0.;
+    wave_step_sync();
     const int t = tile_ticket(sw);
     const int tj = t % sw.ntj, tk = t / sw.ntj;
     const int tile = tk * sw.ntj + tj;
@@ -2816,23 +2818,17 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             const int n = cid(b, i, j, k);
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
-            if (j == 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = 0.;
-            } else if (jj > 0) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) bj[r] = xj[rb][r][lane - 1];
-            } else {
-                halo_take<NV>(hring[s % HR][0] + kk, hj_slot(sw, b, tj, i, k), sw.err, bj);
+            {
+                const bool edge = j == 1, in = jj > 0;
+                const double *src = edge ? zx + kk : in ? &xj[rb][0][lane - 1] : hring[s % HR][0] + kk;
+                face_in<NV>(src, !edge && in ? TW : TK, !edge && !in,
+                            [&] { return hj_slot(sw, b, tj, i, k); }, sw.err, bj);
             }
-            if (k == 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) ckc[r] = 0.;
-            } else if (kk > 0) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) ckc[r] = xk[rb][r][lane - TJ];
-            } else {
-                halo_take<NV>(hring[s % HR][1] + jj, hk_slot(sw, b, tk, i, j), sw.err, ckc);
+            {
+                const bool edge = k == 1, in = kk > 0;
+                const double *src = edge ? zx + jj : in ? &xk[rb][0][lane - TJ] : hring[s % HR][1] + jj;
+                face_in<NV>(src, !edge && in ? TW : TJ, !edge && !in,
+                            [&] { return hk_slot(sw, b, tk, i, j); }, sw.err, ckc);
             }
             double d[NV];
 #pragma unroll
@@ -2869,19 +2865,15 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             face_xfer<KE>(dn, sc, basis_get(bv), ci, yi);
             face_xfer<KE>(dn, sc, basis_get(bv + 9), cj, oj);
             face_xfer<KE>(dn, sc, basis_get(bv + 18), ck, ok);
+            // every lane leaves its outputs in LDS (an edge lane's are never read there)
             const int wb = s & 1;
-            if (jj < TJ - 1) {
 #pragma unroll
-                for (int r = 0; r < NV; r++) xj[wb][r][lane] = oj[r];
-            } else if (j < b.jmax) {
-                halo_put<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), oj);
+            for (int r = 0; r < NV; r++) {
+                xj[wb][r][lane] = oj[r];
+                xk[wb][r][lane] = ok[r];
             }
-            if (kk < TK - 1) {
-#pragma unroll
-                for (int r = 0; r < NV; r++) xk[wb][r][lane] = ok[r];
-            } else if (k < b.kmax) {
-                halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
-            }
+            if (jj == TJ - 1 && j < b.jmax) halo_put<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), oj);
+            if (kk == TK - 1 && k < b.kmax) halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), ok);
         }
         // the LDS reads of this step are complete before its slots are refilled
         wave_step_sync();
