@@ -2648,6 +2648,27 @@ __device__ __forceinline__ void face_in(const double *src, int stride, bool halo
     for (int q = 0; q < NV; q++) v[q] = __longlong_as_double((long long)h[q]);
 }
 
+// Where a lane takes one face's inputs from: a lane-constant LDS base (its neighbour's
+// column of the exchange buffers, its column of the halo ring, or the zero block) to which
+// each step adds the uniform offset of the exchange buffer or of the halo ring slot under a
+// lane mask, so that the choice costs no branch.
+struct FaceSrc {
+    const char *base;
+    unsigned xmask, hmask;       // all ones on in-tile lanes / on halo lanes
+    int stride;                  // doubles between words
+    __device__ FaceSrc(bool edge, bool in, const double *x, const double *h, const double *z)
+    {
+        base = (const char *)(edge ? z : in ? x : h);
+        xmask = !edge && in ? ~0u : 0u;
+        hmask = !edge && !in ? ~0u : 0u;
+        stride = !edge && in ? TW : TK;
+    }
+    __device__ const double *at(unsigned xoff, unsigned hoff) const
+    {
+        return (const double *)(base + ((xoff & xmask) | (hoff & hmask)));
+    }
+};
+
 // PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles.  Issue order
 // within step s: the step's stores, the halo blocks of step s + 2, the record of step s + 3.
 // At the top of step s at least the two later record copies (56 loads) follow the halo
@@ -2674,15 +2695,18 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
     const int lag = (TJ - 1 - jj) + (TK - 1 - kk);
     const int nsteps = st.nstep;
     // the edge lanes of step s read anti-diagonal d = IMAX + 6 - s of the upstream faces
+    const unsigned long long *hj0 = hj_blk(sw, b, tj + 1, tk, 0), *hk0 = hk_blk(sw, b, tk + 1, tj, 0);
     auto halo_copy = [&](int s) {
         const int d = b.imax + TK - 2 - s;
         if (d < 0 || d > b.imax + TK - 2) return;
-        if (hasj) halo_dma(hj_blk(sw, b, tj + 1, tk, d), hring[s % HR][0], lane);
-        if (hask) halo_dma(hk_blk(sw, b, tk + 1, tj, d), hring[s % HR][1], lane);
+        if (hasj) halo_dma(hj0 + (size_t)d * HBLK, hring[s % HR][0], lane);
+        if (hask) halo_dma(hk0 + (size_t)d * HBLK, hring[s % HR][1], lane);
     };
     auto rec_copy = [&](int s) {
         ring_copy<RBYTES>(st.recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
     };
+    const FaceSrc fj(j == b.jmax, jj < TJ - 1, &xj[0][0][min(lane + 1, TW - 1)], hring[0][0] + kk, zx + kk);
+    const FaceSrc fk(k == b.kmax, kk < TK - 1, &xk[0][0][min(lane + TJ, TW - 1)], hring[0][1] + jj, zx + jj);
     double yi[NV];
 #pragma unroll
     for (int r = 0; r < NV; r++) yi[r] = 0.;
@@ -2700,18 +2724,11 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
             const int i = b.imax - q;
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
-            {
-                const bool edge = j == b.jmax, in = jj < TJ - 1;
-                const double *src = edge ? zx + kk : in ? &xj[rb][0][lane + 1] : hring[s % HR][0] + kk;
-                face_in<NV>(src, !edge && in ? TW : TK, !edge && !in,
-                            [&] { return hj_slot(sw, b, tj + 1, i, k); }, sw.err, bj);
-            }
-            {
-                const bool edge = k == b.kmax, in = kk < TK - 1;
-                const double *src = edge ? zx + jj : in ? &xk[rb][0][lane + TJ] : hring[s % HR][1] + jj;
-                face_in<NV>(src, !edge && in ? TW : TJ, !edge && !in,
-                            [&] { return hk_slot(sw, b, tk + 1, i, j); }, sw.err, ckc);
-            }
+            const unsigned xoff = rb * (NV * TW * 8), hoff = (s % HR) * (2 * HBLK * 8);
+            face_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
+                        [&] { return hj_slot(sw, b, tj + 1, i, k); }, sw.err, bj);
+            face_in<NV>(fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
+                        [&] { return hk_slot(sw, b, tk + 1, i, j); }, sw.err, ckc);
             // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
             // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
             if (!KE && j == b.jmax) {
@@ -2792,11 +2809,14 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
     const int lag = jj + kk;
     const int nsteps = st.nstep;
     // the edge lanes of step s read anti-diagonal d = s of the upstream faces
+    const unsigned long long *hj0 = hj_blk(sw, b, tj, tk, 0), *hk0 = hk_blk(sw, b, tk, tj, 0);
     auto halo_copy = [&](int s) {
         if (s > b.imax + TK - 2) return;
-        if (hasj) halo_dma(hj_blk(sw, b, tj, tk, s), hring[s % HR][0], lane);
-        if (hask) halo_dma(hk_blk(sw, b, tk, tj, s), hring[s % HR][1], lane);
+        if (hasj) halo_dma(hj0 + (size_t)s * HBLK, hring[s % HR][0], lane);
+        if (hask) halo_dma(hk0 + (size_t)s * HBLK, hring[s % HR][1], lane);
     };
+    const FaceSrc fj(j == 1, jj > 0, &xj[0][0][max(lane - 1, 0)], hring[0][0] + kk, zx + kk);
+    const FaceSrc fk(k == 1, kk > 0, &xk[0][0][max(lane - TJ, 0)], hring[0][1] + jj, zx + jj);
     auto rec_copy = [&](int s) {
         ring_copy<RBYTES>(st.recc + st_row(st, tile, s, 0), ring[s % RS], lane);
         ring_copy<DBYTES>(st.dn + (((size_t)tile * nsteps + s) * NDN) * TW, dring[s % RS], lane);
@@ -2818,18 +2838,11 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             const int n = cid(b, i, j, k);
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
-            {
-                const bool edge = j == 1, in = jj > 0;
-                const double *src = edge ? zx + kk : in ? &xj[rb][0][lane - 1] : hring[s % HR][0] + kk;
-                face_in<NV>(src, !edge && in ? TW : TK, !edge && !in,
-                            [&] { return hj_slot(sw, b, tj, i, k); }, sw.err, bj);
-            }
-            {
-                const bool edge = k == 1, in = kk > 0;
-                const double *src = edge ? zx + jj : in ? &xk[rb][0][lane - TJ] : hring[s % HR][1] + jj;
-                face_in<NV>(src, !edge && in ? TW : TJ, !edge && !in,
-                            [&] { return hk_slot(sw, b, tk, i, j); }, sw.err, ckc);
-            }
+            const unsigned xoff = rb * (NV * TW * 8), hoff = (s % HR) * (2 * HBLK * 8);
+            face_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
+                        [&] { return hj_slot(sw, b, tj, i, k); }, sw.err, bj);
+            face_in<NV>(fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
+                        [&] { return hk_slot(sw, b, tk, i, j); }, sw.err, ckc);
             double d[NV];
 #pragma unroll
             for (int r = 0; r < NV; r++) d[r] = dni[r * TW] * cur[RC_FAC * TW];
