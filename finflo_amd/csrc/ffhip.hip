@@ -2598,17 +2598,43 @@ __device__ __forceinline__ bool half_empty(unsigned long long x)
 // The NV words a lane takes across one tile face at a step, all from LDS: the in-tile
 // neighbour's output of the previous step (words TW apart), the upstream tile's halo block
 // (words TK apart) or a zero block at the block edge (words TK apart).  One address and
-// stride per lane replace a branch per source.  A halo word copied while still empty
-// (either 32-bit half equal to the marker's) is polled from its slot (slotf gives the
-// address) in one branch that is rarely taken; marker halves in words from the other two
-// sources are ordinary values and are not polled.  The staged sweeps leave their slots
+// stride per lane replace a branch per source.  marker_hit reads the words and tests them;
+// halo_poll polls a halo word copied while still empty (either 32-bit half equal to the
+// marker's) from its slot, in a branch that is rarely taken.  Marker halves in words from
+// the other two sources are ordinary values and are not polled.  The staged sweeps leave their slots
 // filled: a fill kernel empties all of them after the sweep, which costs less than a
 // device-coherent store per word on the wavefront's critical path.
-template <int NV, typename SlotF>
-__device__ __forceinline__ void face_in(const double *src, int stride, bool halo, SlotF slotf,
-                                        unsigned int *err, double v[NV])
+template <int NV>
+__device__ __forceinline__ void halo_poll(unsigned long long *slot, unsigned int *err,
+                                          unsigned long long h[NV])
 {
-    unsigned long long h[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        if (!half_empty(h[q])) continue;
+        unsigned long long x = HALO_EMPTY;
+        int spins = 0;
+        while (x == HALO_EMPTY) {
+            if ((++spins & 255) == 0) {
+                if (spins > SPIN_LIMIT) {
+                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    x = 0;
+                    break;
+                }
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    x = 0;
+                    break;
+                }
+            }
+            x = __hip_atomic_load(slot + q * HQS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (x == HALO_EMPTY) __builtin_amdgcn_s_sleep(1);
+        }
+        h[q] = x;
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ unsigned marker_hit(const double *src, int stride, unsigned long long h[NV])
+{
     unsigned hit = 0xffffffffu;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
@@ -2617,35 +2643,30 @@ __device__ __forceinline__ void face_in(const double *src, int stride, bool halo
         const unsigned hi = (unsigned)(h[q] >> 32) ^ (unsigned)(HALO_EMPTY >> 32);
         hit = min(hit, min(lo, hi));
     }
-    const bool poll = halo && hit == 0;
-    // a wave-uniform test first, so that the common case skips the region in one branch
-    if (__builtin_amdgcn_ballot_w64(poll) && poll) {
-        unsigned long long *slot = slotf();
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            if (!half_empty(h[q])) continue;
-            unsigned long long x = HALO_EMPTY;
-            int spins = 0;
-            while (x == HALO_EMPTY) {
-                if ((++spins & 255) == 0) {
-                    if (spins > SPIN_LIMIT) {
-                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        x = 0;
-                        break;
-                    }
-                    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        x = 0;
-                        break;
-                    }
-                }
-                x = __hip_atomic_load(slot + q * HQS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (x == HALO_EMPTY) __builtin_amdgcn_s_sleep(1);
-            }
-            h[q] = x;
-        }
+    return hit;
+}
+
+// Both faces' inputs of a step (J into vj, K into vk): all 2 x NV LDS reads are issued
+// together, and one wave-uniform vote guards both polling regions.
+template <int NV, typename SlotJ, typename SlotK>
+__device__ __forceinline__ void faces_in(const double *sj, int strj, bool haloj, SlotJ slotj,
+                                         const double *sk, int strk, bool halok, SlotK slotk,
+                                         unsigned int *err, double vj[NV], double vk[NV])
+{
+    unsigned long long hj[NV], hk[NV];
+    const unsigned hitj = marker_hit<NV>(sj, strj, hj);
+    const unsigned hitk = marker_hit<NV>(sk, strk, hk);
+    const bool pj = haloj && hitj == 0, pk = halok && hitk == 0;
+    // a wave-uniform test first, so that the common case skips both regions in one branch
+    if (__builtin_amdgcn_ballot_w64(pj || pk)) {
+        if (pj) halo_poll<NV>(slotj(), err, hj);
+        if (pk) halo_poll<NV>(slotk(), err, hk);
     }
 #pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = __longlong_as_double((long long)h[q]);
+    for (int q = 0; q < NV; q++) {
+        vj[q] = __longlong_as_double((long long)hj[q]);
+        vk[q] = __longlong_as_double((long long)hk[q]);
+    }
 }
 
 // Where a lane takes one face's inputs from: a lane-constant LDS base (its neighbour's
@@ -2725,10 +2746,10 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
             const unsigned xoff = rb * (NV * TW * 8), hoff = (s % HR) * (2 * HBLK * 8);
-            face_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
-                        [&] { return hj_slot(sw, b, tj + 1, i, k); }, sw.err, bj);
-            face_in<NV>(fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
-                        [&] { return hk_slot(sw, b, tk + 1, i, j); }, sw.err, ckc);
+            faces_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
+                         [&] { return hj_slot(sw, b, tj + 1, i, k); },
+                         fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
+                         [&] { return hk_slot(sw, b, tk + 1, i, j); }, sw.err, bj, ckc);
             // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
             // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
             if (!KE && j == b.jmax) {
@@ -2839,10 +2860,10 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             const int rb = (s + 1) & 1;
             double bj[NV], ckc[NV];
             const unsigned xoff = rb * (NV * TW * 8), hoff = (s % HR) * (2 * HBLK * 8);
-            face_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
-                        [&] { return hj_slot(sw, b, tj, i, k); }, sw.err, bj);
-            face_in<NV>(fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
-                        [&] { return hk_slot(sw, b, tk, i, j); }, sw.err, ckc);
+            faces_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
+                         [&] { return hj_slot(sw, b, tj, i, k); },
+                         fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
+                         [&] { return hk_slot(sw, b, tk, i, j); }, sw.err, bj, ckc);
             double d[NV];
 #pragma unroll
             for (int r = 0; r < NV; r++) d[r] = dni[r * TW] * cur[RC_FAC * TW];
