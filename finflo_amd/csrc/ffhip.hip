@@ -4008,6 +4008,20 @@ int ff_finish(ff_ctx *c)
 }
 
 static inline dim3 cells_grid(const Blk &b, int tpb) { return dim3((b.imax + tpb - 1) / tpb, b.jmax, b.kmax); }
+// Threads per workgroup of a launch over rows of IMAX cells: of maxtpb, maxtpb/2, ... 64
+// the size that leaves the fewest idle lanes in the last workgroup of a row (the larger on
+// a tie); IMAX = 192 takes 64 (three full waves) where 128 would leave a quarter idle.
+// Used where it measured faster (TIME3 + COPR); the light per-cell kernels ran slower in
+// 64-thread workgroups than with the idle lanes of 256.
+static inline int row_tpb(const Blk &b, int maxtpb)
+{
+    int best = maxtpb, idle = (b.imax + maxtpb - 1) / maxtpb * maxtpb - b.imax;
+    for (int t = maxtpb / 2; t >= 64; t /= 2) {
+        const int w = (b.imax + t - 1) / t * t - b.imax;
+        if (w < idle) best = t, idle = w;
+    }
+    return best;
+}
 static inline dim3 all_grid(const Blk &b) { return dim3((b.ntot + 255) / 256); }
 
 int ff_state(ff_ctx *c, int level, int block)
@@ -4195,7 +4209,7 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
     const Blk &b = r->dev;
     c->lev = level;
     Prm kp = make_prm(&c->prm, level);
-    int rc = launch(c, "k_time3_copr", k_time3_copr, cells_grid(b, 128), dim3(128), kp, b);
+    int rc = launch(c, "k_time3_copr", k_time3_copr, cells_grid(b, row_tpb(b, 128)), dim3(row_tpb(b, 128)), kp, b);
     if (rc) return rc;
     if (c->lusgs_mode == FF_LUSGS_KMARCH) {
         // DRO/CP is applied inside the forward sweep
