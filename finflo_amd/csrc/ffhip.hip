@@ -2465,16 +2465,18 @@ __device__ __forceinline__ FaceBasis basis_get(const double *v)
 // workgroup, so that a workgroup's loads of a column run along I and its record stores are
 // whole rows.  The records are read back only by the sweeps, long after: non-temporal
 // stores keep them from displacing the caches' working set on the way out.
+constexpr int PSW = 4;             // record-pass steps per workgroup (one wave each)
+
 template <bool KE>
-__global__ void __launch_bounds__(4 * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st,
-                                                      TileGrid tg)
+__global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st,
+                                                        TileGrid tg)
 {
     constexpr int NV = KE ? 7 : 5;
     // logical tiles: (step chunk, tile), chunks of one tile consecutive, so that a tile's
     // columns stay in one XCD's L2 while their cells are read
     const int tl = xcd_tile(tg);
     if (tl < 0) return;
-    const int lane = threadIdx.x % TW, s = (tl % tg.nx) * 4 + threadIdx.x / TW;
+    const int lane = threadIdx.x % TW, s = (tl % tg.nx) * PSW + threadIdx.x / TW;
     if (s >= st.nstep) return;
     const int tile = tl / tg.nx, tj = tile % sw.ntj, tk = tile / sw.ntj;
     const int jj = lane % TJ, kk = lane / TJ;
@@ -4225,11 +4227,11 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         // records of both sweeps in step order, then the two wavefronts on them
         if ((rc = staged_alloc(c, r))) return rc;
         dim3 g(r->sw.ntj * r->sw.ntk);
-        const TileGrid tp = tile_grid((r->st.nstep + 3) / 4, r->sw.ntj * r->sw.ntk, 1);
+        const TileGrid tp = tile_grid((r->st.nstep + PSW - 1) / PSW, r->sw.ntj * r->sw.ntk, 1);
         dim3 gp(NXCD * tp.per);
         const bool ke = c->prm.iturb == 6;
-        rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(4 * TW), kp, b, r->sw, r->st, tp)
-                : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(4 * TW), kp, b, r->sw, r->st, tp);
+        rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(PSW * TW), kp, b, r->sw, r->st, tp)
+                : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(PSW * TW), kp, b, r->sw, r->st, tp);
         if (rc) return rc;
         // each sweep leaves its halo slots filled; empty them for the next one
         const dim3 gf((unsigned)((r->ntile_slots + 255) / 256));
