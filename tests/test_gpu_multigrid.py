@@ -224,6 +224,27 @@ def test_vcycle_graph_replays(hip_device, iturb, nlev):
     ce.close()
 
 
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_vcycle_graph_fullns(hip_device, iturb):
+    """FULLNS V-cycles (TEMDER gradients and FLUXBL/FLKEBL on level 1) replayed from the
+    graph, bit for bit the eager ones of a twin context, with eager cycles between."""
+    print(f"case V-cycle graph FULLNS iturb={iturb} nlev=3", flush=True)
+    prm, blocks = hierarchy(iturb, nlev=3, dims=(16, 8, 12), seed=83, fullns=1)
+    twin = [b.copy() for b in blocks]
+    cg, ce = context(prm, blocks), context(prm, twin)
+    for rep in range(3):
+        ce.vcycle(3, ncycles=rep + 1)
+        cg.vcycle_graph(3, ncycles=rep + 1)
+        for m in range(3):
+            cg.download(level=m + 1)
+            ce.download(level=m + 1)
+            np.testing.assert_array_equal(blocks[m].F, twin[m].F, err_msg=f"replay {rep} level {m + 1}")
+        ce.vcycle(3)
+        cg.vcycle(3)
+    cg.close()
+    ce.close()
+
+
 def test_vcycle_graph_refuses_walls(hip_device):
     from finflo_amd.ffhip import FFError
     prm, blocks = hierarchy(0, nlev=2, dims=(16, 8, 8), seed=81)
