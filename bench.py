@@ -21,7 +21,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from finflo_amd.cases import flat_plate_hierarchy   # noqa: E402
+from finflo_amd.cases import flat_plate_hierarchy, plate_wall   # noqa: E402
 from finflo_amd.layout import default_params        # noqa: E402
 
 IMAX, JMAX, KMAX = 192, 96, 96
@@ -80,6 +80,19 @@ KERNEL_BYTES_PER_CELL = {
         "k_corr_st": 8 * (56 + 7 + 7),
     },
 }
+# Compulsory bytes per cell of the three phases of one finest-level iteration, with S state
+# scalars, G geometry scalars, R unknowns and X stored eddy viscosity per cell (FP64):
+#   P1 residual R(W)                     8 [(S + G) + R + X]
+#   P2 backward sweep (PREDGS + DIAGGS)  8 [(R + S + G + X) + R]
+#   P3 forward sweep (CORRGS) + FIN      8 [(R + S + G + X) + S]
+# laminar S=5 G=16 R=5 X=0 -> 208 + 248 + 248 = 704 B; k-omega S=7 G=17 R=7 X=1 ->
+# 256 + 312 + 312 = 880 B.  A kernel is credited with the bytes of the phase it carries;
+# passes that a fused design would not need (state, record staging, ...) are credited none.
+PHASE_BYTES = {0: (208, 248, 248), 6: (256, 312, 312)}
+PHASE_OF = {"k_residual_kw": 0, "k_residual_kw_ns": 0, "k_residual_tile": 0,
+            "k_residual_tile_ns": 0, "k_residual": 0,
+            "k_pred_st": 1, "k_pred_tiles": 1, "k_pred_km": 1, "k_pred_plane": 1,
+            "k_corr_st": 2, "k_corr_tiles": 2, "k_corr_km": 2, "k_corr_plane": 2}
 ALL_CELL_KERNELS = {"k_state", "k_save_old", "k_limits", "k_limits_mg", "k_forcing"}
 # whole finest-level iteration, every field touched once per phase (residual; backward
 # sweep; forward sweep + update): laminar 88 doubles = 704 B, k-omega 110 doubles = 880 B
@@ -119,12 +132,14 @@ def freestream_turbulence(prm, tu=0.01, mut_ratio=1.0):
 
 
 def make_workload(name="sst-mg", seed=0):
+    """The flat plate of the workload and its no-slip wall (the J-low face of every level)."""
     w = WORKLOADS[name]
     prm = workload_params(w["iturb"], w["reynolds"])
     turb = freestream_turbulence(prm) if w["iturb"] == 6 else None
     blocks = flat_plate_hierarchy(IMAX, JMAX, KMAX, w["nlev"], prm, first=1e-6, ratio=1.15,
                                   seed=seed, amp=0.005, turb=turb)
-    return prm, blocks
+    walls = [plate_wall(b, prm) for b in blocks]
+    return prm, blocks, walls
 
 
 def run_steps(ctx, nlev, n, graph=False):
@@ -137,26 +152,20 @@ def run_steps(ctx, nlev, n, graph=False):
     return eager(n)
 
 
-def cpu_baseline(prm, blocks, seconds=20.0):
-    """Time the CPU path on the host, single thread, on a bounded sample of the same
-    workload: whole V-cycles (or iterations) of the full block.  One level: the reference's
-    own Fortran (oracle/_ref) when built, else the C restatement; multigrid: the C
-    restatement (the reference's multigrid transfers are not compiled here)."""
-    try:
-        from oracle.oracle_py import OracleBlock, OracleHierarchy, RefBlock, have_ref
-    except Exception as e:   # the oracle is optional for the bench
-        return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "none",
-                "sample": f"oracle unavailable: {e}"}
-    bl = [b.copy() for b in blocks]
-    if len(bl) > 1:
-        kind = "port"
-        o = OracleHierarchy(prm, bl)
-        step, what = o.vcycle, "V-cycles"
+def _cpu_worker(prm, blocks, walls, seconds, use_ref, q):
+    """One process of the CPU baseline: whole V-cycles (or iterations) of the workload on
+    this process's copy-on-write view of the blocks until `seconds` have passed."""
+    from oracle.oracle_py import OracleBlock, OracleHierarchy, RefBlock
+    if len(blocks) > 1:
+        o = OracleHierarchy(prm, blocks)
+        for lv, (wp, patches) in zip(o.levels, walls):
+            lv.set_walls(wp, patches)
+        step = o.vcycle
     else:
-        kind = "reference" if have_ref() else "port"
-        o = (RefBlock if have_ref() else OracleBlock)(prm, bl[0])
-        step, what = o.cycle, "iterations"
-    ncell = blocks[0].imax * blocks[0].jmax * blocks[0].kmax
+        o = (RefBlock if use_ref else OracleBlock)(prm, blocks[0])
+        o.set_walls(*walls[0])
+        step = o.cycle
+    step()        # untimed: first touch of this process's copy of the blocks
     n = 0
     t0 = time.perf_counter()
     while True:
@@ -165,10 +174,56 @@ def cpu_baseline(prm, blocks, seconds=20.0):
         el = time.perf_counter() - t0
         if el >= seconds or el / n * (n + 1) > 1.5 * seconds:
             break
+    q.put((n, el))
+
+
+def cpu_baseline(prm, blocks, walls, seconds=20.0):
+    """Time the CPU path on the host cores on a bounded sample of the same workload: whole
+    V-cycles (or iterations) of the full block with its wall.  First one process alone (the
+    per-core rate), then one independent process per available core (at most 16) at once
+    (the whole-host rate, which is `value`).  One level: the reference's own Fortran
+    (oracle/_ref) when built, else the C restatement; multigrid: the C restatement (the
+    reference's multigrid transfers are not compiled here).  Runs before the GPU is touched,
+    so the forked workers inherit no device state."""
+    import multiprocessing as mp
+    try:
+        from oracle.oracle_py import have_ref, oracle
+        oracle()
+    except Exception as e:   # the oracle is optional for the bench
+        return {"value": None, "unit": "cell-updates/s", "cores": 0, "kind": "none",
+                "sample": f"oracle unavailable: {e}"}
+    use_ref = len(blocks) == 1 and have_ref()
+    kind = "reference" if use_ref else "port"
+    ncell = blocks[0].imax * blocks[0].jmax * blocks[0].kmax
+    ncores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    fork = mp.get_context("fork")
+
+    def run(nproc, secs):
+        q = fork.Queue()
+        ps = [fork.Process(target=_cpu_worker, args=(prm, blocks, walls, secs, use_ref, q))
+              for _ in range(nproc)]
+        for p in ps:
+            p.start()
+        res = [q.get() for _ in ps]
+        for p in ps:
+            p.join()
+        return res
+
+    one = run(1, seconds / 2)
+    allc = run(ncores, seconds)
+    per_core = sum(ncell * n / el for n, el in one)
+    host = sum(ncell * n / el for n, el in allc)
+    what = "V-cycles" if len(blocks) > 1 else "iterations"
     b0 = blocks[0]
-    return {"value": ncell * n / el, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{n} full {what} of the {b0.imax}x{b0.jmax}x{b0.kmax} workload "
-                      f"({el:.1f} s, one thread)"}
+    return {"value": host, "unit": "cell-updates/s", "cores": ncores, "kind": kind,
+            "per_core": per_core, "nproc_host": os.cpu_count(),
+            "sample": f"{ncores} independent processes at once, each running whole {what} of "
+                      f"the {b0.imax}x{b0.jmax}x{b0.kmax} workload with its wall for about "
+                      f"{seconds:.0f} s after one untimed step ({sum(n for n, _ in allc)} timed in "
+                      f"total); per_core: one "
+                      f"process alone, {one[0][0]} {what} in {one[0][1]:.1f} s; "
+                      f"{'reference Fortran' if use_ref else 'C restatement'} built "
+                      f"-O3 -march=x86-64-v3 -ffp-contract=off"}
 
 
 def committed_traffic(workload, kernel):
@@ -339,8 +394,14 @@ def main():
 
     w = WORKLOADS[args.workload]
     nlev, iturb = w["nlev"], w["iturb"]
-    prm, blocks = make_workload(args.workload)
+    prm, blocks, walls = make_workload(args.workload)
     ncell = IMAX * JMAX * KMAX
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(prm, blocks, walls, args.cpu_seconds)
+        print(f"[bench] CPU baseline {cpu['value']} cell-updates/s on {cpu['cores']} cores",
+              file=sys.stderr, flush=True)
     print(f"[bench] workload {args.workload} {IMAX}x{JMAX}x{KMAX} ({ncell} cells, {nlev} "
           f"level(s)), building context", file=sys.stderr, flush=True)
     ctx = Context(prm)
@@ -349,6 +410,8 @@ def main():
     ctx.register_levels(blocks)
     for m in range(1, nlev + 1):
         ctx.upload(level=m)
+        ctx.wall_register(*walls[m - 1], level=m)
+        ctx.wall_sync(0, level=m)
     resi0 = run_steps(ctx, nlev, args.warmup, args.graph)
     ctx.finish()
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
@@ -393,25 +456,37 @@ def main():
         ent["cells_total"] = ncells
         kstats[name] = ent
     models = KERNEL_BYTES_PER_CELL[iturb]
-    # dominant kernel: the largest share of the step among those with a byte model; its
-    # algorithmic bytes per launch are the per-cell bytes times the mean cells per launch
-    # over all its launches (every level), which is what a kernel trace averages over
-    dom = max((k for k in kstats if k in models), key=lambda k: kstats[k]["ms_total"])
+    # dominant kernel: the largest share of the step among the kernels that carry a phase of
+    # the iteration; its algorithmic bytes per launch are its phase's compulsory bytes per
+    # cell times the mean cells per launch over all its launches (every level), which is
+    # what a kernel trace averages over
+    dom = max((k for k in kstats if k in PHASE_OF), key=lambda k: kstats[k]["ms_total"])
     ks = kstats[dom]
     avg_ms = ks["ms_total"] / ks["launches"]
     cells_per_launch = ks["cells_total"] / ks["launches"]
-    achieved = models[dom] * cells_per_launch / (avg_ms * 1e-3) / 1e9
+    bpc = PHASE_BYTES[iturb][PHASE_OF[dom]]
+    achieved = bpc * cells_per_launch / (avg_ms * 1e-3) / 1e9
     traffic, tsrc = committed_traffic(args.workload, dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "avg_launch_ms": avg_ms, "bytes_per_cell": models[dom],
-                "cells_per_launch": cells_per_launch}
+                "phase": ("P1 residual", "P2 backward sweep", "P3 forward sweep + update")[PHASE_OF[dom]],
+                "avg_launch_ms": avg_ms, "bytes_per_cell": bpc,
+                "cells_per_launch": cells_per_launch,
+                "design_bytes_per_cell": models.get(dom),
+                "design_model_GBps": (models[dom] * cells_per_launch / (avg_ms * 1e-3) / 1e9
+                                      if dom in models else None)}
     if traffic:
         roofline["traffic_source"] = tsrc
-        roofline["traffic_over_algorithmic"] = traffic / (models[dom] * cells_per_launch)
+        roofline["traffic_over_algorithmic"] = traffic / (bpc * cells_per_launch)
+        if dom in models:
+            roofline["traffic_over_design_model"] = traffic / (models[dom] * cells_per_launch)
     l1 = ks.get("level1")
     if l1:
-        roofline["level1_GBps"] = models[dom] * cells[1] / (l1["ms_total"] / l1["launches"] * 1e-3) / 1e9
+        roofline["level1_GBps"] = bpc * cells[1] / (l1["ms_total"] / l1["launches"] * 1e-3) / 1e9
+    # every phase kernel against its phase bytes, for the record
+    roofline["phase_kernels_GBps"] = {
+        k: PHASE_BYTES[iturb][PHASE_OF[k]] * kstats[k]["cells_total"] / (kstats[k]["ms_total"] * 1e-3) / 1e9
+        for k in kstats if k in PHASE_OF}
     sweep_ms = sum(kstats.get(k, {}).get("ms_total", 0.0)
                    for k in ("k_pred_plane", "k_corr_plane", "k_pred_tiles", "k_corr_tiles",
                              "k_pred_km", "k_corr_km", "k_sweep_prep", "k_pred_st",
@@ -425,11 +500,6 @@ def main():
         lusgs_bytes = (models["k_pred_tiles"] + models["k_corr_tiles"]) * corr["cells_total"]
         lusgs_gbps = lusgs_bytes / args.steps / (sweep_ms * 1e-3) / 1e9
     iter_bw = ITER_BYTES_PER_CELL[iturb] * ncell / (ms_per_step * 1e-3) / 1e9
-
-    cpu = None
-    if not args.no_cpu:
-        print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(prm, blocks, args.cpu_seconds)
 
     line = {
         "metric": "cell-updates/s per MI355X (residual+LU-SGS) and time to 8-decade residual drop",

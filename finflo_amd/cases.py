@@ -9,7 +9,7 @@ upper cell of the face, as the reference stores it.
 """
 import numpy as np
 
-from .layout import GHOST, Block, default_params
+from .layout import GHOST, Block, WallParams, WallPatch, default_params
 
 
 def stretched(n, length, first=None, ratio=None):
@@ -235,3 +235,23 @@ def flat_plate_hierarchy(imax, jmax, kmax, nlev, prm, first=1e-6, ratio=1.15, le
     nodes = flat_plate_nodes(imax, jmax, kmax, first, ratio, length, width)
     return make_hierarchy(imax, jmax, kmax, nlev, prm, seed=seed, amp=amp, kind="velocity",
                           turb=turb, nodes=nodes, turb_amp=0.0)
+
+
+def plate_wall(blk, prm, iheat=0):
+    """The no-slip wall of the flat-plate and box cases: one solid-wall patch over the whole
+    J-low face (ICON(3) = 2, KX = 1..IMAX, KY = 1..KMAX), at rest, smooth (RBK = 0),
+    adiabatic by default, moment arms from the face centres (the cell centres of J = 1
+    projected onto the y = 0 plane), no gravity; force sums relative to the free-stream
+    pressure."""
+    wp = WallParams()
+    wp.t0 = prm.frstem
+    wp.difpre = prm.frspre
+    wp.ground, wp.altitude = 0.0, 1.0          # ALTITUDE >= GROUND: no hydrostatic part
+    p = WallPatch(2, 1, blk.imax, 1, blk.kmax, iheat)
+    L = p.LN
+    # J-face patch arrays: K fastest (na = KMAX), I slowest (nb = IMAX)
+    for name, src in (("xcp", "XC"), ("zcp", "ZC")):
+        a = p.a[name].reshape(p.nb + 2 * L, p.na + 2 * L)
+        a[L:L + p.nb, L:L + p.na] = blk.interior(src)[:, 0, :].T
+    p.a["twall"][:] = prm.frstem
+    return wp, [p]

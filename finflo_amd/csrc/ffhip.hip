@@ -873,7 +873,7 @@ struct WallDev {
     const unsigned char *flag;
     double *flux;       // [7][nt]
     double *corr;       // [2][4][nt]
-    double *qterm;      // [2][nt]
+    double *qterm;      // [3][nt]: the loop-1000 QY, QZ terms and the wall pressure PB
 };
 
 // One patch in BOTSKE's frame: face (0..5), direction, ILMO, KBOTOP, the KX/KY ranges
@@ -1039,6 +1039,9 @@ __global__ void __launch_bounds__(256) k_wall(Prm p, Blk b, WallDev w, WallPatch
     const double pb4 = dmax(-.99 * p.frspre, rc1 * PDI[II] - rc2 * PDI[II2]);
     const double pb = yp1 * pb1 + yp4 * pb4;
     pa[WA_CPWALL][NP] = pb4;
+    // the force pass runs on a side stream while later stages rewrite P and PDIFF: it reads
+    // the wall pressure of this residual from here
+    w.qterm[2 * w.nt + g] = pb;
     double alpha = 1.;
     const double yp = 1. / dist;
     if (turcor) {
@@ -1152,13 +1155,9 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
     __shared__ double run[3][64];
     const double *A3 = FLD(b, FF_A1 + 4 * wp.dir), *A3X = FLD(b, FF_A1X + 4 * wp.dir),
                  *A3Y = FLD(b, FF_A1Y + 4 * wp.dir), *A3Z = FLD(b, FF_A1Z + 4 * wp.dir);
-    const double *P = FLD(b, FF_P), *PDI = FLD(b, FF_PDIFF);
     double *const *pa = wp.arr;
     const int LN = 2, nai = wp.a2 - wp.a1 + 1;
-    const int lstrid = wp.sw * wp.idir;
-    double rc1 = 1.500, rc2 = rc1 - 1.;
-    if (p.inter[wp.dir] == 6) { rc1 = 1.; rc2 = rc1 - 1.; }
-    const double ompor = 1. - 0.0, ypg = ev.difpre, yp1 = 1., yp4 = 0.;
+    const double ompor = 1. - 0.0, ypg = ev.difpre;
     const double idd = (double)wp.idir;
     double s[FF_NFORCE];
 #pragma unroll
@@ -1170,7 +1169,6 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
         if (t < wp.n) {
             const int I = wp.a1 + t % nai, J = wp.b1 + t / nai;
             const int II = (1 + I) * wp.sa + (1 + J) * wp.sb + (1 + wp.kbot) * wp.sw;
-            const int II2 = II + lstrid;
             const int IF1 = wp.idir == 1 ? II : II + wp.sw;
             NP = (LN + J - wp.b1) * wp.na + (I - wp.a1 + LN);
             int ci, cj, ck;
@@ -1201,9 +1199,8 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
             pa[WA_SURFX][NP] = bx / (an + 1.E-20);
             pa[WA_SURFY][NP] = by / (an + 1.E-20);
             pa[WA_SURFZ][NP] = bz / (an + 1.E-20);
-            const double pbb1 = dmax(0.01 * p.frspre, rc1 * P[II] - rc2 * P[II2]);
-            const double pbb4 = rc1 * PDI[II] - rc2 * PDI[II2];
-            const double pb = yp1 * pbb1 + yp4 * pbb4;
+            // PB = YP1*PBB1 + YP4*PBB4 as k_wall formed it from P and PDIFF at II, II2
+            const double pb = w.qterm[2 * w.nt + g];
             const double qx_ = an * ompor * nx * (pb - ypg), qy_ = an * ompor * ny * (pb - ypg);
             const double qz_ = an * ompor * nz * (pb - ypg);
             const double bx2 = bx + idd * qx_, by2 = by + idd * qy_, bz2 = bz + idd * qz_;
@@ -4501,9 +4498,6 @@ int ff_vcycle_graph(ff_ctx *c, int block, int nlev, int ncycles)
     if (rc) return rc;
     if (c->profile) return fail(c, FF_E_STATE, "ff_vcycle_graph: per-kernel timing is on%s");
     if (!c->stream) return fail(c, FF_E_STATE, "ff_vcycle_graph: the null stream cannot be captured%s");
-    for (BlockRec *r : lv)
-        if (!r->walls.dev.empty())
-            return fail(c, FF_E_UNSUP, "ff_vcycle_graph: blocks with wall patches use ff_vcycle%s");
     const long long key[6] = {block, nlev, c->lusgs_mode, c->residual_mode, c->earsm_mode, c->gen};
     bool same = c->vg_exec != nullptr;
     for (int q = 0; q < 6; q++) same = same && c->vg_key[q] == key[q];
@@ -4513,8 +4507,23 @@ int ff_vcycle_graph(ff_ctx *c, int block, int nlev, int ncycles)
             for (BlockRec *r : lv)
                 if ((rc = staged_alloc(c, r))) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        // a force pass of an eager call may still run on the side stream: finish it, so that
+        // the capture forks and joins that stream only through events recorded inside it
+        for (BlockRec *r : lv)
+            if (r->walls.pending) {
+                HIPCHK(c, hipEventSynchronize(r->walls.done));
+                r->walls.pending = false;
+            }
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         rc = vcycle_once(c, lv, nlev, block, nullptr);
+        // join the last force pass of each level back into the captured stream: a replay
+        // ends only when its force sums are complete
+        for (BlockRec *r : lv)
+            if (r->walls.pending) {
+                if (!rc && hipStreamWaitEvent(c->stream, r->walls.done, 0) != hipSuccess)
+                    rc = fail(c, FF_E_HIP, "ff_vcycle_graph: joining the force pass failed%s");
+                r->walls.pending = false;
+            }
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(c->stream, &g);
         if (rc) {
@@ -4529,6 +4538,12 @@ int ff_vcycle_graph(ff_ctx *c, int block, int nlev, int ncycles)
         }
         for (int q = 0; q < 6; q++) c->vg_key[q] = key[q];
     }
+    // a force pass of an eager call may still read the wall scratch the graph rewrites
+    for (BlockRec *r : lv)
+        if (r->walls.pending) {
+            HIPCHK(c, hipStreamWaitEvent(c->stream, r->walls.done, 0));
+            r->walls.pending = false;
+        }
     for (int it = 0; it < ncycles; it++) HIPCHK(c, hipGraphLaunch(c->vg_exec, c->stream));
     if (c->lusgs_mode != FF_LUSGS_PLANES)
         for (int m = 0; m < nlev; m++)
@@ -4622,7 +4637,7 @@ int ff_wall_register(ff_ctx *c, int level, int block, const ff_wall_params *wpar
     wd.corr_on = corr ? 1 : 0;
     w.env = WallEnv{wpar->t0, wpar->difpre, wpar->xmom, wpar->ymom, wpar->zmom,
                     wpar->gx, wpar->gy, wpar->gz, wpar->ground, wpar->altitude};
-    const size_t nscr = (size_t)wd.nt * (7 + 8 + 2);
+    const size_t nscr = (size_t)wd.nt * (7 + 8 + 3);
     if (hipMalloc(&w.parr, sizeof(double) * NWARR * nparr) != hipSuccess ||
         hipMalloc(&w.flag, (size_t)wd.nt) != hipSuccess ||
         hipMalloc(&w.scr, sizeof(double) * nscr) != hipSuccess ||

@@ -261,8 +261,9 @@ int  ff_vcycle(ff_ctx *ctx, int block, int nlev, int ncycles, double *resi);
 /* ff_vcycle without residual sums, as one HIP graph: the first call captures one V-cycle
  * of the context's stream (every kernel of ff_vcycle, same launches, same results bit for
  * bit) and each call replays it ncycles times; nlev = 1 is the iteration of ff_cycle.  A registration or option change makes the
- * next call capture again.  Not for blocks with wall patches (FF_E_UNSUP), with per-kernel
- * timing on, or on the null stream (FF_E_STATE).  No counterpart in the reference: CYCLES
+ * next call capture again.  The force pass of wall patches stays on its side stream inside
+ * the graph (forked and joined through events).  Not with per-kernel timing on, or on the
+ * null stream (FF_E_STATE).  No counterpart in the reference: CYCLES
  * (ns3c.f:1400-2137) launches nothing; this replaces the per-stage launches of ff_vcycle. */
 int  ff_vcycle_graph(ff_ctx *ctx, int block, int nlev, int ncycles);
 /* ncycles device-resident iterations of the finest-level sequence of CYCLES for block N:

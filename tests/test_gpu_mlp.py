@@ -41,11 +41,11 @@ def test_mlp_infer_matches_oracle(hip_device):
     print(f"inference of {n} samples", flush=True)
     y = ctx.mlp_infer(torch.from_numpy(x).cuda()).cpu().numpy()
     ref = M.predict(p, norm, x, emulate=True)
-    # BF16 rounding of an activation can land on the other side of a tie when the FP32
-    # accumulation order (or asinhf) differs: bounded as a relative error of the outputs
+    # the FP32 accumulation order and asinhf/expf differ from the oracle's float64 in the
+    # last bits; measured 2.0e-7 on MI355X, held to 10 times that
     err = np.abs(y - ref) / np.abs(ref)
     print("max normalised error", err.max(), flush=True)
-    assert err.max() < 5e-3
+    assert err.max() < 2e-6
     p2, n2 = ctx.mlp_get()
     np.testing.assert_array_equal(p2, p)
     np.testing.assert_array_equal(n2, norm)
@@ -67,13 +67,15 @@ def test_mlp_gradient_matches_oracle(hip_device):
     lref, gref = M.loss_grad(p, norm, x, t, emulate=True)
     rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
     print("loss", loss, lref, "relative gradient error", rel, flush=True)
-    assert abs(loss - lref) <= 1e-3 * lref
-    assert rel < 1e-2
+    # measured on MI355X: loss 3.4e-8, gradient 1.9e-6 relative; held to about 10 times that
+    assert abs(loss - lref) <= 1e-6 * lref
+    assert rel < 2e-5
     # every parameter block is covered (a transposed or dropped block shows here)
     for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2),
                  (M.OFF_B2, M.OFF_W3), (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.NP)):
         rb = np.linalg.norm(g[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
-        assert rb < 3e-2, (a, b, rb)
+        print(f"parameters {a}:{b} relative gradient error {rb:.3e}", flush=True)
+        assert rb < 2e-4, (a, b, rb)
     ctx.close()
 
 
@@ -156,7 +158,8 @@ def test_training_and_learned_closure(hip_device):
     outside = np.any((xn < nrm[16:21] - margin) | (xn > nrm[21:26] + margin), axis=1)
     print(f"held-out cells: {inside.sum()} inside the box, {outside.sum()} outside", flush=True)
     assert inside.sum() > 0
-    np.testing.assert_array_equal(got[outside], ref[outside])
+    if outside.any():      # the fallback has its own test below, with cells on both sides
+        np.testing.assert_array_equal(got[outside], ref[outside])
     # the kernel's closure is the network's N put through EARSM1's formulas
     want = M.bij(M.betas_from_learned(yp, inv), terms)
     assert np.all(np.abs(got[inside] - want[inside]) <= 1e-9 * np.abs(want).max(0))
@@ -187,4 +190,50 @@ def test_learned_closure_falls_back_outside_the_box(hip_device):
     ctx.download()
     for f in BIJ:
         np.testing.assert_array_equal(blk.interior(f), exact[f])
+    ctx.close()
+
+
+def test_learned_closure_mixed_wave(hip_device):
+    """A training box that holds only part of a block's cells, so that 64-cell waves of
+    k_earsm_mlp hold cells on both sides of it: the outside cells take the exact EARSM1 bit
+    for bit, the inside cells the network's N through EARSM1's formulas."""
+    import torch
+    prm, _ = make_earsm_case((4, 4, 4), EARSM, dict(kind="all", amp=0.01, seed=1))
+    ctx = _ctx(prm)
+    (blk,) = _earsm_blocks(ctx, [HELD])
+    x, y = ctx.earsm_samples()
+    ctx.download()
+    exact = np.stack([blk.interior(f).ravel().copy() for f in BIJ], 1)
+    ctx.mlp_fit_norm(x, y)
+    _, norm = ctx.mlp_get()
+    xh = x.cpu().numpy()
+    xn = (M.feature(xh) - norm[0:5]) / norm[5:10]
+    # the box from the 10th to the 90th percentile of each normalised input
+    norm = norm.copy()
+    norm[16:21] = np.percentile(xn, 10, axis=0)
+    norm[21:26] = np.percentile(xn, 90, axis=0)
+    p = M.init_params(7).astype(np.float32)
+    ctx.mlp_set(p, norm)
+    ctx.earsm_mode("mlp")
+    ctx.turbdi()
+    ctx.download()
+    got = np.stack([blk.interior(f).ravel() for f in BIJ], 1)
+    margin = 1e-4
+    inside = np.all((xn >= norm[16:21] + margin) & (xn <= norm[21:26] - margin), axis=1)
+    outside = np.any((xn < norm[16:21] - margin) | (xn > norm[21:26] + margin), axis=1)
+    n = len(xn)
+    waves = [(inside[w:w + 64].any() and outside[w:w + 64].any()) for w in range(0, n, 64)]
+    print(f"{n} cells: {inside.sum()} inside, {outside.sum()} outside, "
+          f"{sum(waves)} of {len(waves)} waves hold both", flush=True)
+    assert inside.sum() > 0 and outside.sum() > 0 and sum(waves) > 0
+    np.testing.assert_array_equal(got[outside], exact[outside])
+    S = np.stack([blk.interior(f).ravel() for f in ("S11", "S12", "S13", "S22", "S23", "S33")], 1)
+    W = np.stack([blk.interior(f).ravel() for f in ("W12", "W13", "W23")], 1)
+    inv, _, terms = M.earsm_parts(S, W, blk.interior("TTS").ravel())
+    yp = ctx.mlp_infer(torch.from_numpy(xh).cuda()).cpu().numpy()
+    want = M.bij(M.betas_from_learned(yp, inv), terms)
+    assert np.all(np.abs(got[inside] - want[inside]) <= 1e-9 * np.abs(want).max(0))
+    # the inside cells did not take the exact EARSM1
+    assert np.any(got[inside] != exact[inside])
+    ctx.earsm_mode("exact")
     ctx.close()

@@ -245,12 +245,41 @@ def test_vcycle_graph_fullns(hip_device, iturb):
     ce.close()
 
 
-def test_vcycle_graph_refuses_walls(hip_device):
-    from finflo_amd.ffhip import FFError
-    prm, blocks = hierarchy(0, nlev=2, dims=(16, 8, 8), seed=81)
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_vcycle_graph_walls(hip_device, iturb):
+    """V-cycles with solid walls on every level replayed from the captured graph (the force
+    pass forked onto its side stream and joined inside the graph) against the oracle's
+    V-cycles: every solved field of every level, the level-1 patch arrays and force sums,
+    with eager V-cycles of the same context between the replays."""
+    print(f"case V-cycle graph with walls iturb={iturb} nlev=3", flush=True)
+    prm, blocks = hierarchy(iturb, nlev=3, dims=(16, 12, 12), seed=87)
+    host = [b.copy() for b in blocks]
+    oh = OracleHierarchy(prm, host)
     ctx = context(prm, blocks)
-    wp, patches = make_walls(blocks[0], prm, [(2, 1, blocks[0].imax, 1, blocks[0].kmax, 0)], 5)
-    ctx.wall_register(wp, patches, level=1)
-    with pytest.raises(FFError, match="wall patches"):
-        ctx.vcycle_graph(2)
+    pw_h, pw_g = [], []
+    for m, b in enumerate(blocks):
+        specs = [(2, 1, b.imax, 1, b.kmax, 0), (6, 2, b.imax - 1, 1, b.jmax, 1)]
+        wp, patches = make_walls(b, prm, specs, 41 + m)
+        pw_h.append([p.copy() for p in patches])
+        pw_g.append([p.copy() for p in patches])
+        oh.levels[m].set_walls(wp, pw_h[m])
+        ctx.wall_register(wp, pw_g[m], level=m + 1)
+    prim = SST_PRIM if iturb else PRIM
+    tol = RTOL if iturb else 0.0
+    for rep, (ng, ne) in enumerate(((2, 0), (1, 1), (3, 0))):
+        for _ in range(ng + ne):
+            oh.vcycle()
+        ctx.vcycle_graph(3, ncycles=ng)
+        if ne:
+            ctx.vcycle(3, ncycles=ne)
+        for m in range(3):
+            ctx.download(level=m + 1)
+            ctx.wall_sync(1, level=m + 1)
+            assert_fields_equal(blocks[m], host[m], prim, interior=False, rtol=tol,
+                                what=f"graph replay {rep} level {m + 1}")
+        assert_patches_equal(pw_g[0], pw_h[0], rtol=tol, what=f"graph replay {rep} patches")
+        for l in range(2):
+            f0 = oh.levels[0].forces(l)
+            np.testing.assert_allclose(ctx.wall_forces(l), f0, rtol=1e-10 if iturb else 0,
+                                       atol=1e-10 * np.abs(f0).max() if iturb else 0)
     ctx.close()
