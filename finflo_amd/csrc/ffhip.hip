@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3541,10 +3542,41 @@ struct ff_ctx {
     hipGraphExec_t vg_exec = nullptr;
     long long vg_key[6] = {0, 0, 0, 0, 0, 0};
     long long gen = 0;
+    int device = -1;              // the HIP device the context was created on
+    double *scratch = nullptr;    // that device's address of the fixed scratch array
 };
 
 static void mlp_free(MlpRec &r);
 static int earsm_mlp_pass(ff_ctx *c, const Blk &b);
+
+// The fixed device scratch of ff_residual_noalloc: the RESL2 partials and sums and the
+// per-workgroup SEPS maxima of the k-omega residual, 36,680 doubles (about 287 KB) that
+// every GPU the library runs on holds until the process ends.  In host code `scratch`
+// names a host-side stand-in; the device address comes from hipGetSymbolAddress, taken
+// once per device when a context is created there, and only that address is passed on.
+constexpr int NA_RED_PARTS = 512;                       // RESL2: 7 partials each
+constexpr int NA_KW_PARTS = 32768;                      // SEPS maxima: 256 entries each
+constexpr size_t NA_OFF_OUT = 7 * NA_RED_PARTS;         // RESL2 sums (8)
+constexpr size_t NA_OFF_KW = NA_OFF_OUT + 8;            // SEPS maxima (NA_KW_PARTS + 1)
+constexpr size_t NA_SCRATCH = NA_OFF_KW + NA_KW_PARTS + 8;
+static __device__ __align__(256) double scratch[NA_SCRATCH];
+constexpr int NA_MAX_DEVICES = 64;
+static double *g_scratch_dev[NA_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the current device's address of `scratch` (looked up once per device)
+static hipError_t scratch_address(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
 
 static long long key(int level, int block) { return ((long long)level << 32) | (unsigned)block; }
 
@@ -3813,6 +3845,11 @@ int ff_ctx_create(ff_ctx **out, const ff_params *prm)
         return FF_E_HIP;
     }
     c->own_stream = true;
+    if (hipGetDevice(&c->device) != hipSuccess || c->device < 0 || c->device >= NA_MAX_DEVICES ||
+        scratch_address(c->device, &c->scratch) != hipSuccess) {
+        ff_ctx_destroy(c);
+        return FF_E_HIP;
+    }
     c->red_parts = 512;
     if (hipMalloc(&c->red_part, sizeof(double) * 7 * c->red_parts) != hipSuccess ||
         hipMalloc(&c->red_out, sizeof(double) * 8) != hipSuccess ||
@@ -4051,7 +4088,16 @@ int ff_turb(ff_ctx *c, int level, int block)
     return launch(c, "k_turb_lam", k_turb_lam, cells_grid(r->dev, 256), dim3(256), r->dev);
 }
 
-static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi)
+// where the RESL2 partials and sums of a residual go: the context's buffers, or the fixed
+// scratch array of ff_residual_noalloc
+struct RedBuf {
+    double *part;
+    int nparts;
+    double *out;
+};
+
+static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi,
+                         const RedBuf *rb = nullptr)
 {
     const Blk &b = r->dev;
     c->lev = level;
@@ -4162,15 +4208,17 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
     if (rc) return rc;
     if ((rc = forces())) return rc;
     if (level == 1 && resi) {
+        const RedBuf own{c->red_part, c->red_parts, c->red_out};
+        const RedBuf &rd = rb ? *rb : own;
         int ncell = b.imax * b.jmax * b.kmax;
         int parts = (ncell + RED_THREADS - 1) / RED_THREADS;
-        if (parts > c->red_parts) parts = c->red_parts;
-        rc = kw ? launch(c, "k_resl2_part", k_resl2_part<7>, dim3(parts), dim3(RED_THREADS), b, c->red_part)
-                : launch(c, "k_resl2_part", k_resl2_part<5>, dim3(parts), dim3(RED_THREADS), b, c->red_part);
+        if (parts > rd.nparts) parts = rd.nparts;
+        rc = kw ? launch(c, "k_resl2_part", k_resl2_part<7>, dim3(parts), dim3(RED_THREADS), b, rd.part)
+                : launch(c, "k_resl2_part", k_resl2_part<5>, dim3(parts), dim3(RED_THREADS), b, rd.part);
         if (rc) return rc;
-        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)c->red_part, parts, nq, c->red_out);
+        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)rd.part, parts, nq, rd.out);
         if (rc) return rc;
-        HIPCHK(c, hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->red_host, rd.out, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int q = 0; q < 7; q++) resi[q] = q < nq ? c->red_host[q] : 0.;
     }
@@ -4184,6 +4232,36 @@ int ff_residual(ff_ctx *c, int level, int block, int m2, double *resi)
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
     return residual_impl(c, r, level, m2, resi);
+}
+
+// ff_residual on caller-owned device arrays with every scratch buffer in the fixed array:
+// no registration, no allocation
+int ff_residual_noalloc(ff_ctx *c, int level, const ff_block *dev, int m2, double *resi)
+{
+    if (!c) return FF_E_ARG;
+    if (level < 1) return fail(c, FF_E_ARG, "ff_residual_noalloc: level must be >= 1%s");
+    int rc = check_dims(c, dev);
+    if (rc) return rc;
+    if ((rc = check_fields(c, dev, "device field %s is NULL"))) return rc;
+    if (c->profile) return fail(c, FF_E_STATE, "ff_residual_noalloc: per-kernel timing is on%s");
+    int cur = -1;
+    HIPCHK(c, hipGetDevice(&cur));
+    if (cur != c->device)
+        return fail(c, FF_E_STATE, "ff_residual_noalloc: the current device is not the context's%s");
+    BlockRec r;
+    r.mirror = false;
+    r.level = level;
+    r.dev = make_blk(dev);
+    if (c->prm.iturb == 6) {
+        r.kw_parts = (r.dev.ntot + 255) / 256;
+        if (r.kw_parts > NA_KW_PARTS)
+            return fail(c, FF_E_ARG, "ff_residual_noalloc: the k-omega residual takes blocks of at most "
+                        "8388608 entries including ghost layers (this one has %s)",
+                        std::to_string(r.dev.ntot).c_str());
+        r.kw_scr = c->scratch + NA_OFF_KW;
+    }
+    const RedBuf rb{c->scratch, NA_RED_PARTS, c->scratch + NA_OFF_OUT};
+    return residual_impl(c, &r, level, m2, resi, &rb);
 }
 
 // the staged LU-SGS records of a block level, allocated on first use and kept

@@ -23,7 +23,7 @@ EXPORTS = [
     "ff_implicit", "ff_update", "ff_save_old", "ff_cycle", "ff_restrict", "ff_forcing",
     "ff_prolong", "ff_vcycle", "ff_vcycle_graph", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
-    "ff_mlp_grad", "ff_mlp_train",
+    "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
@@ -62,6 +62,7 @@ def lib():
         for fn in ("ff_state", "ff_turb", "ff_update", "ff_save_old"):
             getattr(L, fn).argtypes = [vp, i, i]
         L.ff_residual.argtypes = [vp, i, i, i, vp]
+        L.ff_residual_noalloc.argtypes = [vp, i, C.POINTER(BlockView), i, vp]
         L.ff_implicit.argtypes = [vp, i, i, i]
         L.ff_cycle.argtypes = [vp, i, i, vp]
         L.ff_restrict.argtypes = [vp, i, i]

@@ -272,6 +272,20 @@ int  ff_vcycle_graph(ff_ctx *ctx, int block, int nlev, int ncycles);
  * of the last cycle. */
 int  ff_cycle(ff_ctx *ctx, int block, int ncycles, double *resi);
 
+/* ff_residual of a block given directly as caller-owned device arrays (the field pointers
+ * of `dev` are device addresses), with no registration and no device allocation: every
+ * scratch buffer (the RESL2 partials and sums, the k-omega SEPS maxima) lives in a fixed
+ * device array of the library, 36,680 doubles (about 287 KB) that each GPU the library
+ * runs on holds until the process ends.  No counterpart in the reference (RESIDI
+ * ns3c.f:2355 is the computation); it is for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * The context comes from ff_ctx_create on the current device (which looks the array up).
+ * Limits: no wall patches; with ITURB = 6 at most 8,388,608 entries per field including the
+ * ghost layers (FF_E_ARG beyond); per-kernel timing off (FF_E_STATE).  Every call on one GPU
+ * shares that GPU's array and leaves its contents behind (each call rewrites what it
+ * reads): calls that overlap on one GPU, from other streams or contexts, race. */
+int  ff_residual_noalloc(ff_ctx *ctx, int level, const ff_block *dev, int m2, double *resi);
+
 /* Register (or replace) the solid-wall patches of block N on level M, in the reference's
  * patch order.  The patch arrays are host arrays mirrored on the device (ff_wall_sync moves
  * them).  From then on ff_residual applies BOTSKE (base3c.f:6147-7095) at the patch faces
