@@ -21,7 +21,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from finflo_amd.cases import flat_plate_hierarchy, plate_wall   # noqa: E402
+from finflo_amd.cases import (config2, config4, flat_plate_hierarchy, flow_params,  # noqa: E402
+                               plate_wall)
 from finflo_amd.layout import default_params        # noqa: E402
 
 IMAX, JMAX, KMAX = 192, 96, 96
@@ -108,38 +109,16 @@ KERNELS = ("k_residual_kw", "k_residual_tile", "k_residual", "k_pred_tiles", "k_
            "k_restrict_ghosts", "k_forcing", "k_prolong")
 
 
-def workload_params(iturb, reynolds):
-    prm = default_params(iturb=iturb)
-    c = np.sqrt(prm.gamma * prm.rgas * prm.frstem)
-    prm.frsvel = MACH * c
-    # Re = rho U L / mu with L = 1 m: scale Sutherland's viscosity (RMULTV) to reach it
-    t = prm.frstem
-    mu_suth = prm.visu0 * t * np.sqrt(t) / (t + prm.tsu0)
-    prm.rmult = prm.frsden * prm.frsvel / reynolds / mu_suth
-    prm.frsvis = mu_suth * prm.rmult
-    prm.re = reynolds
-    prm.cfl = prm.cfll = 5.0
-    return prm
-
-
-def freestream_turbulence(prm, tu=0.01, mut_ratio=1.0):
-    """(RK, REPS) = (rho k, rho omega) of a free stream with turbulence intensity `tu` and
-    eddy-to-molecular viscosity ratio `mut_ratio`."""
-    k = 1.5 * (tu * prm.frsvel) ** 2
-    rk = prm.frsden * k
-    omega = prm.frsden * k / (mut_ratio * prm.frsvis)
-    return rk, prm.frsden * omega
-
-
 def make_workload(name="sst-mg", seed=0):
-    """The flat plate of the workload and its no-slip wall (the J-low face of every level)."""
-    w = WORKLOADS[name]
-    prm = workload_params(w["iturb"], w["reynolds"])
-    turb = freestream_turbulence(prm) if w["iturb"] == 6 else None
-    blocks = flat_plate_hierarchy(IMAX, JMAX, KMAX, w["nlev"], prm, first=1e-6, ratio=1.15,
-                                  seed=seed, amp=0.005, turb=turb)
-    walls = [plate_wall(b, prm) for b in blocks]
-    return prm, blocks, walls
+    """The flat plate of the workload and its no-slip wall (the J-low face of every level):
+    sst-mg is the 192x96x96 k-omega SST plate with 3 multigrid levels; laminar is the same
+    grid, laminar, finest level only."""
+    if name == "sst-mg":
+        return config2(seed=seed, nlev=3)
+    prm = flow_params(iturb=0, mach=MACH, reynolds=WORKLOADS[name]["reynolds"], cfl=5.0)
+    blocks = flat_plate_hierarchy(IMAX, JMAX, KMAX, 1, prm, first=1e-6, ratio=1.15,
+                                  seed=seed, amp=0.005)
+    return prm, blocks, [plate_wall(b, prm) for b in blocks]
 
 
 def run_steps(ctx, nlev, n, graph=False):
@@ -266,14 +245,8 @@ def earsm_mlp_bench(args):
     events and compared field by field.  Prints one JSON line."""
     import torch
     from finflo_amd.ffhip import Context
-    from finflo_amd.cases import make_hierarchy, flat_plate_nodes
     ni, nj, nk = 256, 128, 128
-    prm = workload_params(6, 2.0e6)
-    prm.istres = 1
-    turb = freestream_turbulence(prm)
-    nodes = flat_plate_nodes(ni, nj, nk, 1e-6, 1.15, 1.0, None)
-    blk = make_hierarchy(ni, nj, nk, 1, prm, seed=4, amp=0.05, kind="velocity", turb=turb,
-                         nodes=nodes, turb_amp=0.3)[0]
+    prm, blk, _ = config4(seed=4)
     ncell = ni * nj * nk
     print(f"[bench] earsm-mlp {ni}x{nj}x{nk} ({ncell} cells), building context", file=sys.stderr, flush=True)
     ctx = Context(prm)
@@ -340,8 +313,9 @@ def earsm_mlp_bench(args):
         "metric": "learned EARSM closure: inference TFLOP/s (BF16 MFMA) and b_ij error against the exact EARSM",
         "value": tflops, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "higher_is_better": True, "dtype": "bf16 (FP32 accumulate)", "data": "synthetic",
-        "config": {"workload": f"flat plate {ni}x{nj}x{nk} single block, k-omega SST + EARSM, "
-                               "velocity noise 5 %, training samples = every cell's exact EARSM",
+        "config": {"workload": f"3-D Gaussian bump {ni}x{nj}x{nk} single block, k-omega SST + "
+                               "EARSM, M=0.3 Re=2e6, velocity noise 5 %, training samples = "
+                               "every cell's exact EARSM",
                    "cells": ncell, "train_steps": steps, "train_batch": batch,
                    "train_seconds": train_s, "loss_first": float(loss[0]), "loss_last": float(loss[-1])},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",

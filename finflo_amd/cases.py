@@ -183,10 +183,11 @@ def make_block(imax, jmax, kmax, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
 
 def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jratio=None,
                    distort=0.0, seed=0, amp=0.01, kind="density", turb=None,
-                   mach_dir=(1.0, 0.0, 0.0), nodes=None, turb_amp=None):
+                   mach_dir=(1.0, 0.0, 0.0), nodes=None, turb_amp=None, surface=None):
     """A block and its nlev-1 coarser multigrid levels (dimensions divisible by 2^(nlev-1)).
     Every level has its own geometry; the state of the coarse levels is free stream (the
-    restriction overwrites what the V-cycle uses)."""
+    restriction overwrites what the V-cycle uses).  `surface(x, z)`, when given, is the
+    height of the J-low wall; the wall distance is then measured vertically from it."""
     prm = prm or default_params()
     f = 2 ** (nlev - 1)
     if imax % f or jmax % f or kmax % f:
@@ -200,6 +201,8 @@ def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jr
     for m in range(nlev):
         b = Block(imax >> m, jmax >> m, kmax >> m)
         fill_geometry(b, nodes)
+        if surface is not None:
+            b["DISTW"] = np.abs(b["YC"] - surface(b["XC"], b["ZC"]))
         fill_state(b, prm, seed=seed + m, amp=amp if m == 0 else 0.0, kind=kind,
                    turb=turb, mach_dir=mach_dir, turb_amp=turb_amp if m == 0 else 0.0)
         blocks.append(b)
@@ -255,3 +258,106 @@ def plate_wall(blk, prm, iheat=0):
         a[L:L + p.nb, L:L + p.na] = blk.interior(src)[:, 0, :].T
     p.a["twall"][:] = prm.frstem
     return wp, [p]
+
+
+# ---- the benchmark configurations ----------------------------------------------------------
+
+def flow_params(iturb=0, mach=0.3, reynolds=1e5, cfl=5.0, **kw):
+    """FINFLO defaults with a case's free stream: the speed from the Mach number at the
+    default temperature, and Sutherland's viscosity scaled (RMULTV) so that rho U L / mu is
+    the Reynolds number per unit length."""
+    prm = default_params(iturb=iturb, **kw)
+    c = np.sqrt(prm.gamma * prm.rgas * prm.frstem)
+    prm.frsvel = mach * c
+    t = prm.frstem
+    mu_suth = prm.visu0 * t * np.sqrt(t) / (t + prm.tsu0)
+    prm.rmult = prm.frsden * prm.frsvel / reynolds / mu_suth
+    prm.frsvis = mu_suth * prm.rmult
+    prm.re = reynolds
+    prm.cfl = prm.cfll = cfl
+    return prm
+
+
+def freestream_turbulence(prm, tu=0.01, mut_ratio=1.0):
+    """(RK, REPS) = (rho k, rho omega) of a free stream with turbulence intensity `tu` and
+    eddy-to-molecular viscosity ratio `mut_ratio`."""
+    k = 1.5 * (tu * prm.frsvel) ** 2
+    rk = prm.frsden * k
+    omega = prm.frsden * k / (mut_ratio * prm.frsvis)
+    return rk, prm.frsden * omega
+
+
+def config1(iflux=1, seed=0):
+    """The plain method's CPU-runnable case: one 48x24x24 block on [0,2]x[0,1]x[0,1],
+    J spacing growing by 1.1 from the no-slip wall on the J-low face, laminar, M = 0.3,
+    Re = 1e5 per unit length, CFL 5, free stream with a seeded +/-1 % density perturbation
+    (T = T_inf/(1+d) at the free-stream pressure); the other faces keep their ghost cells
+    as given (free stream).  iflux 1: Roe, 0: van Leer."""
+    prm = flow_params(iturb=0, mach=0.3, reynolds=1e5, cfl=5.0, iflux=iflux)
+    blk = make_block(48, 24, 24, prm, lengths=(2.0, 1.0, 1.0), jratio=1.1, seed=seed,
+                     amp=0.01, kind="density")
+    return prm, blk, plate_wall(blk, prm)
+
+
+def config2(seed=0, nlev=3):
+    """The flat plate: 192x96x96 cells, cosine-clustered I over the plate length, first wall
+    cell 1e-6 of it and J growth 1.15, k-omega SST, M = 0.2, Re = 5e6, velocity noise
+    +/-0.5 %, CFL 5, nlev multigrid levels, the no-slip wall on every level's J-low face."""
+    prm = flow_params(iturb=6, mach=0.2, reynolds=5e6, cfl=5.0)
+    turb = freestream_turbulence(prm)
+    blocks = flat_plate_hierarchy(192, 96, 96, nlev, prm, first=1e-6, ratio=1.15, seed=seed,
+                                  amp=0.005, turb=turb)
+    return prm, blocks, [plate_wall(b, prm) for b in blocks]
+
+
+def gaussian_bump(h=0.05, w=0.2, x0=0.0, z0=0.0, sweep=0.0):
+    """Height of a smoothed Gaussian wall bump, h exp(-((x-x0)^2 + (z-z0)^2)/w^2); with a
+    sweep angle (radians) its crest line runs at that angle across the span instead."""
+    def surface(x, z):
+        if sweep:
+            xs = (x - x0) - np.tan(sweep) * (z - z0)
+            return h * np.exp(-(xs * xs) / (w * w))
+        return h * np.exp(-((x - x0) ** 2 + (z - z0) ** 2) / (w * w))
+    return surface
+
+
+def bump_nodes(imax, jmax, kmax, surface, xlen=2.0, height=1.0, zlen=1.0, first=1e-5,
+               ratio=None):
+    """Nodes of a box over a wall bump: I and K uniform over [-xlen/2, xlen/2] and
+    [-zlen/2, zlen/2], J lines from the bump surface to y = height, geometric from the wall
+    (first cell `first` of the local height and growth `ratio`, or the growth that fills
+    the height when ratio is None)."""
+    xi = np.linspace(-0.5 * xlen, 0.5 * xlen, imax + 1)
+    zk = np.linspace(-0.5 * zlen, 0.5 * zlen, kmax + 1)
+    if ratio is None:
+        lo, hi = 1.0 + 1e-9, 2.0
+        for _ in range(200):                     # growth with first*(r^n - 1)/(r - 1) = 1
+            r = 0.5 * (lo + hi)
+            if first * (r ** jmax - 1.0) / (r - 1.0) > 1.0:
+                hi = r
+            else:
+                lo = r
+        ratio = 0.5 * (lo + hi)
+    eta = geometric_from_wall(jmax, 1.0, ratio)
+    eta = eta / eta[-1]
+    X = _extend(xi)
+    Z = _extend(zk)
+    E = _extend(eta)
+    Zg, Eg, Xg = np.meshgrid(Z, E, X, indexing="ij")
+    ys = surface(Xg, Zg)
+    Y = ys + (height - ys) * Eg
+    return np.stack([Xg, Y, Zg], axis=-1)
+
+
+def config4(seed=0):
+    """The learned-closure case: one 256x128x128 block over a 3-D Gaussian bump (height
+    0.05, width 0.2, on a 2 x 1 x 1 box, first wall cell 1e-5), k-omega SST with the EARSM
+    (ISTRES = 1), M = 0.3, Re = 2e6, free stream with +/-5 % velocity and 30 % turbulence
+    noise so that the anisotropy is not trivial everywhere."""
+    prm = flow_params(iturb=6, mach=0.3, reynolds=2e6, cfl=5.0, istres=1)
+    turb = freestream_turbulence(prm)
+    surf = gaussian_bump(0.05, 0.2)
+    nodes = bump_nodes(256, 128, 128, surf, xlen=2.0, height=1.0, zlen=1.0, first=1e-5)
+    blk = make_hierarchy(256, 128, 128, 1, prm, seed=seed, amp=0.05, kind="velocity",
+                         turb=turb, nodes=nodes, turb_amp=0.3, surface=surf)[0]
+    return prm, blk, plate_wall(blk, prm)

@@ -1140,32 +1140,53 @@ __global__ void __launch_bounds__(256) k_wall(Prm p, Blk b, WallDev w, WallPatch
     }
 }
 
-// BOTSKE loop 2000 (base3c.f:6988-7093) on level 1: the force sums of every patch (one wave
-// per patch) in the reference's face order.  A chunk of 64 faces is prepared by the lanes
-// in parallel (moment-arm products, pressure parts), the sums run through them in order on
-// lane 0 (the reference's association, e.g. CMX = (CMX + BZ*YD) - BY*ZD), and the lanes then
-// form SURFPX.. from the running DX.. that lane 0 left in LDS.
+// BOTSKE loop 2000 (base3c.f:6988-7093) on level 1: the force sums of every patch (one
+// workgroup per patch) in the reference's face order.  A chunk of 256 faces is prepared by
+// the threads in parallel (moment-arm products, pressure parts) into LDS; then lane q < 19
+// of the first wave carries sum q through the chunk's faces in order, with the reference's
+// association (e.g. CMX = (CMX + BZ*YD) - BY*ZD, taken as two additions of the signed
+// terms, which is the same IEEE operation), so the 19 sequential chains run side by side;
+// every lane leaves its running sum per face in LDS, and every thread forms SURFPX.. of its
+// face from the running DX, DY, DZ.
 constexpr int WF_T = 29;
-__global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
-                                                    const WallPatchDev *wps, WallEnv ev,
-                                                    double *sums)
+constexpr int WF_N = 256;
+// per force sum (enum ff_force order): first term and its sign, second term and its sign.
+// A sum of one term takes slot 28 (zero) as its second: adding +0. leaves every value the
+// sum can hold unchanged (it starts at +0. and so never holds -0.), and QY's "+ 0." of the
+// rotating-frame term (UROT = 0) is exactly that.
+__constant__ signed char kWfTerms[FF_NFORCE][4] = {
+    {6, 1, 28, 1}, {7, 1, 28, 1}, {8, 1, 28, 1},                     // CX CY CZ
+    {0, 1, 1, -1}, {2, 1, 3, -1}, {4, 1, 5, -1},                     // CMX CMY CMZ
+    {9, -1, 28, 1},                                                  // TOM
+    {10, -1, 28, 1}, {11, -1, 28, 1}, {12, -1, 28, 1},               // DX DY DZ
+    {22, -1, 28, 1}, {23, 1, 28, 1}, {24, 1, 28, 1},                 // QX QY QZ
+    {13, 1, 14, -1}, {15, 1, 16, -1}, {17, 1, 18, -1},               // VMX VMY VMZ
+    {19, 1, 28, 1}, {20, 1, 28, 1}, {21, 1, 28, 1}};                 // BUX BUY BUZ
+__global__ void __launch_bounds__(WF_N) k_wall_forces(Prm p, Blk b, WallDev w,
+                                                      const WallPatchDev *wps, WallEnv ev,
+                                                      double *sums)
 {
     const WallPatchDev wp = wps[blockIdx.x];
-    const int lane = threadIdx.x;
-    __shared__ double tm[WF_T][64];
-    __shared__ double run[3][64];
+    const int tid = threadIdx.x;
+    // rows padded by one so that the 19 lanes, each reading its own row at the same face,
+    // hit different banks
+    __shared__ double tm[WF_T][WF_N + 1];
+    __shared__ double run[FF_NFORCE][WF_N + 1];
+    __shared__ double fin[FF_NFORCE];
     const double *A3 = FLD(b, FF_A1 + 4 * wp.dir), *A3X = FLD(b, FF_A1X + 4 * wp.dir),
                  *A3Y = FLD(b, FF_A1Y + 4 * wp.dir), *A3Z = FLD(b, FF_A1Z + 4 * wp.dir);
     double *const *pa = wp.arr;
     const int LN = 2, nai = wp.a2 - wp.a1 + 1;
     const double ompor = 1. - 0.0, ypg = ev.difpre;
     const double idd = (double)wp.idir;
-    double s[FF_NFORCE];
-#pragma unroll
-    for (int q = 0; q < FF_NFORCE; q++) s[q] = 0.;
-    for (int c0 = 0; c0 < wp.n; c0 += 64) {
-        const int t = c0 + lane;
-        int NP = 0, g = 0;
+    // the sum this lane carries
+    const int q = tid < FF_NFORCE ? tid : 0;
+    const int ia = kWfTerms[q][0], ib = kWfTerms[q][2];
+    const double sa = (double)kWfTerms[q][1], sb = (double)kWfTerms[q][3];
+    double acc = 0.;
+    for (int c0 = 0; c0 < wp.n; c0 += WF_N) {
+        const int t = c0 + tid;
+        int NP = 0;
         double an = 0.;
         if (t < wp.n) {
             const int I = wp.a1 + t % nai, J = wp.b1 + t / nai;
@@ -1176,7 +1197,7 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
             if (wp.dir == 0) { ci = wp.kbot; cj = I; ck = J; }
             else if (wp.dir == 1) { ci = J; cj = wp.kbot; ck = I; }
             else { ci = I; cj = J; ck = wp.kbot; }
-            g = w.off[wp.face] + wall_idx(b, wp.dir, ci, cj, ck);
+            const int g = w.off[wp.face] + wall_idx(b, wp.dir, ci, cj, ck);
             an = A3[IF1];
             const double nx = A3X[IF1], ny = A3Y[IF1], nz = A3Z[IF1];
             const double xd = pa[WA_XCP][NP] - ev.xmom, yd = pa[WA_YCP][NP] - ev.ymom;
@@ -1211,44 +1232,33 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
                                     bxdp, bydp, bzdp, idd * fe, w.qterm[g], w.qterm[w.nt + g],
                                     -idd * qx_, -idd * qy_, -idd * qz_, 0.};
 #pragma unroll
-            for (int q = 0; q < WF_T; q++) tm[q][lane] = v[q];
+            for (int k = 0; k < WF_T; k++) tm[k][tid] = v[k];
         }
         __syncthreads();
-        const int m = min(64, wp.n - c0);
-        if (lane == 0) {
+        const int m = min(WF_N, wp.n - c0);
+        if (tid < FF_NFORCE) {
+            // every lane runs the same two additions per face: no divergent paths
+#pragma unroll 4
             for (int r = 0; r < m; r++) {
-                s[FF_CMX] = s[FF_CMX] + tm[0][r]; s[FF_CMX] = s[FF_CMX] - tm[1][r];
-                s[FF_CMY] = s[FF_CMY] + tm[2][r]; s[FF_CMY] = s[FF_CMY] - tm[3][r];
-                s[FF_CMZ] = s[FF_CMZ] + tm[4][r]; s[FF_CMZ] = s[FF_CMZ] - tm[5][r];
-                s[FF_CX] = s[FF_CX] + tm[6][r];
-                s[FF_CY] = s[FF_CY] + tm[7][r];
-                s[FF_CZ] = s[FF_CZ] + tm[8][r];
-                s[FF_TOM] = s[FF_TOM] - tm[9][r];
-                s[FF_DX] = s[FF_DX] - tm[10][r];
-                s[FF_DY] = s[FF_DY] - tm[11][r];
-                s[FF_DZ] = s[FF_DZ] - tm[12][r];
-                run[0][r] = s[FF_DX]; run[1][r] = s[FF_DY]; run[2][r] = s[FF_DZ];
-                s[FF_VMX] = s[FF_VMX] + tm[13][r]; s[FF_VMX] = s[FF_VMX] - tm[14][r];
-                s[FF_VMY] = s[FF_VMY] + tm[15][r]; s[FF_VMY] = s[FF_VMY] - tm[16][r];
-                s[FF_VMZ] = s[FF_VMZ] + tm[17][r]; s[FF_VMZ] = s[FF_VMZ] - tm[18][r];
-                s[FF_BUX] = s[FF_BUX] + tm[19][r];
-                s[FF_BUY] = s[FF_BUY] + tm[20][r];
-                s[FF_BUZ] = s[FF_BUZ] + tm[21][r];
-                s[FF_QX] = s[FF_QX] - tm[22][r];
-                s[FF_QY] = s[FF_QY] + tm[23][r];
-                s[FF_QY] = s[FF_QY] + 0.;                    // + RSUM*UROT*PB, UROT = 0
-                s[FF_QZ] = s[FF_QZ] + tm[24][r];
+                acc = acc + sa * tm[ia][r];
+                acc = acc + sb * tm[ib][r];
+                run[q][r] = acc;
             }
         }
         __syncthreads();
         if (t < wp.n) {
-            pa[WA_SURFPX][NP] = (run[0][lane] + tm[25][lane]) / (an + 1.E-20);
-            pa[WA_SURFPY][NP] = (run[1][lane] + tm[26][lane]) / (an + 1.E-20);
-            pa[WA_SURFPZ][NP] = (run[2][lane] + tm[27][lane]) / (an + 1.E-20);
+            pa[WA_SURFPX][NP] = (run[FF_DX][tid] + tm[25][tid]) / (an + 1.E-20);
+            pa[WA_SURFPY][NP] = (run[FF_DY][tid] + tm[26][tid]) / (an + 1.E-20);
+            pa[WA_SURFPZ][NP] = (run[FF_DZ][tid] + tm[27][tid]) / (an + 1.E-20);
         }
         __syncthreads();
     }
-    if (lane == 0) {
+    if (tid < FF_NFORCE) fin[tid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double s[FF_NFORCE];
+#pragma unroll
+        for (int k = 0; k < FF_NFORCE; k++) s[k] = fin[k];
         s[FF_TOM] = s[FF_TOM] + idd * s[FF_QZ];
         s[FF_QZ] = idd * s[FF_QZ];
         s[FF_QY] = idd * s[FF_QY];
@@ -1256,7 +1266,7 @@ __global__ void __launch_bounds__(64) k_wall_forces(Prm p, Blk b, WallDev w,
         s[FF_DY] = s[FF_DY] + s[FF_BUY];
         s[FF_DZ] = s[FF_DZ] + s[FF_BUZ];
 #pragma unroll
-        for (int q = 0; q < FF_NFORCE; q++) sums[blockIdx.x * FF_NFORCE + q] = s[q];
+        for (int k = 0; k < FF_NFORCE; k++) sums[blockIdx.x * FF_NFORCE + k] = s[k];
     }
 }
 
@@ -3495,6 +3505,7 @@ struct BlockRec {
     WallRec walls;
     SweepSt st{};              // staged records of the (J,K)-tile sweeps (allocated on first use)
     double *st_mem = nullptr;
+    void *dmx_mem = nullptr;   // DROMAX chunk maxima and candidates (allocated on first use)
 };
 
 struct PendingEv {
@@ -3756,6 +3767,8 @@ static void sweep_free(BlockRec &r)
     r.sw_mem = nullptr;
     if (r.kw_scr) (void)hipFree(r.kw_scr);
     r.kw_scr = nullptr;
+    if (r.dmx_mem) (void)hipFree(r.dmx_mem);
+    r.dmx_mem = nullptr;
 }
 
 // Read the error word of the block's sweeps; after a failed halo wait, refill the slots
@@ -4152,7 +4165,7 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
         HIPCHK(c, hipEventRecord(wr.fwd, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->wstream, wr.fwd, 0));
         int e = launch_on(c, c->wstream, "k_wall_forces", k_wall_forces, dim3((unsigned)wr.dev.size()),
-                          dim3(64), make_prm(&c->prm, level), b, wr.wd,
+                          dim3(WF_N), make_prm(&c->prm, level), b, wr.wd,
                           (const WallPatchDev *)wr.dev_tab, wr.env, wr.sums);
         if (e) return e;
         HIPCHK(c, hipEventRecord(wr.done, c->wstream));
@@ -4795,3 +4808,4 @@ int ff_wall_forces(ff_ctx *c, int block, int patch, double *sums)
 }  // extern "C"
 
 #include "ffmlp.hip"
+#include "ffconv.hip"

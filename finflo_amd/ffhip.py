@@ -23,7 +23,7 @@ EXPORTS = [
     "ff_implicit", "ff_update", "ff_save_old", "ff_cycle", "ff_restrict", "ff_forcing",
     "ff_prolong", "ff_vcycle", "ff_vcycle_graph", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
-    "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc",
+    "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc", "ff_dromax",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
@@ -63,6 +63,7 @@ def lib():
             getattr(L, fn).argtypes = [vp, i, i]
         L.ff_residual.argtypes = [vp, i, i, i, vp]
         L.ff_residual_noalloc.argtypes = [vp, i, C.POINTER(BlockView), i, vp]
+        L.ff_dromax.argtypes = [vp, i, i, C.POINTER(d), C.POINTER(i), C.POINTER(i)]
         L.ff_implicit.argtypes = [vp, i, i, i]
         L.ff_cycle.argtypes = [vp, i, i, vp]
         L.ff_restrict.argtypes = [vp, i, i]
@@ -177,6 +178,14 @@ class Context:
         resi = np.zeros(7)
         self._chk(lib().ff_cycle(self.h, block, ncycles, resi.ctypes.data), "ff_cycle")
         return resi
+
+    def dromax(self, state=(1e-20, 1, 0), block=1, idrxx=6):
+        """DROMXX of block N on the running state (DROMAX, IXERR, INERR) of SUBIN's block
+        loop; returns the new state."""
+        dm, ix, inn = C.c_double(state[0]), C.c_int(state[1]), C.c_int(state[2])
+        self._chk(lib().ff_dromax(self.h, block, idrxx, C.byref(dm), C.byref(ix), C.byref(inn)),
+                  "ff_dromax")
+        return dm.value, ix.value, inn.value
 
     # ---- multigrid (TRANSF/TRANKE, FORCFU/ADDRES, INTERP/INTERT, the V-cycle) ----------
     def register_levels(self, blocks, block=1):

@@ -272,6 +272,18 @@ int  ff_vcycle_graph(ff_ctx *ctx, int block, int nlev, int ncycles);
  * of the last cycle. */
 int  ff_cycle(ff_ctx *ctx, int block, int ncycles, double *resi);
 
+/* SUBIN's convergence monitor for block N (ns3c.f:9877-9905): the change of the cycle that
+ * IDRXX selects, over every entry of the level-1 slab including the ghost layers (SUBIN
+ * ns3c.f:9836-9844: 2 DM = RO*(U-UOLD), 3 DN, 4 DW, 5 TEMP-TOLD, 6 DE = PDIFF-POLD, the
+ * default, main.f:5280; 8 DEPS = REPS-EPSOLD), scanned as DROMXX (res3c.f:9293-9324) does:
+ * *dromax, *ixerr (1-based slab entry) and *inerr (block) are the running maximum, its
+ * entry and its block in and out, as SUBIN carries them across its block loop (DROMAX =
+ * 1E-20, IXERR = 1 before the first block, ns3c.f:9799-9800).  On return *dromax is the
+ * signed, ghost-weighted change when this block holds the maximum.  Reads the state after
+ * ff_update and the *OLD copies of ff_save_old; writes no field.  IDRXX 1 and 7 (which need
+ * ROLD and SUBIN's k limiter) are FF_E_UNSUP. */
+int  ff_dromax(ff_ctx *ctx, int block, int idrxx, double *dromax, int *ixerr, int *inerr);
+
 /* ff_residual of a block given directly as caller-owned device arrays (the field pointers
  * of `dev` are device addresses), with no registration and no device allocation: every
  * scratch buffer (the RESL2 partials and sums, the k-omega SEPS maxima) lives in a fixed

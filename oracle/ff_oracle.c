@@ -2716,3 +2716,71 @@ void ffo_lusgs(const ff_params *p, ff_block *b)
     diaggs(p, b, ke);
     corrgs(p, b, ke);
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* SUBIN's change of the cycle (ns3c.f:9836-9859) and DROMXX (res3c.f:9293-9324)          */
+/* ------------------------------------------------------------------------------------ */
+
+/* The change of entry n (0-based, ghosts included) that IDRXX selects (ns3c.f:9877-9900):
+ * 2 DM = RO*(U-UOLD), 3 DN, 4 DW (ns3c.f:9840-9842), 5 HAT1 = TEMP-TOLD (9844),
+ * 6 DE = PDIFF-POLD (9843), 8 DEPS = REPS-EPSOLD (9819). */
+static double dro_change(const ff_block *b, int idrxx, long n)
+{
+    switch (idrxx) {
+    case 2: return F(b, FF_RO)[n] * (F(b, FF_U)[n] - F(b, FF_UOLD)[n]);
+    case 3: return F(b, FF_RO)[n] * (F(b, FF_V)[n] - F(b, FF_VOLD)[n]);
+    case 4: return F(b, FF_RO)[n] * (F(b, FF_W)[n] - F(b, FF_WOLD)[n]);
+    case 5: return F(b, FF_TEMP)[n] - F(b, FF_TOLD)[n];
+    case 6: return F(b, FF_PDIFF)[n] - F(b, FF_POLD)[n];
+    default: return F(b, FF_REPS)[n] - F(b, FF_EPSOLD)[n];
+    }
+}
+
+/* IJKPAI (multi.f:3236-3260): 1-based I, J, K of the 0-based slab entry n; ghost layers
+ * give I, J, K <= 0 or > IMAX, JMAX, KMAX */
+static void ijkpai(const dims_t *d, long n, int *i, int *j, int *k)
+{
+    *k = (int)(n / d->il) - G + 1;
+    long r = n - (long)(*k + G - 1) * d->il;
+    *j = (int)(r / d->is) - G + 1;
+    r -= (long)(*j + G - 1) * d->is;
+    *i = (int)r - G + 1;
+}
+
+/* DROMXX over block N's level-1 slab: the running maximum DROMAX (in/out), its entry IXERR
+ * (1-based, in/out) and block INERR (in/out), as SUBIN carries them across its block loop
+ * (DROMAX = 1E-20, IXERR = 1 before the first block, ns3c.f:9799-9800). */
+void ffo_dromax(const ff_params *p, const ff_block *b, int idrxx, int nblock,
+                double *dromax, int *ixerr, int *inerr)
+{
+    (void)p;
+    dims_t d = mkdims(b);
+    double dm = *dromax;
+    int ix = *ixerr, in = *inerr;
+    for (long n = 0; n < d.ntot; n++) {
+        double apu = fabs(dro_change(b, idrxx, n));
+        if (apu > fabs(dm) * 1.000001) {
+            int ii, jj, kk;
+            ijkpai(&d, n, &ii, &jj, &kk);
+            if (ii <= 0 || ii > d.imax) apu = apu * .99999;
+            if (jj <= 0 || jj > d.jmax) apu = apu * .99999;
+            if (kk <= 0 || kk > d.kmax) apu = apu * .99999;
+            if (apu > fabs(dm)) {
+                ix = (int)(n + 1);
+                in = nblock;
+                dm = apu;
+            }
+        }
+    }
+    if (nblock == in) {
+        int ii, jj, kk;
+        dm = dro_change(b, idrxx, ix - 1);
+        ijkpai(&d, ix - 1, &ii, &jj, &kk);
+        if (ii <= 0 || ii > d.imax) dm = dm * .99999;
+        if (jj <= 0 || jj > d.jmax) dm = dm * .99999;
+        if (kk <= 0 || kk > d.kmax) dm = dm * .99999;
+    }
+    *dromax = dm;
+    *ixerr = ix;
+    *inerr = in;
+}

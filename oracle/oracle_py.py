@@ -55,6 +55,9 @@ def oracle():
         _ora.ffo_wall_set.restype = None
         _ora.ffo_wall_forces.argtypes = [C.c_int, C.c_int, C.c_void_p]
         _ora.ffo_wall_forces.restype = None
+        _ora.ffo_dromax.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+        _ora.ffo_dromax.restype = None
     return _ora
 
 
@@ -136,6 +139,15 @@ class OracleBlock:
         out = np.zeros(19)
         oracle().ffo_wall_forces(self.level, patch, out.ctypes.data)
         return out
+
+    def dromax(self, state, nblock=1, idrxx=6):
+        """DROMXX of this block on the running state (DROMAX, IXERR, INERR); returns the
+        new state."""
+        dm = C.c_double(state[0])
+        ix, inn = C.c_int(state[1]), C.c_int(state[2])
+        oracle().ffo_dromax(C.byref(self.prm), C.byref(self._view), idrxx, nblock,
+                            C.byref(dm), C.byref(ix), C.byref(inn))
+        return dm.value, ix.value, inn.value
 
 
 class OracleHierarchy:
