@@ -9,7 +9,7 @@ upper cell of the face, as the reference stores it.
 """
 import numpy as np
 
-from .layout import GHOST, Block, WallParams, WallPatch, default_params
+from .layout import GHOST, BcPatch, Block, WallParams, WallPatch, default_params
 
 
 def stretched(n, length, first=None, ratio=None):
@@ -361,3 +361,16 @@ def config4(seed=0):
     blk = make_hierarchy(256, 128, 128, 1, prm, seed=seed, amp=0.05, kind="velocity",
                          turb=turb, nodes=nodes, turb_amp=0.3, surface=surf)[0]
     return prm, blk, plate_wall(blk, prm)
+
+
+def plate_bcs(blk, wall_faces=(2,)):
+    """The boundary patches of the flat-plate and box cases: a no-slip wall (REFLEC's
+    viscous branch, at rest) on each face of `wall_faces`, the free stream (EXT, GHOSTEXT)
+    on every other face."""
+    ext = {1: (blk.jmax, blk.kmax), 2: (blk.imax, blk.kmax), 3: (blk.imax, blk.jmax)}
+    out = []
+    for face in range(1, 7):
+        na, nb = ext[(face - 1) % 3 + 1]
+        ibc = 8 if face in wall_faces else 2
+        out.append(BcPatch(ibc, face, 1, na, 1, nb, iheat=0, viscous=1))
+    return out

@@ -3489,6 +3489,15 @@ struct WallRec {
     bool pending = false;              // a force pass may still read the scratch
 };
 
+// One REFLEC / GHOSTEXT patch in REFLEC's frame (ffbc.hip): ICON(1,L), face, ILMO, the
+// face plane, the KX/KY ranges after the J-face swap, the stride normal to the face, IHEAT,
+// the viscous-branch flag, TSURF and the device wall velocity arrays (or null)
+struct BcPatchDev {
+    int ibc, iface, dir, idir, kpl, kx1, kx2, ky1, ky2, kstr, iheat, viscous, n;
+    double tsurf;
+    const double *uw, *vw, *ww;
+};
+
 struct BlockRec {
     ff_block host{};
     Blk dev{};
@@ -3506,6 +3515,8 @@ struct BlockRec {
     SweepSt st{};              // staged records of the (J,K)-tile sweeps (allocated on first use)
     double *st_mem = nullptr;
     void *dmx_mem = nullptr;   // DROMAX chunk maxima and candidates (allocated on first use)
+    std::vector<BcPatchDev> bcs;   // boundary patches (ff_bc_register)
+    double *bc_mem = nullptr;      // their wall velocity arrays on the device
 };
 
 struct PendingEv {
@@ -3769,6 +3780,9 @@ static void sweep_free(BlockRec &r)
     r.kw_scr = nullptr;
     if (r.dmx_mem) (void)hipFree(r.dmx_mem);
     r.dmx_mem = nullptr;
+    if (r.bc_mem) (void)hipFree(r.bc_mem);
+    r.bc_mem = nullptr;
+    r.bcs.clear();
 }
 
 // Read the error word of the block's sweeps; after a failed halo wait, refill the slots
@@ -4073,13 +4087,19 @@ static inline int row_tpb(const Blk &b, int maxtpb)
 }
 static inline dim3 all_grid(const Blk &b) { return dim3((b.ntot + 255) / 256); }
 
+static int reflec_impl(ff_ctx *c, BlockRec *r, int level);
+
+// STATIM(M,N,INN,.TRUE.): the state, then REFLEC over the block level's boundary patches
+// (ns3c.f:10497-10512)
 int ff_state(ff_ctx *c, int level, int block)
 {
     if (!c) return FF_E_ARG;
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
     c->lev = level;
-    return launch(c, "k_state", k_state, all_grid(r->dev), dim3(256), make_prm(&c->prm, level), r->dev);
+    int rc = launch(c, "k_state", k_state, all_grid(r->dev), dim3(256), make_prm(&c->prm, level), r->dev);
+    if (rc) return rc;
+    return reflec_impl(c, r, level);
 }
 
 int ff_turb(ff_ctx *c, int level, int block)
@@ -4809,3 +4829,4 @@ int ff_wall_forces(ff_ctx *c, int block, int patch, double *sums)
 
 #include "ffmlp.hip"
 #include "ffconv.hip"
+#include "ffbc.hip"

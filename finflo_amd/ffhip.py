@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .layout import FID, Block, BlockView, Params, WallParams, wall_views
+from .layout import FID, Block, BlockView, Params, WallParams, bc_views, wall_views
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libffhip.so")
@@ -24,6 +24,7 @@ EXPORTS = [
     "ff_prolong", "ff_vcycle", "ff_vcycle_graph", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
     "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc", "ff_dromax",
+    "ff_bc_register", "ff_reflec", "ff_ghostext",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
@@ -64,6 +65,9 @@ def lib():
         L.ff_residual.argtypes = [vp, i, i, i, vp]
         L.ff_residual_noalloc.argtypes = [vp, i, C.POINTER(BlockView), i, vp]
         L.ff_dromax.argtypes = [vp, i, i, C.POINTER(d), C.POINTER(i), C.POINTER(i)]
+        L.ff_bc_register.argtypes = [vp, i, i, vp, i]
+        L.ff_reflec.argtypes = [vp, i, i]
+        L.ff_ghostext.argtypes = [vp, i, i, d, d]
         L.ff_implicit.argtypes = [vp, i, i, i]
         L.ff_cycle.argtypes = [vp, i, i, vp]
         L.ff_restrict.argtypes = [vp, i, i]
@@ -186,6 +190,19 @@ class Context:
         self._chk(lib().ff_dromax(self.h, block, idrxx, C.byref(dm), C.byref(ix), C.byref(inn)),
                   "ff_dromax")
         return dm.value, ix.value, inn.value
+
+    # ---- boundary patches (REFLEC, GHOSTEXT) -----------------------------------------------
+    def bc_register(self, patches, level=1, block=1):
+        """Boundary patches (finflo_amd.layout.BcPatch) of a block level; ff_state then ends
+        with REFLEC over them."""
+        views = bc_views(patches)
+        self._chk(lib().ff_bc_register(self.h, level, block, views, len(patches)), "ff_bc_register")
+
+    def reflec(self, level=1, block=1):
+        self._chk(lib().ff_reflec(self.h, level, block), "ff_reflec")
+
+    def ghostext(self, level=1, block=1, alpha=0.0, beta=0.0):
+        self._chk(lib().ff_ghostext(self.h, level, block, alpha, beta), "ff_ghostext")
 
     # ---- multigrid (TRANSF/TRANKE, FORCFU/ADDRES, INTERP/INTERT, the V-cycle) ----------
     def register_levels(self, blocks, block=1):

@@ -214,3 +214,44 @@ def wall_views(patches):
     for i, wp in enumerate(patches):
         arr[i] = wp.view()
     return arr
+
+
+# ---- boundary patches (ff_bc_patch of include/ffhip.h) -----------------------------------
+
+class BcPatchView(C.Structure):
+    """ff_bc_patch: one REFLEC / GHOSTEXT patch."""
+    _fields_ = [("ibc", C.c_int), ("iface", C.c_int), ("kx1", C.c_int), ("kx2", C.c_int),
+                ("ky1", C.c_int), ("ky2", C.c_int), ("iheat", C.c_int), ("viscous", C.c_int),
+                ("tsurf", C.c_double), ("reserved", C.c_double * 3),
+                ("uwall", C.c_void_p), ("vwall", C.c_void_p), ("wwall", C.c_void_p)]
+
+
+class BcPatch:
+    """A boundary patch: ICON(1,L) = ibc (2 EXT, 7 symmetry, 8-10 solid wall), ICON(3..7,L),
+    ICON(20,L), REFLEC's viscous-branch flag, RCON(5,L) and the wall velocity arrays
+    ((NA+4)*(NB+4), as WallPatch; None = at rest)."""
+
+    def __init__(self, ibc, iface, kx1, kx2, ky1, ky2, iheat=0, viscous=1, tsurf=0.0,
+                 uwall=None, vwall=None, wwall=None):
+        self.ibc, self.iface = ibc, iface
+        self.kx1, self.kx2, self.ky1, self.ky2 = kx1, kx2, ky1, ky2
+        self.iheat, self.viscous, self.tsurf = iheat, viscous, tsurf
+        self.uwall, self.vwall, self.wwall = uwall, vwall, wwall
+        self.size = (kx2 - kx1 + 5) * (ky2 - ky1 + 5)
+
+    def view(self):
+        v = BcPatchView()
+        v.ibc, v.iface, v.kx1, v.kx2, v.ky1, v.ky2 = (self.ibc, self.iface, self.kx1, self.kx2,
+                                                      self.ky1, self.ky2)
+        v.iheat, v.viscous, v.tsurf = self.iheat, self.viscous, self.tsurf
+        for n in ("uwall", "vwall", "wwall"):
+            a = getattr(self, n)
+            setattr(v, n, None if a is None else a.ctypes.data)
+        return v
+
+
+def bc_views(patches):
+    arr = (BcPatchView * max(len(patches), 1))()
+    for i, bp in enumerate(patches):
+        arr[i] = bp.view()
+    return arr

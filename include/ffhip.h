@@ -272,6 +272,44 @@ int  ff_vcycle_graph(ff_ctx *ctx, int block, int nlev, int ncycles);
  * of the last cycle. */
 int  ff_cycle(ff_ctx *ctx, int block, int ncycles, double *resi);
 
+/* ---- boundary patches: REFLEC (turb3.f:5294-5878) and GHOSTEXT (ns3c.f:10979-11226) ---- */
+/* One boundary patch of a block level, in the reference's ICON terms. */
+typedef struct ff_bc_patch {
+    int ibc;        /* ICON(1,L): 2 free stream (EXT: GHOSTEXT), 7 symmetry plane or
+                       singularity, 8-10 solid wall (REFLEC) */
+    int iface;      /* ICON(3,L): 1-3 the I-, J-, K-low face, 4-6 the high faces */
+    int kx1, kx2, ky1, ky2;   /* ICON(4..7,L): J and K on an I-face, I and K on a J-face,
+                                 I and J on a K-face */
+    int iheat;      /* ICON(20,L): 1 takes TSURF = tsurf for the ghost temperature */
+    int viscous;    /* IBC 8-10: 1 takes REFLEC's viscous-surface branch, which the
+                       reference selects with (IDI /= 0 .AND. IFACE <= 3) .OR. (IDI > K
+                       .AND. IFACE >= 4) from the block's IDI1-3 (turb3.f:5540-5541); 0 the
+                       inviscid one */
+    double tsurf;   /* RCON(5,L): the wall temperature of IHEAT = 1 */
+    double reserved[3];
+    const double *uwall, *vwall, *wwall;   /* viscous walls: the wall velocity per face,
+                       (NA+4) x (NB+4) as the patch arrays of ff_wall_patch (host arrays,
+                       copied at registration); NULL: a wall at rest */
+} ff_bc_patch;
+/* Register (or replace, or with n = 0 remove) the boundary patches of block N on level M.
+ * From then on ff_state ends with REFLEC over its IBC 7-10 patches, as every STATIM of
+ * CYCLES does (STATIM(M,N,INN,.TRUE.), ns3c.f:1450, 1622, 1848 -> ns3c.f:10497-10512):
+ * the ghost temperature, VIS, CH, C, DRDP, DRDH (loop 1200), and on viscous walls the
+ * reflected velocities with the wall velocity, the extrapolated RO, P, PDIFF, EPS2 = 1,
+ * VIST = 0, RM, RN, RW, E and the k-omega RK = -RK, REPS = MAX(0, 2 REPS - REPS') (loop
+ * 500); on symmetry planes and inviscid walls the velocities mirrored in the face normal
+ * (loop 1150; as in the reference a K-face takes the I-face normal there, turb3.f:5687-
+ * 5707).  REFLECL = .FALSE., a perfect gas (ISTATE /= 10), no DDEPS. */
+int  ff_bc_register(ff_ctx *ctx, int level, int block, const ff_bc_patch *p, int n);
+/* REFLEC alone over the registered IBC 7-10 patches (CYCLES' own call, ns3c.f:2168). */
+int  ff_reflec(ff_ctx *ctx, int level, int block);
+/* GHOSTEXT over the registered IBC 2 patches: both ghost layers take the free stream at
+ * flow angles alpha, beta (radians; U = FRSVEL cos(alpha) cos(beta), V = FRSVEL sin(alpha),
+ * W = FRSVEL cos(alpha) sin(beta)), RK = RKLIM, REPS = EPSLIM, VIST = FRSMUT with the
+ * k-omega model; no atmosphere model (ALTITUDE < GROUND).  The reference calls it once at
+ * start-up (ns3c.f:294); the ghost cells then stay as set. */
+int  ff_ghostext(ff_ctx *ctx, int level, int block, double alpha, double beta);
+
 /* SUBIN's convergence monitor for block N (ns3c.f:9877-9905): the change of the cycle that
  * IDRXX selects, over every entry of the level-1 slab including the ghost layers (SUBIN
  * ns3c.f:9836-9844: 2 DM = RO*(U-UOLD), 3 DN, 4 DW, 5 TEMP-TOLD, 6 DE = PDIFF-POLD, the

@@ -2619,21 +2619,24 @@ void ffo_prolong(const ff_params *p, const ff_block *cb, ff_block *fb)
 /* One CYCLES iteration (ns3c.f:1400-2137) of a block with nlev multigrid levels
  * b[0] (finest) .. b[nlev-1]: the level-1 stages, the pre-smoothing-only FAS down-loop and
  * the prolongation loop.  resi receives the RESL2 sums of the last level-1 residual. */
+void ffo_statim(const ff_params *p, ff_block *b, int level);
+
+/* CYCLES ns3c.f:1450-2137 with MGRID = nlev: every STATIM with REFLEL = .TRUE. */
 void ffo_vcycle(const ff_params *p, ff_block **b, int nlev, double *resi)
 {
     ffo_save_old(p, b[0]);
-    ffo_state(p, b[0]);
+    ffo_statim(p, b[0], 1);
     ffo_turb(p, b[0]);
     ffo_residual(p, b[0], 1, 1, resi);
     ffo_implicit(p, b[0], 1);
     ffo_update_level(p, b[0], 1);
     for (int m = 2; m <= nlev; m++) {
         ff_block *fine = b[m - 2], *coarse = b[m - 1];
-        ffo_state(p, fine);
+        ffo_statim(p, fine, m - 1);
         ffo_residual(p, fine, m - 1, m, m - 1 == 1 ? resi : NULL);
         if (m > 2) ffo_forcing(p, fine, 1);
         ffo_restrict(p, fine, coarse);
-        ffo_state(p, coarse);
+        ffo_statim(p, coarse, m);
         ffo_residual(p, coarse, m, m, NULL);
         ffo_forcing(p, coarse, 0);
         ffo_forcing(p, coarse, 1);
@@ -2783,4 +2786,240 @@ void ffo_dromax(const ff_params *p, const ff_block *b, int idrxx, int nblock,
     *dromax = dm;
     *ixerr = ix;
     *inerr = in;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Boundary patches: REFLEC turb3.f:5294-5878, GHOSTEXT ns3c.f:10979-11226               */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    const ff_bc_patch *p;
+    int n;
+} bcs_t;
+static bcs_t g_bcs[FFO_MAXLEV + 1];
+
+/* the boundary patches of block level `level` (the caller keeps the array alive) */
+void ffo_bc_set(int level, const ff_bc_patch *p, int n)
+{
+    if (level < 1 || level > FFO_MAXLEV || n < 0) return;
+    g_bcs[level].p = n > 0 ? p : NULL;
+    g_bcs[level].n = n > 0 ? n : 0;
+}
+
+/* one patch's geometry in REFLEC's terms: direction, ILMO (IDIR), the face plane K, the
+ * KX/KY ranges after the J-face swap (turb3.f:5391-5398), the stride KSTR normal to the
+ * face, and the 0-based cell of loop indices (I over KX, J over KY) */
+typedef struct {
+    int dir, idir, kpl, kx1, kx2, ky1, ky2;
+    long kstr;
+} bcgeo_t;
+
+static bcgeo_t bc_geo(const dims_t *d, const ff_bc_patch *bc)
+{
+    bcgeo_t g;
+    g.dir = (bc->iface - 1) % 3;
+    g.idir = bc->iface >= 4 ? -1 : 1;
+    const int nd = g.dir == 0 ? d->imax : (g.dir == 1 ? d->jmax : d->kmax);
+    g.kpl = bc->iface <= 3 ? 1 : nd;            /* IBOT = 1, ITOP = IMAX ...; IBC 7 the same */
+    g.kx1 = bc->kx1; g.kx2 = bc->kx2; g.ky1 = bc->ky1; g.ky2 = bc->ky2;
+    if (g.dir == 1) { g.ky1 = bc->kx1; g.ky2 = bc->kx2; g.kx1 = bc->ky1; g.kx2 = bc->ky2; }
+    g.kstr = g.dir == 0 ? 1 : (g.dir == 1 ? d->is : d->il);
+    return g;
+}
+
+/* L of REFLEC (turb3.f:5528-5530) minus one: I-faces (i, j, k) = (K, I, J), J-faces
+ * (J, K, I), K-faces (I, J, K) */
+static long bc_cell(const dims_t *d, const bcgeo_t *g, int I, int J)
+{
+    if (g->dir == 0) return cid(d, g->kpl, I, J);
+    if (g->dir == 1) return cid(d, J, g->kpl, I);
+    return cid(d, I, J, g->kpl);
+}
+
+static void reflec_patch(const ff_params *p, ff_block *b, const ff_bc_patch *bc)
+{
+    dims_t d = mkdims(b);
+    const int ibc = bc->ibc;
+    if (ibc < 7 || ibc > 10) return;
+    const bcgeo_t g = bc_geo(&d, bc);
+    const int LN = 2;
+    double tsurf = 0.;
+    int ihfl = 0;
+    if (bc->iheat == 1) { tsurf = bc->tsurf; ihfl = 1; }
+    double *U = F(b, FF_U), *V = F(b, FF_V), *W = F(b, FF_W), *T = F(b, FF_TEMP),
+           *RO = F(b, FF_RO), *P = F(b, FF_P), *PD = F(b, FF_PDIFF), *RM = F(b, FF_RM),
+           *RN = F(b, FF_RN), *RW = F(b, FF_RW), *E = F(b, FF_E), *VIS = F(b, FF_VIS),
+           *VIST = F(b, FF_VIST), *CH = F(b, FF_CH), *EPS2 = F(b, FF_EPS2), *C = F(b, FF_C),
+           *DRDP = F(b, FF_DRDP), *DRDH = F(b, FF_DRDH), *RK = F(b, FF_RK), *REPS = F(b, FF_REPS);
+    const int kw = p->iturb >= 3 && p->iturb != 8;
+    /* loop 1200: thermodynamics (turb3.f:5511-5535) */
+    for (int J = g.ky1; J <= g.ky2; J++)
+        for (int I = g.kx1; I <= g.kx2; I++) {
+            const long L = bc_cell(&d, &g, I, J);
+            const long LB = L - g.idir * g.kstr, LT = L + g.idir * g.kstr, LC = L - 2 * g.idir * g.kstr;
+            double tb = (double)(1 - ihfl) * (1. * T[L] - 0. * T[LT]) + (double)ihfl * (2. * tsurf - T[L]);
+            tb = tb < p->tuplim ? tb : p->tuplim;
+            tb = tb > p->tlolim ? tb : p->tlolim;
+            T[LB] = tb;
+            T[LC] = T[LB];
+            VIS[LB] = VIS[L];
+            CH[LB] = CH[L];
+            C[LB] = C[L];
+            DRDP[LB] = DRDP[L];
+            DRDH[LB] = DRDH[L];
+        }
+    if (ibc != 7 && bc->viscous) {
+        /* loop 500: viscous surfaces (turb3.f:5545-5655), REFLECL = .FALSE., ISTATE /= 10 */
+        const int na = g.kx2 - g.kx1 + 1 + 2 * LN;
+        for (int J = g.ky1; J <= g.ky2; J++)
+            for (int I = g.kx1; I <= g.kx2; I++) {
+                const long L = bc_cell(&d, &g, I, J);
+                const long LB = L - g.idir * g.kstr, LT = L + g.idir * g.kstr, LC = L - 2 * g.idir * g.kstr;
+                const long NP = (long)(LN + J - g.ky1) * na + (I - g.kx1 + LN);
+                const double uw = bc->uwall ? bc->uwall[NP] : 0., vw = bc->vwall ? bc->vwall[NP] : 0.;
+                const double ww = bc->wwall ? bc->wwall[NP] : 0.;
+                U[LB] = -U[L] + 2. * uw;
+                V[LB] = -V[L] + 2. * vw;
+                W[LB] = -W[L] + 2. * ww;
+                U[LC] = U[LB];
+                V[LC] = V[LB];
+                W[LC] = W[LB];
+                double r = 1. * RO[L] - 0. * RO[LT];
+                RO[LB] = 0.001 * p->frsden > r ? 0.001 * p->frsden : r;
+                r = 2. * P[L] - P[LT];
+                P[LB] = 0.001 * p->frspre > r ? 0.001 * p->frspre : r;
+                r = 2. * PD[L] - PD[LT];
+                PD[LB] = -0.999 * p->frspre > r ? -0.999 * p->frspre : r;
+                EPS2[LB] = 1.;
+                VIST[LB] = 0.;
+                RM[LB] = RO[LB] * U[LB];
+                RN[LB] = RO[LB] * V[LB];
+                RW[LB] = RO[LB] * W[LB];
+                E[LB] = E[L];
+                RM[LC] = RM[LB];
+                RN[LC] = RN[LB];
+                RW[LC] = RW[LB];
+                E[LC] = E[LB];
+                EPS2[LC] = EPS2[LB];
+                VIST[LC] = VIST[LB];
+                PD[LC] = PD[LB];
+                P[LC] = P[LB];
+                RO[LC] = RO[LB];
+                if (kw) {
+                    RK[LB] = -RK[L];
+                    if (p->iturb != 6) {
+                        REPS[LB] = -REPS[L] + 2. * 0.;
+                    } else {
+                        r = 2. * REPS[L] - REPS[LT];
+                        REPS[LB] = 0. > r ? 0. : r;
+                    }
+                }
+            }
+    } else {
+        /* loop 1150: inviscid surfaces and singularities (turb3.f:5659-5768) */
+        const double *A1X = F(b, FF_A1X), *A1Y = F(b, FF_A1Y), *A1Z = F(b, FF_A1Z),
+                     *A2X = F(b, FF_A2X), *A2Y = F(b, FF_A2Y), *A2Z = F(b, FF_A2Z);
+        for (int J = g.ky1; J <= g.ky2; J++)
+            for (int I = g.kx1; I <= g.kx2; I++) {
+                const long L = bc_cell(&d, &g, I, J);
+                const long LB = L - g.idir * g.kstr, LT = L + g.idir * g.kstr;
+                double r = 1. * RO[L] - 0. * RO[LT];
+                RO[LB] = 0.001 * p->frsden > r ? 0.001 * p->frsden : r;
+                r = 2. * P[L] - P[LT];
+                P[LB] = 0.001 * p->frspre > r ? 0.001 * p->frspre : r;
+                r = 2. * PD[L] - PD[LT];
+                PD[LB] = -0.999 * p->frspre > r ? -0.999 * p->frspre : r;
+                /* the face normal: the low faces at L, the high ones at L + KSTR; the K-faces
+                 * take the I-face normals, as the reference does */
+                const long la = bc->iface >= 4 ? L + g.kstr : L;
+                double a11, a12, a13;
+                if (bc->iface == 2 || bc->iface == 5) { a11 = A2X[la]; a12 = A2Y[la]; a13 = A2Z[la]; }
+                else { a11 = A1X[la]; a12 = A1Y[la]; a13 = A1Z[la]; }
+                double a21, a22, a23, a31, a32, a33;
+                surfvects(a11, a12, a13, &a21, &a22, &a23, &a31, &a32, &a33);
+                double un = a11 * U[L] + a12 * V[L] + a13 * W[L];
+                const double vn = a21 * U[L] + a22 * V[L] + a23 * W[L];
+                const double wn = a31 * U[L] + a32 * V[L] + a33 * W[L];
+                un = -un;
+                U[LB] = a11 * un + a21 * vn + a31 * wn;
+                V[LB] = a12 * un + a22 * vn + a32 * wn;
+                W[LB] = a13 * un + a23 * vn + a33 * wn;
+                EPS2[LB] = 1. * EPS2[L] - 0. * EPS2[LT];
+                VIST[LB] = 1. * VIST[L] - 0. * VIST[LT];
+                if (kw) {
+                    r = 1. * RK[L] - 0. * RK[LT];
+                    RK[LB] = 0. > r ? 0. : r;
+                    r = 1. * REPS[L] - 0. * REPS[LT];
+                    REPS[LB] = 0. > r ? 0. : r;
+                    EPS2[LB] = 1. * EPS2[L] - 0. * EPS2[LT];
+                    VIST[LB] = 1. * VIST[L] - 0. * VIST[LT];
+                }
+            }
+    }
+}
+
+/* REFLEC over the registered patches of this level, in patch order */
+void ffo_reflec(const ff_params *p, ff_block *b, int level)
+{
+    if (level < 1 || level > FFO_MAXLEV) return;
+    for (int l = 0; l < g_bcs[level].n; l++) reflec_patch(p, b, &g_bcs[level].p[l]);
+}
+
+/* STATIM(M,N,INN,.TRUE.): the state, then REFLEC (ns3c.f:10497-10512) */
+void ffo_statim(const ff_params *p, ff_block *b, int level)
+{
+    ffo_state(p, b);
+    ffo_reflec(p, b, level);
+}
+
+/* GHOSTEXT over the registered EXT patches (ICON(1,L) = 2), without the atmosphere model
+ * (ALTITUDE < GROUND) and with INITC /= 2: FRSSIE = -E0REF + RGAS/GM1*(FRSTEM-T0REF) with
+ * E0REF = T0REF = 0 (main.f:3631-3632) */
+void ffo_ghostext(const ff_params *p, ff_block *b, int level, double alpha, double beta)
+{
+    if (level < 1 || level > FFO_MAXLEV) return;
+    dims_t d = mkdims(b);
+    const double gm1 = p->gamma - 1.0;
+    const double frssie = -0. + p->rgas / gm1 * (p->frstem - 0.);
+    const double u10 = p->frsvel * cos(alpha) * cos(beta);
+    const double v10 = p->frsvel * sin(alpha);
+    const double w10 = p->frsvel * cos(alpha) * sin(beta);
+    for (int l = 0; l < g_bcs[level].n; l++) {
+        const ff_bc_patch *bc = &g_bcs[level].p[l];
+        if (bc->ibc != 2) continue;
+        int i0, i1, j0, j1, k0, k1;
+        const int ia = bc->kx1, im = bc->kx2, ja = bc->ky1, jm = bc->ky2;
+        switch (bc->iface) {
+        case 1: i0 = -1; i1 = 0; j0 = ia; j1 = im; k0 = ja; k1 = jm; break;
+        case 2: i0 = ia; i1 = im; j0 = -1; j1 = 0; k0 = ja; k1 = jm; break;
+        case 3: i0 = ia; i1 = im; j0 = ja; j1 = jm; k0 = -1; k1 = 0; break;
+        case 4: i0 = d.imax + 1; i1 = d.imax + 2; j0 = ia; j1 = im; k0 = ja; k1 = jm; break;
+        case 5: i0 = ia; i1 = im; j0 = d.jmax + 1; j1 = d.jmax + 2; k0 = ja; k1 = jm; break;
+        default: i0 = ia; i1 = im; j0 = ja; j1 = jm; k0 = d.kmax + 1; k1 = d.kmax + 2; break;
+        }
+        for (int k = k0; k <= k1; k++)
+            for (int j = j0; j <= j1; j++)
+                for (int i = i0; i <= i1; i++) {
+                    const long kr = cid(&d, i, j, k);
+                    F(b, FF_RO)[kr] = p->frsden;
+                    F(b, FF_P)[kr] = p->frspre;
+                    F(b, FF_PDIFF)[kr] = p->frspre - p->frspre;
+                    F(b, FF_RM)[kr] = p->frsden * u10;
+                    F(b, FF_RN)[kr] = p->frsden * v10;
+                    F(b, FF_RW)[kr] = p->frsden * w10;
+                    F(b, FF_U)[kr] = u10;
+                    F(b, FF_V)[kr] = v10;
+                    F(b, FF_W)[kr] = w10;
+                    F(b, FF_E)[kr] = p->frsden * (frssie + .5 * (u10 * u10 + v10 * v10 + w10 * w10));
+                    F(b, FF_TEMP)[kr] = p->frstem;
+                    F(b, FF_VIS)[kr] = p->frsvis;
+                    if (p->iturb >= 3 && p->iturb != 8) {
+                        F(b, FF_RK)[kr] = p->rklim;
+                        F(b, FF_REPS)[kr] = p->epslim;
+                        F(b, FF_VIST)[kr] = p->frsmut;
+                        F(b, FF_EPS2)[kr] = 1. + p->frsmut / p->frsvis;
+                        F(b, FF_E)[kr] = F(b, FF_E)[kr] + F(b, FF_RK)[kr];
+                    }
+                }
+    }
 }

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from finflo_amd.layout import wall_views
+from finflo_amd.layout import bc_views, wall_views
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libff_oracle.so")
@@ -58,6 +58,14 @@ def oracle():
         _ora.ffo_dromax.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
         _ora.ffo_dromax.restype = None
+        _ora.ffo_bc_set.argtypes = [C.c_int, C.c_void_p, C.c_int]
+        _ora.ffo_bc_set.restype = None
+        _ora.ffo_reflec.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _ora.ffo_reflec.restype = None
+        _ora.ffo_statim.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _ora.ffo_statim.restype = None
+        _ora.ffo_ghostext.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double]
+        _ora.ffo_ghostext.restype = None
     return _ora
 
 
@@ -80,6 +88,10 @@ def ref():
         _ref.ref_wall_set.restype = None
         _ref.ref_wall_forces.argtypes = [C.c_int, C.c_int, C.c_void_p]
         _ref.ref_wall_forces.restype = None
+        _ref.ref_reflec.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_int]
+        _ref.ref_reflec.restype = None
     return _ref
 
 
@@ -89,14 +101,28 @@ class OracleBlock:
     def __init__(self, prm, blk, level=1):
         self.prm, self.blk, self.level = prm, blk, level
         self._view = blk.view_struct()
-        # the oracle keeps wall patches per level: a new block starts without any
+        # the oracle keeps wall and boundary patches per level: a new block starts without
         oracle().ffo_wall_set(level, None, None, 0)
+        oracle().ffo_bc_set(level, None, 0)
 
     def _args(self):
         return C.byref(self.prm), C.byref(self._view)
 
     def state(self):
-        oracle().ffo_state(*self._args())
+        """STATIM(M,N,INN,.TRUE.): the state, then REFLEC over the boundary patches."""
+        oracle().ffo_statim(*self._args(), self.level)
+
+    def set_bcs(self, patches):
+        """Boundary patches (finflo_amd.layout.BcPatch) of this level."""
+        self._bcv = bc_views(patches)
+        self._bcp = patches
+        oracle().ffo_bc_set(self.level, self._bcv, len(patches))
+
+    def reflec(self):
+        oracle().ffo_reflec(*self._args(), self.level)
+
+    def ghostext(self, alpha=0.0, beta=0.0):
+        oracle().ffo_ghostext(*self._args(), self.level, alpha, beta)
 
     def turb(self):
         oracle().ffo_turb(*self._args())
@@ -197,7 +223,13 @@ class RefBlock:
                 self.iv.ctypes.data)
 
     def state(self):
+        """STATIM: the state, then (REFLEL = .TRUE.) REFLEC over the patches of set_bcs."""
         ref().ref_state(*self._a())
+        if getattr(self, "_bcs", None):
+            self.reflec(self._bcs)
+
+    def set_bcs(self, patches):
+        self._bcs = list(patches)
 
     def turb(self):
         ref().ref_turb(*self._a())
@@ -246,3 +278,20 @@ class RefBlock:
         out = np.zeros(19)
         ref().ref_wall_forces(self.level, patch, out.ctypes.data)
         return out
+
+    def reflec(self, patches):
+        """The reference's REFLEC over the given boundary patches (BcPatch)."""
+        n = len(patches)
+        ints = np.zeros((max(n, 1), 8), dtype=np.int32)
+        tsurf = np.zeros(max(n, 1))
+        arrs = {k: [] for k in ("uwall", "vwall", "wwall")}
+        for l, bp in enumerate(patches):
+            ints[l] = [bp.ibc, bp.iface, bp.kx1, bp.kx2, bp.ky1, bp.ky2, bp.iheat, bp.viscous]
+            tsurf[l] = bp.tsurf
+            for k in arrs:
+                a = getattr(bp, k)
+                arrs[k].append(np.zeros(bp.size) if a is None else np.asarray(a, float))
+        cat = {k: (np.concatenate(v) if v else np.zeros(1)) for k, v in arrs.items()}
+        ref().ref_reflec(*self._a(), n, ints.ctypes.data, tsurf.ctypes.data,
+                         cat["uwall"].ctypes.data, cat["vwall"].ctypes.data,
+                         cat["wwall"].ctypes.data, len(cat["uwall"]))

@@ -794,3 +794,65 @@ subroutine ref_update(f, ntot, prm, ipr) bind(C, name="ref_update")
        f(1,IYC), f(1,IZC), imax, jmax, kmax, 2, 2, 2)
   deallocate(fi, dfi, fiold)
 end subroutine ref_update
+
+! REFLEC (turb3.f:5294-5878) over npatch boundary patches, as STATIM calls it with
+! REFLEL = .TRUE. (ns3c.f:10497-10512): ICON(1,L) = ibc, ICON(2,L) = L, ICON(3..7,L) =
+! iface, kx1..ky2, ICON(20,L) = iheat, RCON(5,L) = tsurf; the wall velocity arrays of the
+! patches are concatenated at IHF offsets.  The viscous branch is selected, per direction,
+! by IDI = IMAX+1 (every wall of that direction viscous) or 0 (none), with IBOT = 1 and
+! ITOP = IMAX; SOLUTION_TYPE = 'FLUID', ISTATE = 1, REFLECL = MULPHL = TRANSL = .FALSE.,
+! IPRESC = 0.
+subroutine ref_reflec(f, ntot, prm, ipr, npatch, ints, tsurf, uw, vw, ww, nwarr) &
+     bind(C, name="ref_reflec")
+  use iso_c_binding
+  use ffref_ids
+  use TYPE_ARRAYS
+  use NS3CO, only : IC9
+  implicit none
+  integer(c_int), value :: ntot, npatch, nwarr
+  real(c_double) :: f(ntot, *), prm(*), tsurf(*), uw(*), vw(*), ww(*)
+  integer(c_int) :: ipr(*), ints(8, *)
+  integer, allocatable :: icon(:,:), ihf(:)
+  real(c_double), allocatable :: rcon(:,:), ddeps(:), twall(:)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(INTERMITTENCY) :: trm(1)
+  type(PRE_COR) :: prc(1)
+  integer :: l, idi(3), imax, jmax, kmax, dir, nd(3), off, na, nb
+  character(len=10) :: soltype
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  nd = (/ imax, jmax, kmax /)
+  allocate(icon(IC9, max(npatch, 1)), rcon(IC9, max(npatch, 1)), ihf(max(npatch, 1)))
+  allocate(ddeps(ntot), twall(max(nwarr, 1)))
+  icon = 0; rcon = 0.; ddeps = 0.; twall = 0.
+  idi = 0
+  off = 0
+  do l = 1, npatch
+     icon(1, l) = ints(1, l)
+     icon(2, l) = l
+     icon(3:7, l) = ints(2:6, l)
+     icon(20, l) = ints(7, l)
+     rcon(5, l) = tsurf(l)
+     dir = mod(ints(2, l) - 1, 3) + 1
+     if (ints(8, l) /= 0) idi(dir) = nd(dir) + 1
+     ! IHF(IPL): the 1-based start of the patch's arrays, each (NA+4)*(NB+4) with LN = 2
+     ! ghost rows each side (NR = ... + IHF(IPL), turb3.f:5545; BOTSKE's patch-local form
+     ! has + 1 in its place, base3c.f:6359)
+     ihf(l) = off + 1
+     na = ints(4, l) - ints(3, l) + 1
+     nb = ints(6, l) - ints(5, l) + 1
+     off = off + (na + 4) * (nb + 4)
+  end do
+  soltype = 'FLUID'
+  call REFLEC(f(1,IU), f(1,IV), f(1,IW), f(1,IXC), f(1,IYC), f(1,IZC), imax, jmax, kmax, &
+       1, imax, 1, jmax, 1, kmax, idi(1), idi(2), idi(3), 2, 2, 2, &
+       prm(PGAMMA), 0._c_double, 0._c_double, prm(PFRSDEN), prm(PFRSPRE), 1, npatch, icon, &
+       f(1,IRO), f(1,IP), f(1,IPDIFF), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IE), &
+       f(1,IVIS), f(1,IVIST), f(1,ICH), f(1,IEPS2), f(1,ITEMP), f(1,IC), f(1,IDRDP), &
+       f(1,IDRDH), f(1,IRK), f(1,IREPS), ddeps, pro, var, prm(PRGAS), prm(PFRSTEM), &
+       ipr(QITURB), 1, 1, 1, rcon, uw, vw, ww, twall, ihf, prm(PTLOLIM), prm(PTUPLIM), &
+       soltype, .false., .false., .false., trm, &
+       f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), &
+       f(1,IA2X), f(1,IA2Y), f(1,IA2Z), prc, 0)
+  deallocate(icon, rcon, ihf, ddeps, twall)
+end subroutine ref_reflec
