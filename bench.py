@@ -156,12 +156,18 @@ def _cpu_worker(prm, blocks, walls, seconds, use_ref, q):
     q.put((n, el))
 
 
-def converge(ctx, nlev, ncell, max_cycles, chunk=10):
+def converge(ctx, nlev, ncell, max_cycles, chunk=10, periodic=False):
     """V-cycles (iterations with one level) from the current device state until the density
     residual RESI(31) = sqrt(sum DRO^2)/N (SUBIN ns3c.f:10248) has fallen 8 decades below
     that of the first cycle, or max_cycles.  The residual is read every `chunk` cycles (one
-    stream synchronisation each); the wall-clock time includes them."""
-    step = (lambda k: ctx.vcycle(nlev, ncycles=k)) if nlev > 1 else (lambda k: ctx.cycle(ncycles=k))
+    stream synchronisation each); the wall-clock time includes them.  periodic: the block's
+    K faces are connected to each other and exchanged after every cycle."""
+    if periodic:
+        step = lambda k: ctx.vcycle_blocks([1], nlev, ncycles=k)
+    elif nlev > 1:
+        step = lambda k: ctx.vcycle(nlev, ncycles=k)
+    else:
+        step = lambda k: ctx.cycle(ncycles=k)
     ctx.finish()
     t0 = time.perf_counter()
     r0 = float(np.sqrt(step(1)[0]) / ncell)
@@ -374,6 +380,11 @@ def main():
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
                     help="replay the steps from a captured HIP graph")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cfl", type=float, default=None,
+                    help="CFL number of every level (default: the workload's, 5)")
+    ap.add_argument("--kbc", choices=["ext", "periodic"], default="ext",
+                    help="the K faces in the convergence run: free stream (EXT) or connected "
+                         "to each other (periodic span)")
     ap.add_argument("--converge-cycles", type=int, default=0,
                     help="after the timing, run from the initial state until an 8-decade drop "
                          "of the density residual or this many cycles (0: skip)")
@@ -400,6 +411,8 @@ def main():
     w = WORKLOADS[args.workload]
     nlev, iturb = w["nlev"], w["iturb"]
     prm, blocks, walls = make_workload(args.workload)
+    if args.cfl:
+        prm.cfl = prm.cfll = args.cfl
     ncell = IMAX * JMAX * KMAX
     cpu = None
     if not args.no_cpu:
@@ -420,7 +433,10 @@ def main():
             ctx.upload(level=m)
             ctx.wall_register(*walls[m - 1], level=m)
             ctx.wall_sync(0, level=m)
-            ctx.bc_register(plate_bcs(blocks[m - 1]), level=m)
+            bcs = plate_bcs(blocks[m - 1])
+            if args.kbc == "periodic":
+                bcs = [bp for bp in bcs if bp.iface not in (3, 6)]
+            ctx.bc_register(bcs, level=m)
             ctx.ghostext(level=m)
         ctx.finish()
 
@@ -441,6 +457,22 @@ def main():
     if not np.all(np.isfinite(resi)):
         print(f"[bench] non-finite residual norms {resi}", file=sys.stderr, flush=True)
         sys.exit(1)
+
+    # the same K steps in the mixed form (the LU-SGS records in FP32), for the record; the
+    # headline value stays the FP64 path's
+    mixed = None
+    if nlev > 1 or iturb == 0:
+        ctx.precision("mixed")
+        run_steps(ctx, nlev, max(1, args.warmup))
+        ctx.finish()
+        t0 = time.perf_counter()
+        rm = run_steps(ctx, nlev, args.steps, args.graph)
+        ctx.finish()
+        elm = time.perf_counter() - t0
+        mixed = {"ms_per_step": elm * 1e3 / args.steps, "value": ncell * args.steps / elm,
+                 "speedup_over_fp64": el / elm, "finite": bool(np.all(np.isfinite(rm)))}
+        ctx.precision("fp64")
+        print(f"[bench] mixed form: {mixed['ms_per_step']:.3f} ms per step", file=sys.stderr, flush=True)
 
     # the same K steps again with a HIP event pair around every launch on the library's
     # stream: per-kernel durations for the roofline (events add host overhead, so they stay
@@ -519,7 +551,14 @@ def main():
         print(f"[bench] convergence from the initial state, up to {args.converge_cycles} "
               f"{'V-cycles' if nlev > 1 else 'iterations'}", file=sys.stderr, flush=True)
         start_state()
-        conv = converge(ctx, nlev, ncell, args.converge_cycles)
+        if args.kbc == "periodic":
+            b0 = blocks[0]
+            ctx.con_register([(3, 1, b0.imax, 1, b0.jmax, 1, 6, 1, 1, 0, 1, 1),
+                              (6, 1, b0.imax, 1, b0.jmax, 1, 3, 1, 1, 0, 1, 1)])
+            ctx.halo_exchange(what="geometry")
+            ctx.halo_exchange(what="flow")
+        conv = converge(ctx, nlev, ncell, args.converge_cycles, periodic=args.kbc == "periodic")
+        conv["k_faces"] = args.kbc
         print(f"[bench] convergence: {conv['decades']} decades in {conv['cycles']} cycles, "
               f"{conv['seconds']:.2f} s", file=sys.stderr, flush=True)
 
@@ -547,6 +586,7 @@ def main():
                    f"iteration_hbm_GBps_at_{ITER_BYTES_PER_CELL[iturb]}B_per_cell": iter_bw,
                    "density_resl2_after_warmup": float(np.sqrt(resi0[0]) / ncell),
                    "density_resl2": float(np.sqrt(resi[0]) / ncell),
+                   "mixed_precision_form": mixed,
                    "time_to_8_decade_drop": conv["time_to_8_decades"] if conv else None,
                    "convergence": conv},
         "roofline": roofline,

@@ -278,13 +278,19 @@ def flow_params(iturb=0, mach=0.3, reynolds=1e5, cfl=5.0, **kw):
     return prm
 
 
-def freestream_turbulence(prm, tu=0.01, mut_ratio=1.0):
+def freestream_turbulence(prm, tu=0.01, mut_ratio=1.0, limits=False):
     """(RK, REPS) = (rho k, rho omega) of a free stream with turbulence intensity `tu` and
-    eddy-to-molecular viscosity ratio `mut_ratio`."""
-    k = 1.5 * (tu * prm.frsvel) ** 2
-    rk = prm.frsden * k
-    omega = prm.frsden * k / (mut_ratio * prm.frsvis)
-    return rk, prm.frsden * omega
+    eddy-to-molecular viscosity ratio `mut_ratio`.  With limits, the parameters take them as
+    the reference does for a two-equation model (main.f:3772-3826: the input RKLIM and EPSLIM
+    are Tu and mu_t/mu; RKLIM = 1.5 rho (Tu U)^2, FRSMUT = (mu_t/mu) mu, EPSLIM = rho RKLIM /
+    FRSMUT for IDIS = 2): the floors of rho k and rho omega, the free-stream mu_t, and the
+    values GHOSTEXT puts into the free-stream ghost cells."""
+    rkii = 1.5 * prm.frsden * (tu * prm.frsvel) ** 2
+    frsmut = mut_ratio * prm.frsvis
+    repsi = prm.frsden * rkii / frsmut
+    if limits:
+        prm.rklim, prm.epslim, prm.frsmut = rkii, repsi, frsmut
+    return rkii, repsi
 
 
 def config1(iflux=1, seed=0):
@@ -304,7 +310,7 @@ def config2(seed=0, nlev=3):
     cell 1e-6 of it and J growth 1.15, k-omega SST, M = 0.2, Re = 5e6, velocity noise
     +/-0.5 %, CFL 5, nlev multigrid levels, the no-slip wall on every level's J-low face."""
     prm = flow_params(iturb=6, mach=0.2, reynolds=5e6, cfl=5.0)
-    turb = freestream_turbulence(prm)
+    turb = freestream_turbulence(prm, limits=True)
     blocks = flat_plate_hierarchy(192, 96, 96, nlev, prm, first=1e-6, ratio=1.15, seed=seed,
                                   amp=0.005, turb=turb)
     return prm, blocks, [plate_wall(b, prm) for b in blocks]
@@ -355,7 +361,7 @@ def config4(seed=0):
     (ISTRES = 1), M = 0.3, Re = 2e6, free stream with +/-5 % velocity and 30 % turbulence
     noise so that the anisotropy is not trivial everywhere."""
     prm = flow_params(iturb=6, mach=0.3, reynolds=2e6, cfl=5.0, istres=1)
-    turb = freestream_turbulence(prm)
+    turb = freestream_turbulence(prm, limits=True)
     surf = gaussian_bump(0.05, 0.2)
     nodes = bump_nodes(256, 128, 128, surf, xlen=2.0, height=1.0, zlen=1.0, first=1e-5)
     blk = make_hierarchy(256, 128, 128, 1, prm, seed=seed, amp=0.05, kind="velocity",
@@ -374,3 +380,157 @@ def plate_bcs(blk, wall_faces=(2,)):
         ibc = 8 if face in wall_faces else 2
         out.append(BcPatch(ibc, face, 1, na, 1, nb, iheat=0, viscous=1))
     return out
+
+
+# ---- multi-block cases ----------------------------------------------------------------------
+
+def _proper(perm, flip):
+    """True when the axis map (sub-block axis n runs along global axis perm[n], reversed when
+    flip[n]) is a rotation, so that volumes stay positive."""
+    inv = sum(1 for a in range(3) for b in range(a + 1, 3) if perm[a] > perm[b])
+    return (inv + sum(bool(f) for f in flip)) % 2 == 0
+
+
+def sub_block(glob, nodes, lo, hi, perm=(0, 1, 2), flip=(False, False, False)):
+    """The sub-block of global cells lo..hi (1-based (i, j, k) inclusive) of the block `glob`
+    on node array `nodes`, with its own index axes: its axis n runs along global axis perm[n]
+    (0 = i, 1 = j, 2 = k), reversed when flip[n].  Geometry from the sub-range's nodes, whose
+    ghost nodes are the global block's on its boundary and extrapolated on a cut side (the
+    connection exchange brings the neighbour's geometry there); every other field, ghost
+    cells included, copied from the global block."""
+    if not _proper(perm, flip):
+        raise ValueError("the axis map must be a rotation")
+    g = GHOST
+    (i0, j0, k0), (i1, j1, k1) = lo, hi
+    # the sub-range with two ghost node layers: the global block's own ghost nodes on its
+    # boundary, linear extrapolation on a cut side
+    sub = nodes[k0 - 1:g + k1 + 1 + g, j0 - 1:g + j1 + 1 + g, i0 - 1:g + i1 + 1 + g].copy()
+    gmax = (glob.imax, glob.jmax, glob.kmax)
+    for ax, (a0, a1) in zip((2, 1, 0), ((i0, i1), (j0, j1), (k0, k1))):
+        gm = gmax[2 - ax]
+        x = np.moveaxis(sub, ax, 0)
+        if a0 > 1:
+            x[1] = 2 * x[2] - x[3]
+            x[0] = 2 * x[1] - x[2]
+        if a1 < gm:
+            x[-2] = 2 * x[-3] - x[-4]
+            x[-1] = 2 * x[-2] - x[-3]
+    A = np.transpose(sub, (2, 1, 0, 3))                   # [i][j][k]
+    B = np.transpose(A, tuple(perm) + (3,))
+    for n in range(3):
+        if flip[n]:
+            B = np.flip(B, axis=n)
+    B = np.ascontiguousarray(np.transpose(B, (2, 1, 0, 3)))   # [k'][j'][i']
+    dims = [(hi[perm[n]] - lo[perm[n]] + 1) for n in range(3)]
+    blk = Block(*dims)
+    fill_geometry(blk, B)
+    geom = set(("VOL", "A1", "A1X", "A1Y", "A1Z", "A2", "A2X", "A2Y", "A2Z", "A3", "A3X",
+                "A3Y", "A3Z", "D1", "D2", "D3", "XC", "YC", "ZC"))
+    from .layout import FIELDS
+    for name in FIELDS:
+        if name in geom:
+            continue
+        v = glob.view3(name)[k0 - 1:k1 + 4, j0 - 1:j1 + 4, i0 - 1:i1 + 4]
+        C = np.transpose(v, (2, 1, 0))
+        C = np.transpose(C, perm)
+        for n in range(3):
+            if flip[n]:
+                C = np.flip(C, axis=n)
+        blk.view3(name)[...] = np.transpose(C, (2, 1, 0))
+    return blk
+
+
+def block_from_nodes(nodes, lo, hi, prm, seed=0, amp=0.0, kind="velocity", turb=None,
+                     turb_amp=None, surface=None):
+    """The block of global cells lo..hi (1-based, inclusive) of node array `nodes`, in the
+    global index orientation: geometry from its nodes with extrapolated ghost nodes, and a
+    seeded state of its own."""
+    g = GHOST
+    (i0, j0, k0), (i1, j1, k1) = lo, hi
+    sub = nodes[g + k0 - 1:g + k1 + 1, g + j0 - 1:g + j1 + 1, g + i0 - 1:g + i1 + 1]
+    for ax in range(3):
+        sub = _extend_axis(sub, ax)
+    blk = Block(i1 - i0 + 1, j1 - j0 + 1, k1 - k0 + 1)
+    fill_geometry(blk, sub)
+    if surface is not None:
+        blk["DISTW"] = np.abs(blk["YC"] - surface(blk["XC"], blk["ZC"]))
+    fill_state(blk, prm, seed=seed, amp=amp, kind=kind, turb=turb, turb_amp=turb_amp)
+    return blk, sub
+
+
+def config3_layout(seed=3, target=8.0e6, tol=0.07):
+    """The segment lengths of the 24-block case: 4 x 3 x 2 blocks whose dimensions are drawn
+    from multiples of 4 in [16, 96] with a seeded generator until the total is within `tol`
+    of `target` cells."""
+    rng = np.random.default_rng(seed)
+    while True:
+        si = rng.integers(4, 25, 4) * 4
+        sj = rng.integers(4, 25, 3) * 4
+        sk = rng.integers(4, 25, 2) * 4
+        n = int(si.sum() * sj.sum() * sk.sum())
+        if abs(n - target) <= tol * target:
+            return si, sj, sk
+
+
+def config3(seed=3, nlev=3):
+    """The batched form's case: 24 blocks (4 x 3 x 2, seeded dimensions in [16, 96], about
+    8M cells) tiling a 2 x 1 x 1 box over a smoothed Gaussian wall bump (height 0.05, width
+    0.2, first wall cell 1e-5), k-omega SST, M = 0.5, Re = 1e6, CFL 5, nlev multigrid levels.
+    Returns the parameters, the blocks (each a list of its levels, block ids 1..24), their
+    connection patches (tuples of ff_con_patch), their walls (the J-low faces of the bottom
+    row) and their boundary patches (walls and free stream)."""
+    prm = flow_params(iturb=6, mach=0.5, reynolds=1e6, cfl=5.0)
+    turb = freestream_turbulence(prm, limits=True)
+    si, sj, sk = config3_layout(seed)
+    surf = gaussian_bump(0.05, 0.2)
+    nodes = bump_nodes(int(si.sum()), int(sj.sum()), int(sk.sum()), surf, xlen=2.0,
+                       height=1.0, zlen=1.0, first=1e-5)
+    ci, cj, ck = np.concatenate([[0], np.cumsum(si)]), np.concatenate([[0], np.cumsum(sj)]), \
+        np.concatenate([[0], np.cumsum(sk)])
+    ids = {}
+    blocks, cons, walls, bcs = [], [], [], []
+    for c in range(2):
+        for b in range(3):
+            for a in range(4):
+                ids[(a, b, c)] = len(ids) + 1
+    for (a, b, c), n in ids.items():
+        lo = (ci[a] + 1, cj[b] + 1, ck[c] + 1)
+        hi = (ci[a + 1], cj[b + 1], ck[c + 1])
+        fine, sub = block_from_nodes(nodes, lo, hi, prm, seed=seed * 100 + n, amp=0.005,
+                                     kind="velocity", turb=turb, turb_amp=0.0, surface=surf)
+        lv = [fine]
+        cn = sub
+        for m in range(1, nlev):
+            cn = coarsen_nodes(cn)
+            cb = Block(fine.imax >> m, fine.jmax >> m, fine.kmax >> m)
+            fill_geometry(cb, cn)
+            cb["DISTW"] = np.abs(cb["YC"] - surf(cb["XC"], cb["ZC"]))
+            fill_state(cb, prm, seed=0, amp=0.0, kind="velocity", turb=turb, turb_amp=0.0)
+            lv.append(cb)
+        blocks.append(lv)
+        # connections: full faces, the same index orientation on both sides
+        pc = []
+        nb = {1: (a - 1, b, c), 4: (a + 1, b, c), 2: (a, b - 1, c), 5: (a, b + 1, c),
+              3: (a, b, c - 1), 6: (a, b, c + 1)}
+        for face, key in nb.items():
+            if key in ids:
+                ea = fine.jmax if face in (1, 4) else fine.imax
+                eb = fine.jmax if face in (3, 6) else fine.kmax
+                opp = face + 3 if face <= 3 else face - 3
+                pc.append((face, 1, ea, 1, eb, ids[key], opp, 1, 1, 0, 1, 1))
+        cons.append(pc)
+        wl, bl = [], []
+        for m, blk in enumerate(lv):
+            wl.append(plate_wall(blk, prm) if b == 0 else None)
+            faces = []
+            for face, key in nb.items():
+                if key in ids:
+                    continue
+                ea = blk.jmax if face in (1, 4) else blk.imax
+                eb = blk.jmax if face in (3, 6) else blk.kmax
+                ibc = 8 if face == 2 else 2
+                faces.append(BcPatch(ibc, face, 1, ea, 1, eb, iheat=0, viscous=1))
+            bl.append(faces)
+        walls.append(wl)
+        bcs.append(bl)
+    return prm, blocks, cons, walls, bcs

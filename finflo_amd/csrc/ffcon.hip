@@ -1,0 +1,175 @@
+// ffcon.hip — one-to-one block connections on the device: the halo exchange of BOUNDI
+// (EXPTOT/IMPVO2 gluebm.f:6819-7262, CONNEC ns3c.f:8605-8857) as a gather of precomputed
+// (ghost cell, source cell) offset pairs, and the multi-block cycle (CYCLES' block loop with
+// BOUNDI(1) once per cycle, ns3c.f:556, 1400-2195).
+// Included at the end of ffhip.hip (it uses the context and launch helpers defined there).
+
+constexpr int HALO_MAXF = 24;
+struct HaloFields {
+    int n;
+    int f[HALO_MAXF];
+};
+
+// the fields the flow exchange copies (the solved variables, what STATIM derives from them,
+// and the eddy-viscosity fields of TURBDI) and those of the geometry exchange
+static const int kHaloFlow[] = {FF_PDIFF, FF_TEMP, FF_U, FF_V, FF_W, FF_RK, FF_REPS,
+                                FF_P, FF_RO, FF_E, FF_C, FF_VIS, FF_CH, FF_CP, FF_DRDP,
+                                FF_DRDH, FF_RM, FF_RN, FF_RW, FF_EPS2, FF_VIST};
+static const int kHaloGeom[] = {FF_VOL, FF_XC, FF_YC, FF_ZC, FF_DISTW, FF_D1, FF_D2, FF_D3};
+
+// thread per (ghost, source) pair: every listed field of the ghost cell of `dst` takes the
+// value of the source cell of `src`
+__global__ void __launch_bounds__(256) k_halo(Blk dst, Blk src, const int *map, int n, HaloFields hf)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int d = map[2 * t], s = map[2 * t + 1];
+#pragma unroll 4
+    for (int q = 0; q < hf.n; q++) FLD(dst, hf.f[q])[d] = FLD(src, hf.f[q])[s];
+}
+
+// the in-plane extents of a face: (J, K) on an I-face, (I, K) on a J-face, (I, J) on a K-face
+static void face_extent(const Blk &b, int face, int *ea, int *eb)
+{
+    const int dir = (face - 1) % 3;
+    *ea = dir == 0 ? b.jmax : b.imax;
+    *eb = dir == 2 ? b.jmax : b.kmax;
+}
+
+// 0-based offset of the cell at in-plane (a, b) on `face`, `depth` cells in from the face
+// (1, 2: the interior layers; 0, -1: the ghost layers behind it)
+static int face_cell(const Blk &b, int face, int a, int bb, int depth)
+{
+    const int dir = (face - 1) % 3;
+    const int nd = dir == 0 ? b.imax : (dir == 1 ? b.jmax : b.kmax);
+    const int nrm = face <= 3 ? depth : nd + 1 - depth;
+    int i, j, k;
+    if (dir == 0) { i = nrm; j = a; k = bb; }
+    else if (dir == 1) { i = a; j = nrm; k = bb; }
+    else { i = a; j = bb; k = nrm; }
+    return (k + 1) * b.il + (j + 1) * b.is + (i + 1);
+}
+
+extern "C" int ff_con_register(ff_ctx *c, int level, int block, const ff_con_patch *p, int n)
+{
+    if (c) c->gen++;
+    if (!c || n < 0 || (n > 0 && !p)) return FF_E_ARG;
+    BlockRec *r = find(c, level, block);
+    if (!r) return fail(c, FF_E_STATE, "block not registered%s");
+    const Blk &A = r->dev;
+    std::vector<int> pairs;
+    std::vector<size_t> off;
+    for (int l = 0; l < n; l++) {
+        const ff_con_patch &h = p[l];
+        const std::string ls = std::to_string(l);
+        if (h.iface < 1 || h.iface > 6 || h.nface < 1 || h.nface > 6)
+            return fail(c, FF_E_ARG, "ff_con_register: patch %s: face outside 1..6", ls.c_str());
+        BlockRec *nr = find(c, level, h.nblock);
+        if (!nr) return fail(c, FF_E_STATE, "ff_con_register: patch %s: connected block not registered", ls.c_str());
+        const Blk &B = nr->dev;
+        int ea, eb, fa, fb;
+        face_extent(A, h.iface, &ea, &eb);
+        face_extent(B, h.nface, &fa, &fb);
+        if (h.ka1 < 1 || h.ka2 > ea || h.ka1 > h.ka2 || h.kb1 < 1 || h.kb2 > eb || h.kb1 > h.kb2)
+            return fail(c, FF_E_ARG, "ff_con_register: patch %s outside its face", ls.c_str());
+        if ((h.da != 1 && h.da != -1) || (h.db != 1 && h.db != -1) || (h.swap != 0 && h.swap != 1))
+            return fail(c, FF_E_ARG, "ff_con_register: patch %s: orientation (da, db = +-1, swap = 0/1)", ls.c_str());
+        const int ndA = (h.iface - 1) % 3 == 0 ? A.imax : ((h.iface - 1) % 3 == 1 ? A.jmax : A.kmax);
+        const int ndB = (h.nface - 1) % 3 == 0 ? B.imax : ((h.nface - 1) % 3 == 1 ? B.jmax : B.kmax);
+        if (ndA < 2 || ndB < 2)
+            return fail(c, FF_E_UNSUP, "ff_con_register: patch %s: blocks must be two cells thick across a connection", ls.c_str());
+        const int np = h.ka2 - h.ka1 + 1, nq = h.kb2 - h.kb1 + 1;
+        // the far corner must lie on the connected face too
+        for (int corner = 0; corner < 4; corner++) {
+            const int pp = corner & 1 ? np - 1 : 0, qq = corner & 2 ? nq - 1 : 0;
+            const int na = h.swap ? h.na0 + h.da * qq : h.na0 + h.da * pp;
+            const int nb = h.swap ? h.nb0 + h.db * pp : h.nb0 + h.db * qq;
+            if (na < 1 || na > fa || nb < 1 || nb > fb)
+                return fail(c, FF_E_ARG, "ff_con_register: patch %s maps outside the connected face", ls.c_str());
+        }
+        off.push_back(pairs.size() / 2);
+        for (int layer = 1; layer <= 2; layer++)
+            for (int q = 0; q < nq; q++)
+                for (int pp = 0; pp < np; pp++) {
+                    const int na = h.swap ? h.na0 + h.da * q : h.na0 + h.da * pp;
+                    const int nb = h.swap ? h.nb0 + h.db * pp : h.nb0 + h.db * q;
+                    pairs.push_back(face_cell(A, h.iface, h.ka1 + pp, h.kb1 + q, 1 - layer));
+                    pairs.push_back(face_cell(B, h.nface, na, nb, layer));
+                }
+    }
+    off.push_back(pairs.size() / 2);
+    int *map = nullptr;
+    if (!pairs.empty()) {
+        if (hipMalloc(&map, sizeof(int) * pairs.size()) != hipSuccess)
+            return fail(c, FF_E_NOMEM, "connection maps%s");
+        if (hipMemcpy(map, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(map);
+            return fail(c, FF_E_HIP, "connection maps: copy failed%s");
+        }
+    }
+    if (r->con_map) (void)hipFree(r->con_map);
+    r->con_map = map;
+    r->cons.assign(p, p + n);
+    r->con_off = off;
+    return FF_OK;
+}
+
+static int halo_impl(ff_ctx *c, int level, int what)
+{
+    HaloFields hf{};
+    const int *src = what == 0 ? kHaloFlow : kHaloGeom;
+    hf.n = what == 0 ? (int)(sizeof(kHaloFlow) / sizeof(int)) : (int)(sizeof(kHaloGeom) / sizeof(int));
+    for (int q = 0; q < hf.n; q++) hf.f[q] = src[q];
+    c->lev = level;
+    for (auto &kv : c->blocks) {
+        BlockRec &r = kv.second;
+        if (r.level != level || r.cons.empty()) continue;
+        for (size_t l = 0; l < r.cons.size(); l++) {
+            BlockRec *nr = find(c, level, r.cons[l].nblock);
+            if (!nr) return fail(c, FF_E_STATE, "ff_halo_exchange: connected block no longer registered%s");
+            const int np = (int)(r.con_off[l + 1] - r.con_off[l]);
+            int rc = launch(c, "k_halo", k_halo, dim3((np + 255) / 256), dim3(256), r.dev, nr->dev,
+                            (const int *)(r.con_map + 2 * r.con_off[l]), np, hf);
+            if (rc) return rc;
+        }
+    }
+    return FF_OK;
+}
+
+extern "C" int ff_halo_exchange(ff_ctx *c, int level, int what)
+{
+    if (!c || (what != 0 && what != 1)) return FF_E_ARG;
+    return halo_impl(c, level, what);
+}
+
+extern "C" int ff_vcycle_blocks(ff_ctx *c, const int *blocks, int nblocks, int nlev, int ncycles,
+                                double *resi)
+{
+    if (!c || !blocks || nblocks < 1 || nlev < 1 || ncycles < 0) return FF_E_ARG;
+    std::vector<std::vector<BlockRec *>> lv(nblocks);
+    for (int n = 0; n < nblocks; n++) {
+        int rc = vcycle_levels(c, blocks[n], nlev, lv[n]);
+        if (rc) return rc;
+    }
+    double sum[7] = {0, 0, 0, 0, 0, 0, 0}, rs[7];
+    for (int it = 0; it < ncycles; it++) {
+        const bool last = it == ncycles - 1;
+        for (int n = 0; n < nblocks; n++) {
+            int rc = vcycle_once(c, lv[n], nlev, blocks[n], last && resi ? rs : nullptr);
+            if (rc) return rc;
+            if (last && resi)
+                for (int q = 0; q < 7; q++) sum[q] += rs[q];
+        }
+        int rc = halo_impl(c, 1, 0);
+        if (rc) return rc;
+    }
+    if (resi)
+        for (int q = 0; q < 7; q++) resi[q] = sum[q];
+    if (c->lusgs_mode != FF_LUSGS_PLANES)
+        for (int n = 0; n < nblocks; n++)
+            for (int m = 0; m < nlev; m++) {
+                int rc = sweep_check(c, *lv[n][m]);
+                if (rc) return rc;
+            }
+    return FF_OK;
+}

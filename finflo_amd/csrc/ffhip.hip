@@ -2423,7 +2423,7 @@ enum { RC_FAC = 0, RC_VOL, RC_MK, RC_DT1, RC_WI, RC_WJ, RC_SCINV, RC_SCINP, RC_S
 constexpr int NDN = 8;             // rows of predictor increments per step (7 used)
 struct SweepSt {
     int nstep;                      // IMAX + TJ + TK - 2
-    double *recp, *recc;            // [tile][step][NRF][TW]
+    void *recp, *recc;              // [tile][step][NRF][TW] of the record type (FP64 or FP32)
     double *dn;                     // predictor increments [tile][step][NDN][TW]
 };
 
@@ -2475,10 +2475,11 @@ __device__ __forceinline__ FaceBasis basis_get(const double *v)
 // stores keep them from displacing the caches' working set on the way out.
 constexpr int PSW = 4;             // record-pass steps per workgroup (one wave each)
 
-template <bool KE>
+template <bool KE, typename RT = double>
 __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st,
                                                         TileGrid tg)
 {
+    RT *const recp = (RT *)st.recp, *const recc = (RT *)st.recc;
     constexpr int NV = KE ? 7 : 5;
     // logical tiles: (step chunk, tile), chunks of one tile consecutive, so that a tile's
     // columns stay in one XCD's L2 while their cells are read
@@ -2511,7 +2512,7 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
         }
         static_assert(RP_X + 9 == NRF, "the predictor record fills NRF fields");
 #pragma unroll
-        for (int f = 0; f < NRF; f++) __builtin_nontemporal_store(v[f], &st.recp[st_row(st, tile, s, f) + lane]);
+        for (int f = 0; f < NRF; f++) __builtin_nontemporal_store((RT)v[f], &recp[st_row(st, tile, s, f) + lane]);
     }
     {   // corrector (DIAGGS + CORRGS / DIKEGS + COKEGS): upper faces
         SweepRaw r;
@@ -2531,7 +2532,7 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
         static_assert(RC_RHO < NRF, "the corrector record fits NRF fields");
         for (int f = RC_RHO + 1; f < NRF; f++) v[f] = 0.;
 #pragma unroll
-        for (int f = 0; f < NRF; f++) __builtin_nontemporal_store(v[f], &st.recc[st_row(st, tile, s, f) + lane]);
+        for (int f = 0; f < NRF; f++) __builtin_nontemporal_store((RT)v[f], &recc[st_row(st, tile, s, f) + lane]);
     }
 }
 
@@ -2544,7 +2545,8 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
 // and the copies of the RS - 2 later steps may stay in flight).  For the same reason a step
 // loads its halo words before it issues its copy: the wait for them then covers only the
 // copies of earlier steps, which have had a step or more to arrive.
-constexpr int RBYTES = NRF * TW * 8;            // one record: 28 KiB
+template <typename RT>
+constexpr int rbytes() { return NRF * TW * (int)sizeof(RT); }   // one record: 28 or 14 KiB
 constexpr int DBYTES = NDN * TW * 8;            // one step of increments: 4 KiB
 
 // four 1 KiB loads from one address pair: the instruction offset advances the global and the
@@ -2559,15 +2561,26 @@ __device__ __forceinline__ void lds_dma16x4(const char *src, unsigned lds)
                  ::"v"(src), "s"(lds) : "memory", "m0");
 }
 
-// copy `bytes` (a multiple of 4 KiB) from global src to LDS dst, 16 bytes per lane per load
+// one 1 KiB load (the remainder of a record that is not a whole number of 4 KiB groups)
+__device__ __forceinline__ void lds_dma16x1(const char *src, unsigned lds)
+{
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\t"
+                 "global_load_lds_dwordx4 %0, off nt"
+                 ::"v"(src), "s"(lds) : "memory", "m0");
+}
+
+// copy `bytes` (a multiple of 1 KiB) from global src to LDS dst, 16 bytes per lane per load:
+// groups of four loads, then single loads
 template <int BYTES>
 __device__ __forceinline__ void ring_copy(const void *src, void *dst, int lane)
 {
-    static_assert(BYTES % 4096 == 0, "whole groups of four loads");
+    static_assert(BYTES % 1024 == 0, "whole 1 KiB loads");
     const char *g = (const char *)src + lane * 16;
     const unsigned l = lds_addr(dst);
 #pragma unroll
     for (int m = 0; m < BYTES / 4096; m++) lds_dma16x4(g + m * 4096, l + m * 4096);
+#pragma unroll
+    for (int m = (BYTES / 4096) * 4; m < BYTES / 1024; m++) lds_dma16x1(g + m * 1024, l + m * 1024);
 }
 
 // Halo words through LDS as well.  At a step, the edge lanes that read across one tile face
@@ -2704,14 +2717,16 @@ struct FaceSrc {
 // within step s: the step's stores, the halo blocks of step s + 2, the record of step s + 3.
 // At the top of step s at least the two later record copies (56 loads) follow the halo
 // blocks of step s, which follow the record of step s, so vmcnt(56) covers both.
-template <bool KE>
+template <bool KE, typename RT = double>
 __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
     constexpr int NV = KE ? 7 : 5;
     constexpr int RS = 4;                               // ring slots: 3 steps of lead
+    constexpr int RBYTES = rbytes<RT>();
     constexpr int NLD = RBYTES / 1024;                  // record loads per step
     static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
-    __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
+    __shared__ __attribute__((aligned(16))) RT ring[RS][NRF * TW];
+    const RT *const recp = (const RT *)st.recp;
     __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
     __shared__ double xj[2][NV][TW], xk[2][NV][TW], zx[NV * TK];
     const int lane = threadIdx.x, jj = lane % TJ, kk = lane / TJ;
@@ -2734,7 +2749,7 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
         if (hask) halo_dma(hk0 + (size_t)d * HBLK, hring[s % HR][1], lane);
     };
     auto rec_copy = [&](int s) {
-        ring_copy<RBYTES>(st.recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
+        ring_copy<RBYTES>(recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
     };
     const FaceSrc fj(j == b.jmax, jj < TJ - 1, &xj[0][0][min(lane + 1, TW - 1)], hring[0][0] + kk, zx + kk);
     const FaceSrc fk(k == b.kmax, kk < TK - 1, &xk[0][0][min(lane + TJ, TW - 1)], hring[0][1] + jj, zx + jj);
@@ -2749,7 +2764,7 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
     for (int s = 0; s < nsteps; s++) {
         const int row = nsteps - 1 - s;
         if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
-        const double *cur = ring[s % RS] + lane;
+        const RT *cur = ring[s % RS] + lane;
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
             const int i = b.imax - q;
@@ -2817,14 +2832,16 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
 // the step's stores, the halo blocks of step s + 2, the record and increments of step s + 2;
 // at the top of step s the copy of step s + 1 (32 loads) follows that of step s, which
 // follows the halo blocks of step s, so vmcnt(32) covers both.
-template <bool KE>
+template <bool KE, typename RT = double>
 __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
     constexpr int NV = KE ? 7 : 5;
     constexpr int RS = 3;                               // ring slots: 2 steps of lead
+    constexpr int RBYTES = rbytes<RT>();
     constexpr int NLD = (RBYTES + DBYTES) / 1024;       // record loads per step
     static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
-    __shared__ __attribute__((aligned(16))) double ring[RS][NRF * TW];
+    __shared__ __attribute__((aligned(16))) RT ring[RS][NRF * TW];
+    const RT *const recc = (const RT *)st.recc;
     __shared__ __attribute__((aligned(16))) double dring[RS][NDN * TW];
     __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
     __shared__ double xj[2][NV][TW], xk[2][NV][TW], zx[NV * TK];
@@ -2849,7 +2866,7 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
     const FaceSrc fj(j == 1, jj > 0, &xj[0][0][max(lane - 1, 0)], hring[0][0] + kk, zx + kk);
     const FaceSrc fk(k == 1, kk > 0, &xk[0][0][max(lane - TJ, 0)], hring[0][1] + jj, zx + jj);
     auto rec_copy = [&](int s) {
-        ring_copy<RBYTES>(st.recc + st_row(st, tile, s, 0), ring[s % RS], lane);
+        ring_copy<RBYTES>(recc + st_row(st, tile, s, 0), ring[s % RS], lane);
         ring_copy<DBYTES>(st.dn + (((size_t)tile * nsteps + s) * NDN) * TW, dring[s % RS], lane);
     };
     double yi[NV];
@@ -2861,7 +2878,7 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
     if (1 < nsteps) rec_copy(1);
     for (int s = 0; s < nsteps; s++) {
         if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
-        const double *cur = ring[s % RS] + lane;
+        const RT *cur = ring[s % RS] + lane;
         const double *dni = dring[s % RS] + lane;
         const int q = s - lag;
         if (col && q >= 0 && q < b.imax) {
@@ -3517,6 +3534,9 @@ struct BlockRec {
     void *dmx_mem = nullptr;   // DROMAX chunk maxima and candidates (allocated on first use)
     std::vector<BcPatchDev> bcs;   // boundary patches (ff_bc_register)
     double *bc_mem = nullptr;      // their wall velocity arrays on the device
+    std::vector<ff_con_patch> cons;   // connection patches (ff_con_register)
+    int *con_map = nullptr;           // their ghost <- source cell offset pairs on the device
+    std::vector<size_t> con_off;      // start of each patch's pairs in con_map
 };
 
 struct PendingEv {
@@ -3557,6 +3577,7 @@ struct ff_ctx {
     int lev = 0;                  // level of the stage being launched (statistics only)
     hipStream_t wstream = nullptr;   // side stream of the wall force sums (default priority)
     int earsm_mode = FF_EARSM_EXACT;
+    int precision = FF_PREC_FP64;
     MlpRec mlp;
     // the captured V-cycle (ff_vcycle_graph) and what it was captured for; gen counts the
     // registrations and option changes that make a capture stale
@@ -3783,6 +3804,10 @@ static void sweep_free(BlockRec &r)
     if (r.bc_mem) (void)hipFree(r.bc_mem);
     r.bc_mem = nullptr;
     r.bcs.clear();
+    if (r.con_map) (void)hipFree(r.con_map);
+    r.con_map = nullptr;
+    r.cons.clear();
+    r.con_off.clear();
 }
 
 // Read the error word of the block's sweeps; after a failed halo wait, refill the slots
@@ -3945,6 +3970,13 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
         if (value != FF_RESIDUAL_TILES && value != FF_RESIDUAL_CELLS)
             return fail(c, FF_E_ARG, "FF_OPT_RESIDUAL: unknown value%s");
         c->residual_mode = value;
+        c->gen++;
+        return FF_OK;
+    }
+    if (option == FF_OPT_PRECISION) {
+        if (value != FF_PREC_FP64 && value != FF_PREC_MIXED)
+            return fail(c, FF_E_ARG, "FF_OPT_PRECISION: unknown value%s");
+        c->precision = value;
         c->gen++;
         return FF_OK;
     }
@@ -4308,9 +4340,10 @@ static int staged_alloc(ff_ctx *c, BlockRec *r)
     const size_t nrec = nt * st.nstep * NRF * TW, ndn = nt * st.nstep * NDN * TW;
     if (hipMalloc(&r->st_mem, sizeof(double) * (2 * nrec + ndn)) != hipSuccess)
         return fail(c, FF_E_NOMEM, "LU-SGS staged records%s");
+    // sized for FP64 records; FP32 records use the first half of each
     st.recp = r->st_mem;
-    st.recc = st.recp + nrec;
-    st.dn = st.recc + nrec;
+    st.recc = r->st_mem + nrec;
+    st.dn = r->st_mem + 2 * nrec;
     return FF_OK;
 }
 
@@ -4338,17 +4371,22 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         const TileGrid tp = tile_grid((r->st.nstep + PSW - 1) / PSW, r->sw.ntj * r->sw.ntk, 1);
         dim3 gp(NXCD * tp.per);
         const bool ke = c->prm.iturb == 6;
-        rc = ke ? launch(c, "k_sweep_prep", k_sweep_prep<true>, gp, dim3(PSW * TW), kp, b, r->sw, r->st, tp)
-                : launch(c, "k_sweep_prep", k_sweep_prep<false>, gp, dim3(PSW * TW), kp, b, r->sw, r->st, tp);
+        // FF_PREC_MIXED: the records (diagonal, coefficients, face bases) in FP32
+        const bool f32 = c->precision == FF_PREC_MIXED;
+        auto prep = ke ? (f32 ? k_sweep_prep<true, float> : k_sweep_prep<true>)
+                       : (f32 ? k_sweep_prep<false, float> : k_sweep_prep<false>);
+        auto pred = ke ? (f32 ? k_pred_st<true, float> : k_pred_st<true>)
+                       : (f32 ? k_pred_st<false, float> : k_pred_st<false>);
+        auto corr = ke ? (f32 ? k_corr_st<true, float> : k_corr_st<true>)
+                       : (f32 ? k_corr_st<false, float> : k_corr_st<false>);
+        rc = launch(c, "k_sweep_prep", prep, gp, dim3(PSW * TW), kp, b, r->sw, r->st, tp);
         if (rc) return rc;
         // each sweep leaves its halo slots filled; empty them for the next one
         const dim3 gf((unsigned)((r->ntile_slots + 255) / 256));
-        rc = ke ? launch(c, "k_pred_st", k_pred_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
-                : launch(c, "k_pred_st", k_pred_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
+        rc = launch(c, "k_pred_st", pred, g, dim3(TW), kp, b, r->sw, r->st);
         if (rc) return rc;
         if ((rc = launch(c, "k_fill_slots", k_fill_u64, gf, dim3(256), r->sw.hj, r->ntile_slots, HALO_EMPTY))) return rc;
-        rc = ke ? launch(c, "k_corr_st", k_corr_st<true>, g, dim3(TW), kp, b, r->sw, r->st)
-                : launch(c, "k_corr_st", k_corr_st<false>, g, dim3(TW), kp, b, r->sw, r->st);
+        rc = launch(c, "k_corr_st", corr, g, dim3(TW), kp, b, r->sw, r->st);
         if (rc) return rc;
         return launch(c, "k_fill_slots", k_fill_u64, gf, dim3(256), r->sw.hj, r->ntile_slots, HALO_EMPTY);
     }
@@ -4830,3 +4868,4 @@ int ff_wall_forces(ff_ctx *c, int block, int patch, double *sums)
 #include "ffmlp.hip"
 #include "ffconv.hip"
 #include "ffbc.hip"
+#include "ffcon.hip"

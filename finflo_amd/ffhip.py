@@ -10,7 +10,8 @@ import os
 
 import numpy as np
 
-from .layout import FID, Block, BlockView, Params, WallParams, bc_views, wall_views
+from .layout import (FID, Block, BlockView, Params, WallParams, bc_views, con_views,
+                     wall_views)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libffhip.so")
@@ -24,12 +25,14 @@ EXPORTS = [
     "ff_prolong", "ff_vcycle", "ff_vcycle_graph", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
     "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc", "ff_dromax",
-    "ff_bc_register", "ff_reflec", "ff_ghostext",
+    "ff_bc_register", "ff_reflec", "ff_ghostext", "ff_con_register", "ff_halo_exchange",
+    "ff_vcycle_blocks",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 FF_OPT_EARSM, FF_EARSM_EXACT, FF_EARSM_MLP = 3, 0, 1
+FF_OPT_PRECISION, FF_PREC_FP64, FF_PREC_MIXED = 4, 0, 1
 FF_MLP_NPARAM, FF_MLP_NNORM = 4739, 26
 
 _lib = None
@@ -68,6 +71,9 @@ def lib():
         L.ff_bc_register.argtypes = [vp, i, i, vp, i]
         L.ff_reflec.argtypes = [vp, i, i]
         L.ff_ghostext.argtypes = [vp, i, i, d, d]
+        L.ff_con_register.argtypes = [vp, i, i, vp, i]
+        L.ff_halo_exchange.argtypes = [vp, i, i]
+        L.ff_vcycle_blocks.argtypes = [vp, vp, i, i, i, vp]
         L.ff_implicit.argtypes = [vp, i, i, i]
         L.ff_cycle.argtypes = [vp, i, i, vp]
         L.ff_restrict.argtypes = [vp, i, i]
@@ -128,6 +134,10 @@ class Context:
         self.set_option(FF_OPT_LUSGS, {"tiles": FF_LUSGS_TILES, "planes": FF_LUSGS_PLANES,
                                        "kmarch": FF_LUSGS_KMARCH,
                                        "staged": FF_LUSGS_STAGED}[mode])
+
+    def precision(self, mode):
+        """"fp64" (everything FP64) or "mixed" (the LU-SGS records in FP32)."""
+        self.set_option(FF_OPT_PRECISION, {"fp64": FF_PREC_FP64, "mixed": FF_PREC_MIXED}[mode])
 
     def residual_mode(self, mode):
         """Residual form: "tiles" (each face once) or "cells" (both faces per cell)."""
@@ -203,6 +213,25 @@ class Context:
 
     def ghostext(self, level=1, block=1, alpha=0.0, beta=0.0):
         self._chk(lib().ff_ghostext(self.h, level, block, alpha, beta), "ff_ghostext")
+
+    # ---- block connections -------------------------------------------------------------
+    def con_register(self, patches, level=1, block=1):
+        """Connection patches of a block level: tuples (iface, ka1, ka2, kb1, kb2, nblock,
+        nface, na0, nb0, swap, da, db)."""
+        self._chk(lib().ff_con_register(self.h, level, block, con_views(patches), len(patches)),
+                  "ff_con_register")
+
+    def halo_exchange(self, level=1, what="flow"):
+        self._chk(lib().ff_halo_exchange(self.h, level, {"flow": 0, "geometry": 1}[what]),
+                  "ff_halo_exchange")
+
+    def vcycle_blocks(self, blocks, nlev, ncycles=1):
+        """ncycles multi-block cycles over the listed blocks; the RESL2 sums of the last."""
+        arr = (C.c_int * len(blocks))(*blocks)
+        resi = np.zeros(7)
+        self._chk(lib().ff_vcycle_blocks(self.h, arr, len(blocks), nlev, ncycles,
+                                         resi.ctypes.data), "ff_vcycle_blocks")
+        return resi
 
     # ---- multigrid (TRANSF/TRANKE, FORCFU/ADDRES, INTERP/INTERT, the V-cycle) ----------
     def register_levels(self, blocks, block=1):

@@ -255,3 +255,24 @@ def bc_views(patches):
     for i, bp in enumerate(patches):
         arr[i] = bp.view()
     return arr
+
+
+# ---- block connections (ff_con_patch of include/ffhip.h) --------------------------------
+
+class ConPatchView(C.Structure):
+    """ff_con_patch: one side of a one-to-one block connection."""
+    _fields_ = [("iface", C.c_int), ("ka1", C.c_int), ("ka2", C.c_int), ("kb1", C.c_int),
+                ("kb2", C.c_int), ("nblock", C.c_int), ("nface", C.c_int), ("na0", C.c_int),
+                ("nb0", C.c_int), ("swap", C.c_int), ("da", C.c_int), ("db", C.c_int),
+                ("reserved", C.c_int * 4)]
+
+
+def con_views(patches):
+    """ff_con_patch array from tuples (iface, ka1, ka2, kb1, kb2, nblock, nface, na0, nb0,
+    swap, da, db)."""
+    arr = (ConPatchView * max(len(patches), 1))()
+    for i, t in enumerate(patches):
+        v = arr[i]
+        (v.iface, v.ka1, v.ka2, v.kb1, v.kb2, v.nblock, v.nface, v.na0, v.nb0, v.swap, v.da,
+         v.db) = t
+    return arr

@@ -193,6 +193,17 @@ const char *ff_last_error(const ff_ctx *ctx);
 #define FF_OPT_EARSM       3
 #define FF_EARSM_EXACT     0
 #define FF_EARSM_MLP       1
+/*   FF_OPT_PRECISION: FF_PREC_FP64 (default) computes and stores everything in FP64, as the
+ *   reference does; FF_PREC_MIXED stores the staged LU-SGS records -- the DIAGGS diagonal,
+ *   the split-eigenvalue coefficients, the face bases and the SYI/SYM1I scalars, i.e. the
+ *   implicit operator -- in FP32 (half the record traffic), while the residual (fluxes,
+ *   sources), its RESL2 norms, the increments of the sweeps, the FIN update and the
+ *   multigrid transfers and corrections stay FP64.  The implicit operator only steers the
+ *   iteration, so the converged state is the FP64 one; iterates differ from it at the
+ *   FP32 rounding of the operator.  Takes effect with FF_LUSGS_STAGED. */
+#define FF_OPT_PRECISION   4
+#define FF_PREC_FP64       0
+#define FF_PREC_MIXED      1
 int  ff_ctx_set_option(ff_ctx *ctx, int option, int value);
 /* Per-kernel event timing (0 off, 1 on); results via ff_kernel_stats. */
 int  ff_ctx_profile(ff_ctx *ctx, int enable);
@@ -309,6 +320,39 @@ int  ff_reflec(ff_ctx *ctx, int level, int block);
  * k-omega model; no atmosphere model (ALTITUDE < GROUND).  The reference calls it once at
  * start-up (ns3c.f:294); the ghost cells then stay as set. */
 int  ff_ghostext(ff_ctx *ctx, int level, int block, double alpha, double beta);
+
+/* ---- one-to-one block connections (CON patches, ICON(1,L) = 1) ------------------------ */
+/* A connection patch of block N: the patch on this block's face, the connected block and
+ * face, and how the connected face's in-plane indices follow this face's.  The in-plane
+ * indices (a, b) of a face are (J, K) on an I-face, (I, K) on a J-face and (I, J) on a
+ * K-face, as in ICON(4..7).  Cell (ka1 + p, kb1 + q) of the patch faces the connected
+ * face's cell (na0 + da*p, nb0 + db*q), or (na0 + da*q, nb0 + db*p) with swap = 1; that
+ * covers the eight orientations ICON(14,L) and PATCHE (gluebm.f:12850-13087) describe.  Both
+ * blocks must be registered on the same level; each block lists its own side. */
+typedef struct ff_con_patch {
+    int iface, ka1, ka2, kb1, kb2;   /* ICON(3..7,L) */
+    int nblock;                      /* ICON(8,L): the connected block */
+    int nface;                       /* ICON(9,L): its face */
+    int na0, nb0;                    /* its in-plane indices of the patch's first cell */
+    int swap, da, db;                /* the orientation (da, db = +1 or -1) */
+    int reserved[4];
+} ff_con_patch;
+/* Register (or replace, n = 0 removes) the connection patches of block N on level M. */
+int  ff_con_register(ff_ctx *ctx, int level, int block, const ff_con_patch *p, int n);
+/* The halo exchange of BOUNDI's connections (EXPTOT/IMPVO2 gluebm.f:6819-7262 with the
+ * conversions of CONNEC ns3c.f:8605-8857) on the device, for every connection registered on
+ * level M: both ghost layers behind each connected face take the connected block's two cell
+ * layers next to its face.  what = 0: the flow fields (PDIFF TEMP U V W RK REPS, P RO E C
+ * VIS CH CP DRDP DRDH RM RN RW EPS2 VIST); what = 1: the cell geometry (VOL, XC YC ZC,
+ * DISTW, D1 D2 D3), once after registration, as IMPVOL does for the volumes (gluebm.f:6166).
+ * The values are copied, not recomputed from exchanged conserved variables. */
+int  ff_halo_exchange(ff_ctx *ctx, int level, int what);
+/* ncycles multi-block cycles: per cycle every listed block's V-cycle (levels 1..nlev, as
+ * ff_vcycle), then the level-1 halo exchange, as CYCLES' block loop and BOUNDI(1) do
+ * (ns3c.f:1400-2195, 556); resi receives the RESL2 sums of the last cycle summed over the
+ * blocks (SUBIN's sums, ns3c.f:10242-10248). */
+int  ff_vcycle_blocks(ff_ctx *ctx, const int *blocks, int nblocks, int nlev, int ncycles,
+                      double *resi);
 
 /* SUBIN's convergence monitor for block N (ns3c.f:9877-9905): the change of the cycle that
  * IDRXX selects, over every entry of the level-1 slab including the ghost layers (SUBIN
