@@ -173,3 +173,100 @@ extern "C" int ff_vcycle_blocks(ff_ctx *c, const int *blocks, int nblocks, int n
             }
     return FF_OK;
 }
+
+// The multi-block cycle as one HIP graph.  Every block's V-cycle is captured on a stream of
+// a pool of eight (block n on stream n mod 8), forked from and joined back to the context's
+// stream through events, so that the graph holds up to eight independent branches that the
+// device runs side by side -- small blocks no longer leave most of the chip idle between
+// their launches -- followed by the level-1 halo exchange.  Replays make no host launches.
+extern "C" int ff_vcycle_blocks_graph(ff_ctx *c, const int *blocks, int nblocks, int nlev,
+                                      int ncycles)
+{
+    if (!c || !blocks || nblocks < 1 || nlev < 1 || ncycles < 0) return FF_E_ARG;
+    if (c->profile) return fail(c, FF_E_STATE, "ff_vcycle_blocks_graph: per-kernel timing is on%s");
+    if (!c->stream) return fail(c, FF_E_STATE, "ff_vcycle_blocks_graph: the null stream cannot be captured%s");
+    std::vector<std::vector<BlockRec *>> lv(nblocks);
+    for (int n = 0; n < nblocks; n++) {
+        int rc = vcycle_levels(c, blocks[n], nlev, lv[n]);
+        if (rc) return rc;
+    }
+    std::vector<long long> key = {nlev, c->lusgs_mode, c->residual_mode, c->earsm_mode,
+                                  c->precision, c->gen};
+    key.insert(key.end(), blocks, blocks + nblocks);
+    if (!c->mb_exec || key != c->mb_key) {
+        if (c->mb_exec) (void)hipGraphExecDestroy(c->mb_exec);
+        if (c->mb_graph) (void)hipGraphDestroy(c->mb_graph);
+        c->mb_exec = nullptr;
+        c->mb_graph = nullptr;
+        for (int q = 0; q < ff_ctx::NPOOL; q++)
+            if (!c->pool[q] && hipStreamCreateWithFlags(&c->pool[q], hipStreamNonBlocking) != hipSuccess)
+                return fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: stream pool%s");
+        for (int q = 0; q <= ff_ctx::NPOOL; q++)
+            if (!c->pool_ev[q] && hipEventCreateWithFlags(&c->pool_ev[q], hipEventDisableTiming) != hipSuccess)
+                return fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: events%s");
+        if (c->lusgs_mode == FF_LUSGS_STAGED)
+            for (auto &l : lv)
+                for (BlockRec *r : l) {
+                    int rc = staged_alloc(c, r);
+                    if (rc) return rc;
+                }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (auto &l : lv)
+            for (BlockRec *r : l)
+                if (r->walls.pending) {
+                    HIPCHK(c, hipEventSynchronize(r->walls.done));
+                    r->walls.pending = false;
+                }
+        const hipStream_t origin = c->stream;
+        HIPCHK(c, hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
+        int rc = FF_OK;
+        const int np = std::min(nblocks, (int)ff_ctx::NPOOL);
+        if (hipEventRecord(c->pool_ev[ff_ctx::NPOOL], origin) != hipSuccess) rc = FF_E_HIP;
+        for (int q = 0; q < np && !rc; q++)
+            if (hipStreamWaitEvent(c->pool[q], c->pool_ev[ff_ctx::NPOOL], 0) != hipSuccess) rc = FF_E_HIP;
+        for (int n = 0; n < nblocks && !rc; n++) {
+            c->stream = c->pool[n % np];
+            rc = vcycle_once(c, lv[n], nlev, blocks[n], nullptr);
+            // the block's last force pass joins its branch
+            for (BlockRec *r : lv[n])
+                if (r->walls.pending) {
+                    if (!rc && hipStreamWaitEvent(c->stream, r->walls.done, 0) != hipSuccess) rc = FF_E_HIP;
+                    r->walls.pending = false;
+                }
+        }
+        c->stream = origin;
+        for (int q = 0; q < np && !rc; q++)
+            if (hipEventRecord(c->pool_ev[q], c->pool[q]) != hipSuccess ||
+                hipStreamWaitEvent(origin, c->pool_ev[q], 0) != hipSuccess)
+                rc = FF_E_HIP;
+        if (!rc) rc = halo_impl(c, 1, 0);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(origin, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc == FF_E_HIP ? fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: capture failed%s") : rc;
+        }
+        if (ec != hipSuccess || !g) return fail(c, FF_E_HIP, "%s", hipGetErrorString(ec));
+        c->mb_graph = g;
+        if (hipGraphInstantiate(&c->mb_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            c->mb_graph = nullptr;
+            return fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: instantiation failed%s");
+        }
+        c->mb_key = key;
+    }
+    for (auto &l : lv)
+        for (BlockRec *r : l)
+            if (r->walls.pending) {
+                HIPCHK(c, hipStreamWaitEvent(c->stream, r->walls.done, 0));
+                r->walls.pending = false;
+            }
+    for (int it = 0; it < ncycles; it++) HIPCHK(c, hipGraphLaunch(c->mb_exec, c->stream));
+    if (c->lusgs_mode != FF_LUSGS_PLANES)
+        for (auto &l : lv)
+            for (BlockRec *r : l) {
+                int rc = sweep_check(c, *r);
+                if (rc) return rc;
+            }
+    return FF_OK;
+}

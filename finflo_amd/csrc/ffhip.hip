@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1422,9 +1423,10 @@ __global__ void __launch_bounds__(128) k_residual(Prm p, Blk b, WallDev wd)
 constexpr int RI = 64, RJ = 4, RKC = 8;
 static_assert(RJ >= 2 && RJ <= RI, "the tile's upper I-faces are formed by the lanes of row 1");
 
-template <int IFLUX, bool FNS = false>
+template <int IFLUX, bool FNS = false, int RI = ::RI, int RJ = ::RJ, int RKC = ::RKC>
 __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev wd, TileGrid tg)
 {
+    static_assert(RJ >= 2 && RI >= RJ, "the tile's upper faces need two rows and RI >= RJ");
     __shared__ double fi[5][RJ][RI + 1];
     __shared__ double fj[5][RJ + 1][RI];
     const int ti = threadIdx.x % RI, tj = threadIdx.x / RI;
@@ -3557,6 +3559,12 @@ struct MlpRec {
     bool ready = false;                                 // parameters and normalisation set
 };
 
+// candidate tile shapes of the residual autotuner (described with it, below)
+struct TileShape { int ri, rj, rk; };
+constexpr TileShape kTileShapes[] = {{64, 4, 8}, {64, 4, 16}, {32, 8, 8}, {32, 4, 4},
+                                     {128, 2, 8}, {32, 4, 2}, {16, 4, 2}, {64, 2, 2}};
+constexpr int NSHAPE = (int)(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
+
 struct ff_ctx {
     ff_params prm{};
     Prm kp{};
@@ -3585,6 +3593,18 @@ struct ff_ctx {
     hipGraphExec_t vg_exec = nullptr;
     long long vg_key[6] = {0, 0, 0, 0, 0, 0};
     long long gen = 0;
+    // the captured multi-block cycle (ff_vcycle_blocks_graph): its streams, fork/join events,
+    // graph and the key it was captured for
+    static constexpr int NPOOL = 8;
+    hipStream_t pool[NPOOL] = {};
+    hipEvent_t pool_ev[NPOOL + 1] = {};
+    hipGraph_t mb_graph = nullptr;
+    hipGraphExec_t mb_exec = nullptr;
+    std::vector<long long> mb_key;
+    // the tile-shape autotuner: on, and the chosen shape per (IMAX, JMAX, KMAX, kind)
+    bool autotune = false;
+    int force_shape = -1;         // FF_OPT_TILE_SHAPE: a candidate shape for every residual
+    std::map<std::tuple<int, int, int, int>, std::pair<int, double>> tuned;
     int device = -1;              // the HIP device the context was created on
     double *scratch = nullptr;    // that device's address of the fixed scratch array
 };
@@ -3917,7 +3937,16 @@ int ff_ctx_destroy(ff_ctx *c)
 {
     if (!c) return FF_E_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int q = 0; q < ff_ctx::NPOOL; q++)
+        if (c->pool[q]) (void)hipStreamSynchronize(c->pool[q]);
+    if (c->wstream) (void)hipStreamSynchronize(c->wstream);
     drain_events(c);
+    if (c->mb_exec) (void)hipGraphExecDestroy(c->mb_exec);
+    if (c->mb_graph) (void)hipGraphDestroy(c->mb_graph);
+    for (int q = 0; q < ff_ctx::NPOOL; q++)
+        if (c->pool[q]) (void)hipStreamDestroy(c->pool[q]);
+    for (int q = 0; q <= ff_ctx::NPOOL; q++)
+        if (c->pool_ev[q]) (void)hipEventDestroy(c->pool_ev[q]);
     for (auto &kv : c->blocks) {
         if (kv.second.mirror && kv.second.arena) (void)hipFree(kv.second.arena);
         sweep_free(kv.second);
@@ -3970,6 +3999,18 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
         if (value != FF_RESIDUAL_TILES && value != FF_RESIDUAL_CELLS)
             return fail(c, FF_E_ARG, "FF_OPT_RESIDUAL: unknown value%s");
         c->residual_mode = value;
+        c->gen++;
+        return FF_OK;
+    }
+    if (option == FF_OPT_TILE_SHAPE) {
+        if (value < -1 || value >= NSHAPE) return fail(c, FF_E_ARG, "FF_OPT_TILE_SHAPE: -1 or a candidate%s");
+        c->force_shape = value;
+        c->gen++;
+        return FF_OK;
+    }
+    if (option == FF_OPT_AUTOTUNE) {
+        if (value != 0 && value != 1) return fail(c, FF_E_ARG, "FF_OPT_AUTOTUNE: 0 or 1%s");
+        c->autotune = value == 1;
         c->gen++;
         return FF_OK;
     }
@@ -4153,6 +4194,84 @@ int ff_turb(ff_ctx *c, int level, int block)
     return launch(c, "k_turb_lam", k_turb_lam, cells_grid(r->dev, 256), dim3(256), r->dev);
 }
 
+}  // extern "C"
+
+// ---- the tile-shape autotuner of the residual kernels ----------------------------------
+// Candidate (I, J, K) tile shapes of the face-once residual kernels: I along the wave for
+// coalesced rows, J rows sharing I-faces through LDS, K planes marched in registers.  Every
+// shape gives the same results bit for bit (each face flux is formed by the same code from
+// the same operands); they differ in occupancy, halo re-reads and how many workgroups a
+// small (coarse-level) block offers the 256 CUs.
+
+// the k-omega (kind 1) or laminar Roe (kind 0) residual kernel of shape s over block b
+template <int S>
+static int launch_shape(ff_ctx *c, bool kw, const Prm &kp, const Blk &b, const WallDev &wd)
+{
+    constexpr TileShape t = kTileShapes[S];
+    const TileGrid tg = tile_grid((b.imax + t.ri - 1) / t.ri, (b.jmax + t.rj - 1) / t.rj,
+                                  (b.kmax + t.rk - 1) / t.rk);
+    const dim3 g(NXCD * tg.per), th(t.ri * t.rj);
+    if (kw) return launch(c, "k_residual_kw", k_residual_kw<t.ri, t.rj, t.rk>, g, th, kp, b, wd, tg);
+    return launch(c, "k_residual_tile", k_residual_tile<1, false, t.ri, t.rj, t.rk>, g, th, kp, b, wd, tg);
+}
+
+static int launch_shape_any(ff_ctx *c, int s, bool kw, const Prm &kp, const Blk &b, const WallDev &wd)
+{
+    switch (s) {
+    case 0: return launch_shape<0>(c, kw, kp, b, wd);
+    case 1: return launch_shape<1>(c, kw, kp, b, wd);
+    case 2: return launch_shape<2>(c, kw, kp, b, wd);
+    case 3: return launch_shape<3>(c, kw, kp, b, wd);
+    case 4: return launch_shape<4>(c, kw, kp, b, wd);
+    case 5: return launch_shape<5>(c, kw, kp, b, wd);
+    case 6: return launch_shape<6>(c, kw, kp, b, wd);
+    default: return launch_shape<7>(c, kw, kp, b, wd);
+    }
+}
+
+// The shape for this block and kind: from the table, or, on the first residual of such a
+// block outside a graph capture, by timing every candidate (one launch to warm up, three
+// timed with HIP events; the residual kernel only writes what it recomputes from the same
+// inputs, so the repeated launches leave the same results).  -1: the built-in shapes.
+static int tuned_shape(ff_ctx *c, bool kw, const Prm &kp, const Blk &b, const WallDev &wd)
+{
+    if (c->force_shape >= 0) return c->force_shape;
+    if (!c->autotune || c->profile) return -1;
+    const auto key = std::make_tuple(b.imax, b.jmax, b.kmax, kw ? 1 : 0);
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) return it->second.first;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -1;
+    }
+    int best = -1;
+    double best_ms = 1e30;
+    for (int sh = 0; sh < NSHAPE; sh++) {
+        if (launch_shape_any(c, sh, kw, kp, b, wd)) break;
+        float ms = 0.f;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) break;
+        bool ok = true;
+        for (int rep = 0; rep < 3 && ok; rep++) ok = launch_shape_any(c, sh, kw, kp, b, wd) == FF_OK;
+        if (!ok || hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            break;
+        if (ms / 3. < best_ms) {
+            best_ms = ms / 3.;
+            best = sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best >= 0) c->tuned[key] = {best, best_ms};
+    return best;
+}
+
+extern "C" {
+
 // where the RESL2 partials and sums of a residual go: the context's buffers, or the fixed
 // scratch array of ff_residual_noalloc
 struct RedBuf {
@@ -4241,7 +4360,10 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
                          (const double *)(r->kw_scr + r->kw_parts)))) return rc;
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
-        if (level >= 2) {
+        const int sh = fns ? -1 : tuned_shape(c, true, kp, b, wr.wd);
+        if (sh >= 0) {
+            rc = launch_shape_any(c, sh, true, kp, b, wr.wd);
+        } else if (level >= 2) {
             // the coarse levels' blocks are small: 32x4 tiles of two planes keep every CU busy
             const TileGrid tg = tile_grid((b.imax + 31) / 32, (b.jmax + 3) / 4, (b.kmax + 1) / 2);
             rc = launch(c, "k_residual_kw", k_residual_kw<32, 4, 2>, dim3(NXCD * tg.per), dim3(128),
@@ -4255,7 +4377,11 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
     } else {
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
-        if (c->residual_mode == FF_RESIDUAL_TILES) {
+        const int sh = (fns || vl || c->residual_mode != FF_RESIDUAL_TILES) ? -1
+                       : tuned_shape(c, false, make_prm(&c->prm, level), b, wr.wd);
+        if (sh >= 0) {
+            rc = launch_shape_any(c, sh, false, make_prm(&c->prm, level), b, wr.wd);
+        } else if (c->residual_mode == FF_RESIDUAL_TILES) {
             const TileGrid tg = tile_grid((b.imax + RI - 1) / RI, (b.jmax + RJ - 1) / RJ,
                                           (b.kmax + RKC - 1) / RKC);
             dim3 g(NXCD * tg.per);
@@ -4297,6 +4423,20 @@ int ff_residual(ff_ctx *c, int level, int block, int m2, double *resi)
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
     return residual_impl(c, r, level, m2, resi);
+}
+
+int ff_tuned_shape(const ff_ctx *c, int level, int block, int *shape, double *ms)
+{
+    if (!c || !shape) return FF_E_ARG;
+    auto it = c->blocks.find(key(level, block));
+    if (it == c->blocks.end()) return FF_E_STATE;
+    const Blk &b = it->second.dev;
+    const auto t = c->tuned.find(std::make_tuple(b.imax, b.jmax, b.kmax, c->prm.iturb == 6 ? 1 : 0));
+    if (t == c->tuned.end()) return FF_E_STATE;
+    const TileShape &sh = kTileShapes[t->second.first];
+    shape[0] = sh.ri; shape[1] = sh.rj; shape[2] = sh.rk;
+    if (ms) *ms = t->second.second;
+    return FF_OK;
 }
 
 // ff_residual on caller-owned device arrays with every scratch buffer in the fixed array:

@@ -26,13 +26,14 @@ EXPORTS = [
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
     "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc", "ff_dromax",
     "ff_bc_register", "ff_reflec", "ff_ghostext", "ff_con_register", "ff_halo_exchange",
-    "ff_vcycle_blocks",
+    "ff_vcycle_blocks", "ff_vcycle_blocks_graph", "ff_tuned_shape",
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 FF_OPT_EARSM, FF_EARSM_EXACT, FF_EARSM_MLP = 3, 0, 1
 FF_OPT_PRECISION, FF_PREC_FP64, FF_PREC_MIXED = 4, 0, 1
+FF_OPT_AUTOTUNE, FF_OPT_TILE_SHAPE = 5, 6
 FF_MLP_NPARAM, FF_MLP_NNORM = 4739, 26
 
 _lib = None
@@ -74,6 +75,8 @@ def lib():
         L.ff_con_register.argtypes = [vp, i, i, vp, i]
         L.ff_halo_exchange.argtypes = [vp, i, i]
         L.ff_vcycle_blocks.argtypes = [vp, vp, i, i, i, vp]
+        L.ff_vcycle_blocks_graph.argtypes = [vp, vp, i, i, i]
+        L.ff_tuned_shape.argtypes = [vp, i, i, vp, vp]
         L.ff_implicit.argtypes = [vp, i, i, i]
         L.ff_cycle.argtypes = [vp, i, i, vp]
         L.ff_restrict.argtypes = [vp, i, i]
@@ -138,6 +141,21 @@ class Context:
     def precision(self, mode):
         """"fp64" (everything FP64) or "mixed" (the LU-SGS records in FP32)."""
         self.set_option(FF_OPT_PRECISION, {"fp64": FF_PREC_FP64, "mixed": FF_PREC_MIXED}[mode])
+
+    def autotune(self, on=True):
+        """Let the library time the residual kernels' tile shapes per block size."""
+        self.set_option(FF_OPT_AUTOTUNE, 1 if on else 0)
+
+    def tile_shape(self, s):
+        """Run every residual in the autotuner's candidate shape s (-1: back to the choice)."""
+        self.set_option(FF_OPT_TILE_SHAPE, s)
+
+    def tuned_shape(self, level=1, block=1):
+        """(shape (I, J, K), mean ms) the autotuner chose for a block level, or None."""
+        sh = np.zeros(3, dtype=np.int32)
+        ms = C.c_double()
+        rc = lib().ff_tuned_shape(self.h, level, block, sh.ctypes.data, C.byref(ms))
+        return (tuple(int(x) for x in sh), ms.value) if rc == 0 else None
 
     def residual_mode(self, mode):
         """Residual form: "tiles" (each face once) or "cells" (both faces per cell)."""
@@ -232,6 +250,12 @@ class Context:
         self._chk(lib().ff_vcycle_blocks(self.h, arr, len(blocks), nlev, ncycles,
                                          resi.ctypes.data), "ff_vcycle_blocks")
         return resi
+
+    def vcycle_blocks_graph(self, blocks, nlev, ncycles=1):
+        """ncycles multi-block cycles replayed from one captured graph (no residual sums)."""
+        arr = (C.c_int * len(blocks))(*blocks)
+        self._chk(lib().ff_vcycle_blocks_graph(self.h, arr, len(blocks), nlev, ncycles),
+                  "ff_vcycle_blocks_graph")
 
     # ---- multigrid (TRANSF/TRANKE, FORCFU/ADDRES, INTERP/INTERT, the V-cycle) ----------
     def register_levels(self, blocks, block=1):

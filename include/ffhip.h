@@ -204,7 +204,19 @@ const char *ff_last_error(const ff_ctx *ctx);
 #define FF_OPT_PRECISION   4
 #define FF_PREC_FP64       0
 #define FF_PREC_MIXED      1
+/*   FF_OPT_AUTOTUNE: 1 lets the library pick the tile shape of the residual kernels (the
+ *   k-omega RESIKE kernel and the laminar Roe RESIDU kernel) per block size: on the first
+ *   residual of a block of new dimensions outside a graph capture it times each of eight
+ *   (I, J, K) tile shapes and keeps the fastest for every later residual of such a block.
+ *   All shapes give the same results bit for bit.  0 (default): the built-in shapes. */
+#define FF_OPT_AUTOTUNE    5
+/*   FF_OPT_TILE_SHAPE: 0-7 runs every residual in that candidate shape of the autotuner
+ *   (for tests and measurements), -1 (default) leaves the choice to FF_OPT_AUTOTUNE. */
+#define FF_OPT_TILE_SHAPE  6
 int  ff_ctx_set_option(ff_ctx *ctx, int option, int value);
+/* The tile shape (I, J, K) the autotuner chose for block N on level M and its measured mean
+ * launch time in ms (ms may be NULL); FF_E_STATE if that block size has not been tuned. */
+int  ff_tuned_shape(const ff_ctx *ctx, int level, int block, int *shape, double *ms);
 /* Per-kernel event timing (0 off, 1 on); results via ff_kernel_stats. */
 int  ff_ctx_profile(ff_ctx *ctx, int enable);
 /* Copy the accumulated timing of kernel `name`: launches and total milliseconds.  `name`
@@ -353,6 +365,14 @@ int  ff_halo_exchange(ff_ctx *ctx, int level, int what);
  * blocks (SUBIN's sums, ns3c.f:10242-10248). */
 int  ff_vcycle_blocks(ff_ctx *ctx, const int *blocks, int nblocks, int nlev, int ncycles,
                       double *resi);
+
+/* ff_vcycle_blocks without residual sums, as one HIP graph: the first call captures one
+ * multi-block cycle with every block's V-cycle on a stream of a pool of eight (independent
+ * branches the device runs side by side) and the halo exchange after them; each call
+ * replays it ncycles times, bit for bit the eager cycles.  A registration or option change
+ * makes the next call capture again.  No counterpart in the reference (CYCLES launches
+ * nothing).  Not with per-kernel timing or on the null stream (FF_E_STATE). */
+int  ff_vcycle_blocks_graph(ff_ctx *ctx, const int *blocks, int nblocks, int nlev, int ncycles);
 
 /* SUBIN's convergence monitor for block N (ns3c.f:9877-9905): the change of the cycle that
  * IDRXX selects, over every entry of the level-1 slab including the ghost layers (SUBIN

@@ -159,7 +159,21 @@ class OracleBlock:
         owned by the caller and are written in place."""
         self._wp = wparams
         self._wv = wall_views(patches)
+        self._wn = len(patches)
+        self._wkeep = patches
         oracle().ffo_wall_set(self.level, C.byref(self._wp), self._wv, len(patches))
+
+    def activate(self):
+        """Make this block's walls and boundary patches the ones of its level again (the
+        oracle keeps one set per level; several blocks of a level take turns)."""
+        if getattr(self, "_wv", None) is not None:
+            oracle().ffo_wall_set(self.level, C.byref(self._wp), self._wv, self._wn)
+        else:
+            oracle().ffo_wall_set(self.level, None, None, 0)
+        if getattr(self, "_bcv", None) is not None:
+            oracle().ffo_bc_set(self.level, self._bcv, len(self._bcp))
+        else:
+            oracle().ffo_bc_set(self.level, None, 0)
 
     def forces(self, patch):
         out = np.zeros(19)

@@ -33,29 +33,43 @@ def global_block(iturb=0, dims=(24, 16, 12), seed=3):
     return prm, G, nodes
 
 
-def halves(G, nodes, rotated):
+def halves(G, nodes, mode):
+    """The two halves of G cut at I = IMAX/2, the second in orientation `mode`: "same";
+    "rot180" (its I and J against the global ones); "rot90" (its I along the global J, its J
+    against the global I); "swap" (its I along the global K, its K against the global I,
+    which turns the in-plane axes of the cut face)."""
     h = G.imax // 2
     A = sub_block(G, nodes, (1, 1, 1), (h, G.jmax, G.kmax))
-    if not rotated:
-        B = sub_block(G, nodes, (h + 1, 1, 1), (G.imax, G.jmax, G.kmax))
+    lo, hi = (h + 1, 1, 1), (G.imax, G.jmax, G.kmax)
+    if mode == "same":
+        B = sub_block(G, nodes, lo, hi)
         ca = [(4, 1, G.jmax, 1, G.kmax, 2, 1, 1, 1, 0, 1, 1)]
         cb = [(1, 1, G.jmax, 1, G.kmax, 1, 4, 1, 1, 0, 1, 1)]
+    elif mode == "rot180":
+        B = sub_block(G, nodes, lo, hi, perm=(0, 1, 2), flip=(True, True, False))
+        ca = [(4, 1, G.jmax, 1, G.kmax, 2, 4, G.jmax, 1, 0, -1, 1)]
+        cb = [(4, 1, G.jmax, 1, G.kmax, 1, 4, G.jmax, 1, 0, -1, 1)]
+    elif mode == "rot90":
+        B = sub_block(G, nodes, lo, hi, perm=(1, 0, 2), flip=(False, True, False))
+        ca = [(4, 1, G.jmax, 1, G.kmax, 2, 5, 1, 1, 0, 1, 1)]
+        cb = [(5, 1, G.jmax, 1, G.kmax, 1, 4, 1, 1, 0, 1, 1)]
     else:
-        # B's I along the global K, J along J, K against the global I: the cut is B's K-high
-        B = sub_block(G, nodes, (h + 1, 1, 1), (G.imax, G.jmax, G.kmax), perm=(2, 1, 0),
-                      flip=(False, False, True))
+        B = sub_block(G, nodes, lo, hi, perm=(2, 1, 0), flip=(False, False, True))
         ca = [(4, 1, G.jmax, 1, G.kmax, 2, 6, 1, 1, 1, 1, 1)]
         cb = [(6, 1, G.kmax, 1, G.jmax, 1, 4, 1, 1, 1, 1, 1)]
     return A, B, ca, cb
 
 
-def to_global_b(G, Bfield, rotated):
+def to_global_b(G, Bfield, mode):
     """A (KMAX, JMAX, IMAX) interior array of B in the global (k, j, i) order of its cells."""
-    if not rotated:
+    if mode == "same":
         return Bfield
+    if mode == "rot180":
+        return Bfield[:, ::-1, ::-1]
+    if mode == "rot90":
+        return np.transpose(Bfield, (0, 2, 1))[:, :, ::-1]
     t = np.transpose(Bfield, (2, 1, 0))        # [i'][j'][k'] = [k][j][reversed i]
-    t = np.flip(t, axis=2)                     # [k][j][i]
-    return t
+    return np.flip(t, axis=2)
 
 
 def context(prm, blocks, cons):
@@ -69,11 +83,12 @@ def context(prm, blocks, cons):
     return ctx
 
 
-@pytest.mark.parametrize("rotated", [False, True], ids=["same", "rotated"])
-def test_geometry_exchange(hip_device, rotated):
-    print(f"case connection geometry rotated={rotated}", flush=True)
+@pytest.mark.parametrize("mode", ["same", "swap"])
+def test_geometry_exchange(hip_device, mode):
+    print(f"case connection geometry {mode}", flush=True)
+    rotated = mode != "same"
     prm, G, nodes = global_block()
-    A, B, ca, cb = halves(G, nodes, rotated)
+    A, B, ca, cb = halves(G, nodes, mode)
     ctx = context(prm, [A, B], [ca, cb])
     ctx.halo_exchange(what="geometry")
     ctx.download(block=1)
@@ -97,12 +112,15 @@ def test_geometry_exchange(hip_device, rotated):
     ctx.close()
 
 
-@pytest.mark.parametrize("rotated", [False, True], ids=["same", "rotated"])
-def test_split_residual_is_the_uncut_residual(hip_device, rotated):
+@pytest.mark.parametrize("mode", ["same", "rot180", "rot90"])
+def test_split_residual_is_the_uncut_residual(hip_device, mode):
+    """The K axis is kept: the reference's FLUXKE takes DERIV's one-sided branch on every
+    K-face (base3c.f:719), so a block whose K axis runs along another global direction has,
+    as in the reference, a different residual there."""
     from finflo_amd.ffhip import Context
-    print(f"case split residual rotated={rotated}", flush=True)
+    print(f"case split residual {mode}", flush=True)
     prm, G, nodes = global_block()
-    A, B, ca, cb = halves(G, nodes, rotated)
+    A, B, ca, cb = halves(G, nodes, mode)
     ctx = context(prm, [A, B], [ca, cb])
     ctx.halo_exchange(what="geometry")
     ctx.halo_exchange(what="flow")
@@ -123,11 +141,12 @@ def test_split_residual_is_the_uncut_residual(hip_device, rotated):
         want = G.interior(f)
         scale = np.abs(want).max()
         got_a = A.interior(f)
-        got_b = to_global_b(G, B.interior(f), rotated)
-        if not rotated:
+        got_b = to_global_b(G, B.interior(f), mode)
+        if mode == "same":
             np.testing.assert_array_equal(got_a, want[:, :, :h], err_msg=f)
             np.testing.assert_array_equal(got_b, want[:, :, h:], err_msg=f)
         else:
+            # the I and J sweeps of the rotated half run in another order: rounding only
             assert np.abs(got_a - want[:, :, :h]).max() <= 1e-12 * scale, f
             assert np.abs(got_b - want[:, :, h:]).max() <= 1e-12 * scale, f
     ctx.close()
@@ -199,3 +218,102 @@ def exchange_host(A, B, geometry=False):
         va, vb = A.view3(f), B.view3(f)
         va[2:-2, 2:-2, ia + 2:ia + 4] = vb[2:-2, 2:-2, 2:4]
         vb[2:-2, 2:-2, 0:2] = va[2:-2, 2:-2, ia:ia + 2]
+
+
+def exchange_con_host(blocks, cons, fields):
+    """The level-1 exchange of same-orientation full-face connections on the host (test
+    infrastructure): for each connection tuple (iface, ka1, ka2, kb1, kb2, nblock, nface, 1,
+    1, 0, 1, 1) of block n, its two ghost layers take the connected block's two layers next
+    to the connected face."""
+    def layer(b, face, depth):
+        """(K, J, I)-view slice of the cell layer `depth` in from `face` (0, -1: ghosts),
+        in-plane ranges 1..max."""
+        d = (face - 1) % 3
+        nd = (b.imax, b.jmax, b.kmax)[d]
+        idx = depth if face <= 3 else nd + 1 - depth
+        sl = [slice(2, -2)] * 3                  # view3 order (k, j, i)
+        sl[2 - d] = slice(idx + 1, idx + 2)
+        return tuple(sl)
+    for n, cl in enumerate(cons):
+        A = blocks[n]
+        for (iface, ka1, ka2, kb1, kb2, nb, nface, *_rest) in cl:
+            B = blocks[nb - 1]
+            for f in fields:
+                va, vb = A.view3(f), B.view3(f)
+                for l in (1, 2):
+                    va[layer(A, iface, 1 - l)] = vb[layer(B, nface, l)]
+
+
+def test_config3_multiblock_cycles(hip_device):
+    """The 24-block bump case at full size (about 7.6M cells, 3 levels, SST, walls on the
+    bottom row): 2 multi-block cycles against the oracle's V-cycles of every block with the
+    exchange done on the host, at 1e-12 of each field's largest magnitude; then the captured
+    multi-block graph against the eager cycles of a twin context, bit for bit."""
+    from finflo_amd.cases import config3
+    from finflo_amd.ffhip import Context
+    print("case config 3: 24 blocks, 2 multi-block cycles", flush=True)
+    prm, blocks, cons, walls, bcs = config3(seed=3, nlev=3)
+    nb = len(blocks)
+    host = [[b.copy() for b in lv] for lv in blocks]
+    twin = [[b.copy() for b in lv] for lv in blocks]
+    ctxs = []
+    for lvs in (blocks, twin):
+        ctx = Context(prm)
+        for n, lv in enumerate(lvs, start=1):
+            for m, b in enumerate(lv, start=1):
+                ctx.register(b, level=m, block=n)
+                ctx.upload(level=m, block=n)
+                if walls[n - 1][m - 1] is not None:
+                    ctx.wall_register(*walls[n - 1][m - 1], level=m, block=n)
+                ctx.bc_register(bcs[n - 1][m - 1], level=m, block=n)
+        for n in range(1, nb + 1):
+            ctx.con_register(cons[n - 1], block=n)
+        for n in range(1, nb + 1):
+            for m in range(1, 4):
+                ctx.ghostext(level=m, block=n)
+        ctx.halo_exchange(what="geometry")
+        ctx.halo_exchange(what="flow")
+        ctxs.append(ctx)
+    ctx, ctx2 = ctxs
+    # the same start on the host: GHOSTEXT, the geometry and flow exchanges
+    ohs = []
+    for n, lv in enumerate(host):
+        oh = OracleHierarchy(prm, lv)
+        for m in range(3):
+            if walls[n][m] is not None:
+                wp, ps = walls[n][m]
+                oh.levels[m].set_walls(wp, [p.copy() for p in ps])
+            oh.levels[m].set_bcs(bcs[n][m])
+            oh.levels[m].ghostext()
+        ohs.append(oh)
+        # only the last block's sets stay registered: each block activates its own later
+    fine = [lv[0] for lv in host]
+    exchange_con_host(fine, cons, GEOM)
+    exchange_con_host(fine, cons, FLOW)
+    for _ in range(2):
+        r = np.zeros(7)
+        for n, oh in enumerate(ohs):
+            # the oracle keeps walls and patches per level: this block's before its cycle
+            for m in range(3):
+                oh.levels[m].activate()
+            r = r + oh.vcycle()
+        exchange_con_host(fine, cons, FLOW)
+    got = ctx.vcycle_blocks(list(range(1, nb + 1)), nlev=3, ncycles=2)
+    for n in range(nb):
+        ctx.download(level=1, block=n + 1)
+        for f in ("TEMP", "U", "V", "W", "PDIFF", "RK", "REPS"):
+            x, y = blocks[n][0][f], host[n][0][f]
+            assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max(), (n + 1, f)
+    np.testing.assert_allclose(got[:7], r[:7], rtol=1e-9)
+    # the graph: two replays against two eager cycles of the twin
+    ctx2.vcycle_blocks(list(range(1, nb + 1)), nlev=3, ncycles=2)
+    ctx.vcycle_blocks_graph(list(range(1, nb + 1)), nlev=3, ncycles=2)
+    ctx2.vcycle_blocks(list(range(1, nb + 1)), nlev=3, ncycles=2)
+    for n in range(nb):
+        for m in range(1, 4):
+            ctx.download(level=m, block=n + 1)
+            ctx2.download(level=m, block=n + 1)
+            np.testing.assert_array_equal(blocks[n][m - 1].F, twin[n][m - 1].F,
+                                          err_msg=f"graph block {n + 1} level {m}")
+    ctx.close()
+    ctx2.close()
