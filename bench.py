@@ -272,6 +272,73 @@ def copy_probe(mib):
                       "launches": 3}))
 
 
+def config3_bench(args):
+    """The batched form's case: 24 blocks (about 7.6M cells) over the Gaussian bump, k-omega
+    SST, 3 multigrid levels, walls on the bottom row, one-to-one connections exchanged on the
+    device after every multi-block cycle.  Times K multi-block cycles four ways: the plain
+    FP64 path with eager launches, the same replayed from the captured multi-block graph
+    (parallel per-block branches), and the chosen form (FP32 LU-SGS records, autotuned tile
+    shapes, the graph).  Prints one JSON line."""
+    from finflo_amd.cases import config3
+    from finflo_amd.ffhip import Context
+    t0 = time.perf_counter()
+    prm, blocks, cons, walls, bcs = config3(seed=3, nlev=3)
+    nb = len(blocks)
+    ncell = sum(lv[0].imax * lv[0].jmax * lv[0].kmax for lv in blocks)
+    print(f"[bench] config3: {nb} blocks, {ncell} cells, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    ctx = Context(prm)
+    for n, lv in enumerate(blocks, start=1):
+        for m, b in enumerate(lv, start=1):
+            ctx.register(b, level=m, block=n)
+            ctx.upload(level=m, block=n)
+            if walls[n - 1][m - 1] is not None:
+                ctx.wall_register(*walls[n - 1][m - 1], level=m, block=n)
+            ctx.bc_register(bcs[n - 1][m - 1], level=m, block=n)
+    for n in range(1, nb + 1):
+        ctx.con_register(cons[n - 1], block=n)
+        for m in range(1, 4):
+            ctx.ghostext(level=m, block=n)
+    ctx.halo_exchange(what="geometry")
+    ctx.halo_exchange(what="flow")
+    ids = list(range(1, nb + 1))
+    res = {}
+    for label, prec, tune, graph in (("fp64 eager", "fp64", False, False),
+                                      ("fp64 graph", "fp64", False, True),
+                                      ("chosen form", "mixed", True, True)):
+        ctx.precision(prec)
+        ctx.autotune(tune)
+        run = (lambda k: ctx.vcycle_blocks_graph(ids, 3, ncycles=k)) if graph else \
+              (lambda k: ctx.vcycle_blocks(ids, 3, ncycles=k))
+        ctx.vcycle_blocks(ids, 3, ncycles=max(1, args.warmup))   # tunes, allocates
+        run(1)
+        ctx.finish()
+        t1 = time.perf_counter()
+        run(args.steps)
+        ctx.finish()
+        el = time.perf_counter() - t1
+        res[label] = {"ms_per_step": el * 1e3 / args.steps, "value": ncell * args.steps / el}
+        print(f"[bench] config3 {label}: {el * 1e3 / args.steps:.2f} ms per cycle", file=sys.stderr, flush=True)
+    r = ctx.vcycle_blocks(ids, 3, ncycles=1)
+    base = res["fp64 eager"]
+    line = {
+        "metric": "cell-updates/s per MI355X (residual+LU-SGS), multi-block batched form",
+        "value": res["chosen form"]["value"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["chosen form"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 with FP32 LU-SGS records", "data": "synthetic",
+        "config": {"workload": f"24-block Gaussian bump case, {ncell} cells, k-omega SST, M=0.5 "
+                               "Re=1e6, 3-level multigrid, one-to-one connections exchanged on "
+                               "the device; a step is one multi-block cycle",
+                   "blocks": nb, "cells": ncell, "forms": res,
+                   "speedup_chosen_over_fp64_eager": base["ms_per_step"] / res["chosen form"]["ms_per_step"],
+                   "density_resl2": float(np.sqrt(r[0]) / ncell)},
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
 def earsm_mlp_bench(args):
     """The learned EARSM closure on a 256x128x128 block (4.19M cells, k-omega SST with
     EARSM): exact-EARSM samples of every cell, training (Adam minibatches), inference of
@@ -374,7 +441,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp"], default="sst-mg")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp", "config3"],
+                    default="sst-mg")
     ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
@@ -385,9 +453,9 @@ def main():
     ap.add_argument("--kbc", choices=["ext", "periodic"], default="ext",
                     help="the K faces in the convergence run: free stream (EXT) or connected "
                          "to each other (periodic span)")
-    ap.add_argument("--converge-cycles", type=int, default=0,
+    ap.add_argument("--converge-cycles", type=int, default=2000,
                     help="after the timing, run from the initial state until an 8-decade drop "
-                         "of the density residual or this many cycles (0: skip)")
+                         "of the density residual or this many cycles (0: skip; default 2000)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the bench: run a plain device copy of MIB MiB three times "
@@ -404,6 +472,9 @@ def main():
 
     if args.workload == "earsm-mlp":
         earsm_mlp_bench(args)
+        return
+    if args.workload == "config3":
+        config3_bench(args)
         return
 
     from finflo_amd.ffhip import Context
