@@ -272,21 +272,27 @@ def copy_probe(mib):
                       "launches": 3}))
 
 
-def config3_bench(args):
-    """The batched form's case: 24 blocks (about 7.6M cells) over the Gaussian bump, k-omega
-    SST, 3 multigrid levels, walls on the bottom row, one-to-one connections exchanged on the
-    device after every multi-block cycle.  Times K multi-block cycles four ways: the plain
-    FP64 path with eager launches, the same replayed from the captured multi-block graph
-    (parallel per-block branches), and the chosen form (FP32 LU-SGS records, autotuned tile
-    shapes, the graph).  Prints one JSON line."""
-    from finflo_amd.cases import config3
+def multiblock_bench(args):
+    """The multi-block cases over the wall bump, with one-to-one connections exchanged on the
+    device after every multi-block cycle:
+    * config3: 24 blocks (about 7.6M cells), k-omega SST, 3 multigrid levels;
+    * config5: 64 blocks (about 40M cells) around a swept bump, k-omega SST with the EARSM,
+      4 multigrid levels; the chosen form adds the learned closure, trained first on the
+      exact EARSM samples of the first blocks.
+    Times K multi-block cycles: the plain FP64 path with eager launches, the same replayed
+    from the captured multi-block graph (parallel per-block branches), and the chosen form
+    (FP32 LU-SGS records, autotuned tile shapes, the graph; with the closure on config5).
+    Prints one JSON line; `value` is the chosen form's."""
+    from finflo_amd.cases import config3, config5
     from finflo_amd.ffhip import Context
     t0 = time.perf_counter()
-    prm, blocks, cons, walls, bcs = config3(seed=3, nlev=3)
+    five = args.workload == "config5"
+    prm, blocks, cons, walls, bcs = (config5 if five else config3)()
+    nlev = len(blocks[0])
     nb = len(blocks)
     ncell = sum(lv[0].imax * lv[0].jmax * lv[0].kmax for lv in blocks)
-    print(f"[bench] config3: {nb} blocks, {ncell} cells, built in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    print(f"[bench] {args.workload}: {nb} blocks, {ncell} cells, {nlev} levels, built in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     ctx = Context(prm)
     for n, lv in enumerate(blocks, start=1):
         for m, b in enumerate(lv, start=1):
@@ -297,20 +303,45 @@ def config3_bench(args):
             ctx.bc_register(bcs[n - 1][m - 1], level=m, block=n)
     for n in range(1, nb + 1):
         ctx.con_register(cons[n - 1], block=n)
-        for m in range(1, 4):
+        for m in range(1, nlev + 1):
             ctx.ghostext(level=m, block=n)
     ctx.halo_exchange(what="geometry")
     ctx.halo_exchange(what="flow")
     ids = list(range(1, nb + 1))
+    closure = None
+    if five:
+        # the learned closure: exact-EARSM samples of the first eight blocks after one
+        # exact cycle, normalisation, Adam steps on the device
+        import torch
+        from finflo_amd.mlp import init_params
+        ctx.vcycle_blocks(ids, nlev, ncycles=1)
+        xs, ys = [], []
+        for n in range(1, 9):
+            ctx.turbdi(block=n)
+            x, y = ctx.earsm_samples(block=n)
+            xs.append(x)
+            ys.append(y)
+        x, y = torch.cat(xs), torch.cat(ys)
+        ctx.mlp_fit_norm(x, y)
+        ctx.mlp_set(init_params(5))
+        t1 = time.perf_counter()
+        loss = ctx.mlp_train(x, y, steps=2000, batch=8192, lr=3e-3, seed=11)
+        closure = {"samples": len(x), "train_steps": 2000, "train_s": time.perf_counter() - t1,
+                   "loss_first": float(loss[0]), "loss_last": float(loss[-1])}
+        print(f"[bench] closure trained on {len(x)} samples: loss {loss[0]:.3g} -> {loss[-1]:.3g}",
+              file=sys.stderr, flush=True)
     res = {}
-    for label, prec, tune, graph in (("fp64 eager", "fp64", False, False),
-                                      ("fp64 graph", "fp64", False, True),
-                                      ("chosen form", "mixed", True, True)):
+    forms = [("fp64 eager", "fp64", False, False, "exact"),
+             ("fp64 graph", "fp64", False, True, "exact"),
+             ("chosen form", "mixed", True, True, "mlp" if five else "exact")]
+    for label, prec, tune, graph, earsm in forms:
         ctx.precision(prec)
         ctx.autotune(tune)
-        run = (lambda k: ctx.vcycle_blocks_graph(ids, 3, ncycles=k)) if graph else \
-              (lambda k: ctx.vcycle_blocks(ids, 3, ncycles=k))
-        ctx.vcycle_blocks(ids, 3, ncycles=max(1, args.warmup))   # tunes, allocates
+        if five:
+            ctx.earsm_mode(earsm)
+        run = (lambda k: ctx.vcycle_blocks_graph(ids, nlev, ncycles=k)) if graph else \
+              (lambda k: ctx.vcycle_blocks(ids, nlev, ncycles=k))
+        ctx.vcycle_blocks(ids, nlev, ncycles=max(1, args.warmup))   # tunes, allocates
         run(1)
         ctx.finish()
         t1 = time.perf_counter()
@@ -318,21 +349,30 @@ def config3_bench(args):
         ctx.finish()
         el = time.perf_counter() - t1
         res[label] = {"ms_per_step": el * 1e3 / args.steps, "value": ncell * args.steps / el}
-        print(f"[bench] config3 {label}: {el * 1e3 / args.steps:.2f} ms per cycle", file=sys.stderr, flush=True)
-    r = ctx.vcycle_blocks(ids, 3, ncycles=1)
+        print(f"[bench] {args.workload} {label}: {el * 1e3 / args.steps:.2f} ms per cycle",
+              file=sys.stderr, flush=True)
+    r = ctx.vcycle_blocks(ids, nlev, ncycles=1)
     base = res["fp64 eager"]
+    tuned = {}
+    for n in ids[:4]:
+        t = ctx.tuned_shape(level=1, block=n)
+        if t:
+            tuned[f"block{n}"] = {"shape": t[0], "ms": t[1]}
     line = {
-        "metric": "cell-updates/s per MI355X (residual+LU-SGS), multi-block batched form",
+        "metric": "cell-updates/s per MI355X (residual+LU-SGS), multi-block chosen form",
         "value": res["chosen form"]["value"], "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": res["chosen form"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 with FP32 LU-SGS records", "data": "synthetic",
-        "config": {"workload": f"24-block Gaussian bump case, {ncell} cells, k-omega SST, M=0.5 "
-                               "Re=1e6, 3-level multigrid, one-to-one connections exchanged on "
-                               "the device; a step is one multi-block cycle",
+        "dtype": "f64 with FP32 LU-SGS records" + (", bf16 closure" if five else ""),
+        "data": "synthetic",
+        "config": {"workload": f"{nb}-block {'swept ' if five else ''}bump case, {ncell} cells, "
+                               f"k-omega SST{' + EARSM' if five else ''}, M={0.6 if five else 0.5} "
+                               f"Re={'1e7' if five else '1e6'}, {nlev}-level multigrid, one-to-one "
+                               "connections exchanged on the device; a step is one multi-block cycle",
                    "blocks": nb, "cells": ncell, "forms": res,
                    "speedup_chosen_over_fp64_eager": base["ms_per_step"] / res["chosen form"]["ms_per_step"],
+                   "tuned_shapes": tuned, "closure": closure,
                    "density_resl2": float(np.sqrt(r[0]) / ncell)},
     }
     print(json.dumps(line))
@@ -441,7 +481,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp", "config3"],
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp", "config3", "config5"],
                     default="sst-mg")
     ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
@@ -473,8 +513,8 @@ def main():
     if args.workload == "earsm-mlp":
         earsm_mlp_bench(args)
         return
-    if args.workload == "config3":
-        config3_bench(args)
+    if args.workload in ("config3", "config5"):
+        multiblock_bench(args)
         return
 
     from finflo_amd.ffhip import Context

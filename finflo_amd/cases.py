@@ -331,8 +331,8 @@ def bump_nodes(imax, jmax, kmax, surface, xlen=2.0, height=1.0, zlen=1.0, first=
                ratio=None):
     """Nodes of a box over a wall bump: I and K uniform over [-xlen/2, xlen/2] and
     [-zlen/2, zlen/2], J lines from the bump surface to y = height, geometric from the wall
-    (first cell `first` of the local height and growth `ratio`, or the growth that fills
-    the height when ratio is None)."""
+    with the first cell `first` of the local height: growing by the rate that fills the
+    height when ratio is None, else by `ratio` up to a spacing cap that fills it."""
     xi = np.linspace(-0.5 * xlen, 0.5 * xlen, imax + 1)
     zk = np.linspace(-0.5 * zlen, 0.5 * zlen, kmax + 1)
     if ratio is None:
@@ -344,8 +344,22 @@ def bump_nodes(imax, jmax, kmax, surface, xlen=2.0, height=1.0, zlen=1.0, first=
             else:
                 lo = r
         ratio = 0.5 * (lo + hi)
-    eta = geometric_from_wall(jmax, 1.0, ratio)
-    eta = eta / eta[-1]
+        eta = geometric_from_wall(jmax, 1.0, ratio)
+        eta = eta / eta[-1]
+    else:
+        # the first cell and the growth as given, the spacing capped where the cells would
+        # otherwise overshoot the height: min(first * ratio^n, cap) summing to 1
+        lo, hi = first, 1.0
+        for _ in range(200):
+            cap = 0.5 * (lo + hi)
+            w = np.minimum(first * ratio ** np.arange(jmax), cap)
+            if w.sum() > 1.0:
+                hi = cap
+            else:
+                lo = cap
+        w = np.minimum(first * ratio ** np.arange(jmax), 0.5 * (lo + hi))
+        eta = np.concatenate([[0.0], np.cumsum(w)])
+        eta = eta / eta[-1]
     X = _extend(xi)
     Z = _extend(zk)
     E = _extend(eta)
@@ -458,45 +472,46 @@ def block_from_nodes(nodes, lo, hi, prm, seed=0, amp=0.0, kind="velocity", turb=
     return blk, sub
 
 
-def config3_layout(seed=3, target=8.0e6, tol=0.07):
-    """The segment lengths of the 24-block case: 4 x 3 x 2 blocks whose dimensions are drawn
-    from multiples of 4 in [16, 96] with a seeded generator until the total is within `tol`
-    of `target` cells."""
+def segment_layout(counts, seed, target, tol=0.07, lo=16, hi=96, mult=4):
+    """Block dimensions along each axis: counts[a] segments per axis a, each a multiple of
+    `mult` in [lo, hi], drawn with a seeded generator until the total cell count is within
+    `tol` of `target`."""
     rng = np.random.default_rng(seed)
     while True:
-        si = rng.integers(4, 25, 4) * 4
-        sj = rng.integers(4, 25, 3) * 4
-        sk = rng.integers(4, 25, 2) * 4
-        n = int(si.sum() * sj.sum() * sk.sum())
-        if abs(n - target) <= tol * target:
-            return si, sj, sk
+        segs = [rng.integers(lo // mult, hi // mult + 1, n) * mult for n in counts]
+        total = int(np.prod([sg.sum() for sg in segs]))
+        if abs(total - target) <= tol * target:
+            return segs
 
 
-def config3(seed=3, nlev=3):
-    """The batched form's case: 24 blocks (4 x 3 x 2, seeded dimensions in [16, 96], about
-    8M cells) tiling a 2 x 1 x 1 box over a smoothed Gaussian wall bump (height 0.05, width
-    0.2, first wall cell 1e-5), k-omega SST, M = 0.5, Re = 1e6, CFL 5, nlev multigrid levels.
-    Returns the parameters, the blocks (each a list of its levels, block ids 1..24), their
-    connection patches (tuples of ff_con_patch), their walls (the J-low faces of the bottom
-    row) and their boundary patches (walls and free stream)."""
-    prm = flow_params(iturb=6, mach=0.5, reynolds=1e6, cfl=5.0)
-    turb = freestream_turbulence(prm, limits=True)
-    si, sj, sk = config3_layout(seed)
-    surf = gaussian_bump(0.05, 0.2)
+def config3_layout(seed=3, target=8.0e6, tol=0.07):
+    """The segment lengths of the 24-block case: 4 x 3 x 2 blocks whose dimensions are drawn
+    from multiples of 4 in [16, 96] until the total is within `tol` of `target` cells."""
+    return segment_layout((4, 3, 2), seed, target, tol)
+
+
+def bump_blocks(prm, segs, surf, nlev, seed, first, ratio=None, turb=None, amp=0.005):
+    """Blocks tiling a 2 x 1 x 1 box over the wall bump `surf`, one per combination of the
+    axis segments `segs`, each with its nlev levels, in the global index orientation; their
+    one-to-one connections (full faces), walls (the J-low faces of the bottom row: BOTSKE
+    patches and REFLEC's viscous branch) and free-stream (EXT) patches on the other outer
+    faces.  Block ids run I fastest, then J, then K, from 1."""
+    si, sj, sk = segs
     nodes = bump_nodes(int(si.sum()), int(sj.sum()), int(sk.sum()), surf, xlen=2.0,
-                       height=1.0, zlen=1.0, first=1e-5)
-    ci, cj, ck = np.concatenate([[0], np.cumsum(si)]), np.concatenate([[0], np.cumsum(sj)]), \
-        np.concatenate([[0], np.cumsum(sk)])
+                       height=1.0, zlen=1.0, first=first, ratio=ratio)
+    ci = np.concatenate([[0], np.cumsum(si)])
+    cj = np.concatenate([[0], np.cumsum(sj)])
+    ck = np.concatenate([[0], np.cumsum(sk)])
     ids = {}
-    blocks, cons, walls, bcs = [], [], [], []
-    for c in range(2):
-        for b in range(3):
-            for a in range(4):
+    for c in range(len(sk)):
+        for b in range(len(sj)):
+            for a in range(len(si)):
                 ids[(a, b, c)] = len(ids) + 1
+    blocks, cons, walls, bcs = [], [], [], []
     for (a, b, c), n in ids.items():
         lo = (ci[a] + 1, cj[b] + 1, ck[c] + 1)
         hi = (ci[a + 1], cj[b + 1], ck[c + 1])
-        fine, sub = block_from_nodes(nodes, lo, hi, prm, seed=seed * 100 + n, amp=0.005,
+        fine, sub = block_from_nodes(nodes, lo, hi, prm, seed=seed * 100 + n, amp=amp,
                                      kind="velocity", turb=turb, turb_amp=0.0, surface=surf)
         lv = [fine]
         cn = sub
@@ -508,7 +523,6 @@ def config3(seed=3, nlev=3):
             fill_state(cb, prm, seed=0, amp=0.0, kind="velocity", turb=turb, turb_amp=0.0)
             lv.append(cb)
         blocks.append(lv)
-        # connections: full faces, the same index orientation on both sides
         pc = []
         nb = {1: (a - 1, b, c), 4: (a + 1, b, c), 2: (a, b - 1, c), 5: (a, b + 1, c),
               3: (a, b, c - 1), 6: (a, b, c + 1)}
@@ -520,7 +534,7 @@ def config3(seed=3, nlev=3):
                 pc.append((face, 1, ea, 1, eb, ids[key], opp, 1, 1, 0, 1, 1))
         cons.append(pc)
         wl, bl = [], []
-        for m, blk in enumerate(lv):
+        for blk in lv:
             wl.append(plate_wall(blk, prm) if b == 0 else None)
             faces = []
             for face, key in nb.items():
@@ -528,9 +542,37 @@ def config3(seed=3, nlev=3):
                     continue
                 ea = blk.jmax if face in (1, 4) else blk.imax
                 eb = blk.jmax if face in (3, 6) else blk.kmax
-                ibc = 8 if face == 2 else 2
-                faces.append(BcPatch(ibc, face, 1, ea, 1, eb, iheat=0, viscous=1))
+                faces.append(BcPatch(8 if face == 2 else 2, face, 1, ea, 1, eb, iheat=0, viscous=1))
             bl.append(faces)
         walls.append(wl)
         bcs.append(bl)
+    return blocks, cons, walls, bcs
+
+
+def config3(seed=3, nlev=3):
+    """The batched form's case: 24 blocks (4 x 3 x 2, seeded dimensions in [16, 96], about
+    8M cells) tiling a 2 x 1 x 1 box over a smoothed Gaussian wall bump (height 0.05, width
+    0.2, first wall cell 1e-5), k-omega SST, M = 0.5, Re = 1e6, CFL 5, nlev multigrid levels.
+    Returns the parameters, the blocks (each a list of its levels, block ids 1..24), their
+    connection patches (tuples of ff_con_patch), their walls (the J-low faces of the bottom
+    row) and their boundary patches (walls and free stream)."""
+    prm = flow_params(iturb=6, mach=0.5, reynolds=1e6, cfl=5.0)
+    turb = freestream_turbulence(prm, limits=True)
+    segs = config3_layout(seed)
+    blocks, cons, walls, bcs = bump_blocks(prm, segs, gaussian_bump(0.05, 0.2), nlev, seed,
+                                           first=1e-5, turb=turb)
+    return prm, blocks, cons, walls, bcs
+
+
+def config5(seed=5, nlev=4, target=4.0e7):
+    """The full chosen form's case: 64 blocks (4 x 4 x 4, seeded dimensions, multiples of 8
+    in [16, 128] so that four multigrid levels halve them, about 40M cells) around a swept
+    bump (height 0.05, width 0.2, crest swept 30 degrees), first wall cell 1e-6 growing by
+    1.12, k-omega SST with the EARSM (ISTRES = 1; the learned closure is the caller's
+    choice), M = 0.6, Re = 1e7, CFL 5."""
+    prm = flow_params(iturb=6, mach=0.6, reynolds=1e7, cfl=5.0, istres=1)
+    turb = freestream_turbulence(prm, limits=True)
+    segs = segment_layout((4, 4, 4), seed, target, tol=0.05, lo=16, hi=128, mult=8)
+    blocks, cons, walls, bcs = bump_blocks(prm, segs, gaussian_bump(0.05, 0.2, sweep=np.pi / 6),
+                                           nlev, seed, first=1e-6, ratio=1.12, turb=turb)
     return prm, blocks, cons, walls, bcs

@@ -4231,8 +4231,10 @@ static int launch_shape_any(ff_ctx *c, int s, bool kw, const Prm &kp, const Blk 
 
 // The shape for this block and kind: from the table, or, on the first residual of such a
 // block outside a graph capture, by timing every candidate (one launch to warm up, three
-// timed with HIP events; the residual kernel only writes what it recomputes from the same
-// inputs, so the repeated launches leave the same results).  -1: the built-in shapes.
+// timed with HIP events).  The k-omega kernel reads SEPS (the cross-diffusion CROSKO left
+// there) before it overwrites it with the omega source, so SEPS is saved first and put back
+// before every launch; every other output is recomputed from unchanged inputs.  -1: the
+// built-in shapes.
 static int tuned_shape(ff_ctx *c, bool kw, const Prm &kp, const Blk &b, const WallDev &wd)
 {
     if (c->force_shape >= 0) return c->force_shape;
@@ -4242,30 +4244,41 @@ static int tuned_shape(ff_ctx *c, bool kw, const Prm &kp, const Blk &b, const Wa
     if (it != c->tuned.end()) return it->second.first;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return -1;
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
+    const size_t bytes = sizeof(double) * (size_t)b.ntot;
+    double *seps = kw ? FLD(b, FF_SEPS) : nullptr, *save = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((kw && hipMalloc(&save, bytes) != hipSuccess) || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        if (save) (void)hipFree(save);
+        if (e0) (void)hipEventDestroy(e0);
         return -1;
     }
+    auto restore = [&]() { return !kw || hipMemcpyAsync(seps, save, bytes, hipMemcpyDeviceToDevice, c->stream) == hipSuccess; };
+    bool ok = !kw || hipMemcpyAsync(save, seps, bytes, hipMemcpyDeviceToDevice, c->stream) == hipSuccess;
     int best = -1;
     double best_ms = 1e30;
-    for (int sh = 0; sh < NSHAPE; sh++) {
-        if (launch_shape_any(c, sh, kw, kp, b, wd)) break;
-        float ms = 0.f;
-        if (hipEventRecord(e0, c->stream) != hipSuccess) break;
-        bool ok = true;
-        for (int rep = 0; rep < 3 && ok; rep++) ok = launch_shape_any(c, sh, kw, kp, b, wd) == FF_OK;
-        if (!ok || hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-            break;
-        if (ms / 3. < best_ms) {
-            best_ms = ms / 3.;
+    for (int sh = 0; sh < NSHAPE && ok; sh++) {
+        ok = restore() && launch_shape_any(c, sh, kw, kp, b, wd) == FF_OK;
+        double tot = 0.;
+        for (int rep = 0; rep < 3 && ok; rep++) {
+            float ms = 0.f;
+            ok = restore() && hipEventRecord(e0, c->stream) == hipSuccess &&
+                 launch_shape_any(c, sh, kw, kp, b, wd) == FF_OK &&
+                 hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            tot += ms;
+        }
+        if (ok && tot / 3. < best_ms) {
+            best_ms = tot / 3.;
             best = sh;
         }
     }
+    ok = restore() && ok;
+    (void)hipStreamSynchronize(c->stream);
+    if (save) (void)hipFree(save);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (!ok) return -1;
     if (best >= 0) c->tuned[key] = {best, best_ms};
     return best;
 }

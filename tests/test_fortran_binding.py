@@ -40,7 +40,7 @@ def test_module_compiles_links_and_matches_the_c_layouts(tmp_path):
     if not os.path.exists(os.path.join(LIBDIR, "libffhip.so")):
         pytest.skip("libffhip.so not built")
     names = bound_functions()
-    assert len(names) >= 20
+    assert len(names) >= 29
     refs = "\n".join(f"      P = C_FUNLOC({n})\n      IF (.NOT. C_ASSOCIATED(P)) STOP 2"
                      for n in names)
     prog = f"""      PROGRAM SIZES
@@ -50,12 +50,14 @@ def test_module_compiles_links_and_matches_the_c_layouts(tmp_path):
       TYPE(FF_WALL_PARAMS) :: WP
       TYPE(FF_WALL_PATCH) :: WPA
       TYPE(FF_BLOCK) :: BLK
+      TYPE(FF_BC_PATCH) :: BCP
+      TYPE(FF_CON_PATCH) :: CNP
       TYPE(C_FUNPTR) :: P
       CHARACTER(KIND=C_CHAR), POINTER :: MSG(:)
 {refs}
       CALL C_F_POINTER(FF_LAST_ERROR(C_NULL_PTR), MSG, [12])
-      WRITE(*,'(4I8)') C_SIZEOF(PRM), C_SIZEOF(WP), C_SIZEOF(WPA),
-     +  C_SIZEOF(BLK)
+      WRITE(*,'(6I8)') C_SIZEOF(PRM), C_SIZEOF(WP), C_SIZEOF(WPA),
+     +  C_SIZEOF(BLK), C_SIZEOF(BCP), C_SIZEOF(CNP)
       WRITE(*,'(12A)') MSG
       END PROGRAM
 """
@@ -68,8 +70,9 @@ def test_module_compiles_links_and_matches_the_c_layouts(tmp_path):
     out = subprocess.check_output([str(tmp_path / "sizes")], cwd=tmp_path, text=True).split("\n")
     fsz = [int(v) for v in out[0].split()]
     assert out[1].strip() == "null context"
-    csrc = ('#include <stdio.h>\n#include "ffhip.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
-            'sizeof(ff_params), sizeof(ff_wall_params), sizeof(ff_wall_patch), sizeof(ff_block));'
+    csrc = ('#include <stdio.h>\n#include "ffhip.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu\\n",'
+            'sizeof(ff_params), sizeof(ff_wall_params), sizeof(ff_wall_patch), sizeof(ff_block),'
+            'sizeof(ff_bc_patch), sizeof(ff_con_patch));'
             'return 0;}\n')
     (tmp_path / "sizes.c").write_text(csrc)
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "sizes.c", "-o", "csizes"],
