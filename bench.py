@@ -483,7 +483,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp", "config3", "config5"],
                     default="sst-mg")
-    ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged"], default=None,
+    ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged", "staged1"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
                     help="replay the steps from a captured HIP graph")

@@ -204,7 +204,7 @@ extern "C" int ff_vcycle_blocks_graph(ff_ctx *c, const int *blocks, int nblocks,
         for (int q = 0; q <= ff_ctx::NPOOL; q++)
             if (!c->pool_ev[q] && hipEventCreateWithFlags(&c->pool_ev[q], hipEventDisableTiming) != hipSuccess)
                 return fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: events%s");
-        if (c->lusgs_mode == FF_LUSGS_STAGED)
+        if (staged_mode(c))
             for (auto &l : lv)
                 for (BlockRec *r : l) {
                     int rc = staged_alloc(c, r);

@@ -2947,6 +2947,287 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
 }
 
 // ------------------------------------------------------------------------------------
+// The staged sweeps with four waves per tile.  Timestamps taken inside the one-wave kernels
+// above (s_memtime at the phase boundaries of every step, 192x96x96 level) split a step of
+// about 4,000 cycles into the face inputs (about 900: LDS reads of the exchange buffers and
+// the halo ring, the marker tests), the recurrence and the three outgoing face
+// contributions (about 2,000) and the issue of the next record's LDS-DMA copies (about
+// 600: 28 loads of 1 KiB, which the memory pipeline accepts at about 64 bytes a clock).
+// All three are latencies on the step's critical path.  Here, per tile:
+// * waves 0-2 all form dn of the step from the same inputs, and wave f then forms only the
+//   contribution across face f (0: I, 1: J, 2: K) and leaves it in LDS; wave 1 publishes
+//   the J-face halo words and wave 2 the K-face ones;
+// * wave 3 issues every LDS-DMA copy (records, increments, halo blocks) and waits for them,
+//   so the three compute waves never wait on memory;
+// * one workgroup barrier per step separates a step's LDS writes from the next step's
+//   reads (the exchange buffers are double-buffered by step parity), and the copy wave
+//   reaches it only once the copies of the next step are complete.
+// Arithmetic is that of k_pred_st / k_corr_st, operation for operation, so the results
+// are the same bit for bit.
+// ------------------------------------------------------------------------------------
+constexpr int SW = 3;              // compute waves per workgroup: one per face
+constexpr int WGW = SW + 1;        // and the copy wave
+
+// The workgroup's exchange point between two steps: LDS writes before it are visible to
+// every wave after it.  Global memory is not ordered (the halo slots carry their own
+// protocol, and the copy wave waits for its LDS-DMA copies with vm_wait before it).
+__device__ __forceinline__ void wg_step_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// the tile ticket of a multi-wave workgroup: taken by its first lane, shared through LDS
+__device__ __forceinline__ int tile_ticket_wg(const Sweep &s, int *slot)
+{
+    if (threadIdx.x == 0) *slot = (int)(atomicAdd(s.ticket, 1ULL) % (unsigned long long)(s.ntj * s.ntk));
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+// PREDGS / PRKEGS on staged records, four waves per (J,K) tile.  The copy wave's issue order
+// within step s: the halo blocks of step s + 2, the record of step s + 3; at the top of step
+// s the two later records (2 NLD loads) follow the halo blocks of step s, which follow the
+// record of step s, so waiting until 2 NLD loads remain covers both.
+template <bool KE, typename RT = double>
+__global__ void __launch_bounds__(WGW * TW) k_pred_wg(Prm p, Blk b, Sweep sw, SweepSt st)
+{
+    constexpr int NV = KE ? 7 : 5;
+    constexpr int RS = 4;                               // ring slots: 3 steps of lead
+    constexpr int RBYTES = rbytes<RT>();
+    constexpr int NLD = RBYTES / 1024;                  // record loads per step
+    static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
+    __shared__ __attribute__((aligned(16))) RT ring[RS][NRF * TW];
+    __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
+    __shared__ double xo[SW][2][NV][TW], zx[NV * TK];
+    __shared__ int tkt;
+    const RT *const recp = (const RT *)st.recp;
+    const int lane = threadIdx.x % TW, w = __builtin_amdgcn_readfirstlane(threadIdx.x / TW);
+    const int jj = lane % TJ, kk = lane / TJ;
+    if (w == 0) {
+        if (lane < NV * TK) zx[lane] = 0.;
+        // a lane's I-face input before its first cell is zero
+#pragma unroll
+        for (int r = 0; r < NV; r++) xo[0][0][r][lane] = xo[0][1][r][lane] = 0.;
+    }
+    const int t = tile_ticket_wg(sw, &tkt);
+    const int tj = sw.ntj - 1 - t % sw.ntj, tk = sw.ntk - 1 - t / sw.ntj;
+    const int tile = tk * sw.ntj + tj;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
+    const bool col = j <= b.jmax && k <= b.kmax;
+    const bool hasj = tj + 1 < sw.ntj, hask = tk + 1 < sw.ntk;   // upstream tiles
+    const int lag = (TJ - 1 - jj) + (TK - 1 - kk);
+    const int nsteps = st.nstep;
+    const unsigned long long *hj0 = hj_blk(sw, b, tj + 1, tk, 0), *hk0 = hk_blk(sw, b, tk + 1, tj, 0);
+    auto halo_copy = [&](int s) {
+        const int d = b.imax + TK - 2 - s;
+        if (d < 0 || d > b.imax + TK - 2) return;
+        if (hasj) halo_dma(hj0 + (size_t)d * HBLK, hring[s % HR][0], lane);
+        if (hask) halo_dma(hk0 + (size_t)d * HBLK, hring[s % HR][1], lane);
+    };
+    auto rec_copy = [&](int s) {
+        ring_copy<RBYTES>(recp + st_row(st, tile, nsteps - 1 - s, 0), ring[s % RS], lane);
+    };
+    const FaceSrc fj(j == b.jmax, jj < TJ - 1, &xo[1][0][0][min(lane + 1, TW - 1)], hring[0][0] + kk, zx + kk);
+    const FaceSrc fk(k == b.kmax, kk < TK - 1, &xo[2][0][0][min(lane + TJ, TW - 1)], hring[0][1] + jj, zx + jj);
+    const bool copier = w == SW;
+    if (copier) {
+        rec_copy(0);
+        halo_copy(0);
+        if (1 < nsteps) rec_copy(1);
+        halo_copy(1);
+        if (2 < nsteps) rec_copy(2);
+    }
+    for (int s = 0; s < nsteps; s++) {
+        const int row = nsteps - 1 - s;
+        if (copier) {
+            if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
+        }
+        wg_step_sync();
+        if (copier) {
+            // the slots of step s - 1, which every wave has finished reading
+            halo_copy(s + 2);
+            if (s + RS - 1 < nsteps) rec_copy(s + RS - 1);
+            continue;
+        }
+        const RT *cur = ring[s % RS] + lane;
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.imax) {
+            const int i = b.imax - q;
+            const int rb = (s + 1) & 1;
+            double yi[NV], bj[NV], ckc[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) yi[r] = xo[0][rb][r][lane];
+            const unsigned xoff = rb * (NV * TW * 8), hoff = (s % HR) * (2 * HBLK * 8);
+            faces_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
+                         [&] { return hj_slot(sw, b, tj + 1, i, k); },
+                         fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
+                         [&] { return hk_slot(sw, b, tk + 1, i, j); }, sw.err, bj, ckc);
+            // the line-buffer shift of PREDGS (imps4.f:556-562) never hands the I+1
+            // contribution to a cell of the row J = JMAX; PRKEGS's (imps4.f:1762) does
+            if (!KE && j == b.jmax) {
+#pragma unroll
+                for (int r = 0; r < NV; r++) yi[r] = 0.;
+            }
+            double dn[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) {
+                double cd = cur[RP_MK * TW] * ckc[r] + cur[(RP_VD + r) * TW];
+                cd = cd - cur[RP_DT1 * TW] * (cur[RP_WI * TW] * yi[r] + cur[RP_WJ * TW] * bj[r]);
+                dn[r] = r != 4 ? cd * cur[RP_SCINV * TW] : cd * cur[RP_SCINP * TW];
+            }
+            // the increments for the corrector, their stores dealt between the waves
+            double *dno = st.dn + (((size_t)tile * nsteps + row) * NDN) * TW + lane;
+#pragma unroll
+            for (int r = 0; r < NV; r++)
+                if (r % SW == w) dno[(size_t)r * TW] = dn[r];
+            SweepCell sc;
+            double scv[7];
+#pragma unroll
+            for (int m = 0; m < 7; m++) scv[m] = cur[(RP_SC + m) * TW];
+            sc_get<KE>(sc, scv);
+            double bv[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) bv[m] = cur[(RP_B + 9 * w + m) * TW];
+            Coef cf;
+            cf.x1 = cur[(RP_X + 3 * w) * TW];
+            cf.x2 = cur[(RP_X + 3 * w + 1) * TW];
+            cf.x5 = cur[(RP_X + 3 * w + 2) * TW];
+            double o[NV];
+            face_xfer<KE>(dn, sc, basis_get(bv), cf, o);
+            const int wb = s & 1;
+#pragma unroll
+            for (int r = 0; r < NV; r++) xo[w][wb][r][lane] = o[r];
+            if (w == 1 && jj == 0 && j > 1) halo_put<NV, HQS>(hj_slot(sw, b, tj, i, k), o);
+            if (w == 2 && kk == 0 && k > 1) halo_put<NV, HQS>(hk_slot(sw, b, tk, i, j), o);
+        }
+    }
+    vm_wait<0>();
+}
+
+// DIAGGS + CORRGS / DIKEGS + COKEGS on staged records, four waves per tile, with the end of
+// IMPS (DRO/CP, D(rho k) and D(rho omega) times RO).  The copy wave's issue order within
+// step s: the halo blocks of step s + 2, the record and increments of step s + 2 (NLD
+// loads); at the top of step s the copy of step s + 1 follows that of step s, which follows
+// the halo blocks of step s, so waiting until NLD loads remain covers both.
+template <bool KE, typename RT = double>
+__global__ void __launch_bounds__(WGW * TW) k_corr_wg(Prm p, Blk b, Sweep sw, SweepSt st)
+{
+    constexpr int NV = KE ? 7 : 5;
+    constexpr int RS = 3;                               // ring slots: 2 steps of lead
+    constexpr int RBYTES = rbytes<RT>();
+    constexpr int NLD = (RBYTES + DBYTES) / 1024;       // record loads per step
+    static_assert(NLD * (RS - 2) <= 63, "vmcnt range");
+    __shared__ __attribute__((aligned(16))) RT ring[RS][NRF * TW];
+    __shared__ __attribute__((aligned(16))) double dring[RS][NDN * TW];
+    __shared__ __attribute__((aligned(16))) double hring[HR][2][HBLK];
+    __shared__ double xo[SW][2][NV][TW], zx[NV * TK];
+    __shared__ int tkt;
+    const RT *const recc = (const RT *)st.recc;
+    const int lane = threadIdx.x % TW, w = __builtin_amdgcn_readfirstlane(threadIdx.x / TW);
+    const int jj = lane % TJ, kk = lane / TJ;
+    if (w == 0) {
+        if (lane < NV * TK) zx[lane] = 0.;
+#pragma unroll
+        for (int r = 0; r < NV; r++) xo[0][0][r][lane] = xo[0][1][r][lane] = 0.;
+    }
+    const int t = tile_ticket_wg(sw, &tkt);
+    const int tj = t % sw.ntj, tk = t / sw.ntj;
+    const int tile = tk * sw.ntj + tj;
+    const int j = tj * TJ + jj + 1, k = tk * TK + kk + 1;
+    const bool col = j <= b.jmax && k <= b.kmax;
+    const bool hasj = tj > 0, hask = tk > 0;                  // upstream tiles
+    const int lag = jj + kk;
+    const int nsteps = st.nstep;
+    const unsigned long long *hj0 = hj_blk(sw, b, tj, tk, 0), *hk0 = hk_blk(sw, b, tk, tj, 0);
+    auto halo_copy = [&](int s) {
+        if (s > b.imax + TK - 2) return;
+        if (hasj) halo_dma(hj0 + (size_t)s * HBLK, hring[s % HR][0], lane);
+        if (hask) halo_dma(hk0 + (size_t)s * HBLK, hring[s % HR][1], lane);
+    };
+    const FaceSrc fj(j == 1, jj > 0, &xo[1][0][0][max(lane - 1, 0)], hring[0][0] + kk, zx + kk);
+    const FaceSrc fk(k == 1, kk > 0, &xo[2][0][0][max(lane - TJ, 0)], hring[0][1] + jj, zx + jj);
+    auto rec_copy = [&](int s) {
+        ring_copy<RBYTES>(recc + st_row(st, tile, s, 0), ring[s % RS], lane);
+        ring_copy<DBYTES>(st.dn + (((size_t)tile * nsteps + s) * NDN) * TW, dring[s % RS], lane);
+    };
+    const bool copier = w == SW;
+    if (copier) {
+        halo_copy(0);
+        rec_copy(0);
+        halo_copy(1);
+        if (1 < nsteps) rec_copy(1);
+    }
+    for (int s = 0; s < nsteps; s++) {
+        if (copier) {
+            if (s + RS - 2 < nsteps) vm_wait<NLD * (RS - 2)>(); else vm_wait<0>();
+        }
+        wg_step_sync();
+        if (copier) {
+            halo_copy(s + 2);
+            if (s + RS - 1 < nsteps) rec_copy(s + RS - 1);
+            continue;
+        }
+        const RT *cur = ring[s % RS] + lane;
+        const double *dni = dring[s % RS] + lane;
+        const int q = s - lag;
+        if (col && q >= 0 && q < b.imax) {
+            const int i = q + 1;
+            const int n = cid(b, i, j, k);
+            const int rb = (s + 1) & 1;
+            double yi[NV], bj[NV], ckc[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) yi[r] = xo[0][rb][r][lane];
+            const unsigned xoff = rb * (NV * TW * 8), hoff = (s % HR) * (2 * HBLK * 8);
+            faces_in<NV>(fj.at(xoff, hoff), fj.stride, fj.hmask != 0,
+                         [&] { return hj_slot(sw, b, tj, i, k); },
+                         fk.at(xoff, hoff), fk.stride, fk.hmask != 0,
+                         [&] { return hk_slot(sw, b, tk, i, j); }, sw.err, bj, ckc);
+            double d[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) d[r] = dni[r * TW] * cur[RC_FAC * TW];
+            double dn[NV];
+#pragma unroll
+            for (int r = 0; r < NV; r++) {
+                double cd = cur[RC_MK * TW] * ckc[r] + cur[RC_VOL * TW] * d[r];
+                cd = cd + cur[RC_DT1 * TW] * (cur[RC_WI * TW] * yi[r] + cur[RC_WJ * TW] * bj[r]);
+                dn[r] = r != 4 ? cd * cur[RC_SCINV * TW] : cd * cur[RC_SCINP * TW];
+            }
+            // the end of IMPS, its stores dealt between the waves
+            if (w == 0) FLD(b, FF_DRO)[n] = dn[0] / cur[RC_CP * TW];
+#pragma unroll
+            for (int r = 1; r < 5; r++)
+                if (r % SW == w) FLD(b, FF_DRO + r)[n] = dn[r];
+            if (KE) {
+                if (w == 2) FLD(b, FF_DRK)[n] = dn[NV > 5 ? 5 : 0] * cur[RC_RHO * TW];
+                if (w == 0) FLD(b, FF_DEPS)[n] = dn[NV > 6 ? 6 : 0] * cur[RC_RHO * TW];
+            }
+            SweepCell sc;
+            double scv[7];
+#pragma unroll
+            for (int m = 0; m < 7; m++) scv[m] = cur[(RC_SC + m) * TW];
+            sc_get<KE>(sc, scv);
+            double bv[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) bv[m] = cur[(RC_B + 9 * w + m) * TW];
+            Coef cf;
+            cf.x1 = cur[(RC_X + 3 * w) * TW];
+            cf.x2 = cur[(RC_X + 3 * w + 1) * TW];
+            cf.x5 = cur[(RC_X + 3 * w + 2) * TW];
+            double o[NV];
+            face_xfer<KE>(dn, sc, basis_get(bv), cf, o);
+            const int wb = s & 1;
+#pragma unroll
+            for (int r = 0; r < NV; r++) xo[w][wb][r][lane] = o[r];
+            if (w == 1 && jj == TJ - 1 && j < b.jmax) halo_put<NV, HQS>(hj_slot(sw, b, tj + 1, i, k), o);
+            if (w == 2 && kk == TK - 1 && k < b.kmax) halo_put<NV, HQS>(hk_slot(sw, b, tk + 1, i, j), o);
+        }
+    }
+    vm_wait<0>();
+}
+
+// ------------------------------------------------------------------------------------
 // LU-SGS as a tile wavefront marching along K.  A wave owns a KI x KJ tile of (I, J)
 // columns (16 x 4: each field load of a step is four coalesced 128-byte rows) and walks
 // K; lane (ii, jj) runs ii + jj steps behind the tile's corner (fewer in the backward
@@ -3551,6 +3832,7 @@ struct PendingEv {
 struct MlpRec {
     float *p = nullptr, *m = nullptr, *v = nullptr;    // parameters and Adam moments
     float *nrm = nullptr;                               // normalisation block
+    double *nscr = nullptr;                             // normalisation passes: partials, means
     float *part = nullptr;                              // per-wave gradient partials
     unsigned *idx = nullptr;                            // minibatch indices
     float *grad = nullptr;                              // reduced gradient and loss
@@ -3982,7 +4264,7 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
     if (!c) return FF_E_ARG;
     if (option == FF_OPT_LUSGS) {
         if (value != FF_LUSGS_TILES && value != FF_LUSGS_PLANES && value != FF_LUSGS_KMARCH &&
-            value != FF_LUSGS_STAGED)
+            value != FF_LUSGS_STAGED && value != FF_LUSGS_STAGED1)
             return fail(c, FF_E_ARG, "FF_OPT_LUSGS: unknown value%s");
         c->lusgs_mode = value;
         c->gen++;
@@ -4483,6 +4765,11 @@ int ff_residual_noalloc(ff_ctx *c, int level, const ff_block *dev, int m2, doubl
 }
 
 // the staged LU-SGS records of a block level, allocated on first use and kept
+static bool staged_mode(const ff_ctx *c)
+{
+    return c->lusgs_mode == FF_LUSGS_STAGED || c->lusgs_mode == FF_LUSGS_STAGED1;
+}
+
 static int staged_alloc(ff_ctx *c, BlockRec *r)
 {
     if (r->st_mem) return FF_OK;
@@ -4517,7 +4804,7 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         if ((rc = launch(c, "k_pred_km", k_sweep_km<false, false>, g, dim3(4 * TW), kp, b, r->swk))) return rc;
         return launch(c, "k_corr_km", k_sweep_km<true, false>, g, dim3(4 * TW), kp, b, r->swk);
     }
-    if (c->lusgs_mode == FF_LUSGS_STAGED) {
+    if (staged_mode(c)) {
         // records of both sweeps in step order, then the two wavefronts on them
         if ((rc = staged_alloc(c, r))) return rc;
         dim3 g(r->sw.ntj * r->sw.ntk);
@@ -4528,18 +4815,27 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
         const bool f32 = c->precision == FF_PREC_MIXED;
         auto prep = ke ? (f32 ? k_sweep_prep<true, float> : k_sweep_prep<true>)
                        : (f32 ? k_sweep_prep<false, float> : k_sweep_prep<false>);
-        auto pred = ke ? (f32 ? k_pred_st<true, float> : k_pred_st<true>)
-                       : (f32 ? k_pred_st<false, float> : k_pred_st<false>);
-        auto corr = ke ? (f32 ? k_corr_st<true, float> : k_corr_st<true>)
-                       : (f32 ? k_corr_st<false, float> : k_corr_st<false>);
+        // FF_LUSGS_STAGED: four waves per tile; FF_LUSGS_STAGED1: one
+        const bool w1 = c->lusgs_mode == FF_LUSGS_STAGED1;
+        auto pred = w1 ? (ke ? (f32 ? k_pred_st<true, float> : k_pred_st<true>)
+                             : (f32 ? k_pred_st<false, float> : k_pred_st<false>))
+                       : (ke ? (f32 ? k_pred_wg<true, float> : k_pred_wg<true>)
+                             : (f32 ? k_pred_wg<false, float> : k_pred_wg<false>));
+        auto corr = w1 ? (ke ? (f32 ? k_corr_st<true, float> : k_corr_st<true>)
+                             : (f32 ? k_corr_st<false, float> : k_corr_st<false>))
+                       : (ke ? (f32 ? k_corr_wg<true, float> : k_corr_wg<true>)
+                             : (f32 ? k_corr_wg<false, float> : k_corr_wg<false>));
+        const dim3 ts(w1 ? TW : WGW * TW);
+        // (forming the corrector's records on a side stream during the predictor's wavefront
+        // was measured slower: the latency-bound sweep loses more than the pass gains)
         rc = launch(c, "k_sweep_prep", prep, gp, dim3(PSW * TW), kp, b, r->sw, r->st, tp);
         if (rc) return rc;
         // each sweep leaves its halo slots filled; empty them for the next one
         const dim3 gf((unsigned)((r->ntile_slots + 255) / 256));
-        rc = launch(c, "k_pred_st", pred, g, dim3(TW), kp, b, r->sw, r->st);
+        rc = launch(c, "k_pred_st", pred, g, ts, kp, b, r->sw, r->st);
         if (rc) return rc;
         if ((rc = launch(c, "k_fill_slots", k_fill_u64, gf, dim3(256), r->sw.hj, r->ntile_slots, HALO_EMPTY))) return rc;
-        rc = launch(c, "k_corr_st", corr, g, dim3(TW), kp, b, r->sw, r->st);
+        rc = launch(c, "k_corr_st", corr, g, ts, kp, b, r->sw, r->st);
         if (rc) return rc;
         return launch(c, "k_fill_slots", k_fill_u64, gf, dim3(256), r->sw.hj, r->ntile_slots, HALO_EMPTY);
     }
@@ -4805,7 +5101,7 @@ int ff_vcycle_graph(ff_ctx *c, int block, int nlev, int ncycles)
     for (int q = 0; q < 6; q++) same = same && c->vg_key[q] == key[q];
     if (!same) {
         vgraph_free(c);
-        if (c->lusgs_mode == FF_LUSGS_STAGED)
+        if (staged_mode(c))
             for (BlockRec *r : lv)
                 if ((rc = staged_alloc(c, r))) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -398,75 +398,108 @@ __global__ void __launch_bounds__(256) k_mlp_batch(unsigned *idx, int nb, int n,
     idx[s] = (unsigned)(h % (unsigned long long)n);
 }
 
-// Normalisation block from samples (one workgroup; fixed-order FP64 sums): means and
-// standard deviations of the transformed inputs and outputs, then the box of the
-// normalised inputs
-__global__ void __launch_bounds__(1024) k_mlp_norm(const float *x, const float *y, int n, float *nrm)
+// Normalisation block from samples: means and standard deviations of the transformed inputs
+// and outputs, then the box of the normalised inputs.  Three grid-wide passes, each a fixed
+// split of the samples over NRM_PARTS workgroups and a fixed-order tree inside each,
+// followed by a one-workgroup pass over the partials (FP64 sums: deterministic, independent
+// of the launch timing).
+constexpr int NRM_COLS = MNIN + MNOUT;
+constexpr int NRM_PARTS = 256;                        // workgroups of a partial pass
+constexpr int NRM_SCR = 2 * NRM_COLS * NRM_PARTS + NRM_COLS;
+
+__device__ __forceinline__ double nrm_value(const float *x, const float *y, int r, int col)
 {
-    __shared__ double red[1024];
-    __shared__ float out[FF_MLP_NNORM];
+    return col < MNIN ? (double)mlp_feature(x[(size_t)r * MNIN + col])
+                      : (double)mlp_target(y[(size_t)r * MNOUT + col - MNIN], col - MNIN);
+}
+
+// fixed-order tree over the workgroup's 256 values (op 0 sum, 1 min, 2 max)
+template <int OP>
+__device__ __forceinline__ double nrm_tree(double *red, double v)
+{
     const int t = threadIdx.x;
-    for (int col = 0; col < MNIN + MNOUT; col++) {
-        const float *a = col < MNIN ? x : y;
-        const int w = col < MNIN ? MNIN : MNOUT, q = col < MNIN ? col : col - MNIN;
-        double s = 0.;
-        auto val = [&](int r) -> double {
-            const float v = a[(size_t)r * w + q];
-            return col < MNIN ? mlp_feature(v) : mlp_target(v, q);
-        };
-        for (int r = t; r < n; r += 1024) s += val(r);
-        red[t] = s;
-        __syncthreads();
-        for (int h = 512; h > 0; h >>= 1) {
-            if (t < h) red[t] += red[t + h];
-            __syncthreads();
-        }
-        const double mean = red[0] / n;
-        __syncthreads();
-        double s2 = 0.;
-        for (int r = t; r < n; r += 1024) {
-            const double d = val(r) - mean;
-            s2 += d * d;
-        }
-        red[t] = s2;
-        __syncthreads();
-        for (int h = 512; h > 0; h >>= 1) {
-            if (t < h) red[t] += red[t + h];
-            __syncthreads();
-        }
-        if (t == 0) {
-            double sd = sqrt(red[0] / n);
-            if (!(sd > 0.)) sd = 1.;
-            out[col < MNIN ? MN_XM + q : MN_YM + q] = (float)mean;
-            out[col < MNIN ? MN_XS + q : MN_YS + q] = (float)sd;
-        }
+    red[t] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] = OP == 0 ? red[t] + red[t + h] : OP == 1 ? fmin(red[t], red[t + h])
+                                                                    : fmax(red[t], red[t + h]);
         __syncthreads();
     }
-    for (int q = 0; q < MNIN; q++) {
-        float lo = 3.4e38f, hi = -3.4e38f;
-        for (int r = t; r < n; r += 1024) {
-            const float v = (mlp_feature(x[(size_t)r * MNIN + q]) - out[MN_XM + q]) / out[MN_XS + q];
-            lo = fminf(lo, v);
-            hi = fmaxf(hi, v);
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// PASS 0: sums of every column; 1: squared deviations from the means (scr tail); 2: min
+// and max of the normalised inputs (nrm's means and deviations, as the network uses them)
+template <int PASS>
+__global__ void __launch_bounds__(256) k_mlp_norm_part(const float *x, const float *y, int n,
+                                                       const float *nrm, double *scr)
+{
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const int r0 = (int)((long long)n * blockIdx.x / NRM_PARTS);
+    const int r1 = (int)((long long)n * (blockIdx.x + 1) / NRM_PARTS);
+    const double *mean = scr + 2 * NRM_COLS * NRM_PARTS;
+    for (int col = 0; col < (PASS < 2 ? NRM_COLS : MNIN); col++) {
+        double a = PASS < 2 ? 0. : 1e300, z = -1e300;
+        for (int r = r0 + t; r < r1; r += 256) {
+            if (PASS == 0) {
+                a += nrm_value(x, y, r, col);
+            } else if (PASS == 1) {
+                const double d = nrm_value(x, y, r, col) - mean[col];
+                a += d * d;
+            } else {
+                const float v = (mlp_feature(x[(size_t)r * MNIN + col]) - nrm[MN_XM + col]) / nrm[MN_XS + col];
+                a = fmin(a, (double)v);
+                z = fmax(z, (double)v);
+            }
         }
-        red[t] = lo;
-        __syncthreads();
-        for (int h = 512; h > 0; h >>= 1) {
-            if (t < h) red[t] = fmin(red[t], red[t + h]);
-            __syncthreads();
+        if (PASS < 2) {
+            a = nrm_tree<0>(red, a);
+            if (t == 0) scr[col * NRM_PARTS + blockIdx.x] = a;
+        } else {
+            a = nrm_tree<1>(red, a);
+            z = nrm_tree<2>(red, z);
+            if (t == 0) {
+                scr[(2 * col) * NRM_PARTS + blockIdx.x] = a;
+                scr[(2 * col + 1) * NRM_PARTS + blockIdx.x] = z;
+            }
         }
-        if (t == 0) out[MN_LO + q] = (float)red[0];
-        __syncthreads();
-        red[t] = hi;
-        __syncthreads();
-        for (int h = 512; h > 0; h >>= 1) {
-            if (t < h) red[t] = fmax(red[t], red[t + h]);
-            __syncthreads();
-        }
-        if (t == 0) out[MN_HI + q] = (float)red[0];
-        __syncthreads();
     }
-    if (t < FF_MLP_NNORM) nrm[t] = out[t];
+}
+
+// one workgroup: the partials of a pass, in a fixed order, into the block
+template <int PASS>
+__global__ void __launch_bounds__(256) k_mlp_norm_fin(int n, float *nrm, double *scr)
+{
+    __shared__ double red[256];
+    static_assert(NRM_PARTS == 256, "one partial per thread");
+    const int t = threadIdx.x;
+    double *mean = scr + 2 * NRM_COLS * NRM_PARTS;
+    for (int col = 0; col < (PASS < 2 ? NRM_COLS : MNIN); col++) {
+        const int q = col < MNIN ? col : col - MNIN;
+        if (PASS < 2) {
+            const double sum = nrm_tree<0>(red, scr[col * NRM_PARTS + t]);
+            if (t == 0) {
+                if (PASS == 0) {
+                    mean[col] = sum / n;
+                    nrm[col < MNIN ? MN_XM + q : MN_YM + q] = (float)(sum / n);
+                } else {
+                    double sd = sqrt(sum / n);
+                    if (!(sd > 0.)) sd = 1.;
+                    nrm[col < MNIN ? MN_XS + q : MN_YS + q] = (float)sd;
+                }
+            }
+        } else {
+            const double lo = nrm_tree<1>(red, scr[(2 * col) * NRM_PARTS + t]);
+            const double hi = nrm_tree<2>(red, scr[(2 * col + 1) * NRM_PARTS + t]);
+            if (t == 0) {
+                nrm[MN_LO + q] = (float)lo;
+                nrm[MN_HI + q] = (float)hi;
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ void earsm_inputs(const EarsmT &e, float v[MNIN])
@@ -552,8 +585,8 @@ __global__ void __launch_bounds__(256) k_earsm_mlp(Blk b, const float *p, const 
 
 static void mlp_free(MlpRec &r)
 {
-    for (void *q : {(void *)r.p, (void *)r.m, (void *)r.v, (void *)r.nrm, (void *)r.part,
-                    (void *)r.idx, (void *)r.grad})
+    for (void *q : {(void *)r.p, (void *)r.m, (void *)r.v, (void *)r.nrm, (void *)r.nscr,
+                    (void *)r.part, (void *)r.idx, (void *)r.grad})
         if (q) (void)hipFree(q);
     r = MlpRec{};
 }
@@ -661,7 +694,17 @@ int ff_mlp_fit_norm(ff_ctx *c, const float *x, const float *y, int n)
     if (!c || !x || !y || n < 1) return FF_E_ARG;
     int rc = mlp_alloc(c, c->mlp);
     if (rc) return rc;
-    if ((rc = launch(c, "k_mlp_norm", k_mlp_norm, dim3(1), dim3(1024), x, y, n, c->mlp.nrm))) return rc;
+    MlpRec &m = c->mlp;
+    if (!m.nscr && hipMalloc(&m.nscr, sizeof(double) * NRM_SCR) != hipSuccess)
+        return fail(c, FF_E_NOMEM, "normalisation partials%s");
+    const dim3 gp(NRM_PARTS), t256(256);
+    if ((rc = launch(c, "k_mlp_norm", k_mlp_norm_part<0>, gp, t256, x, y, n, (const float *)m.nrm, m.nscr)) ||
+        (rc = launch(c, "k_mlp_norm", k_mlp_norm_fin<0>, dim3(1), t256, n, m.nrm, m.nscr)) ||
+        (rc = launch(c, "k_mlp_norm", k_mlp_norm_part<1>, gp, t256, x, y, n, (const float *)m.nrm, m.nscr)) ||
+        (rc = launch(c, "k_mlp_norm", k_mlp_norm_fin<1>, dim3(1), t256, n, m.nrm, m.nscr)) ||
+        (rc = launch(c, "k_mlp_norm", k_mlp_norm_part<2>, gp, t256, x, y, n, (const float *)m.nrm, m.nscr)) ||
+        (rc = launch(c, "k_mlp_norm", k_mlp_norm_fin<2>, dim3(1), t256, n, m.nrm, m.nscr)))
+        return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->mlp.ready = true;
     return FF_OK;

@@ -30,6 +30,7 @@ EXPORTS = [
 ]
 
 FF_OPT_LUSGS, FF_LUSGS_TILES, FF_LUSGS_PLANES, FF_LUSGS_KMARCH, FF_LUSGS_STAGED = 1, 0, 1, 2, 3
+FF_LUSGS_STAGED1 = 4
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 FF_OPT_EARSM, FF_EARSM_EXACT, FF_EARSM_MLP = 3, 0, 1
 FF_OPT_PRECISION, FF_PREC_FP64, FF_PREC_MIXED = 4, 0, 1
@@ -136,7 +137,8 @@ class Context:
         marching K) or "planes" (one launch per hyperplane)."""
         self.set_option(FF_OPT_LUSGS, {"tiles": FF_LUSGS_TILES, "planes": FF_LUSGS_PLANES,
                                        "kmarch": FF_LUSGS_KMARCH,
-                                       "staged": FF_LUSGS_STAGED}[mode])
+                                       "staged": FF_LUSGS_STAGED,
+                                       "staged1": FF_LUSGS_STAGED1}[mode])
 
     def precision(self, mode):
         """"fp64" (everything FP64) or "mixed" (the LU-SGS records in FP32)."""

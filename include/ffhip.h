@@ -178,9 +178,13 @@ const char *ff_last_error(const ff_ctx *ctx);
 #define FF_LUSGS_KMARCH  2
 /*   FF_LUSGS_STAGED: the (J,K) tiles of FF_LUSGS_TILES on cell records that a parallel pass
  *   forms beforehand in the sweeps' step order (coalesced, the recurrence-independent work
- *   off the wavefront); needs 2 x 56 + 8 doubles per cell and step of device memory per
- *   level (about 1 KB per cell), allocated on the block's first implicit stage. */
+ *   off the wavefront), four waves per tile (three form one face's outgoing contribution
+ *   each, one streams the records into LDS);
+ *   needs 2 x 56 + 8 doubles per cell and step of device memory per level (about 1 KB per
+ *   cell), allocated on the block's first implicit stage.  FF_LUSGS_STAGED1: the same with
+ *   one wave per tile. */
 #define FF_LUSGS_STAGED  3
+#define FF_LUSGS_STAGED1 4
 /*   FF_OPT_RESIDUAL: FF_RESIDUAL_TILES (default) forms every face flux once in (I,J) tiles
  *   marching along K; FF_RESIDUAL_CELLS forms both faces of every direction per cell.
  *   Both give the same results bit for bit. */

@@ -163,12 +163,22 @@ def test_training_and_learned_closure(hip_device):
     # the kernel's closure is the network's N put through EARSM1's formulas
     want = M.bij(M.betas_from_learned(yp, inv), terms)
     assert np.all(np.abs(got[inside] - want[inside]) <= 1e-9 * np.abs(want).max(0))
+    # the method without the network: EARSM1 with its cubic solve dropped, N at the cubic's
+    # leading-order root C1' = 9/4 (C1 - 1) (earsm.f:170-193 without the solve)
+    c1p = 9.0 * (1.8 - 1.0) / 4.0
+    const = M.bij(M.betas_from_n(np.full(len(inv), c1p), inv[:, 1], inv[:, 3]), terms)
+    err_l, err_c = [], []
     for q, f in enumerate(BIJ):
-        a, b = got[inside, q], ref[inside, q]
+        a, b, c = got[inside, q], ref[inside, q], const[inside, q]
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-        print(f"{f}: relative L2 error of the learned closure against the exact EARSM {rel:.3e}",
-              flush=True)
+        relc = np.linalg.norm(c - b) / np.linalg.norm(b)
+        err_l.append(rel)
+        err_c.append(relc)
+        print(f"{f}: relative L2 error against the exact EARSM: learned closure {rel:.3e}, "
+              f"constant N {relc:.3e}", flush=True)
         assert rel < 0.25
+    # what the trained network gains over the closure without it
+    assert np.mean(err_l) < 0.5 * np.mean(err_c), (err_l, err_c)
     ctx.earsm_mode("exact")
     ctx.close()
 

@@ -102,7 +102,7 @@ def test_coarse_level_stages(hip_device, iturb):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("nlev", [2, 3])
 @pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
 def test_vcycles(hip_device, iturb, nlev, lusgs):
@@ -213,7 +213,7 @@ def test_vcycle_graph_replays(hip_device, iturb, nlev):
     ce.vcycle(nlev, ncycles=2)
     cg.vcycle_graph(nlev, ncycles=2)
     check("two replays in one call")
-    for mode in ("tiles", "staged"):
+    for mode in ("tiles", "staged1", "staged"):
         ce.lusgs_mode(mode)
         cg.lusgs_mode(mode)
         for rep in range(2):

@@ -48,7 +48,7 @@ def test_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "planes", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "planes", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("name,dims,pover,ckw", CASES[:4], ids=[c[0] for c in CASES[:4]])
 def test_cycles(hip_device, name, dims, pover, ckw, lusgs):
     print(f"case {name} dims {dims} 5 cycles, LU-SGS {lusgs}", flush=True)
@@ -70,7 +70,7 @@ def test_cycles(hip_device, name, dims, pover, ckw, lusgs):
 MULTI_TILE = [(40, 24, 20), (20, 19, 13), (9, 17, 26), (33, 8, 16)]
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("dims", MULTI_TILE, ids=["x".join(map(str, d)) for d in MULTI_TILE])
 def test_larger_block_cycle(hip_device, dims, lusgs):
     """Blocks with more cells than one workgroup per row/plane/tile."""
@@ -89,10 +89,10 @@ def test_larger_block_cycle(hip_device, dims, lusgs):
 
 def test_implicit_stage_tiles_vs_planes(hip_device):
     """The three LU-SGS forms give the same increments bit for bit, stage by stage."""
-    print("case tiles-vs-planes-vs-kmarch 26x21x18", flush=True)
+    print("case tiles-vs-planes-vs-kmarch-vs-staged 26x21x18", flush=True)
     prm, blk = make_case((26, 21, 18), {}, dict(kind="all", amp=0.04, distort=0.1, seed=31))
     out = {}
-    for mode in ("tiles", "planes", "kmarch", "staged"):
+    for mode in ("tiles", "planes", "kmarch", "staged", "staged1"):
         g = blk.copy()
         ctx = _ctx(prm, g, mode)
         ctx.save_old(); ctx.statim(); ctx.turbdi(); ctx.residi(); ctx.impsdi()
@@ -102,6 +102,7 @@ def test_implicit_stage_tiles_vs_planes(hip_device):
     assert_fields_equal(out["tiles"], out["planes"], ["DTL"] + RES, what="tiles vs planes")
     assert_fields_equal(out["kmarch"], out["planes"], ["DTL"] + RES, what="kmarch vs planes")
     assert_fields_equal(out["staged"], out["planes"], ["DTL"] + RES, what="staged vs planes")
+    assert_fields_equal(out["staged1"], out["planes"], ["DTL"] + RES, what="staged1 vs planes")
 
 
 @pytest.mark.parametrize("dims", [(70, 9, 17), (5, 3, 2), (130, 6, 9)], ids=["70x9x17", "5x3x2", "130x6x9"])
@@ -162,7 +163,7 @@ def test_sst_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("name,dims,pover,ckw", SST_CASES[:2], ids=[c[0] for c in SST_CASES[:2]])
 def test_sst_cycles(hip_device, name, dims, pover, ckw, lusgs):
     print(f"case {name} dims {dims} 5 SST cycles, LU-SGS {lusgs}", flush=True)
@@ -179,7 +180,7 @@ def test_sst_cycles(hip_device, name, dims, pover, ckw, lusgs):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("dims", [(40, 24, 20), (21, 19, 13)], ids=["40x24x20", "21x19x13"])
 def test_sst_multi_tile(hip_device, dims, lusgs):
     """SST blocks spanning several LU-SGS tiles and residual tiles, with ragged last tiles."""
@@ -223,7 +224,7 @@ def test_earsm_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("name,dims,pover,ckw", EARSM_CASES[:2], ids=[c[0] for c in EARSM_CASES[:2]])
 def test_earsm_cycles(hip_device, name, dims, pover, ckw, lusgs):
     print(f"case {name} dims {dims} 5 EARSM cycles, LU-SGS {lusgs}", flush=True)
@@ -271,7 +272,7 @@ def test_fullns_stages(hip_device, name, dims, pover, ckw):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("name,dims,pover,ckw", [FULLNS_CASES[i] for i in (0, 2, 3, 5)],
                          ids=[FULLNS_CASES[i][0] for i in (0, 2, 3, 5)])
 def test_fullns_cycles(hip_device, name, dims, pover, ckw, lusgs):
@@ -355,7 +356,7 @@ def test_walls_stages(hip_device, name, dims, pover, ckw, specs, grav):
     ctx.close()
 
 
-@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged"])
+@pytest.mark.parametrize("lusgs", ["tiles", "kmarch", "staged", "staged1"])
 @pytest.mark.parametrize("name,dims,pover,ckw,specs,grav", [WALL_CASES[i] for i in (1, 2)],
                          ids=[WALL_CASES[i][0] for i in (1, 2)])
 def test_walls_cycles(hip_device, name, dims, pover, ckw, specs, grav, lusgs):
