@@ -2624,36 +2624,46 @@ __device__ __forceinline__ bool half_empty(unsigned long long x)
 // neighbour's output of the previous step (words TW apart), the upstream tile's halo block
 // (words TK apart) or a zero block at the block edge (words TK apart).  One address and
 // stride per lane replace a branch per source.  marker_hit reads the words and tests them;
-// halo_poll polls a halo word copied while still empty (either 32-bit half equal to the
-// marker's) from its slot, in a branch that is rarely taken.  Marker halves in words from
-// the other two sources are ordinary values and are not polled.  The staged sweeps leave their slots
-// filled: a fill kernel empties all of them after the sweep, which costs less than a
-// device-coherent store per word on the wavefront's critical path.
+// halo_poll polls the halo words copied while still empty (either 32-bit half equal to the
+// marker's) from their slot, in a branch that is rarely taken: every round reloads all the
+// words still missing at once, so a lane that has caught up with its upstream tile pays one
+// memory round trip per step, not one per word (the producer stores the words together).
+// Marker halves in words from the other two sources are ordinary values and are not
+// polled.  The staged sweeps leave their slots filled: a fill kernel empties all of them
+// after the sweep, which costs less than a device-coherent store per word on the
+// wavefront's critical path.
 template <int NV>
 __device__ __forceinline__ void halo_poll(unsigned long long *slot, unsigned int *err,
                                           unsigned long long h[NV])
 {
+    unsigned need = 0;                   // bit q: word q still to be read from its slot
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        if (!half_empty(h[q])) continue;
-        unsigned long long x = HALO_EMPTY;
-        int spins = 0;
-        while (x == HALO_EMPTY) {
-            if ((++spins & 255) == 0) {
-                if (spins > SPIN_LIMIT) {
-                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    x = 0;
-                    break;
-                }
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    x = 0;
-                    break;
-                }
-            }
-            x = __hip_atomic_load(slot + q * HQS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (x == HALO_EMPTY) __builtin_amdgcn_s_sleep(1);
+    for (int q = 0; q < NV; q++)
+        if (half_empty(h[q])) need |= 1u << q;
+    int spins = 0;
+    while (need) {
+        if ((++spins & 255) == 0 &&
+            (spins > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            if (spins > SPIN_LIMIT) __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                if (need >> q & 1u) h[q] = 0;
+            break;
         }
-        h[q] = x;
+        unsigned long long x[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+            x[q] = (need >> q & 1u) ? __hip_atomic_load(slot + q * HQS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : 0ull;
+        unsigned left = 0;
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            if (!(need >> q & 1u)) continue;
+            if (x[q] == HALO_EMPTY) left |= 1u << q;
+            else h[q] = x[q];
+        }
+        need = left;
+        if (need) __builtin_amdgcn_s_sleep(1);
     }
 }
 
