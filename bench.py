@@ -248,12 +248,19 @@ def committed_traffic(workload, kernel):
                         f"hbm_traffic_{tag}_{IMAX}x{JMAX}x{KMAX}.md")
     if not os.path.exists(path):
         return None, None
-    for line in open(path):
-        if line.startswith("| `"):
-            cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            name = cells[0].strip("`")
-            if name == kernel or name.startswith(kernel + "<"):
-                return float(cells[4]) * 1e6, os.path.relpath(path, os.path.dirname(path) + "/..")
+    src = os.path.relpath(path, os.path.dirname(path) + "/..")
+    lines = [ln for ln in open(path) if ln.startswith("| `")]
+    # the launch-label rows (label, launches, traffic MB/launch, ...): every launch of the
+    # label on every level, as the roofline averages them
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells[0].strip("`") == kernel:
+            return float(cells[2]) * 1e6, src
+    # per-kernel rows (kernel, launches, FETCH_SIZE, WRITE_SIZE, traffic MB/launch)
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells[0].strip("`").startswith(kernel + "<"):
+            return float(cells[4]) * 1e6, src
     return None, None
 
 
@@ -287,6 +294,11 @@ def multiblock_bench(args):
     from finflo_amd.ffhip import Context
     t0 = time.perf_counter()
     five = args.workload == "config5"
+    if five:
+        # the closure's sample arrays are torch device tensors: bring up torch's device
+        # context before the library's (it reported no device when started after it)
+        import torch
+        torch.zeros(1, device="cuda")
     prm, blocks, cons, walls, bcs = (config5 if five else config3)()
     nlev = len(blocks[0])
     nb = len(blocks)

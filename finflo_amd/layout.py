@@ -50,7 +50,7 @@ class Params(C.Structure):
         ("iturb", C.c_int), ("iflux", C.c_int), ("idiff", C.c_int),
         ("inter", C.c_int * 3), ("kover", C.c_int), ("katol", C.c_int),
         ("istres", C.c_int), ("fullns", C.c_int), ("turcor", C.c_int),
-        ("reserved_i", C.c_int * 5)]
+        ("ddadi", C.c_int), ("reserved_i", C.c_int * 4)]
 
     DOUBLES = ("gamma", "rgas", "pr", "prt", "visu0", "expsu", "tsu0",
                "frspre", "frsden", "frstem", "frsvel", "frsvis",

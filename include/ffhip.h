@@ -99,7 +99,10 @@ typedef struct ff_params {
                         FLKEBL with TEMDER), 0 = the thin-layer FLUXDO/FLUDKE */
     int turcor;      /* TURCOR (main.f:5257, default YES): the near-wall blending and flux
                         corrections of BOTSKE with the SST; the laminar FLUCYC passes .FALSE. */
-    int reserved_i[5];
+    int ddadi;       /* 1: the implicit stage is DDADI (BLKS%LUSGS = 0, imps3.f:321-373): a K,
+                        an I and a J sweep of PRED/DIAG/CORR(KE); 0 (default): LU-SGS
+                        (LUSGS = 1, PREDGS/DIAGGS/CORRGS) */
+    int reserved_i[4];
 } ff_params;
 
 /* ---- solid-wall patches (BOTSKE base3c.f:6147-7095) ---------------------------------- */

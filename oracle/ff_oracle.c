@@ -2215,7 +2215,296 @@ static void imsoko(const ff_params *p, ff_block *b)
             }
 }
 
-/* IMPS imps3.f:6-572 with LUSGS = 1, CFL >= .05, OMEGA = 0.  Laminar (ITURB <= 2): LOCDT,
+/* ------------------------------------------------------------------------------------ */
+/* DDADI (LUSGS = 0): IMPS imps3.f:321-373 -> IMPSWE imps3.f:578-778 per direction         */
+/* ------------------------------------------------------------------------------------ */
+
+/* SYIKE imps4.f:982-1035: primitive -> characteristic variables of the k-omega form (its
+ * DRH and DRP are formed but unused there) */
+static void syike(double w[7], const double x[7], double c, double ro, double a2x,
+                  double a2y, double a2z, double *a21, double *a22, double *a23, double *a31,
+                  double *a32, double *a33)
+{
+    double z1 = x[0], z5 = x[4], z6 = x[5], z7 = x[6];
+    surfvects(a2x, a2y, a2z, a21, a22, a23, a31, a32, a33);
+    double z2 = a2x * x[1] + a2y * x[2] + a2z * x[3];
+    double z3 = *a21 * x[1] + *a22 * x[2] + *a23 * x[3];
+    double z4 = *a31 * x[1] + *a32 * x[2] + *a33 * x[3];
+    double roc = ro * c;
+    double pc2 = 1. / ro;
+    double dpres = z5;
+    w[0] = z1 - pc2 * dpres;
+    w[1] = roc * z2 + dpres;
+    w[2] = z3;
+    w[3] = z4;
+    w[5] = z6;
+    w[6] = z7;
+    w[4] = -roc * z2 + dpres;
+}
+
+/* SYM1KE imps4.f:1041-1072: characteristic -> primitive variables of the k-omega form */
+static void sym1ke(double x[7], const double w[7], double rk, double c, double ro,
+                   double drdp, double drdh, double a2x, double a2y, double a2z, double a21,
+                   double a22, double a23, double a31, double a32, double a33)
+{
+    double ypro = 1. / ro;
+    double pc2 = .5 * ypro;
+    double proc = pc2 / c;
+    double drh = .667 * rk * (ypro * ypro) * drdh;
+    double ydrp = 1. / (1. + .667 * rk * ypro * drdp);
+    double dh = w[0] + pc2 * (w[1] + w[4]);
+    x[0] = dh;
+    double x2i = proc * (w[1] - w[4]);
+    double x3i = w[2], x4i = w[3];
+    double x5 = .5 * (w[1] + w[4]);
+    x[4] = ydrp * (x5 - drh * ro * dh - .667 * ro * w[5]);
+    x[5] = w[5];
+    x[6] = w[6];
+    x[1] = a2x * x2i + a21 * x3i + a31 * x4i;
+    x[2] = a2y * x2i + a22 * x3i + a32 * x4i;
+    x[3] = a2z * x2i + a23 * x3i + a33 * x4i;
+}
+
+/* One DDADI direction.  The reference sorts its arrays so that the swept direction runs
+ * slowest (IMPS imps3.f:345, 364 with SORT38/SORTZZ, and IMPSWE's re-sort, multi.f:2317-
+ * 2527): the K sweep in IJK order, the I sweep in JKI order, the J sweep in KIJ order.
+ * Here the IJK arrays are addressed through strides instead: the fast sorted index a runs
+ * with its two ghost layers, the middle index b over its interior, the swept index m.
+ * PRED/DIAG/CORR loop over the plane entries f = IN .. NB*(NA+4)-1 of (a, b) (DO I = IN+1,
+ * JMAX*ISTRID; ghost columns of a included, as in the reference), every 2*IN+1-th with
+ * NA = 1.  Entry (f, m) is cell a*sa + (b+2)*sb + (m+1)*sm. */
+typedef struct {
+    long sa, sb, sm;
+    int na, nb, nm;
+    const double *A, *AX, *AY, *AZ;     /* the faces of the swept direction */
+} ddir_t;
+
+static ddir_t ddadi_dir(const ff_block *b, int dir)
+{
+    dims_t d = mkdims(b);
+    ddir_t r;
+    if (dir == 0) {         /* K sweep, IJK */
+        r.sa = 1; r.na = d.imax; r.sb = d.is; r.nb = d.jmax; r.sm = d.il; r.nm = d.kmax;
+        r.A = F(b, FF_A3); r.AX = F(b, FF_A3X); r.AY = F(b, FF_A3Y); r.AZ = F(b, FF_A3Z);
+    } else if (dir == 1) {  /* I sweep, JKI */
+        r.sa = d.is; r.na = d.jmax; r.sb = d.il; r.nb = d.kmax; r.sm = 1; r.nm = d.imax;
+        r.A = F(b, FF_A1); r.AX = F(b, FF_A1X); r.AY = F(b, FF_A1Y); r.AZ = F(b, FF_A1Z);
+    } else {                /* J sweep, KIJ */
+        r.sa = d.il; r.na = d.kmax; r.sb = 1; r.nb = d.imax; r.sm = d.is; r.nm = d.jmax;
+        r.A = F(b, FF_A2); r.AX = F(b, FF_A2X); r.AY = F(b, FF_A2Y); r.AZ = F(b, FF_A2Z);
+    }
+    return r;
+}
+
+static inline long dd_cell(const ddir_t *r, long f, int m)
+{
+    long ist = r->na + 2 * G;
+    return (f % ist) * r->sa + (f / ist + G) * r->sb + (long)(m + G - 1) * r->sm;
+}
+
+/* PRED imps4.f:788-905 (PREDKE imps4.f:2084-2229): the backward sweep along m, the flux
+ * of the upper neighbour entering through the upper face; IDI = all cells viscous
+ * (CDIFF = 2), UROT = 0 */
+static void dd_pred(const ff_params *p, ff_block *b, const ddir_t *r, int ke)
+{
+    const int nv = ke ? 7 : 5;
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
+    const double *dtl = F(b, FF_DTL), *vol = F(b, FF_VOL), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *c = F(b, FF_C), *ro = F(b, FF_RO), *vis = F(b, FF_VIS),
+                 *vist = F(b, FF_VIST), *rk = F(b, FF_RK), *drdp = F(b, FF_DRDP),
+                 *drdh = F(b, FF_DRDH);
+    const long ist = r->na + 2 * G, nf = (long)r->nb * ist;
+    const long step = r->na == 1 ? 2 * G + 1 : 1;
+    const double cdiff = 2., dturb = 1., vrel = 1.E-21;
+    for (long f = G; f < nf; f += step) {
+        double bd[7] = {0., 0., 0., 0., 0., 0., 0.};
+        for (int m = r->nm; m >= 1; m--) {
+            long n = dd_cell(r, f, m), nu = n + r->sm, nl = n - r->sm;
+            double ff = m == 1 ? .25 : .5;
+            double dt1 = p->theta * dtl[n];
+            double x[7], y[7], a21, a22, a23, a31, a32, a33;
+            for (int q = 0; q < 5; q++) x[q] = dt1 * r->A[nu] * bd[q] + vol[n] * D[q][n];
+            if (ke)
+                for (int q = 5; q < 7; q++) x[q] = dturb * dt1 * r->A[nu] * bd[q] + vol[n] * D[q][n];
+            double dt = dt1;
+            if (ke)
+                syike(y, x, c[n], ro[n], r->AX[n], r->AY[n], r->AZ[n], &a21, &a22, &a23, &a31,
+                      &a32, &a33);
+            else
+                syi(&y[0], &y[1], &y[2], &y[3], &y[4], x[0], x[1], x[2], x[3], x[4], c[n], ro[n],
+                    r->AX[n], r->AY[n], r->AZ[n], &a21, &a22, &a23, &a31, &a32, &a33);
+            double rl1 = r->AX[n] * u[n] + r->AY[n] * v[n] + r->AZ[n] * w[n] - 0.;
+            double rl2 = rl1 + c[n], rl5 = rl1 - c[n];
+            double rnum = ff * (vol[n] + vol[nl]) * (ro[n] + ro[nl]);
+            double t = cdiff * r->A[n] / rnum * (vist[n] + vis[n] + vist[nl] + vis[nl]);
+            double x1 = dmax(-rl1, 0.) + t, x2 = dmax(-rl2, 0.) + t, x5 = dmax(-rl5, 0.) + t;
+            double x1a = fabs(rl1) + t, x2a = fabs(rl2) + t, x5a = fabs(rl5) + t;
+            double xa[5] = {x1a, x2a, x1a, x1a, x5a};
+            if (ke) {
+                for (int q = 0; q < 5; q++) y[q] = y[q] / (vol[n] + vrel + dt * xa[q] * r->A[n]);
+                for (int q = 5; q < 7; q++)
+                    y[q] = y[q] / (vol[n] + vrel + dturb * dt * (x1a * r->A[n] + 0.));
+                double o[7];
+                sym1ke(o, y, rk[n], c[n], ro[n], drdp[n], drdh[n], r->AX[n], r->AY[n], r->AZ[n],
+                       a21, a22, a23, a31, a32, a33);
+                for (int q = 0; q < 7; q++) D[q][n] = o[q];
+            } else {
+                for (int q = 0; q < 5; q++) y[q] = y[q] / (vol[n] + dt * xa[q] * r->A[n]);
+                sym1i(&D[0][n], &D[1][n], &D[2][n], &D[3][n], &D[4][n], y[0], y[1], y[2], y[3],
+                      y[4], c[n], ro[n], r->AX[n], r->AY[n], r->AZ[n], a21, a22, a23, a31, a32, a33);
+            }
+            double xs[7] = {x1, x2, x1, x1, x5, x1, x1};
+            for (int q = 0; q < nv; q++) y[q] = xs[q] * y[q];
+            if (ke)
+                sym1ke(bd, y, rk[n], c[n], ro[n], drdp[n], drdh[n], r->AX[n], r->AY[n], r->AZ[n],
+                       a21, a22, a23, a31, a32, a33);
+            else
+                sym1i(&bd[0], &bd[1], &bd[2], &bd[3], &bd[4], y[0], y[1], y[2], y[3], y[4], c[n],
+                      ro[n], r->AX[n], r->AY[n], r->AZ[n], a21, a22, a23, a31, a32, a33);
+        }
+    }
+}
+
+/* DIAG imps4.f:911-971 (DIAGKE imps4.f:2235-2309): the diagonal of DD-ADI, pointwise */
+static void dd_diag(const ff_params *p, ff_block *b, const ddir_t *r, int ke)
+{
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
+    const double *dtl = F(b, FF_DTL), *vol = F(b, FF_VOL), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *c = F(b, FF_C), *ro = F(b, FF_RO), *rk = F(b, FF_RK),
+                 *drdp = F(b, FF_DRDP), *drdh = F(b, FF_DRDH);
+    const long ist = r->na + 2 * G, nf = (long)r->nb * ist;
+    const long step = r->na == 1 ? 2 * G + 1 : 1;
+    const double dturb = 1.;
+    for (int m = 1; m <= r->nm; m++)
+        for (long f = G; f < nf; f += step) {
+            long n = dd_cell(r, f, m), nu = n + r->sm;
+            double dtx = p->theta * dtl[n];
+            double x[7], y[7], a21, a22, a23, a31, a32, a33;
+            for (int q = 0; q < 7; q++) x[q] = q < 5 || ke ? D[q][n] : 0.;
+            if (ke)
+                syike(y, x, c[n], ro[n], r->AX[nu], r->AY[nu], r->AZ[nu], &a21, &a22, &a23, &a31,
+                      &a32, &a33);
+            else
+                syi(&y[0], &y[1], &y[2], &y[3], &y[4], x[0], x[1], x[2], x[3], x[4], c[n], ro[n],
+                    r->AX[nu], r->AY[nu], r->AZ[nu], &a21, &a22, &a23, &a31, &a32, &a33);
+            double ucon = r->AX[nu] * u[n] + r->AY[nu] * v[n] + r->AZ[nu] * w[n] - 0.;
+            double rl2n = dmax(-(ucon + c[n]), 0.), rl5n = dmax(-(ucon - c[n]), 0.);
+            double rl1n = dmax(-ucon, 0.);
+            ucon = r->AX[n] * u[n] + r->AY[n] * v[n] + r->AZ[n] * w[n] - 0.;
+            double rl2p = dmax(ucon + c[n], 0.), rl5p = dmax(ucon - c[n], 0.);
+            double rl1p = dmax(ucon, 0.);
+            double dt1 = ke ? dtx / (vol[n] + 1.E-21) : dtx / vol[n];
+            double s1 = r->A[n] * rl1p + r->A[nu] * rl1n, s2 = r->A[n] * rl2p + r->A[nu] * rl2n;
+            double s5 = r->A[n] * rl5p + r->A[nu] * rl5n;
+            y[0] = y[0] * (1. + dt1 * s1);
+            y[1] = y[1] * (1. + dt1 * s2);
+            y[2] = y[2] * (1. + dt1 * s1);
+            y[3] = y[3] * (1. + dt1 * s1);
+            y[4] = y[4] * (1. + dt1 * s5);
+            if (ke) {
+                y[5] = y[5] * (1. + dturb * dt1 * (s1 + 0.));
+                y[6] = y[6] * (1. + dturb * dt1 * (s1 + 0.));
+                double o[7];
+                sym1ke(o, y, rk[n], c[n], ro[n], drdp[n], drdh[n], r->AX[nu], r->AY[nu], r->AZ[nu],
+                       a21, a22, a23, a31, a32, a33);
+                for (int q = 0; q < 7; q++) D[q][n] = o[q];
+            } else {
+                sym1i(&D[0][n], &D[1][n], &D[2][n], &D[3][n], &D[4][n], y[0], y[1], y[2], y[3],
+                      y[4], c[n], ro[n], r->AX[nu], r->AY[nu], r->AZ[nu], a21, a22, a23, a31, a32,
+                      a33);
+            }
+        }
+}
+
+/* CORR imps4.f:671-787 (CORRKE imps4.f:1939-2078): the forward sweep along m, the flux of
+ * the lower neighbour entering through the lower face */
+static void dd_corr(const ff_params *p, ff_block *b, const ddir_t *r, int ke)
+{
+    const int nv = ke ? 7 : 5;
+    double *D[7] = {F(b, FF_DRO), F(b, FF_DM), F(b, FF_DN), F(b, FF_DW), F(b, FF_DE),
+                    F(b, FF_DRK), F(b, FF_DEPS)};
+    const double *dtl = F(b, FF_DTL), *vol = F(b, FF_VOL), *u = F(b, FF_U), *v = F(b, FF_V),
+                 *w = F(b, FF_W), *c = F(b, FF_C), *ro = F(b, FF_RO), *vis = F(b, FF_VIS),
+                 *vist = F(b, FF_VIST), *rk = F(b, FF_RK), *drdp = F(b, FF_DRDP),
+                 *drdh = F(b, FF_DRDH);
+    const long ist = r->na + 2 * G, nf = (long)r->nb * ist;
+    const long step = r->na == 1 ? 2 * G + 1 : 1;
+    const double cdiff = 2., dturb = 1., vrel = 1.E-21;
+    for (long f = G; f < nf; f += step) {
+        double bd[7] = {0., 0., 0., 0., 0., 0., 0.};
+        for (int m = 1; m <= r->nm; m++) {
+            long n = dd_cell(r, f, m), nu = n + r->sm;
+            double dt1 = p->theta * dtl[n];
+            double x[7], y[7], a21, a22, a23, a31, a32, a33;
+            for (int q = 0; q < 5; q++) x[q] = dt1 * r->A[n] * bd[q] + vol[n] * D[q][n];
+            if (ke)
+                for (int q = 5; q < 7; q++) x[q] = dturb * dt1 * r->A[n] * bd[q] + vol[n] * D[q][n];
+            double dt = dt1;
+            if (ke)
+                syike(y, x, c[n], ro[n], r->AX[nu], r->AY[nu], r->AZ[nu], &a21, &a22, &a23, &a31,
+                      &a32, &a33);
+            else
+                syi(&y[0], &y[1], &y[2], &y[3], &y[4], x[0], x[1], x[2], x[3], x[4], c[n], ro[n],
+                    r->AX[nu], r->AY[nu], r->AZ[nu], &a21, &a22, &a23, &a31, &a32, &a33);
+            double rl1 = r->AX[nu] * u[n] + r->AY[nu] * v[n] + r->AZ[nu] * w[n] - 0.;
+            double rl2 = rl1 + c[n], rl5 = rl1 - c[n];
+            double rnum = .5 * (vol[n] + vol[nu]) * (ro[n] + ro[nu]);
+            double t = cdiff * r->A[nu] / rnum * (vist[n] + vis[n] + vist[nu] + vis[nu]);
+            double x1 = dmax(rl1, 0.) + t, x2 = dmax(rl2, 0.) + t, x5 = dmax(rl5, 0.) + t;
+            double x1a = fabs(rl1) + t, x2a = fabs(rl2) + t, x5a = fabs(rl5) + t;
+            double xa[5] = {x1a, x2a, x1a, x1a, x5a};
+            if (ke) {
+                for (int q = 0; q < 5; q++) y[q] = y[q] / (vol[n] + vrel + dt * xa[q] * r->A[nu]);
+                for (int q = 5; q < 7; q++)
+                    y[q] = y[q] / (vol[n] + vrel + dturb * dt * (x1a * r->A[nu] + 0.));
+                double o[7];
+                sym1ke(o, y, rk[n], c[n], ro[n], drdp[n], drdh[n], r->AX[nu], r->AY[nu], r->AZ[nu],
+                       a21, a22, a23, a31, a32, a33);
+                for (int q = 0; q < 7; q++) D[q][n] = o[q];
+            } else {
+                for (int q = 0; q < 5; q++) y[q] = y[q] / (vol[n] + dt * xa[q] * r->A[nu]);
+                sym1i(&D[0][n], &D[1][n], &D[2][n], &D[3][n], &D[4][n], y[0], y[1], y[2], y[3],
+                      y[4], c[n], ro[n], r->AX[nu], r->AY[nu], r->AZ[nu], a21, a22, a23, a31, a32,
+                      a33);
+            }
+            double xs[7] = {x1, x2, x1, x1, x5, x1, x1};
+            for (int q = 0; q < nv; q++) y[q] = xs[q] * y[q];
+            if (ke)
+                sym1ke(bd, y, rk[n], c[n], ro[n], drdp[n], drdh[n], r->AX[nu], r->AY[nu], r->AZ[nu],
+                       a21, a22, a23, a31, a32, a33);
+            else
+                sym1i(&bd[0], &bd[1], &bd[2], &bd[3], &bd[4], y[0], y[1], y[2], y[3], y[4], c[n],
+                      ro[n], r->AX[nu], r->AY[nu], r->AZ[nu], a21, a22, a23, a31, a32, a33);
+        }
+    }
+}
+
+/* The three DDADI directions in IMPS's order: K, I, J (imps3.f:321-373), each PRED, DIAG,
+ * CORR (IMPSWE imps3.f:604-617 / 618-637) */
+static void ddadi(const ff_params *p, ff_block *b, int ke)
+{
+    for (int dir = 0; dir < 3; dir++) {
+        ddir_t r = ddadi_dir(b, dir);
+        dd_pred(p, b, &r, ke);
+        dd_diag(p, b, &r, ke);
+        dd_corr(p, b, &r, ke);
+    }
+}
+
+/* One DDADI direction alone (0 K, 1 I, 2 J), for the pins */
+void ffo_ddadi_dir(const ff_params *p, ff_block *b, int dir, int phase)
+{
+    ddir_t r = ddadi_dir(b, dir);
+    int ke = p->iturb == 6;
+    if (phase == 0) dd_pred(p, b, &r, ke);
+    else if (phase == 1) dd_diag(p, b, &r, ke);
+    else dd_corr(p, b, &r, ke);
+}
+
+/* IMPS imps3.f:6-572 with LUSGS = 1 (or 0 with p->ddadi: the DDADI sweeps above in place of
+ * PREDGS/DIAGGS/CORRGS), CFL >= .05, OMEGA = 0.  Laminar (ITURB <= 2): LOCDT,
  * COPR, PREDGS/DIAGGS/CORRGS.  SST (ITURB = 6): LOCDTK (DTURB = 1, imps3.f:61), IMSOKO,
  * COPRKE (imps3.f:1113-1211), PRKEGS/DIKEGS/COKEGS, then D(rho k), D(rho omega) times RO
  * (imps3.f:497-500).  Both end with dT = dh/CP (imps3.f:483). */
@@ -2287,9 +2576,13 @@ void ffo_implicit(const ff_params *p, ff_block *b, int level)
                     D[4][n] = dpi + .667 * dki;
                 }
             }
-    predgs(p, b, ke);
-    diaggs(p, b, ke);
-    corrgs(p, b, ke);
+    if (p->ddadi) {
+        ddadi(p, b, ke);
+    } else {
+        predgs(p, b, ke);
+        diaggs(p, b, ke);
+        corrgs(p, b, ke);
+    }
     for (int k = 1; k <= d.kmax; k++)
         for (int j = 1; j <= d.jmax; j++)
             for (int i = 1; i <= d.imax; i++) {

@@ -66,6 +66,8 @@ def oracle():
         _ora.ffo_statim.restype = None
         _ora.ffo_ghostext.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double]
         _ora.ffo_ghostext.restype = None
+        _ora.ffo_ddadi_dir.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        _ora.ffo_ddadi_dir.restype = None
     return _ora
 
 
@@ -92,6 +94,9 @@ def ref():
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_int]
         _ref.ref_reflec.restype = None
+        _ref.ref_ddadi_dir.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_int]
+        _ref.ref_ddadi_dir.restype = None
     return _ref
 
 
@@ -134,6 +139,11 @@ class OracleBlock:
 
     def implicit(self):
         oracle().ffo_implicit(*self._args(), self.level)
+
+    def ddadi_dir(self, direction, phase):
+        """One DDADI direction (0 K, 1 I, 2 J) and phase (0 PRED, 1 DIAG, 2 CORR) on the
+        current increments (DTL given)."""
+        oracle().ffo_ddadi_dir(*self._args(), direction, phase)
 
     def save_old(self):
         oracle().ffo_save_old(*self._args())
@@ -225,7 +235,7 @@ class RefBlock:
         inter = list(prm.inter) if level == 1 else [55, 55, 55]
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
                             inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1,
-                            prm.istres, prm.fullns, prm.turcor],
+                            prm.istres, prm.fullns, prm.turcor, prm.ddadi],
                            dtype=np.int32)
         # the reference driver keeps wall patches per level: a new block starts without any
         if have_ref():
@@ -256,6 +266,10 @@ class RefBlock:
 
     def implicit(self):
         ref().ref_implicit(*self._a())
+
+    def ddadi_dir(self, direction, phase):
+        """The reference's PRED/DIAG/CORR(KE) of one DDADI direction (phase 3: all three)."""
+        ref().ref_ddadi_dir(*self._a(), direction, phase)
 
     def save_old(self):
         b = self.blk

@@ -18,7 +18,12 @@
 !                 replaces FLUDKE (res3c.f:6802-6848, 7839-7846)
 !   ref_implicit  IMPS imps3.f:111-483 with LUSGS=1: TIME3, LOCDT, COPR, PREDGS, DIAGGS,
 !                 CORRGS, DIVV12; ITURB=6: LOCDTK, IMSOKO, COPRKE, PRKEGS, DIKEGS,
-!                 COKEGS, DIVV12, MULV12
+!                 COKEGS, DIVV12, MULV12.  With LUSGS=0 (IPR(QDDADI) /= 0) the DDADI
+!                 sweeps of imps3.f:321-373 in place of the LU-SGS: IMPSWE's PRED, DIAG,
+!                 CORR (PREDKE, DIAGKE, CORRKE) along K, I and J.  The reference sorts its
+!                 arrays for the I and J sweeps with SORT38/SORTZZ of multi.f, which does
+!                 not build here; ref_ddadi_dir makes the same JKI and KIJ orders with its
+!                 own transposed copies (dd_sort) and copies the increments back
 !   ref_update    CYCLES ns3c.f:1517-1596: FIN, UPLIM/LOLIM, PDTOP, LOLIMP
 !
 ! All fields of a block are packed as F(NTOT, NF) in the order of enum ff_field
@@ -49,7 +54,7 @@ module ffref_ids
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
   integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12, QM2=13
-  integer, parameter :: QISTRES=14, QFULLNS=15, QTURCOR=16
+  integer, parameter :: QISTRES=14, QFULLNS=15, QTURCOR=16, QDDADI=17
 end module ffref_ids
 
 ! Solid-wall patches per level, set by ref_wall_set: ICON(3..7,L) and ICON(20,L), the
@@ -693,6 +698,11 @@ subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
           f(1,IRK), f(1,IREPS), f(1,IDRDP), f(1,IDRDH), prm(PFRSVEL), 'FLUID     ', pro, var, &
           dxi, f(1,IDTL), ntot, .false., gii, reti, dgi, dreti, pdfor, rksi, f(1,IVOL), 1, &
           imax, jmax, kmax)
+     if (ipr(QDDADI) /= 0) then
+        call ddadi_dir(f, ntot, prm, ipr, 0, 3)
+        call ddadi_dir(f, ntot, prm, ipr, 1, 3)
+        call ddadi_dir(f, ntot, prm, ipr, 2, 3)
+     else
      call PRKEGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
           f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
           f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
@@ -713,6 +723,7 @@ subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
           idm1, idm2, idm3, f(1,IVIS), f(1,IVIST), f(1,IDRK), f(1,IDEPS), f(1,IRK), f(1,IREPS), &
           f(1,ISRK), f(1,ISEPS), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, &
           maw, maxw, .false., dgi, dreti)
+     end if
      call DIVV12(f(1,IDRO), f(1,IDRO), f(1,ICP), ntot)
      call MULV12(f(1,IDRK), f(1,IRO), ntot)
      call MULV12(f(1,IDEPS), f(1,IRO), ntot)
@@ -723,6 +734,11 @@ subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
      call COPR(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IRO), f(1,IE), f(1,IP), &
           f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), f(1,IDRDP), f(1,IDRDH), &
           prm(PFRSVEL), ntot)
+     if (ipr(QDDADI) /= 0) then
+        call ddadi_dir(f, ntot, prm, ipr, 0, 3)
+        call ddadi_dir(f, ntot, prm, ipr, 1, 3)
+        call ddadi_dir(f, ntot, prm, ipr, 2, 3)
+     else
      ! LU-SGS (imps3.f:383-391)
      call PREDGS(f(1,IDRO), f(1,IDM), f(1,IDN), f(1,IDW), f(1,IDE), f(1,IVOL), f(1,IA1), &
           f(1,IA1X), f(1,IA1Y), f(1,IA1Z), f(1,IA2), f(1,IA2X), f(1,IA2Y), f(1,IA2Z), f(1,IA3), &
@@ -739,11 +755,143 @@ subroutine ref_implicit(f, ntot, prm, ipr) bind(C, name="ref_implicit")
           f(1,IA3X), f(1,IA3Y), f(1,IA3Z), f(1,IDTL), f(1,IU), f(1,IV), f(1,IW), f(1,IC), &
           f(1,IRO), f(1,IP), imax, jmax, kmax, theta, 0._c_double, idm1, idm2, idm3, f(1,IVIS), &
           f(1,IVIST), prm(PPR), prm(PPRT), 2, 2, 2, zero, zero, zero, rksi, zzz, maw, maxw)
+     end if
      ! dh -> dT (imps3.f:483)
      call DIVV12(f(1,IDRO), f(1,IDRO), f(1,ICP), ntot)
   end if
   deallocate(zzz, rksi, zero, abv)
 end subroutine ref_implicit
+
+! The sorted copy (dir 1: JKI, dir 2: KIJ) of one IJK field of an (imax+4, jmax+4, kmax+4)
+! block, or (back = .true.) the IJK field from its sorted copy
+subroutine dd_sort(x, xs, imax, jmax, kmax, dir, back)
+  use iso_c_binding
+  implicit none
+  integer :: imax, jmax, kmax, dir
+  logical :: back
+  real(c_double) :: x(imax+4, jmax+4, kmax+4)
+  real(c_double) :: xs(*)
+  integer :: i, j, k, n, ni, nj, nk
+  ni = imax + 4; nj = jmax + 4; nk = kmax + 4
+  do k = 1, nk
+     do j = 1, nj
+        do i = 1, ni
+           if (dir == 1) then
+              n = j + (k - 1)*nj + (i - 1)*nj*nk
+           else
+              n = k + (i - 1)*nk + (j - 1)*nk*ni
+           end if
+           if (back) then
+              x(i, j, k) = xs(n)
+           else
+              xs(n) = x(i, j, k)
+           end if
+        end do
+     end do
+  end do
+end subroutine dd_sort
+
+! One DDADI direction of IMPS (imps3.f:321-373): dir 0 the K sweep on the IJK arrays, 1 the
+! I sweep on JKI copies, 2 the J sweep on KIJ copies; phase 0 PRED, 1 DIAG, 2 CORR, 3 all
+! three (IMPSWE imps3.f:604-637; the k-omega PREDKE/DIAGKE/CORRKE with DTURB = 1)
+subroutine ref_ddadi_dir(f, ntot, prm, ipr, dir, phase) bind(C, name="ref_ddadi_dir")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: ntot, dir, phase
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  call ddadi_dir(f, ntot, prm, ipr, dir, phase)
+end subroutine ref_ddadi_dir
+
+subroutine ddadi_dir(f, ntot, prm, ipr, dir, phase)
+  use iso_c_binding
+  use ffref_ids
+  implicit none
+  integer(c_int) :: ntot, dir, phase
+  real(c_double) :: f(ntot, *)
+  real(c_double) :: prm(*)
+  integer(c_int) :: ipr(*)
+  integer, parameter :: NSRT = 27
+  integer :: ids(NSRT), q, imax, jmax, kmax, i1, j1, k1, idi, maw, maxw, nq
+  integer :: kcp(1)
+  real(c_double), allocatable :: g(:,:)
+  real(c_double), allocatable :: zzz(:), urot(:)
+  real(c_double) :: theta, dturb
+  integer :: ia, iax, iay, iaz
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  theta = prm(PTHETA); dturb = 1.
+  if (dir == 0) then
+     ia = IA3; iax = IA3X; iay = IA3Y; iaz = IA3Z
+     i1 = imax; j1 = jmax; k1 = kmax; idi = kmax + 1
+  else if (dir == 1) then
+     ia = IA1; iax = IA1X; iay = IA1Y; iaz = IA1Z
+     i1 = jmax; j1 = kmax; k1 = imax; idi = imax + 1
+  else
+     ia = IA2; iax = IA2X; iay = IA2Y; iaz = IA2Z
+     i1 = kmax; j1 = imax; k1 = jmax; idi = jmax + 1
+  end if
+  ! the fields the sweeps read or write, in a fixed order
+  ids = (/ IDRO, IDM, IDN, IDW, IDE, IDRK, IDEPS, ia, IVOL, iax, iay, iaz, IDTL, IU, IV, IW, &
+           IC, IRO, IP, IVIS, IVIST, IDRDP, IDRDH, IRK, IREPS, ISRK, ISEPS /)
+  nq = NSRT
+  allocate(g(ntot, NSRT), urot(ntot))
+  urot = 0.
+  do q = 1, nq
+     if (dir == 0) then
+        g(:, q) = f(:, ids(q))
+     else
+        call dd_sort(f(1, ids(q)), g(1, q), imax, jmax, kmax, dir, .false.)
+     end if
+  end do
+  maw = (i1 + 4)*(j1 + 4)
+  maxw = 8*maw
+  allocate(zzz(maxw))
+  zzz = 0.
+  if (ipr(QITURB) == 6) then
+     if (phase == 0 .or. phase == 3) &
+        call PREDKE(g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,8), g(1,9), g(1,10), g(1,11), &
+             g(1,12), g(1,13), g(1,14), g(1,15), g(1,16), g(1,17), g(1,18), g(1,19), g(1,22), &
+             g(1,23), i1, j1, k1, theta, 0._c_double, idi, g(1,20), g(1,21), g(1,6), g(1,7), &
+             g(1,24), g(1,25), g(1,26), g(1,27), prm(PPR), prm(PPRT), kcp, 2, 2, 2, urot, dturb, &
+             zzz, maw, maxw)
+     if (phase == 1 .or. phase == 3) &
+        call DIAGKE(g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,8), g(1,9), g(1,10), g(1,11), &
+             g(1,12), g(1,13), g(1,14), g(1,15), g(1,16), g(1,17), g(1,18), g(1,19), g(1,22), &
+             g(1,23), i1, j1, k1, theta, 0._c_double, idi, g(1,20), g(1,21), g(1,6), g(1,7), &
+             g(1,24), g(1,25), g(1,26), g(1,27), prm(PPR), prm(PPRT), kcp, 2, 2, 2, urot, dturb, &
+             zzz, maw, maxw)
+     if (phase == 2 .or. phase == 3) &
+        call CORRKE(g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,8), g(1,9), g(1,10), g(1,11), &
+             g(1,12), g(1,13), g(1,14), g(1,15), g(1,16), g(1,17), g(1,18), g(1,19), g(1,22), &
+             g(1,23), i1, j1, k1, theta, 0._c_double, idi, g(1,20), g(1,21), g(1,6), g(1,7), &
+             g(1,24), g(1,25), g(1,26), g(1,27), prm(PPR), prm(PPRT), kcp, 2, 2, 2, urot, dturb, &
+             zzz, maw, maxw)
+  else
+     if (phase == 0 .or. phase == 3) &
+        call PRED(g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,8), g(1,9), g(1,10), g(1,11), &
+             g(1,12), g(1,13), g(1,14), g(1,15), g(1,16), g(1,17), g(1,18), g(1,19), i1, j1, k1, &
+             theta, 0._c_double, idi, g(1,20), g(1,21), prm(PPR), prm(PPRT), 2, 2, 2, urot, zzz, &
+             maw, maxw)
+     if (phase == 1 .or. phase == 3) &
+        call DIAG(g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,8), g(1,9), g(1,10), g(1,11), &
+             g(1,12), g(1,13), g(1,14), g(1,15), g(1,16), g(1,17), g(1,18), g(1,19), i1, j1, k1, &
+             theta, 0._c_double, idi, g(1,20), g(1,21), prm(PPR), prm(PPRT), 2, 2, 2, urot)
+     if (phase == 2 .or. phase == 3) &
+        call CORR(g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,8), g(1,9), g(1,10), g(1,11), &
+             g(1,12), g(1,13), g(1,14), g(1,15), g(1,16), g(1,17), g(1,18), g(1,19), i1, j1, k1, &
+             theta, 0._c_double, idi, g(1,20), g(1,21), prm(PPR), prm(PPRT), 2, 2, 2, urot, zzz, &
+             maw, maxw)
+  end if
+  ! the increments back
+  do q = 1, 7
+     if (dir == 0) then
+        f(:, ids(q)) = g(:, q)
+     else
+        call dd_sort(f(1, ids(q)), g(1, q), imax, jmax, kmax, dir, .true.)
+     end if
+  end do
+  deallocate(g, urot, zzz)
+end subroutine ddadi_dir
 
 subroutine ref_update(f, ntot, prm, ipr) bind(C, name="ref_update")
   use iso_c_binding
