@@ -106,7 +106,7 @@ KERNELS = ("k_residual_kw", "k_residual_tile", "k_residual", "k_pred_tiles", "k_
            "k_turb_sst", "k_grad_zero", "k_derfun", "k_extend", "k_crosko", "k_kw_round",
            "k_max_final", "k_fun1_prod", "k_limits", "k_limits_mg", "k_fin", "k_zero", "k_fill_slots",
            "k_save_old", "k_divcp", "k_resl2_part", "k_resl2_final", "k_restrict",
-           "k_restrict_ghosts", "k_forcing", "k_prolong")
+           "k_restrict_ghosts", "k_forcing", "k_prolong", "k_ddadi", "k_ddadi_end", "k_reflec")
 
 
 def make_workload(name="sst-mg", seed=0):
@@ -288,7 +288,8 @@ def multiblock_bench(args):
       exact EARSM samples of the first blocks.
     Times K multi-block cycles: the plain FP64 path with eager launches, the same replayed
     from the captured multi-block graph (parallel per-block branches), and the chosen form
-    (FP32 LU-SGS records, autotuned tile shapes, the graph; with the closure on config5).
+    (FP32 LU-SGS records, autotuned tile shapes, the graph where the library captures one;
+    with the closure on config5).
     Prints one JSON line; `value` is the chosen form's."""
     from finflo_amd.cases import config3, config5
     from finflo_amd.ffhip import Context
@@ -343,10 +344,16 @@ def multiblock_bench(args):
         print(f"[bench] closure trained on {len(x)} samples: loss {loss[0]:.3g} -> {loss[-1]:.3g}",
               file=sys.stderr, flush=True)
     res = {}
+    # the library captures at most 3 blocks x 3 levels per graph branch (ff_vcycle_blocks_graph)
+    can_graph = (nb + 7) // 8 * nlev <= 9
     forms = [("fp64 eager", "fp64", False, False, "exact"),
              ("fp64 graph", "fp64", False, True, "exact"),
-             ("chosen form", "mixed", True, True, "mlp" if five else "exact")]
+             ("chosen form", "mixed", True, can_graph, "mlp" if five else "exact")]
     for label, prec, tune, graph, earsm in forms:
+        if graph and not can_graph:
+            res[label] = {"skipped": f"{nb} blocks of {nlev} levels exceed the library's graph "
+                                     "capture limit (3 blocks x 3 levels per branch)"}
+            continue
         ctx.precision(prec)
         ctx.autotune(tune)
         if five:
@@ -377,6 +384,7 @@ def multiblock_bench(args):
         "ms_per_step": res["chosen form"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 with FP32 LU-SGS records" + (", bf16 closure" if five else ""),
+        "chosen_form_launch": "HIP graph" if can_graph else "eager launches",
         "data": "synthetic",
         "config": {"workload": f"{nb}-block {'swept ' if five else ''}bump case, {ncell} cells, "
                                f"k-omega SST{' + EARSM' if five else ''}, M={0.6 if five else 0.5} "
@@ -495,6 +503,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["earsm-mlp", "config3", "config5"],
                     default="sst-mg")
+    ap.add_argument("--implicit", choices=["lusgs", "ddadi"], default="lusgs",
+                    help="the implicit stage: LU-SGS (BLKS%%LUSGS = 1, default) or DDADI (0)")
     ap.add_argument("--lusgs", choices=["tiles", "kmarch", "planes", "staged", "staged1"], default=None,
                     help="LU-SGS sweep form (default: the library's)")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
@@ -536,6 +546,7 @@ def main():
     prm, blocks, walls = make_workload(args.workload)
     if args.cfl:
         prm.cfl = prm.cfll = args.cfl
+    prm.ddadi = 1 if args.implicit == "ddadi" else 0
     ncell = IMAX * JMAX * KMAX
     cpu = None
     if not args.no_cpu:
@@ -699,7 +710,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"flat plate {IMAX}x{JMAX}x{KMAX} single block, {w['label']}, "
-                               f"M={MACH} Re={w['reynolds']:g}, Roe+FLUXDO, LU-SGS CFL=5; "
+                               f"M={MACH} Re={w['reynolds']:g}, Roe+FLUXDO, "
+                               f"{'DDADI' if prm.ddadi else 'LU-SGS'} CFL=5; "
                                f"a step is one {'V-cycle' if nlev > 1 else 'iteration'}",
                    "cells": ncell, "levels": nlev,
                    "launch": "HIP graph replay of K-1 steps + 1 eager step (residual sums)"

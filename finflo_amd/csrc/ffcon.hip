@@ -185,6 +185,13 @@ extern "C" int ff_vcycle_blocks_graph(ff_ctx *c, const int *blocks, int nblocks,
     if (!c || !blocks || nblocks < 1 || nlev < 1 || ncycles < 0) return FF_E_ARG;
     if (c->profile) return fail(c, FF_E_STATE, "ff_vcycle_blocks_graph: per-kernel timing is on%s");
     if (!c->stream) return fail(c, FF_E_STATE, "ff_vcycle_blocks_graph: the null stream cannot be captured%s");
+    // A capture of 64 blocks of 4 levels (8 V-cycles of 4 levels on each of the eight
+    // branches) ended the process with a segmentation fault while it was captured or first
+    // replayed; 24 blocks of 3 levels (3 V-cycles per branch) replay correctly.  Until the
+    // cause is known, captures beyond 3 blocks x 3 levels per branch are refused.
+    if ((nblocks + ff_ctx::NPOOL - 1) / ff_ctx::NPOOL * nlev > 9)
+        return fail(c, FF_E_UNSUP, "ff_vcycle_blocks_graph: more than 9 block levels per branch "
+                                   "(3 blocks x 3 levels) are not captured%s");
     std::vector<std::vector<BlockRec *>> lv(nblocks);
     for (int n = 0; n < nblocks; n++) {
         int rc = vcycle_levels(c, blocks[n], nlev, lv[n]);

@@ -4797,6 +4797,8 @@ static int staged_alloc(ff_ctx *c, BlockRec *r)
     return FF_OK;
 }
 
+static int ddadi_impl(ff_ctx *c, const Prm &kp, const Blk &b);   // ffddadi.hip
+
 static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
 {
     const Blk &b = r->dev;
@@ -4804,6 +4806,8 @@ static int implicit_impl(ff_ctx *c, BlockRec *r, int level, bool divcp)
     Prm kp = make_prm(&c->prm, level);
     int rc = launch(c, "k_time3_copr", k_time3_copr, cells_grid(b, row_tpb(b, 128)), dim3(row_tpb(b, 128)), kp, b);
     if (rc) return rc;
+    // BLKS%LUSGS = 0: the DDADI sweeps in place of the LU-SGS
+    if (c->prm.ddadi) return ddadi_impl(c, kp, b);
     if (c->lusgs_mode == FF_LUSGS_KMARCH) {
         // DRO/CP is applied inside the forward sweep
         dim3 g(r->swk.nti * r->swk.ntj);
@@ -5328,3 +5332,4 @@ int ff_wall_forces(ff_ctx *c, int block, int patch, double *sums)
 #include "ffconv.hip"
 #include "ffbc.hip"
 #include "ffcon.hip"
+#include "ffddadi.hip"

@@ -378,7 +378,9 @@ int  ff_vcycle_blocks(ff_ctx *ctx, const int *blocks, int nblocks, int nlev, int
  * branches the device runs side by side) and the halo exchange after them; each call
  * replays it ncycles times, bit for bit the eager cycles.  A registration or option change
  * makes the next call capture again.  No counterpart in the reference (CYCLES launches
- * nothing).  Not with per-kernel timing or on the null stream (FF_E_STATE). */
+ * nothing).  Not with per-kernel timing or on the null stream (FF_E_STATE); at most 9
+ * block levels per branch, i.e. ceil(nblocks/8) * nlev <= 9 (FF_E_UNSUP beyond: a 64-block,
+ * 4-level capture crashed the process and its cause is not known). */
 int  ff_vcycle_blocks_graph(ff_ctx *ctx, const int *blocks, int nblocks, int nlev, int ncycles);
 
 /* SUBIN's convergence monitor for block N (ns3c.f:9877-9905): the change of the cycle that

@@ -317,3 +317,17 @@ def test_config3_multiblock_cycles(hip_device):
                                           err_msg=f"graph block {n + 1} level {m}")
     ctx.close()
     ctx2.close()
+
+
+def test_graph_capture_limit(hip_device):
+    """ff_vcycle_blocks_graph refuses, before touching any block, a capture of more than 3
+    blocks x 3 levels per branch of its eight-stream pool (FF_E_UNSUP)."""
+    from finflo_amd.ffhip import Context, FFError
+    from finflo_amd.layout import default_params
+    print("case multi-block graph capture limit", flush=True)
+    ctx = Context(default_params())
+    with pytest.raises(FFError):
+        ctx.vcycle_blocks_graph(list(range(1, 26)), 3, ncycles=1)
+    with pytest.raises(FFError):
+        ctx.vcycle_blocks_graph(list(range(1, 9)), 10, ncycles=1)
+    ctx.close()
