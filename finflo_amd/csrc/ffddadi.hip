@@ -92,124 +92,213 @@ __device__ __forceinline__ void sym1ke(double x[7], const double w[7], double rk
     x[3] = a2z * x2i + a23 * x3i + a33 * x4i;
 }
 
-// One DDADI pass along every line of one direction: PH 0 PRED (backward, the upper
-// neighbour's flux entering through the upper face), 1 DIAG, 2 CORR (forward, the lower
-// neighbour's flux through the lower face).  Thread t owns plane entry f = IN + t*step.
-template <bool KE, int PH>
-__global__ void __launch_bounds__(128) k_ddadi(Prm p, Blk b, DdDir r, int nlines)
+// The sweep's line of thread t, as the flat offset of its cell m = 1 minus one step: the
+// reference's plane entry f = IN + t*step on the K and I sweeps (consecutive threads take
+// neighbouring I columns on the K sweep; on the I sweep each thread walks its own
+// contiguous row); on the J sweep the entries are renumbered with the middle index b (= I)
+// fastest so that consecutive threads again take neighbouring I columns.  The set of
+// entries is the reference's: f = a + b*(NA+4) >= IN, every 2*IN+1-th with NA = 1.
+__device__ __forceinline__ bool dd_line(const DdDir &r, int dir, int t, int &base)
+{
+    const int ist = r.na + 2 * FF_GHOST;
+    int a0, b0;
+    if (dir == 2) {
+        b0 = t % r.nb;
+        a0 = t / r.nb;
+        if (a0 >= ist || (b0 == 0 && a0 < FF_GHOST) || (r.na == 1 && a0 != FF_GHOST)) return false;
+    } else {
+        const int step = r.na == 1 ? 2 * FF_GHOST + 1 : 1;
+        const int f = FF_GHOST + t * step;
+        if (f >= r.nb * ist) return false;
+        a0 = f % ist;
+        b0 = f / ist;
+    }
+    base = a0 * r.sa + (b0 + FF_GHOST) * r.sb + (FF_GHOST - 1) * r.sm;
+    return true;
+}
+
+// threads of a sweep launch
+static int dd_threads(const DdDir &r, int dir)
+{
+    return dir == 2 ? (r.na + 2 * FF_GHOST) * r.nb : dd_lines(r);
+}
+
+// What a PRED (FWD = false) or CORR (FWD = true) step reads of its cell: the neighbour nn
+// across the face whose viscous term T and diagonal it forms (the lower neighbour in PRED,
+// the upper one in CORR), the area of the face the carried flux crosses (PRED: the upper
+// face, CORR: the lower face) and of the diagonal's face, and the normal of the face whose
+// characteristic variables it uses (PRED: the lower face, CORR: the upper face).
+struct DdIn {
+    double dt1, vol, voln, u, v, w, c, ro, ron, vis, vist, visn, vistn;
+    double aflux, adiag, fx, fy, fz;
+    double d[7], rk, drdp, drdh;
+};
+
+template <bool KE, bool FWD>
+__device__ __forceinline__ void dd_load(const Prm &p, const Blk &b, const DdDir &r, int n, DdIn &x)
+{
+    const int nu = n + r.sm, nn = FWD ? nu : n - r.sm, nf = FWD ? nu : n;
+    x.dt1 = p.theta * FLD(b, FF_DTL)[n];
+    x.vol = FLD(b, FF_VOL)[n];
+    x.voln = FLD(b, FF_VOL)[nn];
+    x.u = FLD(b, FF_U)[n];
+    x.v = FLD(b, FF_V)[n];
+    x.w = FLD(b, FF_W)[n];
+    x.c = FLD(b, FF_C)[n];
+    x.ro = FLD(b, FF_RO)[n];
+    x.ron = FLD(b, FF_RO)[nn];
+    x.vis = FLD(b, FF_VIS)[n];
+    x.vist = FLD(b, FF_VIST)[n];
+    x.visn = FLD(b, FF_VIS)[nn];
+    x.vistn = FLD(b, FF_VIST)[nn];
+    x.aflux = FWD ? r.A[n] : r.A[nu];
+    x.adiag = FWD ? r.A[nu] : r.A[n];
+    x.fx = r.AX[nf];
+    x.fy = r.AY[nf];
+    x.fz = r.AZ[nf];
+#pragma unroll
+    for (int q = 0; q < (KE ? 7 : 5); q++) x.d[q] = FLD(b, FF_DRO + q)[n];
+    if (KE) {
+        x.rk = FLD(b, FF_RK)[n];
+        x.drdp = FLD(b, FF_DRDP)[n];
+        x.drdh = FLD(b, FF_DRDH)[n];
+    }
+}
+
+// PRED (FWD = false: backward along the line) or CORR (FWD = true: forward) along every line
+// of one direction.  The reads of the next cell are issued before the current cell's
+// arithmetic (they do not depend on the recurrence), so their latency overlaps it.
+template <bool KE, bool FWD>
+__global__ void __launch_bounds__(128) k_ddadi_sweep(Prm p, Blk b, DdDir r, int dir, int nthreads)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlines) return;
+    int base;
+    if (t >= nthreads || !dd_line(r, dir, t, base)) return;
     constexpr int NV = KE ? 7 : 5;
-    const int ist = r.na + 2 * FF_GHOST, step = r.na == 1 ? 2 * FF_GHOST + 1 : 1;
-    const int f = FF_GHOST + t * step;
-    const int base = (f % ist) * r.sa + (f / ist + FF_GHOST) * r.sb;
     double *D[7] = {FLD(b, FF_DRO), FLD(b, FF_DM), FLD(b, FF_DN), FLD(b, FF_DW), FLD(b, FF_DE),
                     FLD(b, FF_DRK), FLD(b, FF_DEPS)};
-    const double *dtl = FLD(b, FF_DTL), *vol = FLD(b, FF_VOL), *u = FLD(b, FF_U),
-                 *v = FLD(b, FF_V), *w = FLD(b, FF_W), *c = FLD(b, FF_C), *ro = FLD(b, FF_RO),
-                 *vis = FLD(b, FF_VIS), *vist = FLD(b, FF_VIST), *rk = FLD(b, FF_RK),
-                 *drdp = FLD(b, FF_DRDP), *drdh = FLD(b, FF_DRDH);
     const double cdiff = 2., dturb = 1., vrel = 1.E-21;
     double bd[7] = {0., 0., 0., 0., 0., 0., 0.};
+    int m = FWD ? 1 : r.nm;
+    DdIn cur, nxt;
+    dd_load<KE, FWD>(p, b, r, base + m * r.sm, cur);
     for (int s = 0; s < r.nm; s++) {
-        const int m = PH == 0 ? r.nm - s : s + 1;
-        const int n = base + (m + FF_GHOST - 1) * r.sm, nu = n + r.sm, nl = n - r.sm;
-        // the face whose normal the characteristic variables use: the lower face in PRED,
-        // the upper face in DIAG and CORR
-        const int nf = PH == 0 ? n : nu;
-        const double fx = r.AX[nf], fy = r.AY[nf], fz = r.AZ[nf];
+        const int n = base + m * r.sm;
+        const int mn = FWD ? m + 1 : m - 1;
+        if (s + 1 < r.nm) dd_load<KE, FWD>(p, b, r, base + mn * r.sm, nxt);
         double x[7], y[7], a21, a22, a23, a31, a32, a33;
-        const double dt1 = p.theta * dtl[n];
-        if (PH == 1) {
 #pragma unroll
-            for (int q = 0; q < NV; q++) x[q] = D[q][n];
-        } else {
-            // the flux carried along the line enters through the upper face in PRED, the
-            // lower one in CORR
-            const double af = PH == 0 ? r.A[nu] : r.A[n];
-#pragma unroll
-            for (int q = 0; q < 5; q++) x[q] = dt1 * af * bd[q] + vol[n] * D[q][n];
-            if (KE) {
-#pragma unroll
-                for (int q = 5; q < 7; q++) x[q] = dturb * dt1 * af * bd[q] + vol[n] * D[q][n];
-            }
-        }
-        if (KE)
-            syike(y, x, c[n], ro[n], fx, fy, fz, a21, a22, a23, a31, a32, a33);
-        else
-            syi(y[0], y[1], y[2], y[3], y[4], x[0], x[1], x[2], x[3], x[4], c[n], ro[n], fx, fy,
-                fz, a21, a22, a23, a31, a32, a33);
-        double xs[7];
-        if (PH == 1) {
-            double ucon = r.AX[nu] * u[n] + r.AY[nu] * v[n] + r.AZ[nu] * w[n] - 0.;
-            const double rl2n = dmax(-(ucon + c[n]), 0.), rl5n = dmax(-(ucon - c[n]), 0.);
-            const double rl1n = dmax(-ucon, 0.);
-            ucon = r.AX[n] * u[n] + r.AY[n] * v[n] + r.AZ[n] * w[n] - 0.;
-            const double rl2p = dmax(ucon + c[n], 0.), rl5p = dmax(ucon - c[n], 0.);
-            const double rl1p = dmax(ucon, 0.);
-            const double dt = KE ? dt1 / (vol[n] + 1.E-21) : dt1 / vol[n];
-            const double s1 = r.A[n] * rl1p + r.A[nu] * rl1n, s2 = r.A[n] * rl2p + r.A[nu] * rl2n;
-            const double s5 = r.A[n] * rl5p + r.A[nu] * rl5n;
-            y[0] = y[0] * (1. + dt * s1);
-            y[1] = y[1] * (1. + dt * s2);
-            y[2] = y[2] * (1. + dt * s1);
-            y[3] = y[3] * (1. + dt * s1);
-            y[4] = y[4] * (1. + dt * s5);
-            if (KE) {
-                y[5] = y[5] * (1. + dturb * dt * (s1 + 0.));
-                y[6] = y[6] * (1. + dturb * dt * (s1 + 0.));
-            }
-        } else {
-            const double rl1 = fx * u[n] + fy * v[n] + fz * w[n] - 0.;
-            const double rl2 = rl1 + c[n], rl5 = rl1 - c[n];
-            const int nn = PH == 0 ? nl : nu;           // the line neighbour of the face
-            const double ff = PH == 0 ? (m == 1 ? .25 : .5) : .5;
-            const double rnum = ff * (vol[n] + vol[nn]) * (ro[n] + ro[nn]);
-            const double af = PH == 0 ? r.A[n] : r.A[nu];
-            const double tt = cdiff * af / rnum * (vist[n] + vis[n] + vist[nn] + vis[nn]);
-            double x1, x2, x5;
-            if (PH == 0) {
-                x1 = dmax(-rl1, 0.) + tt; x2 = dmax(-rl2, 0.) + tt; x5 = dmax(-rl5, 0.) + tt;
-            } else {
-                x1 = dmax(rl1, 0.) + tt; x2 = dmax(rl2, 0.) + tt; x5 = dmax(rl5, 0.) + tt;
-            }
-            const double x1a = fabs(rl1) + tt, x2a = fabs(rl2) + tt, x5a = fabs(rl5) + tt;
-            const double xa[5] = {x1a, x2a, x1a, x1a, x5a};
-            if (KE) {
-#pragma unroll
-                for (int q = 0; q < 5; q++) y[q] = y[q] / (vol[n] + vrel + dt1 * xa[q] * af);
-#pragma unroll
-                for (int q = 5; q < 7; q++) y[q] = y[q] / (vol[n] + vrel + dturb * dt1 * (x1a * af + 0.));
-            } else {
-#pragma unroll
-                for (int q = 0; q < 5; q++) y[q] = y[q] / (vol[n] + dt1 * xa[q] * af);
-            }
-            xs[0] = x1; xs[1] = x2; xs[2] = x1; xs[3] = x1; xs[4] = x5; xs[5] = x1; xs[6] = x1;
-        }
-        // the new increments of the cell
+        for (int q = 0; q < 5; q++) x[q] = cur.dt1 * cur.aflux * bd[q] + cur.vol * cur.d[q];
         if (KE) {
+#pragma unroll
+            for (int q = 5; q < 7; q++) x[q] = dturb * cur.dt1 * cur.aflux * bd[q] + cur.vol * cur.d[q];
+            syike(y, x, cur.c, cur.ro, cur.fx, cur.fy, cur.fz, a21, a22, a23, a31, a32, a33);
+        } else {
+            syi(y[0], y[1], y[2], y[3], y[4], x[0], x[1], x[2], x[3], x[4], cur.c, cur.ro, cur.fx,
+                cur.fy, cur.fz, a21, a22, a23, a31, a32, a33);
+        }
+        const double rl1 = cur.fx * cur.u + cur.fy * cur.v + cur.fz * cur.w - 0.;
+        const double rl2 = rl1 + cur.c, rl5 = rl1 - cur.c;
+        const double ff = FWD ? .5 : (m == 1 ? .25 : .5);
+        const double rnum = ff * (cur.vol + cur.voln) * (cur.ro + cur.ron);
+        const double tt = cdiff * cur.adiag / rnum * (cur.vist + cur.vis + cur.vistn + cur.visn);
+        double x1, x2, x5;
+        if (FWD) {
+            x1 = dmax(rl1, 0.) + tt; x2 = dmax(rl2, 0.) + tt; x5 = dmax(rl5, 0.) + tt;
+        } else {
+            x1 = dmax(-rl1, 0.) + tt; x2 = dmax(-rl2, 0.) + tt; x5 = dmax(-rl5, 0.) + tt;
+        }
+        const double x1a = fabs(rl1) + tt, x2a = fabs(rl2) + tt, x5a = fabs(rl5) + tt;
+        const double xa[5] = {x1a, x2a, x1a, x1a, x5a};
+        if (KE) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) y[q] = y[q] / (cur.vol + vrel + cur.dt1 * xa[q] * cur.adiag);
+#pragma unroll
+            for (int q = 5; q < 7; q++) y[q] = y[q] / (cur.vol + vrel + dturb * cur.dt1 * (x1a * cur.adiag + 0.));
             double o[7];
-            sym1ke(o, y, rk[n], c[n], ro[n], drdp[n], drdh[n], fx, fy, fz, a21, a22, a23, a31,
-                   a32, a33);
+            sym1ke(o, y, cur.rk, cur.c, cur.ro, cur.drdp, cur.drdh, cur.fx, cur.fy, cur.fz, a21, a22,
+                   a23, a31, a32, a33);
 #pragma unroll
             for (int q = 0; q < 7; q++) D[q][n] = o[q];
         } else {
+#pragma unroll
+            for (int q = 0; q < 5; q++) y[q] = y[q] / (cur.vol + cur.dt1 * xa[q] * cur.adiag);
             double o0, o1, o2, o3, o4;
-            sym1i(o0, o1, o2, o3, o4, y[0], y[1], y[2], y[3], y[4], c[n], ro[n], fx, fy, fz, a21,
-                  a22, a23, a31, a32, a33);
+            sym1i(o0, o1, o2, o3, o4, y[0], y[1], y[2], y[3], y[4], cur.c, cur.ro, cur.fx, cur.fy,
+                  cur.fz, a21, a22, a23, a31, a32, a33);
             D[0][n] = o0; D[1][n] = o1; D[2][n] = o2; D[3][n] = o3; D[4][n] = o4;
         }
-        if (PH == 1) continue;
         // the flux handed to the next cell of the line
+        const double xs[7] = {x1, x2, x1, x1, x5, x1, x1};
 #pragma unroll
         for (int q = 0; q < NV; q++) y[q] = xs[q] * y[q];
         if (KE)
-            sym1ke(bd, y, rk[n], c[n], ro[n], drdp[n], drdh[n], fx, fy, fz, a21, a22, a23, a31, a32,
-                   a33);
+            sym1ke(bd, y, cur.rk, cur.c, cur.ro, cur.drdp, cur.drdh, cur.fx, cur.fy, cur.fz, a21, a22,
+                   a23, a31, a32, a33);
         else
-            sym1i(bd[0], bd[1], bd[2], bd[3], bd[4], y[0], y[1], y[2], y[3], y[4], c[n], ro[n], fx,
-                  fy, fz, a21, a22, a23, a31, a32, a33);
+            sym1i(bd[0], bd[1], bd[2], bd[3], bd[4], y[0], y[1], y[2], y[3], y[4], cur.c, cur.ro,
+                  cur.fx, cur.fy, cur.fz, a21, a22, a23, a31, a32, a33);
+        cur = nxt;
+        m = mn;
+    }
+}
+
+// DIAG (DIAGKE) on every entry of the direction's plane and line, one thread each: x runs
+// along I (stride 1), y and z over the other two sorted indices
+template <bool KE>
+__global__ void __launch_bounds__(256) k_ddadi_diag(Prm p, Blk b, DdDir r, int dir)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x, gy = blockIdx.y, gz = blockIdx.z;
+    int a0, b0, m;
+    if (dir == 0) { a0 = gx; b0 = gy; m = gz + 1; }      // a = I, b = J, m = K
+    else if (dir == 1) { m = gx + 1; a0 = gy; b0 = gz; } // a = J, b = K, m = I
+    else { b0 = gx; a0 = gy; m = gz + 1; }                // a = K, b = I, m = J
+    const int ist = r.na + 2 * FF_GHOST;
+    if (a0 >= ist || b0 >= r.nb || m > r.nm) return;
+    if ((b0 == 0 && a0 < FF_GHOST) || (r.na == 1 && a0 != FF_GHOST)) return;
+    const int n = a0 * r.sa + (b0 + FF_GHOST) * r.sb + (m + FF_GHOST - 1) * r.sm, nu = n + r.sm;
+    double *D[7] = {FLD(b, FF_DRO), FLD(b, FF_DM), FLD(b, FF_DN), FLD(b, FF_DW), FLD(b, FF_DE),
+                    FLD(b, FF_DRK), FLD(b, FF_DEPS)};
+    const double *vol = FLD(b, FF_VOL), *u = FLD(b, FF_U), *v = FLD(b, FF_V), *w = FLD(b, FF_W),
+                 *c = FLD(b, FF_C), *ro = FLD(b, FF_RO);
+    const double dturb = 1.;
+    const double dt1 = p.theta * FLD(b, FF_DTL)[n];
+    const double fx = r.AX[nu], fy = r.AY[nu], fz = r.AZ[nu];
+    double xv[7], y[7], a21, a22, a23, a31, a32, a33;
+#pragma unroll
+    for (int q = 0; q < (KE ? 7 : 5); q++) xv[q] = D[q][n];
+    if (KE)
+        syike(y, xv, c[n], ro[n], fx, fy, fz, a21, a22, a23, a31, a32, a33);
+    else
+        syi(y[0], y[1], y[2], y[3], y[4], xv[0], xv[1], xv[2], xv[3], xv[4], c[n], ro[n], fx, fy, fz,
+            a21, a22, a23, a31, a32, a33);
+    double ucon = fx * u[n] + fy * v[n] + fz * w[n] - 0.;
+    const double rl2n = dmax(-(ucon + c[n]), 0.), rl5n = dmax(-(ucon - c[n]), 0.);
+    const double rl1n = dmax(-ucon, 0.);
+    ucon = r.AX[n] * u[n] + r.AY[n] * v[n] + r.AZ[n] * w[n] - 0.;
+    const double rl2p = dmax(ucon + c[n], 0.), rl5p = dmax(ucon - c[n], 0.);
+    const double rl1p = dmax(ucon, 0.);
+    const double dt = KE ? dt1 / (vol[n] + 1.E-21) : dt1 / vol[n];
+    const double s1 = r.A[n] * rl1p + r.A[nu] * rl1n, s2 = r.A[n] * rl2p + r.A[nu] * rl2n;
+    const double s5 = r.A[n] * rl5p + r.A[nu] * rl5n;
+    y[0] = y[0] * (1. + dt * s1);
+    y[1] = y[1] * (1. + dt * s2);
+    y[2] = y[2] * (1. + dt * s1);
+    y[3] = y[3] * (1. + dt * s1);
+    y[4] = y[4] * (1. + dt * s5);
+    if (KE) {
+        y[5] = y[5] * (1. + dturb * dt * (s1 + 0.));
+        y[6] = y[6] * (1. + dturb * dt * (s1 + 0.));
+        double o[7];
+        sym1ke(o, y, FLD(b, FF_RK)[n], c[n], ro[n], FLD(b, FF_DRDP)[n], FLD(b, FF_DRDH)[n], fx, fy,
+               fz, a21, a22, a23, a31, a32, a33);
+#pragma unroll
+        for (int q = 0; q < 7; q++) D[q][n] = o[q];
+    } else {
+        double o0, o1, o2, o3, o4;
+        sym1i(o0, o1, o2, o3, o4, y[0], y[1], y[2], y[3], y[4], c[n], ro[n], fx, fy, fz, a21, a22,
+              a23, a31, a32, a33);
+        D[0][n] = o0; D[1][n] = o1; D[2][n] = o2; D[3][n] = o3; D[4][n] = o4;
     }
 }
 
@@ -235,18 +324,22 @@ static int ddadi_impl(ff_ctx *c, const Prm &kp, const Blk &b)
     const bool ke = c->prm.iturb == 6;
     for (int dir = 0; dir < 3; dir++) {
         const DdDir r = dd_dir(b, dir);
-        const int nl = dd_lines(r);
-        const dim3 g((unsigned)((nl + 127) / 128)), t(128);
+        const int nt = dd_threads(r, dir);
+        const dim3 g((unsigned)((nt + 127) / 128)), t(128);
+        // DIAG: x along I, y and z over the other two indices of the sorted order
+        const int ist = r.na + 2 * FF_GHOST;
+        const int xn = dir == 0 ? ist : dir == 1 ? r.nm : r.nb;
+        const dim3 gd((unsigned)((xn + 255) / 256), dir == 0 ? r.nb : ist, dir == 1 ? r.nb : r.nm);
         int rc;
         if (ke) {
-            if ((rc = launch(c, "k_ddadi", k_ddadi<true, 0>, g, t, kp, b, r, nl)) ||
-                (rc = launch(c, "k_ddadi", k_ddadi<true, 1>, g, t, kp, b, r, nl)) ||
-                (rc = launch(c, "k_ddadi", k_ddadi<true, 2>, g, t, kp, b, r, nl)))
+            if ((rc = launch(c, "k_ddadi", k_ddadi_sweep<true, false>, g, t, kp, b, r, dir, nt)) ||
+                (rc = launch(c, "k_ddadi", k_ddadi_diag<true>, gd, dim3(256), kp, b, r, dir)) ||
+                (rc = launch(c, "k_ddadi", k_ddadi_sweep<true, true>, g, t, kp, b, r, dir, nt)))
                 return rc;
         } else {
-            if ((rc = launch(c, "k_ddadi", k_ddadi<false, 0>, g, t, kp, b, r, nl)) ||
-                (rc = launch(c, "k_ddadi", k_ddadi<false, 1>, g, t, kp, b, r, nl)) ||
-                (rc = launch(c, "k_ddadi", k_ddadi<false, 2>, g, t, kp, b, r, nl)))
+            if ((rc = launch(c, "k_ddadi", k_ddadi_sweep<false, false>, g, t, kp, b, r, dir, nt)) ||
+                (rc = launch(c, "k_ddadi", k_ddadi_diag<false>, gd, dim3(256), kp, b, r, dir)) ||
+                (rc = launch(c, "k_ddadi", k_ddadi_sweep<false, true>, g, t, kp, b, r, dir, nt)))
                 return rc;
         }
     }
