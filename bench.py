@@ -515,6 +515,9 @@ def main():
     ap.add_argument("--kbc", choices=["ext", "periodic"], default="ext",
                     help="the K faces in the convergence run: free stream (EXT) or connected "
                          "to each other (periodic span)")
+    ap.add_argument("--outer", choices=["ext", "inout"], default="ext",
+                    help="the I faces of the plate: free stream (EXT, GHOSTEXT at start-up) or "
+                         "a subsonic inlet (INLOUT 4) and a pressure outlet (INLOUT 3)")
     ap.add_argument("--converge-cycles", type=int, default=2000,
                     help="after the timing, run from the initial state until an 8-decade drop "
                          "of the density residual or this many cycles (0: skip; default 2000)")
@@ -567,7 +570,7 @@ def main():
             ctx.upload(level=m)
             ctx.wall_register(*walls[m - 1], level=m)
             ctx.wall_sync(0, level=m)
-            bcs = plate_bcs(blocks[m - 1])
+            bcs = plate_bcs(blocks[m - 1], outer=args.outer, prm=prm)
             if args.kbc == "periodic":
                 bcs = [bp for bp in bcs if bp.iface not in (3, 6)]
             ctx.bc_register(bcs, level=m)
@@ -693,6 +696,7 @@ def main():
             ctx.halo_exchange(what="flow")
         conv = converge(ctx, nlev, ncell, args.converge_cycles, periodic=args.kbc == "periodic")
         conv["k_faces"] = args.kbc
+        conv["i_faces"] = "inlet 4 / outlet 3" if args.outer == "inout" else "ext"
         print(f"[bench] convergence: {conv['decades']} decades in {conv['cycles']} cycles, "
               f"{conv['seconds']:.2f} s", file=sys.stderr, flush=True)
 
@@ -711,7 +715,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"flat plate {IMAX}x{JMAX}x{KMAX} single block, {w['label']}, "
                                f"M={MACH} Re={w['reynolds']:g}, Roe+FLUXDO, "
-                               f"{'DDADI' if prm.ddadi else 'LU-SGS'} CFL=5; "
+                               f"{'DDADI' if prm.ddadi else 'LU-SGS'} CFL=5, "
+                               f"{'inlet/outlet on the I faces' if args.outer == 'inout' else 'free stream on the outer faces'}; "
                                f"a step is one {'V-cycle' if nlev > 1 else 'iteration'}",
                    "cells": ncell, "levels": nlev,
                    "launch": "HIP graph replay of K-1 steps + 1 eager step (residual sums)"

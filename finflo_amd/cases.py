@@ -383,16 +383,30 @@ def config4(seed=0):
     return prm, blk, plate_wall(blk, prm)
 
 
-def plate_bcs(blk, wall_faces=(2,)):
+def plate_bcs(blk, wall_faces=(2,), outer="ext", prm=None):
     """The boundary patches of the flat-plate and box cases: a no-slip wall (REFLEC's
     viscous branch, at rest) on each face of `wall_faces`, the free stream (EXT, GHOSTEXT)
-    on every other face."""
+    on every other face.  outer = "inout" makes the I-low face a subsonic inlet (INLOUT 4:
+    the free stream's density, velocity along I and temperature given, the pressure
+    extrapolated; the k-omega free-stream limits) and the I-high face a pressure outlet
+    (INLOUT 3: the free-stream pressure given, the velocity extrapolated); prm gives the
+    free stream then."""
     ext = {1: (blk.jmax, blk.kmax), 2: (blk.imax, blk.kmax), 3: (blk.imax, blk.jmax)}
     out = []
     for face in range(1, 7):
         na, nb = ext[(face - 1) % 3 + 1]
-        ibc = 8 if face in wall_faces else 2
-        out.append(BcPatch(ibc, face, 1, na, 1, nb, iheat=0, viscous=1))
+        if face in wall_faces:
+            out.append(BcPatch(8, face, 1, na, 1, nb, iheat=0, viscous=1))
+        elif outer == "inout" and face in (1, 4):
+            bp = BcPatch(3 if face == 1 else 5, face, 1, na, 1, nb, inlout=4 if face == 1 else 3)
+            n = bp.size
+            full = lambda v: np.full(n, float(v))
+            bp.boun.update(bounr=full(prm.frsden), bounu=full(prm.frsvel), bounv=full(0.0),
+                           bounw=full(0.0), bount=full(prm.frstem), bounp=full(prm.frspre),
+                           bounpd=full(0.0), bounrk=full(prm.rklim), bounep=full(prm.epslim))
+            out.append(bp)
+        else:
+            out.append(BcPatch(2, face, 1, na, 1, nb, iheat=0, viscous=1))
     return out
 
 

@@ -3687,10 +3687,12 @@ __global__ void __launch_bounds__(256) k_divcp(Blk b)
     FLD(b, FF_DRO)[n] = FLD(b, FF_DRO)[n] / FLD(b, FF_CP)[n];
 }
 
-// Start-of-cycle copies (TOLD ...) on every cell
-__global__ void __launch_bounds__(256) k_save_old(Blk b)
+// Start-of-cycle copies (TOLD ...) on every cell; a cycle of a block with INL/OUT patches
+// also counts its ICYCLE (icyc, null otherwise)
+__global__ void __launch_bounds__(256) k_save_old(Blk b, int *icyc)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (icyc && n == 0) *icyc = *icyc + 1;
     if (n >= b.ntot) return;
     FLD(b, FF_TOLD)[n] = FLD(b, FF_TEMP)[n];
     FLD(b, FF_UOLD)[n] = FLD(b, FF_U)[n];
@@ -3799,13 +3801,15 @@ struct WallRec {
     bool pending = false;              // a force pass may still read the scratch
 };
 
-// One REFLEC / GHOSTEXT patch in REFLEC's frame (ffbc.hip): ICON(1,L), face, ILMO, the
-// face plane, the KX/KY ranges after the J-face swap, the stride normal to the face, IHEAT,
-// the viscous-branch flag, TSURF and the device wall velocity arrays (or null)
+// One REFLEC / GHOSTEXT / INLET / OUTLET patch in REFLEC's frame (ffbc.hip): ICON(1,L),
+// face, ILMO, the face plane, the KX/KY ranges after the J-face swap, the stride normal to
+// the face, IHEAT, the viscous-branch flag, TSURF, the device wall velocity arrays (or
+// null), the INLOUT type and the device patch arrays BOUNR .. BOUNEP of INL/OUT patches
 struct BcPatchDev {
-    int ibc, iface, dir, idir, kpl, kx1, kx2, ky1, ky2, kstr, iheat, viscous, n;
+    int ibc, iface, dir, idir, kpl, kx1, kx2, ky1, ky2, kstr, iheat, viscous, n, inlout;
     double tsurf;
     const double *uw, *vw, *ww;
+    double *bn[9];
 };
 
 struct BlockRec {
@@ -3826,7 +3830,8 @@ struct BlockRec {
     double *st_mem = nullptr;
     void *dmx_mem = nullptr;   // DROMAX chunk maxima and candidates (allocated on first use)
     std::vector<BcPatchDev> bcs;   // boundary patches (ff_bc_register)
-    double *bc_mem = nullptr;      // their wall velocity arrays on the device
+    double *bc_mem = nullptr;      // their wall velocity and INL/OUT arrays on the device
+    int *bc_icyc = nullptr;        // the cycle count ICYCLE INLET reads (INL/OUT patches)
     std::vector<ff_con_patch> cons;   // connection patches (ff_con_register)
     int *con_map = nullptr;           // their ghost <- source cell offset pairs on the device
     std::vector<size_t> con_off;      // start of each patch's pairs in con_map
@@ -4115,6 +4120,8 @@ static void sweep_free(BlockRec &r)
     r.dmx_mem = nullptr;
     if (r.bc_mem) (void)hipFree(r.bc_mem);
     r.bc_mem = nullptr;
+    if (r.bc_icyc) (void)hipFree(r.bc_icyc);
+    r.bc_icyc = nullptr;
     r.bcs.clear();
     if (r.con_map) (void)hipFree(r.con_map);
     r.con_map = nullptr;
@@ -4585,6 +4592,8 @@ struct RedBuf {
     double *out;
 };
 
+static int inlout_impl(ff_ctx *c, BlockRec *r, int level);   // ffbc.hip
+
 static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi,
                          const RedBuf *rb = nullptr)
 {
@@ -4665,6 +4674,7 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
                          (const double *)(r->kw_scr + r->kw_parts)))) return rc;
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
+        if (level == 1 && (rc = inlout_impl(c, r, level))) return rc;
         const int sh = fns ? -1 : tuned_shape(c, true, kp, b, wr.wd);
         if (sh >= 0) {
             rc = launch_shape_any(c, sh, true, kp, b, wr.wd);
@@ -4682,6 +4692,7 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
     } else {
         if (fns && (rc = gradients())) return rc;
         if ((rc = walls())) return rc;
+        if (level == 1 && (rc = inlout_impl(c, r, level))) return rc;
         const int sh = (fns || vl || c->residual_mode != FF_RESIDUAL_TILES) ? -1
                        : tuned_shape(c, false, make_prm(&c->prm, level), b, wr.wd);
         if (sh >= 0) {
@@ -4984,7 +4995,7 @@ int ff_save_old(ff_ctx *c, int level, int block)
     BlockRec *r = find(c, level, block);
     if (!r) return fail(c, FF_E_STATE, "block not registered%s");
     c->lev = level;
-    return launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev);
+    return launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev, (int *)nullptr);
 }
 
 int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
@@ -4996,7 +5007,7 @@ int ff_cycle(ff_ctx *c, int block, int ncycles, double *resi)
     for (int it = 0; it < ncycles; it++) {
         bool last = it == ncycles - 1;
         c->lev = 1;
-        if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev))) return rc;
+        if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r->dev), dim3(256), r->dev, r->bc_icyc))) return rc;
         if ((rc = ff_state(c, 1, block))) return rc;
         if ((rc = ff_turb(c, 1, block))) return rc;
         if ((rc = residual_impl(c, r, 1, 1, last ? resi : nullptr))) return rc;
@@ -5040,7 +5051,7 @@ static int vcycle_once(ff_ctx *c, const std::vector<BlockRec *> &lv, int nlev, i
     BlockRec *r1 = lv[0];
     // level 1: CYCLES ns3c.f:1450-1596
     c->lev = 1;
-    if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev))) return rc;
+    if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev, r1->bc_icyc))) return rc;
     if ((rc = ff_state(c, 1, block))) return rc;
     if ((rc = ff_turb(c, 1, block))) return rc;
     if ((rc = residual_impl(c, r1, 1, 1, rs))) return rc;

@@ -25,7 +25,8 @@ EXPORTS = [
     "ff_prolong", "ff_vcycle", "ff_vcycle_graph", "ff_wall_register", "ff_wall_sync", "ff_wall_forces",
     "ff_mlp_set", "ff_mlp_get", "ff_earsm_samples", "ff_mlp_fit_norm", "ff_mlp_infer",
     "ff_mlp_grad", "ff_mlp_train", "ff_residual_noalloc", "ff_dromax",
-    "ff_bc_register", "ff_reflec", "ff_ghostext", "ff_con_register", "ff_halo_exchange",
+    "ff_bc_register", "ff_reflec", "ff_ghostext", "ff_inlout", "ff_bc_cycle", "ff_bc_read",
+    "ff_con_register", "ff_halo_exchange",
     "ff_vcycle_blocks", "ff_vcycle_blocks_graph", "ff_tuned_shape",
 ]
 
@@ -73,6 +74,9 @@ def lib():
         L.ff_bc_register.argtypes = [vp, i, i, vp, i]
         L.ff_reflec.argtypes = [vp, i, i]
         L.ff_ghostext.argtypes = [vp, i, i, d, d]
+        L.ff_inlout.argtypes = [vp, i, i]
+        L.ff_bc_cycle.argtypes = [vp, i, i, i]
+        L.ff_bc_read.argtypes = [vp, i, i, i, vp, vp]
         L.ff_con_register.argtypes = [vp, i, i, vp, i]
         L.ff_halo_exchange.argtypes = [vp, i, i]
         L.ff_vcycle_blocks.argtypes = [vp, vp, i, i, i, vp]
@@ -221,10 +225,11 @@ class Context:
                   "ff_dromax")
         return dm.value, ix.value, inn.value
 
-    # ---- boundary patches (REFLEC, GHOSTEXT) -----------------------------------------------
+    # ---- boundary patches (REFLEC, GHOSTEXT, INLET/OUTLET) ---------------------------------
     def bc_register(self, patches, level=1, block=1):
         """Boundary patches (finflo_amd.layout.BcPatch) of a block level; ff_state then ends
-        with REFLEC over them."""
+        with REFLEC over them, and a level-1 residual starts its fluxes with INLET/OUTLET
+        over the INL/OUT patches."""
         views = bc_views(patches)
         self._chk(lib().ff_bc_register(self.h, level, block, views, len(patches)), "ff_bc_register")
 
@@ -233,6 +238,21 @@ class Context:
 
     def ghostext(self, level=1, block=1, alpha=0.0, beta=0.0):
         self._chk(lib().ff_ghostext(self.h, level, block, alpha, beta), "ff_ghostext")
+
+    def inlout(self, level=1, block=1):
+        """INLET/OUTLET over the registered INL/OUT patches."""
+        self._chk(lib().ff_inlout(self.h, level, block), "ff_inlout")
+
+    def bc_cycle(self, icycle, level=1, block=1):
+        """Set the block's cycle count ICYCLE."""
+        self._chk(lib().ff_bc_cycle(self.h, level, block, icycle), "ff_bc_cycle")
+
+    def bc_read(self, patch, size, level=1, block=1):
+        """BOUNP and BOUNPD of INL/OUT patch `patch` as two arrays of `size` entries."""
+        bp, bpd = np.zeros(size), np.zeros(size)
+        self._chk(lib().ff_bc_read(self.h, level, block, patch, bp.ctypes.data, bpd.ctypes.data),
+                  "ff_bc_read")
+        return bp, bpd
 
     # ---- block connections -------------------------------------------------------------
     def con_register(self, patches, level=1, block=1):

@@ -218,26 +218,48 @@ def wall_views(patches):
 
 # ---- boundary patches (ff_bc_patch of include/ffhip.h) -----------------------------------
 
+# the INL/OUT patch arrays of ff_bc_patch, in its order
+BOUN_ARRAYS = ("bounr", "bounu", "bounv", "bounw", "bount", "bounp", "bounpd", "bounrk",
+               "bounep")
+
+
 class BcPatchView(C.Structure):
-    """ff_bc_patch: one REFLEC / GHOSTEXT patch."""
+    """ff_bc_patch: one REFLEC / GHOSTEXT / INLET / OUTLET patch."""
     _fields_ = [("ibc", C.c_int), ("iface", C.c_int), ("kx1", C.c_int), ("kx2", C.c_int),
                 ("ky1", C.c_int), ("ky2", C.c_int), ("iheat", C.c_int), ("viscous", C.c_int),
                 ("tsurf", C.c_double), ("reserved", C.c_double * 3),
-                ("uwall", C.c_void_p), ("vwall", C.c_void_p), ("wwall", C.c_void_p)]
+                ("uwall", C.c_void_p), ("vwall", C.c_void_p), ("wwall", C.c_void_p),
+                ("inlout", C.c_int), ("reserved_i", C.c_int * 3)] + \
+        [(n, C.c_void_p) for n in BOUN_ARRAYS]
 
 
 class BcPatch:
-    """A boundary patch: ICON(1,L) = ibc (2 EXT, 7 symmetry, 8-10 solid wall), ICON(3..7,L),
-    ICON(20,L), REFLEC's viscous-branch flag, RCON(5,L) and the wall velocity arrays
-    ((NA+4)*(NB+4), as WallPatch; None = at rest)."""
+    """A boundary patch: ICON(1,L) = ibc (2 EXT, 3 INL, 5 OUT, 7 symmetry, 8-10 solid wall),
+    ICON(3..7,L), ICON(20,L), REFLEC's viscous-branch flag, RCON(5,L), the wall velocity
+    arrays ((NA+4)*(NB+4), as WallPatch; None = at rest), and for INL/OUT patches ICON(8,L)
+    (inlout) and the patch arrays BOUNR ... BOUNEP (boun: a dict of (NA+4)*(NB+4) arrays by
+    the names of BOUN_ARRAYS; missing ones are None)."""
 
     def __init__(self, ibc, iface, kx1, kx2, ky1, ky2, iheat=0, viscous=1, tsurf=0.0,
-                 uwall=None, vwall=None, wwall=None):
+                 uwall=None, vwall=None, wwall=None, inlout=0, boun=None):
         self.ibc, self.iface = ibc, iface
         self.kx1, self.kx2, self.ky1, self.ky2 = kx1, kx2, ky1, ky2
         self.iheat, self.viscous, self.tsurf = iheat, viscous, tsurf
         self.uwall, self.vwall, self.wwall = uwall, vwall, wwall
         self.size = (kx2 - kx1 + 5) * (ky2 - ky1 + 5)
+        self.inlout = inlout
+        self.boun = {n: None for n in BOUN_ARRAYS}
+        for n, a in (boun or {}).items():
+            if n not in self.boun:
+                raise ValueError(f"unknown patch array {n}")
+            self.boun[n] = None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+    def copy(self):
+        """The same patch with its own copies of the arrays."""
+        c = BcPatch(self.ibc, self.iface, self.kx1, self.kx2, self.ky1, self.ky2, self.iheat,
+                    self.viscous, self.tsurf, self.uwall, self.vwall, self.wwall, self.inlout,
+                    {n: (None if a is None else a.copy()) for n, a in self.boun.items()})
+        return c
 
     def view(self):
         v = BcPatchView()
@@ -246,6 +268,10 @@ class BcPatch:
         v.iheat, v.viscous, v.tsurf = self.iheat, self.viscous, self.tsurf
         for n in ("uwall", "vwall", "wwall"):
             a = getattr(self, n)
+            setattr(v, n, None if a is None else a.ctypes.data)
+        v.inlout = self.inlout
+        for n in BOUN_ARRAYS:
+            a = self.boun[n]
             setattr(v, n, None if a is None else a.ctypes.data)
         return v
 

@@ -305,8 +305,9 @@ int  ff_cycle(ff_ctx *ctx, int block, int ncycles, double *resi);
 /* ---- boundary patches: REFLEC (turb3.f:5294-5878) and GHOSTEXT (ns3c.f:10979-11226) ---- */
 /* One boundary patch of a block level, in the reference's ICON terms. */
 typedef struct ff_bc_patch {
-    int ibc;        /* ICON(1,L): 2 free stream (EXT: GHOSTEXT), 7 symmetry plane or
-                       singularity, 8-10 solid wall (REFLEC) */
+    int ibc;        /* ICON(1,L): 2 free stream (EXT: GHOSTEXT), 3 inlet (INL: INLET), 5
+                       outlet (OUT: OUTLET), 7 symmetry plane or singularity, 8-10 solid
+                       wall (REFLEC) */
     int iface;      /* ICON(3,L): 1-3 the I-, J-, K-low face, 4-6 the high faces */
     int kx1, kx2, ky1, ky2;   /* ICON(4..7,L): J and K on an I-face, I and K on a J-face,
                                  I and J on a K-face */
@@ -320,6 +321,23 @@ typedef struct ff_bc_patch {
     const double *uwall, *vwall, *wwall;   /* viscous walls: the wall velocity per face,
                        (NA+4) x (NB+4) as the patch arrays of ff_wall_patch (host arrays,
                        copied at registration); NULL: a wall at rest */
+    /* INL and OUT patches (IBC 3 and 5) */
+    int inlout;     /* ICON(8,L), the INLOUT type (11-20 and 21-30 reduce by 10 and 20, as
+                       FLXCYC does, res3c.f:6647-6648).  Inlets (INLET base3c.f:9223-9589):
+                       4 subsonic (velocity, density and temperature given, pressure
+                       extrapolated), 5 supersonic (everything given).  Outlets (OUTLET
+                       base3c.f:9595-10216): 2 and 3 pressure given, velocity extrapolated
+                       on outflow and on inflow relaxed towards the given direction (0.7 of
+                       the old ghost value with 2, none with 3); 4 every variable
+                       extrapolated; 5 the x-momentum given (the face value BOUNR BOUNU),
+                       the rest extrapolated */
+    int reserved_i[3];
+    /* the patch arrays BOUNR, BOUNU, BOUNV, BOUNW, BOUNT, BOUNP, BOUNPD, BOUNRK, BOUNEP,
+       (NA+4) x (NB+4) as uwall (host arrays, copied at registration).  Each INLOUT type
+       needs the ones it reads: inlet 4 R U V W T, inlet 5 R U V W T P PD, outlets 2 and 3
+       R U V W T P PD, outlet 4 none, outlet 5 R U P PD; inlets with the k-omega model RK
+       EP.  The others may be NULL.  INLET 4 writes BOUNP and BOUNPD (ff_bc_read). */
+    const double *bounr, *bounu, *bounv, *bounw, *bount, *bounp, *bounpd, *bounrk, *bounep;
 } ff_bc_patch;
 /* Register (or replace, or with n = 0 remove) the boundary patches of block N on level M.
  * From then on ff_state ends with REFLEC over its IBC 7-10 patches, as every STATIM of
@@ -331,6 +349,27 @@ typedef struct ff_bc_patch {
  * (loop 1150; as in the reference a K-face takes the I-face normal there, turb3.f:5687-
  * 5707).  REFLECL = .FALSE., a perfect gas (ISTATE /= 10), no DDEPS. */
 int  ff_bc_register(ff_ctx *ctx, int level, int block, const ff_bc_patch *p, int n);
+/* With IBC 3 and 5 patches registered on level 1, every level-1 residual (ff_residual,
+ * and the ones inside ff_cycle, ff_vcycle, the graphs and the multi-block cycles) first
+ * updates their ghost cells by INLET/OUTLET, as FLXCYC and FLUCYC do at M = 1 before a
+ * direction's fluxes (res3c.f:6639-6735, 7724-7830): after the gradients, CROSKO and
+ * FUN1KO, before the face fluxes.  Single phase, ISTATE = 1, a grid at rest (WROT = 0), no
+ * scalars, no transition model; not with the EARSM (ISTRES = 1: FF_E_UNSUP).  The laminar
+ * residual passes TURLIM = 1e-6 as FLUCYC does (res3c.f:7750), the k-omega one
+ * ff_params.turlim.  INLET's ICYCLE = 1 branch (the ghost density from BOUNR) is taken
+ * while the block's cycle count is 1: registration sets it to 0, and the cycles above add
+ * one at their start (ff_bc_cycle sets it, e.g. for a restart). */
+/* INLET/OUTLET alone over the registered IBC 3 and 5 patches of block N on level M, with
+ * the block's current cycle count (FLXCYC's update, res3c.f:6639-6735). */
+int  ff_inlout(ff_ctx *ctx, int level, int block);
+/* Set block N's cycle count ICYCLE on level M (0 after ff_bc_register); a cycle adds one at
+ * its start, so the next cycle runs as ICYCLE = icycle + 1.  No counterpart in the
+ * reference: CYCLES keeps ICYCLE itself (ns3c.f:1400). */
+int  ff_bc_cycle(ff_ctx *ctx, int level, int block, int icycle);
+/* Copy patch `patch` (0-based, the order of ff_bc_register) of block N on level M back:
+ * its device BOUNP and BOUNPD ((NA+4) x (NB+4) each; INLET 4 writes the extrapolated
+ * pressure there, base3c.f:9404-9405).  Either pointer may be NULL. */
+int  ff_bc_read(ff_ctx *ctx, int level, int block, int patch, double *bounp, double *bounpd);
 /* REFLEC alone over the registered IBC 7-10 patches (CYCLES' own call, ns3c.f:2168). */
 int  ff_reflec(ff_ctx *ctx, int level, int block);
 /* GHOSTEXT over the registered IBC 2 patches: both ghost layers take the free stream at

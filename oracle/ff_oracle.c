@@ -1667,6 +1667,8 @@ void ffo_turb(const ff_params *p, ff_block *b)
  * and SOURKO/ADDSKE after (res3c.f:4957-5068); then RESL2 sums on level 1
  * (ns3c.f:2668-2674, res3c.f:8964-9006).  Coarse levels are first order (INTI = 55,
  * ns3c.f:2467-2472); with M2 >= 2 RESIKE forms the velocity gradients itself (res3c.f:3749). */
+static void inlout_dir(const ff_params *p, ff_block *b, int level, int dir, double turlim);
+
 void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *resi)
 {
     dims_t d = mkdims(b);
@@ -1707,8 +1709,14 @@ void ffo_residual(const ff_params *p0, ff_block *b, int level, int m2, double *r
     }
     if (level == 1 && level <= FFO_MAXLEV)     /* RESIDI zeroes the force sums (ns3c.f:2376) */
         memset(g_walls[level].forces, 0, sizeof(g_walls[level].forces));
+    /* each FLUCYC/FLXCYC call starts, on M = 1, with INLET/OUTLET on the direction's INL
+     * and OUT patches (res3c.f:6639-6735, 7724-7830); FLUCYC passes TURLIM = 1E-6 */
+    const double tl = kwm ? p->turlim : 1.E-6;
+    if (level == 1) inlout_dir(p, b, level, 2, tl);
     direction_sweep(p, b, level, 2, 0, kwm ? sig : NULL);   /* res3c.f:270 / 4018, SWEEP(K) */
+    if (level == 1) inlout_dir(p, b, level, 0, tl);
     direction_sweep(p, b, level, 0, 0, kwm ? sig : NULL);   /* res3c.f:402 / 4274 */
+    if (level == 1) inlout_dir(p, b, level, 1, tl);
     direction_sweep(p, b, level, 1, 1, kwm ? sig : NULL);   /* res3c.f:515 / 4499 */
     if (kwm) {
         sourko(p, &s, b);
@@ -2914,9 +2922,13 @@ void ffo_prolong(const ff_params *p, const ff_block *cb, ff_block *fb)
  * the prolongation loop.  resi receives the RESL2 sums of the last level-1 residual. */
 void ffo_statim(const ff_params *p, ff_block *b, int level);
 
-/* CYCLES ns3c.f:1450-2137 with MGRID = nlev: every STATIM with REFLEL = .TRUE. */
+void ffo_bc_cycle(int level, int icycle);
+
+/* CYCLES ns3c.f:1450-2137 with MGRID = nlev: every STATIM with REFLEL = .TRUE.; the cycle
+ * count ICYCLE of level 1 grows by one at the start */
 void ffo_vcycle(const ff_params *p, ff_block **b, int nlev, double *resi)
 {
+    ffo_bc_cycle(1, -1);
     ffo_save_old(p, b[0]);
     ffo_statim(p, b[0], 1);
     ffo_turb(p, b[0]);
@@ -3091,12 +3103,25 @@ typedef struct {
 } bcs_t;
 static bcs_t g_bcs[FFO_MAXLEV + 1];
 
-/* the boundary patches of block level `level` (the caller keeps the array alive) */
+/* the cycle count ICYCLE of each level's block, read by INLET */
+static int g_icyc[FFO_MAXLEV + 1];
+
+/* the boundary patches of block level `level` (the caller keeps the array alive; INLET 4
+ * writes BOUNP and BOUNPD into the arrays of its patches); the cycle count restarts at 0 */
 void ffo_bc_set(int level, const ff_bc_patch *p, int n)
 {
     if (level < 1 || level > FFO_MAXLEV || n < 0) return;
     g_bcs[level].p = n > 0 ? p : NULL;
     g_bcs[level].n = n > 0 ? n : 0;
+    g_icyc[level] = 0;
+}
+
+/* set the cycle count of a level (icycle >= 0), or with icycle < 0 add one (the start of a
+ * cycle) */
+void ffo_bc_cycle(int level, int icycle)
+{
+    if (level < 1 || level > FFO_MAXLEV) return;
+    g_icyc[level] = icycle >= 0 ? icycle : g_icyc[level] + 1;
 }
 
 /* one patch's geometry in REFLEC's terms: direction, ILMO (IDIR), the face plane K, the
@@ -3133,6 +3158,21 @@ static void reflec_patch(const ff_params *p, ff_block *b, const ff_bc_patch *bc)
 {
     dims_t d = mkdims(b);
     const int ibc = bc->ibc;
+    if (ibc == 3) {
+        /* an inlet: its pressures extrapolated into both ghost layers (turb3.f:5493-5509) */
+        const bcgeo_t g = bc_geo(&d, bc);
+        double *P = F(b, FF_P), *PD = F(b, FF_PDIFF);
+        for (int J = g.ky1; J <= g.ky2; J++)
+            for (int I = g.kx1; I <= g.kx2; I++) {
+                const long L = bc_cell(&d, &g, I, J);
+                const long LB = L - g.idir * g.kstr, LC = L - 2 * g.idir * g.kstr;
+                P[LB] = P[L];
+                PD[LB] = PD[L];
+                P[LC] = P[L];
+                PD[LC] = PD[L];
+            }
+        return;
+    }
     if (ibc < 7 || ibc > 10) return;
     const bcgeo_t g = bc_geo(&d, bc);
     const int LN = 2;
@@ -3315,4 +3355,309 @@ void ffo_ghostext(const ff_params *p, ff_block *b, int level, double alpha, doub
                     }
                 }
     }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* INLET base3c.f:9223-9589 and OUTLET base3c.f:9595-10216 at M = 1                      */
+/* ------------------------------------------------------------------------------------ */
+
+/* The INLOUT type of a patch as FLXCYC reduces ICON(8,L) (res3c.f:6647-6648) */
+static int inlout_type(int icon8)
+{
+    int t = icon8;
+    if (t > 10 && t <= 20) t = t - 10;
+    if (t > 20 && t <= 30) t = t - 20;
+    return t;
+}
+
+/* One INL (IBC 3) or OUT (IBC 5) patch, in FLXCYC's frame: the loops J = KY1..KY2, I =
+ * KX1..KX2 (KX and KY swapped on J-faces, res3c.f:6671-6677), II the first interior cell,
+ * II2 the second, IM1 and IM2 the ghost cells, IF1 the boundary face, NP the patch-array
+ * entry with LN = 2.  Single phase (NPHASE = 1), ISTATE = 1 (EFPT: RGAS/GM1*T, EFPRO:
+ * (P + 0)/(GM1*RO) - E0REF with E0REF = 0), WROT = 0, FRESUL = TRANSL = .FALSE., NSCAL = 0,
+ * ISTRES = 0, IPRESC = 0, RELAX = RK1 = 1, RK2 = 0 (base3c.f:9282-9284, 9650-9652). */
+static void inlout_patch(const ff_params *p, ff_block *b, const ff_bc_patch *bc, int icycle,
+                         double turlim)
+{
+    dims_t d = mkdims(b);
+    bcgeo_t g = bc_geo(&d, bc);
+    const int type = inlout_type(bc->inlout);
+    const int kw = p->iturb >= 3 && p->iturb != 8;
+    const int dir = g.dir;
+    const long lstrid = g.kstr * g.idir;
+    const double *a3x = F(b, FF_A1X + 4 * dir), *a3y = F(b, FF_A1Y + 4 * dir),
+                 *a3z = F(b, FF_A1Z + 4 * dir);
+    double *P = F(b, FF_P), *PD = F(b, FF_PDIFF), *U = F(b, FF_U), *V = F(b, FF_V),
+           *W = F(b, FF_W), *E = F(b, FF_E), *RO = F(b, FF_RO), *RM = F(b, FF_RM),
+           *RN = F(b, FF_RN), *RW = F(b, FF_RW), *T = F(b, FF_TEMP), *EPS2 = F(b, FF_EPS2),
+           *VIST = F(b, FF_VIST), *VIS = F(b, FF_VIS), *RK = F(b, FF_RK), *REPS = F(b, FF_REPS);
+    const double *br = bc->bounr, *bu = bc->bounu, *bv = bc->bounv, *bw = bc->bounw,
+                 *bt = bc->bount, *brk = bc->bounrk, *bep = bc->bounep;
+    double *bp = (double *)bc->bounp, *bpd = (double *)bc->bounpd;
+    const double gm1 = p->gamma - 1.;
+    const double relax = 1.0, rk1 = 1., rk2 = 0.;
+    const double wrot = 0.;                 /* WROT(IF1): a grid at rest */
+    const double idir = (double)g.idir;
+    const int na = g.kx2 - g.kx1 + 1;
+    for (int J = g.ky1; J <= g.ky2; J++)
+        for (int I = g.kx1; I <= g.kx2; I++) {
+            const long ii = bc_cell(&d, &g, I, J);
+            const long ii2 = ii + lstrid, im1 = ii - lstrid, im2 = im1 - lstrid;
+            const long if1 = ii + (g.idir == 1 ? 0 : -lstrid);          /* II + NFL */
+            const long np = (long)(2 + J - g.ky1) * (na + 4) + (I - g.kx1 + 2);
+            if (bc->ibc == 3) {
+                if (type == 4) {
+                    /* subsonic inflow: the pressure extrapolated (base3c.f:9300-9425) */
+                    const double dpkoe = relax * (rk1 * P[ii] + rk2 * P[ii2]) + (1. - relax) * P[im1];
+                    const double dpdif = relax * (rk1 * PD[ii] + rk2 * PD[ii2]) + (1. - relax) * PD[im1];
+                    EPS2[im1] = rk1 * EPS2[ii] + rk2 * EPS2[ii2];
+                    EPS2[im2] = EPS2[im1];
+                    VIST[im1] = rk1 * VIST[ii] + rk2 * VIST[ii2];
+                    VIST[im2] = VIST[im1];
+                    RM[im1] = bu[np] * br[np];
+                    RN[im1] = bv[np] * br[np];
+                    RW[im1] = bw[np] * br[np];
+                    if (icycle == 1) RO[im1] = br[np];
+                    T[im1] = bt[np];
+                    T[im2] = T[im1];
+                    if (icycle >= 1) {
+                        const double roave = RO[im1];
+                        RM[im1] = RM[im1] + a3x[if1] * wrot * (roave - br[np]);
+                        RN[im1] = RN[im1] + a3y[if1] * wrot * (roave - br[np]);
+                        RW[im1] = RW[im1] + a3z[if1] * wrot * (roave - br[np]);
+                    }
+                    RM[im2] = RM[im1];
+                    RN[im2] = RN[im1];
+                    RW[im2] = RW[im1];
+                    P[im1] = dpkoe;
+                    P[im2] = P[im1];
+                    PD[im1] = dpdif;
+                    PD[im2] = PD[im1];
+                    bp[np] = dpkoe;
+                    bpd[np] = dpdif;
+                    E[im1] = p->rgas / gm1 * T[im1];      /* EFPT */
+                    E[im2] = E[im1];
+                    U[im1] = RM[im1] / RO[im1];
+                    U[im2] = RM[im2] / RO[im2];
+                    V[im1] = RN[im1] / RO[im1];
+                    V[im2] = RN[im2] / RO[im2];
+                    W[im1] = RW[im1] / RO[im1];
+                    W[im2] = RW[im2] / RO[im2];
+                    E[im1] = RO[im1] * (E[im1] + .5 * (U[im1] * U[im1] + V[im1] * V[im1] + W[im1] * W[im1]));
+                    E[im2] = RO[im2] * (E[im2] + .5 * (U[im2] * U[im2] + V[im2] * V[im2] + W[im2] * W[im2]));
+                } else {
+                    /* supersonic inflow: every variable given (base3c.f:9427-9526) */
+                    EPS2[im1] = rk1 * EPS2[ii] + rk2 * EPS2[ii2];
+                    EPS2[im2] = EPS2[im1];
+                    VIST[im1] = rk1 * VIST[ii] + rk2 * VIST[ii2];
+                    VIST[im2] = VIST[im1];
+                    P[im1] = bp[np];
+                    PD[im1] = bpd[np];
+                    P[im2] = P[im1];
+                    PD[im2] = PD[im1];
+                    RM[im1] = bu[np] * br[np];
+                    RN[im1] = bv[np] * br[np];
+                    RW[im1] = bw[np] * br[np];
+                    if (icycle == 1) RO[im1] = br[np];
+                    T[im1] = bt[np];
+                    T[im2] = T[im1];
+                    if (icycle >= 1) {
+                        const double roave = RO[im1];
+                        RM[im1] = RM[im1] + a3x[if1] * wrot * (roave - br[np]);
+                        RN[im1] = RN[im1] + a3y[if1] * wrot * (roave - br[np]);
+                        RW[im1] = RW[im1] + a3z[if1] * wrot * (roave - br[np]);
+                    }
+                    RM[im2] = RM[im1];
+                    RN[im2] = RN[im1];
+                    RW[im2] = RW[im1];
+                    E[im1] = p->rgas / gm1 * T[im1];
+                    E[im2] = E[im1];
+                    U[im1] = RM[im1] / br[np];
+                    U[im2] = RM[im2] / br[np];
+                    V[im1] = RN[im1] / br[np];
+                    V[im2] = RN[im2] / br[np];
+                    W[im1] = RW[im1] / br[np];
+                    W[im2] = RW[im2] / br[np];
+                    E[im1] = RO[im1] * (E[im1] + .5 * (U[im1] * U[im1] + V[im1] * V[im1] + W[im1] * W[im1]));
+                    E[im2] = RO[im2] * (E[im2] + .5 * (U[im2] * U[im2] + V[im2] * V[im2] + W[im2] * W[im2]));
+                }
+                if (kw) {
+                    /* base3c.f:9535-9545 */
+                    RK[im1] = brk[np];
+                    RK[im2] = RK[im1];
+                    REPS[im1] = bep[np];
+                    REPS[im2] = REPS[im1];
+                    E[im1] = E[im1] + RK[im1];
+                    E[im2] = E[im2] + RK[im2];
+                }
+                continue;
+            }
+            /* OUTLET */
+            const double a11 = a3x[if1], a12 = a3y[if1], a13 = a3z[if1];
+            double a21, a22, a23, a31, a32, a33;
+            surfvects(a11, a12, a13, &a21, &a22, &a23, &a31, &a32, &a33);
+            double veloc;
+            if (type == 2 || type == 3) {
+                /* pressure given, velocity extrapolated (base3c.f:9694-9892) */
+                const double rk3 = type == 2 ? 0.7 : 1., rk4 = type == 2 ? 0.3 : 0.;
+                const double uim1 = bu[np], vim1 = bv[np], wim1 = bw[np];
+                veloc = .5 * (a11 * (U[ii] + uim1) + a12 * (V[ii] + vim1) + a13 * (W[ii] + wim1));
+                const double ploss3 = 0., ploss4 = 0.;
+                P[im1] = rk1 * (bp[np] - ploss3) + rk2 * (bp[np] - ploss4);
+                P[im2] = P[im1];
+                PD[im1] = rk1 * (bpd[np] - ploss3) + rk2 * (bpd[np] - ploss4);
+                PD[im2] = PD[im1];
+                if (-veloc * idir > 0.) {
+                    U[im1] = (rk1 * RO[ii] * U[ii] + rk2 * RO[ii2] * U[ii2]) / RO[im1];
+                    U[im2] = U[im1];
+                    V[im1] = rk1 * V[ii] + rk2 * V[ii2];
+                    V[im2] = V[im1];
+                    W[im1] = rk1 * W[ii] + rk2 * W[ii2];
+                    W[im2] = W[im1];
+                    EPS2[im2] = rk1 * EPS2[ii] + rk2 * EPS2[ii2];
+                    VIST[im2] = rk1 * VIST[ii] + rk2 * VIST[ii2];
+                    EPS2[im2] = dmin(EPS2[im2], turlim);
+                    VIST[im2] = dmin(VIST[im2], turlim * VIS[im2]);
+                    EPS2[im1] = EPS2[im2];
+                    VIST[im1] = VIST[im2];
+                    T[im1] = rk1 * T[ii] + rk2 * T[ii2];
+                    T[im2] = T[im1];
+                } else {
+                    const double ukm1 = rk1 * U[ii] + rk2 * U[ii2];
+                    const double vkm1 = rk1 * V[ii] + rk2 * V[ii2];
+                    const double wkm1 = rk1 * W[ii] + rk2 * W[ii2];
+                    const double velc1 = a11 * ukm1 + a12 * vkm1 + a13 * wkm1;
+                    const double ukp1 = bu[np], vkp1 = bv[np], wkp1 = bw[np];
+                    const double velc2 = a21 * ukp1 + a22 * vkp1 + a23 * wkp1;
+                    const double velc3 = a31 * ukp1 + a32 * vkp1 + a33 * wkp1;
+                    const double uu = a11 * velc1 + a21 * velc2 + a31 * velc3;
+                    const double vv = a12 * velc1 + a22 * velc2 + a32 * velc3;
+                    const double ww = a13 * velc1 + a23 * velc2 + a33 * velc3;
+                    U[im1] = rk3 * U[im1] + rk4 * uu;
+                    U[im2] = U[im1];
+                    V[im1] = rk3 * V[im1] + rk4 * vv;
+                    V[im2] = V[im1];
+                    W[im1] = rk3 * W[im1] + rk4 * ww;
+                    W[im2] = W[im1];
+                    EPS2[im2] = rk1 * EPS2[ii] + rk2 * EPS2[ii2];
+                    VIST[im2] = rk1 * VIST[ii] + rk2 * VIST[ii2];
+                    EPS2[im2] = dmin(EPS2[im2], turlim);
+                    VIST[im2] = dmin(VIST[im2], turlim * VIS[im2]);
+                    EPS2[im1] = EPS2[im2];
+                    VIST[im1] = VIST[im2];
+                    T[im1] = bt[np];
+                    T[im2] = T[im1];
+                    RO[im1] = br[np];
+                    RO[im2] = RO[im1];
+                }
+                RM[im1] = RO[im1] * U[im1];
+                RM[im2] = RO[im2] * U[im2];
+                RN[im1] = RO[im1] * V[im1];
+                RN[im2] = RO[im2] * V[im2];
+                RW[im1] = RO[im1] * W[im1];
+                RW[im2] = RO[im2] * W[im2];
+            } else if (type == 4) {
+                /* every variable extrapolated (base3c.f:9894-9968) */
+                veloc = a11 * U[ii] + a12 * V[ii] + a13 * W[ii];
+                P[im1] = rk1 * P[ii] + rk2 * P[ii2];
+                P[im2] = rk1 * P[im1] + rk2 * P[ii];
+                PD[im1] = rk1 * PD[ii] + rk2 * PD[ii2];
+                PD[im2] = rk1 * PD[im1] + rk2 * PD[ii];
+                T[im1] = rk1 * T[ii] + rk2 * T[ii2];
+                T[im2] = rk1 * T[im1] + rk2 * T[ii];
+                EPS2[im1] = rk1 * EPS2[ii] + rk2 * EPS2[ii2];
+                VIST[im1] = rk1 * VIST[ii] + rk2 * VIST[ii2];
+                EPS2[im1] = dmin(EPS2[im1], turlim);
+                VIST[im1] = dmin(VIST[im1], turlim * VIS[im1]);
+                EPS2[im2] = EPS2[im1];
+                VIST[im2] = VIST[im1];
+                RO[im1] = rk1 * RO[ii] + rk2 * RO[ii2];
+                RO[im2] = rk1 * RO[im1] + rk2 * RO[ii];
+                RM[im1] = rk1 * RM[ii] + rk2 * RM[ii2];
+                RM[im2] = rk1 * RM[im1] + rk2 * RM[ii];
+                RN[im1] = rk1 * RN[ii] + rk2 * RN[ii2];
+                RN[im2] = rk1 * RN[im1] + rk2 * RN[ii];
+                RW[im1] = rk1 * RW[ii] + rk2 * RW[ii2];
+                RW[im2] = rk1 * RW[im1] + rk2 * RW[ii];
+                U[im1] = RM[im1] / RO[im1];
+                U[im2] = RM[im2] / RO[im2];
+                V[im1] = RN[im1] / RO[im1];
+                V[im2] = RN[im2] / RO[im2];
+                W[im1] = RW[im1] / RO[im1];
+                W[im2] = RW[im2] / RO[im2];
+                double e1 = (P[im1] + 0.) / (gm1 * RO[im1]) - 0.;        /* EFPRO */
+                E[im1] = RO[im1] * (e1 + .5 * (U[im1] * U[im1] + V[im1] * V[im1] + W[im1] * W[im1]));
+                e1 = (P[im2] + 0.) / (gm1 * RO[im2]) - 0.;
+                E[im2] = RO[im2] * (e1 + .5 * (U[im2] * U[im2] + V[im2] * V[im2] + W[im2] * W[im2]));
+            } else {
+                /* type 5: the x-momentum given (base3c.f:9970-10017) */
+                veloc = a11 * U[im2] + a12 * V[im2] + a13 * W[im2];
+                const double pefec = bp[np];
+                P[im2] = 2. * pefec - P[ii];
+                P[im1] = P[im2];
+                PD[im2] = 2. * bpd[np] - PD[ii];
+                PD[im1] = PD[im2];
+                EPS2[im2] = rk1 * EPS2[ii] + rk2 * EPS2[ii2];
+                VIST[im2] = rk1 * VIST[ii] + rk2 * VIST[ii2];
+                EPS2[im2] = dmin(EPS2[im2], turlim);
+                VIST[im2] = dmin(VIST[im2], turlim * VIS[ii]);
+                EPS2[im1] = EPS2[im2];
+                VIST[im1] = VIST[im2];
+                P[im1] = rk1 * P[ii] + rk2 * P[ii2];
+                P[im2] = P[im1];
+                PD[im1] = rk1 * PD[ii] + rk2 * PD[ii2];
+                PD[im2] = PD[im1];
+                RO[im1] = rk1 * RO[ii] + rk2 * RO[ii2];
+                RO[im2] = RO[im1];
+                RM[im1] = 2. * br[np] * bu[np] - RM[ii];
+                RM[im2] = 2. * RM[im1] - RM[ii];
+                U[im1] = RM[im1] / RO[im1];
+                U[im2] = RM[im2] / RO[im2];
+                RN[im1] = rk1 * RN[ii] + rk2 * RN[ii2];
+                RN[im2] = RN[im1];
+                V[im1] = RN[im1] / RO[im1];
+                V[im2] = RN[im2] / RO[im2];
+                RW[im1] = rk1 * RW[ii] + rk2 * RW[ii2];
+                RW[im2] = RW[im1];
+                W[im1] = RW[im1] / RO[im1];
+                W[im2] = RW[im2] / RO[im2];
+                E[im1] = (P[im1] + 0.) / (gm1 * RO[im1]) - 0.;          /* EFPRO */
+                /* CONSVE base3c.f:668-701 on the one cell */
+                const double rkin = (U[im1] * U[im1] + V[im1] * V[im1] + W[im1] * W[im1]) * .5;
+                const double epot = 0.;
+                if (!kw) E[im1] = (E[im1] + rkin + epot) * RO[im1];
+                else E[im1] = (E[im1] + rkin + epot) * RO[im1] + RK[im1];
+                E[im2] = E[im1];
+            }
+            if (kw && -veloc * idir > 0.) {
+                /* base3c.f:10135-10150 */
+                RK[im1] = rk1 * RK[ii] + rk2 * RK[ii2];
+                RK[im2] = RK[im1];
+                REPS[im1] = rk1 * REPS[ii] + rk2 * REPS[ii2];
+                REPS[im2] = REPS[im1];
+                E[im1] = E[im1] + RK[im1];
+                E[im2] = E[im2] + RK[im2];
+            }
+        }
+}
+
+/* FLXCYC/FLUCYC's update of the direction's INL and OUT patches, in patch order */
+static void inlout_dir(const ff_params *p, ff_block *b, int level, int dir, double turlim)
+{
+    if (level < 1 || level > FFO_MAXLEV) return;
+    for (int l = 0; l < g_bcs[level].n; l++) {
+        const ff_bc_patch *bc = &g_bcs[level].p[l];
+        if ((bc->ibc != 3 && bc->ibc != 5) || (bc->iface - 1) % 3 != dir) continue;
+        inlout_patch(p, b, bc, g_icyc[level], turlim);
+    }
+}
+
+/* INLET/OUTLET alone over every direction's INL and OUT patches (K, I, J, as RESIDU's
+ * FLUCYC calls), with the level's cycle count and the residual's TURLIM */
+void ffo_inlout(const ff_params *p, ff_block *b, int level)
+{
+    const double tl = p->iturb == 6 ? p->turlim : 1.E-6;
+    inlout_dir(p, b, level, 2, tl);
+    inlout_dir(p, b, level, 0, tl);
+    inlout_dir(p, b, level, 1, tl);
 }

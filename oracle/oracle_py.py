@@ -68,6 +68,10 @@ def oracle():
         _ora.ffo_ghostext.restype = None
         _ora.ffo_ddadi_dir.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
         _ora.ffo_ddadi_dir.restype = None
+        _ora.ffo_inlout.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _ora.ffo_inlout.restype = None
+        _ora.ffo_bc_cycle.argtypes = [C.c_int, C.c_int]
+        _ora.ffo_bc_cycle.restype = None
     return _ora
 
 
@@ -97,6 +101,10 @@ def ref():
         _ref.ref_ddadi_dir.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                        C.c_int]
         _ref.ref_ddadi_dir.restype = None
+        _ref.ref_bc_set.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        _ref.ref_bc_set.restype = None
+        _ref.ref_inlout.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        _ref.ref_inlout.restype = None
     return _ref
 
 
@@ -125,6 +133,14 @@ class OracleBlock:
 
     def reflec(self):
         oracle().ffo_reflec(*self._args(), self.level)
+
+    def inlout(self):
+        """INLET/OUTLET over the INL and OUT patches with the level's cycle count."""
+        oracle().ffo_inlout(*self._args(), self.level)
+
+    def bc_cycle(self, icycle=-1):
+        """Set the level's cycle count ICYCLE (icycle < 0: add one, as a cycle starts)."""
+        oracle().ffo_bc_cycle(self.level, icycle)
 
     def ghostext(self, alpha=0.0, beta=0.0):
         oracle().ffo_ghostext(*self._args(), self.level, alpha, beta)
@@ -156,6 +172,7 @@ class OracleBlock:
         oracle().ffo_forcing(*self._args(), phase)
 
     def cycle(self):
+        self.bc_cycle()
         self.save_old()
         self.state()
         self.turb()
@@ -235,12 +252,13 @@ class RefBlock:
         inter = list(prm.inter) if level == 1 else [55, 55, 55]
         self.iv = np.array([blk.imax, blk.jmax, blk.kmax, prm.iturb, prm.iflux, prm.idiff,
                             inter[0], inter[1], inter[2], level, prm.kover, prm.katol, 1,
-                            prm.istres, prm.fullns, prm.turcor, prm.ddadi],
+                            prm.istres, prm.fullns, prm.turcor, prm.ddadi, 0],
                            dtype=np.int32)
         # the reference driver keeps wall patches per level: a new block starts without any
         if have_ref():
             z = np.zeros(32)
             ref().ref_wall_set(level, 0, z.ctypes.data, z.ctypes.data, z.ctypes.data)
+            ref().ref_bc_set(level, 0, z.ctypes.data, z.ctypes.data)
 
     def _a(self):
         return (self.blk.F.ctypes.data, self.blk.ntot, self.pv.ctypes.data,
@@ -253,7 +271,34 @@ class RefBlock:
             self.reflec(self._bcs)
 
     def set_bcs(self, patches):
+        """REFLEC's patches for STATIM, and the INL/OUT patches for the reference's INLET
+        and OUTLET (its ICYCLE restarts at 0; INLET 4 writes BOUNP/BOUNPD into these
+        patches' arrays)."""
+        from finflo_amd.layout import BOUN_ARRAYS
         self._bcs = list(patches)
+        io = [bp for bp in patches if bp.ibc in (3, 5)]
+        self._io = io
+        n = len(io)
+        self._bints = np.zeros((max(n, 1), 8), dtype=np.int32)
+        self._bptrs = np.zeros((max(n, 1), len(BOUN_ARRAYS)), dtype=np.uint64)
+        self._bkeep = []
+        for l, bp in enumerate(io):
+            self._bints[l, :7] = [bp.ibc, bp.iface, bp.kx1, bp.kx2, bp.ky1, bp.ky2, bp.inlout]
+            for q, name in enumerate(BOUN_ARRAYS):
+                a = bp.boun[name]
+                if a is None:
+                    a = np.zeros(bp.size)
+                    self._bkeep.append(a)
+                self._bptrs[l, q] = a.ctypes.data
+        self.iv[17] = 0
+        ref().ref_bc_set(self.level, n, self._bints.ctypes.data, self._bptrs.ctypes.data)
+
+    def inlout(self):
+        """The reference's INLET/OUTLET over the INL/OUT patches (directions K, I, J)."""
+        ref().ref_inlout(*self._a())
+
+    def bc_cycle(self, icycle):
+        self.iv[17] = icycle
 
     def turb(self):
         ref().ref_turb(*self._a())
@@ -281,6 +326,7 @@ class RefBlock:
         ref().ref_update(*self._a())
 
     def cycle(self):
+        self.iv[17] += 1
         self.save_old()
         self.state()
         self.turb()

@@ -54,7 +54,7 @@ module ffref_ids
   ! integer vector IPR
   integer, parameter :: QIMAX=1, QJMAX=2, QKMAX=3, QITURB=4, QIFLUX=5, QIDIFF=6
   integer, parameter :: QINTI=7, QINTJ=8, QINTK=9, QLEVEL=10, QKOVER=11, QKATOL=12, QM2=13
-  integer, parameter :: QISTRES=14, QFULLNS=15, QTURCOR=16, QDDADI=17
+  integer, parameter :: QISTRES=14, QFULLNS=15, QTURCOR=16, QDDADI=17, QICYCLE=18
 end module ffref_ids
 
 ! Solid-wall patches per level, set by ref_wall_set: ICON(3..7,L) and ICON(20,L), the
@@ -70,6 +70,154 @@ module ffref_walls
   real(c_double) :: wenv(16, MAXL) = 0.
   real(c_double) :: wforce(19, MAXP, MAXL) = 0.
 end module ffref_walls
+
+! INL and OUT patches per level, set by ref_bc_set: ICON(1,L), ICON(3..7,L), ICON(8,L) and
+! the patch arrays BOUNR, BOUNU, BOUNV, BOUNW, BOUNT, BOUNP, BOUNPD, BOUNRK, BOUNEP
+module ffref_bcs
+  use iso_c_binding
+  implicit none
+  integer, parameter :: MAXB = 64, NBA = 9, MAXBL = 8
+  integer :: nbc(MAXBL) = 0
+  integer :: bint(8, MAXB, MAXBL) = 0
+  type(c_ptr) :: bptr(NBA, MAXB, MAXBL)
+end module ffref_bcs
+
+subroutine ref_bc_set(level, n, ints, ptrs) bind(C, name="ref_bc_set")
+  use iso_c_binding
+  use ffref_bcs
+  implicit none
+  integer(c_int), value :: level, n
+  integer(c_int) :: ints(8, *)
+  type(c_ptr) :: ptrs(NBA, *)
+  integer :: l
+  if (level < 1 .or. level > MAXBL .or. n < 0 .or. n > MAXB) return
+  nbc(level) = n
+  do l = 1, n
+     bint(:, l, level) = ints(:, l)
+     bptr(:, l, level) = ptrs(:, l)
+  end do
+end subroutine ref_bc_set
+
+! The update of FLXCYC (res3c.f:6639-6735) or FLUCYC (res3c.f:7724-7830) at M = 1 for
+! direction IDIR: INLET (ICON(1,L) = 3, IBTYPE <= 6) and OUTLET (5, IBTYPE <= 5) on the
+! direction's patches in patch order, with FLXCYC's rotated strides (res3c.f:4018, 4274,
+! 4499), KX/KY swapped on J-faces, KBOTOP = 1 or IMAXRE/JMAXRE/KMAXRE, ILMO = +-1,
+! ICYCLE = IPR(QICYCLE); FLUCYC's TURLIM = 1E-6 and ISTRES = 0 (res3c.f:7750, 7797) for
+! the laminar residual.  A grid at rest (WROT = 0), one phase, ISTATE = 1, E0REF = T0REF = 0,
+! no scalars, FRESUL = TRANSL = .FALSE., SOLUTION_TYPE 'FLUID'.
+subroutine ref_inlout_dir(f, ntot, prm, ipr, idir)
+  use iso_c_binding
+  use ffref_ids
+  use ffref_bcs
+  use TYPE_ARRAYS
+  implicit none
+  integer :: ntot, idir
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  type(PROPERTIES) :: pro(1)
+  type(MPHASE_VARIABLES) :: var(1)
+  type(INTERMITTENCY) :: trm(1)
+  type(PRE_COR) :: prc(1)
+  type pa
+     real(c_double), pointer :: a(:)
+  end type pa
+  type(pa) :: w(NBA)
+  real(c_double), allocatable :: wrot(:), dum(:)
+  real(c_double) :: s11(1), fi(1), bij(1), turlim, bounfi(1), bounbi(1)
+  integer :: jstate(1), iwaveb(1)
+  integer :: lev, l, q, iface, pdir, kx1, kx2, ky1, ky2, ilmo, kbotop, npa, ityp, ibtype
+  integer :: imax, jmax, kmax, is, js, istr, jstr, kstr, ia, iax, iay, iaz, nd(3)
+  lev = ipr(QLEVEL)
+  if (lev /= 1) return
+  if (nbc(lev) == 0) return
+  imax = ipr(QIMAX); jmax = ipr(QJMAX); kmax = ipr(QKMAX)
+  nd = (/ imax, jmax, kmax /)
+  is = imax + 4; js = jmax + 4
+  select case (idir)
+  case (1)
+     istr = is; jstr = is*js; kstr = 1; ia = IA1
+  case (2)
+     istr = is*js; jstr = 1; kstr = is; ia = IA2
+  case default
+     istr = 1; jstr = is; kstr = is*js; ia = IA3
+  end select
+  iax = ia + 1; iay = ia + 2; iaz = ia + 3
+  if (ipr(QITURB) == 6) then
+     turlim = prm(PTURLIM)
+  else
+     turlim = 1.E-6_c_double
+  end if
+  allocate(wrot(ntot))
+  wrot = 0.
+  s11 = 0.; fi = 0.; bij = 0.; bounfi = 0.; bounbi = 0.; jstate = 0; iwaveb = 0
+  do l = 1, nbc(lev)
+     ityp = bint(1, l, lev)
+     iface = bint(2, l, lev)
+     ibtype = bint(7, l, lev)
+     if (ibtype > 10 .and. ibtype <= 20) ibtype = ibtype - 10
+     if (ibtype > 20 .and. ibtype <= 30) ibtype = ibtype - 20
+     pdir = iface
+     if (iface > 3) pdir = iface - 3
+     if (pdir /= idir) cycle
+     if (.not. ((ityp == 3 .and. ibtype <= 6) .or. (ityp == 5 .and. ibtype <= 5))) cycle
+     kx1 = bint(3, l, lev); kx2 = bint(4, l, lev); ky1 = bint(5, l, lev); ky2 = bint(6, l, lev)
+     if (idir == 2) then
+        ky1 = bint(3, l, lev); ky2 = bint(4, l, lev); kx1 = bint(5, l, lev); kx2 = bint(6, l, lev)
+     end if
+     if (iface <= 3) then
+        ilmo = 1; kbotop = 1
+     else
+        ilmo = -1; kbotop = nd(idir)
+     end if
+     npa = (kx2 - kx1 + 5)*(ky2 - ky1 + 5)
+     do q = 1, NBA
+        call c_f_pointer(bptr(q, l, lev), w(q)%a, [npa])
+     end do
+     allocate(dum(npa))
+     dum = 0.
+     if (ityp == 3) then
+        call INLET(f(1,IP), f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), f(1,IE), &
+             f(1,IVOL), f(1,ia), f(1,iax), f(1,iay), f(1,iaz), f(1,IXC), f(1,IYC), f(1,IZC), &
+             f(1,IOHMI), s11, f(1,IVIS), f(1,IEPS2), f(1,IVIST), is, js, kmax + 4, 0, kbotop, &
+             kx1, kx2, ky1, ky2, f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), &
+             wrot, f(1,ITEMP), f(1,ICH), f(1,ICP), prm(PPRT), ipr(QIFLUX), 1, ipr(QM2), 2, 2, &
+             2, prm(PFRSDEN), prm(PFRSPRE), prm(PFRSVEL), prm(PFRSVIS), 0._c_double, &
+             0._c_double, ipr(QITURB), istr, jstr, kstr, 0, ipr(QICYCLE), 1, ilmo, 1, 1, 1, &
+             1, ibtype, prm(PRGAS), prm(PGAMMA), 0._c_double, 0._c_double, fi, bij, &
+             w(1)%a, w(2)%a, w(3)%a, w(4)%a, dum, w(6)%a, w(7)%a, w(5)%a, w(8)%a, w(9)%a, &
+             bounfi, bounbi, dum, dum, 1, 1, 1, 1, 0, turlim, 0, jstate, pro, var, 1, &
+             iwaveb, .false., 0._c_double, dum, .false., trm, dum, dum, dum, dum, dum, &
+             dum, dum, dum, dum, 'FLUID', prc)
+     else
+        call OUTLET(f(1,IP), f(1,IPDIFF), f(1,IU), f(1,IV), f(1,IW), f(1,IC), f(1,IE), &
+             f(1,IVOL), f(1,ia), f(1,iax), f(1,iay), f(1,iaz), f(1,IXC), f(1,IYC), f(1,IZC), &
+             f(1,IOHMI), s11, f(1,IVIS), f(1,IEPS2), f(1,IVIST), is, js, kmax + 4, 0, kbotop, &
+             kx1, kx2, ky1, ky2, f(1,IRO), f(1,IRM), f(1,IRN), f(1,IRW), f(1,IRK), f(1,IREPS), &
+             wrot, f(1,ITEMP), f(1,ICH), f(1,ICP), prm(PPRT), ipr(QIFLUX), 1, ipr(QM2), 2, 2, &
+             2, prm(PFRSDEN), prm(PFRSPRE), prm(PFRSVEL), prm(PFRSVIS), 0._c_double, &
+             0._c_double, ipr(QITURB), istr, jstr, kstr, 0, ipr(QICYCLE), 1, ilmo, 1, 1, 1, &
+             1, ibtype, prm(PRGAS), prm(PGAMMA), 0._c_double, 0._c_double, fi, bij, &
+             w(1)%a, w(2)%a, w(3)%a, w(4)%a, dum, w(6)%a, w(7)%a, w(5)%a, w(8)%a, w(9)%a, &
+             bounfi, bounbi, dum, dum, 1, 1, 1, 1, 0, turlim, 0, jstate, pro, var, 1, &
+             iwaveb, .false., dum, .false., trm, dum, dum, dum, dum, dum, &
+             dum, dum, dum, dum, 'FLUID', 1, prc)
+     end if
+     deallocate(dum)
+  end do
+  deallocate(wrot)
+end subroutine ref_inlout_dir
+
+! INLET/OUTLET alone over the directions K, I, J (the order of RESIDU's flux cycles)
+subroutine ref_inlout(f, ntot, prm, ipr) bind(C, name="ref_inlout")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: ntot
+  real(c_double) :: f(ntot, *), prm(*)
+  integer(c_int) :: ipr(*)
+  call ref_inlout_dir(f, ntot, prm, ipr, 3)
+  call ref_inlout_dir(f, ntot, prm, ipr, 1)
+  call ref_inlout_dir(f, ntot, prm, ipr, 2)
+end subroutine ref_inlout
 
 subroutine ref_wall_set(level, n, ints, env, ptrs) bind(C, name="ref_wall_set")
   use iso_c_binding
@@ -313,6 +461,8 @@ subroutine ref_direction(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, hat,
      istr = is*js; id = ID3
   end select
   iax = ia + 1; iay = ia + 2; iaz = ia + 3
+  ! INLET/OUTLET at M = 1 (FLUCYC res3c.f:7724-7830)
+  call ref_inlout_dir(f, ntot, prm, ipr, idir)
   fl = 0.
   if (ipr(QFULLNS) /= 0 .and. ipr(QLEVEL) == 1) then
   ! FLUXBL (res3c.f:7839): full viscous flux from the cell gradients
@@ -485,6 +635,8 @@ subroutine ref_direction_kw(f, ntot, prm, ipr, idir, fl, zzz, maw, maxw, rksi, h
   intet = inter
   if (inter > -5 .and. inter < 5) intet = +1
   iax = ia + 1; iay = ia + 2; iaz = ia + 3
+  ! INLET/OUTLET at M = 1 (FLXCYC res3c.f:6639-6735)
+  call ref_inlout_dir(f, ntot, prm, ipr, idir)
   fl = 0.
   if (fullns) then
   ! FLKEBL (res3c.f:6808): k and omega diffusion from the cell gradients of k/rho, omega/rho
