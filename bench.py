@@ -36,6 +36,9 @@ WORKLOADS = {
     # laminar ideal gas on the finest level only (MGRID = 1)
     "laminar": dict(iturb=0, nlev=1, reynolds=5.0e6,
                     label="laminar ideal gas, finest level only"),
+    # k-omega SST on the finest level only (MGRID = 1)
+    "sst": dict(iturb=6, nlev=1, reynolds=5.0e6,
+                label="k-omega SST, finest level only"),
 }
 
 # Algorithmic (compulsory, unique) FP64 bytes per cell of each kernel: every field the
@@ -111,10 +114,10 @@ KERNELS = ("k_residual_kw", "k_residual_tile", "k_residual", "k_pred_tiles", "k_
 
 def make_workload(name="sst-mg", seed=0):
     """The flat plate of the workload and its no-slip wall (the J-low face of every level):
-    sst-mg is the 192x96x96 k-omega SST plate with 3 multigrid levels; laminar is the same
-    grid, laminar, finest level only."""
-    if name == "sst-mg":
-        return config2(seed=seed, nlev=3)
+    sst-mg is the 192x96x96 k-omega SST plate with 3 multigrid levels, sst the same on its
+    finest level only; laminar is the same grid, laminar, finest level only."""
+    if name in ("sst-mg", "sst"):
+        return config2(seed=seed, nlev=WORKLOADS[name]["nlev"])
     prm = flow_params(iturb=0, mach=MACH, reynolds=WORKLOADS[name]["reynolds"], cfl=5.0)
     blocks = flat_plate_hierarchy(IMAX, JMAX, KMAX, 1, prm, first=1e-6, ratio=1.15,
                                   seed=seed, amp=0.005)
@@ -156,7 +159,7 @@ def _cpu_worker(prm, blocks, walls, seconds, use_ref, q):
     q.put((n, el))
 
 
-def converge(ctx, nlev, ncell, max_cycles, chunk=10, periodic=False):
+def converge(ctx, nlev, ncell, max_cycles, chunk=10, periodic=False, dims=None):
     """V-cycles (iterations with one level) from the current device state until the density
     residual RESI(31) = sqrt(sum DRO^2)/N (SUBIN ns3c.f:10248) has fallen 8 decades below
     that of the first cycle, or max_cycles.  The residual is read every `chunk` cycles (one
@@ -184,9 +187,15 @@ def converge(ctx, nlev, ncell, max_cycles, chunk=10, periodic=False):
     el = time.perf_counter() - t0
     last = hist[-1][1]
     dm = ctx.dromax()
+    cell = None
+    if dims is not None:
+        # IXERR (1-based, two ghost layers) as the interior (i, j, k) of the largest change
+        isz, il = dims[0] + 4, (dims[0] + 4) * (dims[1] + 4)
+        q = dm[1] - 1
+        cell = [q % isz - 1, (q % il) // isz - 1, q // il - 1]
     return {"cycles": n, "seconds": el, "first_resi": r0, "last_resi": last,
             "decades": float(np.log10(r0 / last)) if last > 0 and np.isfinite(last) else None,
-            "time_to_8_decades": t8, "dromax_last_cycle": dm[0],
+            "time_to_8_decades": t8, "dromax_last_cycle": dm[0], "dromax_cell": cell,
             "history": hist[:: max(1, len(hist) // 40)] + [hist[-1]]}
 
 
@@ -243,7 +252,9 @@ def committed_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` on this workload from the committed counter summary
     (profiles/hbm_traffic_*.md: FETCH_SIZE and WRITE_SIZE passes of this bench, each scaled
     by what the counter reads for a copy kernel of known size), or None."""
-    tag = {"sst-mg": "sst_mg3", "laminar": "laminar"}[workload]
+    tag = {"sst-mg": "sst_mg3", "laminar": "laminar"}.get(workload)
+    if tag is None:
+        return None, None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"hbm_traffic_{tag}_{IMAX}x{JMAX}x{KMAX}.md")
     if not os.path.exists(path):
@@ -694,7 +705,8 @@ def main():
                               (6, 1, b0.imax, 1, b0.jmax, 1, 3, 1, 1, 0, 1, 1)])
             ctx.halo_exchange(what="geometry")
             ctx.halo_exchange(what="flow")
-        conv = converge(ctx, nlev, ncell, args.converge_cycles, periodic=args.kbc == "periodic")
+        conv = converge(ctx, nlev, ncell, args.converge_cycles, periodic=args.kbc == "periodic",
+                        dims=(IMAX, JMAX, KMAX))
         conv["k_faces"] = args.kbc
         conv["i_faces"] = "inlet 4 / outlet 3" if args.outer == "inout" else "ext"
         print(f"[bench] convergence: {conv['decades']} decades in {conv['cycles']} cycles, "

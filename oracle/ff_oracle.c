@@ -8,23 +8,32 @@
  * reference is FP64 throughout (-r8 / -fdefault-real-8, Makefile:63,73), so is this file.
  * Build with -ffp-contract=off (no FMA contraction), as the reference oracle build is.
  *
- * Scope restated here (laminar ideal gas, Roe flux, LU-SGS, single-phase, OMEGA=0, no
- * chimera, ENTROPY_FIX=NO, XXTRAL=NO, FULLNS=NO, IDI* = all cells viscous):
+ * Scope restated here (ideal gas ISTATE = 1, single phase, OMEGA = 0, no chimera,
+ * ENTROPY_FIX = NO, XXTRAL = NO, IDI* = all cells viscous):
  *   ffo_state     STATIM ns3c.f:10318 = PDTOP state.f:1690 + PERFGA state.f:159 + CONSVA
- *                 base3c.f:489
- *   ffo_turb      TURBDI ITURB=0 -> ADJUIN (ns3c.f:2987-2991)
- *   ffo_residual  RESIDU res3c.f:6-787 with FLUCYC res3c.f:7572 -> FLUXDO base3c.f:3677 and
- *                 FLUXKE base3c.f:4753 (XXTRAP base3c.f:750, EFPRO state.f:620, DERIV
- *                 base3c.f:707, SURFVECTS base3c.f:12245) or FLUXVL base3c.f:1733 (van
- *                 Leer, IFLUX = 0), SWEEP res3c.f:992, RESL2 res3c.f:8920
- *   ffo_implicit  IMPS imps3.f:6 = TIME3 base3c.f:79, LOCDT imps3.f:784, COPR imps3.f:891,
- *                 PREDGS imps4.f:345, DIAGGS imps4.f:604, CORRGS imps4.f:76 (SYI/SYM1I
- *                 imps4.f:10-70), DIVV12 by CP (imps3.f:483)
+ *                 base3c.f:489; ffo_statim adds REFLEC turb3.f:5294-5878 (ffo_reflec)
+ *   ffo_turb      TURBDI ns3c.f:2874: ITURB = 0 ADJUIN (ns3c.f:2987-2991); ITURB = 6 TURBEQ
+ *                 res3c.f:5845 (DERFUN turb3.f:2476, TTIMES, EARSM1 earsm.f with ISTRES = 1,
+ *                 EPS2KO turb3.f:7454)
+ *   ffo_residual  RESIDU res3c.f:6-787 / RESIKE res3c.f:3423 with FLUCYC/FLXCYC: INLET and
+ *                 OUTLET base3c.f:9223-10216 on level 1 (ffo_inlout), FLUXDO base3c.f:3677
+ *                 or FLUXBL (FULLNS), FLUXKE base3c.f:4753 (XXTRAP, EFPRO, DERIV, SURFVECTS)
+ *                 or FLUXVL base3c.f:1733, FLUDKE/FLKEBL, ANISFLUX, BOTSKE base3c.f:6147
+ *                 (ffo_wall_set), SWEEP(K), CROSKO, FUN1KO, PRODKE/PEEKOO, SOURKO, RESL2
+ *   ffo_implicit  IMPS imps3.f:6: TIME3, LOCDT(K), IMSOKO, COPR(KE), the LU-SGS PREDGS/
+ *                 DIAGGS/CORRGS (PRKEGS/DIKEGS/COKEGS) or with ff_params.ddadi the DDADI
+ *                 PRED/DIAG/CORR(KE) along K, I, J (ffo_ddadi_dir), DIVV12, MULV12
  *   ffo_update    FIN res3c.f:1077 + UPLIM/LOLIM/PDTOP/LOLIMP of CYCLES ns3c.f:1553-1596
+ *   ffo_restrict, ffo_forcing, ffo_prolong, ffo_vcycle   TRANSF/TRANKE, FORCFU/ADDRES,
+ *                 INTERP/INTERT of multi.f and the CYCLES V-cycle ns3c.f:1450-2137
+ *   ffo_ghostext  GHOSTEXT ns3c.f:10979-11226;  ffo_dromax  DROMXX res3c.f:9293-9324
  *
- * Parity pins: every function is compared with oracle/_ref (the reference's own Fortran
- * compiled from its sources, see oracle/Makefile) on seeded blocks, and with the
- * closed-form checks in tests/test_oracle_math.py.
+ * Parity pins: every function that restates a routine of the files oracle/Makefile
+ * compiles into oracle/_ref (the reference's own Fortran) is compared with it on seeded
+ * blocks, bit for bit; the multigrid transfers (multi.f does not build), GHOSTEXT (ns3c.f
+ * does not build) and DROMXX's scan are pinned by closed forms, invariants and the FAS
+ * definition (tests/test_oracle_mg.py, test_oracle_bc.py, test_oracle_dromax.py), and
+ * single routines also by tests/test_oracle_math.py.
  */
 #include <math.h>
 #include <string.h>
