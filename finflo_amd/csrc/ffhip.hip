@@ -1279,6 +1279,7 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev
 {
     __shared__ double fi[7][RJW][RIW + 1];
     __shared__ double fj[7][RJW + 1][RIW];
+    __shared__ double fk[7][RIW * RJW];
     const SstC sc = coefko(p.kover);
     const int ti = threadIdx.x % RIW, tj = threadIdx.x / RIW;
     const int tl = xcd_tile(tg);
@@ -1296,14 +1297,14 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev
     // One call site per face kind keeps the kernel's code within the instruction cache: the
     // plane loop starts one plane early to form the tile's lowest K-faces, and a plane's I-
     // and J-faces go through one rolled loop.
-    Flux7 klo, khi;
     for (int k = k0 - 1; k <= k1; k++) {
-        if (cell) {
-            khi = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
-            wall_fix<7>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), khi);
-        }
         if (k < k0) {
-            klo = khi;
+            if (cell) {
+                Flux7 kf = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+                wall_fix<7>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), kf);
+#pragma unroll
+                for (int q = 0; q < 7; q++) fk[q][threadIdx.x] = kf.f[q];
+            }
             continue;
         }
         const int n = cid(b, i, j, k);
@@ -1335,14 +1336,19 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev
         }
         __syncthreads();
         if (cell) {
+            // the cell's upper K-face after its I- and J-faces: the K-face of the plane below
+            // waits in LDS, so no flux is held in registers while the others are formed
+            Flux7 khi = face_flux_kw<true, FNS>(p, sc, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+            wall_fix<7>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), khi);
             double dt1 = 1. / FLD(b, FF_VOL)[n];
             double d[7];
 #pragma unroll
             for (int q = 0; q < 7; q++) {
                 double x = 0.;
-                x = x - (khi.f[q] - klo.f[q]);                          // SWEEPK, K
+                x = x - (khi.f[q] - fk[q][threadIdx.x]);                // SWEEPK, K
                 x = x - (fi[q][tj][ti + 1] - fi[q][tj][ti]);             // SWEEPK, I
                 d[q] = (x - (fj[q][tj + 1][ti] - fj[q][tj][ti])) * dt1;  // SWEEPK, J, /VOL
+                fk[q][threadIdx.x] = khi.f[q];
             }
             double srk, som;
             sourko_cell(p, b, n, srk, som);
@@ -1352,7 +1358,6 @@ __global__ void __launch_bounds__(RIW * RJW) k_residual_kw(Prm p, Blk b, WallDev
             d[6] = d[6] + som;
 #pragma unroll
             for (int q = 0; q < 7; q++) FLD(b, FF_DRO + q)[n] = d[q];
-            klo = khi;
         }
         __syncthreads();
     }
@@ -1429,6 +1434,7 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev
     static_assert(RJ >= 2 && RI >= RJ, "the tile's upper faces need two rows and RI >= RJ");
     __shared__ double fi[5][RJ][RI + 1];
     __shared__ double fj[5][RJ + 1][RI];
+    __shared__ double fk[5][RI * RJ];
     const int ti = threadIdx.x % RI, tj = threadIdx.x / RI;
     const int tl = xcd_tile(tg);
     if (tl < 0) return;
@@ -1443,15 +1449,17 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev
     const double *a2 = FLD(b, FF_A2), *a2x = FLD(b, FF_A2X), *a2y = FLD(b, FF_A2Y), *a2z = FLD(b, FF_A2Z);
     const double *d1 = FLD(b, FF_D1), *d2 = FLD(b, FF_D2), *d3 = FLD(b, FF_D3);
     // one call site per face kind (the instruction cache): the plane loop starts one plane
-    // early to form the tile's lowest K-faces; a plane's I- and J-faces share a rolled loop
-    Flux5 klo, khi;
+    // early to form the tile's lowest K-faces; a plane's I- and J-faces share a rolled loop.
+    // A cell's upper K-face is formed after its I- and J-faces and kept in LDS (fk) for the
+    // plane above, so no flux is held in registers while another is formed.
     for (int k = k0 - 1; k <= k1; k++) {
-        if (cell) {
-            khi = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
-            wall_fix<5>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), khi);
-        }
         if (k < k0) {
-            klo = khi;
+            if (cell) {
+                Flux5 kf = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+                wall_fix<5>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), kf);
+#pragma unroll
+                for (int q = 0; q < 5; q++) fk[q][threadIdx.x] = kf.f[q];
+            }
             continue;
         }
         const int n = cid(b, i, j, k);
@@ -1484,16 +1492,18 @@ __global__ void __launch_bounds__(RI * RJ) k_residual_tile(Prm p, Blk b, WallDev
         }
         __syncthreads();
         if (cell) {
+            Flux5 khi = face_flux<true, IFLUX, FNS>(p, b, cid(b, i, j, k + 1), b.il, p.inter[2], a3, a3x, a3y, a3z, d3);
+            wall_fix<5>(wd, 2, k + 1, b.kmax, wall_idx(b, 2, i, j, 0), khi);
             double dt1 = 1. / FLD(b, FF_VOL)[n];
 #pragma unroll
             for (int q = 0; q < 5; q++) {
                 double d = 0.;
-                d = d - (khi.f[q] - klo.f[q]);                          // SWEEP, K (res3c.f:330)
+                d = d - (khi.f[q] - fk[q][threadIdx.x]);                // SWEEP, K (res3c.f:330)
                 d = d - (fi[q][tj][ti + 1] - fi[q][tj][ti]);             // SWEEP, I (res3c.f:448)
                 d = (d - (fj[q][tj + 1][ti] - fj[q][tj][ti])) * dt1;     // SWEEP, J, /VOL (res3c.f:559)
                 FLD(b, FF_DRO + q)[n] = d;
+                fk[q][threadIdx.x] = khi.f[q];
             }
-            klo = khi;
         }
         __syncthreads();
     }
