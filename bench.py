@@ -297,10 +297,11 @@ def multiblock_bench(args):
     * config5: 64 blocks (about 40M cells) around a swept bump, k-omega SST with the EARSM,
       4 multigrid levels; the chosen form adds the learned closure, trained first on the
       exact EARSM samples of the first blocks.
-    Times K multi-block cycles: the plain FP64 path with eager launches, the same replayed
-    from the captured multi-block graph (parallel per-block branches), and the chosen form
-    (FP32 LU-SGS records, autotuned tile shapes, the graph where the library captures one;
-    with the closure on config5).
+    Times K multi-block cycles: the plain FP64 path with eager launches, one block after the
+    other; the same replayed from the captured multi-block graph (parallel per-block
+    branches, where the library captures one); the same with the blocks side by side on the
+    library's stream pool (FF_OPT_BLOCK_STREAMS); and the chosen form (FP32 LU-SGS records,
+    autotuned tile shapes, the blocks on the stream pool; with the closure on config5).
     Prints one JSON line; `value` is the chosen form's."""
     from finflo_amd.cases import config3, config5
     from finflo_amd.ffhip import Context
@@ -357,19 +358,21 @@ def multiblock_bench(args):
     res = {}
     # the library captures at most 3 blocks x 3 levels per graph branch (ff_vcycle_blocks_graph)
     can_graph = (nb + 7) // 8 * nlev <= 9
-    forms = [("fp64 eager", "fp64", False, False, "exact"),
-             ("fp64 graph", "fp64", False, True, "exact"),
-             ("chosen form", "mixed", True, can_graph, "mlp" if five else "exact")]
-    for label, prec, tune, graph, earsm in forms:
-        if graph and not can_graph:
+    forms = [("fp64 eager", "fp64", False, "eager", "exact"),
+             ("fp64 graph", "fp64", False, "graph", "exact"),
+             ("fp64 streams", "fp64", False, "streams", "exact"),
+             ("chosen form", "mixed", True, "streams", "mlp" if five else "exact")]
+    for label, prec, tune, launch, earsm in forms:
+        if launch == "graph" and not can_graph:
             res[label] = {"skipped": f"{nb} blocks of {nlev} levels exceed the library's graph "
                                      "capture limit (3 blocks x 3 levels per branch)"}
             continue
         ctx.precision(prec)
         ctx.autotune(tune)
+        ctx.block_streams(launch == "streams")
         if five:
             ctx.earsm_mode(earsm)
-        run = (lambda k: ctx.vcycle_blocks_graph(ids, nlev, ncycles=k)) if graph else \
+        run = (lambda k: ctx.vcycle_blocks_graph(ids, nlev, ncycles=k)) if launch == "graph" else \
               (lambda k: ctx.vcycle_blocks(ids, nlev, ncycles=k))
         ctx.vcycle_blocks(ids, nlev, ncycles=max(1, args.warmup))   # tunes, allocates
         run(1)
@@ -395,7 +398,7 @@ def multiblock_bench(args):
         "ms_per_step": res["chosen form"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 with FP32 LU-SGS records" + (", bf16 closure" if five else ""),
-        "chosen_form_launch": "HIP graph" if can_graph else "eager launches",
+        "chosen_form_launch": "eager launches, blocks side by side on the stream pool",
         "data": "synthetic",
         "config": {"workload": f"{nb}-block {'swept ' if five else ''}bump case, {ncell} cells, "
                                f"k-omega SST{' + EARSM' if five else ''}, M={0.6 if five else 0.5} "

@@ -142,6 +142,82 @@ extern "C" int ff_halo_exchange(ff_ctx *c, int level, int what)
     return halo_impl(c, level, what);
 }
 
+// The pool streams and fork/join events (created once, default priority).
+static int pool_ready(ff_ctx *c)
+{
+    for (int q = 0; q < ff_ctx::NPOOL; q++)
+        if (!c->pool[q] && hipStreamCreateWithFlags(&c->pool[q], hipStreamNonBlocking) != hipSuccess)
+            return fail(c, FF_E_HIP, "stream pool%s");
+    for (int q = 0; q <= ff_ctx::NPOOL; q++)
+        if (!c->pool_ev[q] && hipEventCreateWithFlags(&c->pool_ev[q], hipEventDisableTiming) != hipSuccess)
+            return fail(c, FF_E_HIP, "stream pool events%s");
+    return FF_OK;
+}
+
+// FF_OPT_BLOCK_STREAMS: every cycle forks the pool from the context's stream, runs every
+// block's V-cycle on a pool stream (the RESL2 sums of the last cycle into the block's own
+// slot, partials in its stream's buffer, no host wait), joins the pool back and exchanges
+// the level-1 halos on the context's stream.  Block n goes to pool stream n mod NPOOL.
+// (Measured on the 24- and 64-block cases: dealing the blocks out largest first to the
+// least loaded stream, or over fewer streams, was slower.)  The staged LU-SGS records are
+// allocated before the first fork, so no allocation runs while the pool works.
+static int vcycle_blocks_streams(ff_ctx *c, const int *blocks, int nblocks, int nlev, int ncycles,
+                                 const std::vector<std::vector<BlockRec *>> &lv, double *resi)
+{
+    int rc = pool_ready(c);
+    if (rc) return rc;
+    for (int q = 0; q < ff_ctx::NPOOL; q++)
+        if (!c->mb_part[q] && hipMalloc(&c->mb_part[q], sizeof(double) * 7 * c->red_parts) != hipSuccess)
+            return fail(c, FF_E_NOMEM, "RESL2 partials of the stream pool%s");
+    if (c->mb_out_n < nblocks) {
+        if (c->mb_out) (void)hipFree(c->mb_out);
+        c->mb_out = nullptr;
+        c->mb_out_n = 0;
+        if (hipMalloc(&c->mb_out, sizeof(double) * 8 * nblocks) != hipSuccess)
+            return fail(c, FF_E_NOMEM, "RESL2 sums of the blocks%s");
+        c->mb_out_n = nblocks;
+    }
+    if (staged_mode(c))
+        for (auto &l : lv)
+            for (BlockRec *r : l)
+                if ((rc = staged_alloc(c, r))) return rc;
+    const hipStream_t origin = c->stream;
+    const int np = std::min(nblocks, (int)ff_ctx::NPOOL);
+    double dummy[7];
+    for (int it = 0; it < ncycles && !rc; it++) {
+        const bool sums = resi && it == ncycles - 1;
+        HIPCHK(c, hipEventRecord(c->pool_ev[ff_ctx::NPOOL], origin));
+        for (int q = 0; q < np; q++) HIPCHK(c, hipStreamWaitEvent(c->pool[q], c->pool_ev[ff_ctx::NPOOL], 0));
+        for (int n = 0; n < nblocks && !rc; n++) {
+            const int q = n % np;
+            c->stream = c->pool[q];
+            const RedBuf rb{c->mb_part[q], c->red_parts, c->mb_out + 8 * n};
+            rc = vcycle_once(c, lv[n], nlev, blocks[n], sums ? dummy : nullptr, &rb, false);
+        }
+        c->stream = origin;
+        for (int q = 0; q < np; q++) {
+            if (hipEventRecord(c->pool_ev[q], c->pool[q]) != hipSuccess ||
+                hipStreamWaitEvent(origin, c->pool_ev[q], 0) != hipSuccess)
+                return fail(c, FF_E_HIP, "ff_vcycle_blocks: join of the stream pool failed%s");
+        }
+        if (rc) return rc;
+        if ((rc = halo_impl(c, 1, 0))) return rc;
+    }
+    if (resi && ncycles > 0) {
+        std::vector<double> h((size_t)8 * nblocks);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->mb_out, sizeof(double) * 8 * nblocks, hipMemcpyDeviceToHost, origin));
+        HIPCHK(c, hipStreamSynchronize(origin));
+        const int nq = c->prm.iturb == 6 ? 7 : 5;
+        double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int n = 0; n < nblocks; n++)
+            for (int q = 0; q < 7; q++) sum[q] += q < nq ? h[(size_t)8 * n + q] : 0.;
+        for (int q = 0; q < 7; q++) resi[q] = sum[q];
+    } else if (resi) {
+        for (int q = 0; q < 7; q++) resi[q] = 0.;
+    }
+    return FF_OK;
+}
+
 extern "C" int ff_vcycle_blocks(ff_ctx *c, const int *blocks, int nblocks, int nlev, int ncycles,
                                 double *resi)
 {
@@ -150,6 +226,15 @@ extern "C" int ff_vcycle_blocks(ff_ctx *c, const int *blocks, int nblocks, int n
     for (int n = 0; n < nblocks; n++) {
         int rc = vcycle_levels(c, blocks[n], nlev, lv[n]);
         if (rc) return rc;
+    }
+    if (c->block_streams && nblocks > 1) {
+        int rc = vcycle_blocks_streams(c, blocks, nblocks, nlev, ncycles, lv, resi);
+        if (rc) return rc;
+        if (c->lusgs_mode != FF_LUSGS_PLANES)
+            for (int n = 0; n < nblocks; n++)
+                for (int m = 0; m < nlev; m++)
+                    if ((rc = sweep_check(c, *lv[n][m]))) return rc;
+        return FF_OK;
     }
     double sum[7] = {0, 0, 0, 0, 0, 0, 0}, rs[7];
     for (int it = 0; it < ncycles; it++) {
@@ -189,7 +274,7 @@ extern "C" int ff_vcycle_blocks_graph(ff_ctx *c, const int *blocks, int nblocks,
     // branches) ended the process with a segmentation fault while it was captured or first
     // replayed; 24 blocks of 3 levels (3 V-cycles per branch) replay correctly.  Until the
     // cause is known, captures beyond 3 blocks x 3 levels per branch are refused.
-    if ((nblocks + ff_ctx::NPOOL - 1) / ff_ctx::NPOOL * nlev > 9)
+    if ((nblocks + ff_ctx::NBRANCH - 1) / ff_ctx::NBRANCH * nlev > 9)
         return fail(c, FF_E_UNSUP, "ff_vcycle_blocks_graph: more than 9 block levels per branch "
                                    "(3 blocks x 3 levels) are not captured%s");
     std::vector<std::vector<BlockRec *>> lv(nblocks);
@@ -205,12 +290,7 @@ extern "C" int ff_vcycle_blocks_graph(ff_ctx *c, const int *blocks, int nblocks,
         if (c->mb_graph) (void)hipGraphDestroy(c->mb_graph);
         c->mb_exec = nullptr;
         c->mb_graph = nullptr;
-        for (int q = 0; q < ff_ctx::NPOOL; q++)
-            if (!c->pool[q] && hipStreamCreateWithFlags(&c->pool[q], hipStreamNonBlocking) != hipSuccess)
-                return fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: stream pool%s");
-        for (int q = 0; q <= ff_ctx::NPOOL; q++)
-            if (!c->pool_ev[q] && hipEventCreateWithFlags(&c->pool_ev[q], hipEventDisableTiming) != hipSuccess)
-                return fail(c, FF_E_HIP, "ff_vcycle_blocks_graph: events%s");
+        if (int e = pool_ready(c)) return e;
         if (staged_mode(c))
             for (auto &l : lv)
                 for (BlockRec *r : l) {
@@ -227,7 +307,7 @@ extern "C" int ff_vcycle_blocks_graph(ff_ctx *c, const int *blocks, int nblocks,
         const hipStream_t origin = c->stream;
         HIPCHK(c, hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
         int rc = FF_OK;
-        const int np = std::min(nblocks, (int)ff_ctx::NPOOL);
+        const int np = std::min(nblocks, (int)ff_ctx::NBRANCH);
         if (hipEventRecord(c->pool_ev[ff_ctx::NPOOL], origin) != hipSuccess) rc = FF_E_HIP;
         for (int q = 0; q < np && !rc; q++)
             if (hipStreamWaitEvent(c->pool[q], c->pool_ev[ff_ctx::NPOOL], 0) != hipSuccess) rc = FF_E_HIP;

@@ -3902,12 +3902,19 @@ struct ff_ctx {
     long long gen = 0;
     // the captured multi-block cycle (ff_vcycle_blocks_graph): its streams, fork/join events,
     // graph and the key it was captured for
-    static constexpr int NPOOL = 8;
+    static constexpr int NPOOL = 16;      // streams of the pool (FF_OPT_BLOCK_STREAMS)
+    static constexpr int NBRANCH = 8;     // graph branches of ff_vcycle_blocks_graph
     hipStream_t pool[NPOOL] = {};
     hipEvent_t pool_ev[NPOOL + 1] = {};
     hipGraph_t mb_graph = nullptr;
     hipGraphExec_t mb_exec = nullptr;
     std::vector<long long> mb_key;
+    // the concurrent multi-block cycle (FF_OPT_BLOCK_STREAMS): RESL2 partials per pool
+    // stream and the sums of every block, read back once after the cycle
+    bool block_streams = false;
+    double *mb_part[NPOOL] = {};
+    double *mb_out = nullptr;
+    int mb_out_n = 0;
     // the tile-shape autotuner: on, and the chosen shape per (IMAX, JMAX, KMAX, kind)
     bool autotune = false;
     int force_shape = -1;         // FF_OPT_TILE_SHAPE: a candidate shape for every residual
@@ -4256,6 +4263,9 @@ int ff_ctx_destroy(ff_ctx *c)
         if (c->pool[q]) (void)hipStreamDestroy(c->pool[q]);
     for (int q = 0; q <= ff_ctx::NPOOL; q++)
         if (c->pool_ev[q]) (void)hipEventDestroy(c->pool_ev[q]);
+    for (int q = 0; q < ff_ctx::NPOOL; q++)
+        if (c->mb_part[q]) (void)hipFree(c->mb_part[q]);
+    if (c->mb_out) (void)hipFree(c->mb_out);
     for (auto &kv : c->blocks) {
         if (kv.second.mirror && kv.second.arena) (void)hipFree(kv.second.arena);
         sweep_free(kv.second);
@@ -4321,6 +4331,11 @@ int ff_ctx_set_option(ff_ctx *c, int option, int value)
         if (value != 0 && value != 1) return fail(c, FF_E_ARG, "FF_OPT_AUTOTUNE: 0 or 1%s");
         c->autotune = value == 1;
         c->gen++;
+        return FF_OK;
+    }
+    if (option == FF_OPT_BLOCK_STREAMS) {
+        if (value != 0 && value != 1) return fail(c, FF_E_ARG, "FF_OPT_BLOCK_STREAMS: 0 or 1%s");
+        c->block_streams = value == 1;
         return FF_OK;
     }
     if (option == FF_OPT_PRECISION) {
@@ -4605,7 +4620,7 @@ struct RedBuf {
 static int inlout_impl(ff_ctx *c, BlockRec *r, int level);   // ffbc.hip
 
 static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi,
-                         const RedBuf *rb = nullptr)
+                         const RedBuf *rb = nullptr, bool host = true)
 {
     const Blk &b = r->dev;
     c->lev = level;
@@ -4735,6 +4750,8 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
         if (rc) return rc;
         rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)rd.part, parts, nq, rd.out);
         if (rc) return rc;
+        // host = false: the sums stay in rd.out for the caller to read back later
+        if (!host) return FF_OK;
         HIPCHK(c, hipMemcpyAsync(c->red_host, rd.out, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int q = 0; q < 7; q++) resi[q] = q < nq ? c->red_host[q] : 0.;
@@ -5055,7 +5072,8 @@ int ff_prolong(ff_ctx *c, int mcoarse, int block)
 
 // One V-cycle (CYCLES ns3c.f:1450-2137 with MGRID = nlev): the finest-level iteration,
 // the FAS down-loop with pre-smoothing, the prolongation; RESL2 on level 1 into rs if given
-static int vcycle_once(ff_ctx *c, const std::vector<BlockRec *> &lv, int nlev, int block, double *rs)
+static int vcycle_once(ff_ctx *c, const std::vector<BlockRec *> &lv, int nlev, int block, double *rs,
+                       const RedBuf *rb = nullptr, bool host = true)
 {
     int rc;
     BlockRec *r1 = lv[0];
@@ -5064,14 +5082,14 @@ static int vcycle_once(ff_ctx *c, const std::vector<BlockRec *> &lv, int nlev, i
     if ((rc = launch(c, "k_save_old", k_save_old, all_grid(r1->dev), dim3(256), r1->dev, r1->bc_icyc))) return rc;
     if ((rc = ff_state(c, 1, block))) return rc;
     if ((rc = ff_turb(c, 1, block))) return rc;
-    if ((rc = residual_impl(c, r1, 1, 1, rs))) return rc;
+    if ((rc = residual_impl(c, r1, 1, 1, rs, rb, host))) return rc;
     if ((rc = implicit_impl(c, r1, 1, true))) return rc;
     if ((rc = update_impl(c, r1, 1))) return rc;
     // FAS down-loop, pre-smoothing only (ns3c.f:1607-1999)
     for (int m = 2; m <= nlev; m++) {
         BlockRec *fr = lv[m - 2], *cr = lv[m - 1];
         if ((rc = ff_state(c, m - 1, block))) return rc;
-        if ((rc = residual_impl(c, fr, m - 1, m, m - 1 == 1 ? rs : nullptr))) return rc;
+        if ((rc = residual_impl(c, fr, m - 1, m, m - 1 == 1 ? rs : nullptr, rb, host))) return rc;
         if (m > 2 && (rc = forcing_impl(c, fr, 1))) return rc;
         if ((rc = restrict_impl(c, fr, cr))) return rc;
         if ((rc = ff_state(c, m, block))) return rc;

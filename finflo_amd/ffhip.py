@@ -35,7 +35,7 @@ FF_LUSGS_STAGED1 = 4
 FF_OPT_RESIDUAL, FF_RESIDUAL_TILES, FF_RESIDUAL_CELLS = 2, 0, 1
 FF_OPT_EARSM, FF_EARSM_EXACT, FF_EARSM_MLP = 3, 0, 1
 FF_OPT_PRECISION, FF_PREC_FP64, FF_PREC_MIXED = 4, 0, 1
-FF_OPT_AUTOTUNE, FF_OPT_TILE_SHAPE = 5, 6
+FF_OPT_AUTOTUNE, FF_OPT_TILE_SHAPE, FF_OPT_BLOCK_STREAMS = 5, 6, 7
 FF_MLP_NPARAM, FF_MLP_NNORM = 4739, 26
 
 _lib = None
@@ -151,6 +151,10 @@ class Context:
     def autotune(self, on=True):
         """Let the library time the residual kernels' tile shapes per block size."""
         self.set_option(FF_OPT_AUTOTUNE, 1 if on else 0)
+
+    def block_streams(self, on=True):
+        """Run the blocks of a multi-block cycle side by side on a pool of streams."""
+        self.set_option(FF_OPT_BLOCK_STREAMS, 1 if on else 0)
 
     def tile_shape(self, s):
         """Run every residual in the autotuner's candidate shape s (-1: back to the choice)."""

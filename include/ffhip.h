@@ -220,6 +220,14 @@ const char *ff_last_error(const ff_ctx *ctx);
 /*   FF_OPT_TILE_SHAPE: 0-7 runs every residual in that candidate shape of the autotuner
  *   (for tests and measurements), -1 (default) leaves the choice to FF_OPT_AUTOTUNE. */
 #define FF_OPT_TILE_SHAPE  6
+/*   FF_OPT_BLOCK_STREAMS: 1 makes ff_vcycle_blocks run the blocks' V-cycles of a cycle side
+ *   by side, block n on the n mod 8-th stream of a pool forked from the context's stream and
+ *   joined back before the level-1 exchange, so that the small blocks' launches and sweep
+ *   wavefronts overlap instead of leaving most of the chip idle.  The blocks of a cycle are
+ *   independent (each reads only its own ghost cells, which the exchange refreshes after
+ *   the cycle), so the results are those of 0 (default, one block after the other) bit for
+ *   bit; the RESL2 sums are read back once after the last cycle. */
+#define FF_OPT_BLOCK_STREAMS 7
 int  ff_ctx_set_option(ff_ctx *ctx, int option, int value);
 /* The tile shape (I, J, K) the autotuner chose for block N on level M and its measured mean
  * launch time in ms (ms may be NULL); FF_E_STATE if that block size has not been tuned. */

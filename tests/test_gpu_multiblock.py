@@ -315,6 +315,19 @@ def test_config3_multiblock_cycles(hip_device):
             ctx2.download(level=m, block=n + 1)
             np.testing.assert_array_equal(blocks[n][m - 1].F, twin[n][m - 1].F,
                                           err_msg=f"graph block {n + 1} level {m}")
+    # the blocks side by side on the stream pool against one after the other, bit for bit,
+    # residual sums included
+    print("case config 3: blocks on the stream pool", flush=True)
+    ctx.block_streams(True)
+    r1 = ctx.vcycle_blocks(list(range(1, nb + 1)), nlev=3, ncycles=2)
+    r2 = ctx2.vcycle_blocks(list(range(1, nb + 1)), nlev=3, ncycles=2)
+    np.testing.assert_array_equal(r1, r2)
+    for n in range(nb):
+        for m in range(1, 4):
+            ctx.download(level=m, block=n + 1)
+            ctx2.download(level=m, block=n + 1)
+            np.testing.assert_array_equal(blocks[n][m - 1].F, twin[n][m - 1].F,
+                                          err_msg=f"stream pool block {n + 1} level {m}")
     ctx.close()
     ctx2.close()
 
