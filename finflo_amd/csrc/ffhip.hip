@@ -2497,7 +2497,13 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
     // columns stay in one XCD's L2 while their cells are read
     const int tl = xcd_tile(tg);
     if (tl < 0) return;
-    const int lane = threadIdx.x % TW, s = (tl % tg.nx) * PSW + threadIdx.x / TW;
+    // wave w takes lanes 16w..16w+15 over the workgroup's PSW steps, the steps fastest: the
+    // PSW consecutive steps of a lane are PSW consecutive cells along I, so a load instruction
+    // touches 16 short runs instead of 64 separate cache lines, and a store 4 runs of 16 lanes
+    static_assert(TW % PSW == 0, "the lanes of a wave split over the record-pass steps");
+    const int li = threadIdx.x % TW;
+    const int lane = (TW / PSW) * (int)(threadIdx.x / TW) + li / PSW;
+    const int s = (tl % tg.nx) * PSW + li % PSW;
     if (s >= st.nstep) return;
     const int tile = tl / tg.nx, tj = tile % sw.ntj, tk = tile / sw.ntj;
     const int jj = lane % TJ, kk = lane / TJ;
