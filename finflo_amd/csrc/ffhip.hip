@@ -1557,14 +1557,32 @@ __global__ void __launch_bounds__(RED_THREADS) k_resl2_part(Blk b, double *part)
     }
 }
 
-// RESL2 stage 2: fixed-order sum of the partials (run-to-run deterministic)
-__global__ void __launch_bounds__(64) k_resl2_final(const double *part, int nparts, int nq, double *out)
+// RESL2 stage 2: fixed-order sum of the partials (run-to-run deterministic): thread t sums
+// partials t, t + RED_THREADS, ... in order, then the same halving tree as stage 1
+__global__ void __launch_bounds__(RED_THREADS) k_resl2_final(const double *part, int nparts, int nq,
+                                                            double *out)
 {
-    int q = threadIdx.x;
-    if (q >= nq) return;
-    double s = 0.;
-    for (int i = 0; i < nparts; i++) s += part[7 * i + q];
-    out[q] = s;
+    __shared__ double sh[7][RED_THREADS];
+    const int t = threadIdx.x;
+    double s[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) s[q] = 0.;
+    for (int i = t; i < nparts; i += RED_THREADS) {
+#pragma unroll
+        for (int q = 0; q < 7; q++)
+            if (q < nq) s[q] += part[7 * i + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) sh[q][t] = s[q];
+    __syncthreads();
+    for (int h = RED_THREADS / 2; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int q = 0; q < 7; q++) sh[q][t] += sh[q][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < nq) out[t] = sh[t][0];
 }
 
 // TIME3 (base3c.f:79-220) + LOCDT (imps3.f:792-798) + COPR (imps3.f:902-927), thread per
@@ -2481,11 +2499,11 @@ __device__ __forceinline__ FaceBasis basis_get(const double *v)
     return f;
 }
 
-// Both records of every cell: thread (lane, step) of tile blockIdx.x, four steps per
+// Both records of every cell: thread (lane, step) of tile blockIdx.x, PSW steps per
 // workgroup, so that a workgroup's loads of a column run along I and its record stores are
 // whole rows.  The records are read back only by the sweeps, long after: non-temporal
 // stores keep them from displacing the caches' working set on the way out.
-constexpr int PSW = 4;             // record-pass steps per workgroup (one wave each)
+constexpr int PSW = 4;             // record-pass steps per workgroup, and waves per workgroup
 
 template <bool KE, typename RT = double>
 __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw, SweepSt st,
@@ -2497,9 +2515,9 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
     // columns stay in one XCD's L2 while their cells are read
     const int tl = xcd_tile(tg);
     if (tl < 0) return;
-    // wave w takes lanes 16w..16w+15 over the workgroup's PSW steps, the steps fastest: the
-    // PSW consecutive steps of a lane are PSW consecutive cells along I, so a load instruction
-    // touches 16 short runs instead of 64 separate cache lines, and a store 4 runs of 16 lanes
+    // wave w takes TW/PSW lanes over the workgroup's PSW steps, the steps fastest: the PSW
+    // consecutive steps of a lane are PSW consecutive cells along I, so a load instruction
+    // touches TW/PSW short runs instead of TW separate cache lines
     static_assert(TW % PSW == 0, "the lanes of a wave split over the record-pass steps");
     const int li = threadIdx.x % TW;
     const int lane = (TW / PSW) * (int)(threadIdx.x / TW) + li / PSW;
@@ -4754,7 +4772,8 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
         rc = kw ? launch(c, "k_resl2_part", k_resl2_part<7>, dim3(parts), dim3(RED_THREADS), b, rd.part)
                 : launch(c, "k_resl2_part", k_resl2_part<5>, dim3(parts), dim3(RED_THREADS), b, rd.part);
         if (rc) return rc;
-        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(64), (const double *)rd.part, parts, nq, rd.out);
+        rc = launch(c, "k_resl2_final", k_resl2_final, dim3(1), dim3(RED_THREADS), (const double *)rd.part,
+                    parts, nq, rd.out);
         if (rc) return rc;
         // host = false: the sums stay in rd.out for the caller to read back later
         if (!host) return FF_OK;
