@@ -54,12 +54,19 @@ def _extend_axis(X, axis):
     return np.moveaxis(X, 0, axis)
 
 
-def coarsen_nodes(nodes):
+def coarsen_nodes(nodes, planar=False):
     """Node array of the next multigrid level: every other interior node of the finer level
     (the reference's coarse levels take every second grid line, main.f:4600-4660), with two
-    ghost node layers extrapolated again."""
+    ghost node layers extrapolated again.  planar: a block one cell thick in K stays one cell
+    thick (KMAXG = MAX0(1, KMAX/NDIV), main.f:1756); its coarse cell spans K node planes 1
+    and 3, every second plane as in I and J, the third being the first ghost plane, so that
+    it is twice as thick and TRANSF's planar sum, which counts each of the four fine cells
+    twice (multi.f:185-209), is the volume average."""
     g = GHOST
-    inner = nodes[g:-g, g:-g, g:-g][::2, ::2, ::2]
+    if planar:
+        inner = nodes[g:g + 3:2, g:-g, g:-g][:, ::2, ::2]
+    else:
+        inner = nodes[g:-g, g:-g, g:-g][::2, ::2, ::2]
     for ax in range(3):
         inner = _extend_axis(inner, ax)
     return inner
@@ -190,7 +197,8 @@ def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jr
     height of the J-low wall; the wall distance is then measured vertically from it."""
     prm = prm or default_params()
     f = 2 ** (nlev - 1)
-    if imax % f or jmax % f or kmax % f:
+    planar = kmax == 1          # one cell thick in K: KMAX = 1 on every level
+    if imax % f or jmax % f or (kmax % f and not planar):
         raise ValueError(f"dimensions must be divisible by {f} for {nlev} levels")
     if nodes is None:
         xi = stretched(imax, lengths[0])
@@ -199,7 +207,7 @@ def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jr
         nodes = node_grid(xi, yj, zk, distort=distort, seed=seed + 1000)
     blocks = []
     for m in range(nlev):
-        b = Block(imax >> m, jmax >> m, kmax >> m)
+        b = Block(imax >> m, jmax >> m, kmax if planar else kmax >> m)
         fill_geometry(b, nodes)
         if surface is not None:
             b["DISTW"] = np.abs(b["YC"] - surface(b["XC"], b["ZC"]))
@@ -207,7 +215,7 @@ def make_hierarchy(imax, jmax, kmax, nlev, prm=None, lengths=(2.0, 1.0, 1.0), jr
                    turb=turb, mach_dir=mach_dir, turb_amp=turb_amp if m == 0 else 0.0)
         blocks.append(b)
         if m < nlev - 1:
-            nodes = coarsen_nodes(nodes)
+            nodes = coarsen_nodes(nodes, planar)
     return blocks
 
 

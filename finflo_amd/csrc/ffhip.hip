@@ -480,19 +480,22 @@ __global__ void __launch_bounds__(256) k_extend(Blk b)
 // ------------------------------------------------------------------------------------
 
 // Volume-weighted sum of the 8 fine cells of a coarse cell in TRANSF's order
-// (multi.f:214-218): IPJPK, IJK, IPJK, IJPK, IPJPKP, IJKP, IPJKP, IJPKP
+// (multi.f:214-218): IPJPK, IJK, IPJK, IJPK, IPJPKP, IJKP, IPJKP, IJPKP.  kp is the K
+// stride, or 0 on a planar fine level (TRANSF's IMJM2 = 0, multi.f:185-191), where the
+// four "KP" cells are the four K cells again.
 __device__ __forceinline__ double restrict8(const Blk &f, const double *vol, const double *x,
-                                            int ijk, double pvol)
+                                            int ijk, int kp, double pvol)
 {
     const int ipjk = ijk + 1, ijpk = ijk + f.is, ipjpk = ijk + f.is + 1;
-    const int ijkp = ijk + f.il, ipjpkp = ipjpk + f.il, ijpkp = ijpk + f.il, ipjkp = ipjk + f.il;
+    const int ijkp = ijk + kp, ipjpkp = ipjpk + kp, ijpkp = ijpk + kp, ipjkp = ipjk + kp;
     return (vol[ipjpk] * x[ipjpk] + vol[ijk] * x[ijk] + vol[ipjk] * x[ipjk] +
             vol[ijpk] * x[ijpk] + vol[ipjpkp] * x[ipjpkp] + vol[ijkp] * x[ijkp] +
             vol[ipjkp] * x[ipjkp] + vol[ijpkp] * x[ijpkp]) * pvol;
 }
 
 // TRANSF (+ TRANKE, KW) loop 1000 and the OLD copies (multi.f:194-278, 453-510): coarse cell
-// per thread over the interior and the first ghost layer
+// per thread over the interior and the first ghost layer.  A planar fine level (KMAX = 1)
+// maps coarse K to fine K (KIND = 1, multi.f:185-196).
 template <bool KW>
 __global__ void __launch_bounds__(256) k_restrict(Blk f, Blk c)
 {
@@ -500,26 +503,27 @@ __global__ void __launch_bounds__(256) k_restrict(Blk f, Blk c)
     int J = blockIdx.y, K = blockIdx.z;
     if (I > c.imax + 1) return;
     const int n = cid(c, I, J, K);
-    const int ijk = cid(f, 2 * I - 1, 2 * J - 1, 2 * K - 1);
+    const bool planar = f.kmax == 1;
+    const int ijk = cid(f, 2 * I - 1, 2 * J - 1, planar ? K : 2 * K - 1), kp = planar ? 0 : f.il;
     const double *vol = FLD(f, FF_VOL);
     const double pvol = 1. / FLD(c, FF_VOL)[n];
     const int st[7] = {FF_TEMP, FF_U, FF_V, FF_W, FF_PDIFF, FF_EPS2, FF_VIST};
     double x[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) {
-        x[q] = restrict8(f, vol, FLD(f, st[q]), ijk, pvol);
+        x[q] = restrict8(f, vol, FLD(f, st[q]), ijk, kp, pvol);
         FLD(c, st[q])[n] = x[q];
     }
 #pragma unroll
-    for (int q = 0; q < 5; q++) FLD(c, FF_ROP2H + q)[n] = restrict8(f, vol, FLD(f, FF_DRO + q), ijk, pvol);
+    for (int q = 0; q < 5; q++) FLD(c, FF_ROP2H + q)[n] = restrict8(f, vol, FLD(f, FF_DRO + q), ijk, kp, pvol);
     double rk = 0., re = 0.;
     if (KW) {
-        rk = restrict8(f, vol, FLD(f, FF_RK), ijk, pvol);
-        re = restrict8(f, vol, FLD(f, FF_REPS), ijk, pvol);
+        rk = restrict8(f, vol, FLD(f, FF_RK), ijk, kp, pvol);
+        re = restrict8(f, vol, FLD(f, FF_REPS), ijk, kp, pvol);
         FLD(c, FF_RK)[n] = rk;
         FLD(c, FF_REPS)[n] = re;
-        FLD(c, FF_RKP2H)[n] = restrict8(f, vol, FLD(f, FF_DRK), ijk, pvol);
-        FLD(c, FF_EPSP2H)[n] = restrict8(f, vol, FLD(f, FF_DEPS), ijk, pvol);
+        FLD(c, FF_RKP2H)[n] = restrict8(f, vol, FLD(f, FF_DRK), ijk, kp, pvol);
+        FLD(c, FF_EPSP2H)[n] = restrict8(f, vol, FLD(f, FF_DEPS), ijk, kp, pvol);
     }
     if (I >= 1 && I <= c.imax && J >= 1 && J <= c.jmax && K >= 1 && K <= c.kmax) {
         FLD(c, FF_TOLD)[n] = x[0];
@@ -5028,8 +5032,12 @@ static int check_pair(ff_ctx *c, BlockRec *fr, BlockRec *cr)
 {
     if (!fr || !cr) return fail(c, FF_E_STATE, "multigrid level not registered%s");
     const Blk &f = fr->dev, &cb = cr->dev;
-    if (f.imax != 2 * cb.imax || f.jmax != 2 * cb.jmax || f.kmax != 2 * cb.kmax)
-        return fail(c, FF_E_ARG, "a coarse level must have half the cells of the finer level in each direction%s");
+    // a planar block (KMAX = 1) keeps KMAX = 1 on every level: KMAXG = MAX0(1, KMAX/NDIV)
+    // (main.f:1756), with TRANSF's planar path
+    const bool planar = f.kmax == 1 && cb.kmax == 1;
+    if (f.imax != 2 * cb.imax || f.jmax != 2 * cb.jmax || (f.kmax != 2 * cb.kmax && !planar))
+        return fail(c, FF_E_ARG, "a coarse level must have half the cells of the finer level in each "
+                                 "direction (or KMAX = 1 on both)%s");
     return FF_OK;
 }
 

@@ -279,7 +279,9 @@ int  ff_update(ff_ctx *ctx, int level, int block);
 /* Copy the solved variables into their *OLD slots (the start-of-cycle values FIN reads). */
 int  ff_save_old(ff_ctx *ctx, int level, int block);
 /* Multigrid transfers between level M and M+1 of block N; the coarse level has half the cells
- * of the finer one in each direction and carries the FAS forcing in its *P2H slots.
+ * of the finer one in each direction (a planar block, KMAX = 1, keeps KMAX = 1 on every level
+ * and takes TRANSF's KIND = 1 path, multi.f:185-191, which counts each of the four fine cells
+ * twice) and carries the FAS forcing in its *P2H slots.
  *   ff_restrict:  TRANSF multi.f:156 (+ TRANKE multi.f:434 for k-omega) from Mfine to Mfine+1:
  *                 state, EPS2, VIST and the residual (into the forcing slots), OLD copies.
  *   ff_forcing:   phase 0 FORCFU multi.f:1609 (P = P - D), phase 1 ADDRES multi.f:1686

@@ -2744,13 +2744,17 @@ void ffo_update_level(const ff_params *p, ff_block *b, int level)
 
 /* The volume-weighted sum of the 8 fine cells of coarse cell (I,J,K) in the order of
  * TRANSF loop 1000 (multi.f:214-218): IPJPK, IJK, IPJK, IJPK, IPJPKP, IJKP, IPJKP, IJPKP,
- * where IJK is the fine cell (2I-1, 2J-1, 2K-1) */
+ * where IJK is the fine cell (2I-1, 2J-1, 2K-1).  A planar fine level (KMAX = 1) takes
+ * TRANSF's KIND = 1, IMJM2 = 0 path (multi.f:185-191): IJK is (2I-1, 2J-1, K) and the four
+ * "KP" cells are the four K cells again, so each enters the sum twice. */
 static double restrict8(const dims_t *f, const double *vol, const double *x, int I, int J,
                         int K, double pvol)
 {
-    long ijk = cid(f, 2 * I - 1, 2 * J - 1, 2 * K - 1);
+    const int planar = f->kmax == 1;
+    long ijk = cid(f, 2 * I - 1, 2 * J - 1, planar ? K : 2 * K - 1);
+    const long kp = planar ? 0 : f->il;
     long ipjk = ijk + 1, ijpk = ijk + f->is, ipjpk = ijk + f->is + 1;
-    long ijkp = ijk + f->il, ipjpkp = ipjpk + f->il, ijpkp = ijpk + f->il, ipjkp = ipjk + f->il;
+    long ijkp = ijk + kp, ipjpkp = ipjpk + kp, ijpkp = ijpk + kp, ipjkp = ipjk + kp;
     return (vol[ipjpk] * x[ipjpk] + vol[ijk] * x[ijk] + vol[ipjk] * x[ipjk] +
             vol[ijpk] * x[ijpk] + vol[ipjpkp] * x[ipjpkp] + vol[ijkp] * x[ijkp] +
             vol[ipjkp] * x[ipjkp] + vol[ijpkp] * x[ijpkp]) * pvol;

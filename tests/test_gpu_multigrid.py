@@ -34,10 +34,11 @@ def context(prm, blocks):
     return ctx
 
 
+@pytest.mark.parametrize("dims", [(16, 8, 8), (16, 8, 1)], ids=["3d", "planar"])
 @pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
-def test_transfers(hip_device, iturb):
-    print(f"case multigrid transfers iturb={iturb}", flush=True)
-    prm, blocks = hierarchy(iturb)
+def test_transfers(hip_device, iturb, dims):
+    print(f"case multigrid transfers iturb={iturb} dims {dims}", flush=True)
+    prm, blocks = hierarchy(iturb, dims=dims)
     rng = np.random.default_rng(5)
     for name in ("DRO", "DM", "DN", "DW", "DE", "DRK", "DEPS"):
         blocks[0][name] = rng.uniform(-1, 1, blocks[0].ntot)
@@ -121,6 +122,28 @@ def test_vcycles(hip_device, iturb, nlev, lusgs):
     for m in range(nlev):
         assert_fields_equal(blocks[m], host[m], prim, interior=False,
                             rtol=RTOL if iturb else 0.0, what=f"V-cycle level {m + 1}")
+    np.testing.assert_allclose(r2, r1, rtol=1e-10 if iturb else 1e-13)
+    ctx.close()
+
+
+@pytest.mark.parametrize("nlev", [2, 3])
+@pytest.mark.parametrize("iturb", [0, 6], ids=["laminar", "sst"])
+def test_vcycles_planar(hip_device, iturb, nlev):
+    """A block one cell thick in K (KMAX = 1 on every level): TRANSF's planar path."""
+    print(f"case planar V-cycles iturb={iturb} nlev={nlev}", flush=True)
+    prm, blocks = hierarchy(iturb, nlev=nlev, dims=(24, 16, 1), seed=75)
+    host = [b.copy() for b in blocks]
+    oh = OracleHierarchy(prm, host)
+    ctx = context(prm, blocks)
+    for _ in range(3):
+        r1 = oh.vcycle()
+    r2 = ctx.vcycle(nlev, ncycles=3)
+    for m in range(nlev):
+        ctx.download(level=m + 1)
+    prim = SST_PRIM if iturb else PRIM
+    for m in range(nlev):
+        assert_fields_equal(blocks[m], host[m], prim, interior=False,
+                            rtol=RTOL if iturb else 0.0, what=f"planar V-cycle level {m + 1}")
     np.testing.assert_allclose(r2, r1, rtol=1e-10 if iturb else 1e-13)
     ctx.close()
 

@@ -218,9 +218,9 @@ def vcycle_by_fas_definition(prm, blocks, with_addres=True):
         h.prolong(m)
 
 
-def vcycle_hierarchy(iturb, nlev, seed):
+def vcycle_hierarchy(iturb, nlev, seed, dims=(8, 8, 8)):
     prm = default_params(iturb=iturb, cfll=3.0)
-    blocks = make_hierarchy(8, 8, 8, nlev, prm, distort=0.1, kind="all", amp=0.03, seed=seed,
+    blocks = make_hierarchy(*dims, nlev, prm, distort=0.1, kind="all", amp=0.03, seed=seed,
                             turb=(1.0, 150.0) if iturb else None)
     return prm, blocks
 
@@ -261,13 +261,15 @@ def test_vcycle_is_the_fas_definition(iturb, nlev):
         assert max_rel_diff(c, ref, names) > 1e3 * max(d, 1e-15)
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 8), (16, 8, 1)], ids=["3d", "planar"])
 @pytest.mark.parametrize("nlev", [2, 3])
 @pytest.mark.parametrize("iturb", [0, 6])
-def test_vcycle_without_coarse_steps_is_the_plain_cycle(iturb, nlev):
+def test_vcycle_without_coarse_steps_is_the_plain_cycle(iturb, nlev, dims):
     """With CFLL = 0 the coarse local time steps are zero (TIME3: DTL = 1/(B/CFLL ...) = 0),
     so the coarse levels produce no increments, the prolonged corrections vanish and a
-    V-cycle leaves level 1 exactly where the plain finest-level iteration leaves it."""
-    prm, blocks = vcycle_hierarchy(iturb, nlev, seed=31 + nlev)
+    V-cycle leaves level 1 exactly where the plain finest-level iteration leaves it.  The
+    planar hierarchy (KMAX = 1 on every level) takes TRANSF's planar path."""
+    prm, blocks = vcycle_hierarchy(iturb, nlev, seed=31 + nlev, dims=dims)
     prm.cfll = 0.0
     a = blocks[0].copy()
     oa = OracleBlock(prm, a)
@@ -292,3 +294,34 @@ def test_vcycle_without_coarse_steps_is_the_plain_cycle(iturb, nlev):
         for n, o in (("TEMP", "TOLD"), ("U", "UOLD"), ("V", "VOLD"), ("W", "WOLD"),
                      ("PDIFF", "POLD")):
             np.testing.assert_array_equal(b.interior(n), b.interior(o), err_msg=n)
+
+
+def test_planar_restriction_counts_each_fine_cell_twice():
+    """A planar fine level (KMAX = 1) takes TRANSF's KIND = 1, IMJM2 = 0 path
+    (multi.f:185-209): coarse K maps to fine K, and the four "KP" cells are the four K cells
+    again.  A field that encodes each fine cell's index must restrict to twice the volume-
+    weighted sum over the four fine cells (2I-1..2I, 2J-1..2J, K) divided by the coarse
+    volume, on interior cells and on the first ghost layers (K = 0 and 2 read fine K = 0 and
+    2, not 2K-1); on the planar hierarchy, whose coarse cell is twice as thick, a constant
+    restricts to itself and the coarse volume integral of a residual is twice the fine one."""
+    prm, (f, c) = uniform_hierarchy(dims=(8, 8, 1))
+    assert (f.kmax, c.kmax) == (1, 1)
+    ks, js, is_ = f.kstrid, f.jstrid, f.istrid
+    K, J, I = np.meshgrid(np.arange(ks) - 1, np.arange(js) - 1, np.arange(is_) - 1, indexing="ij")
+    code = (I + 100.0 * J + 10000.0 * K).ravel()
+    f["TEMP"] = code
+    rng = np.random.default_rng(4)
+    f["DRO"] = rng.uniform(-1, 1, f.ntot)
+    f["U"] = 2.5
+    h = OracleHierarchy(prm, [f, c])
+    h.restrict(1)
+    vol = f["VOL"]
+    for (I2, J2, K2) in [(1, 1, 1), (4, 2, 1), (0, 3, 1), (2, 4, 0), (3, 1, 2), (5, 5, 2)]:
+        offs = [f.cell(2 * I2 - 1 + a, 2 * J2 - 1 + b, K2) for b in (0, 1) for a in (0, 1)]
+        want = 2.0 * sum(vol[o] * code[o] for o in offs) / c["VOL"][c.cell(I2, J2, K2)]
+        got = c["TEMP"][c.cell(I2, J2, K2)]
+        assert abs(got - want) <= 1e-12 * abs(want), (I2, J2, K2, got, want)
+    np.testing.assert_allclose(c.interior("U"), 2.5, rtol=1e-14)
+    fine = np.sum(f.interior("VOL") * f.interior("DRO"))
+    coarse = np.sum(c.interior("VOL") * c.interior("ROP2H"))
+    assert abs(2.0 * fine - coarse) <= 1e-13 * np.sum(f.interior("VOL") * np.abs(f.interior("DRO")))
