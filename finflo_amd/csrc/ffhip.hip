@@ -2442,17 +2442,24 @@ __device__ __forceinline__ unsigned lds_addr(const void *p)
 // i = s - (jj + kk) + 1; the predictor reaches the same cell at step NSTEP - 1 - s, so both
 // sweeps read whole rows.  The sweeps then carry only the recurrence and the three outgoing
 // contributions per step.  The predictor's increments go to a row buffer the corrector reads
-// the same way.  Arithmetic is that of the tile kernels, operation for operation.
+// the same way.  Arithmetic is that of the tile kernels, operation for operation.  A face
+// basis is stored as its unit normal and first tangent: the second tangent is their cross
+// product, which the sweeps form again with SURFVECTS's operations (basis_get), so the
+// record is 48 fields instead of 56 and the results are the same bit for bit.  (Storing the
+// normal alone and forming both tangents in the sweeps saved more bytes but put SURFVECTS's
+// square root and division on the sweeps' step latency, and the V-cycle was no faster.)
 // ------------------------------------------------------------------------------------
-constexpr int NRF = 56;            // record fields (doubles) per cell and sweep
-// predictor record: vol*d (7), -dt1*wk, dt1, wi, wj, SCINV, SCINP, 7 scalars, 3 bases (27),
-// 3 x (X1, X2, X5)
+constexpr int NRF = 48;            // record fields per cell and sweep (a multiple of 4)
+// predictor record: vol*d (7), -dt1*wk, dt1, wi, wj, SCINV, SCINP, 7 scalars, 3 x (normal,
+// first tangent) (18), 3 x (X1, X2, X5), 1 unused
 enum { RP_VD = 0, RP_MK = 7, RP_DT1, RP_WI, RP_WJ, RP_SCINV, RP_SCINP, RP_SC = 13,
-       RP_B = 20, RP_X = 47 };
+       RP_N = 20, RP_X = 38 };
 // corrector record: DIAGGS factor, vol, dt1*wk, dt1, wi, wj, SCINV, SCINP, 7 scalars,
-// 3 bases, 3 x (X1, X2, X5), CP, RO
+// 3 x (normal, first tangent), 3 x (X1, X2, X5), CP, RO, 4 unused
 enum { RC_FAC = 0, RC_VOL, RC_MK, RC_DT1, RC_WI, RC_WJ, RC_SCINV, RC_SCINP, RC_SC = 8,
-       RC_B = 15, RC_X = 42, RC_CP = 51, RC_RHO = 52 };
+       RC_N = 15, RC_X = 33, RC_CP = 42, RC_RHO = 43 };
+constexpr int NB = 6;              // record fields of one face basis
+static_assert(NRF % 4 == 0, "a record is a whole number of 1 KiB LDS-DMA loads in FP32 too");
 
 constexpr int NDN = 8;             // rows of predictor increments per step (7 used)
 struct SweepSt {
@@ -2489,17 +2496,20 @@ __device__ __forceinline__ void sc_get(SweepCell &c, const double *v)
     }
 }
 
+// a face basis in a record: the unit normal and the first tangent; the second tangent is
+// their cross product, with the operations of SURFVECTS (a31 = a12*b3 - a13*b2, ...)
 __device__ __forceinline__ void basis_put(const FaceBasis &f, double *v)
 {
     v[0] = f.nx; v[1] = f.ny; v[2] = f.nz; v[3] = f.a21; v[4] = f.a22; v[5] = f.a23;
-    v[6] = f.a31; v[7] = f.a32; v[8] = f.a33;
 }
 
 __device__ __forceinline__ FaceBasis basis_get(const double *v)
 {
     FaceBasis f;
     f.nx = v[0]; f.ny = v[1]; f.nz = v[2]; f.a21 = v[3]; f.a22 = v[4]; f.a23 = v[5];
-    f.a31 = v[6]; f.a32 = v[7]; f.a33 = v[8];
+    f.a31 = f.ny * f.a23 - f.nz * f.a22;
+    f.a32 = f.nz * f.a21 - f.nx * f.a23;
+    f.a33 = f.nx * f.a22 - f.ny * f.a21;
     return f;
 }
 
@@ -2544,13 +2554,14 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
         v[RP_DT1] = c.dt1; v[RP_WI] = c.wi; v[RP_WJ] = c.wj;
         v[RP_SCINV] = c.scinv; v[RP_SCINP] = c.scinp;
         sc_put<KE>(c, v + RP_SC);
-        basis_put(c.fi, v + RP_B); basis_put(c.fj, v + RP_B + 9); basis_put(c.fk, v + RP_B + 18);
+        basis_put(c.fi, v + RP_N); basis_put(c.fj, v + RP_N + NB); basis_put(c.fk, v + RP_N + 2 * NB);
         const Coef *cf[3] = {&c.ci, &c.cj, &c.ck};
 #pragma unroll
         for (int f = 0; f < 3; f++) {
             v[RP_X + 3 * f] = cf[f]->x1; v[RP_X + 3 * f + 1] = cf[f]->x2; v[RP_X + 3 * f + 2] = cf[f]->x5;
         }
-        static_assert(RP_X + 9 == NRF, "the predictor record fills NRF fields");
+        static_assert(RP_X + 9 <= NRF, "the predictor record fits NRF fields");
+        for (int f = RP_X + 9; f < NRF; f++) v[f] = 0.;
 #pragma unroll
         for (int f = 0; f < NRF; f++) __builtin_nontemporal_store((RT)v[f], &recp[st_row(st, tile, s, f) + lane]);
     }
@@ -2562,7 +2573,7 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
         v[RC_FAC] = c.fac; v[RC_VOL] = c.vol; v[RC_MK] = c.dt1 * c.wk; v[RC_DT1] = c.dt1;
         v[RC_WI] = c.wi; v[RC_WJ] = c.wj; v[RC_SCINV] = c.scinv; v[RC_SCINP] = c.scinp;
         sc_put<KE>(c, v + RC_SC);
-        basis_put(c.fi, v + RC_B); basis_put(c.fj, v + RC_B + 9); basis_put(c.fk, v + RC_B + 18);
+        basis_put(c.fi, v + RC_N); basis_put(c.fj, v + RC_N + NB); basis_put(c.fk, v + RC_N + 2 * NB);
         const Coef *cf[3] = {&c.ci, &c.cj, &c.ck};
 #pragma unroll
         for (int f = 0; f < 3; f++) {
@@ -2586,7 +2597,7 @@ __global__ void __launch_bounds__(PSW * TW) k_sweep_prep(Prm p, Blk b, Sweep sw,
 // loads its halo words before it issues its copy: the wait for them then covers only the
 // copies of earlier steps, which have had a step or more to arrive.
 template <typename RT>
-constexpr int rbytes() { return NRF * TW * (int)sizeof(RT); }   // one record: 28 or 14 KiB
+constexpr int rbytes() { return NRF * TW * (int)sizeof(RT); }   // one record: 24 or 12 KiB
 constexpr int DBYTES = NDN * TW * 8;            // one step of increments: 4 KiB
 
 // four 1 KiB loads from one address pair: the instruction offset advances the global and the
@@ -2765,8 +2776,8 @@ struct FaceSrc {
 
 // PREDGS / PRKEGS on staged records, the (J,K)-tile wavefront of k_pred_tiles.  Issue order
 // within step s: the step's stores, the halo blocks of step s + 2, the record of step s + 3.
-// At the top of step s at least the two later record copies (56 loads) follow the halo
-// blocks of step s, which follow the record of step s, so vmcnt(56) covers both.
+// At the top of step s at least the two later record copies (2 NLD loads) follow the halo
+// blocks of step s, which follow the record of step s, so vmcnt(2 NLD) covers both.
 template <bool KE, typename RT = double>
 __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt st)
 {
@@ -2846,9 +2857,9 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
 #pragma unroll
             for (int m = 0; m < 7; m++) scv[m] = cur[(RP_SC + m) * TW];
             sc_get<KE>(sc, scv);
-            double bv[27], xv[9];
+            double bv[3 * NB], xv[9];
 #pragma unroll
-            for (int m = 0; m < 27; m++) bv[m] = cur[(RP_B + m) * TW];
+            for (int m = 0; m < 3 * NB; m++) bv[m] = cur[(RP_N + m) * TW];
 #pragma unroll
             for (int m = 0; m < 9; m++) xv[m] = cur[(RP_X + m) * TW];
             Coef ci, cj, ck;
@@ -2857,8 +2868,8 @@ __global__ void __launch_bounds__(TW) k_pred_st(Prm p, Blk b, Sweep sw, SweepSt 
             ck.x1 = xv[6]; ck.x2 = xv[7]; ck.x5 = xv[8];
             double oj[NV], ok[NV];
             face_xfer<KE>(dn, sc, basis_get(bv), ci, yi);
-            face_xfer<KE>(dn, sc, basis_get(bv + 9), cj, oj);
-            face_xfer<KE>(dn, sc, basis_get(bv + 18), ck, ok);
+            face_xfer<KE>(dn, sc, basis_get(bv + NB), cj, oj);
+            face_xfer<KE>(dn, sc, basis_get(bv + 2 * NB), ck, ok);
             // every lane leaves its outputs in LDS (an edge lane's are never read there)
             const int wb = s & 1;
 #pragma unroll
@@ -2963,9 +2974,9 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
 #pragma unroll
             for (int m = 0; m < 7; m++) scv[m] = cur[(RC_SC + m) * TW];
             sc_get<KE>(sc, scv);
-            double bv[27], xv[9];
+            double bv[3 * NB], xv[9];
 #pragma unroll
-            for (int m = 0; m < 27; m++) bv[m] = cur[(RC_B + m) * TW];
+            for (int m = 0; m < 3 * NB; m++) bv[m] = cur[(RC_N + m) * TW];
 #pragma unroll
             for (int m = 0; m < 9; m++) xv[m] = cur[(RC_X + m) * TW];
             Coef ci, cj, ck;
@@ -2974,8 +2985,8 @@ __global__ void __launch_bounds__(TW) k_corr_st(Prm p, Blk b, Sweep sw, SweepSt 
             ck.x1 = xv[6]; ck.x2 = xv[7]; ck.x5 = xv[8];
             double oj[NV], ok[NV];
             face_xfer<KE>(dn, sc, basis_get(bv), ci, yi);
-            face_xfer<KE>(dn, sc, basis_get(bv + 9), cj, oj);
-            face_xfer<KE>(dn, sc, basis_get(bv + 18), ck, ok);
+            face_xfer<KE>(dn, sc, basis_get(bv + NB), cj, oj);
+            face_xfer<KE>(dn, sc, basis_get(bv + 2 * NB), ck, ok);
             // every lane leaves its outputs in LDS (an edge lane's are never read there)
             const int wb = s & 1;
 #pragma unroll
@@ -3135,9 +3146,9 @@ __global__ void __launch_bounds__(WGW * TW) k_pred_wg(Prm p, Blk b, Sweep sw, Sw
 #pragma unroll
             for (int m = 0; m < 7; m++) scv[m] = cur[(RP_SC + m) * TW];
             sc_get<KE>(sc, scv);
-            double bv[9];
+            double bv[NB];
 #pragma unroll
-            for (int m = 0; m < 9; m++) bv[m] = cur[(RP_B + 9 * w + m) * TW];
+            for (int m = 0; m < NB; m++) bv[m] = cur[(RP_N + NB * w + m) * TW];
             Coef cf;
             cf.x1 = cur[(RP_X + 3 * w) * TW];
             cf.x2 = cur[(RP_X + 3 * w + 1) * TW];
@@ -3256,9 +3267,9 @@ __global__ void __launch_bounds__(WGW * TW) k_corr_wg(Prm p, Blk b, Sweep sw, Sw
 #pragma unroll
             for (int m = 0; m < 7; m++) scv[m] = cur[(RC_SC + m) * TW];
             sc_get<KE>(sc, scv);
-            double bv[9];
+            double bv[NB];
 #pragma unroll
-            for (int m = 0; m < 9; m++) bv[m] = cur[(RC_B + 9 * w + m) * TW];
+            for (int m = 0; m < NB; m++) bv[m] = cur[(RC_N + NB * w + m) * TW];
             Coef cf;
             cf.x1 = cur[(RC_X + 3 * w) * TW];
             cf.x2 = cur[(RC_X + 3 * w + 1) * TW];
