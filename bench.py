@@ -43,6 +43,7 @@ WORKLOADS = {
 
 # Algorithmic (compulsory, unique) FP64 bytes per cell of each kernel: every field the
 # kernel must read or write, once (neighbour values come from the same arrays).
+RECORD_FIELDS = 48      # values per cell in each staged LU-SGS record (ffhip.hip NRF)
 KERNEL_BYTES_PER_CELL = {
     0: {
         # flux state (RO U V W P C DRDP DRDH), viscous state (VIS EPS2 VIST CP CH TEMP), VOL,
@@ -60,11 +61,11 @@ KERNEL_BYTES_PER_CELL = {
         # 3 faces x 4, VOL, 11 state fields, D in, D and DTL out
         "k_time3_copr": 8 * (12 + 1 + 11 + 5 + 6),
         # staged LU-SGS: the pass reads D (5), U V W C RO VOL VIS VIST DTL CP, A1..A3 and
-        # their normals (12) and writes the two 56-double records; the sweeps read a record
-        # (56) and write (predictor) or read and write (corrector) the 5 increments
-        "k_sweep_prep": 8 * (5 + 10 + 12 + 2 * 56),
-        "k_pred_st": 8 * (56 + 5),
-        "k_corr_st": 8 * (56 + 5 + 5),
+        # their normals (12) and writes the two 48-double records; the sweeps read a record
+        # (48) and write (predictor) or read and write (corrector) the 5 increments
+        "k_sweep_prep": 8 * (5 + 10 + 12 + 2 * RECORD_FIELDS),
+        "k_pred_st": 8 * (RECORD_FIELDS + 5),
+        "k_corr_st": 8 * (RECORD_FIELDS + 5 + 5),
         # every cell incl. ghosts: 5 solved variables in, 12 derived fields out
         "k_state": 8 * (5 + 12),
     },
@@ -79,9 +80,9 @@ KERNEL_BYTES_PER_CELL = {
         "k_pred_km": 8 * (7 + 12 + 12 + 7),
         "k_corr_km": 8 * (7 + 13 + 12 + 7),
         # staged LU-SGS (7 increments; RK DRDP DRDH read by the pass as well)
-        "k_sweep_prep": 8 * (7 + 13 + 12 + 2 * 56),
-        "k_pred_st": 8 * (56 + 7),
-        "k_corr_st": 8 * (56 + 7 + 7),
+        "k_sweep_prep": 8 * (7 + 13 + 12 + 2 * RECORD_FIELDS),
+        "k_pred_st": 8 * (RECORD_FIELDS + 7),
+        "k_corr_st": 8 * (RECORD_FIELDS + 7 + 7),
     },
 }
 # Compulsory bytes per cell of the three phases of one finest-level iteration, with S state
