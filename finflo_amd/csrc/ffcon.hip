@@ -28,6 +28,23 @@ __global__ void __launch_bounds__(256) k_halo(Blk dst, Blk src, const int *map, 
     for (int q = 0; q < hf.n; q++) FLD(dst, hf.f[q])[d] = FLD(src, hf.f[q])[s];
 }
 
+// The same copies for every connection of a level in one launch: workgroup w takes pairs
+// wg[w].y .. wg[w].y + 255 of connection wg[w].x.  Every ghost cell has one source, and the
+// sources are interior cells, which no pair writes, so the order of the copies does not
+// matter and the result is that of the launches one connection after the other.
+__global__ void __launch_bounds__(256) k_halo_batch(const Blk *blks, const HaloConnDev *conns,
+                                                    const int2 *wg, HaloFields hf)
+{
+    const int2 w = wg[blockIdx.x];
+    const HaloConnDev cn = conns[w.x];
+    const int t = w.y + threadIdx.x;
+    if (t >= cn.n) return;
+    const int d = cn.map[2 * t], s = cn.map[2 * t + 1];
+    const Blk &dst = blks[cn.dst], &src = blks[cn.src];
+#pragma unroll 4
+    for (int q = 0; q < hf.n; q++) FLD(dst, hf.f[q])[d] = FLD(src, hf.f[q])[s];
+}
+
 // the in-plane extents of a face: (J, K) on an I-face, (I, K) on a J-face, (I, J) on a K-face
 static void face_extent(const Blk &b, int face, int *ea, int *eb)
 {
@@ -98,6 +115,12 @@ extern "C" int ff_con_register(ff_ctx *c, int level, int block, const ff_con_pat
                 }
     }
     off.push_back(pairs.size() / 2);
+    // the sources are interior layers 1 and 2 (two cells thick, checked above); a ghost cell
+    // named by two patches (overlapping ranges on one face) makes the copies order-dependent
+    std::vector<int> dsts(pairs.size() / 2);
+    for (size_t q = 0; q < dsts.size(); q++) dsts[q] = pairs[2 * q];
+    std::sort(dsts.begin(), dsts.end());
+    const bool unique = std::adjacent_find(dsts.begin(), dsts.end()) == dsts.end();
     int *map = nullptr;
     if (!pairs.empty()) {
         if (hipMalloc(&map, sizeof(int) * pairs.size()) != hipSuccess)
@@ -111,7 +134,70 @@ extern "C" int ff_con_register(ff_ctx *c, int level, int block, const ff_con_pat
     r->con_map = map;
     r->cons.assign(p, p + n);
     r->con_off = off;
+    r->con_unique = unique;
     return FF_OK;
+}
+
+// The device tables of the level's batched exchange, (re)built when the registrations have
+// changed since the last build.  Returns 1 when the batch can run, 0 when the exchange must go
+// connection by connection (a ghost cell named twice, or a stale table while a graph is being
+// captured, when nothing may be allocated), or an error.
+static int halo_batch_ready(ff_ctx *c, int level)
+{
+    HaloBatch &hb = c->halo_batch[level];
+    if (hb.gen == c->gen) return hb.nwg > 0 ? 1 : 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(c, hipStreamIsCapturing(c->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return 0;
+    std::vector<Blk> blks;
+    std::map<int, int> index;          // block number -> position in blks
+    // the context's blocks are keyed by (level, block number), block number in the low word
+    auto number = [](long long k) { return (int)(unsigned)(k & 0xffffffffLL); };
+    for (auto &kv : c->blocks)
+        if (kv.second.level == level) {
+            index[number(kv.first)] = (int)blks.size();
+            blks.push_back(kv.second.dev);
+        }
+    std::vector<HaloConnDev> conns;
+    std::vector<int2> wg;
+    bool ok = true;
+    for (auto &kv : c->blocks) {
+        const BlockRec &r = kv.second;
+        if (r.level != level || r.cons.empty()) continue;
+        ok = ok && r.con_unique;
+        for (size_t l = 0; l < r.cons.size(); l++) {
+            auto it = index.find(r.cons[l].nblock);
+            if (it == index.end())
+                return fail(c, FF_E_STATE, "ff_halo_exchange: connected block no longer registered%s");
+            const int np = (int)(r.con_off[l + 1] - r.con_off[l]);
+            if (np == 0) continue;
+            const int ci = (int)conns.size();
+            conns.push_back({index[number(kv.first)], it->second, r.con_map + 2 * r.con_off[l], np});
+            for (int first = 0; first < np; first += 256) wg.push_back(make_int2(ci, first));
+        }
+    }
+    if (hb.blks) {
+        // the last exchange that read the old tables ran on this stream
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(hb.blks);
+    }
+    if (hb.conns) (void)hipFree(hb.conns);
+    if (hb.wg) (void)hipFree(hb.wg);
+    hb = HaloBatch{};
+    if (!ok || wg.empty()) {
+        hb.gen = c->gen;
+        return 0;
+    }
+    if (hipMalloc(&hb.blks, sizeof(Blk) * blks.size()) != hipSuccess ||
+        hipMalloc(&hb.conns, sizeof(HaloConnDev) * conns.size()) != hipSuccess ||
+        hipMalloc(&hb.wg, sizeof(int2) * wg.size()) != hipSuccess)
+        return fail(c, FF_E_NOMEM, "halo exchange tables%s");
+    HIPCHK(c, hipMemcpy(hb.blks, blks.data(), sizeof(Blk) * blks.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(hb.conns, conns.data(), sizeof(HaloConnDev) * conns.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(hb.wg, wg.data(), sizeof(int2) * wg.size(), hipMemcpyHostToDevice));
+    hb.nwg = (int)wg.size();
+    hb.gen = c->gen;
+    return 1;
 }
 
 static int halo_impl(ff_ctx *c, int level, int what)
@@ -121,6 +207,13 @@ static int halo_impl(ff_ctx *c, int level, int what)
     hf.n = what == 0 ? (int)(sizeof(kHaloFlow) / sizeof(int)) : (int)(sizeof(kHaloGeom) / sizeof(int));
     for (int q = 0; q < hf.n; q++) hf.f[q] = src[q];
     c->lev = level;
+    const int batch = halo_batch_ready(c, level);
+    if (batch < 0) return batch;
+    if (batch == 1) {
+        const HaloBatch &hb = c->halo_batch[level];
+        return launch(c, "k_halo", k_halo_batch, dim3((unsigned)hb.nwg), dim3(256),
+                      (const Blk *)hb.blks, (const HaloConnDev *)hb.conns, (const int2 *)hb.wg, hf);
+    }
     for (auto &kv : c->blocks) {
         BlockRec &r = kv.second;
         if (r.level != level || r.cons.empty()) continue;

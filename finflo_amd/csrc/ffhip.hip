@@ -3884,6 +3884,7 @@ struct BlockRec {
     std::vector<ff_con_patch> cons;   // connection patches (ff_con_register)
     int *con_map = nullptr;           // their ghost <- source cell offset pairs on the device
     std::vector<size_t> con_off;      // start of each patch's pairs in con_map
+    bool con_unique = true;           // no ghost cell is the target of two of its pairs
 };
 
 struct PendingEv {
@@ -3910,6 +3911,18 @@ struct TileShape { int ri, rj, rk; };
 constexpr TileShape kTileShapes[] = {{64, 4, 8}, {64, 4, 16}, {32, 8, 8}, {32, 4, 4},
                                      {128, 2, 8}, {32, 4, 2}, {16, 4, 2}, {64, 2, 2}};
 constexpr int NSHAPE = (int)(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
+
+// The halo exchange of one level as a single launch: every connection of every block, one
+// workgroup per 256 pairs (ffcon.hip).  Built outside graph capture, rebuilt when the
+// registrations change (gen).
+struct HaloConnDev { int dst, src; const int *map; int n; };
+struct HaloBatch {
+    long long gen = -1;
+    Blk *blks = nullptr;              // the level's blocks (device copy)
+    HaloConnDev *conns = nullptr;     // the connections, in block and patch order
+    int2 *wg = nullptr;               // per workgroup: connection, first pair
+    int nwg = 0;
+};
 
 struct ff_ctx {
     ff_params prm{};
@@ -3958,6 +3971,7 @@ struct ff_ctx {
     bool autotune = false;
     int force_shape = -1;         // FF_OPT_TILE_SHAPE: a candidate shape for every residual
     std::map<std::tuple<int, int, int, int>, std::pair<int, double>> tuned;
+    std::map<int, HaloBatch> halo_batch;   // per level (ffcon.hip)
     int device = -1;              // the HIP device the context was created on
     double *scratch = nullptr;    // that device's address of the fixed scratch array
 };
@@ -4305,6 +4319,11 @@ int ff_ctx_destroy(ff_ctx *c)
     for (int q = 0; q < ff_ctx::NPOOL; q++)
         if (c->mb_part[q]) (void)hipFree(c->mb_part[q]);
     if (c->mb_out) (void)hipFree(c->mb_out);
+    for (auto &kv : c->halo_batch) {
+        if (kv.second.blks) (void)hipFree(kv.second.blks);
+        if (kv.second.conns) (void)hipFree(kv.second.conns);
+        if (kv.second.wg) (void)hipFree(kv.second.wg);
+    }
     for (auto &kv : c->blocks) {
         if (kv.second.mirror && kv.second.arena) (void)hipFree(kv.second.arena);
         sweep_free(kv.second);
