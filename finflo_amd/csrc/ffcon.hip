@@ -281,12 +281,14 @@ static int vcycle_blocks_streams(ff_ctx *c, const int *blocks, int nblocks, int 
         const bool sums = resi && it == ncycles - 1;
         HIPCHK(c, hipEventRecord(c->pool_ev[ff_ctx::NPOOL], origin));
         for (int q = 0; q < np; q++) HIPCHK(c, hipStreamWaitEvent(c->pool[q], c->pool_ev[ff_ctx::NPOOL], 0));
+        c->in_pool = true;
         for (int n = 0; n < nblocks && !rc; n++) {
             const int q = n % np;
             c->stream = c->pool[q];
             const RedBuf rb{c->mb_part[q], c->red_parts, c->mb_out + 8 * n};
             rc = vcycle_once(c, lv[n], nlev, blocks[n], sums ? dummy : nullptr, &rb, false);
         }
+        c->in_pool = false;
         c->stream = origin;
         for (int q = 0; q < np; q++) {
             if (hipEventRecord(c->pool_ev[q], c->pool[q]) != hipSuccess ||

@@ -3964,6 +3964,7 @@ struct ff_ctx {
     // the concurrent multi-block cycle (FF_OPT_BLOCK_STREAMS): RESL2 partials per pool
     // stream and the sums of every block, read back once after the cycle
     bool block_streams = false;
+    bool in_pool = false;         // a pool stream is the context's stream (its V-cycles)
     double *mb_part[NPOOL] = {};
     double *mb_out = nullptr;
     int mb_out_n = 0;
@@ -4730,6 +4731,14 @@ static int residual_impl(ff_ctx *c, BlockRec *r, int level, int m2, double *resi
     // stages that follow; the next wall pass of this block waits for them
     auto forces = [&]() -> int {
         if (wr.dev.empty() || level != 1) return FF_OK;
+        if (c->in_pool) {
+            // blocks side by side on the stream pool: the block's own stream is already one
+            // of many running at once, and a single side stream for every block's serial
+            // sums would hold up the pool streams that share its hardware queue
+            return launch(c, "k_wall_forces", k_wall_forces, dim3((unsigned)wr.dev.size()), dim3(WF_N),
+                          make_prm(&c->prm, level), b, wr.wd, (const WallPatchDev *)wr.dev_tab,
+                          wr.env, wr.sums);
+        }
         HIPCHK(c, hipEventRecord(wr.fwd, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->wstream, wr.fwd, 0));
         int e = launch_on(c, c->wstream, "k_wall_forces", k_wall_forces, dim3((unsigned)wr.dev.size()),
