@@ -413,7 +413,10 @@ int  ff_con_register(ff_ctx *ctx, int level, int block, const ff_con_patch *p, i
  * layers next to its face.  what = 0: the flow fields (PDIFF TEMP U V W RK REPS, P RO E C
  * VIS CH CP DRDP DRDH RM RN RW EPS2 VIST); what = 1: the cell geometry (VOL, XC YC ZC,
  * DISTW, D1 D2 D3), once after registration, as IMPVOL does for the volumes (gluebm.f:6166).
- * The values are copied, not recomputed from exchanged conserved variables. */
+ * The values are copied, not recomputed from exchanged conserved variables.  The level's
+ * connections go in one launch (a table built, outside graph capture, after registrations
+ * change; the first exchange after a change allocates it), or one launch per connection
+ * when two patches of a block name the same ghost cell. */
 int  ff_halo_exchange(ff_ctx *ctx, int level, int what);
 /* ncycles multi-block cycles: per cycle every listed block's V-cycle (levels 1..nlev, as
  * ff_vcycle), then the level-1 halo exchange, as CYCLES' block loop and BOUNDI(1) do
