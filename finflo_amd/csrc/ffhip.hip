@@ -3954,7 +3954,7 @@ struct ff_ctx {
     long long gen = 0;
     // the captured multi-block cycle (ff_vcycle_blocks_graph): its streams, fork/join events,
     // graph and the key it was captured for
-    static constexpr int NPOOL = 16;      // streams of the pool (FF_OPT_BLOCK_STREAMS)
+    static constexpr int NPOOL = 32;      // streams of the pool (FF_OPT_BLOCK_STREAMS)
     static constexpr int NBRANCH = 8;     // graph branches of ff_vcycle_blocks_graph
     hipStream_t pool[NPOOL] = {};
     hipEvent_t pool_ev[NPOOL + 1] = {};
